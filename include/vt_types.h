@@ -1,0 +1,67 @@
+/* vt_types.h: the data types that cross the drop-in boundary, laid out exactly as the reference lays them out so a
+ * host program built against the reference's headers passes the same bytes.
+ *
+ *   struct vt_plane  src/kern/plane.h:16-35   padded fp32 Q6 plane (pixel x 64, integer valued), 32 lanes / 4 rows
+ *                                             of replicated border, stride a multiple of 16 lanes
+ *   struct lk_cfg    src/kern/lk.h:24-31      LK window, iteration limit, thresholds
+ *   struct lk_out    src/kern/lk.h:33-40      32 B per point: estimate, min eigenvalue, residual, covariance,
+ *                                             iterations summed over levels, status bits
+ *
+ * Each block is guarded by the reference header's own include guard, so a translation unit that already included
+ * plane.h or lk.h from the reference sees one definition. */
+#ifndef VTG_TYPES_H
+#define VTG_TYPES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifndef VT_PLANE_H
+#define VT_PLANE_H
+#define VT_PAD_X 32
+#define VT_PAD_Y 4
+#define VT_Q6 64
+
+struct vt_plane {
+    float *p; /* pixel (0, 0), inside the padded block */
+    int w, h;
+    int stride; /* floats per row, a multiple of 16 */
+};
+
+static inline int plane_stride(int w) { return ((w + 15) & ~15) + 2 * VT_PAD_X; }
+static inline size_t plane_bytes(int w, int h) { return (size_t)plane_stride(w) * (size_t)(h + 2 * VT_PAD_Y) * sizeof(float); }
+static inline struct vt_plane plane_at(void *mem, int w, int h)
+{
+    const int s = plane_stride(w);
+    struct vt_plane pl = {(float *)mem + (size_t)s * VT_PAD_Y + VT_PAD_X, w, h, s};
+    return pl;
+}
+static inline float *plane_row(const struct vt_plane *pl, int y) { return pl->p + (ptrdiff_t)y * pl->stride; }
+#endif
+
+#ifndef VT_LK_H
+#define VT_LK_H
+#define LK_MAX_W 48
+#define LK_MAX_H 31
+
+enum { LK_OK = 0, LK_OUT = 1, LK_LOWEIG = 2, LK_RESIDUAL = 4, LK_FLOWBACK = 8 };
+
+struct lk_cfg {
+    int w, h;           /* window, pixels: 5 .. LK_MAX_W by 5 .. LK_MAX_H */
+    int max_iter;       /* iterations per level */
+    float eps;          /* stop when a step is below this, pixels of the level */
+    float min_eig;      /* smaller eigenvalue of G per window pixel, 8-bit grey per pixel gradients */
+    float max_residual; /* mean |T - J| in 8-bit grey */
+    int prefetch;       /* CPU cache hint in the reference; no effect on results, ignored here */
+};
+
+struct lk_out {
+    float x, y;                   /* estimate at level 0 */
+    float min_eig;                /* at level 0 */
+    float residual;               /* mean |T - J|, 8-bit grey */
+    float cov_xx, cov_xy, cov_yy; /* s^2 G^-1, pixels^2 */
+    uint16_t iters;               /* summed over the levels */
+    uint16_t status;              /* LK_* bits */
+};
+#endif
+
+#endif
