@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""bench.py: 1080p frame-pair throughput of the vision path on one MI355X.
+
+A step is one 1920 x 1080 u8 frame pair already in HBM: both 4-level pyramids, sparse pyramidal LK of 16,200 grid
+points (180 x 90, 21 x 21 window, 10 iterations, eps 0.01 px, min_eig 1e-4), and a 64 x 64 NCC template cut from
+frame 0 searched exhaustively over frame 1 (cell map, top 8, half-pixel refinement, result to the host). Inputs
+rotate over more than the 256 MB Infinity Cache. Prints one JSON line; details go to stderr.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--storage q6|f32|f16|bf16] [--cpu-seconds S]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "intel-phi-visiontrack_amd"))
+
+METRIC = "LK pts/s + 1080p frame-pairs/s vs plain; p50/p99 ms @ ≤0.05 px err"
+W, H, LEVELS, NX, NY, WIN, TPL = 1920, 1080, 4, 180, 90, 21, 64
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector (published; with dual-issue packed FP32)
+PEAK_DOT4_TOPS = 314.6        # u8 dot4 on the vector ALUs: 256 CU x 4 SIMD x 16 lanes x 4 MAC x 2 x 2.4 GHz
+STORAGES = {"q6": 0, "f32": 1, "f16": 2, "bf16": 3}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def lk_flops(out, xs, ys, top, w, h, win):
+    """Algorithmic fp32 flops of one LK batch: per level processed 6(w+2)(h+2) + 26wh (build), per iteration
+    3w(h+1) + 8wh, one residual 3w(h+1) + 11wh at level 0 for a converged point."""
+    build = 6 * (win + 2) ** 2 + 26 * win * win
+    it = 3 * win * (win + 1) + 8 * win * win
+    res = 3 * win * (win + 1) + 11 * win * win
+    m = win // 2 + 1
+    levels = np.zeros(len(xs))
+    for L in range(top + 1):
+        lw, lh = w, h
+        for _ in range(L):
+            lw, lh = (lw + 1) // 2, (lh + 1) // 2
+        px, py = xs / 2 ** L, ys / 2 ** L
+        levels += (px >= m) & (py >= m) & (px < lw - m - 1) & (py < lh - m - 1)
+    return float((levels * build).sum() + out["iters"].astype(np.float64).sum() * it + (out["status"] == 0).sum() * res)
+
+
+def pyr_bytes(w, h, levels, elem):
+    """Algorithmic HBM bytes of one pyramid: the u8 frame read once, each upper level written once."""
+    b, lw, lh = w * h, w, h
+    for _ in range(1, levels):
+        lw, lh = (lw + 1) // 2, (lh + 1) // 2
+        b += lw * lh * elem
+    return b
+
+
+def cpu_baseline(f0s, f1s, xs, ys, budget_s, gpu_out):
+    """The oracle (reference-faithful tier, C, pthreads) on this host: pyramids of both frames, LK on a sample of
+    the grid, the NCC map on a band of cells; scaled to frame pairs per second. Also the GPU's flow error against
+    the FP64 oracle on the same sample."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from oracle_ffi import Oracle, lk_cfg
+    o = Oracle()
+    cores = min(os.cpu_count() or 1, 16)
+    cfg = lk_cfg(WIN, WIN, 10, 0.01, 1e-4, 40.0)
+    f0, f1 = f0s[0], f1s[0]
+    t = time.perf_counter()
+    k0, I = o.pyramid(f0, W, H, LEVELS, threads=cores)
+    k1, J = o.pyramid(f1, W, H, LEVELS, threads=cores)
+    t_pyr = time.perf_counter() - t
+    stride = 8
+    sx, sy = xs[::stride], ys[::stride]
+    t = time.perf_counter()
+    ref = o.lk_batch(0, I, J, LEVELS - 1, cfg, sx, sy, threads=cores)
+    t_lk = (time.perf_counter() - t) * stride
+    f64 = o.lk_batch(1, I, J, LEVELS - 1, cfg, sx, sy, threads=cores)
+    g = gpu_out[::stride]
+    both = (g["status"] == 0) & (f64["status"] == 0)
+    err = np.maximum(np.abs(g["x"] - f64["x"]), np.abs(g["y"] - f64["y"]))[both]
+    same_status = float((g["status"] == ref["status"]).mean())
+    dref = np.maximum(np.abs(g["x"] - ref["x"]), np.abs(g["y"] - ref["y"]))[(g["status"] == 0) & (ref["status"] == 0)]
+    app, norm = o.ncc_template(k0[0], 700, 400, TPL, TPL)
+    x1, y1 = W - TPL - 2, H - TPL - 2
+    cols, rows = (x1 - 1) // 16 + 1, (y1 - 1) // 16 + 1
+    band = max(1, min(rows, int(budget_s / 2)))
+    t = time.perf_counter()
+    o.ncc_search(k1[0], app, TPL, TPL, norm, 1, x1, 1, 16 * band, 1, threads=cores)
+    t_ncc = (time.perf_counter() - t) * rows / band
+    t_pair = t_pyr + t_lk + t_ncc
+    sample = (f"1 pair: both pyramids, LK on every {stride}th grid point ({len(sx)} pts) x{stride}, NCC map on "
+              f"{band} of {rows} cell rows x{rows / band:.1f}, lane-model tier")
+    log(f"cpu: {cores} threads: pyramids {t_pyr * 1e3:.1f} ms, LK {t_lk * 1e3:.0f} ms/16,200 pts, NCC "
+        f"{t_ncc * 1e3:.0f} ms -> {1 / t_pair:.3f} pairs/s")
+    return ({"value": round(1.0 / t_pair, 4), "unit": "frame-pairs/s", "cores": cores, "kind": "port",
+             "sample": sample, "lk_pts_per_s": round(16200 / t_lk, 1), "stage_ms": {
+                 "pyramids": round(t_pyr * 1e3, 2), "lk": round(t_lk * 1e3, 1), "ncc": round(t_ncc * 1e3, 1)}},
+            {"vs_f64_median_px": float(np.median(err)), "vs_f64_p99_px": float(np.percentile(err, 99)),
+             "vs_f64_max_px": float(err.max()), "status_equal_vs_reference_tier": same_status,
+             "vs_reference_tier_max_px": float(dref.max()) if len(dref) else 0.0})
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--storage", default="q6", choices=sorted(STORAGES))
+    ap.add_argument("--pairs", type=int, default=72, help="device-resident frame pairs rotated through")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample (0: skip)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (streams shard with no "
+              f"exchange, so N GPUs run N independent copies of this step)", file=sys.stderr)
+        sys.exit(2)
+    import vtgpu
+    storage = STORAGES[a.storage]
+    ctx = vtgpu.Context(0)
+    # 8 distinct seeded pairs, copied to `pairs` device slots (> 256 MB of inputs with the pyramids)
+    uniq = 8
+    f0s, f1s = [], []
+    for s in range(uniq):
+        f0, f1, _ = vtgpu.frame_pair(W, H, 100 + s)
+        f0s.append(f0)
+        f1s.append(f1)
+    xs, ys = vtgpu.grid(NX, NY, W, H)
+    npts = len(xs)
+    d_frames = []
+    for i in range(a.pairs):
+        d_frames.append((ctx.to_device(f0s[i % uniq]), ctx.to_device(f1s[i % uniq])))
+    pyrs = [(ctx.pyramid(W, H, LEVELS, storage), ctx.pyramid(W, H, LEVELS, storage)) for _ in range(min(a.pairs, 8))]
+    d_xy = ctx.to_device(np.stack([xs, ys], 1).ravel())
+    d_out = ctx.alloc(32 * npts)
+    cfg = vtgpu.LkCfg(WIN, WIN, 10, 0.01, 1e-4, 40.0, 0)
+    rng = np.random.default_rng(2)
+    corners = [(int(rng.integers(100, W - 200)), int(rng.integers(100, H - 200))) for _ in range(uniq)]
+    tmpl = None
+    ncell = ((W - TPL - 3) // 16 + 1) * ((H - TPL - 3) // 16 + 1)
+    d_cells = ctx.alloc(12 * ncell)
+
+    def step(i):
+        fa, fb = d_frames[i % a.pairs]
+        pa, pb = pyrs[i % len(pyrs)]
+        pa.build(fa, on_device=True)
+        pb.build(fb, on_device=True)
+        ctx.lk_track(pa, pb, LEVELS - 1, cfg, d_xy, None, 0.0, d_out)
+        nonlocal tmpl
+        cx, cy = corners[i % uniq]
+        if tmpl is None:
+            tmpl = ctx.ncc_template(pa, 0, cx, cy, TPL, TPL)
+        else:
+            tmpl.update(pa, 0, cx, cy)
+        return ctx.ncc_search(pb, tmpl, 1, W, 1, H, 1, d_cells)
+
+    for i in range(a.warmup):
+        step(i)
+    ctx.sync()
+    # per-pair algorithmic counts, from each distinct pair's own LK output
+    flops = []
+    for s in range(uniq):
+        step(s)
+        ctx.sync()
+        out = d_out.get(np.zeros(npts, vtgpu.LK_OUT_DTYPE))
+        flops.append(lk_flops(out, xs, ys, LEVELS - 1, W, H, WIN))
+        if s == 0:
+            out0 = out.copy()
+    ctx.timing(True)
+    ctx.timing_reset()
+    lat = []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        ts = time.perf_counter()
+        f = step(i)  # ends with the found record on the host
+        lat.append(time.perf_counter() - ts)
+    ctx.sync()
+    total = time.perf_counter() - t0
+    kernels = {}
+    for k in ("pyr_reduce", "pyr_u8_to_f32", "lk_track", "ncc_template", "ncc_cells", "ncc_cells_generic",
+              "ncc_finish"):
+        ms, n = ctx.timing_read(k)
+        if n:
+            kernels[k] = {"avg_us": round(1e3 * ms / n, 2), "launches": n, "ms_per_step": round(ms / a.steps, 4)}
+    ctx.timing(False)
+    log("kernels:", json.dumps(kernels))
+    lat_ms = np.array(lat) * 1e3
+    value = a.steps / total
+    lk_flop = float(np.mean([flops[i % uniq] for i in range(a.steps)]))
+    npos = (W - TPL - 2) * (H - TPL - 2)
+    elem = {0: 2, 1: 4, 2: 2, 3: 2}[storage]
+    # algorithmic work per step of each kernel: fp32 flops (LK), u8 MAC ops (NCC), HBM bytes (pyramids)
+    work = {
+        "lk_track": ("valu", lk_flop / 1e12, PEAK_FP32_TFLOPS, "TFLOP/s"),
+        "ncc_cells": ("valu", 2.0 * TPL * TPL * npos / 1e12, PEAK_DOT4_TOPS, "TOP/s"),
+        "pyr_reduce": ("hbm", 2 * pyr_bytes(W, H, LEVELS, elem) / 1e9, PEAK_HBM_GBS, "GB/s"),
+    }
+    for k, (b, wk, pk, u) in work.items():
+        if k in kernels:
+            ach = wk / (kernels[k]["ms_per_step"] * 1e-3)
+            kernels[k].update({"achieved": round(ach, 3), "unit": u, "frac": round(ach / pk, 4)})
+    dom = max((k for k in kernels if k in work), key=lambda k: kernels[k]["ms_per_step"])
+    bound, per_step, peak, unit = work[dom]
+    achieved = per_step / (kernels[dom]["ms_per_step"] * 1e-3)
+    lk_us = kernels.get("lk_track", {}).get("avg_us", float("nan"))
+    cpu, err = ({}, {})
+    if a.cpu_seconds > 0:
+        cpu, err = cpu_baseline(f0s, f1s, xs, ys, a.cpu_seconds, out0)
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "frame-pairs/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * total / a.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8/u16 int + fp32 (fp64 per-point solve)",
+        "data": "synthetic (seeded 200-blob frames, rigid motion + noise)",
+        "config": {"workload": "1080p frame pair: 2 x 4-level pyramid + LK 16,200 pts 21x21 + NCC 64x64 exhaustive",
+                   "w": W, "h": H, "levels": LEVELS, "points": npts, "window": f"{WIN}x{WIN}", "ncc_template":
+                   f"{TPL}x{TPL}", "ncc_corners": npos, "storage": a.storage, "parallelism": "1 GPU",
+                   "input_rotation": f"{a.pairs} device frame pairs ({a.pairs * 2 * W * H / 2 ** 20:.0f} MiB)"},
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
+        "lk_pts_per_s": round(npts / (lk_us * 1e-6), 1),
+        "lk_pts_per_s_in_step": round(npts * value, 1),
+        "kernels": kernels,
+        "roofline": {"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                     "frac": round(achieved / peak, 4), "traffic": None},
+        "flow_error": err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,161 @@
+/* vtgpu.h: the C-ABI of the MI355X vision path: Gaussian pyramid, sparse pyramidal Lucas-Kanade, NCC template
+ * search. Plain pointers and sizes; no HIP or torch types cross it. Every entry point returns 0 or a negative
+ * errno-style code (VTG_E*) and leaves a reason in vtg_ctx_error(). One host thread per context; contexts are
+ * independent. All work of a context is ordered on its one HIP stream.
+ *
+ * Two surfaces:
+ *   (1) reference-shaped calls over host planes (struct vt_plane, fp32 Q6 with replicated padding), the drop-in for
+ *       the reference's kernel-level functions: each copies in, runs the GPU kernels, copies out.
+ *   (2) device-resident calls: pyramids live in HBM (vtg_pyr), points and results are device buffers. This is the
+ *       hot path a host program batches through.
+ * The reference interface each entry point replaces is cited beside it (paths relative to the reference tree). */
+#ifndef VTGPU_H
+#define VTGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "vt_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VTG_MAX_LEVELS 6 /* VT_MAX_LEVELS, src/common/vt_proto.h:59 */
+
+enum {
+    VTG_OK = 0,
+    VTG_EINVAL = -22, /* bad argument: sizes, levels, window, pointers */
+    VTG_ENOMEM = -12, /* device or host allocation failed */
+    VTG_EIO = -5,     /* a HIP call failed (launch, copy, device lost) */
+    VTG_ERANGE = -34, /* a size beyond what the kernels support */
+};
+
+/* ------------------------------------------------------------------ context and memory */
+
+typedef struct vtg_ctx vtg_ctx;
+
+/* A context on HIP device `device` with its own stream (default priority). */
+int vtg_ctx_create(int device, vtg_ctx **out);
+int vtg_ctx_destroy(vtg_ctx *ctx);
+/* Wait for all work of the context. */
+int vtg_ctx_sync(vtg_ctx *ctx);
+/* The last failure's reason, "" if none. */
+const char *vtg_ctx_error(const vtg_ctx *ctx);
+/* The context's hipStream_t, as an opaque pointer (for callers that order their own work after it). */
+void *vtg_ctx_stream(vtg_ctx *ctx);
+
+int vtg_alloc(vtg_ctx *ctx, size_t bytes, void **dptr);
+int vtg_free(vtg_ctx *ctx, void *dptr);
+/* Stream-ordered copies; these two wait for completion (the host buffer may be pageable). */
+int vtg_upload(vtg_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+int vtg_download(vtg_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+/* Stream-ordered device-to-device copy and fill (no wait). */
+int vtg_copy_d2d(vtg_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
+
+/* Per-kernel timing with HIP events on the context's stream. While on, every launch is bracketed by two events;
+ * vtg_timing_read waits for them and returns the summed milliseconds and launch count of kernels named `kernel`. */
+int vtg_timing_enable(vtg_ctx *ctx, int on);
+int vtg_timing_read(vtg_ctx *ctx, const char *kernel, double *total_ms, long *launches);
+int vtg_timing_reset(vtg_ctx *ctx);
+
+/* ------------------------------------------------------------------ pyramids (tk_build, src/kern/track.c:584-629) */
+
+/* Storage of the levels in HBM. Q6 is exact: the reference's integer-valued fp32 planes hold at most 16,320, so a
+ * u8 level 0 (the frame itself, Q6 = 64 x byte) and u16 upper levels hold the same values in 1 and 2 bytes. F32 is
+ * the reference's own fp32 Q6 (4 B per pixel, the plain port). F16 and BF16 hold grey = Q6 / 64 above level 0 and
+ * are not exact. */
+enum { VTG_STORE_Q6 = 0, VTG_STORE_F32 = 1, VTG_STORE_F16 = 2, VTG_STORE_BF16 = 3 };
+
+typedef struct vtg_pyr vtg_pyr;
+
+/* A pyramid of `levels` levels of a w x h frame, level l of size ((w + 2^l - 1) >> l) x ((h + 2^l - 1) >> l) as
+ * (w + 1) / 2 per step (src/kern/img.h:30-31). */
+int vtg_pyr_create(vtg_ctx *ctx, int w, int h, int levels, int storage, vtg_pyr **out);
+int vtg_pyr_destroy(vtg_ctx *ctx, vtg_pyr *p);
+/* Build every level from a u8 frame (w bytes per row, `pitch` apart): level 0 = 64 x byte
+ * (img_u8_to_q6_*, src/kern/img.h:16), level l = reduce(level l-1) (img_reduce_*, src/kern/img.h:30-33):
+ * (sum of (1 4 6 4 1)^2 x src over the 5 x 5 at (2x, 2y), replicated border, + 128) >> 8.
+ * on_device: frame is a device pointer (else host memory, copied up first). */
+int vtg_pyr_build(vtg_ctx *ctx, vtg_pyr *p, const uint8_t *frame, int pitch, int on_device);
+/* Level l's device pointer, size, pitch in bytes and element size. */
+int vtg_pyr_level(const vtg_pyr *p, int level, void **dptr, int *w, int *h, size_t *pitch_bytes, int *elem_bytes);
+/* Level l into a host fp32 Q6 plane, with the reference's replicated padding filled (img_pad_rows, img_pad_tb,
+ * src/kern/img.h:10-12): the form the reference's kernels read. */
+int vtg_pyr_read_level(vtg_ctx *ctx, const vtg_pyr *p, int level, const struct vt_plane *dst);
+
+/* ------------------------------------------------------------------ LK (lk_track_*, lk_flowback_*, src/kern/lk.h:45-53) */
+
+/* The 2x2 solve: VTG_SOLVE_F64 runs the whole per-point driver in fp64 (the reference's is x87 long double);
+ * VTG_SOLVE_MIXED runs it in fp32 and falls back to fp64 for a level whose G has condition number above 1e4. */
+enum { VTG_SOLVE_F64 = 0, VTG_SOLVE_MIXED = 1 };
+
+/* n points tracked from pyramid I to pyramid J from level `top` down (the OP_LK batch loop, src/kern/track.c:275-292):
+ * xy[2i], xy[2i+1] the point in level-0 pixels, guess the same shape or NULL for zero. fb_max > 0 adds the
+ * forward-backward check (lk_flowback_*). out: n records of struct lk_out (32 B). All device pointers. Window
+ * 5..48 x 5..31; top < levels of both pyramids. */
+int vtg_lk_track(vtg_ctx *ctx, const vtg_pyr *I, const vtg_pyr *J, int top, const struct lk_cfg *cfg, int n,
+                 const float *xy, const float *guess, float fb_max, struct lk_out *out);
+/* The same with the solve mode chosen. */
+int vtg_lk_track_ex(vtg_ctx *ctx, const vtg_pyr *I, const vtg_pyr *J, int top, const struct lk_cfg *cfg, int n,
+                    const float *xy, const float *guess, float fb_max, int solve_mode, struct lk_out *out);
+
+/* ------------------------------------------------------------------ NCC search (src/kern/track.c:184-208, 301-329, 935-974, 1615-1734) */
+
+typedef struct vtg_tmpl vtg_tmpl;
+
+/* One cell of a search's map: the best score of its 16 x 16 template corners and where (OP_SEARCH, track.c:301-329). */
+struct vtg_cell { float v; int x, y; };
+
+/* What a search found (search_slot, track.c:1661-1726): the best corner in half pixels of the level after the top-8
+ * half-pixel refinement, its score, the best score clear of it by half a template, the parabola offsets of the peak
+ * (peak_offset, track.c:1650-1657) and the whole-pixel argmax of the map. */
+struct vtg_found {
+    int hx, hy;          /* half pixels; -1 score when nothing was searched */
+    double ncc, second;
+    double ox, oy;       /* sub-half-pixel offsets of the peak, each in [-0.5, 0.5] */
+    int cells;           /* cells in the map */
+    int best_x, best_y;  /* whole-pixel argmax over the map, first in cell order */
+    float best_v;
+};
+
+/* A zero-mean template of the tw x th window at corner (x0, y0) of pyramid level `level` (store_appearance,
+ * track.c:935-974; the reference's is 32 x 21, here tw a multiple of 16 up to 64, th up to 64). */
+int vtg_ncc_template(vtg_ctx *ctx, const vtg_pyr *p, int level, int x0, int y0, int tw, int th, vtg_tmpl **out);
+int vtg_ncc_template_destroy(vtg_ctx *ctx, vtg_tmpl *t);
+/* Re-cut an existing template of the same size from another window or pyramid (store_appearance into a slot that
+ * already exists, track.c:935-974), stream-ordered, no allocation. */
+int vtg_ncc_template_update(vtg_ctx *ctx, vtg_tmpl *t, const vtg_pyr *p, int level, int x0, int y0);
+/* The template's norm (sum of squares of its zero-mean values) and fp32 pixels (zero-mean, row-major) to host. */
+int vtg_ncc_template_read(vtg_ctx *ctx, const vtg_tmpl *t, float *app_host, float *norm);
+/* Search corners [x0, x1] x [y0, y1] at `step` (clipped to where the template fits, as search_slot clips), on the
+ * template's level of pyramid p. cells: device buffer of at least ((x1-x0)/16+1) * ((y1-y0)/16+1) records, or NULL.
+ * found: host record, written on return (this call waits for the search). */
+int vtg_ncc_search(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
+                   struct vtg_cell *cells, struct vtg_found *found);
+/* Scores at n corners (tk_correlate, track.h:57-59; half != 0: half-pixel corners, search_half): xy device int
+ * pairs, out device doubles: -1 outside, 0 for a window under 1/16 of the template's energy. */
+int vtg_ncc_score(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out);
+
+/* ------------------------------------------------------------------ reference-shaped calls over host planes */
+
+/* img_u8_to_q6_* (src/kern/img.h:16-17): dst rows y0..y1 = 64 x src, padding of those rows filled. */
+int vtg_img_u8_to_q6(vtg_ctx *ctx, const struct vt_plane *dst, const uint8_t *src, int src_stride, int y0, int y1);
+/* img_reduce_* (src/kern/img.h:30-33): dst rows y0..y1 from src (whose padding need not be current: the border is
+ * replicated by the kernel), padding of those rows filled. */
+int vtg_img_reduce(vtg_ctx *ctx, const struct vt_plane *dst, const struct vt_plane *src, int y0, int y1);
+/* lk_track_* / lk_flowback_* (src/kern/lk.h:45-53) for n points over host pyramids I[0..top], J[0..top]:
+ * gx, gy may be NULL (zero guess); fb_max > 0 runs the flowback check. */
+int vtg_lk_track_planes(vtg_ctx *ctx, const struct vt_plane *I, const struct vt_plane *J, int top,
+                        const struct lk_cfg *cfg, int n, const float *x, const float *y, const float *gx,
+                        const float *gy, float fb_max, struct lk_out *out);
+/* tk_correlate (src/kern/track.h:57-59) at n corners of a host plane, template tw x th (tw a multiple of 16):
+ * out[i] as the reference returns it. */
+int vtg_correlate_planes(vtg_ctx *ctx, const struct vt_plane *pl, const float *app, int tw, int th, double norm,
+                         int n, const int *x, const int *y, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

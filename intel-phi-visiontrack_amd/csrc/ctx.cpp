@@ -1,0 +1,255 @@
+// ctx.cpp: contexts, device memory, errors and the per-kernel event timer of the C-ABI.
+#include <cstring>
+
+#include "vtg_internal.hpp"
+
+namespace vtg {
+
+int fail(vtg_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof c->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+int hip_fail(vtg_ctx *c, hipError_t e, const char *what)
+{
+    return fail(c, e == hipErrorOutOfMemory ? VTG_ENOMEM : VTG_EIO, "%s: %s", what, hipGetErrorString(e));
+}
+
+static hipEvent_t take_event(vtg_ctx *c)
+{
+    if (!c->free_events.empty()) {
+        hipEvent_t e = c->free_events.back();
+        c->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
+    return e;
+}
+
+void timer_begin(vtg_ctx *c, const char *name, hipEvent_t *a)
+{
+    (void)name;
+    *a = nullptr;
+    if (!c->timing)
+        return;
+    *a = take_event(c);
+    if (*a)
+        (void)hipEventRecord(*a, c->stream);
+}
+
+void timer_end(vtg_ctx *c, const char *name, hipEvent_t a)
+{
+    if (!c->timing || !a)
+        return;
+    hipEvent_t b = take_event(c);
+    if (!b)
+        return;
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back(Timer{name, a, b});
+}
+
+static void collect(vtg_ctx *c)
+{
+    for (auto &t : c->pending) {
+        (void)hipEventSynchronize(t.b);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, t.a, t.b);
+        bool found = false;
+        for (auto &s : c->stats)
+            if (s.name == t.name) {
+                s.ms += ms;
+                s.launches++;
+                found = true;
+                break;
+            }
+        if (!found)
+            c->stats.push_back({t.name, (double)ms, 1});
+        c->free_events.push_back(t.a);
+        c->free_events.push_back(t.b);
+    }
+    c->pending.clear();
+}
+
+int scratch(vtg_ctx *c, size_t bytes, void **p)
+{
+    if (bytes > c->scratch_bytes) {
+        if (c->scratch) {
+            VTG_HIP(c, hipStreamSynchronize(c->stream));
+            VTG_HIP(c, hipFree(c->scratch));
+            c->scratch = nullptr;
+            c->scratch_bytes = 0;
+        }
+        const size_t want = (bytes + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+        VTG_HIP(c, hipMalloc(&c->scratch, want));
+        c->scratch_bytes = want;
+    }
+    *p = c->scratch;
+    return VTG_OK;
+}
+
+int pinned(vtg_ctx *c, size_t bytes, void **p)
+{
+    if (bytes > c->pinned_bytes) {
+        if (c->pinned) {
+            VTG_HIP(c, hipStreamSynchronize(c->stream));
+            VTG_HIP(c, hipHostFree(c->pinned));
+            c->pinned = nullptr;
+            c->pinned_bytes = 0;
+        }
+        const size_t want = (bytes + 4095) & ~(size_t)4095;
+        VTG_HIP(c, hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
+        c->pinned_bytes = want;
+    }
+    *p = c->pinned;
+    return VTG_OK;
+}
+
+}  // namespace vtg
+
+using namespace vtg;
+
+extern "C" {
+
+int vtg_ctx_create(int device, vtg_ctx **out)
+{
+    if (!out)
+        return VTG_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return VTG_EIO;
+    if (device < 0 || device >= n)
+        return VTG_EINVAL;
+    vtg_ctx *c = new vtg_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return VTG_EIO;
+    }
+    *out = c;
+    return VTG_OK;
+}
+
+int vtg_ctx_destroy(vtg_ctx *c)
+{
+    if (!c)
+        return VTG_EINVAL;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &t : c->pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (auto e : c->free_events)
+        (void)hipEventDestroy(e);
+    if (c->scratch)
+        (void)hipFree(c->scratch);
+    if (c->pinned)
+        (void)hipHostFree(c->pinned);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return VTG_OK;
+}
+
+int vtg_ctx_sync(vtg_ctx *c)
+{
+    if (!c)
+        return VTG_EINVAL;
+    VTG_HIP(c, hipStreamSynchronize(c->stream));
+    return VTG_OK;
+}
+
+const char *vtg_ctx_error(const vtg_ctx *c) { return c ? c->err : "no context"; }
+
+void *vtg_ctx_stream(vtg_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int vtg_alloc(vtg_ctx *c, size_t bytes, void **dptr)
+{
+    if (!c || !dptr || bytes == 0)
+        return fail(c, VTG_EINVAL, "vtg_alloc: bad argument");
+    VTG_HIP(c, hipMalloc(dptr, bytes));
+    return VTG_OK;
+}
+
+int vtg_free(vtg_ctx *c, void *dptr)
+{
+    if (!c)
+        return VTG_EINVAL;
+    if (dptr) {
+        VTG_HIP(c, hipStreamSynchronize(c->stream));
+        VTG_HIP(c, hipFree(dptr));
+    }
+    return VTG_OK;
+}
+
+int vtg_upload(vtg_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c || (!dst && bytes) || (!src && bytes))
+        return fail(c, VTG_EINVAL, "vtg_upload: bad argument");
+    VTG_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    VTG_HIP(c, hipStreamSynchronize(c->stream));
+    return VTG_OK;
+}
+
+int vtg_download(vtg_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c || (!dst && bytes) || (!src && bytes))
+        return fail(c, VTG_EINVAL, "vtg_download: bad argument");
+    VTG_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    VTG_HIP(c, hipStreamSynchronize(c->stream));
+    return VTG_OK;
+}
+
+int vtg_copy_d2d(vtg_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c || (!dst && bytes) || (!src && bytes))
+        return fail(c, VTG_EINVAL, "vtg_copy_d2d: bad argument");
+    VTG_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return VTG_OK;
+}
+
+int vtg_timing_enable(vtg_ctx *c, int on)
+{
+    if (!c)
+        return VTG_EINVAL;
+    c->timing = on != 0;
+    return VTG_OK;
+}
+
+int vtg_timing_read(vtg_ctx *c, const char *kernel, double *total_ms, long *launches)
+{
+    if (!c || !kernel)
+        return VTG_EINVAL;
+    collect(c);
+    double ms = 0;
+    long n = 0;
+    for (auto &s : c->stats)
+        if (s.name == kernel) {
+            ms = s.ms;
+            n = s.launches;
+        }
+    if (total_ms)
+        *total_ms = ms;
+    if (launches)
+        *launches = n;
+    return VTG_OK;
+}
+
+int vtg_timing_reset(vtg_ctx *c)
+{
+    if (!c)
+        return VTG_EINVAL;
+    collect(c);
+    c->stats.clear();
+    return VTG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,522 @@
+// lk.hip: sparse pyramidal Lucas-Kanade on the GPU, one wavefront per point (lk_driver.inc, src/kern/lk_driver.inc:7-121;
+// patch operations lk_vx.c:36-175; the OP_LK batch loop, src/kern/track.c:275-292).
+//
+// Lane layout. A w x h window is cut into G = 64 / w row groups of w lanes each (21 x 21: 3 groups of 7 rows, 63 lanes;
+// 15 x 15: 4 groups of 4 rows, 60 lanes). Lane (k, g) owns window column k and rows [g * chunk, (g + 1) * chunk): its
+// template values T, Tx, Ty live in registers for the whole level, and each iteration it walks down its rows carrying
+// the x-interpolated row, two loads per row, as the reference's vector code carries a row (lk_vx.c:100-126).
+// The window build samples the (h + 2) x (w + 2) bilinear window S into the wave's own LDS slice, then each lane takes
+// its 3 x 3 Sobel neighbourhoods from it. Sums over the window (G, b, the residual) reduce across the wave with DPP
+// (quad permutes, row mirrors, row broadcasts) and one readlane, so every lane holds the same uniform total.
+// The per-point driver (level walk, eigenvalue test, 2x2 solve, convergence, bounds) runs in fp64 on those uniform
+// values: the reference runs it in x87 long double; fp64 keeps flow within 1e-4 px of it. The patch arithmetic is fp32
+// with fused multiply-adds where the reference's vector code fuses them.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include "vtg_internal.hpp"
+
+namespace vtg {
+
+namespace {
+
+struct Cfg {
+    int w, h, max_iter;
+    float eps, min_eig, max_res;
+};
+
+struct LkArgs {
+    PyrView I, J;
+    Cfg cfg;
+    int top, n, s_floats;
+    const float *xy, *guess;
+    float fb_max;
+    lk_out *out;
+};
+
+template <typename T> struct Ld;
+template <> struct Ld<uint8_t> {
+    static __device__ __forceinline__ float get(uint8_t v) { return (float)v * 64.0f; }
+};
+template <> struct Ld<uint16_t> {
+    static __device__ __forceinline__ float get(uint16_t v) { return (float)v; }
+};
+template <> struct Ld<float> {
+    static __device__ __forceinline__ float get(float v) { return v; }
+};
+template <> struct Ld<__half> {
+    static __device__ __forceinline__ float get(__half v) { return __half2float(v) * 64.0f; }
+};
+template <> struct Ld<__hip_bfloat16> {
+    static __device__ __forceinline__ float get(__hip_bfloat16 v) { return __bfloat162float(v) * 64.0f; }
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// LDS written by one lane and read by another lane of the same wave: order the accesses, no workgroup barrier.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float v)
+{
+    const int moved = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false);
+    return v + __int_as_float(moved);
+}
+
+// Sum over the 64 lanes, the same value in every lane. Every lane must be active.
+__device__ __forceinline__ float wave_sum(float v)
+{
+    v = dpp_add<0xb1, 0xf>(v);  // quad_perm [1,0,3,2]
+    v = dpp_add<0x4e, 0xf>(v);  // quad_perm [2,3,0,1]
+    v = dpp_add<0x141, 0xf>(v); // row_half_mirror
+    v = dpp_add<0x140, 0xf>(v); // row_mirror: each row of 16 holds its sum
+    v = dpp_add<0x142, 0xa>(v); // row_bcast:15 into rows 1 and 3
+    v = dpp_add<0x143, 0xc>(v); // row_bcast:31 into rows 2 and 3: lane 63 holds the total
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// This lane's place in the window.
+struct Geo {
+    int W, H, k, r0, nr;
+};
+
+__device__ __forceinline__ Geo geometry(const Cfg &c, int lane)
+{
+    Geo g;
+    g.W = c.w;
+    g.H = c.h;
+    const int groups = 64 / c.w, chunk = (c.h + groups - 1) / groups;
+    const int grp = lane / c.w;
+    g.k = lane - grp * c.w;
+    g.r0 = grp * chunk;
+    int nr = c.h - g.r0;
+    nr = nr < chunk ? nr : chunk;
+    g.nr = grp < groups && nr > 0 ? nr : 0;
+    return g;
+}
+
+template <int MAXR> struct Tile {
+    float t[MAXR], tx[MAXR], ty[MAXR];
+};
+
+// The template at (px, py) of level lv: S = bilinear window over rows -1..h, columns -1..w; T = S interior,
+// Tx, Ty = Sobel(S) / 512; returns the wave totals of Tx^2, Tx Ty, Ty^2 (build_vx, lk_vx.c:36-97).
+template <int MAXR, typename T>
+__device__ __forceinline__ void build(const Level &lv, double px, double py, const Geo &g, int lane, float *S,
+                                      Tile<MAXR> &tl, float *G)
+{
+    const double flx = floor(px), fly = floor(py);
+    const float fx = (float)(px - flx), fy = (float)(py - fly);
+    const int SW = g.W + 2, SH = g.H + 2;
+    const int xs = (int)flx - g.W / 2 - 1, ys = (int)fly - g.H / 2 - 1;
+    const T *base = (const T *)lv.p;
+    const int nS = SW * SH, nb = (nS + 63) / 64;
+#pragma unroll 3
+    for (int blk = 0; blk < nb; blk++) {
+        const int i = lane + 64 * blk, ic = i < nS ? i : nS - 1;
+        const int r = ic / SW, cc = ic - r * SW;
+        const int y = clampi(ys + r, 0, lv.h - 2), x = clampi(xs + cc, 0, lv.w - 2);
+        const T *p = base + (size_t)y * lv.pitch + x;
+        const float a0 = Ld<T>::get(p[0]), a1 = Ld<T>::get(p[1]);
+        const float b0 = Ld<T>::get(p[lv.pitch]), b1 = Ld<T>::get(p[lv.pitch + 1]);
+        const float top = fmaf(a1 - a0, fx, a0), bot = fmaf(b1 - b0, fx, b0);
+        if (i < nS)
+            S[i] = fmaf(bot - top, fy, top);
+    }
+    wave_sync();
+    float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+#pragma unroll
+    for (int i = 0; i < MAXR; i++) {
+        if (i < g.nr) {
+            const float *u = S + (g.r0 + i) * SW + g.k, *m = u + SW, *d = m + SW;
+            const float du = u[2] - u[0], dm = m[2] - m[0], dd = d[2] - d[0];
+            const float tx = fmaf(dm, 2.0f, du + dd) * (1.0f / 512.0f);
+            const float vl = d[0] - u[0], vc = d[1] - u[1], vr = d[2] - u[2];
+            const float ty = fmaf(vc, 2.0f, vl + vr) * (1.0f / 512.0f);
+            tl.t[i] = m[1];
+            tl.tx[i] = tx;
+            tl.ty[i] = ty;
+            sxx = fmaf(tx, tx, sxx);
+            sxy = fmaf(tx, ty, sxy);
+            syy = fmaf(ty, ty, syy);
+        }
+    }
+    G[0] = wave_sum(sxx);
+    G[1] = wave_sum(sxy);
+    G[2] = wave_sum(syy);
+    wave_sync(); // S is rewritten by the next build
+}
+
+// J sampled at (qx, qy): It = T - J; RES = false: totals of It Tx and It Ty (iterate_vx, lk_vx.c:100-140);
+// RES = true: totals of |It| / 64 and (It / 64)^2 (residual_vx, lk_vx.c:142-175).
+template <int MAXR, typename T, bool RES>
+__device__ __forceinline__ void sample(const Level &lv, double qx, double qy, const Geo &g, const Tile<MAXR> &tl,
+                                       float *b)
+{
+    const double flx = floor(qx), fly = floor(qy);
+    const float fx = (float)(qx - flx), fy = (float)(qy - fly);
+    const int x = clampi((int)flx - g.W / 2 + g.k, 0, lv.w - 2);
+    const int y0 = (int)fly - g.H / 2 + g.r0;
+    const T *col = (const T *)lv.p + x;
+    const size_t pitch = (size_t)lv.pitch;
+    // every load of the pass issued at once (rows clamped to the level, so idle lanes read valid memory too)
+    float a0[MAXR + 1], a1[MAXR + 1];
+#pragma unroll
+    for (int i = 0; i <= MAXR; i++) {
+        const T *p = col + (size_t)clampi(y0 + i, 0, lv.h - 1) * pitch;
+        a0[i] = Ld<T>::get(p[0]);
+        a1[i] = Ld<T>::get(p[1]);
+    }
+    float s0 = 0.0f, s1 = 0.0f;
+    float above = fmaf(a1[0] - a0[0], fx, a0[0]);
+#pragma unroll
+    for (int i = 0; i < MAXR; i++) {
+        const float here = fmaf(a1[i + 1] - a0[i + 1], fx, a0[i + 1]);
+        const float it = tl.t[i] - fmaf(here - above, fy, above);
+        const bool on = i < g.nr;
+        if (RES) {
+            const float q = it * (1.0f / 64.0f);
+            s0 = on ? s0 + fabsf(q) : s0;
+            s1 = on ? fmaf(q, q, s1) : s1;
+        } else {
+            s0 = on ? fmaf(it, tl.tx[i], s0) : s0;
+            s1 = on ? fmaf(it, tl.ty[i], s1) : s1;
+        }
+        above = here;
+    }
+    b[0] = wave_sum(s0);
+    b[1] = wave_sum(s1);
+}
+
+struct Track {
+    float x, y, min_eig, residual, cxx, cxy, cyy;
+    int iters, status;
+};
+
+// The 2x2 system of one level: eigenvalue test and inverse. MIXED: fp32 unless G's condition number passes 1e4.
+struct Solve {
+    double ia, ib, ic, det, eig;
+};
+
+template <int SOLVE>
+__device__ __forceinline__ Solve solve2(double gxx, double gxy, double gyy, double area)
+{
+    Solve s;
+    if (SOLVE == VTG_SOLVE_MIXED) {
+        const float fxx = (float)gxx, fxy = (float)gxy, fyy = (float)gyy;
+        const float tr = fxx + fyy, df = fxx - fyy, rt = sqrtf(df * df + 4.0f * fxy * fxy);
+        const float lo = 0.5f * (tr - rt), hi = 0.5f * (tr + rt);
+        if (lo > 0.0f && hi < 1e4f * lo) {
+            const float det = fxx * fyy - fxy * fxy;
+            const float inv = 1.0f / det;
+            s.det = det;
+            s.eig = (double)(lo / (float)area);
+            s.ia = fyy * inv;
+            s.ib = -fxy * inv;
+            s.ic = fxx * inv;
+            return s;
+        }
+    }
+    s.det = gxx * gyy - gxy * gxy;
+    const double tr = gxx + gyy, df = gxx - gyy;
+    s.eig = (tr - sqrt(df * df + 4.0 * gxy * gxy)) * 0.5 / area;
+    s.ia = gyy / s.det;
+    s.ib = -gxy / s.det;
+    s.ic = gxx / s.det;
+    return s;
+}
+
+// One point from pyramid I to J: the reference's level walk (lk_driver.inc:7-109).
+template <int MAXR, typename T0, typename T1, int SOLVE>
+__device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c, const Geo &g, int lane, float *S,
+                      float x, float y, float gx0, float gy0, Track &o)
+{
+    Tile<MAXR> tl;
+    double gx = ldexp((double)gx0, -top), gy = ldexp((double)gy0, -top);
+    const double area = (double)(c.w * c.h), eps2 = (double)c.eps * (double)c.eps;
+    const int mx = c.w / 2 + 1, my = c.h / 2 + 1;
+    double det0 = 0, gxx0 = 0, gxy0 = 0, gyy0 = 0;
+    int status = 0, iters = 0;
+    bool valid0 = false;
+    float eig0 = 0.0f;
+    for (int L = top; L >= 0; L--) {
+        const double px = ldexp((double)x, -L), py = ldexp((double)y, -L);
+        const Level &li = I.l[L], &lj = J.l[L];
+        if (px < mx || py < my || px >= (double)(li.w - mx - 1) || py >= (double)(li.h - my - 1)) {
+            if (L == 0)
+                status |= LK_OUT;
+            else
+                gx *= 2, gy *= 2;
+            continue;
+        }
+        float G[3];
+        if (L > 0)
+            build<MAXR, T1>(li, px, py, g, lane, S, tl, G);
+        else
+            build<MAXR, T0>(li, px, py, g, lane, S, tl, G);
+        const double gxx = G[0], gxy = G[1], gyy = G[2];
+        const Solve s = solve2<SOLVE>(gxx, gxy, gyy, area);
+        if (L == 0)
+            eig0 = (float)s.eig;
+        if (s.eig < (double)c.min_eig || s.det <= 0) {
+            if (L == 0)
+                status |= LK_LOWEIG;
+            else
+                gx *= 2, gy *= 2;
+            continue;
+        }
+        if (L == 0) {
+            det0 = gxx * gyy - gxy * gxy, gxx0 = gxx, gxy0 = gxy, gyy0 = gyy;
+            valid0 = true;
+        }
+        const double lx = (double)(lj.w - mx - 1), ly = (double)(lj.h - my - 1);
+        double vx = 0, vy = 0;
+        for (int it = 0; it < c.max_iter; it++) {
+            const double qx = px + gx + vx, qy = py + gy + vy;
+            if (qx < mx || qy < my || qx >= lx || qy >= ly) {
+                if (L == 0)
+                    status |= LK_OUT, valid0 = false;
+                break;
+            }
+            float b[2];
+            if (L > 0)
+                sample<MAXR, T1, false>(lj, qx, qy, g, tl, b);
+            else
+                sample<MAXR, T0, false>(lj, qx, qy, g, tl, b);
+            const double bx = (double)b[0] * (1.0 / 64.0), by = (double)b[1] * (1.0 / 64.0);
+            const double ex = s.ia * bx + s.ib * by, ey = s.ib * bx + s.ic * by;
+            vx += ex;
+            vy += ey;
+            iters++;
+            if (ex * ex + ey * ey < eps2)
+                break;
+        }
+        gx += vx;
+        gy += vy;
+        if (L > 0)
+            gx *= 2, gy *= 2;
+    }
+    const double qx = (double)x + gx, qy = (double)y + gy;
+    o.x = (float)qx;
+    o.y = (float)qy;
+    o.min_eig = eig0;
+    o.iters = iters;
+    o.residual = 0.0f;
+    o.cxx = o.cxy = o.cyy = 0.0f;
+    if (valid0 && !status) {
+        const Level &l0 = J.l[0];
+        if (qx < mx || qy < my || qx >= (double)(l0.w - mx - 1) || qy >= (double)(l0.h - my - 1)) {
+            status |= LK_OUT;
+        } else {
+            float r[2];
+            sample<MAXR, T0, true>(l0, qx, qy, g, tl, r);
+            const double n = area;
+            o.residual = (float)((double)r[0] / n);
+            if (o.residual > c.max_res)
+                status |= LK_RESIDUAL;
+            const double s2 = ((double)r[1] / n) * area / (area - 2);
+            o.cxx = (float)(s2 * gyy0 / det0);
+            o.cxy = (float)(-s2 * gxy0 / det0);
+            o.cyy = (float)(s2 * gxx0 / det0);
+        }
+    }
+    o.status = status;
+}
+
+template <int MAXR, typename T0, typename T1, int SOLVE>
+__global__ __launch_bounds__(256) void lk_kernel(LkArgs a)
+{
+    extern __shared__ float smem[];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int pt = blockIdx.x * 4 + wave;
+    if (pt >= a.n)
+        return;
+    float *S = smem + wave * a.s_floats;
+    const Geo g = geometry(a.cfg, lane);
+    const float x = a.xy[2 * pt], y = a.xy[2 * pt + 1];
+    const float gx = a.guess ? a.guess[2 * pt] : 0.0f, gy = a.guess ? a.guess[2 * pt + 1] : 0.0f;
+    Track o;
+    track<MAXR, T0, T1, SOLVE>(a.I, a.J, a.top, a.cfg, g, lane, S, x, y, gx, gy, o);
+    if (a.fb_max > 0.0f && o.status == 0) {
+        // back from J to I, the template rebuilt from J (lk_driver.inc:111-121)
+        Track b;
+        track<MAXR, T0, T1, SOLVE>(a.J, a.I, a.top, a.cfg, g, lane, S, o.x, o.y, (float)((double)x - (double)o.x),
+                                   (float)((double)y - (double)o.y), b);
+        const double ex = (double)b.x - (double)x, ey = (double)b.y - (double)y;
+        if (b.status || ex * ex + ey * ey > (double)a.fb_max * (double)a.fb_max)
+            o.status |= LK_FLOWBACK;
+    }
+    if (lane == 0) {
+        lk_out r;
+        r.x = o.x;
+        r.y = o.y;
+        r.min_eig = o.min_eig;
+        r.residual = o.residual;
+        r.cov_xx = o.cxx;
+        r.cov_xy = o.cxy;
+        r.cov_yy = o.cyy;
+        r.iters = (uint16_t)o.iters;
+        r.status = (uint16_t)o.status;
+        a.out[pt] = r;
+    }
+}
+
+template <int MAXR, int SOLVE>
+int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, size_t lds)
+{
+    switch (storage) {
+    case VTG_STORE_Q6:
+        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, uint16_t, SOLVE>), grid, dim3(256), lds, c->stream, a);
+        break;
+    case VTG_STORE_F32:
+        hipLaunchKernelGGL((lk_kernel<MAXR, float, float, SOLVE>), grid, dim3(256), lds, c->stream, a);
+        break;
+    case VTG_STORE_F16:
+        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __half, SOLVE>), grid, dim3(256), lds, c->stream, a);
+        break;
+    case VTG_STORE_BF16:
+        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __hip_bfloat16, SOLVE>), grid, dim3(256), lds, c->stream, a);
+        break;
+    default:
+        return fail(c, VTG_EINVAL, "storage %d", storage);
+    }
+    VTG_HIP(c, hipGetLastError());
+    return VTG_OK;
+}
+
+template <int SOLVE>
+int dispatch_rows(vtg_ctx *c, const LkArgs &a, int storage, int chunk, dim3 grid, size_t lds)
+{
+    if (chunk <= 4)
+        return dispatch_storage<4, SOLVE>(c, a, storage, grid, lds);
+    if (chunk <= 8)
+        return dispatch_storage<8, SOLVE>(c, a, storage, grid, lds);
+    if (chunk <= 16)
+        return dispatch_storage<16, SOLVE>(c, a, storage, grid, lds);
+    return dispatch_storage<32, SOLVE>(c, a, storage, grid, lds);
+}
+
+}  // namespace
+
+int launch_lk(vtg_ctx *c, const PyrView &I, const PyrView &J, int top, const lk_cfg *cfg, int n, const float *xy,
+              const float *guess, float fb_max, int solve_mode, lk_out *out)
+{
+    if (n <= 0)
+        return VTG_OK;
+    LkArgs a;
+    a.I = I;
+    a.J = J;
+    a.cfg = Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->eps, cfg->min_eig, cfg->max_residual};
+    a.top = top;
+    a.n = n;
+    a.s_floats = (cfg->w + 2) * (cfg->h + 2);
+    a.xy = xy;
+    a.guess = guess;
+    a.fb_max = fb_max;
+    a.out = out;
+    const int groups = 64 / cfg->w, chunk = (cfg->h + groups - 1) / groups;
+    const dim3 grid((n + 3) / 4);
+    const size_t lds = (size_t)4 * a.s_floats * sizeof(float);
+    hipEvent_t ev;
+    timer_begin(c, "lk_track", &ev);
+    const int rc = solve_mode == VTG_SOLVE_MIXED ? dispatch_rows<VTG_SOLVE_MIXED>(c, a, I.storage, chunk, grid, lds)
+                                                 : dispatch_rows<VTG_SOLVE_F64>(c, a, I.storage, chunk, grid, lds);
+    timer_end(c, "lk_track", ev);
+    return rc;
+}
+
+}  // namespace vtg
+
+using namespace vtg;
+
+extern "C" {
+
+int vtg_lk_track_ex(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top, const struct lk_cfg *cfg, int n,
+                    const float *xy, const float *guess, float fb_max, int solve_mode, struct lk_out *out)
+{
+    if (!c || !I || !J || !cfg || (n > 0 && (!xy || !out)) || n < 0)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: bad argument");
+    if (cfg->w < 5 || cfg->w > LK_MAX_W || cfg->h < 5 || cfg->h > LK_MAX_H || cfg->max_iter < 0)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: window %dx%d (5..%d x 5..%d), max_iter %d", cfg->w, cfg->h,
+                    LK_MAX_W, LK_MAX_H, cfg->max_iter);
+    if (top < 0 || top >= I->levels || top >= J->levels)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: top %d with %d / %d levels", top, I->levels, J->levels);
+    if (I->storage != J->storage)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: pyramids of different storage");
+    if (solve_mode != VTG_SOLVE_F64 && solve_mode != VTG_SOLVE_MIXED)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: solve mode %d", solve_mode);
+    for (int l = 0; l <= top; l++)
+        if (I->lv[l].w != J->lv[l].w || I->lv[l].h != J->lv[l].h)
+            return fail(c, VTG_EINVAL, "vtg_lk_track: level %d sizes differ", l);
+    return launch_lk(c, view(I), view(J), top, cfg, n, xy, guess, fb_max, solve_mode, out);
+}
+
+int vtg_lk_track(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top, const struct lk_cfg *cfg, int n,
+                 const float *xy, const float *guess, float fb_max, struct lk_out *out)
+{
+    return vtg_lk_track_ex(c, I, J, top, cfg, n, xy, guess, fb_max, VTG_SOLVE_F64, out);
+}
+
+}  // extern "C"
+
+extern "C" int vtg_lk_track_planes(vtg_ctx *c, const struct vt_plane *I, const struct vt_plane *J, int top,
+                                   const struct lk_cfg *cfg, int n, const float *x, const float *y, const float *gx,
+                                   const float *gy, float fb_max, struct lk_out *out)
+{
+    if (!c || !I || !J || !cfg || n < 0 || (n > 0 && (!x || !y || !out)) || top < 0 || top >= VTG_MAX_LEVELS)
+        return fail(c, VTG_EINVAL, "vtg_lk_track_planes: bad argument");
+    if (n == 0)
+        return VTG_OK;
+    vtg_pyr *pi = nullptr, *pj = nullptr;
+    int rc = pyr_from_planes(c, I, top + 1, &pi);
+    if (!rc)
+        rc = pyr_from_planes(c, J, top + 1, &pj);
+    std::vector<float> xy((size_t)2 * n), gs;
+    for (int i = 0; i < n; i++)
+        xy[2 * i] = x[i], xy[2 * i + 1] = y[i];
+    if (gx || gy) {
+        gs.resize((size_t)2 * n);
+        for (int i = 0; i < n; i++)
+            gs[2 * i] = gx ? gx[i] : 0.0f, gs[2 * i + 1] = gy ? gy[i] : 0.0f;
+    }
+    void *dev = nullptr;
+    const size_t xyb = xy.size() * sizeof(float), outb = (size_t)n * sizeof(lk_out);
+    if (!rc) {
+        hipError_t e = hipMalloc(&dev, 2 * xyb + outb);
+        if (e != hipSuccess)
+            rc = hip_fail(c, e, "vtg_lk_track_planes: hipMalloc");
+    }
+    if (!rc) {
+        float *dxy = (float *)dev, *dg = gs.empty() ? nullptr : (float *)((char *)dev + xyb);
+        lk_out *dout = (lk_out *)((char *)dev + 2 * xyb);
+        hipError_t e = hipMemcpyAsync(dxy, xy.data(), xyb, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess && dg)
+            e = hipMemcpyAsync(dg, gs.data(), xyb, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess)
+            rc = hip_fail(c, e, "vtg_lk_track_planes: upload");
+        if (!rc)
+            rc = vtg_lk_track(c, pi, pj, top, cfg, n, dxy, dg, fb_max, dout);
+        if (!rc) {
+            e = hipMemcpyAsync(out, dout, outb, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess)
+                rc = hip_fail(c, e, "vtg_lk_track_planes: download");
+        }
+    }
+    if (dev) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(dev);
+    }
+    if (pi)
+        vtg_pyr_destroy(c, pi);
+    if (pj)
+        vtg_pyr_destroy(c, pj);
+    return rc;
+}

@@ -1,0 +1,874 @@
+// ncc.hip: zero-mean NCC template search on the GPU (search_at / tk_correlate, src/kern/track.c:184-208; the cell map
+// OP_SEARCH, track.c:301-329; search_slot / search_half / peak_offset, track.c:1615-1726; the template,
+// store_appearance, track.c:935-974).
+//
+// The reference scores a window as st / sqrt(norm * var) with window values taken relative to its first pixel
+// v = p - p0, st = sum v t, var = sum v^2 - (sum v)^2 / n, in fp32 lane sums. At level 0 every pixel is 64 x a byte,
+// so with u = bytes and t = fl(64 u_t - mean) the same sums are exact integers up to one known term:
+//     st  = 4096 (sum u u_t - u0 sum u_t) - 64 mean (sum u - n u0)       [+ 64 sum (u - u0) (t - 64 u_t + mean)]
+//     sum v = 64 (sum u - n u0),   sum v^2 = 4096 (sum u^2 - 2 u0 sum u + n u0^2)
+// The bracketed term is the fp32 rounding of the stored template (|t - 64 u_t + mean| <= 2^-11 for |t| < 2^14):
+// at most 64 * 255 * 2^-11 per pixel, random in sign; it is left out, which moves a score by ~1e-8 (bounded by
+// 1.3e-5 only if every rounding had the same sign and the window were at the energy floor).
+// The fast search computes sum u u_t with v_dot4_u32_u8 (4 byte products per lane per instruction, exact u32), and
+// sum u, sum u^2 per row with v_sad_u8 / dot4 as each frame row enters the wave's registers; the finish is in fp64.
+//
+// Fast kernel layout: a workgroup owns 256 x 16 template corners = 16 cells of the map. Its frame tile (corners plus
+// the template's extent) sits in LDS as 32-bit words. Wave s (0..3) takes corners x = X0 + 4 q + s for lane q, so all
+// lanes of a wave share one byte alignment: the template is pre-shifted by that alignment (4 copies, tw/4 + 1 words per
+// row) and read as uniform scalar loads, and each lane reads tw/4 + 1 aligned words per frame row. A lane keeps 8 frame
+// rows in registers and slides them down, so each row read from LDS feeds 8 corners; two passes of 8 rows make 16.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include "vtg_internal.hpp"
+
+namespace vtg {
+
+namespace {
+
+constexpr int NX = 256, NY = 16, YB = 8;
+
+template <typename T> struct Q6v;
+template <> struct Q6v<uint8_t> {
+    static __device__ __forceinline__ float get(uint8_t v) { return (float)v * 64.0f; }
+};
+template <> struct Q6v<uint16_t> {
+    static __device__ __forceinline__ float get(uint16_t v) { return (float)v; }
+};
+template <> struct Q6v<float> {
+    static __device__ __forceinline__ float get(float v) { return v; }
+};
+template <> struct Q6v<__half> {
+    static __device__ __forceinline__ float get(__half v) { return __half2float(v) * 64.0f; }
+};
+template <> struct Q6v<__hip_bfloat16> {
+    static __device__ __forceinline__ float get(__hip_bfloat16 v) { return __bfloat162float(v) * 64.0f; }
+};
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    return v;
+}
+
+// ---------------------------------------------------------------- template
+
+template <typename T> struct Exact { static constexpr bool value = false; };
+template <> struct Exact<uint8_t> { static constexpr bool value = true; };
+template <> struct Exact<uint16_t> { static constexpr bool value = true; };
+template <> struct Exact<float> { static constexpr bool value = true; };
+
+// The reference's 16-lane horizontal sum: pairs i^1, i^2, i^4, i^8, every lane updated each step (vx.h:213-221).
+__device__ __forceinline__ float tree16(const float *p)
+{
+    float a[16], b[16];
+    for (int i = 0; i < 16; i++)
+        a[i] = p[i];
+    for (int st = 1; st < 16; st <<= 1) {
+        for (int i = 0; i < 16; i++)
+            b[i] = a[i] + a[i ^ st];
+        for (int i = 0; i < 16; i++)
+            a[i] = b[i];
+    }
+    return a[0];
+}
+
+// Pixels as Q6 floats; mean and norm by the reference's 16-lane order: lane j sums columns j, j+16, ... of every
+// row in row order, then the tree (store_appearance, track.c:948-966). For integer-valued levels the lane sums are
+// exact integers (at most 256 x 16,320 < 2^24), so they are formed in any order; the sum of squares of the zero-mean
+// values is not, and runs as 16 sequential chains.
+template <typename T>
+__global__ __launch_bounds__(256) void ncc_template_kernel(Level lv, int x0, int y0, int tw, int th, float *t, float *raw,
+                                                           float *out_norm_mean, long long *out_sum_u,
+                                                           uint32_t *words, int wpr)
+{
+    __shared__ float px[64 * 64];
+    __shared__ int ipart[256];
+    __shared__ float part[16];
+    __shared__ float mean_s;
+    const int tid = threadIdx.x, n = tw * th;
+    const T *base = (const T *)lv.p;
+    for (int i = tid; i < n; i += 256) {
+        const int r = i / tw, k = i - r * tw;
+        const float v = Q6v<T>::get(base[(size_t)(y0 + r) * lv.pitch + x0 + k]);
+        px[i] = v;
+        raw[i] = v;
+    }
+    __syncthreads();
+    if (words) {
+        // byte b of word m of shift s, row r: template byte at column 4 m + b - s, 0 outside (level 0: Q6 = 64 u)
+        for (int i = tid; i < 4 * th * wpr; i += 256) {
+            const int sft = i / (th * wpr), rem = i - sft * th * wpr, r = rem / wpr, m = rem - r * wpr;
+            uint32_t w = 0;
+            for (int b = 0; b < 4; b++) {
+                const int col = 4 * m + b - sft;
+                if (col >= 0 && col < tw)
+                    w |= (uint32_t)((int)px[r * tw + col] >> 6) << (8 * b);
+            }
+            words[i] = w;
+        }
+    }
+    if constexpr (Exact<T>::value) {
+        const int j = tid & 15, g = tid >> 4;
+        int s = 0;
+        for (int r = g; r < th; r += 16)
+            for (int k = 0; k < tw; k += 16)
+                s += (int)px[r * tw + k + j];
+        ipart[tid] = s;
+        __syncthreads();
+        if (tid < 16) {
+            int tot = 0;
+            for (int q = 0; q < 16; q++)
+                tot += ipart[q * 16 + tid];
+            part[tid] = (float)tot;
+        }
+    } else {
+        if (tid < 16) {
+            float s = 0.0f;
+            for (int r = 0; r < th; r++)
+                for (int k = 0; k < tw; k += 16)
+                    s = s + px[r * tw + k + tid];
+            part[tid] = s;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const float sum = tree16(part);
+        mean_s = (float)((double)sum / (double)n);
+        if (words) {
+            long long tot = 0; // the lane sums are exact integers; their float tree need not be
+            for (int q = 0; q < 16; q++)
+                tot += (long long)part[q];
+            *out_sum_u = tot >> 6;
+        }
+    }
+    __syncthreads();
+    const float mean = mean_s;
+    for (int i = tid; i < n; i += 256) {
+        const float z = px[i] - mean;
+        px[i] = z;
+        t[i] = z;
+    }
+    __syncthreads();
+    if (tid < 16) {
+        float s = 0.0f;
+        for (int r = 0; r < th; r++)
+            for (int k = 0; k < tw; k += 16) {
+                const float z = px[r * tw + k + tid];
+                s = fmaf(z, z, s);
+            }
+        part[tid] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        out_norm_mean[0] = tree16(part);
+        out_norm_mean[1] = mean;
+    }
+}
+
+// ---------------------------------------------------------------- scores at given corners, the reference's arithmetic
+
+// The score of one corner by 16 lanes that stand for the reference's 16 vector lanes: lane j sums columns j, j+16, ...
+// of each row in row order with fp32 fused multiply-adds, the group sums with the same pair tree, and the finish is
+// fp64 (the reference's long double) (search_at, track.c:184-206; search_half, track.c:1615-1647 with half set and
+// (hx, hy) in half pixels). Four groups of a wave score four corners; every lane of the wave must call it.
+template <typename T>
+__device__ double score16(const Level &lv, const float *t, int tw, int th, double norm, int hx, int hy, bool half,
+                          int lane)
+{
+    const int j = lane & 15;
+    const int x = half ? (hx >> 1) : hx, y = half ? (hy >> 1) : hy;
+    const int ox = half ? (hx & 1) : 0, oy = half ? (hy & 1) : 0;
+    const int ext = half ? 1 : 0;
+    const bool fits = !(x < 1 || y < 1 || x + tw + ext >= lv.w - 1 || y + th + ext >= lv.h - 1);
+    float st = 0.0f, sj = 0.0f, sjj = 0.0f;
+    if (fits) {
+        const T *base = (const T *)lv.p;
+        const size_t pitch = (size_t)lv.pitch;
+        const float first = Q6v<T>::get(base[(size_t)y * pitch + x]);
+        for (int r = 0; r < th; r++) {
+            const T *row = base + (size_t)(y + r) * pitch + x + j;
+            for (int k = 0; k < tw; k += 16) {
+                float v = Q6v<T>::get(row[k]);
+                if (ox)
+                    v = (v + Q6v<T>::get(row[k + 1])) * 0.5f;
+                if (oy) {
+                    float w = Q6v<T>::get(row[k + pitch]);
+                    if (ox)
+                        w = (w + Q6v<T>::get(row[k + pitch + 1])) * 0.5f;
+                    v = (v + w) * 0.5f;
+                }
+                v = v - first;
+                st = fmaf(v, t[r * tw + k + j], st);
+                sj = sj + v;
+                sjj = fmaf(v, v, sjj);
+            }
+        }
+    }
+    for (int step = 1; step < 16; step <<= 1) {
+        st = st + __shfl_xor(st, step);
+        sj = sj + __shfl_xor(sj, step);
+        sjj = sjj + __shfl_xor(sjj, step);
+    }
+    if (!fits)
+        return -1.0;
+    const double n = (double)(tw * th), a = st, b = sj, c = sjj;
+    const double var = c - b * b / n;
+    return (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : a / sqrt(norm * var);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void ncc_score_kernel(Level lv, const float *t, int tw, int th, const float *nm,
+                                                        int n, const int *xy, int half, double *out)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int first = (blockIdx.x * 4 + wave) * 4;
+    if (first >= n)
+        return;
+    const int i = first + (lane >> 4);
+    const bool valid = i < n;
+    const double v = score16<T>(lv, t, tw, th, (double)nm[0], valid ? xy[2 * i] : -64, valid ? xy[2 * i + 1] : -64,
+                                half != 0, lane);
+    if (valid && (lane & 15) == 0)
+        out[i] = v;
+}
+
+// ---------------------------------------------------------------- cell map, generic: one wave per cell
+
+// Any level type, any template: the cell's corners in row-major order, four at a time, the first best kept by a
+// strict > (OP_SEARCH, track.c:301-329), scores in the reference's arithmetic.
+template <typename T>
+__global__ __launch_bounds__(256) void ncc_cells_generic(Level lv, const float *t, int tw, int th, const float *nm,
+                                                         int x0, int x1, int y0, int y1, int step, int cols, int ncell,
+                                                         vtg_cell *cells)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int cell = blockIdx.x * 4 + wave;
+    if (cell >= ncell)
+        return;
+    const int cy0 = y0 + (cell / cols) * 16, cx0 = x0 + (cell % cols) * 16;
+    const int cy1 = cy0 + 16 <= y1 + 1 ? cy0 + 16 : y1 + 1, cx1 = cx0 + 16 <= x1 + 1 ? cx0 + 16 : x1 + 1;
+    const int nx = (cx1 - cx0 + step - 1) / step, ny = (cy1 - cy0 + step - 1) / step, np = nx * ny;
+    float top = -2.0f;
+    int bx = cx0, by = cy0;
+    for (int b = 0; b < np; b += 4) {
+        const int p = b + (lane >> 4);
+        const int x = p < np ? cx0 + (p % nx) * step : -64, y = p < np ? cy0 + (p / nx) * step : -64;
+        const float v = (float)score16<T>(lv, t, tw, th, (double)nm[0], x, y, false, lane);
+        for (int q = 0; q < 4 && b + q < np; q++) {
+            const float vq = __shfl(v, 16 * q);
+            if (vq > top)
+                top = vq, bx = cx0 + ((b + q) % nx) * step, by = cy0 + ((b + q) / nx) * step;
+        }
+    }
+    if (lane == 0)
+        cells[cell] = vtg_cell{top, bx, by};
+}
+
+// ---------------------------------------------------------------- cell map, fast: u8 level, dot4
+
+struct FastArgs {
+    const uint8_t *img;
+    int w, h, pitch;
+    const uint32_t *words;
+    int tw, th;
+    int x0, x1, y0, y1, step, cols, rows;
+    const float *nm; // norm, mean
+    const long long *sum_t;
+    vtg_cell *cells;
+};
+
+__device__ __forceinline__ bool better(float v, int y, int x, float v2, int y2, int x2)
+{
+    return v > v2 || (v == v2 && (y < y2 || (y == y2 && x < x2)));
+}
+
+template <int WPR>
+__global__ __launch_bounds__(256) void ncc_cells_fast(FastArgs a)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ float cv[4][16];
+    __shared__ int cx[4][16], cyy[4][16];
+    constexpr int LW = NX / 4 + WPR + 1;
+    const int LR = NY + a.th - 1;
+    const int X0 = a.x0 + blockIdx.x * NX, Y0 = a.y0 + blockIdx.y * NY;
+    const int A = X0 & ~3, off = X0 - A;
+    for (int i = threadIdx.x; i < LR * LW; i += 256) {
+        const int r = i / LW, c = i - r * LW;
+        const int gy = Y0 + r < a.h ? Y0 + r : a.h - 1;
+        const int gx = A + 4 * c;
+        uint32_t v = 0;
+        if (gx + 3 < a.pitch)
+            v = *(const uint32_t *)(a.img + (size_t)gy * a.pitch + gx);
+        lds[i] = v;
+    }
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int sh = (off + wave) & 3, wb = (off + wave) >> 2;
+    const uint32_t *T = a.words + (size_t)sh * a.th * WPR;
+    const uint32_t mask0 = 0xffffffffu << (8 * sh), maskl = sh ? (0xffffffffu >> (32 - 8 * sh)) : 0u;
+    const int x = X0 + 4 * lane + wave;
+    const bool x_ok = x <= a.x1 && ((x - a.x0) % a.step) == 0;
+    const double n = (double)(a.tw * a.th), norm = (double)a.nm[0], mean = (double)a.nm[1];
+    const double sum_t = (double)a.sum_t[0];
+    float best_v = -3.0f;
+    int best_x = 0, best_y = 0;
+    const uint32_t *lbase = lds + lane + wb;
+    for (int sp = 0; sp < NY / YB; sp++) {
+        uint32_t F[YB][WPR];
+        uint32_t P[YB];
+        uint32_t c1 = 0, c2 = 0, s1[YB], s2[YB], e1[YB], e2[YB];
+#pragma unroll
+        for (int d = 0; d < YB; d++)
+            P[d] = 0;
+        s1[0] = s2[0] = 0;
+        // rows 0..YB-2 of the pass: into the ring, their row sums into the running totals
+#pragma unroll
+        for (int j = 0; j < YB - 1; j++) {
+            const uint32_t *src = lbase + (sp * YB + j) * LW;
+#pragma unroll
+            for (int m = 0; m < WPR; m++)
+                F[j][m] = src[m];
+#pragma unroll
+            for (int m = 0; m < WPR; m++) {
+                const uint32_t w = m == 0 ? (F[j][m] & mask0) : (m == WPR - 1 ? (F[j][m] & maskl) : F[j][m]);
+                c1 = __builtin_amdgcn_sad_u8(w, 0u, c1);
+                c2 = __builtin_amdgcn_udot4(w, w, c2, false);
+            }
+            s1[j + 1] = c1;
+            s2[j + 1] = c2;
+        }
+        // template rows r = 0 .. th-1; before step r the ring holds rows r .. r+6, step r brings in row r+7
+        auto step = [&](const int r, const int j, const bool last) __attribute__((always_inline)) {
+            const int slot = (j + YB - 1) % YB;
+            const uint32_t *src = lbase + (sp * YB + r + YB - 1) * LW;
+#pragma unroll
+            for (int m = 0; m < WPR; m++)
+                F[slot][m] = src[m];
+            uint32_t tr[WPR];
+#pragma unroll
+            for (int m = 0; m < WPR; m++)
+                tr[m] = T[r * WPR + m];
+#pragma unroll
+            for (int d = 0; d < YB; d++) {
+#pragma unroll
+                for (int m = 0; m < WPR; m++)
+                    P[d] = __builtin_amdgcn_udot4(F[(j + d) % YB][m], tr[m], P[d], false);
+            }
+#pragma unroll
+            for (int m = 0; m < WPR; m++) {
+                const uint32_t w =
+                    m == 0 ? (F[slot][m] & mask0) : (m == WPR - 1 ? (F[slot][m] & maskl) : F[slot][m]);
+                c1 = __builtin_amdgcn_sad_u8(w, 0u, c1);
+                c2 = __builtin_amdgcn_udot4(w, w, c2, false);
+            }
+            if (last) {
+                e1[j] = c1;
+                e2[j] = c2;
+            }
+        };
+        int r = 0;
+        for (; r + YB < a.th; r += YB) {
+#pragma unroll
+            for (int j = 0; j < YB; j++)
+                step(r + j, j, false);
+        }
+#pragma unroll
+        for (int j = 0; j < YB; j++)
+            step(r + j, j, true);
+        // scores of the pass's 8 corners of this lane
+#pragma unroll
+        for (int d = 0; d < YB; d++) {
+            const int y = Y0 + sp * YB + d;
+            const uint8_t *rowb = (const uint8_t *)(lds + (sp * YB + d) * LW + lane + wb);
+            const int u0 = rowb[sh];
+            const double su = (double)(e1[d] - s1[d]), sq = (double)(e2[d] - s2[d]);
+            const double pc = (double)P[d] - (double)u0 * sum_t;
+            const double sv = su - n * (double)u0;
+            const double st = 4096.0 * pc - 64.0 * mean * sv;
+            const double sj = 64.0 * sv;
+            const double sjj = 4096.0 * (sq - 2.0 * (double)u0 * su + n * (double)u0 * (double)u0);
+            const double var = sjj - sj * sj / n;
+            const double sc = (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : st / sqrt(norm * var);
+            const float v = (float)sc;
+            const bool ok = x_ok && y <= a.y1 && ((y - a.y0) % a.step) == 0;
+            if (ok && better(v, y, x, best_v, best_y, best_x))
+                best_v = v, best_x = x, best_y = y;
+        }
+    }
+    // the cell of this lane: lanes 4c .. 4c+3 of every wave; reduce the quad, then the 4 waves
+#pragma unroll
+    for (int o = 1; o < 4; o <<= 1) {
+        const float v2 = __shfl_xor(best_v, o);
+        const int x2 = __shfl_xor(best_x, o), y2 = __shfl_xor(best_y, o);
+        if (better(v2, y2, x2, best_v, best_y, best_x))
+            best_v = v2, best_x = x2, best_y = y2;
+    }
+    if ((lane & 3) == 0) {
+        cv[wave][lane >> 2] = best_v;
+        cx[wave][lane >> 2] = best_x;
+        cyy[wave][lane >> 2] = best_y;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        const int c = threadIdx.x;
+        float v = cv[0][c];
+        int bx = cx[0][c], by = cyy[0][c];
+        for (int w = 1; w < 4; w++)
+            if (better(cv[w][c], cyy[w][c], cx[w][c], v, by, bx))
+                v = cv[w][c], bx = cx[w][c], by = cyy[w][c];
+        const int col = blockIdx.x * 16 + c;
+        if (col < a.cols)
+            a.cells[(size_t)blockIdx.y * a.cols + col] = vtg_cell{v, bx, by};
+    }
+}
+
+// ---------------------------------------------------------------- finish: top 8, half-pixel refinement, peaks
+
+struct Cand { float v; int idx; };
+
+__device__ __forceinline__ bool before(const Cand &a, const Cand &b)
+{
+    return a.v > b.v || (a.v == b.v && a.idx < b.idx);
+}
+
+// What the finish kernels pass on: the top cells, then every half-pixel score of the refinement.
+struct Aux {
+    int ntop, tops[8];
+    int pad[7];
+    double hs[8 * 49];
+};
+
+// The 8 best cells by (score, cell order), scores above -2 only: what search_slot's insertion keeps
+// (track.c:1686-1694). Per thread over a strided share in index order, then a merge tree.
+__global__ __launch_bounds__(256) void ncc_top8(const vtg_cell *cells, int ncell, Aux *aux)
+{
+    __shared__ Cand lists[256][8];
+    __shared__ int nlist[256];
+    const int tid = threadIdx.x;
+    Cand loc[8];
+    int nl = 0;
+    for (int i = tid; i < ncell; i += 256) {
+        const float v = cells[i].v;
+        if (v <= -2.0f || (nl == 8 && !(v > loc[7].v)))
+            continue;
+        int at = nl < 8 ? nl++ : 7;
+        for (; at > 0 && loc[at - 1].v < v; at--)
+            loc[at] = loc[at - 1];
+        loc[at] = Cand{v, i};
+    }
+    for (int k = 0; k < 8; k++)
+        lists[tid][k] = k < nl ? loc[k] : Cand{-3.0f, 0x7fffffff};
+    nlist[tid] = nl;
+    __syncthreads();
+    for (int s = 1; s < 256; s <<= 1) {
+        if ((tid % (2 * s)) == 0) {
+            Cand m[8];
+            int ia = 0, ib = 0, nm_ = 0;
+            const int na = nlist[tid], nb = nlist[tid + s];
+            while (nm_ < 8 && (ia < na || ib < nb)) {
+                if (ib >= nb || (ia < na && before(lists[tid][ia], lists[tid + s][ib])))
+                    m[nm_++] = lists[tid][ia++];
+                else
+                    m[nm_++] = lists[tid + s][ib++];
+            }
+            for (int k = 0; k < 8; k++)
+                lists[tid][k] = k < nm_ ? m[k] : Cand{-3.0f, 0x7fffffff};
+            nlist[tid] = nm_;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        aux->ntop = nlist[0];
+        for (int k = 0; k < 8; k++)
+            aux->tops[k] = lists[0][k].idx;
+    }
+}
+
+// Every half-pixel corner within 2 step - 1 (1 at step 1) of each top cell's best, four per wave (track.c:1697-1711).
+template <typename T>
+__global__ __launch_bounds__(256) void ncc_refine(Level lv, const float *t, int tw, int th, const float *nm,
+                                                  const vtg_cell *cells, int step, Aux *aux)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int around = step > 1 ? 2 * step - 1 : 1, side = 2 * around + 1, per = side * side;
+    const int total = aux->ntop * per;
+    const int first = (blockIdx.x * 4 + wave) * 4;
+    if (first >= total)
+        return;
+    const int p = first + (lane >> 4);
+    int hx = -64, hy = -64;
+    if (p < total) {
+        const int c = p / per, q = p - c * per;
+        const vtg_cell cl = cells[aux->tops[c]];
+        hy = 2 * cl.y - around + q / side;
+        hx = 2 * cl.x - around + q % side;
+    }
+    const double v = score16<T>(lv, t, tw, th, (double)nm[0], hx, hy, true, lane);
+    if (p < total && (lane & 15) == 0)
+        aux->hs[p] = v;
+}
+
+// The pick, the second peak clear of it by half a template, and the parabola through the peak's half-pixel
+// neighbours (search_slot, track.c:1704-1725; take_back's offsets, track.c:1737-1741; peak_offset, 1650-1657).
+template <typename T>
+__global__ __launch_bounds__(64) void ncc_final(Level lv, const float *t, int tw, int th, const float *nm,
+                                                const vtg_cell *cells, int ncell, int step, const Aux *aux,
+                                                vtg_found *found)
+{
+    __shared__ int fh[2];
+    __shared__ double fn_s;
+    const int lane = threadIdx.x;
+    const int ntop = aux->ntop;
+    const int around = step > 1 ? 2 * step - 1 : 1, side = 2 * around + 1, per = side * side;
+    if (lane == 0) {
+        double fn = -2.0, sec = -2.0;
+        int fhx = 0, fhy = 0;
+        double sc[8];
+        int shx[8], shy[8];
+        for (int c = 0; c < ntop; c++) {
+            const vtg_cell cl = cells[aux->tops[c]];
+            sc[c] = -2.0;
+            shx[c] = 2 * cl.x;
+            shy[c] = 2 * cl.y;
+            for (int q = 0; q < per; q++) {
+                const double v = aux->hs[c * per + q];
+                if (v > sc[c])
+                    sc[c] = v, shx[c] = 2 * cl.x - around + q % side, shy[c] = 2 * cl.y - around + q / side;
+            }
+            if (sc[c] > fn)
+                fn = sc[c], fhx = shx[c], fhy = shy[c];
+        }
+        for (int c = 0; c < ntop; c++) {
+            const int dx = shx[c] - fhx, dy = shy[c] - fhy;
+            if ((dx * dx > tw * tw || dy * dy > th * th) && sc[c] > sec)
+                sec = sc[c];
+        }
+        found->hx = fhx;
+        found->hy = fhy;
+        found->ncc = fn;
+        found->second = sec;
+        found->cells = ncell;
+        if (ntop > 0) {
+            const vtg_cell b = cells[aux->tops[0]];
+            found->best_x = b.x;
+            found->best_y = b.y;
+            found->best_v = b.v;
+        } else {
+            found->best_x = found->best_y = 0;
+            found->best_v = -2.0f;
+        }
+        fh[0] = fhx;
+        fh[1] = fhy;
+        fn_s = fn;
+    }
+    __syncthreads();
+    const int g = lane >> 4;
+    const int hx = fh[0] + (g == 0 ? -1 : g == 1 ? 1 : 0), hy = fh[1] + (g == 2 ? -1 : g == 3 ? 1 : 0);
+    const double v = score16<T>(lv, t, tw, th, (double)nm[0], ntop > 0 ? hx : -64, hy, true, lane);
+    const double l = __shfl(v, 0), r = __shfl(v, 16), u = __shfl(v, 32), d = __shfl(v, 48);
+    if (lane == 0) {
+        auto peak = [](double a, double c, double b) {
+            const double den = a - 2.0 * c + b;
+            if (den >= 0.0)
+                return 0.0;
+            const double o = 0.5 * (a - b) / den;
+            return o < -0.5 ? -0.5 : o > 0.5 ? 0.5 : o;
+        };
+        found->ox = ntop > 0 ? peak(l, fn_s, r) : 0.0;
+        found->oy = ntop > 0 ? peak(u, fn_s, d) : 0.0;
+    }
+}
+
+template <typename F>
+int by_level_type(vtg_ctx *c, const vtg_pyr *p, int level, F &&f)
+{
+    const int eb = p->elem[level];
+    switch (p->storage) {
+    case VTG_STORE_F32: return f((float *)nullptr);
+    case VTG_STORE_Q6: return eb == 1 ? f((uint8_t *)nullptr) : f((uint16_t *)nullptr);
+    case VTG_STORE_F16: return eb == 1 ? f((uint8_t *)nullptr) : f((__half *)nullptr);
+    case VTG_STORE_BF16: return eb == 1 ? f((uint8_t *)nullptr) : f((__hip_bfloat16 *)nullptr);
+    }
+    return fail(c, VTG_EINVAL, "storage %d", p->storage);
+}
+
+}  // namespace
+
+int launch_ncc_template(vtg_ctx *c, const vtg_pyr *p, int level, int x0, int y0, vtg_tmpl *t)
+{
+    const Level lv = p->lv[level];
+    const bool u8 = p->elem[level] == 1;
+    float *nm = (float *)((char *)t->block + 0);
+    long long *su = (long long *)((char *)t->block + 16);
+    return by_level_type(c, p, level, [&](auto *tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        hipEvent_t ev;
+        timer_begin(c, "ncc_template", &ev);
+        hipLaunchKernelGGL(ncc_template_kernel<T>, dim3(1), dim3(256), 0, c->stream, lv, x0, y0, t->tw, t->th, t->t,
+                           t->raw, nm, su, u8 ? t->words : nullptr, t->words_per_row);
+        VTG_HIP(c, hipGetLastError());
+        timer_end(c, "ncc_template", ev);
+        return VTG_OK;
+    });
+}
+
+int launch_ncc_score(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out)
+{
+    if (n <= 0)
+        return VTG_OK;
+    const Level lv = p->lv[t->level];
+    const float *nm = (const float *)t->block;
+    return by_level_type(c, p, t->level, [&](auto *tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL(ncc_score_kernel<T>, dim3((n + 15) / 16), dim3(256), 0, c->stream, lv, t->t, t->tw, t->th,
+                           nm, n, xy, half, out);
+        VTG_HIP(c, hipGetLastError());
+        return VTG_OK;
+    });
+}
+
+size_t search_scratch_bytes(int ncell)
+{
+    return (((size_t)ncell * sizeof(vtg_cell) + 255) & ~(size_t)255) + 256 + ((sizeof(Aux) + 255) & ~(size_t)255);
+}
+
+// cells: the map; found, aux: device records after it in the same scratch.
+int launch_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
+                      vtg_cell *cells, vtg_found *found, void *aux_mem)
+{
+    const Level lv = p->lv[t->level];
+    const int cols = (x1 - x0) / 16 + 1, rows = (y1 - y0) / 16 + 1, ncell = cols * rows;
+    const float *nm = (const float *)t->block;
+    const long long *su = (const long long *)((const char *)t->block + 16);
+    Aux *aux = (Aux *)aux_mem;
+    const bool fast = p->elem[t->level] == 1 && t->words && (t->th % YB) == 0 && t->th >= YB &&
+                      (t->words_per_row == 17 || t->words_per_row == 9 || t->words_per_row == 5);
+    hipEvent_t ev;
+    if (fast) {
+        FastArgs a{(const uint8_t *)lv.p, lv.w, lv.h, lv.pitch, t->words, t->tw, t->th, x0, x1, y0, y1, step, cols,
+                   rows, nm, su, cells};
+        const dim3 grid((cols + 15) / 16, rows);
+        const int lw = NX / 4 + t->words_per_row + 1;
+        const size_t lds = (size_t)(NY + t->th - 1) * lw * sizeof(uint32_t);
+        timer_begin(c, "ncc_cells", &ev);
+        if (t->words_per_row == 17)
+            hipLaunchKernelGGL(ncc_cells_fast<17>, grid, dim3(256), lds, c->stream, a);
+        else if (t->words_per_row == 9)
+            hipLaunchKernelGGL(ncc_cells_fast<9>, grid, dim3(256), lds, c->stream, a);
+        else
+            hipLaunchKernelGGL(ncc_cells_fast<5>, grid, dim3(256), lds, c->stream, a);
+        VTG_HIP(c, hipGetLastError());
+        timer_end(c, "ncc_cells", ev);
+    } else {
+        int rc = by_level_type(c, p, t->level, [&](auto *tag) -> int {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            timer_begin(c, "ncc_cells_generic", &ev);
+            hipLaunchKernelGGL(ncc_cells_generic<T>, dim3((ncell + 3) / 4), dim3(256), 0, c->stream, lv, t->t, t->tw,
+                               t->th, nm, x0, x1, y0, y1, step, cols, ncell, cells);
+            VTG_HIP(c, hipGetLastError());
+            timer_end(c, "ncc_cells_generic", ev);
+            return VTG_OK;
+        });
+        if (rc)
+            return rc;
+    }
+    timer_begin(c, "ncc_top8", &ev);
+    hipLaunchKernelGGL(ncc_top8, dim3(1), dim3(256), 0, c->stream, cells, ncell, aux);
+    VTG_HIP(c, hipGetLastError());
+    timer_end(c, "ncc_top8", ev);
+    const int around = step > 1 ? 2 * step - 1 : 1, per = (2 * around + 1) * (2 * around + 1);
+    return by_level_type(c, p, t->level, [&](auto *tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        timer_begin(c, "ncc_refine", &ev);
+        hipLaunchKernelGGL(ncc_refine<T>, dim3((8 * per + 15) / 16), dim3(256), 0, c->stream, lv, t->t, t->tw, t->th,
+                           nm, cells, step, aux);
+        VTG_HIP(c, hipGetLastError());
+        timer_end(c, "ncc_refine", ev);
+        timer_begin(c, "ncc_final", &ev);
+        hipLaunchKernelGGL(ncc_final<T>, dim3(1), dim3(64), 0, c->stream, lv, t->t, t->tw, t->th, nm, cells, ncell,
+                           step, aux, found);
+        VTG_HIP(c, hipGetLastError());
+        timer_end(c, "ncc_final", ev);
+        return VTG_OK;
+    });
+}
+
+}  // namespace vtg
+
+using namespace vtg;
+
+extern "C" {
+
+int vtg_ncc_template(vtg_ctx *c, const vtg_pyr *p, int level, int x0, int y0, int tw, int th, vtg_tmpl **out)
+{
+    if (!c || !p || !out || level < 0 || level >= p->levels)
+        return fail(c, VTG_EINVAL, "vtg_ncc_template: bad argument");
+    *out = nullptr;
+    if (tw < 16 || tw > 64 || (tw & 15) || th < 1 || th > 64)
+        return fail(c, VTG_ERANGE, "vtg_ncc_template: %dx%d (width 16..64 in 16s, height 1..64)", tw, th);
+    const Level &lv = p->lv[level];
+    if (x0 < 1 || y0 < 1 || x0 + tw >= lv.w - 1 || y0 + th >= lv.h - 1)
+        return fail(c, VTG_EINVAL, "vtg_ncc_template: window at (%d,%d) does not fit level %d (%dx%d)", x0, y0, level,
+                    lv.w, lv.h);
+    vtg_tmpl *t = new vtg_tmpl();
+    t->tw = tw;
+    t->th = th;
+    t->level = level;
+    t->words_per_row = tw / 4 + 1;
+    const size_t head = 256, fbytes = (size_t)tw * th * sizeof(float);
+    const size_t wbytes = (size_t)4 * th * t->words_per_row * sizeof(uint32_t);
+    const size_t bytes = head + 2 * ((fbytes + 255) & ~(size_t)255) + wbytes;
+    hipError_t e = hipMalloc(&t->block, bytes);
+    if (e != hipSuccess) {
+        delete t;
+        return hip_fail(c, e, "vtg_ncc_template: hipMalloc");
+    }
+    t->t = (float *)((char *)t->block + head);
+    t->raw = (float *)((char *)t->t + ((fbytes + 255) & ~(size_t)255));
+    t->words = p->elem[level] == 1 ? (uint32_t *)((char *)t->raw + ((fbytes + 255) & ~(size_t)255)) : nullptr;
+    int rc = launch_ncc_template(c, p, level, x0, y0, t);
+    if (rc) {
+        (void)hipFree(t->block);
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return VTG_OK;
+}
+
+int vtg_ncc_template_update(vtg_ctx *c, vtg_tmpl *t, const vtg_pyr *p, int level, int x0, int y0)
+{
+    if (!c || !t || !p || level < 0 || level >= p->levels)
+        return fail(c, VTG_EINVAL, "vtg_ncc_template_update: bad argument");
+    const Level &lv = p->lv[level];
+    if (x0 < 1 || y0 < 1 || x0 + t->tw >= lv.w - 1 || y0 + t->th >= lv.h - 1)
+        return fail(c, VTG_EINVAL, "vtg_ncc_template_update: window at (%d,%d) does not fit level %d", x0, y0, level);
+    if ((p->elem[level] == 1) != (t->words != nullptr))
+        return fail(c, VTG_EINVAL, "vtg_ncc_template_update: level element size differs from the template's");
+    t->level = level;
+    return launch_ncc_template(c, p, level, x0, y0, t);
+}
+
+int vtg_ncc_template_destroy(vtg_ctx *c, vtg_tmpl *t)
+{
+    if (!c || !t)
+        return VTG_EINVAL;
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(t->block);
+    delete t;
+    return VTG_OK;
+}
+
+int vtg_ncc_template_read(vtg_ctx *c, const vtg_tmpl *t, float *app, float *norm)
+{
+    if (!c || !t)
+        return VTG_EINVAL;
+    float nm[2];
+    VTG_HIP(c, hipMemcpyAsync(nm, t->block, sizeof nm, hipMemcpyDeviceToHost, c->stream));
+    if (app)
+        VTG_HIP(c, hipMemcpyAsync(app, t->t, (size_t)t->tw * t->th * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    VTG_HIP(c, hipStreamSynchronize(c->stream));
+    if (norm)
+        *norm = nm[0];
+    return VTG_OK;
+}
+
+int vtg_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
+                   struct vtg_cell *cells, struct vtg_found *found)
+{
+    if (!c || !p || !t || !found || (step != 1 && step != 2) || t->level >= p->levels)
+        return fail(c, VTG_EINVAL, "vtg_ncc_search: bad argument");
+    const Level &lv = p->lv[t->level];
+    const int hi_x = lv.w - t->tw - 2, hi_y = lv.h - t->th - 2;
+    x0 = x0 < 1 ? 1 : x0;
+    x1 = x1 > hi_x ? hi_x : x1;
+    y0 = y0 < 1 ? 1 : y0;
+    y1 = y1 > hi_y ? hi_y : y1;
+    *found = vtg_found{0, 0, -2.0, -2.0, 0.0, 0.0, 0, 0, 0, -2.0f};
+    if (x1 < x0 || y1 < y0)
+        return VTG_OK;
+    const int ncell = ((x1 - x0) / 16 + 1) * ((y1 - y0) / 16 + 1);
+    void *sc = nullptr;
+    const size_t cell_bytes = ((size_t)ncell * sizeof(vtg_cell) + 255) & ~(size_t)255;
+    int rc = scratch(c, search_scratch_bytes(ncell), &sc);
+    if (rc)
+        return rc;
+    vtg_cell *cl = cells ? cells : (vtg_cell *)sc;
+    vtg_found *fd = (vtg_found *)((char *)sc + cell_bytes);
+    rc = launch_ncc_search(c, p, t, x0, x1, y0, y1, step, cl, fd, (char *)sc + cell_bytes + 256);
+    if (rc)
+        return rc;
+    void *host = nullptr;
+    rc = pinned(c, sizeof(vtg_found), &host);
+    if (rc)
+        return rc;
+    VTG_HIP(c, hipMemcpyAsync(host, fd, sizeof(vtg_found), hipMemcpyDeviceToHost, c->stream));
+    VTG_HIP(c, hipStreamSynchronize(c->stream));
+    *found = *(vtg_found *)host;
+    return VTG_OK;
+}
+
+int vtg_ncc_score(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out)
+{
+    if (!c || !p || !t || n < 0 || (n > 0 && (!xy || !out)) || t->level >= p->levels)
+        return fail(c, VTG_EINVAL, "vtg_ncc_score: bad argument");
+    return launch_ncc_score(c, p, t, n, xy, half, out);
+}
+
+}  // extern "C"
+
+extern "C" int vtg_correlate_planes(vtg_ctx *c, const struct vt_plane *pl, const float *app, int tw, int th,
+                                    double norm, int n, const int *x, const int *y, double *out)
+{
+    if (!c || !pl || !app || n < 0 || (n > 0 && (!x || !y || !out)) || tw < 1 || th < 1 || tw * th > 64 * 64)
+        return fail(c, VTG_EINVAL, "vtg_correlate_planes: bad argument");
+    if (n == 0)
+        return VTG_OK;
+    vtg_pyr *p = nullptr;
+    int rc = pyr_from_planes(c, pl, 1, &p);
+    if (rc)
+        return rc;
+    // a template record around the caller's values: norm, mean (unused) and the fp32 pixels
+    vtg_tmpl t;
+    t.tw = tw;
+    t.th = th;
+    t.level = 0;
+    const size_t fb = (size_t)tw * th * sizeof(float), xyb = (size_t)2 * n * sizeof(int);
+    std::vector<int> xy((size_t)2 * n);
+    for (int i = 0; i < n; i++)
+        xy[2 * i] = x[i], xy[2 * i + 1] = y[i];
+    void *dev = nullptr;
+    hipError_t e = hipMalloc(&dev, 256 + fb + xyb + (size_t)n * sizeof(double));
+    if (e != hipSuccess) {
+        vtg_pyr_destroy(c, p);
+        return hip_fail(c, e, "vtg_correlate_planes: hipMalloc");
+    }
+    t.block = dev;
+    t.t = (float *)((char *)dev + 256);
+    int *dxy = (int *)((char *)dev + 256 + fb);
+    double *dout = (double *)((char *)dxy + xyb);
+    const float nm[2] = {(float)norm, 0.0f};
+    e = hipMemcpyAsync(dev, nm, sizeof nm, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t.t, app, fb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dxy, xy.data(), xyb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        // norm travels as a float: the reference passes the template's float norm widened (track.c:312)
+        rc = launch_ncc_score(c, p, &t, n, dxy, 0, dout);
+        if (!rc)
+            e = hipMemcpyAsync(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (!rc && e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+    }
+    if (e != hipSuccess && !rc)
+        rc = hip_fail(c, e, "vtg_correlate_planes");
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(dev);
+    t.block = nullptr;
+    vtg_pyr_destroy(c, p);
+    return rc;
+}

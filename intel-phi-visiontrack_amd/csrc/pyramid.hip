@@ -1,0 +1,512 @@
+// pyramid.hip: the Gaussian pyramid on the GPU. Level 0 is the u8 frame (Q6 = 64 x byte); level l is the (1 4 6 4 1)^2
+// blur of level l-1 sampled at even pixels, (sum + 128) >> 8 in exact integers (img_reduce_ref, src/kern/img_ref.c:98-114),
+// with the border replicated (the reference's padding, src/kern/img_ref.c:9-30, is a clamp of the coordinates).
+//
+// One launch per level. A workgroup makes a 64 x 16 tile of the output: it stages the 131 x 35 input tile (with the 2-pixel
+// halo, coordinates clamped) in LDS as integers, sums the 5 rows at each even row, then the 5 columns at each even column.
+// The level-0 launch also writes the frame into the pyramid's level 0 from the same tile, so the frame is read once.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include "vtg_internal.hpp"
+
+namespace vtg {
+
+namespace {
+
+constexpr int TW = 64, TH = 16, IW = 2 * TW + 3, IH = 2 * TH + 3;
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Element -> value in Q6 units, and back. Exact storages carry integers; fp16 / bf16 carry grey = Q6 / 64.
+template <typename T> struct Q6;
+template <> struct Q6<uint8_t> {
+    static __device__ __forceinline__ float get(uint8_t v) { return (float)v * 64.0f; }
+};
+template <> struct Q6<uint16_t> {
+    static __device__ __forceinline__ float get(uint16_t v) { return (float)v; }
+    static __device__ __forceinline__ uint16_t put(int q) { return (uint16_t)q; }
+};
+template <> struct Q6<float> {
+    static __device__ __forceinline__ float get(float v) { return v; }
+    static __device__ __forceinline__ float put(int q) { return (float)q; }
+    static __device__ __forceinline__ float putf(float q) { return q; }
+};
+template <> struct Q6<__half> {
+    static __device__ __forceinline__ float get(__half v) { return __half2float(v) * 64.0f; }
+    static __device__ __forceinline__ __half putf(float q) { return __float2half(q * (1.0f / 64.0f)); }
+};
+template <> struct Q6<__hip_bfloat16> {
+    static __device__ __forceinline__ float get(__hip_bfloat16 v) { return __bfloat162float(v) * 64.0f; }
+    static __device__ __forceinline__ __hip_bfloat16 putf(float q) { return __float2bfloat16(q * (1.0f / 64.0f)); }
+};
+
+// Exact integer reduce: TS holds integer Q6 values (u8, u16, f32), TD an exact store (u16, f32).
+template <typename TS, typename TD, bool COPY>
+__global__ __launch_bounds__(256) void reduce_exact(const TS *__restrict__ src, int sw, int sh, int sp,
+                                                    TD *__restrict__ dst, int dw, int dh, int dp,
+                                                    TS *__restrict__ copy, int cp)
+{
+    __shared__ int tin[IH][IW + 1];
+    __shared__ int tv[TH][IW + 1];
+    const int ox0 = blockIdx.x * TW, oy0 = blockIdx.y * TH;
+    const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
+    for (int i = threadIdx.x; i < IH * IW; i += 256) {
+        const int r = i / IW, c = i - r * IW;
+        const int yy = iy0 + r, xx = ix0 + c;
+        const TS v = src[(size_t)clampi(yy, 0, sh - 1) * sp + clampi(xx, 0, sw - 1)];
+        tin[r][c] = (int)Q6<TS>::get(v);
+        if (COPY && r >= 2 && r < 2 + 2 * TH && c >= 2 && c < 2 + 2 * TW && yy < sh && xx < sw)
+            copy[(size_t)yy * cp + xx] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TH * IW; i += 256) {
+        const int j = i / IW, c = i - j * IW, r = 2 * j;
+        tv[j][c] = tin[r][c] + tin[r + 4][c] + 4 * (tin[r + 1][c] + tin[r + 3][c]) + 6 * tin[r + 2][c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TH * TW; i += 256) {
+        const int j = i / TW, k = i - j * TW;
+        const int ox = ox0 + k, oy = oy0 + j;
+        if (ox < dw && oy < dh) {
+            const int c = 2 * k;
+            const int h = tv[j][c] + tv[j][c + 4] + 4 * (tv[j][c + 1] + tv[j][c + 3]) + 6 * tv[j][c + 2];
+            dst[(size_t)oy * dp + ox] = Q6<TD>::put((h + 128) >> 8);
+        }
+    }
+}
+
+// fp16 / bf16 levels: the same filter in fp32 without the integer rounding, stored as grey.
+template <typename TS, typename TD, bool COPY>
+__global__ __launch_bounds__(256) void reduce_float(const TS *__restrict__ src, int sw, int sh, int sp,
+                                                    TD *__restrict__ dst, int dw, int dh, int dp,
+                                                    TS *__restrict__ copy, int cp)
+{
+    __shared__ float tin[IH][IW + 1];
+    __shared__ float tv[TH][IW + 1];
+    const int ox0 = blockIdx.x * TW, oy0 = blockIdx.y * TH;
+    const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
+    for (int i = threadIdx.x; i < IH * IW; i += 256) {
+        const int r = i / IW, c = i - r * IW;
+        const int yy = iy0 + r, xx = ix0 + c;
+        const TS v = src[(size_t)clampi(yy, 0, sh - 1) * sp + clampi(xx, 0, sw - 1)];
+        tin[r][c] = Q6<TS>::get(v);
+        if (COPY && r >= 2 && r < 2 + 2 * TH && c >= 2 && c < 2 + 2 * TW && yy < sh && xx < sw)
+            copy[(size_t)yy * cp + xx] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TH * IW; i += 256) {
+        const int j = i / IW, c = i - j * IW, r = 2 * j;
+        tv[j][c] = tin[r][c] + tin[r + 4][c] + 4.0f * (tin[r + 1][c] + tin[r + 3][c]) + 6.0f * tin[r + 2][c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TH * TW; i += 256) {
+        const int j = i / TW, k = i - j * TW;
+        const int ox = ox0 + k, oy = oy0 + j;
+        if (ox < dw && oy < dh) {
+            const int c = 2 * k;
+            const float h = tv[j][c] + tv[j][c + 4] + 4.0f * (tv[j][c + 1] + tv[j][c + 3]) + 6.0f * tv[j][c + 2];
+            dst[(size_t)oy * dp + ox] = Q6<TD>::putf(h * (1.0f / 256.0f));
+        }
+    }
+}
+
+// Level 0 of an F32 pyramid from the u8 frame: 64 x byte (img_u8_to_q6, src/kern/img_ref.c:32-40).
+__global__ __launch_bounds__(256) void u8_to_f32(const uint8_t *__restrict__ src, int sp, float *__restrict__ dst,
+                                                 int dp, int w, int h)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x < w && y < h)
+        dst[(size_t)y * dp + x] = (float)src[(size_t)y * sp + x] * 64.0f;
+}
+
+// A level as the reference's padded fp32 plane: every lane of rows -4 .. h+3 and columns -32 .. stride-33, the
+// border replicated (img_pad_rows + img_pad_tb).
+template <typename T>
+__global__ __launch_bounds__(256) void to_plane(const T *__restrict__ src, int sp, int w, int h, float *__restrict__ dst,
+                                                int stride)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y; // r = padded row 0 .. h + 7
+    if (x >= stride)
+        return;
+    const int sx = clampi(x - VT_PAD_X, 0, w - 1), sy = clampi(r - VT_PAD_Y, 0, h - 1);
+    dst[(size_t)r * stride + x] = Q6<T>::get(src[(size_t)sy * sp + sx]);
+}
+
+// The visible pixels of a padded fp32 host plane (already uploaded, `stride` floats per row) into a device F32 level.
+__global__ __launch_bounds__(256) void from_plane(const float *__restrict__ src, int stride, int w, int h,
+                                                  float *__restrict__ dst, int dp)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x < w && y < h)
+        dst[(size_t)y * dp + x] = src[(size_t)(y + VT_PAD_Y) * stride + VT_PAD_X + x];
+}
+
+size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+template <typename TS, typename TD, bool COPY, bool EXACT>
+int launch_reduce(vtg_ctx *c, const TS *src, int sw, int sh, int sp, TD *dst, int dw, int dh, int dp, TS *copy,
+                  int cp, const char *name)
+{
+    const dim3 grid((dw + TW - 1) / TW, (dh + TH - 1) / TH);
+    hipEvent_t ev;
+    timer_begin(c, name, &ev);
+    if constexpr (EXACT)
+        hipLaunchKernelGGL((reduce_exact<TS, TD, COPY>), grid, dim3(256), 0, c->stream, src, sw, sh, sp, dst, dw, dh,
+                           dp, copy, cp);
+    else
+        hipLaunchKernelGGL((reduce_float<TS, TD, COPY>), grid, dim3(256), 0, c->stream, src, sw, sh, sp, dst, dw, dh,
+                           dp, copy, cp);
+    VTG_HIP(c, hipGetLastError());
+    timer_end(c, name, ev);
+    return VTG_OK;
+}
+
+}  // namespace
+
+PyrView view(const vtg_pyr *p)
+{
+    PyrView v{};
+    v.levels = p->levels;
+    v.storage = p->storage;
+    for (int l = 0; l < p->levels; l++)
+        v.l[l] = p->lv[l];
+    return v;
+}
+
+// frame_dev: the u8 frame on the device, `pitch` bytes per row.
+int launch_pyr_build(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame, int pitch)
+{
+    const Level *L = p->lv;
+    if (p->storage == VTG_STORE_F32) {
+        float *l0 = (float *)L[0].p;
+        hipEvent_t ev;
+        timer_begin(c, "pyr_u8_to_f32", &ev);
+        hipLaunchKernelGGL(u8_to_f32, dim3((p->w + 255) / 256, p->h), dim3(256), 0, c->stream, frame, pitch, l0,
+                           L[0].pitch, p->w, p->h);
+        VTG_HIP(c, hipGetLastError());
+        timer_end(c, "pyr_u8_to_f32", ev);
+        for (int l = 1; l < p->levels; l++) {
+            int rc = launch_reduce<float, float, false, true>(c, (const float *)L[l - 1].p, L[l - 1].w, L[l - 1].h,
+                                                              L[l - 1].pitch, (float *)L[l].p, L[l].w, L[l].h,
+                                                              L[l].pitch, nullptr, 0, "pyr_reduce");
+            if (rc)
+                return rc;
+        }
+        return VTG_OK;
+    }
+    if (p->levels == 1) {
+        VTG_HIP(c, hipMemcpy2DAsync((void *)L[0].p, L[0].pitch, frame, pitch, p->w, p->h, hipMemcpyDeviceToDevice,
+                                    c->stream));
+        return VTG_OK;
+    }
+    uint8_t *l0 = (uint8_t *)L[0].p;
+    int rc = VTG_OK;
+    switch (p->storage) {
+    case VTG_STORE_Q6:
+        rc = launch_reduce<uint8_t, uint16_t, true, true>(c, frame, p->w, p->h, pitch, (uint16_t *)L[1].p, L[1].w,
+                                                          L[1].h, L[1].pitch, l0, L[0].pitch, "pyr_reduce");
+        for (int l = 2; l < p->levels && !rc; l++)
+            rc = launch_reduce<uint16_t, uint16_t, false, true>(c, (const uint16_t *)L[l - 1].p, L[l - 1].w,
+                                                                L[l - 1].h, L[l - 1].pitch, (uint16_t *)L[l].p,
+                                                                L[l].w, L[l].h, L[l].pitch, nullptr, 0, "pyr_reduce");
+        break;
+    case VTG_STORE_F16:
+        rc = launch_reduce<uint8_t, __half, true, false>(c, frame, p->w, p->h, pitch, (__half *)L[1].p, L[1].w, L[1].h,
+                                                         L[1].pitch, l0, L[0].pitch, "pyr_reduce");
+        for (int l = 2; l < p->levels && !rc; l++)
+            rc = launch_reduce<__half, __half, false, false>(c, (const __half *)L[l - 1].p, L[l - 1].w, L[l - 1].h,
+                                                             L[l - 1].pitch, (__half *)L[l].p, L[l].w, L[l].h,
+                                                             L[l].pitch, nullptr, 0, "pyr_reduce");
+        break;
+    case VTG_STORE_BF16:
+        rc = launch_reduce<uint8_t, __hip_bfloat16, true, false>(c, frame, p->w, p->h, pitch,
+                                                                 (__hip_bfloat16 *)L[1].p, L[1].w, L[1].h, L[1].pitch,
+                                                                 l0, L[0].pitch, "pyr_reduce");
+        for (int l = 2; l < p->levels && !rc; l++)
+            rc = launch_reduce<__hip_bfloat16, __hip_bfloat16, false, false>(
+                c, (const __hip_bfloat16 *)L[l - 1].p, L[l - 1].w, L[l - 1].h, L[l - 1].pitch,
+                (__hip_bfloat16 *)L[l].p, L[l].w, L[l].h, L[l].pitch, nullptr, 0, "pyr_reduce");
+        break;
+    default:
+        return fail(c, VTG_EINVAL, "pyramid storage %d", p->storage);
+    }
+    return rc;
+}
+
+int pyr_from_planes(vtg_ctx *c, const vt_plane *levels, int n, vtg_pyr **out)
+{
+    *out = nullptr;
+    for (int l = 1; l < n; l++)
+        if (levels[l].w != (levels[l - 1].w + 1) / 2 || levels[l].h != (levels[l - 1].h + 1) / 2)
+            return fail(c, VTG_EINVAL, "level %d is %dx%d, not half of %dx%d", l, levels[l].w, levels[l].h,
+                        levels[l - 1].w, levels[l - 1].h);
+    vtg_pyr *p = nullptr;
+    int rc = vtg_pyr_create(c, levels[0].w, levels[0].h, n, VTG_STORE_F32, &p);
+    if (rc)
+        return rc;
+    for (int l = 0; l < n && !rc; l++) {
+        const vt_plane &pl = levels[l];
+        if (pl.stride != plane_stride(pl.w)) {
+            rc = fail(c, VTG_EINVAL, "level %d stride %d, expected %d", l, pl.stride, plane_stride(pl.w));
+            break;
+        }
+        void *tmp = nullptr;
+        const size_t bytes = plane_bytes(pl.w, pl.h);
+        rc = scratch(c, bytes, &tmp);
+        if (rc)
+            break;
+        const float *base = pl.p - (size_t)pl.stride * VT_PAD_Y - VT_PAD_X;
+        hipError_t e = hipMemcpyAsync(tmp, base, bytes, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(from_plane, dim3((pl.w + 255) / 256, pl.h), dim3(256), 0, c->stream,
+                               (const float *)tmp, pl.stride, pl.w, pl.h, (float *)p->lv[l].p, p->lv[l].pitch);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream); // the scratch is reused by the next level
+        if (e != hipSuccess)
+            rc = hip_fail(c, e, "pyr_from_planes");
+    }
+    if (rc) {
+        vtg_pyr_destroy(c, p);
+        return rc;
+    }
+    *out = p;
+    return VTG_OK;
+}
+
+static int elem_bytes(int storage, int level)
+{
+    switch (storage) {
+    case VTG_STORE_Q6: return level == 0 ? 1 : 2;
+    case VTG_STORE_F32: return 4;
+    case VTG_STORE_F16:
+    case VTG_STORE_BF16: return level == 0 ? 1 : 2;
+    }
+    return 0;
+}
+
+}  // namespace vtg
+
+using namespace vtg;
+
+extern "C" {
+
+int vtg_pyr_create(vtg_ctx *c, int w, int h, int levels, int storage, vtg_pyr **out)
+{
+    if (!c || !out)
+        return VTG_EINVAL;
+    *out = nullptr;
+    if (w < 8 || h < 8 || w > 16384 || h > 16384)
+        return fail(c, VTG_ERANGE, "vtg_pyr_create: frame %dx%d outside 8..16384", w, h);
+    if (levels < 1 || levels > VTG_MAX_LEVELS)
+        return fail(c, VTG_EINVAL, "vtg_pyr_create: %d levels (1..%d)", levels, VTG_MAX_LEVELS);
+    if (storage < VTG_STORE_Q6 || storage > VTG_STORE_BF16)
+        return fail(c, VTG_EINVAL, "vtg_pyr_create: storage %d", storage);
+    vtg_pyr *p = new vtg_pyr();
+    p->w = w;
+    p->h = h;
+    p->levels = levels;
+    p->storage = storage;
+    size_t off = 0;
+    int lw = w, lh = h;
+    for (int l = 0; l < levels; l++) {
+        const int eb = elem_bytes(storage, l);
+        const size_t pitch_b = round_up((size_t)lw * eb, 256);
+        p->lv[l].w = lw;
+        p->lv[l].h = lh;
+        p->lv[l].pitch = (int)(pitch_b / eb);
+        p->offset[l] = off;
+        p->elem[l] = eb;
+        off += round_up(pitch_b * lh, 4096);
+        lw = (lw + 1) / 2;
+        lh = (lh + 1) / 2;
+    }
+    p->bytes = off;
+    hipError_t e = hipMalloc(&p->block, off);
+    if (e != hipSuccess) {
+        delete p;
+        return hip_fail(c, e, "vtg_pyr_create: hipMalloc");
+    }
+    for (int l = 0; l < levels; l++)
+        p->lv[l].p = (char *)p->block + p->offset[l];
+    *out = p;
+    return VTG_OK;
+}
+
+int vtg_pyr_destroy(vtg_ctx *c, vtg_pyr *p)
+{
+    if (!c || !p)
+        return VTG_EINVAL;
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(p->block);
+    delete p;
+    return VTG_OK;
+}
+
+int vtg_pyr_build(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame, int pitch, int on_device)
+{
+    if (!c || !p || !frame || pitch < p->w)
+        return fail(c, VTG_EINVAL, "vtg_pyr_build: bad argument");
+    if (!on_device) {
+        // the host frame goes straight into level 0's bytes when that level is u8, else into scratch
+        void *dst = nullptr;
+        size_t dp = (size_t)p->w;
+        if (p->storage == VTG_STORE_F32) {
+            int rc = scratch(c, (size_t)p->w * p->h, &dst);
+            if (rc)
+                return rc;
+        } else {
+            dst = (void *)p->lv[0].p;
+            dp = (size_t)p->lv[0].pitch;
+        }
+        VTG_HIP(c, hipMemcpy2DAsync(dst, dp, frame, pitch, p->w, p->h, hipMemcpyHostToDevice, c->stream));
+        frame = (const uint8_t *)dst;
+        pitch = (int)dp;
+    }
+    return launch_pyr_build(c, p, frame, pitch);
+}
+
+int vtg_pyr_level(const vtg_pyr *p, int level, void **dptr, int *w, int *h, size_t *pitch_bytes, int *elem)
+{
+    if (!p || level < 0 || level >= p->levels)
+        return VTG_EINVAL;
+    if (dptr)
+        *dptr = (void *)p->lv[level].p;
+    if (w)
+        *w = p->lv[level].w;
+    if (h)
+        *h = p->lv[level].h;
+    if (pitch_bytes)
+        *pitch_bytes = (size_t)p->lv[level].pitch * p->elem[level];
+    if (elem)
+        *elem = p->elem[level];
+    return VTG_OK;
+}
+
+int vtg_pyr_read_level(vtg_ctx *c, const vtg_pyr *p, int level, const struct vt_plane *dst)
+{
+    if (!c || !p || !dst || level < 0 || level >= p->levels)
+        return VTG_EINVAL;
+    const Level &L = p->lv[level];
+    if (dst->w != L.w || dst->h != L.h || dst->stride != plane_stride(L.w))
+        return fail(c, VTG_EINVAL, "vtg_pyr_read_level: plane %dx%d stride %d, level %dx%d", dst->w, dst->h,
+                    dst->stride, L.w, L.h);
+    const size_t bytes = plane_bytes(L.w, L.h);
+    void *tmp = nullptr;
+    int rc = scratch(c, bytes, &tmp);
+    if (rc)
+        return rc;
+    const dim3 grid((dst->stride + 255) / 256, L.h + 2 * VT_PAD_Y);
+    const int es = p->elem[level];
+    if (p->storage == VTG_STORE_F32)
+        hipLaunchKernelGGL(to_plane<float>, grid, dim3(256), 0, c->stream, (const float *)L.p, L.pitch, L.w, L.h,
+                           (float *)tmp, dst->stride);
+    else if (es == 1)
+        hipLaunchKernelGGL(to_plane<uint8_t>, grid, dim3(256), 0, c->stream, (const uint8_t *)L.p, L.pitch, L.w, L.h,
+                           (float *)tmp, dst->stride);
+    else if (p->storage == VTG_STORE_Q6)
+        hipLaunchKernelGGL(to_plane<uint16_t>, grid, dim3(256), 0, c->stream, (const uint16_t *)L.p, L.pitch, L.w,
+                           L.h, (float *)tmp, dst->stride);
+    else if (p->storage == VTG_STORE_F16)
+        hipLaunchKernelGGL(to_plane<__half>, grid, dim3(256), 0, c->stream, (const __half *)L.p, L.pitch, L.w, L.h,
+                           (float *)tmp, dst->stride);
+    else
+        hipLaunchKernelGGL(to_plane<__hip_bfloat16>, grid, dim3(256), 0, c->stream, (const __hip_bfloat16 *)L.p,
+                           L.pitch, L.w, L.h, (float *)tmp, dst->stride);
+    VTG_HIP(c, hipGetLastError());
+    float *base = dst->p - (size_t)dst->stride * VT_PAD_Y - VT_PAD_X;
+    VTG_HIP(c, hipMemcpyAsync(base, tmp, bytes, hipMemcpyDeviceToHost, c->stream));
+    VTG_HIP(c, hipStreamSynchronize(c->stream));
+    return VTG_OK;
+}
+
+// ---- reference-shaped calls over host planes
+
+int vtg_img_u8_to_q6(vtg_ctx *c, const struct vt_plane *dst, const uint8_t *src, int src_stride, int y0, int y1)
+{
+    if (!c || !dst || !src || y0 < 0 || y1 > dst->h || y0 > y1 || src_stride < dst->w)
+        return fail(c, VTG_EINVAL, "vtg_img_u8_to_q6: bad argument");
+    if (y0 == y1)
+        return VTG_OK;
+    vtg_pyr *p = nullptr;
+    int rc = vtg_pyr_create(c, dst->w, y1 - y0, 1, VTG_STORE_F32, &p);
+    if (rc)
+        return rc;
+    rc = vtg_pyr_build(c, p, src + (size_t)y0 * src_stride, src_stride, 0);
+    if (!rc) {
+        // rows y0..y1 of dst as a plane of their own: the padding of each row is the same either way
+        struct vt_plane part = *dst;
+        part.p = plane_row(dst, y0);
+        part.h = y1 - y0;
+        const Level &L = p->lv[0];
+        void *tmp = nullptr;
+        const size_t bytes = (size_t)dst->stride * part.h * sizeof(float);
+        rc = scratch(c, plane_bytes(dst->w, part.h), &tmp);
+        if (!rc) {
+            const dim3 grid((dst->stride + 255) / 256, part.h + 2 * VT_PAD_Y);
+            hipLaunchKernelGGL(to_plane<float>, grid, dim3(256), 0, c->stream, (const float *)L.p, L.pitch, L.w, L.h,
+                               (float *)tmp, dst->stride);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(part.p - VT_PAD_X, (float *)tmp + (size_t)VT_PAD_Y * dst->stride, bytes,
+                                   hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess)
+                rc = hip_fail(c, e, "vtg_img_u8_to_q6");
+        }
+    }
+    vtg_pyr_destroy(c, p);
+    return rc;
+}
+
+int vtg_img_reduce(vtg_ctx *c, const struct vt_plane *dst, const struct vt_plane *src, int y0, int y1)
+{
+    if (!c || !dst || !src || dst->w != (src->w + 1) / 2 || dst->h != (src->h + 1) / 2 || y0 < 0 || y1 > dst->h ||
+        y0 > y1)
+        return fail(c, VTG_EINVAL, "vtg_img_reduce: bad argument");
+    if (y0 == y1)
+        return VTG_OK;
+    vtg_pyr *p = nullptr;
+    int rc = vtg_pyr_create(c, src->w, src->h, 2, VTG_STORE_F32, &p);
+    if (rc)
+        return rc;
+    void *tmp = nullptr;
+    const size_t sbytes = plane_bytes(src->w, src->h), dbytes = plane_bytes(dst->w, dst->h);
+    rc = scratch(c, sbytes > dbytes ? sbytes : dbytes, &tmp);
+    const Level *L = p->lv;
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        const float *base = src->p - (size_t)src->stride * VT_PAD_Y - VT_PAD_X;
+        e = hipMemcpyAsync(tmp, base, sbytes, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(from_plane, dim3((src->w + 255) / 256, src->h), dim3(256), 0, c->stream,
+                               (const float *)tmp, src->stride, src->w, src->h, (float *)L[0].p, L[0].pitch);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            rc = launch_reduce<float, float, false, true>(c, (const float *)L[0].p, L[0].w, L[0].h, L[0].pitch,
+                                                          (float *)L[1].p, L[1].w, L[1].h, L[1].pitch, nullptr, 0,
+                                                          "pyr_reduce");
+        if (e == hipSuccess && !rc) {
+            const dim3 grid((dst->stride + 255) / 256, dst->h + 2 * VT_PAD_Y);
+            hipLaunchKernelGGL(to_plane<float>, grid, dim3(256), 0, c->stream, (const float *)L[1].p, L[1].pitch,
+                               L[1].w, L[1].h, (float *)tmp, dst->stride);
+            e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(plane_row(dst, y0) - VT_PAD_X,
+                                   (float *)tmp + (size_t)(VT_PAD_Y + y0) * dst->stride,
+                                   (size_t)(y1 - y0) * dst->stride * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(c->stream);
+        }
+        if (e != hipSuccess)
+            rc = hip_fail(c, e, "vtg_img_reduce");
+    }
+    vtg_pyr_destroy(c, p);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+// vtg_internal.hpp: what the library's translation units share: the context, device pyramid and template records,
+// error plumbing and the launch timer. Host-side C++ plus the device-side level descriptors.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "vtgpu.h"
+
+namespace vtg {
+
+constexpr int kMaxLevels = VTG_MAX_LEVELS;
+
+// One level as the kernels see it: element pointer, size, pitch in elements.
+struct Level {
+    const void *p;
+    int w, h, pitch;
+};
+
+// A pyramid as the kernels see it, passed by value in kernel arguments.
+struct PyrView {
+    Level l[kMaxLevels];
+    int levels;
+    int storage;
+};
+
+struct Timer {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace vtg
+
+struct vtg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char err[512] = {0};
+    bool timing = false;
+    std::vector<vtg::Timer> pending;         // launches bracketed by events, not yet read
+    std::vector<hipEvent_t> free_events;     // recycled events
+    struct Stat { std::string name; double ms; long launches; };
+    std::vector<Stat> stats;
+    // scratch owned by the context (grown on demand, freed on destroy)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    void *pinned = nullptr;                  // small pinned host buffer for results
+    size_t pinned_bytes = 0;
+};
+
+struct vtg_pyr {
+    int w = 0, h = 0, levels = 0, storage = 0;
+    void *block = nullptr;
+    size_t bytes = 0;
+    vtg::Level lv[vtg::kMaxLevels];
+    size_t offset[vtg::kMaxLevels];
+    int elem[vtg::kMaxLevels];
+};
+
+struct vtg_tmpl {
+    int tw = 0, th = 0, level = 0;
+    float norm = 0.0f, mean = 0.0f;
+    int64_t sum_u = 0;          // sum of the template's pixels in the level's own units (u8 at level 0)
+    void *block = nullptr;      // device: t (fp32 zero-mean, tw*th), raw (tw*th fp32 Q6), shifted u8 words
+    float *t = nullptr;
+    float *raw = nullptr;
+    uint32_t *words = nullptr;  // 4 shifts x th rows x (tw/4 + 1) words, level 0 only
+    int words_per_row = 0;
+};
+
+namespace vtg {
+
+int fail(vtg_ctx *c, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+int hip_fail(vtg_ctx *c, hipError_t e, const char *what);
+
+#define VTG_HIP(ctx, call)                                  \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess)                               \
+            return ::vtg::hip_fail((ctx), e_, #call);       \
+    } while (0)
+
+// Launch bracketing for the per-kernel timer: begin() before a launch, end() after it (no-ops when timing is off).
+void timer_begin(vtg_ctx *c, const char *name, hipEvent_t *a);
+void timer_end(vtg_ctx *c, const char *name, hipEvent_t a);
+
+// Context scratch of at least `bytes` (device), stream-ordered reuse.
+int scratch(vtg_ctx *c, size_t bytes, void **p);
+int pinned(vtg_ctx *c, size_t bytes, void **p);
+
+PyrView view(const vtg_pyr *p);
+// An F32 pyramid holding copies of host planes levels[0..n-1] (their visible pixels).
+int pyr_from_planes(vtg_ctx *c, const vt_plane *levels, int n, vtg_pyr **out);
+
+// kernels (one per translation unit)
+int launch_pyr_build(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame_dev, int pitch);
+int launch_lk(vtg_ctx *c, const PyrView &I, const PyrView &J, int top, const lk_cfg *cfg, int n, const float *xy,
+              const float *guess, float fb_max, int solve_mode, lk_out *out);
+int launch_ncc_template(vtg_ctx *c, const vtg_pyr *p, int level, int x0, int y0, vtg_tmpl *t);
+size_t search_scratch_bytes(int ncell);
+int launch_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
+                      vtg_cell *cells, vtg_found *found, void *aux);
+int launch_ncc_score(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out);
+
+}  // namespace vtg

@@ -1,0 +1,380 @@
+"""vtgpu: Python host binding of the MI355X vision path's C-ABI (include/vtgpu.h, libvtgpu.so).
+
+The mirror of the reference's operator surface for this path: pyramids (tk_build / img_reduce, src/kern/track.c:584-629,
+src/kern/img.h:30-33), sparse pyramidal LK (lk_track / lk_flowback, src/kern/lk.h:45-53), NCC search (search_slot /
+tk_correlate, src/kern/track.c:1661-1726, track.h:57-59). All compute runs in libvtgpu.so on the GPU; this module only
+moves bytes and checks return codes. It raises if the library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_GPU = os.path.join(HERE, "libvtgpu.so")
+LIB_SYNTH = os.path.join(HERE, "libvtsynth.so")
+
+PAD_X, PAD_Y, Q6 = 32, 4, 64
+STORE_Q6, STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2, 3
+SOLVE_F64, SOLVE_MIXED = 0, 1
+LK_OK, LK_OUT, LK_LOWEIG, LK_RESIDUAL, LK_FLOWBACK = 0, 1, 2, 4, 8
+
+LK_OUT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("min_eig", "<f4"), ("residual", "<f4"),
+                         ("cov_xx", "<f4"), ("cov_xy", "<f4"), ("cov_yy", "<f4"),
+                         ("iters", "<u2"), ("status", "<u2")])
+CELL_DTYPE = np.dtype([("v", "<f4"), ("x", "<i4"), ("y", "<i4")])
+
+
+class Plane(C.Structure):
+    _fields_ = [("p", C.POINTER(C.c_float)), ("w", C.c_int), ("h", C.c_int), ("stride", C.c_int)]
+
+
+class LkCfg(C.Structure):
+    _fields_ = [("w", C.c_int), ("h", C.c_int), ("max_iter", C.c_int), ("eps", C.c_float),
+                ("min_eig", C.c_float), ("max_residual", C.c_float), ("prefetch", C.c_int)]
+
+
+class Found(C.Structure):
+    _fields_ = [("hx", C.c_int), ("hy", C.c_int), ("ncc", C.c_double), ("second", C.c_double),
+                ("ox", C.c_double), ("oy", C.c_double), ("cells", C.c_int), ("best_x", C.c_int),
+                ("best_y", C.c_int), ("best_v", C.c_float)]
+
+
+class VtgError(RuntimeError):
+    pass
+
+
+_lib = None
+_synth = None
+
+
+def lib():
+    """libvtgpu.so, loaded once. Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_GPU):
+        raise VtgError(f"{LIB_GPU} is missing: build it (python intel-phi-visiontrack_amd/build.py)")
+    L = C.CDLL(LIB_GPU)
+    P, I, F, V, S = C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_size_t
+    sig = {
+        "vtg_ctx_create": [I, C.POINTER(V)], "vtg_ctx_destroy": [V], "vtg_ctx_sync": [V],
+        "vtg_alloc": [V, S, C.POINTER(V)], "vtg_free": [V, V], "vtg_upload": [V, V, V, S],
+        "vtg_download": [V, V, V, S], "vtg_copy_d2d": [V, V, V, S],
+        "vtg_timing_enable": [V, I], "vtg_timing_read": [V, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)],
+        "vtg_timing_reset": [V],
+        "vtg_pyr_create": [V, I, I, I, I, C.POINTER(V)], "vtg_pyr_destroy": [V, V],
+        "vtg_pyr_build": [V, V, V, I, I],
+        "vtg_pyr_level": [V, I, C.POINTER(V), C.POINTER(I), C.POINTER(I), C.POINTER(S), C.POINTER(I)],
+        "vtg_pyr_read_level": [V, V, I, C.POINTER(Plane)],
+        "vtg_lk_track": [V, V, V, I, C.POINTER(LkCfg), I, V, V, F, V],
+        "vtg_lk_track_ex": [V, V, V, I, C.POINTER(LkCfg), I, V, V, F, I, V],
+        "vtg_ncc_template": [V, V, I, I, I, I, I, C.POINTER(V)], "vtg_ncc_template_destroy": [V, V],
+        "vtg_ncc_template_update": [V, V, V, I, I, I],
+        "vtg_ncc_template_read": [V, V, V, C.POINTER(F)],
+        "vtg_ncc_search": [V, V, V, I, I, I, I, I, V, C.POINTER(Found)],
+        "vtg_ncc_score": [V, V, V, I, V, I, V],
+        "vtg_img_u8_to_q6": [V, C.POINTER(Plane), V, I, I, I],
+        "vtg_img_reduce": [V, C.POINTER(Plane), C.POINTER(Plane), I, I],
+        "vtg_lk_track_planes": [V, V, V, I, C.POINTER(LkCfg), I, V, V, V, V, F, V],
+        "vtg_correlate_planes": [V, C.POINTER(Plane), V, I, I, C.c_double, I, V, V, V],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = I
+    L.vtg_ctx_error.argtypes = [V]
+    L.vtg_ctx_error.restype = C.c_char_p
+    L.vtg_ctx_stream.argtypes = [V]
+    L.vtg_ctx_stream.restype = V
+    _lib = L
+    return L
+
+
+# every symbol include/vtgpu.h declares
+EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_error", "vtg_ctx_stream", "vtg_alloc",
+           "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_timing_enable", "vtg_timing_read",
+           "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_level",
+           "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_ncc_template", "vtg_ncc_template_destroy",
+           "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_search", "vtg_ncc_score", "vtg_img_u8_to_q6", "vtg_img_reduce",
+           "vtg_lk_track_planes", "vtg_correlate_planes"]
+
+
+def synth():
+    """libvtsynth.so (the synthetic-frame generator)."""
+    global _synth
+    if _synth is not None:
+        return _synth
+    if not os.path.exists(LIB_SYNTH):
+        raise VtgError(f"{LIB_SYNTH} is missing: build it")
+    L = C.CDLL(LIB_SYNTH)
+    D = C.c_double
+    L.vts_frame_pair.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_void_p,
+                                 C.c_void_p, D, D, C.c_void_p, C.c_int, C.c_int]
+    L.vts_frame.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_void_p, C.c_void_p, D,
+                            C.c_uint64, C.c_int]
+    L.vts_draw_motion.argtypes = [C.c_uint64, D, D, C.c_void_p]
+    L.vts_truth.argtypes = [C.c_void_p, C.c_int, C.c_int, D, D, C.POINTER(D), C.POINTER(D)]
+    L.vts_grid.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.vts_points.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.vts_rand.argtypes = [C.POINTER(C.c_uint64)]
+    L.vts_rand.restype = C.c_uint64
+    L.vts_uniform.argtypes = [C.POINTER(C.c_uint64)]
+    L.vts_uniform.restype = D
+    _synth = L
+    return L
+
+
+class Scene(C.Structure):
+    _fields_ = [("nblobs", C.c_int), ("sig_lo", C.c_double), ("sig_hi", C.c_double), ("amp_lo", C.c_double),
+                ("amp_hi", C.c_double)]
+
+
+class Motion(C.Structure):
+    _fields_ = [("dx", C.c_double), ("dy", C.c_double), ("theta_deg", C.c_double)]
+
+
+BLOB_SCENE = Scene(200, 4.0, 40.0, 20.0, 120.0)
+
+
+def frame_pair(w, h, seed, max_shift=8.0, max_rot=2.0, noise=2.0, motion=None, flat_amp=0.0, flat=None,
+               threads=8):
+    """(f0, f1, motion): the benchmark's seeded blob frames, f1 = f0 moved rigidly plus noise."""
+    S = synth()
+    if motion is None:
+        motion = Motion()
+        S.vts_draw_motion(seed, max_shift, max_rot, C.byref(motion))
+    f0 = np.zeros((h, w), np.uint8)
+    f1 = np.zeros((h, w), np.uint8)
+    fl = None if flat is None else np.ascontiguousarray(flat, np.int32)
+    S.vts_frame_pair(f0.ctypes.data, f1.ctypes.data, w, h, w, seed, C.byref(BLOB_SCENE), C.byref(motion), noise,
+                     flat_amp, None if fl is None else fl.ctypes.data, 0 if fl is None else len(fl) // 4, threads)
+    return f0, f1, motion
+
+
+def truth(motion, w, h, xs, ys):
+    S = synth()
+    u, v = C.c_double(), C.c_double()
+    out = np.zeros((len(xs), 2))
+    for i, (x, y) in enumerate(zip(xs, ys)):
+        S.vts_truth(C.byref(motion), w, h, float(x), float(y), C.byref(u), C.byref(v))
+        out[i] = (u.value, v.value)
+    return out
+
+
+def grid(nx, ny, w, h):
+    xs = np.zeros(nx * ny, np.float32)
+    ys = np.zeros(nx * ny, np.float32)
+    synth().vts_grid(nx, ny, w, h, xs.ctypes.data, ys.ctypes.data)
+    return xs, ys
+
+
+def points(seed, n, w, h, margin):
+    xs = np.zeros(n, np.float32)
+    ys = np.zeros(n, np.float32)
+    synth().vts_points(seed, n, w, h, margin, xs.ctypes.data, ys.ctypes.data)
+    return xs, ys
+
+
+def lk_cfg(w=21, h=21, max_iter=10, eps=0.01, min_eig=1e-4, max_residual=40.0):
+    return LkCfg(w, h, max_iter, eps, min_eig, max_residual, 0)
+
+
+class Context:
+    """A vtg_ctx: one device, one stream."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.vtg_ctx_create(device, C.byref(h))
+        if rc:
+            raise VtgError(f"vtg_ctx_create({device}) = {rc}")
+        self.h = h
+
+    def check(self, rc, what=""):
+        if rc:
+            raise VtgError(f"{what} failed ({rc}): {self.L.vtg_ctx_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.L.vtg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self.check(self.L.vtg_ctx_sync(self.h), "sync")
+
+    # memory
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        self.check(self.L.vtg_alloc(self.h, max(int(nbytes), 1), C.byref(p)), "alloc")
+        return DevBuf(self, p, int(nbytes))
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = self.alloc(arr.nbytes)
+        self.check(self.L.vtg_upload(self.h, b.ptr, arr.ctypes.data, arr.nbytes), "upload")
+        b.dtype, b.shape = arr.dtype, arr.shape
+        return b
+
+    # timing
+    def timing(self, on=True):
+        self.check(self.L.vtg_timing_enable(self.h, 1 if on else 0))
+
+    def timing_read(self, kernel):
+        ms, n = C.c_double(), C.c_long()
+        self.check(self.L.vtg_timing_read(self.h, kernel.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def timing_reset(self):
+        self.check(self.L.vtg_timing_reset(self.h))
+
+    # pyramids
+    def pyramid(self, w, h, levels, storage=STORE_Q6):
+        return Pyramid(self, w, h, levels, storage)
+
+    # LK
+    def lk_track(self, I, J, top, cfg, xy, guess=None, fb_max=0.0, out=None, solve=SOLVE_F64):
+        """xy, guess, out: DevBuf. Returns out."""
+        n = xy.nbytes // 8
+        if out is None:
+            out = self.alloc(n * 32)
+        self.check(self.L.vtg_lk_track_ex(self.h, I.h, J.h, top, C.byref(cfg), n, xy.ptr,
+                                          guess.ptr if guess is not None else None, fb_max, solve, out.ptr),
+                   "lk_track")
+        return out
+
+    def lk_track_host(self, I, J, top, cfg, xs, ys, gx=None, gy=None, fb_max=0.0, solve=SOLVE_F64):
+        xy = np.stack([np.asarray(xs, np.float32), np.asarray(ys, np.float32)], 1).ravel()
+        dxy = self.to_device(xy)
+        dg = None
+        if gx is not None:
+            dg = self.to_device(np.stack([np.asarray(gx, np.float32), np.asarray(gy, np.float32)], 1).ravel())
+        out = self.lk_track(I, J, top, cfg, dxy, dg, fb_max, solve=solve)
+        res = out.get(np.zeros(len(xs), LK_OUT_DTYPE))
+        for b in (dxy, dg, out):
+            if b is not None:
+                b.free()
+        return res
+
+    # NCC
+    def ncc_template(self, pyr, level, x0, y0, tw, th):
+        return Template(self, pyr, level, x0, y0, tw, th)
+
+    def ncc_search(self, pyr, tmpl, x0, x1, y0, y1, step=1, cells=None):
+        f = Found()
+        self.check(self.L.vtg_ncc_search(self.h, pyr.h, tmpl.h, x0, x1, y0, y1, step,
+                                         cells.ptr if cells is not None else None, C.byref(f)), "ncc_search")
+        return f
+
+    def ncc_score(self, pyr, tmpl, xy, half=False):
+        xy = np.ascontiguousarray(xy, np.int32).ravel()
+        n = len(xy) // 2
+        d = self.to_device(xy)
+        o = self.alloc(8 * n)
+        self.check(self.L.vtg_ncc_score(self.h, pyr.h, tmpl.h, n, d.ptr, 1 if half else 0, o.ptr), "ncc_score")
+        r = o.get(np.zeros(n, np.float64))
+        d.free()
+        o.free()
+        return r
+
+
+class DevBuf:
+    def __init__(self, ctx, ptr, nbytes):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def get(self, arr):
+        self.ctx.check(self.ctx.L.vtg_download(self.ctx.h, arr.ctypes.data, self.ptr, arr.nbytes), "download")
+        return arr
+
+    def put(self, arr):
+        arr = np.ascontiguousarray(arr)
+        self.ctx.check(self.ctx.L.vtg_upload(self.ctx.h, self.ptr, arr.ctypes.data, arr.nbytes), "upload")
+
+    def free(self):
+        if self.ptr:
+            self.ctx.L.vtg_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class Pyramid:
+    def __init__(self, ctx, w, h, levels, storage):
+        self.ctx, self.w, self.h, self.levels, self.storage = ctx, w, h, levels, storage
+        p = C.c_void_p()
+        ctx.check(ctx.L.vtg_pyr_create(ctx.h, w, h, levels, storage, C.byref(p)), "pyr_create")
+        self.h = p
+
+    def build(self, frame, on_device=False, pitch=None):
+        """frame: host uint8 (h, w) array, or a DevBuf with on_device=True."""
+        if on_device:
+            self.ctx.check(self.ctx.L.vtg_pyr_build(self.ctx.h, self.h, frame.ptr, pitch or self.w, 1), "pyr_build")
+        else:
+            frame = np.ascontiguousarray(frame, np.uint8)
+            self.ctx.check(self.ctx.L.vtg_pyr_build(self.ctx.h, self.h, frame.ctypes.data, frame.strides[0], 0),
+                           "pyr_build")
+
+    def level_info(self, level):
+        p, w, h, pb, eb = C.c_void_p(), C.c_int(), C.c_int(), C.c_size_t(), C.c_int()
+        self.ctx.check(self.ctx.L.vtg_pyr_level(self.h, level, C.byref(p), C.byref(w), C.byref(h), C.byref(pb),
+                                                C.byref(eb)), "pyr_level")
+        return p.value, w.value, h.value, pb.value, eb.value
+
+    def read_level(self, level):
+        """Level as a padded fp32 Q6 host plane (numpy buffer + struct), the reference's layout."""
+        _, w, h, _, _ = self.level_info(level)
+        hp = HostPlane(w, h)
+        self.ctx.check(self.ctx.L.vtg_pyr_read_level(self.ctx.h, self.h, level, C.byref(hp.c)), "pyr_read_level")
+        return hp
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.vtg_pyr_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+class Template:
+    def __init__(self, ctx, pyr, level, x0, y0, tw, th):
+        self.ctx, self.tw, self.th, self.level = ctx, tw, th, level
+        p = C.c_void_p()
+        ctx.check(ctx.L.vtg_ncc_template(ctx.h, pyr.h, level, x0, y0, tw, th, C.byref(p)), "ncc_template")
+        self.h = p
+
+    def update(self, pyr, level, x0, y0):
+        self.ctx.check(self.ctx.L.vtg_ncc_template_update(self.ctx.h, self.h, pyr.h, level, x0, y0), "template_update")
+        self.level = level
+
+    def read(self):
+        app = np.zeros(self.tw * self.th, np.float32)
+        norm = C.c_float()
+        self.ctx.check(self.ctx.L.vtg_ncc_template_read(self.ctx.h, self.h, app.ctypes.data, C.byref(norm)))
+        return app, norm.value
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.vtg_ncc_template_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+def plane_stride(w):
+    return ((w + 15) & ~15) + 2 * PAD_X
+
+
+class HostPlane:
+    """A padded fp32 Q6 plane in host memory, laid out as struct vt_plane."""
+
+    def __init__(self, w, h):
+        self.w, self.h, self.stride = w, h, plane_stride(w)
+        self.buf = np.zeros((h + 2 * PAD_Y) * self.stride + 16, dtype=np.float32)
+        off = PAD_Y * self.stride + PAD_X
+        self.c = Plane(self.buf[off:].ctypes.data_as(C.POINTER(C.c_float)), w, h, self.stride)
+
+    def padded(self):
+        return self.buf[: (self.h + 2 * PAD_Y) * self.stride].reshape(self.h + 2 * PAD_Y, self.stride)
+
+    def view(self):
+        return self.padded()[PAD_Y:PAD_Y + self.h, PAD_X:PAD_X + self.w]

@@ -12,13 +12,20 @@ REF_SO = os.path.join(REPO, "oracle", "_ref", "libvtref.so")
 PAD_X, PAD_Y, Q6 = 32, 4, 64
 
 
-class Plane(C.Structure):
-    _fields_ = [("p", C.POINTER(C.c_float)), ("w", C.c_int), ("h", C.c_int), ("stride", C.c_int)]
+try:
+    # one ctypes class per C struct, shared with the GPU binding, so planes and configs pass to either library
+    import sys as _sys
+    _sys.path.insert(0, os.path.join(REPO, "intel-phi-visiontrack_amd"))
+    from vtgpu import HostPlane as _VtgHostPlane, LkCfg, Plane  # noqa: E402
+except ImportError:  # pragma: no cover
+    _VtgHostPlane = None
 
+    class Plane(C.Structure):
+        _fields_ = [("p", C.POINTER(C.c_float)), ("w", C.c_int), ("h", C.c_int), ("stride", C.c_int)]
 
-class LkCfg(C.Structure):
-    _fields_ = [("w", C.c_int), ("h", C.c_int), ("max_iter", C.c_int), ("eps", C.c_float),
-                ("min_eig", C.c_float), ("max_residual", C.c_float), ("prefetch", C.c_int)]
+    class LkCfg(C.Structure):
+        _fields_ = [("w", C.c_int), ("h", C.c_int), ("max_iter", C.c_int), ("eps", C.c_float),
+                    ("min_eig", C.c_float), ("max_residual", C.c_float), ("prefetch", C.c_int)]
 
 
 LK_OUT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("min_eig", "<f4"), ("residual", "<f4"),
@@ -38,20 +45,7 @@ def plane_stride(w):
     return ((w + 15) & ~15) + 2 * PAD_X
 
 
-class HostPlane:
-    """A padded fp32 Q6 plane in a numpy buffer, laid out as the reference's struct vt_plane."""
-
-    def __init__(self, w, h):
-        self.w, self.h, self.stride = w, h, plane_stride(w)
-        self.buf = np.zeros((h + 2 * PAD_Y) * self.stride + 16, dtype=np.float32)
-        off = PAD_Y * self.stride + PAD_X
-        self.c = Plane(self.buf[off:].ctypes.data_as(C.POINTER(C.c_float)), w, h, self.stride)
-        self._off = off
-
-    def view(self):
-        """The visible h x w pixels (a view)."""
-        full = self.buf[: (self.h + 2 * PAD_Y) * self.stride].reshape(self.h + 2 * PAD_Y, self.stride)
-        return full[PAD_Y:PAD_Y + self.h, PAD_X:PAD_X + self.w]
+HostPlane = _VtgHostPlane
 
 
 def _ptr(a, t=C.c_void_p):

@@ -1,0 +1,116 @@
+"""GPU NCC search against the oracle: the template bit for bit, scores within 1e-5, the same per-cell argmax where
+the oracle's winner is clear, and the same found corner and score; plus the full 1080p 64 x 64 exhaustive search."""
+import numpy as np
+import pytest
+
+import vtgpu
+from gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(ctx, oracle, w, h, seed, levels=1):
+    f0, f1, mo = vtgpu.frame_pair(w, h, seed)
+    p0, p1 = ctx.pyramid(w, h, levels), ctx.pyramid(w, h, levels)
+    p0.build(f0)
+    p1.build(f1)
+    k0, a0 = oracle.pyramid(f0, w, h, levels, threads=8)
+    k1, a1 = oracle.pyramid(f1, w, h, levels, threads=8)
+    return f0, f1, mo, p0, p1, k0, k1
+
+
+@pytest.mark.parametrize("level,tw,th", [(0, 64, 64), (0, 32, 21), (1, 32, 21), (0, 16, 24)])
+def test_template_exact(ctx, oracle, level, tw, th):
+    f0, f1, mo, p0, p1, k0, k1 = _setup(ctx, oracle, 640, 480, 21, levels=2)
+    say("ncc template", level, tw, th)
+    t = ctx.ncc_template(p0, level, 101, 77, tw, th)
+    app, norm = t.read()
+    oapp, onorm = oracle.ncc_template(k0[level], 101, 77, tw, th)
+    assert np.array_equal(app, oapp) and norm == np.float32(onorm)
+    t.destroy()
+
+
+@pytest.mark.parametrize("level,tw,th", [(0, 64, 64), (0, 32, 21), (1, 32, 21)])
+def test_scores_whole_and_half(ctx, oracle, level, tw, th):
+    """tk_correlate / search_half at random corners, the border (-1) and blank-ish windows: within 1e-5."""
+    f0, f1, mo, p0, p1, k0, k1 = _setup(ctx, oracle, 640, 480, 22, levels=2)
+    t = ctx.ncc_template(p0, level, 150, 100, tw, th)
+    app, norm = t.read()
+    rng = np.random.default_rng(level * 7 + tw)
+    lw, lh = k1[level].w, k1[level].h
+    xy = np.stack([rng.integers(-2, lw - tw + 2, 300), rng.integers(-2, lh - th + 2, 300)], 1).astype(np.int32)
+    xy[0] = (150 + int(round(mo.dx)) >> level, 100 + int(round(mo.dy)) >> level)
+    say("ncc scores", level, tw, th)
+    got = ctx.ncc_score(p1, t, xy)
+    ref = np.array([oracle.ncc_at(k1[level], app, tw, th, norm, int(x), int(y)) for x, y in xy])
+    assert np.all((ref == -1) == (got == -1)) and np.all((ref == 0) == (got == 0))
+    assert np.abs(got - ref).max() <= 1e-5, np.abs(got - ref).max()
+    hxy = np.stack([rng.integers(-2, 2 * (lw - tw), 300), rng.integers(-2, 2 * (lh - th), 300)], 1).astype(np.int32)
+    got = ctx.ncc_score(p1, t, hxy, half=True)
+    ref = np.array([oracle.lib.vo_ncc_half(__import__("ctypes").byref(k1[level].c), app.ctypes.data, tw, th, norm,
+                                           int(x), int(y)) for x, y in hxy])
+    assert np.abs(got - ref).max() <= 1e-5
+    t.destroy()
+
+
+def _check_cells(gcells, ocells, score_at):
+    """Scores within 1e-5; the same corner unless the oracle's best is within 1e-5 of the GPU's pick."""
+    assert len(gcells) == len(ocells)
+    dv = np.abs(gcells["v"].astype(np.float64) - ocells["v"])
+    assert dv.max() <= 1e-5, dv.max()
+    same = (gcells["x"] == ocells["x"]) & (gcells["y"] == ocells["y"])
+    for i in np.nonzero(~same)[0]:
+        assert abs(score_at(gcells["x"][i], gcells["y"][i]) - ocells["v"][i]) <= 1e-5
+    return same.mean()
+
+
+@pytest.mark.parametrize("w,h,tw,th,step", [(400, 300, 64, 64, 1), (400, 300, 32, 24, 1), (400, 300, 16, 16, 2),
+                                            (400, 300, 32, 21, 1), (400, 300, 64, 64, 2)])
+def test_search_matches_oracle(ctx, oracle, w, h, tw, th, step):
+    f0, f1, mo, p0, p1, k0, k1 = _setup(ctx, oracle, w, h, 31 + tw + step)
+    x0, y0 = 170, 120
+    t = ctx.ncc_template(p0, 0, x0, y0, tw, th)
+    app, norm = t.read()
+    ncell = ((w - tw - 2 - 1) // 16 + 1) * ((h - th - 2 - 1) // 16 + 1)
+    cells = ctx.alloc(12 * ncell)
+    say("ncc search", w, h, tw, th, step)
+    f = ctx.ncc_search(p1, t, 1, w, 1, h, step, cells)
+    g = cells.get(np.zeros(ncell, vtgpu.CELL_DTYPE))
+    of, oc = oracle.ncc_search(k1[0], app, tw, th, norm, 1, w, 1, h, step)
+    frac = _check_cells(g, oc, lambda x, y: oracle.ncc_at(k1[0], app, tw, th, norm, int(x), int(y)))
+    print(f"[ncc] {w}x{h} {tw}x{th} step {step}: cells {ncell}, same corner {frac:.4f}, found ({f.hx},{f.hy}) "
+          f"{f.ncc:.6f} vs ({of.hx},{of.hy}) {float(of.ncc):.6f}")
+    assert (f.hx, f.hy) == (of.hx, of.hy)
+    assert abs(f.ncc - float(of.ncc)) <= 1e-5 and abs(f.second - float(of.second)) <= 1e-5
+    hp = k1[0]
+    import ctypes as C
+    l = oracle.lib.vo_ncc_half(C.byref(hp.c), app.ctypes.data, tw, th, norm, of.hx - 1, of.hy)
+    r = oracle.lib.vo_ncc_half(C.byref(hp.c), app.ctypes.data, tw, th, norm, of.hx + 1, of.hy)
+    assert abs(f.ox - float(oracle.lib.vo_peak_offset(l, of.ncc, r))) <= 1e-3
+    cells.free()
+    t.destroy()
+
+
+def test_search_1080_full(ctx, oracle):
+    """The native workload: a 64 x 64 template cut from frame 0 at a seeded corner, exhaustive over frame 1."""
+    w, h = 1920, 1080
+    f0, f1, mo, p0, p1, k0, k1 = _setup(ctx, oracle, w, h, 2)
+    rng = np.random.default_rng(2)
+    x0, y0 = int(rng.integers(200, w - 264)), int(rng.integers(200, h - 264))
+    t = ctx.ncc_template(p0, 0, x0, y0, 64, 64)
+    app, norm = t.read()
+    ncell = ((w - 66 - 1) // 16 + 1) * ((h - 66 - 1) // 16 + 1)
+    cells = ctx.alloc(12 * ncell)
+    say("ncc search 1080 full")
+    f = ctx.ncc_search(p1, t, 1, w, 1, h, 1, cells)
+    g = cells.get(np.zeros(ncell, vtgpu.CELL_DTYPE))
+    of, oc = oracle.ncc_search(k1[0], app, 64, 64, norm, 1, w, 1, h, 1, threads=16)
+    frac = _check_cells(g, oc, lambda x, y: oracle.ncc_at(k1[0], app, 64, 64, norm, int(x), int(y)))
+    tr = vtgpu.truth(mo, w, h, [x0 + 32], [y0 + 32])[0]
+    print(f"[ncc] 1080p: same corner {frac:.4f}; found ({f.hx / 2},{f.hy / 2}) {f.ncc:.6f}; the template moved to "
+          f"({x0 + tr[0]:.2f},{y0 + tr[1]:.2f})")
+    assert (f.best_x, f.best_y) == (int(oc[np.argmax(oc['v'])]["x"]), int(oc[np.argmax(oc['v'])]["y"]))
+    assert (f.hx, f.hy) == (of.hx, of.hy) and abs(f.ncc - float(of.ncc)) <= 1e-5
+    assert abs(f.hx / 2 - (x0 + tr[0])) <= 2 and abs(f.hy / 2 - (y0 + tr[1])) <= 2
+    cells.free()
+    t.destroy()

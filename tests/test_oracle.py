@@ -1,0 +1,260 @@
+"""The CPU oracle pinned: against the reference's own committed hash lines, the reference built from its sources
+(oracle/_ref, where present), closed forms and brute force. No GPU."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle_ffi import CELL_DTYPE, HostPlane, LK_OUT_DTYPE, REF_SO, lk_cfg
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _l0(o, img, w, h):
+    a = HostPlane(w, h)
+    o.lib.vo_u8_to_q6(C.byref(a.c), img.ctypes.data, img.strides[0], 0, h)
+    o.lib.vo_pad_rows(C.byref(a.c), 0, h)
+    o.lib.vo_pad_tb(C.byref(a.c))
+    return a
+
+
+@pytest.mark.parametrize("size", ["640x480", "1280x720", "1920x1080"])
+def test_m1_hash_lines(oracle, size):
+    """Every image primitive against logs/m1/host/m1-host-<size>.txt (pure-integer m1 input)."""
+    gold = json.load(open(os.path.join(GOLD, "m1_hashes.json")))[size]
+    o = oracle
+    w, h = map(int, size.split("x"))
+    img = o.m1_image(w, h)
+    a = _l0(o, img, w, h)
+    assert "%016x" % o.hash_plane(a) == gold["u8_to_q6"]
+    c = HostPlane(w, h)
+    go = np.array([0.75, 512.0], np.float32)
+    o.lib.vo_norm(C.byref(c.c), C.byref(a.c), go.ctypes.data, 0, h)
+    assert "%016x" % o.hash_plane(c) == gold["norm"]
+    mm = np.zeros(2, np.float32)
+    o.lib.vo_minmax(C.byref(a.c), mm.ctypes.data, 0, h)
+    assert list(mm) == gold["minmax"]
+    o.lib.vo_gauss5(C.byref(c.c), C.byref(a.c), 0, h)
+    assert "%016x" % o.hash_plane(c) == gold["gauss5"]
+    r = HostPlane((w + 1) // 2, (h + 1) // 2)
+    o.lib.vo_reduce(C.byref(r.c), C.byref(a.c), 0, r.h)
+    assert "%016x" % o.hash_plane(r) == gold["reduce"]
+    d = HostPlane(w, h)
+    o.lib.vo_sobel(C.byref(c.c), C.byref(d.c), C.byref(a.c), 0, h)
+    assert "%016x" % o.hash_plane(c) == gold["sobel_x"]
+    assert "%016x" % o.hash_plane(d) == gold["sobel_y"]
+    b = HostPlane(w, h)
+    o.lib.vo_gauss5(C.byref(b.c), C.byref(a.c), 0, h)
+    o.lib.vo_absdiff(C.byref(c.c), C.byref(a.c), C.byref(b.c), 0, h)
+    assert "%016x" % o.hash_plane(c) == gold["absdiff"]
+
+
+def _reduce_numpy(q6):
+    """Textbook reduce: replicate-pad by 2, separable (1 4 6 4 1) in int64, even samples, (s + 128) >> 8."""
+    k = np.array([1, 4, 6, 4, 1], np.int64)
+    p = np.pad(q6.astype(np.int64), 2, mode="edge")
+    h, w = q6.shape
+    v = sum(k[j] * p[j:j + h, :] for j in range(5))
+    s = sum(k[i] * v[:, i:i + w] for i in range(5))
+    return ((s[::2, ::2] + 128) >> 8).astype(np.float32)
+
+
+@pytest.mark.parametrize("w,h", [(8, 8), (9, 7), (33, 17), (64, 48)])
+def test_reduce_brute_force(oracle, w, h):
+    """Odd sizes and borders: the oracle's reduce equals an independent int64 numpy restatement."""
+    rng = np.random.default_rng(w * 100 + h)
+    img = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    a = _l0(oracle, img, w, h)
+    r = HostPlane((w + 1) // 2, (h + 1) // 2)
+    oracle.lib.vo_reduce(C.byref(r.c), C.byref(a.c), 0, r.h)
+    assert np.array_equal(r.view(), _reduce_numpy(img.astype(np.int64) * 64))
+
+
+def test_reduce_constant_plane(oracle):
+    """A constant plane stays the same constant at every level (the kernel sums to 256)."""
+    img = np.full((50, 70), 173, np.uint8)
+    planes, _ = oracle.pyramid(img, 70, 50, 4)
+    for p in planes:
+        assert np.all(p.view() == 173 * 64)
+
+
+def _m2_setup(oracle):
+    g = json.load(open(os.path.join(GOLD, "m2_lines.json")))
+    w, h = g["w"], g["h"]
+    c = g["cfg"]
+    cfg = lk_cfg(c["w"], c["h"], c["max_iter"], c["eps"], c["min_eig"], c["max_residual"])
+    m = g["grid"]["start"]
+    pts = [(x, y) for y in range(m, h - m, g["grid"]["step"]) for x in range(m, w - m, g["grid"]["step"])]
+    xs = np.array([p[0] for p in pts], np.float32)
+    ys = np.array([p[1] for p in pts], np.float32)
+    return g, w, h, cfg, xs, ys
+
+
+def test_m2_lk_lines(oracle):
+    """The lane tier reproduces logs/m2/host/640x480.txt bit for bit: point 0's 7 floats, iters, status, and the
+    FNV of all 330 records, for all ten displacements (a 4-level pyramid, 21 x 21)."""
+    g, w, h, cfg, xs, ys = _m2_setup(oracle)
+    off = g["offset"]
+    keep0, I = oracle.pyramid(oracle.m2_render(w, h, off, off), w, h, g["levels"])
+    for case in g["cases"]:
+        keep1, J = oracle.pyramid(oracle.m2_render(w, h, off - case["dx"], off - case["dy"]), w, h, g["levels"])
+        out = oracle.lk_batch(0, I, J, g["levels"] - 1, cfg, xs, ys)
+        bits = ["%08x" % b for b in out.view(np.uint32).reshape(-1, 8)[0][:7]]
+        assert bits == case["bits"], case
+        assert int(out[0]["iters"]) == case["iters"] and int(out[0]["status"]) == case["status"]
+        assert "%016x" % oracle.lib.vo_hash_lk(out.ctypes.data, len(out)) == case["hash"], case
+
+
+def test_m2_f64_accuracy(oracle):
+    """The FP64 tier on the m2 inputs, whose displacement is exact: EPE within the reference's own measured band
+    (median <= 0.0115, p95 <= 0.0209 px for |d| <= 14.4 px, docs/measurements.md:93-103)."""
+    g, w, h, cfg, xs, ys = _m2_setup(oracle)
+    off = g["offset"]
+    keep0, I = oracle.pyramid(oracle.m2_render(w, h, off, off), w, h, g["levels"])
+    for case in g["cases"][:9]:
+        keep1, J = oracle.pyramid(oracle.m2_render(w, h, off - case["dx"], off - case["dy"]), w, h, g["levels"])
+        out = oracle.lk_batch(1, I, J, g["levels"] - 1, cfg, xs, ys)
+        ok = out["status"] == 0
+        epe = np.hypot(out["x"] - xs - case["dx"] / 4, out["y"] - ys - case["dy"] / 4)[ok]
+        assert ok.mean() >= 0.9
+        assert np.median(epe) <= 0.0115 and np.percentile(epe, 95) <= 0.0209, (case, np.median(epe))
+        lane = oracle.lk_batch(0, I, J, g["levels"] - 1, cfg, xs, ys)
+        both = ok & (lane["status"] == 0)
+        assert np.max(np.abs(lane["x"] - out["x"])[both]) < 1e-3
+
+
+def test_lk_reference_blob_workload(oracle):
+    """The lane tier equals oracle/_ref's lk_track_vx on the 640 x 480 blob workload (fixture written from it)."""
+    import vtgpu
+    g = json.load(open(os.path.join(GOLD, "lk_ref.json")))
+    w, h, L = g["w"], g["h"], g["levels"]
+    f0, f1, mo = vtgpu.frame_pair(w, h, g["seed"])
+    assert np.allclose([mo.dx, mo.dy, mo.theta_deg], g["motion"], rtol=0, atol=1e-12)
+    k0, I = oracle.pyramid(f0, w, h, L)
+    k1, J = oracle.pyramid(f1, w, h, L)
+    xs, ys = vtgpu.grid(*g["grid"], w, h)
+    c = g["cfg"]
+    cfg = lk_cfg(c["w"], c["h"], c["max_iter"], c["eps"], c["min_eig"], c["max_residual"])
+    out = oracle.lk_batch(0, I, J, L - 1, cfg, xs, ys)
+    assert "%016x" % oracle.lib.vo_hash_lk(out.ctypes.data, len(out)) == g["fnv"]
+    out = oracle.lk_batch(0, I, J, L - 1, cfg, xs, ys, fb_max=0.5)
+    assert "%016x" % oracle.lib.vo_hash_lk(out.ctypes.data, len(out)) == g["fnv_fb05"]
+
+
+def _plane_from(img):
+    h, w = img.shape
+    hp = HostPlane(w, h)
+    hp.view()[:] = img.astype(np.float32) * 64
+    return hp
+
+
+def _pad(o, hp):
+    o.lib.vo_pad_rows(C.byref(hp.c), 0, hp.h)
+    o.lib.vo_pad_tb(C.byref(hp.c))
+
+
+def test_ncc_reference_vectors(oracle):
+    """vo_ncc_at at 32 x 21 equals the reference's tk_correlate on the committed vectors (long double bits)."""
+    g = json.load(open(os.path.join(GOLD, "ncc_ref.json")))
+    for case in g["cases"]:
+        img = oracle.m2_render(case["w"], case["h"], case["ox"], case["oy"])[:, :case["w"]]
+        hp = _plane_from(img)
+        _pad(oracle, hp)
+        app, norm = oracle.ncc_template(hp, case["x0"], case["y0"], 32, 21)
+        assert norm == np.float32(case["norm"])
+        got = [oracle.ncc_at(hp, app, 32, 21, norm, x, y) for x, y in case["corners"]]
+        assert got == case["ncc"]
+
+
+def _textbook_zncc(win, tmpl):
+    a = win.astype(np.float64) - win.mean()
+    t = tmpl.astype(np.float64) - tmpl.mean()
+    return float((a * t).sum() / np.sqrt((a * a).sum() * (t * t).sum()))
+
+
+def test_ncc_closed_forms(oracle):
+    """Affine copies of the template score +1 and -1; a blank window scores 0 (the energy floor); corners that do
+    not fit score -1; small windows agree with the textbook fp64 ZNCC."""
+    rng = np.random.default_rng(5)
+    w, h = 120, 90
+    img = rng.integers(40, 200, (h, w)).astype(np.uint8)
+    hp = _plane_from(img)
+    _pad(oracle, hp)
+    app, norm = oracle.ncc_template(hp, 10, 12, 32, 21)
+    assert abs(oracle.ncc_at(hp, app, 32, 21, norm, 10, 12) - 1.0) < 1e-6
+    # affine copy and inverted copy elsewhere
+    win = img[12:33, 10:42].astype(np.int64)
+    img2 = img.copy()
+    img2[50:71, 60:92] = np.clip(win // 2 + 30, 0, 255)
+    img2[20:41, 70:102] = 255 - win
+    hp2 = _plane_from(img2)
+    _pad(oracle, hp2)
+    assert oracle.ncc_at(hp2, app, 32, 21, norm, 60, 50) > 0.999
+    assert oracle.ncc_at(hp2, app, 32, 21, norm, 70, 20) < -0.9999
+    flat = _plane_from(np.full((h, w), 100, np.uint8))
+    _pad(oracle, flat)
+    assert oracle.ncc_at(flat, app, 32, 21, norm, 30, 30) == 0.0
+    assert oracle.ncc_at(hp, app, 32, 21, norm, 0, 5) == -1.0
+    assert oracle.ncc_at(hp, app, 32, 21, norm, w - 33, 5) == -1.0
+    for (x, y) in [(3, 4), (40, 30), (77, 60)]:
+        ref = _textbook_zncc(img[y:y + 21, x:x + 32], img[12:33, 10:42])
+        assert abs(oracle.ncc_at(hp, app, 32, 21, norm, x, y) - ref) < 2e-6
+
+
+def test_ncc_search_brute_force(oracle):
+    """search_slot's map and pick against a Python brute force over the same scores: per-cell first maximum,
+    top 8 by (score, cell order), half-pixel refinement."""
+    rng = np.random.default_rng(11)
+    img = oracle.m2_render(96, 80, 300, 500)[:, :96]
+    hp = _plane_from(img)
+    _pad(oracle, hp)
+    tw, th = 16, 16
+    app, norm = oracle.ncc_template(hp, 40, 30, tw, th)
+    f, cells = oracle.ncc_search(hp, app, tw, th, norm, 1, 96, 1, 80, 1)
+    x1, y1 = 96 - tw - 2, 80 - th - 2
+    cols, rows = (x1 - 1) // 16 + 1, (y1 - 1) // 16 + 1
+    assert len(cells) == cols * rows
+    for ci in range(len(cells)):
+        cy0, cx0 = 1 + (ci // cols) * 16, 1 + (ci % cols) * 16
+        best = (-2.0, cx0, cy0)
+        for y in range(cy0, min(cy0 + 16, y1 + 1)):
+            for x in range(cx0, min(cx0 + 16, x1 + 1)):
+                v = np.float32(oracle.ncc_at(hp, app, tw, th, norm, x, y))
+                if v > best[0]:
+                    best = (v, x, y)
+        assert (cells[ci]["v"], cells[ci]["x"], cells[ci]["y"]) == best
+    assert (f.hx, f.hy) == (80, 60) and abs(f.ncc - 1.0) < 1e-6
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SO), reason="oracle/_ref not built (no reference tree here)")
+def test_live_reference_lk_and_ncc(oracle):
+    """Where oracle/_ref exists: the lane tier equals the reference's lk_track_vx and lk_track_ref's distance stays
+    in the published x87-vs-MVEX band on the m2 inputs; vo_ncc_at equals tk_correlate at random corners."""
+    from oracle_ffi import LkCfg, Plane
+    ref = C.CDLL(REF_SO)
+    ref.lk_track_vx.argtypes = [C.POINTER(Plane), C.POINTER(Plane), C.c_int, C.POINTER(LkCfg), C.c_float, C.c_float,
+                                C.c_float, C.c_float, C.c_void_p, C.c_void_p]
+    ref.lk_tile_bytes.restype = C.c_size_t
+    g, w, h, cfg, xs, ys = _m2_setup(oracle)
+    tile = np.zeros(ref.lk_tile_bytes() // 4 + 64, np.float32)
+    tp = (tile.ctypes.data + 63) & ~63
+    k0, I = oracle.pyramid(oracle.m2_render(w, h, 2048, 2048), w, h, 4)
+    k1, J = oracle.pyramid(oracle.m2_render(w, h, 2048 - 13, 2048 - 7), w, h, 4)
+    mine = oracle.lk_batch(0, I, J, 3, cfg, xs, ys)
+    theirs = np.zeros(len(xs), LK_OUT_DTYPE)
+    for i in range(len(xs)):
+        ref.lk_track_vx(I, J, 3, C.byref(cfg), xs[i], ys[i], 0.0, 0.0, theirs[i:i + 1].ctypes.data, tp)
+    assert mine.tobytes() == theirs.tobytes()
+    ref.tk_correlate.restype = C.c_longdouble
+    ref.tk_correlate.argtypes = [C.POINTER(Plane), C.c_void_p, C.c_longdouble, C.c_int, C.c_int]
+    rng = np.random.default_rng(3)
+    img = oracle.m2_render(200, 150, 77, 99)[:, :200]
+    hp = _plane_from(img)
+    _pad(oracle, hp)
+    app, norm = oracle.ncc_template(hp, 50, 60, 32, 21)
+    for _ in range(200):
+        x, y = int(rng.integers(-3, 175)), int(rng.integers(-3, 135))
+        assert oracle.ncc_at(hp, app, 32, 21, norm, x, y) == ref.tk_correlate(C.byref(hp.c), app.ctypes.data, norm, x,
+                                                                              y)
