@@ -78,6 +78,10 @@ int vtg_pyr_destroy(vtg_ctx *ctx, vtg_pyr *p);
  * (sum of (1 4 6 4 1)^2 x src over the 5 x 5 at (2x, 2y), replicated border, + 128) >> 8.
  * on_device: frame is a device pointer (else host memory, copied up first). */
 int vtg_pyr_build(vtg_ctx *ctx, vtg_pyr *p, const uint8_t *frame, int pitch, int on_device);
+/* n pyramids of one storage at once (the batched form of tk_build for frames of any sizes): one launch per level
+ * covers every pyramid that has it. frames[i], pitches[i] as for vtg_pyr_build. */
+int vtg_pyr_build_many(vtg_ctx *ctx, int n, vtg_pyr *const *pyrs, const uint8_t *const *frames, const int *pitches,
+                       int on_device);
 /* Level l's device pointer, size, pitch in bytes and element size. */
 int vtg_pyr_level(const vtg_pyr *p, int level, void **dptr, int *w, int *h, size_t *pitch_bytes, int *elem_bytes);
 /* Level l into a host fp32 Q6 plane, with the reference's replicated padding filled (img_pad_rows, img_pad_tb,
@@ -99,6 +103,17 @@ int vtg_lk_track(vtg_ctx *ctx, const vtg_pyr *I, const vtg_pyr *J, int top, cons
 /* The same with the solve mode chosen. */
 int vtg_lk_track_ex(vtg_ctx *ctx, const vtg_pyr *I, const vtg_pyr *J, int top, const struct lk_cfg *cfg, int n,
                     const float *xy, const float *guess, float fb_max, int solve_mode, struct lk_out *out);
+
+/* Points of many frame pairs of any sizes in one launch per 64 pairs (the batched OP_LK): pair k tracks points
+ * offsets[k] .. offsets[k+1] - 1 of xy (and guess, out) from I[k] to J[k] starting at level top[k]. One window
+ * configuration and one storage for the whole batch. offsets: host array of npairs + 1 entries. */
+int vtg_lk_track_many(vtg_ctx *ctx, int npairs, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
+                      const int *offsets, const struct lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
+                      int solve_mode, struct lk_out *out);
+/* The LK schedule: margin > 0 stages each level's template patch and J's neighbourhood (the window plus `margin`
+ * pixels) in LDS with one batch of loads; 0 reads the pyramid in memory at every iteration. Results are the same
+ * bit for bit either way; only time differs (default 3). */
+int vtg_lk_set_margin(vtg_ctx *ctx, int margin);
 
 /* ------------------------------------------------------------------ NCC search (src/kern/track.c:184-208, 301-329, 935-974, 1615-1734) */
 

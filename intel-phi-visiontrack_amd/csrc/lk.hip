@@ -4,15 +4,22 @@
 // Lane layout. A w x h window is cut into G = 64 / w row groups of w lanes each (21 x 21: 3 groups of 7 rows, 63 lanes;
 // 15 x 15: 4 groups of 4 rows, 60 lanes). Lane (k, g) owns window column k and rows [g * chunk, (g + 1) * chunk): its
 // template values T, Tx, Ty live in registers for the whole level, and each iteration it walks down its rows carrying
-// the x-interpolated row, two loads per row, as the reference's vector code carries a row (lk_vx.c:100-126).
-// The window build samples the (h + 2) x (w + 2) bilinear window S into the wave's own LDS slice, then each lane takes
-// its 3 x 3 Sobel neighbourhoods from it. Sums over the window (G, b, the residual) reduce across the wave with DPP
-// (quad permutes, row mirrors, row broadcasts) and one readlane, so every lane holds the same uniform total.
-// The per-point driver (level walk, eigenvalue test, 2x2 solve, convergence, bounds) runs in fp64 on those uniform
-// values: the reference runs it in x87 long double; fp64 keeps flow within 1e-4 px of it. The patch arithmetic is fp32
-// with fused multiply-adds where the reference's vector code fuses them.
+// the x-interpolated row, as the reference's vector code carries a row (lk_vx.c:100-126).
+// Memory. At the start of a level the wave issues, in one batch, the loads of the template's (w + 3) x (h + 3) patch of I
+// and of J's neighbourhood of the level's starting estimate (the window plus a margin of m pixels), into its own LDS
+// slice: one round trip to memory per level. The bilinear window S, the Sobel gradients and every iteration's samples
+// of J then come from LDS; only an estimate that leaves the margin reloads J's neighbourhood.
+// Sums over the window (G, b, the residual) reduce across the wave with DPP (quad permutes, row mirrors, row
+// broadcasts) and one readlane, so every lane holds the same total. The per-point driver (level walk, eigenvalue test,
+// 2x2 solve, convergence, bounds) runs in fp64 on those uniform values, the reference's being x87 long double; the
+// patch arithmetic is fp32 with fused multiply-adds where the reference's vector code fuses them.
+// Batches. A launch covers points of up to 64 frame pairs of any sizes: a table maps each pair to its pyramids'
+// device descriptors, its top level and its slice of the point array.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <climits>
 
 #include "vtg_internal.hpp"
 
@@ -25,10 +32,15 @@ struct Cfg {
     float eps, min_eig, max_res;
 };
 
+constexpr int BMAX = 64;
+
 struct LkArgs {
-    PyrView I, J;
+    const PyrView *I[BMAX], *J[BMAX];
+    int top[BMAX];
+    int off[BMAX + 1];
+    int npairs, n;
     Cfg cfg;
-    int top, n, s_floats;
+    int margin, lds_floats;
     const float *xy, *guess;
     float fb_max;
     lk_out *out;
@@ -104,11 +116,143 @@ template <int MAXR> struct Tile {
     float t[MAXR], tx[MAXR], ty[MAXR];
 };
 
-// The template at (px, py) of level lv: S = bilinear window over rows -1..h, columns -1..w; T = S interior,
-// Tx, Ty = Sobel(S) / 512; returns the wave totals of Tx^2, Tx Ty, Ty^2 (build_vx, lk_vx.c:36-97).
-template <int MAXR, typename T>
-__device__ __forceinline__ void build(const Level &lv, double px, double py, const Geo &g, int lane, float *S,
+// A rectangle of a level copied into LDS: rows ry .. ry + rh - 1, columns rx .. rx + rw - 1, coordinates clamped to the
+// level (the reference's replicated border).
+struct Region {
+    int rx, ry, rw, rh;
+};
+
+// The wave's LDS slice: P = the template's patch of I ((h + 3) rows), directly followed by R = J's neighbourhood (both
+// rows of the region width w + 1 + 2 m), and S = the bilinear window ((w + 2) x (h + 2)).
+struct Slice {
+    float *P, *S, *R;
+};
+
+// The wave's loads of one level, issued together: rows 0 .. np - 1 of the combined region come from level li at
+// (rp.rx, rp.ry) (the template patch, np = 0 to skip it), the next rg.rh rows from level lj at (rg.rx, rg.ry) (J's
+// neighbourhood); every row is rg.rw wide. Element i = lane + 64 k sits at row i / rw, column i % rw, stepped without
+// a division per element. Coordinates are clamped to the level (the replicated border).
+template <typename T, int BATCH>
+__device__ __forceinline__ void load_regions(const Level &li, const Region &rp, int np, const Level &lj,
+                                             const Region &rg, float *dst, int lane)
+{
+    const int rw = rg.rw, n = (np + rg.rh) * rw;
+    const int dq = 64 / rw, dr = 64 - dq * rw;
+    int r = lane / rw, cc = lane - (lane / rw) * rw;
+    const T *pi = (const T *)li.p, *pj = (const T *)lj.p;
+    wave_sync(); // earlier readers of the slice are done
+    for (int base = 0; base < n; base += 64 * BATCH) {
+        float v[BATCH];
+        int rr = r, c2 = cc;
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const bool fromI = rr < np;
+            const int y = fromI ? clampi(rp.ry + rr, 0, li.h - 1) : clampi(rg.ry + rr - np, 0, lj.h - 1);
+            const int x = fromI ? clampi(rp.rx + c2, 0, li.w - 1) : clampi(rg.rx + c2, 0, lj.w - 1);
+            v[k] = Ld<T>::get(fromI ? pi[(size_t)y * li.pitch + x] : pj[(size_t)y * lj.pitch + x]);
+            c2 += dr;
+            rr += dq;
+            if (c2 >= rw)
+                c2 -= rw, rr++;
+        }
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const int i = base + lane + 64 * k;
+            if (i < n)
+                dst[i] = v[k];
+        }
+        r = rr;
+        cc = c2;
+    }
+    wave_sync();
+}
+
+// The template at (px, py) from the staged patch P (origin = the window's corner minus one row and column): S =
+// bilinear window over rows -1..h, columns -1..w; T = S interior, Tx, Ty = Sobel(S) / 512; returns the wave totals of
+// Tx^2, Tx Ty, Ty^2 (build_vx, lk_vx.c:36-97).
+template <int MAXR>
+__device__ __forceinline__ void build(const Slice &sl, int PW, float fx, float fy, const Geo &g, int lane,
                                       Tile<MAXR> &tl, float *G)
+{
+    const int SW = g.W + 2, SH = g.H + 2;
+    for (int i = lane; i < SW * SH; i += 64) {
+        const int r = i / SW, cc = i - r * SW;
+        const float *p = sl.P + r * PW + cc;
+        const float top = fmaf(p[1] - p[0], fx, p[0]), bot = fmaf(p[PW + 1] - p[PW], fx, p[PW]);
+        sl.S[i] = fmaf(bot - top, fy, top);
+    }
+    wave_sync();
+    float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+#pragma unroll
+    for (int i = 0; i < MAXR; i++) {
+        if (i < g.nr) {
+            const float *u = sl.S + (g.r0 + i) * SW + g.k, *m = u + SW, *d = m + SW;
+            const float du = u[2] - u[0], dm = m[2] - m[0], dd = d[2] - d[0];
+            const float tx = fmaf(dm, 2.0f, du + dd) * (1.0f / 512.0f);
+            const float vl = d[0] - u[0], vc = d[1] - u[1], vr = d[2] - u[2];
+            const float ty = fmaf(vc, 2.0f, vl + vr) * (1.0f / 512.0f);
+            tl.t[i] = m[1];
+            tl.tx[i] = tx;
+            tl.ty[i] = ty;
+            sxx = fmaf(tx, tx, sxx);
+            sxy = fmaf(tx, ty, sxy);
+            syy = fmaf(ty, ty, syy);
+        }
+    }
+    G[0] = wave_sum(sxx);
+    G[1] = wave_sum(sxy);
+    G[2] = wave_sum(syy);
+}
+
+// Whether the window at integer corner (x0, y0), with its extra column and row, lies inside the region.
+__device__ __forceinline__ bool covers(const Region &rg, int x0, int y0, int W, int H)
+{
+    return x0 >= rg.rx && y0 >= rg.ry && x0 + W <= rg.rx + rg.rw - 1 && y0 + H <= rg.ry + rg.rh - 1;
+}
+
+// J sampled at (qx, qy) from the staged neighbourhood: It = T - J; RES = false: totals of It Tx and It Ty
+// (iterate_vx, lk_vx.c:100-140); RES = true: totals of |It| / 64 and (It / 64)^2 (residual_vx, lk_vx.c:142-175).
+template <int MAXR, bool RES>
+__device__ __forceinline__ void sample(const float *R, const Region &rg, double qx, double qy, const Geo &g,
+                                       const Tile<MAXR> &tl, float *b)
+{
+    const double flx = floor(qx), fly = floor(qy);
+    const float fx = (float)(qx - flx), fy = (float)(qy - fly);
+    const int x = clampi((int)flx - g.W / 2 + g.k - rg.rx, 0, rg.rw - 2);
+    const int y0 = (int)fly - g.H / 2 + g.r0 - rg.ry;
+    float a0[MAXR + 1], a1[MAXR + 1];
+#pragma unroll
+    for (int i = 0; i <= MAXR; i++) {
+        const float *p = R + clampi(y0 + i, 0, rg.rh - 1) * rg.rw + x;
+        a0[i] = p[0];
+        a1[i] = p[1];
+    }
+    float s0 = 0.0f, s1 = 0.0f;
+    float above = fmaf(a1[0] - a0[0], fx, a0[0]);
+#pragma unroll
+    for (int i = 0; i < MAXR; i++) {
+        const float here = fmaf(a1[i + 1] - a0[i + 1], fx, a0[i + 1]);
+        const float it = tl.t[i] - fmaf(here - above, fy, above);
+        const bool on = i < g.nr;
+        if (RES) {
+            const float q = it * (1.0f / 64.0f);
+            s0 = on ? s0 + fabsf(q) : s0;
+            s1 = on ? fmaf(q, q, s1) : s1;
+        } else {
+            s0 = on ? fmaf(it, tl.tx[i], s0) : s0;
+            s1 = on ? fmaf(it, tl.ty[i], s1) : s1;
+        }
+        above = here;
+    }
+    b[0] = wave_sum(s0);
+    b[1] = wave_sum(s1);
+}
+
+// ---- the direct schedule: no staging; the window build and every iteration read the level in memory
+
+template <int MAXR, typename T>
+__device__ __forceinline__ void build_direct(const Level &lv, double px, double py, const Geo &g, int lane, float *S,
+                                             Tile<MAXR> &tl, float *G)
 {
     const double flx = floor(px), fly = floor(py);
     const float fx = (float)(px - flx), fy = (float)(py - fly);
@@ -116,6 +260,7 @@ __device__ __forceinline__ void build(const Level &lv, double px, double py, con
     const int xs = (int)flx - g.W / 2 - 1, ys = (int)fly - g.H / 2 - 1;
     const T *base = (const T *)lv.p;
     const int nS = SW * SH, nb = (nS + 63) / 64;
+    wave_sync();
 #pragma unroll 3
     for (int blk = 0; blk < nb; blk++) {
         const int i = lane + 64 * blk, ic = i < nS ? i : nS - 1;
@@ -149,14 +294,11 @@ __device__ __forceinline__ void build(const Level &lv, double px, double py, con
     G[0] = wave_sum(sxx);
     G[1] = wave_sum(sxy);
     G[2] = wave_sum(syy);
-    wave_sync(); // S is rewritten by the next build
 }
 
-// J sampled at (qx, qy): It = T - J; RES = false: totals of It Tx and It Ty (iterate_vx, lk_vx.c:100-140);
-// RES = true: totals of |It| / 64 and (It / 64)^2 (residual_vx, lk_vx.c:142-175).
 template <int MAXR, typename T, bool RES>
-__device__ __forceinline__ void sample(const Level &lv, double qx, double qy, const Geo &g, const Tile<MAXR> &tl,
-                                       float *b)
+__device__ __forceinline__ void sample_direct(const Level &lv, double qx, double qy, const Geo &g,
+                                              const Tile<MAXR> &tl, float *b)
 {
     const double flx = floor(qx), fly = floor(qy);
     const float fx = (float)(qx - flx), fy = (float)(qy - fly);
@@ -164,7 +306,6 @@ __device__ __forceinline__ void sample(const Level &lv, double qx, double qy, co
     const int y0 = (int)fly - g.H / 2 + g.r0;
     const T *col = (const T *)lv.p + x;
     const size_t pitch = (size_t)lv.pitch;
-    // every load of the pass issued at once (rows clamped to the level, so idle lanes read valid memory too)
     float a0[MAXR + 1], a1[MAXR + 1];
 #pragma unroll
     for (int i = 0; i <= MAXR; i++) {
@@ -231,11 +372,29 @@ __device__ __forceinline__ Solve solve2(double gxx, double gxy, double gyy, doub
     return s;
 }
 
-// One point from pyramid I to J: the reference's level walk (lk_driver.inc:7-109).
-template <int MAXR, typename T0, typename T1, int SOLVE>
-__device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c, const Geo &g, int lane, float *S,
-                      float x, float y, float gx0, float gy0, Track &o)
+// Load the level's template patch (with_patch) and J's neighbourhood together (T: the level's element type).
+template <typename T, int BATCH>
+__device__ __forceinline__ void level_loads(const Level &li, const Region &rp, const Level &lj, const Region &rg,
+                                            const Slice &sl, bool with_patch, int lane)
 {
+    if (with_patch)
+        load_regions<T, BATCH>(li, rp, rp.rh, lj, rg, sl.P, lane);
+    else
+        load_regions<T, BATCH>(li, rp, 0, lj, rg, sl.R, lane);
+}
+
+// One point from pyramid I to J: the reference's level walk (lk_driver.inc:7-109).
+template <int MAXR, typename T0, typename T1, int SOLVE, bool STAGED>
+__device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c, const Geo &g, int lane,
+                      const Slice &sl, int margin, float x, float y, float gx0, float gy0, Track &o)
+{
+    constexpr int BATCH = MAXR <= 8 ? 26 : 16;
+    Region rg, rp;
+    rg.rw = c.w + 1 + 2 * margin;
+    rg.rh = c.h + 1 + 2 * margin;
+    rg.rx = rg.ry = INT_MIN; // nothing staged
+    rp.rw = rg.rw; // the patch needs w + 3 columns; it shares the region's row width (margin >= 1)
+    rp.rh = c.h + 3;
     Tile<MAXR> tl;
     double gx = ldexp((double)gx0, -top), gy = ldexp((double)gy0, -top);
     const double area = (double)(c.w * c.h), eps2 = (double)c.eps * (double)c.eps;
@@ -255,10 +414,23 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             continue;
         }
         float G[3];
-        if (L > 0)
-            build<MAXR, T1>(li, px, py, g, lane, S, tl, G);
-        else
-            build<MAXR, T0>(li, px, py, g, lane, S, tl, G);
+        if (STAGED) {
+            // the template patch and J around the level's first estimate, one batch
+            const double flx = floor(px), fly = floor(py);
+            rp.rx = (int)flx - c.w / 2 - 1;
+            rp.ry = (int)fly - c.h / 2 - 1;
+            rg.rx = (int)floor(px + gx) - c.w / 2 - margin;
+            rg.ry = (int)floor(py + gy) - c.h / 2 - margin;
+            if (L > 0)
+                level_loads<T1, BATCH>(li, rp, lj, rg, sl, true, lane);
+            else
+                level_loads<T0, BATCH>(li, rp, lj, rg, sl, true, lane);
+            build<MAXR>(sl, rg.rw, (float)(px - flx), (float)(py - fly), g, lane, tl, G);
+        } else if (L > 0) {
+            build_direct<MAXR, T1>(li, px, py, g, lane, sl.S, tl, G);
+        } else {
+            build_direct<MAXR, T0>(li, px, py, g, lane, sl.S, tl, G);
+        }
         const double gxx = G[0], gxy = G[1], gyy = G[2];
         const Solve s = solve2<SOLVE>(gxx, gxy, gyy, area);
         if (L == 0)
@@ -284,10 +456,22 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
                 break;
             }
             float b[2];
-            if (L > 0)
-                sample<MAXR, T1, false>(lj, qx, qy, g, tl, b);
-            else
-                sample<MAXR, T0, false>(lj, qx, qy, g, tl, b);
+            if (STAGED) {
+                const int wx = (int)floor(qx) - c.w / 2, wy = (int)floor(qy) - c.h / 2;
+                if (!covers(rg, wx, wy, c.w, c.h)) {
+                    rg.rx = wx - margin;
+                    rg.ry = wy - margin;
+                    if (L > 0)
+                        level_loads<T1, BATCH>(li, rp, lj, rg, sl, false, lane);
+                    else
+                        level_loads<T0, BATCH>(li, rp, lj, rg, sl, false, lane);
+                }
+                sample<MAXR, false>(sl.R, rg, qx, qy, g, tl, b);
+            } else if (L > 0) {
+                sample_direct<MAXR, T1, false>(lj, qx, qy, g, tl, b);
+            } else {
+                sample_direct<MAXR, T0, false>(lj, qx, qy, g, tl, b);
+            }
             const double bx = (double)b[0] * (1.0 / 64.0), by = (double)b[1] * (1.0 / 64.0);
             const double ex = s.ia * bx + s.ib * by, ey = s.ib * bx + s.ic * by;
             vx += ex;
@@ -314,7 +498,18 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             status |= LK_OUT;
         } else {
             float r[2];
-            sample<MAXR, T0, true>(l0, qx, qy, g, tl, r);
+            if (STAGED) {
+                // level 0 was the last level walked, so rg describes level 0's neighbourhood in R
+                const int wx = (int)floor(qx) - c.w / 2, wy = (int)floor(qy) - c.h / 2;
+                if (!covers(rg, wx, wy, c.w, c.h)) {
+                    rg.rx = wx - margin;
+                    rg.ry = wy - margin;
+                    level_loads<T0, BATCH>(I.l[0], rp, l0, rg, sl, false, lane);
+                }
+                sample<MAXR, true>(sl.R, rg, qx, qy, g, tl, r);
+            } else {
+                sample_direct<MAXR, T0, true>(l0, qx, qy, g, tl, r);
+            }
             const double n = area;
             o.residual = (float)((double)r[0] / n);
             if (o.residual > c.max_res)
@@ -328,7 +523,7 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
     o.status = status;
 }
 
-template <int MAXR, typename T0, typename T1, int SOLVE>
+template <int MAXR, typename T0, typename T1, int SOLVE, bool STAGED>
 __global__ __launch_bounds__(256) void lk_kernel(LkArgs a)
 {
     extern __shared__ float smem[];
@@ -337,16 +532,29 @@ __global__ __launch_bounds__(256) void lk_kernel(LkArgs a)
     const int pt = blockIdx.x * 4 + wave;
     if (pt >= a.n)
         return;
-    float *S = smem + wave * a.s_floats;
+    // this point's pair: static indices only, so the table stays in the kernel arguments
+    const PyrView *Ip = a.I[0], *Jp = a.J[0];
+    int top = a.top[0];
+#pragma unroll
+    for (int q = 1; q < BMAX; q++)
+        if (q < a.npairs && pt >= a.off[q])
+            Ip = a.I[q], Jp = a.J[q], top = a.top[q];
+    const PyrView &I = *Ip, &J = *Jp;
+    const int rw = a.cfg.w + 1 + 2 * a.margin, ph = a.cfg.h + 3;
+    float *base = smem + wave * a.lds_floats;
+    // layout: S first, then P and R contiguous (one combined region)
+    const int sS = (a.cfg.w + 2) * (a.cfg.h + 2);
+    const Slice sl{base + sS, base, base + sS + rw * ph};
     const Geo g = geometry(a.cfg, lane);
     const float x = a.xy[2 * pt], y = a.xy[2 * pt + 1];
     const float gx = a.guess ? a.guess[2 * pt] : 0.0f, gy = a.guess ? a.guess[2 * pt + 1] : 0.0f;
     Track o;
-    track<MAXR, T0, T1, SOLVE>(a.I, a.J, a.top, a.cfg, g, lane, S, x, y, gx, gy, o);
+    track<MAXR, T0, T1, SOLVE, STAGED>(I, J, top, a.cfg, g, lane, sl, a.margin, x, y, gx, gy, o);
     if (a.fb_max > 0.0f && o.status == 0) {
         // back from J to I, the template rebuilt from J (lk_driver.inc:111-121)
         Track b;
-        track<MAXR, T0, T1, SOLVE>(a.J, a.I, a.top, a.cfg, g, lane, S, o.x, o.y, (float)((double)x - (double)o.x),
+        track<MAXR, T0, T1, SOLVE, STAGED>(J, I, top, a.cfg, g, lane, sl, a.margin, o.x, o.y,
+                                           (float)((double)x - (double)o.x),
                                    (float)((double)y - (double)o.y), b);
         const double ex = (double)b.x - (double)x, ey = (double)b.y - (double)y;
         if (b.status || ex * ex + ey * ey > (double)a.fb_max * (double)a.fb_max)
@@ -367,21 +575,22 @@ __global__ __launch_bounds__(256) void lk_kernel(LkArgs a)
     }
 }
 
-template <int MAXR, int SOLVE>
+template <int MAXR, int SOLVE, bool STAGED>
 int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, size_t lds)
 {
     switch (storage) {
     case VTG_STORE_Q6:
-        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, uint16_t, SOLVE>), grid, dim3(256), lds, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, uint16_t, SOLVE, STAGED>), grid, dim3(256), lds, c->stream, a);
         break;
     case VTG_STORE_F32:
-        hipLaunchKernelGGL((lk_kernel<MAXR, float, float, SOLVE>), grid, dim3(256), lds, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<MAXR, float, float, SOLVE, STAGED>), grid, dim3(256), lds, c->stream, a);
         break;
     case VTG_STORE_F16:
-        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __half, SOLVE>), grid, dim3(256), lds, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __half, SOLVE, STAGED>), grid, dim3(256), lds, c->stream, a);
         break;
     case VTG_STORE_BF16:
-        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __hip_bfloat16, SOLVE>), grid, dim3(256), lds, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __hip_bfloat16, SOLVE, STAGED>), grid, dim3(256), lds, c->stream,
+                           a);
         break;
     default:
         return fail(c, VTG_EINVAL, "storage %d", storage);
@@ -390,43 +599,83 @@ int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, size_t
     return VTG_OK;
 }
 
-template <int SOLVE>
+template <int SOLVE, bool STAGED>
 int dispatch_rows(vtg_ctx *c, const LkArgs &a, int storage, int chunk, dim3 grid, size_t lds)
 {
     if (chunk <= 4)
-        return dispatch_storage<4, SOLVE>(c, a, storage, grid, lds);
+        return dispatch_storage<4, SOLVE, STAGED>(c, a, storage, grid, lds);
     if (chunk <= 8)
-        return dispatch_storage<8, SOLVE>(c, a, storage, grid, lds);
+        return dispatch_storage<8, SOLVE, STAGED>(c, a, storage, grid, lds);
     if (chunk <= 16)
-        return dispatch_storage<16, SOLVE>(c, a, storage, grid, lds);
-    return dispatch_storage<32, SOLVE>(c, a, storage, grid, lds);
+        return dispatch_storage<16, SOLVE, STAGED>(c, a, storage, grid, lds);
+    return dispatch_storage<32, SOLVE, STAGED>(c, a, storage, grid, lds);
+}
+
+int check_cfg(vtg_ctx *c, const lk_cfg *cfg, int solve_mode)
+{
+    if (cfg->w < 5 || cfg->w > LK_MAX_W || cfg->h < 5 || cfg->h > LK_MAX_H || cfg->max_iter < 0)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: window %dx%d (5..%d x 5..%d), max_iter %d", cfg->w, cfg->h,
+                    LK_MAX_W, LK_MAX_H, cfg->max_iter);
+    if (solve_mode != VTG_SOLVE_F64 && solve_mode != VTG_SOLVE_MIXED)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: solve mode %d", solve_mode);
+    return VTG_OK;
+}
+
+int check_pair(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top)
+{
+    if (!I || !J)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: null pyramid");
+    if (top < 0 || top >= I->levels || top >= J->levels)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: top %d with %d / %d levels", top, I->levels, J->levels);
+    if (I->storage != J->storage)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: pyramids of different storage");
+    for (int l = 0; l <= top; l++)
+        if (I->lv[l].w != J->lv[l].w || I->lv[l].h != J->lv[l].h)
+            return fail(c, VTG_EINVAL, "vtg_lk_track: level %d sizes differ", l);
+    return VTG_OK;
 }
 
 }  // namespace
 
-int launch_lk(vtg_ctx *c, const PyrView &I, const PyrView &J, int top, const lk_cfg *cfg, int n, const float *xy,
-              const float *guess, float fb_max, int solve_mode, lk_out *out)
+// Pairs [0, np) with points off[k] .. off[k+1] of xy; np <= BMAX.
+int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
+                   const int *off, const lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
+                   int solve_mode, lk_out *out)
 {
-    if (n <= 0)
-        return VTG_OK;
     LkArgs a;
-    a.I = I;
-    a.J = J;
+    a.npairs = np;
+    for (int k = 0; k < np; k++) {
+        a.I[k] = I[k]->dview;
+        a.J[k] = J[k]->dview;
+        a.top[k] = top[k];
+        a.off[k] = off[k] - off[0];
+    }
+    a.off[np] = off[np] - off[0];
+    a.n = off[np] - off[0];
+    if (a.n <= 0)
+        return VTG_OK;
     a.cfg = Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->eps, cfg->min_eig, cfg->max_residual};
-    a.top = top;
-    a.n = n;
-    a.s_floats = (cfg->w + 2) * (cfg->h + 2);
-    a.xy = xy;
-    a.guess = guess;
+    a.margin = c->lk_margin > 0 ? c->lk_margin : 1;
+    a.lds_floats = c->lk_margin > 0 ? (cfg->w + 2) * (cfg->h + 2) +
+                                          (cfg->w + 1 + 2 * a.margin) * (cfg->h + 3 + cfg->h + 1 + 2 * a.margin)
+                                    : (cfg->w + 2) * (cfg->h + 2);
+    a.xy = xy + 2 * (size_t)off[0];
+    a.guess = guess ? guess + 2 * (size_t)off[0] : nullptr;
     a.fb_max = fb_max;
-    a.out = out;
+    a.out = out + off[0];
     const int groups = 64 / cfg->w, chunk = (cfg->h + groups - 1) / groups;
-    const dim3 grid((n + 3) / 4);
-    const size_t lds = (size_t)4 * a.s_floats * sizeof(float);
+    const dim3 grid((a.n + 3) / 4);
+    const size_t lds = (size_t)4 * a.lds_floats * sizeof(float);
     hipEvent_t ev;
     timer_begin(c, "lk_track", &ev);
-    const int rc = solve_mode == VTG_SOLVE_MIXED ? dispatch_rows<VTG_SOLVE_MIXED>(c, a, I.storage, chunk, grid, lds)
-                                                 : dispatch_rows<VTG_SOLVE_F64>(c, a, I.storage, chunk, grid, lds);
+    const bool staged = c->lk_margin > 0;
+    int rc;
+    if (solve_mode == VTG_SOLVE_MIXED)
+        rc = staged ? dispatch_rows<VTG_SOLVE_MIXED, true>(c, a, I[0]->storage, chunk, grid, lds)
+                    : dispatch_rows<VTG_SOLVE_MIXED, false>(c, a, I[0]->storage, chunk, grid, lds);
+    else
+        rc = staged ? dispatch_rows<VTG_SOLVE_F64, true>(c, a, I[0]->storage, chunk, grid, lds)
+                    : dispatch_rows<VTG_SOLVE_F64, false>(c, a, I[0]->storage, chunk, grid, lds);
     timer_end(c, "lk_track", ev);
     return rc;
 }
@@ -440,27 +689,55 @@ extern "C" {
 int vtg_lk_track_ex(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top, const struct lk_cfg *cfg, int n,
                     const float *xy, const float *guess, float fb_max, int solve_mode, struct lk_out *out)
 {
-    if (!c || !I || !J || !cfg || (n > 0 && (!xy || !out)) || n < 0)
+    if (!c || !cfg || (n > 0 && (!xy || !out)) || n < 0)
         return fail(c, VTG_EINVAL, "vtg_lk_track: bad argument");
-    if (cfg->w < 5 || cfg->w > LK_MAX_W || cfg->h < 5 || cfg->h > LK_MAX_H || cfg->max_iter < 0)
-        return fail(c, VTG_EINVAL, "vtg_lk_track: window %dx%d (5..%d x 5..%d), max_iter %d", cfg->w, cfg->h,
-                    LK_MAX_W, LK_MAX_H, cfg->max_iter);
-    if (top < 0 || top >= I->levels || top >= J->levels)
-        return fail(c, VTG_EINVAL, "vtg_lk_track: top %d with %d / %d levels", top, I->levels, J->levels);
-    if (I->storage != J->storage)
-        return fail(c, VTG_EINVAL, "vtg_lk_track: pyramids of different storage");
-    if (solve_mode != VTG_SOLVE_F64 && solve_mode != VTG_SOLVE_MIXED)
-        return fail(c, VTG_EINVAL, "vtg_lk_track: solve mode %d", solve_mode);
-    for (int l = 0; l <= top; l++)
-        if (I->lv[l].w != J->lv[l].w || I->lv[l].h != J->lv[l].h)
-            return fail(c, VTG_EINVAL, "vtg_lk_track: level %d sizes differ", l);
-    return launch_lk(c, view(I), view(J), top, cfg, n, xy, guess, fb_max, solve_mode, out);
+    int rc = check_cfg(c, cfg, solve_mode);
+    if (!rc)
+        rc = check_pair(c, I, J, top);
+    if (rc || n == 0)
+        return rc;
+    const int off[2] = {0, n};
+    return launch_lk_many(c, 1, &I, &J, &top, off, cfg, xy, guess, fb_max, solve_mode, out);
 }
 
 int vtg_lk_track(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top, const struct lk_cfg *cfg, int n,
                  const float *xy, const float *guess, float fb_max, struct lk_out *out)
 {
     return vtg_lk_track_ex(c, I, J, top, cfg, n, xy, guess, fb_max, VTG_SOLVE_F64, out);
+}
+
+int vtg_lk_track_many(vtg_ctx *c, int npairs, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
+                      const int *offsets, const struct lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
+                      int solve_mode, struct lk_out *out)
+{
+    if (!c || npairs < 0 || (npairs > 0 && (!I || !J || !top || !offsets)) || !cfg)
+        return fail(c, VTG_EINVAL, "vtg_lk_track_many: bad argument");
+    int rc = check_cfg(c, cfg, solve_mode);
+    if (rc)
+        return rc;
+    for (int k = 0; k < npairs; k++) {
+        if ((rc = check_pair(c, I[k], J[k], top[k])))
+            return rc;
+        if (offsets[k + 1] < offsets[k] || I[k]->storage != I[0]->storage)
+            return fail(c, VTG_EINVAL, "vtg_lk_track_many: pair %d: offsets or storage", k);
+    }
+    if (npairs == 0 || offsets[npairs] == offsets[0])
+        return VTG_OK;
+    if (!xy || !out)
+        return fail(c, VTG_EINVAL, "vtg_lk_track_many: null points or output");
+    for (int k = 0; k < npairs && !rc; k += BMAX) {
+        const int m = std::min(BMAX, npairs - k);
+        rc = launch_lk_many(c, m, I + k, J + k, top + k, offsets + k, cfg, xy, guess, fb_max, solve_mode, out);
+    }
+    return rc;
+}
+
+int vtg_lk_set_margin(vtg_ctx *c, int margin)
+{
+    if (!c || margin < 0 || margin > 16)
+        return fail(c, VTG_EINVAL, "vtg_lk_set_margin: %d (0..16)", margin);
+    c->lk_margin = margin;
+    return VTG_OK;
 }
 
 }  // extern "C"

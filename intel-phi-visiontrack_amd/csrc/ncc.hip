@@ -90,11 +90,22 @@ __global__ __launch_bounds__(256) void ncc_template_kernel(Level lv, int x0, int
     __shared__ float mean_s;
     const int tid = threadIdx.x, n = tw * th;
     const T *base = (const T *)lv.p;
-    for (int i = tid; i < n; i += 256) {
-        const int r = i / tw, k = i - r * tw;
-        const float v = Q6v<T>::get(base[(size_t)(y0 + r) * lv.pitch + x0 + k]);
-        px[i] = v;
-        raw[i] = v;
+    {
+        float v[16]; // n <= 64 x 64: every load issued before the first store
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int i = tid + 256 * k, ic = i < n ? i : n - 1;
+            const int r = ic / tw, kk = ic - r * tw;
+            v[k] = Q6v<T>::get(base[(size_t)(y0 + r) * lv.pitch + x0 + kk]);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int i = tid + 256 * k;
+            if (i < n) {
+                px[i] = v[k];
+                raw[i] = v[k];
+            }
+        }
     }
     __syncthreads();
     if (words) {
@@ -434,152 +445,245 @@ __device__ __forceinline__ bool before(const Cand &a, const Cand &b)
     return a.v > b.v || (a.v == b.v && a.idx < b.idx);
 }
 
-// What the finish kernels pass on: the top cells, then every half-pixel score of the refinement.
+constexpr int RING_MAX = 9; // half-pixel grid of one candidate: 2 (around + 1) + 1, around <= 3
+
+// What the finish kernels pass on: the top cells, then every half-pixel score around each.
 struct Aux {
     int ntop, tops[8];
     int pad[7];
-    double hs[8 * 49];
+    double hs[8][RING_MAX * RING_MAX];
 };
 
-// The 8 best cells by (score, cell order), scores above -2 only: what search_slot's insertion keeps
-// (track.c:1686-1694). Per thread over a strided share in index order, then a merge tree.
-__global__ __launch_bounds__(256) void ncc_top8(const vtg_cell *cells, int ncell, Aux *aux)
+// A cell's rank key: smaller is better. High word: the score, order-inverted (higher score, smaller key); low word:
+// the cell index (earlier cell, smaller key): exactly the order search_slot's insertion keeps (track.c:1686-1694).
+__device__ __forceinline__ unsigned long long cell_key(float v, int idx)
 {
-    __shared__ Cand lists[256][8];
-    __shared__ int nlist[256];
-    const int tid = threadIdx.x;
-    Cand loc[8];
-    int nl = 0;
-    for (int i = tid; i < ncell; i += 256) {
-        const float v = cells[i].v;
-        if (v <= -2.0f || (nl == 8 && !(v > loc[7].v)))
-            continue;
-        int at = nl < 8 ? nl++ : 7;
-        for (; at > 0 && loc[at - 1].v < v; at--)
-            loc[at] = loc[at - 1];
-        loc[at] = Cand{v, i};
-    }
-    for (int k = 0; k < 8; k++)
-        lists[tid][k] = k < nl ? loc[k] : Cand{-3.0f, 0x7fffffff};
-    nlist[tid] = nl;
-    __syncthreads();
-    for (int s = 1; s < 256; s <<= 1) {
-        if ((tid % (2 * s)) == 0) {
-            Cand m[8];
-            int ia = 0, ib = 0, nm_ = 0;
-            const int na = nlist[tid], nb = nlist[tid + s];
-            while (nm_ < 8 && (ia < na || ib < nb)) {
-                if (ib >= nb || (ia < na && before(lists[tid][ia], lists[tid + s][ib])))
-                    m[nm_++] = lists[tid][ia++];
-                else
-                    m[nm_++] = lists[tid + s][ib++];
-            }
-            for (int k = 0; k < 8; k++)
-                lists[tid][k] = k < nm_ ? m[k] : Cand{-3.0f, 0x7fffffff};
-            nlist[tid] = nm_;
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        aux->ntop = nlist[0];
-        for (int k = 0; k < 8; k++)
-            aux->tops[k] = lists[0][k].idx;
-    }
+    const unsigned int b = __float_as_uint(v);
+    const unsigned int ord = (b & 0x80000000u) ? ~b : (b | 0x80000000u); // increasing with v
+    return ((unsigned long long)(~ord) << 32) | (unsigned int)idx;
 }
 
-// Every half-pixel corner within 2 step - 1 (1 at step 1) of each top cell's best, four per wave (track.c:1697-1711).
+__device__ __forceinline__ unsigned long long min64(unsigned long long a, unsigned long long b) { return a < b ? a : b; }
+
+// The 8 best cells, scores above -2 only: 8 rounds of a workgroup-wide minimum of the keys above the last one taken.
+__global__ __launch_bounds__(1024) void ncc_top8(const vtg_cell *cells, int ncell, Aux *aux)
+{
+    __shared__ unsigned long long wmin[16];
+    __shared__ unsigned long long taken;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    unsigned long long prev = 0;
+    int n = 0;
+    for (int round = 0; round < 8; round++) {
+        unsigned long long m = ~0ull;
+        for (int i = tid; i < ncell; i += 1024) {
+            const float v = cells[i].v;
+            const unsigned long long k = cell_key(v, i);
+            if (v > -2.0f && (round == 0 || k > prev))
+                m = min64(m, k);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned int lo = __shfl_xor((unsigned int)m, o), hi = __shfl_xor((unsigned int)(m >> 32), o);
+            m = min64(m, ((unsigned long long)hi << 32) | lo);
+        }
+        if (lane == 0)
+            wmin[wave] = m;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long t = wmin[0];
+            for (int w = 1; w < 16; w++)
+                t = min64(t, wmin[w]);
+            taken = t;
+        }
+        __syncthreads();
+        prev = taken;
+        if (prev == ~0ull)
+            break;
+        if (tid == 0)
+            aux->tops[round] = (int)(unsigned int)prev;
+        n++;
+        __syncthreads(); // taken is rewritten next round
+    }
+    if (tid == 0)
+        aux->ntop = n;
+}
+
+// The score at half-pixel corner (hx, hy) from a staged region: pixel (x, y) of the level at reg[(y - ry) * rw + x - rx],
+// template in LDS too; same arithmetic as score16.
+__device__ double score16_lds(const float *reg, int rx, int ry, int rw, const float *t, int tw, int th, double norm,
+                              int lw, int lh, int hx, int hy, int lane)
+{
+    const int j = lane & 15;
+    const int x = hx >> 1, y = hy >> 1, ox = hx & 1, oy = hy & 1;
+    const bool fits = !(x < 1 || y < 1 || x + tw + 1 >= lw - 1 || y + th + 1 >= lh - 1);
+    float st = 0.0f, sj = 0.0f, sjj = 0.0f;
+    if (fits) {
+        const float *base = reg + (y - ry) * rw + (x - rx);
+        const float first = base[0];
+        for (int r = 0; r < th; r++) {
+            const float *row = base + r * rw + j;
+            for (int k = 0; k < tw; k += 16) {
+                float v = row[k];
+                if (ox)
+                    v = (v + row[k + 1]) * 0.5f;
+                if (oy) {
+                    float w = row[k + rw];
+                    if (ox)
+                        w = (w + row[k + rw + 1]) * 0.5f;
+                    v = (v + w) * 0.5f;
+                }
+                v = v - first;
+                st = fmaf(v, t[r * tw + k + j], st);
+                sj = sj + v;
+                sjj = fmaf(v, v, sjj);
+            }
+        }
+    }
+    for (int step = 1; step < 16; step <<= 1) {
+        st = st + __shfl_xor(st, step);
+        sj = sj + __shfl_xor(sj, step);
+        sjj = sjj + __shfl_xor(sjj, step);
+    }
+    if (!fits)
+        return -1.0;
+    const double n = (double)(tw * th), a = st, b = sj, c = sjj;
+    const double var = c - b * b / n;
+    return (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : a / sqrt(norm * var);
+}
+
+// One workgroup per top cell: every half-pixel corner within around + 1 of twice its best corner (the inner
+// 2 around + 1 square is search_slot's refinement, track.c:1697-1711; the ring gives take_back's neighbours,
+// track.c:1737-1741), the window and the template staged in LDS, 16 corners at a time.
 template <typename T>
 __global__ __launch_bounds__(256) void ncc_refine(Level lv, const float *t, int tw, int th, const float *nm,
                                                   const vtg_cell *cells, int step, Aux *aux)
 {
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int around = step > 1 ? 2 * step - 1 : 1, side = 2 * around + 1, per = side * side;
-    const int total = aux->ntop * per;
-    const int first = (blockIdx.x * 4 + wave) * 4;
-    if (first >= total)
+    extern __shared__ float sm[];
+    const int c = blockIdx.x;
+    if (c >= aux->ntop)
         return;
-    const int p = first + (lane >> 4);
-    int hx = -64, hy = -64;
-    if (p < total) {
-        const int c = p / per, q = p - c * per;
-        const vtg_cell cl = cells[aux->tops[c]];
-        hy = 2 * cl.y - around + q / side;
-        hx = 2 * cl.x - around + q % side;
-    }
-    const double v = score16<T>(lv, t, tw, th, (double)nm[0], hx, hy, true, lane);
-    if (p < total && (lane & 15) == 0)
-        aux->hs[p] = v;
-}
-
-// The pick, the second peak clear of it by half a template, and the parabola through the peak's half-pixel
-// neighbours (search_slot, track.c:1704-1725; take_back's offsets, track.c:1737-1741; peak_offset, 1650-1657).
-template <typename T>
-__global__ __launch_bounds__(64) void ncc_final(Level lv, const float *t, int tw, int th, const float *nm,
-                                                const vtg_cell *cells, int ncell, int step, const Aux *aux,
-                                                vtg_found *found)
-{
-    __shared__ int fh[2];
-    __shared__ double fn_s;
-    const int lane = threadIdx.x;
-    const int ntop = aux->ntop;
-    const int around = step > 1 ? 2 * step - 1 : 1, side = 2 * around + 1, per = side * side;
-    if (lane == 0) {
-        double fn = -2.0, sec = -2.0;
-        int fhx = 0, fhy = 0;
-        double sc[8];
-        int shx[8], shy[8];
-        for (int c = 0; c < ntop; c++) {
-            const vtg_cell cl = cells[aux->tops[c]];
-            sc[c] = -2.0;
-            shx[c] = 2 * cl.x;
-            shy[c] = 2 * cl.y;
-            for (int q = 0; q < per; q++) {
-                const double v = aux->hs[c * per + q];
-                if (v > sc[c])
-                    sc[c] = v, shx[c] = 2 * cl.x - around + q % side, shy[c] = 2 * cl.y - around + q / side;
-            }
-            if (sc[c] > fn)
-                fn = sc[c], fhx = shx[c], fhy = shy[c];
+    const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 4;
+    const int around = step > 1 ? 2 * step - 1 : 1, R = around + 1, side = 2 * R + 1;
+    const vtg_cell cl = cells[aux->tops[c]];
+    const int cx = 2 * cl.x, cy = 2 * cl.y;
+    const int rx = (cx - R) >> 1, ry = (cy - R) >> 1;
+    const int rw = ((cx + R) >> 1) - rx + tw + 2, rh = ((cy + R) >> 1) - ry + th + 2;
+    float *tl = sm, *reg = sm + tw * th;
+    const T *base = (const T *)lv.p;
+    {
+        // every load of the staging issued before the first store: the template (<= 16 per thread) and the region
+        // (<= 24 per thread)
+        float tv[16], rv[24];
+        const int nt = tw * th, nr = rw * rh;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int i = tid + 256 * k;
+            tv[k] = t[i < nt ? i : nt - 1];
         }
-        for (int c = 0; c < ntop; c++) {
-            const int dx = shx[c] - fhx, dy = shy[c] - fhy;
-            if ((dx * dx > tw * tw || dy * dy > th * th) && sc[c] > sec)
-                sec = sc[c];
+#pragma unroll
+        for (int k = 0; k < 24; k++) {
+            const int i = tid + 256 * k, ic = i < nr ? i : nr - 1;
+            const int r = ic / rw, kk = ic - r * rw;
+            const int y = ry + r < 0 ? 0 : (ry + r >= lv.h ? lv.h - 1 : ry + r);
+            const int x = rx + kk < 0 ? 0 : (rx + kk >= lv.w ? lv.w - 1 : rx + kk);
+            rv[k] = Q6v<T>::get(base[(size_t)y * lv.pitch + x]);
         }
-        found->hx = fhx;
-        found->hy = fhy;
-        found->ncc = fn;
-        found->second = sec;
-        found->cells = ncell;
-        if (ntop > 0) {
-            const vtg_cell b = cells[aux->tops[0]];
-            found->best_x = b.x;
-            found->best_y = b.y;
-            found->best_v = b.v;
-        } else {
-            found->best_x = found->best_y = 0;
-            found->best_v = -2.0f;
-        }
-        fh[0] = fhx;
-        fh[1] = fhy;
-        fn_s = fn;
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (tid + 256 * k < nt)
+                tl[tid + 256 * k] = tv[k];
+#pragma unroll
+        for (int k = 0; k < 24; k++)
+            if (tid + 256 * k < nr)
+                reg[tid + 256 * k] = rv[k];
     }
     __syncthreads();
-    const int g = lane >> 4;
-    const int hx = fh[0] + (g == 0 ? -1 : g == 1 ? 1 : 0), hy = fh[1] + (g == 2 ? -1 : g == 3 ? 1 : 0);
-    const double v = score16<T>(lv, t, tw, th, (double)nm[0], ntop > 0 ? hx : -64, hy, true, lane);
-    const double l = __shfl(v, 0), r = __shfl(v, 16), u = __shfl(v, 32), d = __shfl(v, 48);
-    if (lane == 0) {
-        auto peak = [](double a, double c, double b) {
-            const double den = a - 2.0 * c + b;
-            if (den >= 0.0)
-                return 0.0;
-            const double o = 0.5 * (a - b) / den;
-            return o < -0.5 ? -0.5 : o > 0.5 ? 0.5 : o;
-        };
-        found->ox = ntop > 0 ? peak(l, fn_s, r) : 0.0;
-        found->oy = ntop > 0 ? peak(u, fn_s, d) : 0.0;
+    const double norm = (double)nm[0];
+    for (int q0 = 0; q0 < side * side; q0 += 16) {
+        const int q = q0 + grp;
+        const int hy = cy - R + q / side, hx = cx - R + q % side;
+        const double v = score16_lds(reg, rx, ry, rw, tl, tw, th, norm, lv.w, lv.h, q < side * side ? hx : -64,
+                                     hy, lane);
+        if (q < side * side && (lane & 15) == 0)
+            aux->hs[c][q] = v;
     }
+}
+
+// The pick, the second peak clear of it by half a template (search_slot, track.c:1704-1725) and the parabola offsets
+// from the pick's ring (take_back, track.c:1737-1741; peak_offset, 1650-1657). Selection only: no scoring here.
+__global__ __launch_bounds__(64) void ncc_final(const vtg_cell *cells, int ncell, int tw, int th, int step,
+                                                const Aux *aux_g, vtg_found *found)
+{
+    __shared__ Aux ax;
+    __shared__ vtg_cell tc[8];
+    {
+        // the record into LDS with every lane, then one lane decides
+        const int nwords = (int)(sizeof(Aux) / 4);
+        const unsigned int *src = (const unsigned int *)aux_g;
+        unsigned int *dst = (unsigned int *)&ax;
+        for (int i = threadIdx.x; i < nwords; i += 64)
+            dst[i] = src[i];
+        __syncthreads();
+        if (threadIdx.x < 8 && (int)threadIdx.x < ax.ntop)
+            tc[threadIdx.x] = cells[ax.tops[threadIdx.x]];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0)
+        return;
+    const Aux *aux = &ax;
+    const int ntop = aux->ntop;
+    const int around = step > 1 ? 2 * step - 1 : 1, R = around + 1, side = 2 * R + 1;
+    double fn = -2.0, sec = -2.0;
+    int fhx = 0, fhy = 0, fc = -1;
+    double sc[8];
+    int shx[8], shy[8];
+    for (int c = 0; c < ntop; c++) {
+        const vtg_cell cl = tc[c];
+        const int cx = 2 * cl.x, cy = 2 * cl.y;
+        sc[c] = -2.0;
+        shx[c] = cx;
+        shy[c] = cy;
+        for (int hy = cy - around; hy <= cy + around; hy++)
+            for (int hx = cx - around; hx <= cx + around; hx++) {
+                const double v = aux->hs[c][(hy - cy + R) * side + hx - cx + R];
+                if (v > sc[c])
+                    sc[c] = v, shx[c] = hx, shy[c] = hy;
+            }
+        if (sc[c] > fn)
+            fn = sc[c], fhx = shx[c], fhy = shy[c], fc = c;
+    }
+    for (int c = 0; c < ntop; c++) {
+        const int dx = shx[c] - fhx, dy = shy[c] - fhy;
+        if ((dx * dx > tw * tw || dy * dy > th * th) && sc[c] > sec)
+            sec = sc[c];
+    }
+    auto peak = [](double a, double m, double b) {
+        const double den = a - 2.0 * m + b;
+        if (den >= 0.0)
+            return 0.0;
+        const double o = 0.5 * (a - b) / den;
+        return o < -0.5 ? -0.5 : o > 0.5 ? 0.5 : o;
+    };
+    vtg_found f;
+    f.hx = fhx;
+    f.hy = fhy;
+    f.ncc = fn;
+    f.second = sec;
+    f.ox = f.oy = 0.0;
+    f.cells = ncell;
+    f.best_x = f.best_y = 0;
+    f.best_v = -2.0f;
+    if (fc >= 0) {
+        const vtg_cell cl = tc[fc];
+        const int ix = fhx - 2 * cl.x + R, iy = fhy - 2 * cl.y + R;
+        const double *g = aux->hs[fc];
+        f.ox = peak(g[iy * side + ix - 1], fn, g[iy * side + ix + 1]);
+        f.oy = peak(g[(iy - 1) * side + ix], fn, g[(iy + 1) * side + ix]);
+        const vtg_cell b = tc[0];
+        f.best_x = b.x;
+        f.best_y = b.y;
+        f.best_v = b.v;
+    }
+    *found = f;
 }
 
 template <typename F>
@@ -676,24 +780,28 @@ int launch_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, i
             return rc;
     }
     timer_begin(c, "ncc_top8", &ev);
-    hipLaunchKernelGGL(ncc_top8, dim3(1), dim3(256), 0, c->stream, cells, ncell, aux);
+    hipLaunchKernelGGL(ncc_top8, dim3(1), dim3(1024), 0, c->stream, cells, ncell, aux);
     VTG_HIP(c, hipGetLastError());
     timer_end(c, "ncc_top8", ev);
-    const int around = step > 1 ? 2 * step - 1 : 1, per = (2 * around + 1) * (2 * around + 1);
-    return by_level_type(c, p, t->level, [&](auto *tag) -> int {
+    const int around = step > 1 ? 2 * step - 1 : 1, R = around + 1;
+    const int rmax = R + 1 + 2; // region extent beyond the template, whole pixels, each way
+    const size_t lds = ((size_t)t->tw * t->th + (size_t)(t->tw + rmax + 1) * (t->th + rmax + 1)) * sizeof(float);
+    int rc = by_level_type(c, p, t->level, [&](auto *tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         timer_begin(c, "ncc_refine", &ev);
-        hipLaunchKernelGGL(ncc_refine<T>, dim3((8 * per + 15) / 16), dim3(256), 0, c->stream, lv, t->t, t->tw, t->th,
-                           nm, cells, step, aux);
+        hipLaunchKernelGGL(ncc_refine<T>, dim3(8), dim3(256), lds, c->stream, lv, t->t, t->tw, t->th, nm, cells, step,
+                           aux);
         VTG_HIP(c, hipGetLastError());
         timer_end(c, "ncc_refine", ev);
-        timer_begin(c, "ncc_final", &ev);
-        hipLaunchKernelGGL(ncc_final<T>, dim3(1), dim3(64), 0, c->stream, lv, t->t, t->tw, t->th, nm, cells, ncell,
-                           step, aux, found);
-        VTG_HIP(c, hipGetLastError());
-        timer_end(c, "ncc_final", ev);
         return VTG_OK;
     });
+    if (rc)
+        return rc;
+    timer_begin(c, "ncc_final", &ev);
+    hipLaunchKernelGGL(ncc_final, dim3(1), dim3(64), 0, c->stream, cells, ncell, t->tw, t->th, step, aux, found);
+    VTG_HIP(c, hipGetLastError());
+    timer_end(c, "ncc_final", ev);
+    return VTG_OK;
 }
 
 }  // namespace vtg

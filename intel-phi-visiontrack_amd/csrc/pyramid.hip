@@ -8,6 +8,8 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 #include "vtg_internal.hpp"
 
 namespace vtg {
@@ -41,72 +43,88 @@ template <> struct Q6<__hip_bfloat16> {
     static __device__ __forceinline__ __hip_bfloat16 putf(float q) { return __float2bfloat16(q * (1.0f / 64.0f)); }
 };
 
-// Exact integer reduce: TS holds integer Q6 values (u8, u16, f32), TD an exact store (u16, f32).
-template <typename TS, typename TD, bool COPY>
-__global__ __launch_bounds__(256) void reduce_exact(const TS *__restrict__ src, int sw, int sh, int sp,
-                                                    TD *__restrict__ dst, int dw, int dh, int dp,
-                                                    TS *__restrict__ copy, int cp)
+// One reduce job: a level made from the level below it, possibly writing the frame into level 0 on the way.
+struct RJob {
+    const void *src;
+    void *dst, *copy;
+    int sw, sh, sp, dw, dh, dp, cp;
+    int tiles_x, t0; // tiles across the output, first linear tile of the job in the launch
+};
+constexpr int JMAX = 16;
+struct RJobs {
+    RJob j[JMAX];
+    int n;
+};
+
+template <typename TC, typename TS> struct CopyConv {
+    static __device__ __forceinline__ TC get(TS v) { return v; }
+};
+template <> struct CopyConv<float, uint8_t> {
+    static __device__ __forceinline__ float get(uint8_t v) { return (float)v * 64.0f; }
+};
+
+template <bool EXACT> struct AccT { using type = float; };
+template <> struct AccT<true> { using type = int; };
+
+// Reduce for a batch of jobs, one 64 x 16 output tile per workgroup. The 131 x 35 input tile (2-pixel halo,
+// coordinates clamped: the replicated border) is read with every load of a thread issued before the first is used,
+// then summed in LDS: 5 rows at each even row, then 5 columns at each even column. EXACT: integer sums and one
+// (h + 128) >> 8, bit-exact to the reference; else fp32 without rounding (fp16 / bf16 levels).
+template <typename TS, typename TD, typename TC, bool COPY, bool EXACT>
+__global__ __launch_bounds__(256) void reduce_batch(RJobs jobs)
 {
-    __shared__ int tin[IH][IW + 1];
-    __shared__ int tv[TH][IW + 1];
-    const int ox0 = blockIdx.x * TW, oy0 = blockIdx.y * TH;
+    using Acc = typename AccT<EXACT>::type;
+    __shared__ Acc tin[IH][IW + 1];
+    __shared__ Acc tv[TH][IW + 1];
+    int jn = 0;
+    for (int k = 1; k < jobs.n; k++)
+        if ((int)blockIdx.x >= jobs.j[k].t0)
+            jn = k;
+    const RJob &J = jobs.j[jn];
+    const int tile = blockIdx.x - J.t0, tyi = tile / J.tiles_x, txi = tile - tyi * J.tiles_x;
+    const int ox0 = txi * TW, oy0 = tyi * TH;
     const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
-    for (int i = threadIdx.x; i < IH * IW; i += 256) {
-        const int r = i / IW, c = i - r * IW;
-        const int yy = iy0 + r, xx = ix0 + c;
-        const TS v = src[(size_t)clampi(yy, 0, sh - 1) * sp + clampi(xx, 0, sw - 1)];
-        tin[r][c] = (int)Q6<TS>::get(v);
-        if (COPY && r >= 2 && r < 2 + 2 * TH && c >= 2 && c < 2 + 2 * TW && yy < sh && xx < sw)
-            copy[(size_t)yy * cp + xx] = v;
+    const TS *src = (const TS *)J.src;
+    constexpr int NIN = IH * IW, NLD = (NIN + 255) / 256;
+    TS raw[NLD];
+#pragma unroll
+    for (int k = 0; k < NLD; k++) {
+        const int i = threadIdx.x + 256 * k, ic = i < NIN ? i : NIN - 1;
+        const int r = ic / IW, cc = ic - r * IW;
+        raw[k] = src[(size_t)clampi(iy0 + r, 0, J.sh - 1) * J.sp + clampi(ix0 + cc, 0, J.sw - 1)];
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < TH * IW; i += 256) {
-        const int j = i / IW, c = i - j * IW, r = 2 * j;
-        tv[j][c] = tin[r][c] + tin[r + 4][c] + 4 * (tin[r + 1][c] + tin[r + 3][c]) + 6 * tin[r + 2][c];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < TH * TW; i += 256) {
-        const int j = i / TW, k = i - j * TW;
-        const int ox = ox0 + k, oy = oy0 + j;
-        if (ox < dw && oy < dh) {
-            const int c = 2 * k;
-            const int h = tv[j][c] + tv[j][c + 4] + 4 * (tv[j][c + 1] + tv[j][c + 3]) + 6 * tv[j][c + 2];
-            dst[(size_t)oy * dp + ox] = Q6<TD>::put((h + 128) >> 8);
+#pragma unroll
+    for (int k = 0; k < NLD; k++) {
+        const int i = threadIdx.x + 256 * k;
+        if (i < NIN) {
+            const int r = i / IW, cc = i - r * IW;
+            tin[r][cc] = (Acc)Q6<TS>::get(raw[k]);
+            if (COPY) {
+                const int yy = iy0 + r, xx = ix0 + cc;
+                if (r >= 2 && r < 2 + 2 * TH && cc >= 2 && cc < 2 + 2 * TW && yy < J.sh && xx < J.sw)
+                    ((TC *)J.copy)[(size_t)yy * J.cp + xx] = CopyConv<TC, TS>::get(raw[k]);
+            }
         }
     }
-}
-
-// fp16 / bf16 levels: the same filter in fp32 without the integer rounding, stored as grey.
-template <typename TS, typename TD, bool COPY>
-__global__ __launch_bounds__(256) void reduce_float(const TS *__restrict__ src, int sw, int sh, int sp,
-                                                    TD *__restrict__ dst, int dw, int dh, int dp,
-                                                    TS *__restrict__ copy, int cp)
-{
-    __shared__ float tin[IH][IW + 1];
-    __shared__ float tv[TH][IW + 1];
-    const int ox0 = blockIdx.x * TW, oy0 = blockIdx.y * TH;
-    const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
-    for (int i = threadIdx.x; i < IH * IW; i += 256) {
-        const int r = i / IW, c = i - r * IW;
-        const int yy = iy0 + r, xx = ix0 + c;
-        const TS v = src[(size_t)clampi(yy, 0, sh - 1) * sp + clampi(xx, 0, sw - 1)];
-        tin[r][c] = Q6<TS>::get(v);
-        if (COPY && r >= 2 && r < 2 + 2 * TH && c >= 2 && c < 2 + 2 * TW && yy < sh && xx < sw)
-            copy[(size_t)yy * cp + xx] = v;
-    }
     __syncthreads();
     for (int i = threadIdx.x; i < TH * IW; i += 256) {
-        const int j = i / IW, c = i - j * IW, r = 2 * j;
-        tv[j][c] = tin[r][c] + tin[r + 4][c] + 4.0f * (tin[r + 1][c] + tin[r + 3][c]) + 6.0f * tin[r + 2][c];
+        const int j = i / IW, cc = i - j * IW, r = 2 * j;
+        tv[j][cc] = tin[r][cc] + tin[r + 4][cc] + (Acc)4 * (tin[r + 1][cc] + tin[r + 3][cc]) + (Acc)6 * tin[r + 2][cc];
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < TH * TW; i += 256) {
-        const int j = i / TW, k = i - j * TW;
-        const int ox = ox0 + k, oy = oy0 + j;
-        if (ox < dw && oy < dh) {
-            const int c = 2 * k;
-            const float h = tv[j][c] + tv[j][c + 4] + 4.0f * (tv[j][c + 1] + tv[j][c + 3]) + 6.0f * tv[j][c + 2];
-            dst[(size_t)oy * dp + ox] = Q6<TD>::putf(h * (1.0f / 256.0f));
+    TD *dst = (TD *)J.dst;
+#pragma unroll
+    for (int k = 0; k < TH * TW / 256; k++) {
+        const int i = threadIdx.x + 256 * k;
+        const int j = i / TW, kk = i - j * TW;
+        const int ox = ox0 + kk, oy = oy0 + j;
+        if (ox < J.dw && oy < J.dh) {
+            const int cc = 2 * kk;
+            const Acc h = tv[j][cc] + tv[j][cc + 4] + (Acc)4 * (tv[j][cc + 1] + tv[j][cc + 3]) + (Acc)6 * tv[j][cc + 2];
+            if constexpr (EXACT)
+                dst[(size_t)oy * J.dp + ox] = Q6<TD>::put((h + 128) >> 8);
+            else
+                dst[(size_t)oy * J.dp + ox] = Q6<TD>::putf(h * (1.0f / 256.0f));
         }
     }
 }
@@ -144,22 +162,58 @@ __global__ __launch_bounds__(256) void from_plane(const float *__restrict__ src,
 
 size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-template <typename TS, typename TD, bool COPY, bool EXACT>
-int launch_reduce(vtg_ctx *c, const TS *src, int sw, int sh, int sp, TD *dst, int dw, int dh, int dp, TS *copy,
-                  int cp, const char *name)
+template <typename TS, typename TD, typename TC, bool COPY, bool EXACT>
+int launch_jobs(vtg_ctx *c, std::vector<RJob> &all)
 {
-    const dim3 grid((dw + TW - 1) / TW, (dh + TH - 1) / TH);
-    hipEvent_t ev;
-    timer_begin(c, name, &ev);
-    if constexpr (EXACT)
-        hipLaunchKernelGGL((reduce_exact<TS, TD, COPY>), grid, dim3(256), 0, c->stream, src, sw, sh, sp, dst, dw, dh,
-                           dp, copy, cp);
-    else
-        hipLaunchKernelGGL((reduce_float<TS, TD, COPY>), grid, dim3(256), 0, c->stream, src, sw, sh, sp, dst, dw, dh,
-                           dp, copy, cp);
-    VTG_HIP(c, hipGetLastError());
-    timer_end(c, name, ev);
+    for (size_t b = 0; b < all.size(); b += JMAX) {
+        RJobs js{};
+        int tiles = 0;
+        js.n = (int)std::min<size_t>(JMAX, all.size() - b);
+        for (int k = 0; k < js.n; k++) {
+            RJob j = all[b + k];
+            j.tiles_x = (j.dw + TW - 1) / TW;
+            j.t0 = tiles;
+            tiles += j.tiles_x * ((j.dh + TH - 1) / TH);
+            js.j[k] = j;
+        }
+        hipEvent_t ev;
+        timer_begin(c, "pyr_reduce", &ev);
+        hipLaunchKernelGGL((reduce_batch<TS, TD, TC, COPY, EXACT>), dim3(tiles), dim3(256), 0, c->stream, js);
+        VTG_HIP(c, hipGetLastError());
+        timer_end(c, "pyr_reduce", ev);
+    }
     return VTG_OK;
+}
+
+// Level 1 of every pyramid from its frame (writing level 0 on the way), then each higher level from the one below,
+// one launch per level for all pyramids together.
+template <typename T1, typename TC, bool EXACT>
+int build_levels(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch)
+{
+    int maxl = 0;
+    for (int i = 0; i < n; i++)
+        maxl = std::max(maxl, ps[i]->levels);
+    std::vector<RJob> jobs;
+    for (int i = 0; i < n; i++) {
+        const Level *L = ps[i]->lv;
+        if (ps[i]->levels < 2)
+            continue;
+        jobs.push_back(RJob{frames[i], (void *)L[1].p, (void *)L[0].p, ps[i]->w, ps[i]->h, pitch[i], L[1].w, L[1].h,
+                            L[1].pitch, L[0].pitch, 0, 0});
+    }
+    int rc = jobs.empty() ? VTG_OK : launch_jobs<uint8_t, T1, TC, true, EXACT>(c, jobs);
+    for (int l = 2; l < maxl && !rc; l++) {
+        jobs.clear();
+        for (int i = 0; i < n; i++) {
+            if (ps[i]->levels <= l)
+                continue;
+            const Level *L = ps[i]->lv;
+            jobs.push_back(RJob{L[l - 1].p, (void *)L[l].p, nullptr, L[l - 1].w, L[l - 1].h, L[l - 1].pitch, L[l].w,
+                                L[l].h, L[l].pitch, 0, 0, 0});
+        }
+        rc = launch_jobs<T1, T1, T1, false, EXACT>(c, jobs);
+    }
+    return rc;
 }
 
 }  // namespace
@@ -174,64 +228,35 @@ PyrView view(const vtg_pyr *p)
     return v;
 }
 
-// frame_dev: the u8 frame on the device, `pitch` bytes per row.
+// frames[i]: pyramid i's u8 frame on the device, pitch[i] bytes per row. All pyramids of one storage.
+int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch)
+{
+    // single-level pyramids: level 0 is the frame (converted for F32)
+    for (int i = 0; i < n; i++) {
+        if (ps[i]->levels != 1)
+            continue;
+        const Level &L0 = ps[i]->lv[0];
+        if (ps[i]->storage == VTG_STORE_F32) {
+            hipLaunchKernelGGL(u8_to_f32, dim3((ps[i]->w + 255) / 256, ps[i]->h), dim3(256), 0, c->stream, frames[i],
+                               pitch[i], (float *)L0.p, L0.pitch, ps[i]->w, ps[i]->h);
+            VTG_HIP(c, hipGetLastError());
+        } else {
+            VTG_HIP(c, hipMemcpy2DAsync((void *)L0.p, L0.pitch, frames[i], pitch[i], ps[i]->w, ps[i]->h,
+                                        hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
+    switch (ps[0]->storage) {
+    case VTG_STORE_Q6: return build_levels<uint16_t, uint8_t, true>(c, ps, n, frames, pitch);
+    case VTG_STORE_F32: return build_levels<float, float, true>(c, ps, n, frames, pitch);
+    case VTG_STORE_F16: return build_levels<__half, uint8_t, false>(c, ps, n, frames, pitch);
+    case VTG_STORE_BF16: return build_levels<__hip_bfloat16, uint8_t, false>(c, ps, n, frames, pitch);
+    }
+    return fail(c, VTG_EINVAL, "pyramid storage %d", ps[0]->storage);
+}
+
 int launch_pyr_build(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame, int pitch)
 {
-    const Level *L = p->lv;
-    if (p->storage == VTG_STORE_F32) {
-        float *l0 = (float *)L[0].p;
-        hipEvent_t ev;
-        timer_begin(c, "pyr_u8_to_f32", &ev);
-        hipLaunchKernelGGL(u8_to_f32, dim3((p->w + 255) / 256, p->h), dim3(256), 0, c->stream, frame, pitch, l0,
-                           L[0].pitch, p->w, p->h);
-        VTG_HIP(c, hipGetLastError());
-        timer_end(c, "pyr_u8_to_f32", ev);
-        for (int l = 1; l < p->levels; l++) {
-            int rc = launch_reduce<float, float, false, true>(c, (const float *)L[l - 1].p, L[l - 1].w, L[l - 1].h,
-                                                              L[l - 1].pitch, (float *)L[l].p, L[l].w, L[l].h,
-                                                              L[l].pitch, nullptr, 0, "pyr_reduce");
-            if (rc)
-                return rc;
-        }
-        return VTG_OK;
-    }
-    if (p->levels == 1) {
-        VTG_HIP(c, hipMemcpy2DAsync((void *)L[0].p, L[0].pitch, frame, pitch, p->w, p->h, hipMemcpyDeviceToDevice,
-                                    c->stream));
-        return VTG_OK;
-    }
-    uint8_t *l0 = (uint8_t *)L[0].p;
-    int rc = VTG_OK;
-    switch (p->storage) {
-    case VTG_STORE_Q6:
-        rc = launch_reduce<uint8_t, uint16_t, true, true>(c, frame, p->w, p->h, pitch, (uint16_t *)L[1].p, L[1].w,
-                                                          L[1].h, L[1].pitch, l0, L[0].pitch, "pyr_reduce");
-        for (int l = 2; l < p->levels && !rc; l++)
-            rc = launch_reduce<uint16_t, uint16_t, false, true>(c, (const uint16_t *)L[l - 1].p, L[l - 1].w,
-                                                                L[l - 1].h, L[l - 1].pitch, (uint16_t *)L[l].p,
-                                                                L[l].w, L[l].h, L[l].pitch, nullptr, 0, "pyr_reduce");
-        break;
-    case VTG_STORE_F16:
-        rc = launch_reduce<uint8_t, __half, true, false>(c, frame, p->w, p->h, pitch, (__half *)L[1].p, L[1].w, L[1].h,
-                                                         L[1].pitch, l0, L[0].pitch, "pyr_reduce");
-        for (int l = 2; l < p->levels && !rc; l++)
-            rc = launch_reduce<__half, __half, false, false>(c, (const __half *)L[l - 1].p, L[l - 1].w, L[l - 1].h,
-                                                             L[l - 1].pitch, (__half *)L[l].p, L[l].w, L[l].h,
-                                                             L[l].pitch, nullptr, 0, "pyr_reduce");
-        break;
-    case VTG_STORE_BF16:
-        rc = launch_reduce<uint8_t, __hip_bfloat16, true, false>(c, frame, p->w, p->h, pitch,
-                                                                 (__hip_bfloat16 *)L[1].p, L[1].w, L[1].h, L[1].pitch,
-                                                                 l0, L[0].pitch, "pyr_reduce");
-        for (int l = 2; l < p->levels && !rc; l++)
-            rc = launch_reduce<__hip_bfloat16, __hip_bfloat16, false, false>(
-                c, (const __hip_bfloat16 *)L[l - 1].p, L[l - 1].w, L[l - 1].h, L[l - 1].pitch,
-                (__hip_bfloat16 *)L[l].p, L[l].w, L[l].h, L[l].pitch, nullptr, 0, "pyr_reduce");
-        break;
-    default:
-        return fail(c, VTG_EINVAL, "pyramid storage %d", p->storage);
-    }
-    return rc;
+    return launch_pyr_build_many(c, &p, 1, &frame, &pitch);
 }
 
 int pyr_from_planes(vtg_ctx *c, const vt_plane *levels, int n, vtg_pyr **out)
@@ -323,6 +348,8 @@ int vtg_pyr_create(vtg_ctx *c, int w, int h, int levels, int storage, vtg_pyr **
         lw = (lw + 1) / 2;
         lh = (lh + 1) / 2;
     }
+    const size_t view_off = off;
+    off += round_up(sizeof(PyrView), 256);
     p->bytes = off;
     hipError_t e = hipMalloc(&p->block, off);
     if (e != hipSuccess) {
@@ -331,6 +358,15 @@ int vtg_pyr_create(vtg_ctx *c, int w, int h, int levels, int storage, vtg_pyr **
     }
     for (int l = 0; l < levels; l++)
         p->lv[l].p = (char *)p->block + p->offset[l];
+    // the descriptor batched kernels read, written once
+    p->dview = (PyrView *)((char *)p->block + view_off);
+    const PyrView v = view(p);
+    e = hipMemcpy(p->dview, &v, sizeof v, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(p->block);
+        delete p;
+        return hip_fail(c, e, "vtg_pyr_create: descriptor");
+    }
     *out = p;
     return VTG_OK;
 }
@@ -366,6 +402,48 @@ int vtg_pyr_build(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame, int pitch, int o
         pitch = (int)dp;
     }
     return launch_pyr_build(c, p, frame, pitch);
+}
+
+int vtg_pyr_build_many(vtg_ctx *c, int n, vtg_pyr *const *pyrs, const uint8_t *const *frames, const int *pitches,
+                       int on_device)
+{
+    if (!c || n < 0 || (n > 0 && (!pyrs || !frames || !pitches)))
+        return fail(c, VTG_EINVAL, "vtg_pyr_build_many: bad argument");
+    if (n == 0)
+        return VTG_OK;
+    for (int i = 0; i < n; i++)
+        if (!pyrs[i] || !frames[i] || pitches[i] < pyrs[i]->w || pyrs[i]->storage != pyrs[0]->storage)
+            return fail(c, VTG_EINVAL, "vtg_pyr_build_many: pyramid %d: bad frame, pitch or storage", i);
+    if (on_device)
+        return launch_pyr_build_many(c, pyrs, n, frames, pitches);
+    // host frames: into level 0 where it is u8, else into scratch, then the same batch
+    std::vector<const uint8_t *> dev(n);
+    std::vector<int> dp(n);
+    size_t need = 0;
+    for (int i = 0; i < n; i++)
+        if (pyrs[i]->storage == VTG_STORE_F32)
+            need += round_up((size_t)pyrs[i]->w * pyrs[i]->h, 256);
+    void *tmp = nullptr;
+    if (need) {
+        int rc = scratch(c, need, &tmp);
+        if (rc)
+            return rc;
+    }
+    size_t off = 0;
+    for (int i = 0; i < n; i++) {
+        vtg_pyr *p = pyrs[i];
+        if (p->storage == VTG_STORE_F32) {
+            dev[i] = (const uint8_t *)tmp + off;
+            dp[i] = p->w;
+            off += round_up((size_t)p->w * p->h, 256);
+        } else {
+            dev[i] = (const uint8_t *)p->lv[0].p;
+            dp[i] = p->lv[0].pitch;
+        }
+        VTG_HIP(c, hipMemcpy2DAsync((void *)dev[i], dp[i], frames[i], pitches[i], p->w, p->h, hipMemcpyHostToDevice,
+                                    c->stream));
+    }
+    return launch_pyr_build_many(c, pyrs, n, dev.data(), dp.data());
 }
 
 int vtg_pyr_level(const vtg_pyr *p, int level, void **dptr, int *w, int *h, size_t *pitch_bytes, int *elem)
@@ -487,9 +565,11 @@ int vtg_img_reduce(vtg_ctx *c, const struct vt_plane *dst, const struct vt_plane
             e = hipGetLastError();
         }
         if (e == hipSuccess)
-            rc = launch_reduce<float, float, false, true>(c, (const float *)L[0].p, L[0].w, L[0].h, L[0].pitch,
-                                                          (float *)L[1].p, L[1].w, L[1].h, L[1].pitch, nullptr, 0,
-                                                          "pyr_reduce");
+        {
+            std::vector<RJob> jobs{RJob{L[0].p, (void *)L[1].p, nullptr, L[0].w, L[0].h, L[0].pitch, L[1].w, L[1].h,
+                                        L[1].pitch, 0, 0, 0}};
+            rc = launch_jobs<float, float, float, false, true>(c, jobs);
+        }
         if (e == hipSuccess && !rc) {
             const dim3 grid((dst->stride + 255) / 256, dst->h + 2 * VT_PAD_Y);
             hipLaunchKernelGGL(to_plane<float>, grid, dim3(256), 0, c->stream, (const float *)L[1].p, L[1].pitch,

@@ -51,6 +51,7 @@ struct vtg_ctx {
     size_t scratch_bytes = 0;
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;
+    int lk_margin = 3;                       // pixels of J staged around an LK estimate
 };
 
 struct vtg_pyr {
@@ -60,6 +61,7 @@ struct vtg_pyr {
     vtg::Level lv[vtg::kMaxLevels];
     size_t offset[vtg::kMaxLevels];
     int elem[vtg::kMaxLevels];
+    vtg::PyrView *dview = nullptr;           // the levels' descriptor in device memory (read by batched kernels)
 };
 
 struct vtg_tmpl {
@@ -99,8 +101,10 @@ int pyr_from_planes(vtg_ctx *c, const vt_plane *levels, int n, vtg_pyr **out);
 
 // kernels (one per translation unit)
 int launch_pyr_build(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame_dev, int pitch);
-int launch_lk(vtg_ctx *c, const PyrView &I, const PyrView &J, int top, const lk_cfg *cfg, int n, const float *xy,
-              const float *guess, float fb_max, int solve_mode, lk_out *out);
+int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch);
+int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
+                   const int *off, const lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
+                   int solve_mode, lk_out *out);
 int launch_ncc_template(vtg_ctx *c, const vtg_pyr *p, int level, int x0, int y0, vtg_tmpl *t);
 size_t search_scratch_bytes(int ncell);
 int launch_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,

@@ -64,11 +64,12 @@ def lib():
         "vtg_timing_enable": [V, I], "vtg_timing_read": [V, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "vtg_timing_reset": [V],
         "vtg_pyr_create": [V, I, I, I, I, C.POINTER(V)], "vtg_pyr_destroy": [V, V],
-        "vtg_pyr_build": [V, V, V, I, I],
+        "vtg_pyr_build": [V, V, V, I, I], "vtg_pyr_build_many": [V, I, V, V, V, I],
         "vtg_pyr_level": [V, I, C.POINTER(V), C.POINTER(I), C.POINTER(I), C.POINTER(S), C.POINTER(I)],
         "vtg_pyr_read_level": [V, V, I, C.POINTER(Plane)],
         "vtg_lk_track": [V, V, V, I, C.POINTER(LkCfg), I, V, V, F, V],
         "vtg_lk_track_ex": [V, V, V, I, C.POINTER(LkCfg), I, V, V, F, I, V],
+        "vtg_lk_track_many": [V, I, V, V, V, V, C.POINTER(LkCfg), V, V, F, I, V], "vtg_lk_set_margin": [V, I],
         "vtg_ncc_template": [V, V, I, I, I, I, I, C.POINTER(V)], "vtg_ncc_template_destroy": [V, V],
         "vtg_ncc_template_update": [V, V, V, I, I, I],
         "vtg_ncc_template_read": [V, V, V, C.POINTER(F)],
@@ -94,8 +95,8 @@ def lib():
 # every symbol include/vtgpu.h declares
 EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_error", "vtg_ctx_stream", "vtg_alloc",
            "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_timing_enable", "vtg_timing_read",
-           "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_level",
-           "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_ncc_template", "vtg_ncc_template_destroy",
+           "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_level",
+           "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_margin", "vtg_ncc_template", "vtg_ncc_template_destroy",
            "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_search", "vtg_ncc_score", "vtg_img_u8_to_q6", "vtg_img_reduce",
            "vtg_lk_track_planes", "vtg_correlate_planes"]
 
@@ -238,6 +239,14 @@ class Context:
     def pyramid(self, w, h, levels, storage=STORE_Q6):
         return Pyramid(self, w, h, levels, storage)
 
+    def build_many(self, pyrs, frames):
+        """Build several pyramids in one launch per level: frames are DevBufs (pitch = width)."""
+        n = len(pyrs)
+        P = (C.c_void_p * n)(*[p.h for p in pyrs])
+        F = (C.c_void_p * n)(*[f.ptr for f in frames])
+        pitch = (C.c_int * n)(*[p.w for p in pyrs])
+        self.check(self.L.vtg_pyr_build_many(self.h, n, P, F, pitch, 1), "pyr_build_many")
+
     # LK
     def lk_track(self, I, J, top, cfg, xy, guess=None, fb_max=0.0, out=None, solve=SOLVE_F64):
         """xy, guess, out: DevBuf. Returns out."""
@@ -247,6 +256,20 @@ class Context:
         self.check(self.L.vtg_lk_track_ex(self.h, I.h, J.h, top, C.byref(cfg), n, xy.ptr,
                                           guess.ptr if guess is not None else None, fb_max, solve, out.ptr),
                    "lk_track")
+        return out
+
+    def lk_track_many(self, Is, Js, tops, offsets, cfg, xy, guess=None, fb_max=0.0, out=None, solve=SOLVE_F64):
+        """Points of several pairs in one launch: offsets (len = pairs + 1) slice the DevBuf xy."""
+        n = len(Is)
+        if out is None:
+            out = self.alloc(32 * max(1, offsets[-1]))
+        PI = (C.c_void_p * n)(*[p.h for p in Is])
+        PJ = (C.c_void_p * n)(*[p.h for p in Js])
+        T = (C.c_int * n)(*tops)
+        O = (C.c_int * (n + 1))(*offsets)
+        self.check(self.L.vtg_lk_track_many(self.h, n, PI, PJ, T, O, C.byref(cfg), xy.ptr,
+                                            guess.ptr if guess is not None else None, fb_max, solve, out.ptr),
+                   "lk_track_many")
         return out
 
     def lk_track_host(self, I, J, top, cfg, xs, ys, gx=None, gy=None, fb_max=0.0, solve=SOLVE_F64):

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import vtgpu
-from gpu_util import flow_diff, say
+from gpu_util import conditioning, flow_diff, say
 from oracle_ffi import lk_cfg
 
 pytestmark = pytest.mark.gpu
@@ -22,23 +22,18 @@ def _gpu_pyr(ctx, img, w, h, levels, storage=vtgpu.STORE_Q6):
     return p
 
 
-def _compare(gpu, ref, f64=None, tol=0.01, max_unstable=0.03, label=""):
-    """Identical status and flow within tol of the reference tier on every rounding-stable point. A point is
-    rounding-stable when the oracle's two arithmetics (reference fp32 lanes + x87 driver, and all-fp64) give the
-    same status and agree within tol / 2: on the others the tracker's own result depends on rounding (a window
-    barely above min_eig, an iteration that wanders to max_iter), and they are counted, not compared."""
-    if f64 is None:
-        stable = np.ones(len(ref), bool)
-    else:
-        stable = (ref["status"] == f64["status"]) & ((ref["status"] != 0) | (flow_diff(ref, f64) <= tol / 2))
+def _compare(gpu, ref, well, tol=0.01, max_ill=0.03, label=""):
+    """Identical status and flow within tol of the reference tier on every well-conditioned point (see
+    gpu_util.conditioning); the ill-conditioned ones are counted and their agreement reported."""
     same = gpu["status"] == ref["status"]
     d = flow_diff(gpu, ref)
-    ok = (ref["status"] == 0) & stable
-    print(f"[lk] {label}: {len(ref)} pts, rounding-unstable {(~stable).sum()}, status equal (stable) "
-          f"{same[stable].mean():.4f} (all {same.mean():.4f}), max |d| stable OK pts "
-          f"{d[ok].max() if ok.any() else 0:.2e}, iters equal {(gpu['iters'] == ref['iters']).mean():.3f}")
-    assert (~stable).mean() <= max_unstable
-    assert same[stable].all(), np.nonzero(~same & stable)[0][:10]
+    ok = (ref["status"] == 0) & well
+    ill = ~well
+    print(f"[lk] {label}: {len(ref)} pts, ill-conditioned {ill.sum()}, status equal: well {same[well].mean():.4f} "
+          f"all {same.mean():.4f}; max |d| well OK {d[ok].max() if ok.any() else 0:.2e}, share of all converged "
+          f"within tol {(d[(ref['status'] == 0) & same] <= tol).mean():.4f}")
+    assert ill.mean() <= max_ill
+    assert same[well].all(), np.nonzero(~same & well)[0][:10]
     assert (d[ok] <= tol).all(), d[ok].max()
 
 
@@ -59,28 +54,24 @@ def test_m2_cases(ctx, oracle):
             i1 = oracle.m2_render(w, h, 2048 - case["dx"], 2048 - case["dy"])[:, :w]
             kJ, J = oracle.pyramid(i1, w, h, L)
             ref = oracle.lk_batch(LANE, I, J, L - 1, cfg, xs, ys)
-            f64 = oracle.lk_batch(F64, I, J, L - 1, cfg, xs, ys)
+            well, _ = conditioning(oracle, I, J, L - 1, cfg, xs, ys)
             say("lk m2", case["dx"], case["dy"], "storage", storage)
             PJ = _gpu_pyr(ctx, i1, w, h, L, storage)
             got = ctx.lk_track_host(PI, PJ, L - 1, cfg, xs, ys)
-            _compare(got, ref, f64, max_unstable=0.1 if abs(case["dx"]) > 60 else 0.01,
+            _compare(got, ref, well, max_ill=0.2 if abs(case["dx"]) > 60 else 0.01,
                      label=f"m2 {case['dx']},{case['dy']} st{storage}")
             PJ.destroy()
         PI.destroy()
 
 
-def _stable(ref, f64, tol=0.01):
-    return (ref["status"] == f64["status"]) & ((ref["status"] != 0) | (flow_diff(ref, f64) <= tol / 2))
-
-
 def _report_f64(got, f64, label, stable=None, p99=0.05):
     """Each result's error against the FP64 oracle, reported over all points both call converged and held to
-    p99 <= 0.05 px over the rounding-stable ones (on the others no arithmetic pins the answer)."""
+    p99 <= 0.05 px over the well-conditioned ones (on the others no arithmetic pins the answer)."""
     both = (got["status"] == 0) & (f64["status"] == 0)
     e = flow_diff(got, f64)
     st = both if stable is None else both & stable
     print(f"[lk] {label} error vs FP64: all converged: median {np.median(e[both]):.2e} p99 "
-          f"{np.percentile(e[both], 99):.2e} share <= 0.05 px {(e[both] <= 0.05).mean():.4f}; stable: p99 "
+          f"{np.percentile(e[both], 99):.2e} share <= 0.05 px {(e[both] <= 0.05).mean():.4f}; well-conditioned: p99 "
           f"{np.percentile(e[st], 99):.2e} max {e[st].max():.2e}")
     assert np.percentile(e[st], 99) <= p99
 
@@ -100,12 +91,12 @@ def test_blob_640(ctx, oracle, fb):
     kI, I = oracle.pyramid(f0, w, h, L)
     kJ, J = oracle.pyramid(f1, w, h, L)
     ref = oracle.lk_batch(LANE, I, J, L - 1, cfg, xs, ys, fb_max=fb)
-    f64 = oracle.lk_batch(F64, I, J, L - 1, cfg, xs, ys, fb_max=fb)
+    well, f64 = conditioning(oracle, I, J, L - 1, cfg, xs, ys, fb=fb)
     say("lk blob 640 fb", fb)
     PI, PJ = _gpu_pyr(ctx, f0, w, h, L), _gpu_pyr(ctx, f1, w, h, L)
     got = ctx.lk_track_host(PI, PJ, L - 1, cfg, xs, ys, fb_max=fb)
-    _compare(got, ref, f64, label=f"blob640 fb{fb}")
-    _report_f64(got, f64, "blob640", _stable(ref, f64))
+    _compare(got, ref, well, max_ill=0.1, label=f"blob640 fb{fb}")
+    _report_f64(got, f64, "blob640", well)
     PI.destroy()
     PJ.destroy()
 
@@ -118,12 +109,12 @@ def test_blob_1080_full(ctx, oracle):
     kI, I = oracle.pyramid(f0, w, h, L, threads=8)
     kJ, J = oracle.pyramid(f1, w, h, L, threads=8)
     ref = oracle.lk_batch(LANE, I, J, L - 1, cfg, xs, ys, threads=16)
-    f64 = oracle.lk_batch(F64, I, J, L - 1, cfg, xs, ys, threads=16)
+    well, f64 = conditioning(oracle, I, J, L - 1, cfg, xs, ys)
     say("lk blob 1080 full")
     PI, PJ = _gpu_pyr(ctx, f0, w, h, L), _gpu_pyr(ctx, f1, w, h, L)
     got = ctx.lk_track_host(PI, PJ, L - 1, cfg, xs, ys)
-    _compare(got, ref, f64, label="blob1080")
-    _report_f64(got, f64, "blob1080", _stable(ref, f64))
+    _compare(got, ref, well, max_ill=0.25, label="blob1080")
+    _report_f64(got, f64, "blob1080", well)
     # and against the analytic motion, for the points that converged
     tr = vtgpu.truth(mo, w, h, xs, ys)
     ok = got["status"] == 0
@@ -143,7 +134,7 @@ def test_guess_and_plane_api(ctx, oracle):
     gx = np.full(len(xs), mo.dx * 0.5, np.float32)
     gy = np.full(len(xs), mo.dy * 0.5, np.float32)
     ref = oracle.lk_batch(LANE, I, J, L - 1, cfg, xs, ys, gx, gy, fb_max=0.5)
-    f64 = oracle.lk_batch(F64, I, J, L - 1, cfg, xs, ys, gx, gy, fb_max=0.5)
+    well, _ = conditioning(oracle, I, J, L - 1, cfg, xs, ys, gx, gy, fb=0.5)
     say("lk plane api")
     got = np.zeros(len(xs), vtgpu.LK_OUT_DTYPE)
     Lb = vtgpu.lib()
@@ -152,7 +143,7 @@ def test_guess_and_plane_api(ctx, oracle):
     ctx.check(Lb.vtg_lk_track_planes(ctx.h, Iv, Jv, L - 1, C.byref(vtgpu.LkCfg(21, 21, 10, 0.01, 1e-4, 40.0, 0)),
                                      len(xs), xs.ctypes.data, ys.ctypes.data, gx.ctypes.data, gy.ctypes.data, 0.5,
                                      got.ctypes.data))
-    _compare(got, ref, f64, label="plane api + guess + fb")
+    _compare(got, ref, well, max_ill=0.1, label="plane api + guess + fb")
 
 
 def test_edges_and_flat(ctx, oracle):
@@ -165,11 +156,11 @@ def test_edges_and_flat(ctx, oracle):
     kI, I = oracle.pyramid(f0, w, h, L)
     kJ, J = oracle.pyramid(f1, w, h, L)
     ref = oracle.lk_batch(LANE, I, J, L - 1, cfg, xs, ys)
-    f64 = oracle.lk_batch(F64, I, J, L - 1, cfg, xs, ys)
+    well, _ = conditioning(oracle, I, J, L - 1, cfg, xs, ys)
     say("lk edges")
     PI, PJ = _gpu_pyr(ctx, f0, w, h, L), _gpu_pyr(ctx, f1, w, h, L)
     got = ctx.lk_track_host(PI, PJ, L - 1, cfg, xs, ys)
-    _compare(got, ref, f64, max_unstable=0.1, label="edges")
+    _compare(got, ref, well, max_ill=0.5, label="edges")
     flat = np.full((h, w), 90, np.uint8)
     kF, F = oracle.pyramid(flat, w, h, L)
     PF = _gpu_pyr(ctx, flat, w, h, L)
@@ -201,9 +192,7 @@ def test_precision_variants_vs_f64(ctx, oracle, storage, solve, tol):
     cfg = lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0)
     kI, I = oracle.pyramid(f0, w, h, L, threads=8)
     kJ, J = oracle.pyramid(f1, w, h, L, threads=8)
-    f64 = oracle.lk_batch(F64, I, J, L - 1, cfg, xs, ys, threads=16)
-    lane = oracle.lk_batch(LANE, I, J, L - 1, cfg, xs, ys, threads=16)
-    stable = _stable(lane, f64)
+    stable, f64 = conditioning(oracle, I, J, L - 1, cfg, xs, ys)
     say("lk precision", storage, solve)
     PI, PJ = _gpu_pyr(ctx, f0, w, h, L, storage), _gpu_pyr(ctx, f1, w, h, L, storage)
     got = ctx.lk_track_host(PI, PJ, L - 1, cfg, xs, ys, solve=solve)
@@ -216,3 +205,39 @@ def test_precision_variants_vs_f64(ctx, oracle, storage, solve, tol):
         _report_f64(got, f64, f"storage {storage} solve {solve}", stable, p99=np.inf)
     PI.destroy()
     PJ.destroy()
+
+
+def test_schedules_bit_identical_and_batch(ctx, oracle):
+    """Every LK schedule (direct, staged with margins 1..4) gives the same bytes, and one batched launch over pairs of
+    uneven sizes gives the same bytes as one launch per pair."""
+    sizes = [(640, 480, 3, 400), (1280, 720, 4, 900), (333, 250, 2, 150), (1920, 1080, 4, 1200)]
+    cfg = lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0)
+    pairs, xys, offs, outs = [], [], [0], []
+    for i, (w, h, L, n) in enumerate(sizes):
+        f0, f1, _ = vtgpu.frame_pair(w, h, 70 + i)
+        PI, PJ = _gpu_pyr(ctx, f0, w, h, L), _gpu_pyr(ctx, f1, w, h, L)
+        xs, ys = vtgpu.points(70 + i, n, w, h, 8)
+        pairs.append((PI, PJ, L - 1))
+        xys.append(np.stack([xs, ys], 1).ravel())
+        offs.append(offs[-1] + n)
+    xy = np.concatenate(xys).astype(np.float32)
+    dxy = ctx.to_device(xy)
+    L = vtgpu.lib()
+    results = {}
+    for margin in (0, 1, 2, 3, 4):
+        say("lk schedule margin", margin)
+        ctx.check(L.vtg_lk_set_margin(ctx.h, margin))
+        out = ctx.lk_track_many([p[0] for p in pairs], [p[1] for p in pairs], [p[2] for p in pairs], offs, cfg, dxy,
+                                fb_max=0.5)
+        results[margin] = out.get(np.zeros(offs[-1], vtgpu.LK_OUT_DTYPE))
+        out.free()
+    for m in (1, 2, 3, 4):
+        assert results[m].tobytes() == results[0].tobytes(), m
+    ctx.check(L.vtg_lk_set_margin(ctx.h, 3))
+    for k, (PI, PJ, top) in enumerate(pairs):
+        one = ctx.lk_track_host(PI, PJ, top, cfg, xys[k][0::2], xys[k][1::2], fb_max=0.5)
+        assert one.tobytes() == results[3][offs[k]:offs[k + 1]].tobytes(), k
+    dxy.free()
+    for PI, PJ, _ in pairs:
+        PI.destroy()
+        PJ.destroy()

@@ -53,6 +53,19 @@ def test_scores_whole_and_half(ctx, oracle, level, tw, th):
     t.destroy()
 
 
+def _textured_corner(img, tw, th, seed):
+    """A corner whose tw x th window has strong texture (the searches are only interesting there)."""
+    rng = np.random.default_rng(seed)
+    h, w = img.shape
+    best, at = -1.0, (2, 2)
+    for _ in range(200):
+        x, y = int(rng.integers(4, w - tw - 4)), int(rng.integers(4, h - th - 4))
+        sd = img[y:y + th, x:x + tw].std()
+        if sd > best:
+            best, at = sd, (x, y)
+    return at
+
+
 def _check_cells(gcells, ocells, score_at):
     """Scores within 1e-5; the same corner unless the oracle's best is within 1e-5 of the GPU's pick."""
     assert len(gcells) == len(ocells)
@@ -68,7 +81,7 @@ def _check_cells(gcells, ocells, score_at):
                                             (400, 300, 32, 21, 1), (400, 300, 64, 64, 2)])
 def test_search_matches_oracle(ctx, oracle, w, h, tw, th, step):
     f0, f1, mo, p0, p1, k0, k1 = _setup(ctx, oracle, w, h, 31 + tw + step)
-    x0, y0 = 170, 120
+    x0, y0 = _textured_corner(f0, tw, th, tw + th)
     t = ctx.ncc_template(p0, 0, x0, y0, tw, th)
     app, norm = t.read()
     ncell = ((w - tw - 2 - 1) // 16 + 1) * ((h - th - 2 - 1) // 16 + 1)
@@ -82,6 +95,7 @@ def test_search_matches_oracle(ctx, oracle, w, h, tw, th, step):
           f"{f.ncc:.6f} vs ({of.hx},{of.hy}) {float(of.ncc):.6f}")
     assert (f.hx, f.hy) == (of.hx, of.hy)
     assert abs(f.ncc - float(of.ncc)) <= 1e-5 and abs(f.second - float(of.second)) <= 1e-5
+    assert f.ncc > 0.5
     hp = k1[0]
     import ctypes as C
     l = oracle.lib.vo_ncc_half(C.byref(hp.c), app.ctypes.data, tw, th, norm, of.hx - 1, of.hy)
@@ -111,6 +125,7 @@ def test_search_1080_full(ctx, oracle):
           f"({x0 + tr[0]:.2f},{y0 + tr[1]:.2f})")
     assert (f.best_x, f.best_y) == (int(oc[np.argmax(oc['v'])]["x"]), int(oc[np.argmax(oc['v'])]["y"]))
     assert (f.hx, f.hy) == (of.hx, of.hy) and abs(f.ncc - float(of.ncc)) <= 1e-5
-    assert abs(f.hx / 2 - (x0 + tr[0])) <= 2 and abs(f.hy / 2 - (y0 + tr[1])) <= 2
+    # the rotated, noisy copy peaks within a few pixels of where the template's centre moved (smooth blobs)
+    assert abs(f.hx / 2 - (x0 + tr[0])) <= 5 and abs(f.hy / 2 - (y0 + tr[1])) <= 5
     cells.free()
     t.destroy()

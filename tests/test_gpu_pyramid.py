@@ -64,6 +64,29 @@ def test_pyramid_from_device_frame(ctx, oracle):
     d.free()
 
 
+@pytest.mark.parametrize("storage", [vtgpu.STORE_Q6, vtgpu.STORE_F32, vtgpu.STORE_F16])
+def test_build_many_uneven(ctx, oracle, storage):
+    """Pyramids of uneven sizes and depths built together (one launch per level) equal the ones built alone."""
+    sizes = [(640, 480, 3), (1280, 720, 5), (97, 61, 2), (1920, 1080, 4), (333, 250, 1)]
+    frames = [vtgpu.frame_pair(w, h, 40 + i)[0] if w >= 64 else None for i, (w, h, _) in enumerate(sizes)]
+    say("build_many", storage)
+    pyrs = [ctx.pyramid(w, h, L, storage) for w, h, L in sizes]
+    dev = [ctx.to_device(f) for f in frames]
+    ctx.build_many(pyrs, dev)
+    for (w, h, L), f, p in zip(sizes, frames, pyrs):
+        q = ctx.pyramid(w, h, L, storage)
+        q.build(f)
+        for lv in range(L):
+            assert p.read_level(lv).padded().tobytes() == q.read_level(lv).padded().tobytes(), (w, h, lv)
+        if storage != vtgpu.STORE_F16:
+            planes, _ = oracle.pyramid(f, w, h, L)
+            assert np.array_equal(p.read_level(L - 1).view(), planes[L - 1].view())
+        q.destroy()
+        p.destroy()
+    for d in dev:
+        d.free()
+
+
 @pytest.mark.parametrize("storage,tol", [(vtgpu.STORE_F16, 2.0 ** -11), (vtgpu.STORE_BF16, 2.0 ** -8)])
 def test_pyramid_half_storage(ctx, oracle, storage, tol):
     """fp16 / bf16 levels hold grey = Q6 / 64 rounded once per level: within a few ulps of the exact pyramid."""
