@@ -1,14 +1,24 @@
 #!/usr/bin/env python3
 """bench.py: 1080p frame-pair throughput of the vision path on one MI355X.
 
-A step is one 1920 x 1080 u8 frame pair already in HBM: both 4-level pyramids, sparse pyramidal LK of 16,200 grid
-points (180 x 90, 21 x 21 window, 10 iterations, eps 0.01 px, min_eig 1e-4), and a 64 x 64 NCC template cut from
-frame 0 searched exhaustively over frame 1 (cell map, top 8, half-pixel refinement, result to the host). Inputs
-rotate over more than the 256 MB Infinity Cache. Prints one JSON line; details go to stderr.
+The work per frame pair: a 1920 x 1080 u8 pair already in HBM; both 4-level pyramids; sparse pyramidal LK of 16,200
+grid points (180 x 90, 21 x 21 window, 10 iterations, eps 0.01 px, min_eig 1e-4); a 64 x 64 template cut from frame 0
+searched exhaustively over frame 1 (cell map, top 8, half-pixel refinement); LK records and the search's pick copied to
+the host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--storage q6|f32|f16|bf16] [--cpu-seconds S]
+Two forms run, on the same inputs:
+  plain    the reference's fp32 Q6 planes, one pair at a time, one launch per pyramid level per frame, the NCC map
+           in the reference's own fp32 arithmetic (the straightforward port);
+  batched  the headline: B pairs per step, exact u8/u16 Q6 pyramids built with one launch per level for all 2B
+           pyramids, one LK launch for all B pairs, the B searches on a second stream overlapping LK, the step
+           captured once as a HIP graph and replayed.
+A step of the batched form is one batch of B pairs; `value` is pairs per second. Inputs rotate over more than the
+256 MB Infinity Cache. Prints one JSON line; details go to stderr.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-graph] [--cpu-seconds S]
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -21,10 +31,11 @@ sys.path.insert(0, os.path.join(REPO, "intel-phi-visiontrack_amd"))
 
 METRIC = "LK pts/s + 1080p frame-pairs/s vs plain; p50/p99 ms @ ≤0.05 px err"
 W, H, LEVELS, NX, NY, WIN, TPL = 1920, 1080, 4, 180, 90, 21, 64
+NPTS = NX * NY
 PEAK_HBM_GBS = 8000.0
-PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector (published; with dual-issue packed FP32)
-PEAK_DOT4_TOPS = 314.6        # u8 dot4 on the vector ALUs: 256 CU x 4 SIMD x 16 lanes x 4 MAC x 2 x 2.4 GHz
-STORAGES = {"q6": 0, "f32": 1, "f16": 2, "bf16": 3}
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, published (dual-issue packed FP32)
+PEAK_DOT4_TOPS = 314.6     # u8 dot4 on the vector ALUs: 256 CU x 4 SIMD x 16 lanes x 4 MAC x 2 x 2.4 GHz (derived)
+UNIQ = 8                   # distinct seeded pairs; device copies rotate through more than the Infinity Cache
 
 
 def log(*a):
@@ -45,7 +56,8 @@ def lk_flops(out, xs, ys, top, w, h, win):
             lw, lh = (lw + 1) // 2, (lh + 1) // 2
         px, py = xs / 2 ** L, ys / 2 ** L
         levels += (px >= m) & (py >= m) & (px < lw - m - 1) & (py < lh - m - 1)
-    return float((levels * build).sum() + out["iters"].astype(np.float64).sum() * it + (out["status"] == 0).sum() * res)
+    return float((levels * build).sum() + out["iters"].astype(np.float64).sum() * it
+                 + (out["status"] == 0).sum() * res)
 
 
 def pyr_bytes(w, h, levels, elem):
@@ -57,16 +69,118 @@ def pyr_bytes(w, h, levels, elem):
     return b
 
 
-def cpu_baseline(f0s, f1s, xs, ys, budget_s, gpu_out):
-    """The oracle (reference-faithful tier, C, pthreads) on this host: pyramids of both frames, LK on a sample of
-    the grid, the NCC map on a band of cells; scaled to frame pairs per second. Also the GPU's flow error against
-    the FP64 oracle on the same sample."""
+class Batched:
+    """B pairs per step on two contexts (A: pyramids and LK, S: the searches)."""
+
+    def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph):
+        self.v, self.B, self.graph = vtgpu, batch, graph
+        self.A, self.S = vtgpu.Context(0), vtgpu.Context(0)
+        self.frames, self.corners = d_frames, corners
+        self.groups = len(d_frames) // batch
+        A = self.A
+        self.pyrs = [[(A.pyramid(W, H, LEVELS), A.pyramid(W, H, LEVELS)) for _ in range(batch)]
+                     for _ in range(self.groups if graph else 1)]
+        self.xy = A.to_device(np.tile(np.stack([xs, ys], 1).ravel(), batch))
+        self.out = A.alloc(32 * NPTS * batch)
+        self.cfg = vtgpu.LkCfg(WIN, WIN, 10, 0.01, 1e-4, 40.0, 0)
+        self.offs = [k * NPTS for k in range(batch + 1)]
+        ncell = ((W - 1) // 16 + 1) * ((H - 1) // 16 + 1)   # the header's bound for corners [1, W] x [1, H]
+        self.cells = [self.S.alloc(12 * ncell) for _ in range(batch)]
+        self.fs = C.sizeof(vtgpu.Found)
+        self.found = self.S.alloc(self.fs * batch)
+        self.tmpl = [None] * batch
+        self.h_out = np.zeros(NPTS * batch, vtgpu.LK_OUT_DTYPE)
+        self.h_found = (vtgpu.Found * batch)()
+        self.graphs = {}
+
+    def enqueue(self, g):
+        """All device work of group g's batch, stream-ordered on A and S (A waits for S at the end)."""
+        A, S, B = self.A, self.S, self.B
+        pyrs = self.pyrs[g if self.graph else 0]
+        fr = self.frames[g * B:(g + 1) * B]
+        A.build_many([p for pair in pyrs for p in pair], [f for pair in fr for f in pair])
+        S.wait(A)
+        A.lk_track_many([p[0] for p in pyrs], [p[1] for p in pyrs], [LEVELS - 1] * B, self.offs, self.cfg, self.xy,
+                        out=self.out)
+        for k in range(B):
+            cx, cy = self.corners[(g * B + k) % UNIQ]
+            if self.tmpl[k] is None:
+                self.tmpl[k] = S.ncc_template(pyrs[k][0], 0, cx, cy, TPL, TPL)
+            else:
+                self.tmpl[k].update(pyrs[k][0], 0, cx, cy)
+            S.check(self.v.lib().vtg_ncc_search_async(S.h, pyrs[k][1].h, self.tmpl[k].h, 1, W, 1, H, 1,
+                                                      self.cells[k].ptr, C.c_void_p(self.found.ptr.value + self.fs * k)),
+                    "ncc_search_async")
+        A.wait(S)
+
+    def prepare(self):
+        """Grow scratch and templates eagerly once per group, then capture each group's step."""
+        for g in range(self.groups):
+            self.enqueue(g)
+        self.A.sync()
+        self.S.sync()
+        if self.graph:
+            for g in range(self.groups):
+                self.A.graph_begin()
+                self.enqueue(g)
+                self.graphs[g] = self.A.graph_end()
+
+    def step(self, i):
+        g = i % self.groups
+        if self.graph:
+            self.A.graph_launch(self.graphs[g])
+        else:
+            self.enqueue(g)
+        # results to the host: every LK record and every pick
+        self.out.get(self.h_out)
+        self.A.check(self.v.lib().vtg_download(self.A.h, C.cast(self.h_found, C.c_void_p), self.found.ptr,
+                                               self.fs * self.B), "download")
+        return self.h_out
+
+
+def plain_rate(vtgpu, d_frames, corners, xs, ys, steps):
+    """The straightforward port: fp32 Q6 planes, one pair at a time, the NCC map in the reference's arithmetic."""
+    ctx = vtgpu.Context(0)
+    p0, p1 = ctx.pyramid(W, H, LEVELS, vtgpu.STORE_F32), ctx.pyramid(W, H, LEVELS, vtgpu.STORE_F32)
+    xy = ctx.to_device(np.stack([xs, ys], 1).ravel())
+    out = ctx.alloc(32 * NPTS)
+    cfg = vtgpu.LkCfg(WIN, WIN, 10, 0.01, 1e-4, 40.0, 0)
+    h_out = np.zeros(NPTS, vtgpu.LK_OUT_DTYPE)
+    tmpl = None
+
+    def one(i):
+        nonlocal tmpl
+        fa, fb = d_frames[i % len(d_frames)]
+        p0.build(fa, on_device=True)
+        p1.build(fb, on_device=True)
+        ctx.lk_track(p0, p1, LEVELS - 1, cfg, xy, None, 0.0, out)
+        cx, cy = corners[i % UNIQ]
+        if tmpl is None:
+            tmpl = ctx.ncc_template(p0, 0, cx, cy, TPL, TPL)
+        else:
+            tmpl.update(p0, 0, cx, cy)
+        ctx.ncc_search(p1, tmpl, 1, W, 1, H, 1)
+        out.get(h_out)
+
+    one(0)
+    ctx.sync()
+    t = time.perf_counter()
+    for i in range(steps):
+        one(i)
+    ctx.sync()
+    return steps / (time.perf_counter() - t)
+
+
+def cpu_baseline(f0, f1, xs, ys, budget_s, gpu_out):
+    """The oracle's reference-faithful tier (C, pthreads) on this host: pyramids of both frames, LK on a sample of the
+    grid, the NCC map on a band of cell rows; scaled to frame pairs per second. Also the GPU's flow error against the
+    FP64 oracle on the sample, over its well-conditioned points."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
+    from gpu_util import conditioning
     from oracle_ffi import Oracle, lk_cfg
     o = Oracle()
     cores = min(os.cpu_count() or 1, 16)
     cfg = lk_cfg(WIN, WIN, 10, 0.01, 1e-4, 40.0)
-    f0, f1 = f0s[0], f1s[0]
     t = time.perf_counter()
     k0, I = o.pyramid(f0, W, H, LEVELS, threads=cores)
     k1, J = o.pyramid(f1, W, H, LEVELS, threads=cores)
@@ -76,34 +190,33 @@ def cpu_baseline(f0s, f1s, xs, ys, budget_s, gpu_out):
     t = time.perf_counter()
     ref = o.lk_batch(0, I, J, LEVELS - 1, cfg, sx, sy, threads=cores)
     t_lk = (time.perf_counter() - t) * stride
-    from gpu_util import conditioning
     well, f64 = conditioning(o, I, J, LEVELS - 1, cfg, sx, sy, threads=cores)
     g = gpu_out[::stride]
     both = (g["status"] == 0) & (f64["status"] == 0)
     e_all = np.maximum(np.abs(g["x"] - f64["x"]), np.abs(g["y"] - f64["y"]))
     err = e_all[both & well]
     same_status = float((g["status"] == ref["status"])[well].mean())
-    dref = np.maximum(np.abs(g["x"] - ref["x"]), np.abs(g["y"] - ref["y"]))[(g["status"] == 0) & (ref["status"] == 0)
-                                                                            & well]
+    ok_ref = (g["status"] == 0) & (ref["status"] == 0) & well
+    dref = np.maximum(np.abs(g["x"] - ref["x"]), np.abs(g["y"] - ref["y"]))[ok_ref]
     app, norm = o.ncc_template(k0[0], 700, 400, TPL, TPL)
     x1, y1 = W - TPL - 2, H - TPL - 2
-    cols, rows = (x1 - 1) // 16 + 1, (y1 - 1) // 16 + 1
+    rows = (y1 - 1) // 16 + 1
     band = max(1, min(rows, int(budget_s / 2)))
     t = time.perf_counter()
     o.ncc_search(k1[0], app, TPL, TPL, norm, 1, x1, 1, 16 * band, 1, threads=cores)
     t_ncc = (time.perf_counter() - t) * rows / band
     t_pair = t_pyr + t_lk + t_ncc
     sample = (f"1 pair: both pyramids, LK on every {stride}th grid point ({len(sx)} pts) x{stride}, NCC map on "
-              f"{band} of {rows} cell rows x{rows / band:.1f}, lane-model tier")
+              f"{band} of {rows} cell rows x{rows / band:.1f}; lane-model tier, {cores} threads")
     log(f"cpu: {cores} threads: pyramids {t_pyr * 1e3:.1f} ms, LK {t_lk * 1e3:.0f} ms/16,200 pts, NCC "
         f"{t_ncc * 1e3:.0f} ms -> {1 / t_pair:.3f} pairs/s")
     return ({"value": round(1.0 / t_pair, 4), "unit": "frame-pairs/s", "cores": cores, "kind": "port",
-             "sample": sample, "lk_pts_per_s": round(16200 / t_lk, 1), "stage_ms": {
+             "sample": sample, "lk_pts_per_s": round(NPTS / t_lk, 1), "stage_ms": {
                  "pyramids": round(t_pyr * 1e3, 2), "lk": round(t_lk * 1e3, 1), "ncc": round(t_ncc * 1e3, 1)}},
             {"sample_points": int(len(sx)), "well_conditioned": int(well.sum()),
              "vs_f64_median_px": float(np.median(err)), "vs_f64_p99_px": float(np.percentile(err, 99)),
-             "vs_f64_max_px": float(err.max()), "share_within_0.05px_all_converged": float(
-                 (e_all[both] <= 0.05).mean()),
+             "vs_f64_max_px": float(err.max()),
+             "share_within_0.05px_all_converged": float((e_all[both] <= 0.05).mean()),
              "status_equal_vs_reference_tier": same_status,
              "vs_reference_tier_max_px": float(dref.max()) if len(dref) else 0.0})
 
@@ -111,10 +224,12 @@ def cpu_baseline(f0s, f1s, xs, ys, budget_s, gpu_out):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--storage", default="q6", choices=sorted(STORAGES))
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=8, help="frame pairs per step of the batched form")
     ap.add_argument("--pairs", type=int, default=72, help="device-resident frame pairs rotated through")
+    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
+    ap.add_argument("--plain-steps", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -122,108 +237,92 @@ def main():
               f"exchange, so N GPUs run N independent copies of this step)", file=sys.stderr)
         sys.exit(2)
     import vtgpu
-    storage = STORAGES[a.storage]
-    ctx = vtgpu.Context(0)
-    # 8 distinct seeded pairs, copied to `pairs` device slots (> 256 MB of inputs with the pyramids)
-    uniq = 8
+    pairs = max(a.batch, (a.pairs // a.batch) * a.batch)
     f0s, f1s = [], []
-    for s in range(uniq):
+    for s in range(UNIQ):
         f0, f1, _ = vtgpu.frame_pair(W, H, 100 + s)
         f0s.append(f0)
         f1s.append(f1)
     xs, ys = vtgpu.grid(NX, NY, W, H)
-    npts = len(xs)
-    d_frames = []
-    for i in range(a.pairs):
-        d_frames.append((ctx.to_device(f0s[i % uniq]), ctx.to_device(f1s[i % uniq])))
-    pyrs = [(ctx.pyramid(W, H, LEVELS, storage), ctx.pyramid(W, H, LEVELS, storage)) for _ in range(min(a.pairs, 8))]
-    d_xy = ctx.to_device(np.stack([xs, ys], 1).ravel())
-    d_out = ctx.alloc(32 * npts)
-    cfg = vtgpu.LkCfg(WIN, WIN, 10, 0.01, 1e-4, 40.0, 0)
     rng = np.random.default_rng(2)
-    corners = [(int(rng.integers(100, W - 200)), int(rng.integers(100, H - 200))) for _ in range(uniq)]
-    tmpl = None
-    ncell = ((W - TPL - 3) // 16 + 1) * ((H - TPL - 3) // 16 + 1)
-    d_cells = ctx.alloc(12 * ncell)
+    corners = [(int(rng.integers(100, W - 200)), int(rng.integers(100, H - 200))) for _ in range(UNIQ)]
+    up = vtgpu.Context(0)
+    d_frames = [(up.to_device(f0s[i % UNIQ]), up.to_device(f1s[i % UNIQ])) for i in range(pairs)]
+    up.sync()
 
-    def step(i):
-        fa, fb = d_frames[i % a.pairs]
-        pa, pb = pyrs[i % len(pyrs)]
-        ctx.build_many([pa, pb], [fa, fb])
-        ctx.lk_track(pa, pb, LEVELS - 1, cfg, d_xy, None, 0.0, d_out)
-        nonlocal tmpl
-        cx, cy = corners[i % uniq]
-        if tmpl is None:
-            tmpl = ctx.ncc_template(pa, 0, cx, cy, TPL, TPL)
-        else:
-            tmpl.update(pa, 0, cx, cy)
-        return ctx.ncc_search(pb, tmpl, 1, W, 1, H, 1, d_cells)
-
+    bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=not a.no_graph)
+    bt.prepare()
     for i in range(a.warmup):
-        step(i)
-    ctx.sync()
-    # per-pair algorithmic counts, from each distinct pair's own LK output
-    flops = []
-    for s in range(uniq):
-        step(s)
-        ctx.sync()
-        out = d_out.get(np.zeros(npts, vtgpu.LK_OUT_DTYPE))
-        flops.append(lk_flops(out, xs, ys, LEVELS - 1, W, H, WIN))
-        if s == 0:
-            out0 = out.copy()
-    ctx.timing(True)
-    ctx.timing_reset()
+        bt.step(i)
     lat = []
-    ctx.sync()
     t0 = time.perf_counter()
     for i in range(a.steps):
         ts = time.perf_counter()
-        f = step(i)  # ends with the found record on the host
+        bt.step(i)
         lat.append(time.perf_counter() - ts)
-    ctx.sync()
     total = time.perf_counter() - t0
-    kernels = {}
-    for k in ("pyr_reduce", "lk_track", "ncc_template", "ncc_cells", "ncc_cells_generic", "ncc_top8", "ncc_refine",
-              "ncc_final"):
-        ms, n = ctx.timing_read(k)
-        if n:
-            kernels[k] = {"avg_us": round(1e3 * ms / n, 2), "launches": n, "ms_per_step": round(ms / a.steps, 4)}
-    ctx.timing(False)
-    log("kernels:", json.dumps(kernels))
+    pairs_per_s = a.steps * a.batch / total
     lat_ms = np.array(lat) * 1e3
-    value = a.steps / total
-    lk_flop = float(np.mean([flops[i % uniq] for i in range(a.steps)]))
+    # per-pair algorithmic counts from each pair's own LK records (group 0 holds pairs 0..B-1)
+    bt.step(0)
+    flops = [lk_flops(bt.h_out[k * NPTS:(k + 1) * NPTS], xs, ys, LEVELS - 1, W, H, WIN) for k in range(a.batch)]
+    gpu_out0 = bt.h_out[:NPTS].copy()
+    found0 = bt.h_found[0]
+    log(f"batched: {pairs_per_s:.1f} pairs/s, batch p50 {np.percentile(lat_ms, 50):.3f} ms p99 "
+        f"{np.percentile(lat_ms, 99):.3f} ms; pair 0 NCC pick ({found0.hx / 2},{found0.hy / 2}) {found0.ncc:.4f}")
+
+    # per-kernel times: the same batched steps launched eagerly, HIP events around every launch on its own stream
+    bt.A.timing(True)
+    bt.S.timing(True)
+    ksteps = min(a.steps, 10)
+    for i in range(ksteps):
+        bt.enqueue(i % bt.groups)
+    bt.A.sync()
+    kernels = {}
+    for ctx in (bt.A, bt.S):
+        for k in ("pyr_reduce", "lk_track", "ncc_template", "ncc_cells", "ncc_top8", "ncc_refine", "ncc_final"):
+            ms, n = ctx.timing_read(k)
+            if n:
+                kernels[k] = {"avg_us": round(1e3 * ms / n, 2), "launches": n, "ms_per_step": round(ms / ksteps, 4)}
+    bt.A.timing(False)
+    bt.S.timing(False)
     npos = (W - TPL - 2) * (H - TPL - 2)
-    elem = {0: 2, 1: 4, 2: 2, 3: 2}[storage]
-    # algorithmic work per step of each kernel: fp32 flops (LK), u8 MAC ops (NCC), HBM bytes (pyramids)
-    work = {
-        "lk_track": ("valu", lk_flop / 1e12, PEAK_FP32_TFLOPS, "TFLOP/s"),
-        "ncc_cells": ("valu", 2.0 * TPL * TPL * npos / 1e12, PEAK_DOT4_TOPS, "TOP/s"),
-        "pyr_reduce": ("hbm", 2 * pyr_bytes(W, H, LEVELS, elem) / 1e9, PEAK_HBM_GBS, "GB/s"),
+    lk_flop_step = float(np.sum(flops))
+    work = {  # per step (one batch): fp32 flops, u8 MAC ops, HBM bytes
+        "lk_track": ("valu", lk_flop_step / 1e12, PEAK_FP32_TFLOPS, "TFLOP/s"),
+        "ncc_cells": ("valu", a.batch * 2.0 * TPL * TPL * npos / 1e12, PEAK_DOT4_TOPS, "TOP/s"),
+        "pyr_reduce": ("hbm", a.batch * 2 * pyr_bytes(W, H, LEVELS, 2) / 1e9, PEAK_HBM_GBS, "GB/s"),
     }
     for k, (b, wk, pk, u) in work.items():
         if k in kernels:
             ach = wk / (kernels[k]["ms_per_step"] * 1e-3)
             kernels[k].update({"achieved": round(ach, 3), "unit": u, "frac": round(ach / pk, 4)})
+    log("kernels:", json.dumps(kernels))
     dom = max((k for k in kernels if k in work), key=lambda k: kernels[k]["ms_per_step"])
     bound, per_step, peak, unit = work[dom]
     achieved = per_step / (kernels[dom]["ms_per_step"] * 1e-3)
-    lk_us = kernels.get("lk_track", {}).get("avg_us", float("nan"))
+    lk_us = kernels["lk_track"]["avg_us"]
+
+    plain = plain_rate(vtgpu, d_frames, corners, xs, ys, a.plain_steps)
+    log(f"plain: {plain:.2f} pairs/s")
     cpu, err = ({}, {})
     if a.cpu_seconds > 0:
-        cpu, err = cpu_baseline(f0s, f1s, xs, ys, a.cpu_seconds, out0)
+        cpu, err = cpu_baseline(f0s[0], f1s[0], xs, ys, a.cpu_seconds, gpu_out0)
     line = {
-        "metric": METRIC, "value": round(value, 2), "unit": "frame-pairs/s", "n_gpus": 1, "steps": a.steps,
+        "metric": METRIC, "value": round(pairs_per_s, 2), "unit": "frame-pairs/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(1e3 * total / a.steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u8/u16 int + fp32 (fp64 per-point solve)",
         "data": "synthetic (seeded 200-blob frames, rigid motion + noise)",
-        "config": {"workload": "1080p frame pair: 2 x 4-level pyramid + LK 16,200 pts 21x21 + NCC 64x64 exhaustive",
-                   "w": W, "h": H, "levels": LEVELS, "points": npts, "window": f"{WIN}x{WIN}", "ncc_template":
-                   f"{TPL}x{TPL}", "ncc_corners": npos, "storage": a.storage, "parallelism": "1 GPU",
-                   "input_rotation": f"{a.pairs} device frame pairs ({a.pairs * 2 * W * H / 2 ** 20:.0f} MiB)"},
+        "config": {"workload": "1080p frame pairs: 2 x 4-level pyramid + LK 16,200 pts 21x21 + NCC 64x64 exhaustive "
+                               "per pair", "w": W, "h": H, "levels": LEVELS, "points_per_pair": NPTS,
+                   "window": f"{WIN}x{WIN}", "ncc_template": f"{TPL}x{TPL}", "ncc_corners": npos,
+                   "pairs_per_step": a.batch, "storage": "q6 (u8 level 0, u16 above)", "graph": not a.no_graph,
+                   "parallelism": "1 GPU, 2 streams",
+                   "input_rotation": f"{pairs} device frame pairs ({pairs * 2 * W * H / 2 ** 20:.0f} MiB)"},
+        "plain_pairs_per_s": round(plain, 2), "vs_plain": round(pairs_per_s / plain, 2),
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
-        "lk_pts_per_s": round(npts / (lk_us * 1e-6), 1),
-        "lk_pts_per_s_in_step": round(npts * value, 1),
+        "latency_unit": f"one step = {a.batch} pairs, submit to results on the host",
+        "lk_pts_per_s": round(a.batch * NPTS / (lk_us * 1e-6), 1),
         "kernels": kernels,
         "roofline": {"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
                      "frac": round(achieved / peak, 4), "traffic": None},

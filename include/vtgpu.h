@@ -42,6 +42,10 @@ int vtg_ctx_destroy(vtg_ctx *ctx);
 int vtg_ctx_sync(vtg_ctx *ctx);
 /* The last failure's reason, "" if none. */
 const char *vtg_ctx_error(const vtg_ctx *ctx);
+/* Work submitted to ctx from now on starts only after all work submitted to `other` so far (both on one device):
+ * two contexts are two streams, and this is how a caller overlaps independent stages, e.g. the NCC search on one
+ * context beside LK on another once both pyramids exist. No host wait. */
+int vtg_ctx_wait(vtg_ctx *ctx, vtg_ctx *other);
 /* The context's hipStream_t, as an opaque pointer (for callers that order their own work after it). */
 void *vtg_ctx_stream(vtg_ctx *ctx);
 
@@ -58,6 +62,14 @@ int vtg_copy_d2d(vtg_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes)
 int vtg_timing_enable(vtg_ctx *ctx, int on);
 int vtg_timing_read(vtg_ctx *ctx, const char *kernel, double *total_ms, long *launches);
 int vtg_timing_reset(vtg_ctx *ctx);
+
+/* Capture the context's stream-ordered work into a replayable HIP graph: begin, make the calls (stream-ordered ones
+ * only, with buffers that outlive the graph; contexts joined by vtg_ctx_wait join the capture), end. Replaying
+ * repeats the same launches on the same buffers. Timing must be off while capturing. */
+int vtg_graph_begin(vtg_ctx *ctx);
+int vtg_graph_end(vtg_ctx *ctx, void **graph);
+int vtg_graph_launch(vtg_ctx *ctx, void *graph);
+int vtg_graph_destroy(vtg_ctx *ctx, void *graph);
 
 /* ------------------------------------------------------------------ pyramids (tk_build, src/kern/track.c:584-629) */
 
@@ -112,7 +124,7 @@ int vtg_lk_track_many(vtg_ctx *ctx, int npairs, const vtg_pyr *const *I, const v
                       int solve_mode, struct lk_out *out);
 /* The LK schedule: margin > 0 stages each level's template patch and J's neighbourhood (the window plus `margin`
  * pixels) in LDS with one batch of loads; 0 reads the pyramid in memory at every iteration. Results are the same
- * bit for bit either way; only time differs (default 3). */
+ * bit for bit either way; only time differs (default 0). */
 int vtg_lk_set_margin(vtg_ctx *ctx, int margin);
 
 /* ------------------------------------------------------------------ NCC search (src/kern/track.c:184-208, 301-329, 935-974, 1615-1734) */
@@ -148,6 +160,10 @@ int vtg_ncc_template_read(vtg_ctx *ctx, const vtg_tmpl *t, float *app_host, floa
  * found: host record, written on return (this call waits for the search). */
 int vtg_ncc_search(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
                    struct vtg_cell *cells, struct vtg_found *found);
+/* The same, stream-ordered: the pick goes to the device record found_dev and the call returns at once (no host
+ * wait, no allocation once the context's scratch has grown to the map's size, so it can be captured in a graph). */
+int vtg_ncc_search_async(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
+                         struct vtg_cell *cells, struct vtg_found *found_dev);
 /* Scores at n corners (tk_correlate, track.h:57-59; half != 0: half-pixel corners, search_half): xy device int
  * pairs, out device doubles: -1 outside, 0 for a window under 1/16 of the template's energy. */
 int vtg_ncc_score(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out);

@@ -154,6 +154,8 @@ int vtg_ctx_destroy(vtg_ctx *c)
         (void)hipFree(c->scratch);
     if (c->pinned)
         (void)hipHostFree(c->pinned);
+    if (c->join)
+        (void)hipEventDestroy(c->join);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return VTG_OK;
@@ -168,6 +170,19 @@ int vtg_ctx_sync(vtg_ctx *c)
 }
 
 const char *vtg_ctx_error(const vtg_ctx *c) { return c ? c->err : "no context"; }
+
+int vtg_ctx_wait(vtg_ctx *c, vtg_ctx *other)
+{
+    if (!c || !other)
+        return VTG_EINVAL;
+    if (c->device != other->device)
+        return fail(c, VTG_EINVAL, "vtg_ctx_wait: contexts on different devices");
+    if (!other->join)
+        VTG_HIP(c, hipEventCreateWithFlags(&other->join, hipEventDisableTiming));
+    VTG_HIP(c, hipEventRecord(other->join, other->stream));
+    VTG_HIP(c, hipStreamWaitEvent(c->stream, other->join, 0));
+    return VTG_OK;
+}
 
 void *vtg_ctx_stream(vtg_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
@@ -213,6 +228,48 @@ int vtg_copy_d2d(vtg_ctx *c, void *dst, const void *src, size_t bytes)
     if (!c || (!dst && bytes) || (!src && bytes))
         return fail(c, VTG_EINVAL, "vtg_copy_d2d: bad argument");
     VTG_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return VTG_OK;
+}
+
+int vtg_graph_begin(vtg_ctx *c)
+{
+    if (!c)
+        return VTG_EINVAL;
+    if (c->timing)
+        return fail(c, VTG_EINVAL, "vtg_graph_begin: per-kernel timing is on (events cannot be captured)");
+    VTG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    return VTG_OK;
+}
+
+int vtg_graph_end(vtg_ctx *c, void **graph_exec)
+{
+    if (!c || !graph_exec)
+        return VTG_EINVAL;
+    hipGraph_t g = nullptr;
+    VTG_HIP(c, hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t e = nullptr;
+    hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (err != hipSuccess)
+        return hip_fail(c, err, "hipGraphInstantiate");
+    *graph_exec = (void *)e;
+    return VTG_OK;
+}
+
+int vtg_graph_launch(vtg_ctx *c, void *graph_exec)
+{
+    if (!c || !graph_exec)
+        return VTG_EINVAL;
+    VTG_HIP(c, hipGraphLaunch((hipGraphExec_t)graph_exec, c->stream));
+    return VTG_OK;
+}
+
+int vtg_graph_destroy(vtg_ctx *c, void *graph_exec)
+{
+    if (!c || !graph_exec)
+        return VTG_EINVAL;
+    VTG_HIP(c, hipStreamSynchronize(c->stream));
+    VTG_HIP(c, hipGraphExecDestroy((hipGraphExec_t)graph_exec));
     return VTG_OK;
 }
 

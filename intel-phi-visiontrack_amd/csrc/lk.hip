@@ -32,7 +32,7 @@ struct Cfg {
     float eps, min_eig, max_res;
 };
 
-constexpr int BMAX = 64;
+constexpr int BMAX = 8; // pairs per launch (inline in the kernel arguments)
 
 struct LkArgs {
     const PyrView *I[BMAX], *J[BMAX];
@@ -261,14 +261,14 @@ __device__ __forceinline__ void build_direct(const Level &lv, double px, double 
     const T *base = (const T *)lv.p;
     const int nS = SW * SH, nb = (nS + 63) / 64;
     wave_sync();
-#pragma unroll 3
+#pragma unroll 1
     for (int blk = 0; blk < nb; blk++) {
         const int i = lane + 64 * blk, ic = i < nS ? i : nS - 1;
         const int r = ic / SW, cc = ic - r * SW;
-        const int y = clampi(ys + r, 0, lv.h - 2), x = clampi(xs + cc, 0, lv.w - 2);
-        const T *p = base + (size_t)y * lv.pitch + x;
-        const float a0 = Ld<T>::get(p[0]), a1 = Ld<T>::get(p[1]);
-        const float b0 = Ld<T>::get(p[lv.pitch]), b1 = Ld<T>::get(p[lv.pitch + 1]);
+        const unsigned o = (unsigned)clampi(ys + r, 0, lv.h - 2) * (unsigned)lv.pitch +
+                           (unsigned)clampi(xs + cc, 0, lv.w - 2);
+        const float a0 = Ld<T>::get(base[o]), a1 = Ld<T>::get(base[o + 1]);
+        const float b0 = Ld<T>::get(base[o + lv.pitch]), b1 = Ld<T>::get(base[o + lv.pitch + 1]);
         const float top = fmaf(a1 - a0, fx, a0), bot = fmaf(b1 - b0, fx, b0);
         if (i < nS)
             S[i] = fmaf(bot - top, fy, top);
@@ -302,16 +302,15 @@ __device__ __forceinline__ void sample_direct(const Level &lv, double qx, double
 {
     const double flx = floor(qx), fly = floor(qy);
     const float fx = (float)(qx - flx), fy = (float)(qy - fly);
-    const int x = clampi((int)flx - g.W / 2 + g.k, 0, lv.w - 2);
+    const unsigned x = (unsigned)clampi((int)flx - g.W / 2 + g.k, 0, lv.w - 2);
     const int y0 = (int)fly - g.H / 2 + g.r0;
-    const T *col = (const T *)lv.p + x;
-    const size_t pitch = (size_t)lv.pitch;
+    const T *base = (const T *)lv.p; // uniform: 32-bit per-lane offsets against it
     float a0[MAXR + 1], a1[MAXR + 1];
 #pragma unroll
     for (int i = 0; i <= MAXR; i++) {
-        const T *p = col + (size_t)clampi(y0 + i, 0, lv.h - 1) * pitch;
-        a0[i] = Ld<T>::get(p[0]);
-        a1[i] = Ld<T>::get(p[1]);
+        const unsigned o = (unsigned)clampi(y0 + i, 0, lv.h - 1) * (unsigned)lv.pitch + x;
+        a0[i] = Ld<T>::get(base[o]);
+        a1[i] = Ld<T>::get(base[o + 1]);
     }
     float s0 = 0.0f, s1 = 0.0f;
     float above = fmaf(a1[0] - a0[0], fx, a0[0]);
@@ -399,7 +398,7 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
     double gx = ldexp((double)gx0, -top), gy = ldexp((double)gy0, -top);
     const double area = (double)(c.w * c.h), eps2 = (double)c.eps * (double)c.eps;
     const int mx = c.w / 2 + 1, my = c.h / 2 + 1;
-    double det0 = 0, gxx0 = 0, gxy0 = 0, gyy0 = 0;
+    float G0[3] = {0.0f, 0.0f, 0.0f}; // level 0's structure tensor, for the covariance
     int status = 0, iters = 0;
     bool valid0 = false;
     float eig0 = 0.0f;
@@ -443,7 +442,7 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             continue;
         }
         if (L == 0) {
-            det0 = gxx * gyy - gxy * gxy, gxx0 = gxx, gxy0 = gxy, gyy0 = gyy;
+            G0[0] = G[0], G0[1] = G[1], G0[2] = G[2];
             valid0 = true;
         }
         const double lx = (double)(lj.w - mx - 1), ly = (double)(lj.h - my - 1);
@@ -515,6 +514,7 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             if (o.residual > c.max_res)
                 status |= LK_RESIDUAL;
             const double s2 = ((double)r[1] / n) * area / (area - 2);
+            const double gxx0 = G0[0], gxy0 = G0[1], gyy0 = G0[2], det0 = gxx0 * gyy0 - gxy0 * gxy0;
             o.cxx = (float)(s2 * gyy0 / det0);
             o.cxy = (float)(-s2 * gxy0 / det0);
             o.cyy = (float)(s2 * gxx0 / det0);

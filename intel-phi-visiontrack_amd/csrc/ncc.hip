@@ -21,6 +21,8 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 #include "vtg_internal.hpp"
 
 namespace vtg {
@@ -884,10 +886,12 @@ int vtg_ncc_template_read(vtg_ctx *c, const vtg_tmpl *t, float *app, float *norm
     return VTG_OK;
 }
 
-int vtg_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
-                   struct vtg_cell *cells, struct vtg_found *found)
+// The search into device records: the clipped window, the map in `cells` (or the context's scratch), the pick in
+// found_dev; stream-ordered, no host wait, no allocation once the scratch has grown to the map's size.
+static int search_dev(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
+                      struct vtg_cell *cells, struct vtg_found *found_dev, bool *empty)
 {
-    if (!c || !p || !t || !found || (step != 1 && step != 2) || t->level >= p->levels)
+    if (!c || !p || !t || (step != 1 && step != 2) || t->level >= p->levels)
         return fail(c, VTG_EINVAL, "vtg_ncc_search: bad argument");
     const Level &lv = p->lv[t->level];
     const int hi_x = lv.w - t->tw - 2, hi_y = lv.h - t->th - 2;
@@ -895,8 +899,8 @@ int vtg_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int 
     x1 = x1 > hi_x ? hi_x : x1;
     y0 = y0 < 1 ? 1 : y0;
     y1 = y1 > hi_y ? hi_y : y1;
-    *found = vtg_found{0, 0, -2.0, -2.0, 0.0, 0.0, 0, 0, 0, -2.0f};
-    if (x1 < x0 || y1 < y0)
+    *empty = x1 < x0 || y1 < y0;
+    if (*empty)
         return VTG_OK;
     const int ncell = ((x1 - x0) / 16 + 1) * ((y1 - y0) / 16 + 1);
     void *sc = nullptr;
@@ -905,13 +909,48 @@ int vtg_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int 
     if (rc)
         return rc;
     vtg_cell *cl = cells ? cells : (vtg_cell *)sc;
-    vtg_found *fd = (vtg_found *)((char *)sc + cell_bytes);
-    rc = launch_ncc_search(c, p, t, x0, x1, y0, y1, step, cl, fd, (char *)sc + cell_bytes + 256);
-    if (rc)
+    vtg_found *fd = found_dev ? found_dev : (vtg_found *)((char *)sc + cell_bytes);
+    return launch_ncc_search(c, p, t, x0, x1, y0, y1, step, cl, fd, (char *)sc + cell_bytes + 256);
+}
+
+int vtg_ncc_search_async(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
+                         struct vtg_cell *cells, struct vtg_found *found_dev)
+{
+    if (!found_dev)
+        return fail(c, VTG_EINVAL, "vtg_ncc_search_async: null found record");
+    bool empty = false;
+    int rc = search_dev(c, p, t, x0, x1, y0, y1, step, cells, found_dev, &empty);
+    if (rc || !empty)
         return rc;
+    // nothing to search: the record says so (stream-ordered, no memset)
     void *host = nullptr;
-    rc = pinned(c, sizeof(vtg_found), &host);
-    if (rc)
+    if ((rc = pinned(c, sizeof(vtg_found), &host)))
+        return rc;
+    VTG_HIP(c, hipStreamSynchronize(c->stream));
+    *(vtg_found *)host = vtg_found{0, 0, -2.0, -2.0, 0.0, 0.0, 0, 0, 0, -2.0f};
+    VTG_HIP(c, hipMemcpyAsync(found_dev, host, sizeof(vtg_found), hipMemcpyHostToDevice, c->stream));
+    VTG_HIP(c, hipStreamSynchronize(c->stream));
+    return VTG_OK;
+}
+
+int vtg_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
+                   struct vtg_cell *cells, struct vtg_found *found)
+{
+    if (!found)
+        return fail(c, VTG_EINVAL, "vtg_ncc_search: null found record");
+    *found = vtg_found{0, 0, -2.0, -2.0, 0.0, 0.0, 0, 0, 0, -2.0f};
+    bool empty = false;
+    int rc = search_dev(c, p, t, x0, x1, y0, y1, step, cells, nullptr, &empty);
+    if (rc || empty)
+        return rc;
+    const Level &lv = p->lv[t->level];
+    const int hx1 = std::min(x1, lv.w - t->tw - 2), hy1 = std::min(y1, lv.h - t->th - 2);
+    const int cx0 = std::max(x0, 1), cy0 = std::max(y0, 1);
+    const int ncell = ((hx1 - cx0) / 16 + 1) * ((hy1 - cy0) / 16 + 1);
+    const size_t cell_bytes = ((size_t)ncell * sizeof(vtg_cell) + 255) & ~(size_t)255;
+    vtg_found *fd = (vtg_found *)((char *)c->scratch + cell_bytes);
+    void *host = nullptr;
+    if ((rc = pinned(c, sizeof(vtg_found), &host)))
         return rc;
     VTG_HIP(c, hipMemcpyAsync(host, fd, sizeof(vtg_found), hipMemcpyDeviceToHost, c->stream));
     VTG_HIP(c, hipStreamSynchronize(c->stream));

@@ -49,9 +49,10 @@ struct vtg_ctx {
     // scratch owned by the context (grown on demand, freed on destroy)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    hipEvent_t join = nullptr;               // for vtg_ctx_wait
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;
-    int lk_margin = 3;                       // pixels of J staged around an LK estimate
+    int lk_margin = 0;                       // LK schedule: 0 direct, > 0 stage J with this margin
 };
 
 struct vtg_pyr {

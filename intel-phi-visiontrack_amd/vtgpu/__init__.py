@@ -58,7 +58,7 @@ def lib():
     L = C.CDLL(LIB_GPU)
     P, I, F, V, S = C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_size_t
     sig = {
-        "vtg_ctx_create": [I, C.POINTER(V)], "vtg_ctx_destroy": [V], "vtg_ctx_sync": [V],
+        "vtg_ctx_create": [I, C.POINTER(V)], "vtg_ctx_destroy": [V], "vtg_ctx_sync": [V], "vtg_ctx_wait": [V, V],
         "vtg_alloc": [V, S, C.POINTER(V)], "vtg_free": [V, V], "vtg_upload": [V, V, V, S],
         "vtg_download": [V, V, V, S], "vtg_copy_d2d": [V, V, V, S],
         "vtg_timing_enable": [V, I], "vtg_timing_read": [V, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)],
@@ -74,6 +74,9 @@ def lib():
         "vtg_ncc_template_update": [V, V, V, I, I, I],
         "vtg_ncc_template_read": [V, V, V, C.POINTER(F)],
         "vtg_ncc_search": [V, V, V, I, I, I, I, I, V, C.POINTER(Found)],
+        "vtg_ncc_search_async": [V, V, V, I, I, I, I, I, V, V],
+        "vtg_graph_begin": [V], "vtg_graph_end": [V, C.POINTER(V)], "vtg_graph_launch": [V, V],
+        "vtg_graph_destroy": [V, V],
         "vtg_ncc_score": [V, V, V, I, V, I, V],
         "vtg_img_u8_to_q6": [V, C.POINTER(Plane), V, I, I, I],
         "vtg_img_reduce": [V, C.POINTER(Plane), C.POINTER(Plane), I, I],
@@ -93,11 +96,12 @@ def lib():
 
 
 # every symbol include/vtgpu.h declares
-EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_error", "vtg_ctx_stream", "vtg_alloc",
+EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", "vtg_ctx_error", "vtg_ctx_stream", "vtg_alloc",
            "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_timing_enable", "vtg_timing_read",
            "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_level",
            "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_margin", "vtg_ncc_template", "vtg_ncc_template_destroy",
-           "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_search", "vtg_ncc_score", "vtg_img_u8_to_q6", "vtg_img_reduce",
+           "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_score",
+           "vtg_graph_begin", "vtg_graph_end", "vtg_graph_launch", "vtg_graph_destroy", "vtg_img_u8_to_q6", "vtg_img_reduce",
            "vtg_lk_track_planes", "vtg_correlate_planes"]
 
 
@@ -210,6 +214,10 @@ class Context:
     def sync(self):
         self.check(self.L.vtg_ctx_sync(self.h), "sync")
 
+    def wait(self, other):
+        """Later work of this context starts after everything submitted to `other` so far."""
+        self.check(self.L.vtg_ctx_wait(self.h, other.h), "wait")
+
     # memory
     def alloc(self, nbytes):
         p = C.c_void_p()
@@ -294,6 +302,26 @@ class Context:
         self.check(self.L.vtg_ncc_search(self.h, pyr.h, tmpl.h, x0, x1, y0, y1, step,
                                          cells.ptr if cells is not None else None, C.byref(f)), "ncc_search")
         return f
+
+    def ncc_search_async(self, pyr, tmpl, x0, x1, y0, y1, step, cells, found_dev):
+        self.check(self.L.vtg_ncc_search_async(self.h, pyr.h, tmpl.h, x0, x1, y0, y1, step,
+                                               cells.ptr if cells is not None else None, found_dev.ptr),
+                   "ncc_search_async")
+
+    # graphs
+    def graph_begin(self):
+        self.check(self.L.vtg_graph_begin(self.h), "graph_begin")
+
+    def graph_end(self):
+        g = C.c_void_p()
+        self.check(self.L.vtg_graph_end(self.h, C.byref(g)), "graph_end")
+        return g
+
+    def graph_launch(self, g):
+        self.check(self.L.vtg_graph_launch(self.h, g), "graph_launch")
+
+    def graph_destroy(self, g):
+        self.check(self.L.vtg_graph_destroy(self.h, g), "graph_destroy")
 
     def ncc_score(self, pyr, tmpl, xy, half=False):
         xy = np.ascontiguousarray(xy, np.int32).ravel()

@@ -1,0 +1,94 @@
+"""Batched and captured forms give the eager single-pair results: pyramids of several pairs built together, LK of all
+pairs in one launch, searches on a second context, the whole step captured as a HIP graph and replayed several times
+with other eager work between replays; every replay is checked against the eager result."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import vtgpu
+from gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _step_objects(ctx, sctx, sizes, seed):
+    frames, pyrs, xys, offs, tm = [], [], [], [0], []
+    for i, (w, h, L, n) in enumerate(sizes):
+        f0, f1, _ = vtgpu.frame_pair(w, h, seed + i)
+        frames.append((ctx.to_device(f0), ctx.to_device(f1)))
+        pyrs.append((ctx.pyramid(w, h, L), ctx.pyramid(w, h, L)))
+        xs, ys = vtgpu.points(seed + i, n, w, h, 8)
+        xys.append(np.stack([xs, ys], 1).ravel())
+        offs.append(offs[-1] + n)
+    return frames, pyrs, np.concatenate(xys).astype(np.float32), offs
+
+
+def test_graph_replays_match_eager(ctx):
+    sctx = vtgpu.Context(0)
+    sizes = [(640, 480, 3, 300), (1280, 720, 4, 500), (1920, 1080, 4, 700)]
+    frames, pyrs, xy, offs = _step_objects(ctx, sctx, sizes, 300)
+    cfg = vtgpu.lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0)
+    dxy = ctx.to_device(xy)
+    out = ctx.alloc(32 * offs[-1])
+    fs = C.sizeof(vtgpu.Found)
+    found = sctx.alloc(fs * len(sizes))
+    # the header's bound for corners [1, w] x [1, h]: ((x1 - x0) / 16 + 1) * ((y1 - y0) / 16 + 1) records
+    cells = [sctx.alloc(12 * (((w - 1) // 16 + 1) * ((h - 1) // 16 + 1))) for w, h, _, _ in sizes]
+    tm = []
+    L = vtgpu.lib()
+
+    def enqueue():
+        ctx.build_many([p for pr in pyrs for p in pr], [f for fr in frames for f in fr])
+        sctx.wait(ctx)
+        ctx.lk_track_many([p[0] for p in pyrs], [p[1] for p in pyrs], [s[2] - 1 for s in sizes], offs, cfg, dxy,
+                          fb_max=0.5, out=out)
+        for k, (w, h, _, _) in enumerate(sizes):
+            if len(tm) <= k:
+                tm.append(sctx.ncc_template(pyrs[k][0], 0, w // 3, h // 3, 32, 32))
+            else:
+                tm[k].update(pyrs[k][0], 0, w // 3, h // 3)
+            sctx.check(L.vtg_ncc_search_async(sctx.h, pyrs[k][1].h, tm[k].h, 1, w, 1, h, 1, cells[k].ptr,
+                                              C.c_void_p(found.ptr.value + fs * k)))
+        ctx.wait(sctx)
+
+    def results():
+        ctx.sync()
+        o = out.get(np.zeros(offs[-1], vtgpu.LK_OUT_DTYPE))
+        f = (vtgpu.Found * len(sizes))()
+        ctx.check(L.vtg_download(ctx.h, C.cast(f, C.c_void_p), found.ptr, fs * len(sizes)))
+        return o.tobytes(), bytes(f)
+
+    say("batch eager")
+    enqueue()
+    ref = results()
+    # the eager batched result equals one pair at a time
+    for k, (w, h, Lv, n) in enumerate(sizes):
+        one = ctx.lk_track_host(pyrs[k][0], pyrs[k][1], Lv - 1, cfg, xy[2 * offs[k]:2 * offs[k + 1]:2],
+                                xy[2 * offs[k] + 1:2 * offs[k + 1]:2], fb_max=0.5)
+        assert one.tobytes() == ref[0][32 * offs[k]:32 * offs[k + 1]]
+        f = ctx.ncc_search(pyrs[k][1], tm[k], 1, w, 1, h, 1)
+        g = vtgpu.Found.from_buffer_copy(ref[1][fs * k:fs * (k + 1)])
+        assert (f.hx, f.hy, f.ncc, f.second, f.ox, f.oy) == (g.hx, g.hy, g.ncc, g.second, g.ox, g.oy)
+    say("batch capture")
+    ctx.graph_begin()
+    enqueue()
+    graph = ctx.graph_end()
+    other = ctx.pyramid(800, 600, 3)
+    for r in range(4):
+        say("batch replay", r)
+        # eager work in between: another pyramid and an LK call that reuses the output buffer
+        f0, _, _ = vtgpu.frame_pair(800, 600, 900 + r)
+        other.build(f0)
+        ctx.lk_track(other, other, 2, cfg, dxy, None, 0.0, out)
+        ctx.sync()
+        ctx.graph_launch(graph)
+        assert results() == ref, r
+    ctx.graph_destroy(graph)
+    for t in tm:
+        t.destroy()
+    other.destroy()
+    for p0, p1 in pyrs:
+        p0.destroy()
+        p1.destroy()
+    sctx.close()
