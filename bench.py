@@ -108,9 +108,8 @@ class Batched:
                 self.tmpl[k] = S.ncc_template(pyrs[k][0], 0, cx, cy, TPL, TPL)
             else:
                 self.tmpl[k].update(pyrs[k][0], 0, cx, cy)
-            S.check(self.v.lib().vtg_ncc_search_async(S.h, pyrs[k][1].h, self.tmpl[k].h, 1, W, 1, H, 1,
-                                                      self.cells[k].ptr, C.c_void_p(self.found.ptr.value + self.fs * k)),
-                    "ncc_search_async")
+        # the B exhaustive searches: one launch per stage for all of them
+        S.ncc_search_many([p[1] for p in pyrs], self.tmpl, [(1, W, 1, H)] * B, 1, self.found, cells=self.cells)
         A.wait(S)
 
     def prepare(self):

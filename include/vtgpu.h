@@ -164,6 +164,13 @@ int vtg_ncc_search(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int x0, in
  * wait, no allocation once the context's scratch has grown to the map's size, so it can be captured in a graph). */
 int vtg_ncc_search_async(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
                          struct vtg_cell *cells, struct vtg_found *found_dev);
+/* n searches at once, one launch per stage for each group of 8 (OP_SEARCH over several slots or streams in one
+ * round, track.c:301-329 and search_slot, track.c:1661-1726): search k covers corners region[4k .. 4k+3] =
+ * x0, x1, y0, y1 of pyramid p[k] with template t[k]. All templates have one size, all searched levels one element
+ * type. cells: NULL, or n device maps (an entry may be NULL: the context's scratch). found_dev: n device records.
+ * Stream-ordered, no host wait; a search whose window clips to nothing gets score -2 and cells 0. */
+int vtg_ncc_search_many(vtg_ctx *ctx, int n, const vtg_pyr *const *p, const vtg_tmpl *const *t, const int *region,
+                        int step, struct vtg_cell *const *cells, struct vtg_found *found_dev);
 /* Scores at n corners (tk_correlate, track.h:57-59; half != 0: half-pixel corners, search_half): xy device int
  * pairs, out device doubles: -1 outside, 0 for a window under 1/16 of the template's energy. */
 int vtg_ncc_score(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out);

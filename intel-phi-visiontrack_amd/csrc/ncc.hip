@@ -22,6 +22,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "vtg_internal.hpp"
 
@@ -140,9 +141,19 @@ __global__ __launch_bounds__(256) void ncc_template_kernel(Level lv, int x0, int
     } else {
         if (tid < 16) {
             float s = 0.0f;
-            for (int r = 0; r < th; r++)
-                for (int k = 0; k < tw; k += 16)
-                    s = s + px[r * tw + k + tid];
+            const int per = tw / 16;
+            for (int r = 0; r < th; r += 4) {
+                float z[16];
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const int rr = r + (q >> 2), kk = q & 3;
+                    z[q] = (rr < th && kk < per) ? px[rr * tw + 16 * kk + tid] : 0.0f;
+                }
+#pragma unroll
+                for (int q = 0; q < 16; q++)
+                    if (r + (q >> 2) < th && (q & 3) < per)
+                        s = s + z[q];
+            }
             part[tid] = s;
         }
     }
@@ -166,12 +177,22 @@ __global__ __launch_bounds__(256) void ncc_template_kernel(Level lv, int x0, int
     }
     __syncthreads();
     if (tid < 16) {
+        // lane tid: columns tid, tid + 16, ... of each row in row order, a sequential fp32 chain; 4 rows of loads
+        // are issued before their fused multiply-adds
         float s = 0.0f;
-        for (int r = 0; r < th; r++)
-            for (int k = 0; k < tw; k += 16) {
-                const float z = px[r * tw + k + tid];
-                s = fmaf(z, z, s);
+        const int per = tw / 16;
+        for (int r = 0; r < th; r += 4) {
+            float z[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int rr = r + (q >> 2), kk = q & 3;
+                z[q] = (rr < th && kk < per) ? px[rr * tw + 16 * kk + tid] : 0.0f;
             }
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+                if (r + (q >> 2) < th && (q & 3) < per)
+                    s = fmaf(z[q], z[q], s);
+        }
         part[tid] = s;
     }
     __syncthreads();
@@ -248,83 +269,115 @@ __global__ __launch_bounds__(256) void ncc_score_kernel(Level lv, const float *t
         out[i] = v;
 }
 
+// ---------------------------------------------------------------- a batch of searches
+
+struct Cand { float v; int idx; };
+
+constexpr int RING_MAX = 9; // half-pixel grid of one candidate: 2 (around + 1) + 1, around <= 3
+
+// What the finish kernels pass on: the top cells, then every half-pixel score around each.
+struct Aux {
+    int ntop, tops[8];
+    int pad[7];
+    double hs[8][RING_MAX * RING_MAX];
+};
+
+// One search of a batch: the level, the template, the clipped corner window and where its map and records go.
+struct SJob {
+    Level lv;
+    const float *t;          // fp32 zero-mean template
+    const float *nm;         // norm, mean
+    const long long *sum_t;  // sum of the template's bytes (u8 levels)
+    const uint32_t *words;   // the 4 pre-shifted byte copies (u8 levels)
+    int x0, x1, y0, y1;      // clipped corner window
+    int cols, rows;          // cells of the map; rows == 0: nothing to search
+    vtg_cell *cells;
+    vtg_found *found;
+    Aux *aux;
+};
+
+constexpr int SMAX = 8; // searches per launch
+
+// Searches launched together: one template size, one level element type, one step.
+struct SBatch {
+    SJob j[SMAX];
+    int n, tw, th, step;
+};
+
 // ---------------------------------------------------------------- cell map, generic: one wave per cell
 
 // Any level type, any template: the cell's corners in row-major order, four at a time, the first best kept by a
-// strict > (OP_SEARCH, track.c:301-329), scores in the reference's arithmetic.
+// strict > (OP_SEARCH, track.c:301-329), scores in the reference's arithmetic. Grid: (cells / 4, searches).
 template <typename T>
-__global__ __launch_bounds__(256) void ncc_cells_generic(Level lv, const float *t, int tw, int th, const float *nm,
-                                                         int x0, int x1, int y0, int y1, int step, int cols, int ncell,
-                                                         vtg_cell *cells)
+__global__ __launch_bounds__(256) void ncc_cells_generic(SBatch b)
 {
+    const SJob &a = b.j[blockIdx.y];
+    const int ncell = a.cols * a.rows;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int cell = blockIdx.x * 4 + wave;
     if (cell >= ncell)
         return;
-    const int cy0 = y0 + (cell / cols) * 16, cx0 = x0 + (cell % cols) * 16;
-    const int cy1 = cy0 + 16 <= y1 + 1 ? cy0 + 16 : y1 + 1, cx1 = cx0 + 16 <= x1 + 1 ? cx0 + 16 : x1 + 1;
+    const int step = b.step;
+    const int cy0 = a.y0 + (cell / a.cols) * 16, cx0 = a.x0 + (cell % a.cols) * 16;
+    const int cy1 = cy0 + 16 <= a.y1 + 1 ? cy0 + 16 : a.y1 + 1, cx1 = cx0 + 16 <= a.x1 + 1 ? cx0 + 16 : a.x1 + 1;
     const int nx = (cx1 - cx0 + step - 1) / step, ny = (cy1 - cy0 + step - 1) / step, np = nx * ny;
+    const double norm = (double)a.nm[0];
     float top = -2.0f;
     int bx = cx0, by = cy0;
-    for (int b = 0; b < np; b += 4) {
-        const int p = b + (lane >> 4);
+    for (int q0 = 0; q0 < np; q0 += 4) {
+        const int p = q0 + (lane >> 4);
         const int x = p < np ? cx0 + (p % nx) * step : -64, y = p < np ? cy0 + (p / nx) * step : -64;
-        const float v = (float)score16<T>(lv, t, tw, th, (double)nm[0], x, y, false, lane);
-        for (int q = 0; q < 4 && b + q < np; q++) {
+        const float v = (float)score16<T>(a.lv, a.t, b.tw, b.th, norm, x, y, false, lane);
+        for (int q = 0; q < 4 && q0 + q < np; q++) {
             const float vq = __shfl(v, 16 * q);
             if (vq > top)
-                top = vq, bx = cx0 + ((b + q) % nx) * step, by = cy0 + ((b + q) / nx) * step;
+                top = vq, bx = cx0 + ((q0 + q) % nx) * step, by = cy0 + ((q0 + q) / nx) * step;
         }
     }
     if (lane == 0)
-        cells[cell] = vtg_cell{top, bx, by};
+        a.cells[cell] = vtg_cell{top, bx, by};
 }
 
 // ---------------------------------------------------------------- cell map, fast: u8 level, dot4
-
-struct FastArgs {
-    const uint8_t *img;
-    int w, h, pitch;
-    const uint32_t *words;
-    int tw, th;
-    int x0, x1, y0, y1, step, cols, rows;
-    const float *nm; // norm, mean
-    const long long *sum_t;
-    vtg_cell *cells;
-};
 
 __device__ __forceinline__ bool better(float v, int y, int x, float v2, int y2, int x2)
 {
     return v > v2 || (v == v2 && (y < y2 || (y == y2 && x < x2)));
 }
 
+// Grid: (column blocks of 16 cells, cell rows, searches); blocks past a search's own map leave at once.
 template <int WPR>
-__global__ __launch_bounds__(256) void ncc_cells_fast(FastArgs a)
+__global__ __launch_bounds__(256) void ncc_cells_fast(SBatch b)
 {
     extern __shared__ uint32_t lds[];
     __shared__ float cv[4][16];
     __shared__ int cx[4][16], cyy[4][16];
+    const SJob &a = b.j[blockIdx.z];
+    if ((int)blockIdx.x * 16 >= a.cols || (int)blockIdx.y >= a.rows)
+        return; // the whole workgroup: no barrier is pending
+    const uint8_t *img = (const uint8_t *)a.lv.p;
+    const int ih = a.lv.h, pitch = a.lv.pitch, tw = b.tw, th = b.th, stp = b.step;
     constexpr int LW = NX / 4 + WPR + 1;
-    const int LR = NY + a.th - 1;
+    const int LR = NY + th - 1;
     const int X0 = a.x0 + blockIdx.x * NX, Y0 = a.y0 + blockIdx.y * NY;
     const int A = X0 & ~3, off = X0 - A;
     for (int i = threadIdx.x; i < LR * LW; i += 256) {
         const int r = i / LW, c = i - r * LW;
-        const int gy = Y0 + r < a.h ? Y0 + r : a.h - 1;
+        const int gy = Y0 + r < ih ? Y0 + r : ih - 1;
         const int gx = A + 4 * c;
         uint32_t v = 0;
-        if (gx + 3 < a.pitch)
-            v = *(const uint32_t *)(a.img + (size_t)gy * a.pitch + gx);
+        if (gx + 3 < pitch)
+            v = *(const uint32_t *)(img + (size_t)gy * pitch + gx);
         lds[i] = v;
     }
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int sh = (off + wave) & 3, wb = (off + wave) >> 2;
-    const uint32_t *T = a.words + (size_t)sh * a.th * WPR;
+    const uint32_t *T = a.words + (size_t)sh * th * WPR;
     const uint32_t mask0 = 0xffffffffu << (8 * sh), maskl = sh ? (0xffffffffu >> (32 - 8 * sh)) : 0u;
     const int x = X0 + 4 * lane + wave;
-    const bool x_ok = x <= a.x1 && ((x - a.x0) % a.step) == 0;
-    const double n = (double)(a.tw * a.th), norm = (double)a.nm[0], mean = (double)a.nm[1];
+    const bool x_ok = x <= a.x1 && ((x - a.x0) % stp) == 0;
+    const double n = (double)(tw * th), norm = (double)a.nm[0], mean = (double)a.nm[1];
     const double sum_t = (double)a.sum_t[0];
     float best_v = -3.0f;
     int best_x = 0, best_y = 0;
@@ -383,7 +436,7 @@ __global__ __launch_bounds__(256) void ncc_cells_fast(FastArgs a)
             }
         };
         int r = 0;
-        for (; r + YB < a.th; r += YB) {
+        for (; r + YB < th; r += YB) {
 #pragma unroll
             for (int j = 0; j < YB; j++)
                 step(r + j, j, false);
@@ -406,7 +459,7 @@ __global__ __launch_bounds__(256) void ncc_cells_fast(FastArgs a)
             const double var = sjj - sj * sj / n;
             const double sc = (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : st / sqrt(norm * var);
             const float v = (float)sc;
-            const bool ok = x_ok && y <= a.y1 && ((y - a.y0) % a.step) == 0;
+            const bool ok = x_ok && y <= a.y1 && ((y - a.y0) % stp) == 0;
             if (ok && better(v, y, x, best_v, best_y, best_x))
                 best_v = v, best_x = x, best_y = y;
         }
@@ -440,21 +493,10 @@ __global__ __launch_bounds__(256) void ncc_cells_fast(FastArgs a)
 
 // ---------------------------------------------------------------- finish: top 8, half-pixel refinement, peaks
 
-struct Cand { float v; int idx; };
-
 __device__ __forceinline__ bool before(const Cand &a, const Cand &b)
 {
     return a.v > b.v || (a.v == b.v && a.idx < b.idx);
 }
-
-constexpr int RING_MAX = 9; // half-pixel grid of one candidate: 2 (around + 1) + 1, around <= 3
-
-// What the finish kernels pass on: the top cells, then every half-pixel score around each.
-struct Aux {
-    int ntop, tops[8];
-    int pad[7];
-    double hs[8][RING_MAX * RING_MAX];
-};
 
 // A cell's rank key: smaller is better. High word: the score, order-inverted (higher score, smaller key); low word:
 // the cell index (earlier cell, smaller key): exactly the order search_slot's insertion keeps (track.c:1686-1694).
@@ -467,21 +509,47 @@ __device__ __forceinline__ unsigned long long cell_key(float v, int idx)
 
 __device__ __forceinline__ unsigned long long min64(unsigned long long a, unsigned long long b) { return a < b ? a : b; }
 
+constexpr int TOP_REG = 32; // keys per thread held in registers: maps of up to 32,768 cells are read once
+
 // The 8 best cells, scores above -2 only: 8 rounds of a workgroup-wide minimum of the keys above the last one taken.
-__global__ __launch_bounds__(1024) void ncc_top8(const vtg_cell *cells, int ncell, Aux *aux)
+// Grid: one workgroup per search.
+__global__ __launch_bounds__(1024) void ncc_top8(SBatch b)
 {
     __shared__ unsigned long long wmin[16];
     __shared__ unsigned long long taken;
+    const SJob &a = b.j[blockIdx.x];
+    const vtg_cell *cells = a.cells;
+    const int ncell = a.cols * a.rows;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool inreg = ncell <= 1024 * TOP_REG;
+    unsigned long long key[TOP_REG];
+    if (inreg) {
+        float v[TOP_REG];
+#pragma unroll
+        for (int q = 0; q < TOP_REG; q++) {
+            const int i = tid + 1024 * q;
+            v[q] = i < ncell ? cells[i].v : -3.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < TOP_REG; q++)
+            key[q] = v[q] > -2.0f ? cell_key(v[q], tid + 1024 * q) : ~0ull;
+    }
     unsigned long long prev = 0;
     int n = 0;
     for (int round = 0; round < 8; round++) {
         unsigned long long m = ~0ull;
-        for (int i = tid; i < ncell; i += 1024) {
-            const float v = cells[i].v;
-            const unsigned long long k = cell_key(v, i);
-            if (v > -2.0f && (round == 0 || k > prev))
-                m = min64(m, k);
+        if (inreg) {
+#pragma unroll
+            for (int q = 0; q < TOP_REG; q++)
+                if (round == 0 || key[q] > prev)
+                    m = min64(m, key[q]);
+        } else {
+            for (int i = tid; i < ncell; i += 1024) {
+                const float v = cells[i].v;
+                const unsigned long long k = cell_key(v, i);
+                if (v > -2.0f && (round == 0 || k > prev))
+                    m = min64(m, k);
+            }
         }
         for (int o = 32; o > 0; o >>= 1) {
             const unsigned int lo = __shfl_xor((unsigned int)m, o), hi = __shfl_xor((unsigned int)(m >> 32), o);
@@ -501,49 +569,61 @@ __global__ __launch_bounds__(1024) void ncc_top8(const vtg_cell *cells, int ncel
         if (prev == ~0ull)
             break;
         if (tid == 0)
-            aux->tops[round] = (int)(unsigned int)prev;
+            a.aux->tops[round] = (int)(unsigned int)prev;
         n++;
         __syncthreads(); // taken is rewritten next round
     }
     if (tid == 0)
-        aux->ntop = n;
+        a.aux->ntop = n;
 }
 
 // The score at half-pixel corner (hx, hy) from a staged region: pixel (x, y) of the level at reg[(y - ry) * rw + x - rx],
-// template in LDS too; same arithmetic as score16.
-__device__ double score16_lds(const float *reg, int rx, int ry, int rw, const float *t, int tw, int th, double norm,
-                              int lw, int lh, int hx, int hy, int lane)
+// template (16 TWQ wide) in LDS too; same arithmetic as score16. The half-pixel averages are selected rather than
+// branched on, so the four corners of a wave do not diverge; every load of a row is issued before its sums.
+template <int TWQ>
+__device__ double score16_lds(const float *reg, int rx, int ry, int rw, const float *t, int th, double norm, int lw,
+                              int lh, int hx, int hy, int lane)
 {
+    constexpr int tw = 16 * TWQ;
     const int j = lane & 15;
-    const int x = hx >> 1, y = hy >> 1, ox = hx & 1, oy = hy & 1;
+    const int x = hx >> 1, y = hy >> 1;
+    const bool ox = hx & 1, oy = hy & 1;
     const bool fits = !(x < 1 || y < 1 || x + tw + 1 >= lw - 1 || y + th + 1 >= lh - 1);
     float st = 0.0f, sj = 0.0f, sjj = 0.0f;
     if (fits) {
-        const float *base = reg + (y - ry) * rw + (x - rx);
-        const float first = base[0];
+        const float *p = reg + (y - ry) * rw + (x - rx);
+        const float first = p[0];
+        p += j;
+        const float *tp = t + j;
+#pragma unroll 2
         for (int r = 0; r < th; r++) {
-            const float *row = base + r * rw + j;
-            for (int k = 0; k < tw; k += 16) {
-                float v = row[k];
-                if (ox)
-                    v = (v + row[k + 1]) * 0.5f;
-                if (oy) {
-                    float w = row[k + rw];
-                    if (ox)
-                        w = (w + row[k + rw + 1]) * 0.5f;
-                    v = (v + w) * 0.5f;
-                }
+            float a0[TWQ], a1[TWQ], b0[TWQ], b1[TWQ], tt[TWQ];
+#pragma unroll
+            for (int q = 0; q < TWQ; q++) {
+                a0[q] = p[16 * q];
+                a1[q] = p[16 * q + 1];
+                b0[q] = p[16 * q + rw];
+                b1[q] = p[16 * q + rw + 1];
+                tt[q] = tp[16 * q];
+            }
+#pragma unroll
+            for (int q = 0; q < TWQ; q++) {
+                float v = ox ? (a0[q] + a1[q]) * 0.5f : a0[q];
+                const float w = ox ? (b0[q] + b1[q]) * 0.5f : b0[q];
+                v = oy ? (v + w) * 0.5f : v;
                 v = v - first;
-                st = fmaf(v, t[r * tw + k + j], st);
+                st = fmaf(v, tt[q], st);
                 sj = sj + v;
                 sjj = fmaf(v, v, sjj);
             }
+            p += rw;
+            tp += tw;
         }
     }
-    for (int step = 1; step < 16; step <<= 1) {
-        st = st + __shfl_xor(st, step);
-        sj = sj + __shfl_xor(sj, step);
-        sjj = sjj + __shfl_xor(sjj, step);
+    for (int s = 1; s < 16; s <<= 1) {
+        st = st + __shfl_xor(st, s);
+        sj = sj + __shfl_xor(sj, s);
+        sjj = sjj + __shfl_xor(sjj, s);
     }
     if (!fits)
         return -1.0;
@@ -552,20 +632,23 @@ __device__ double score16_lds(const float *reg, int rx, int ry, int rw, const fl
     return (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : a / sqrt(norm * var);
 }
 
-// One workgroup per top cell: every half-pixel corner within around + 1 of twice its best corner (the inner
-// 2 around + 1 square is search_slot's refinement, track.c:1697-1711; the ring gives take_back's neighbours,
-// track.c:1737-1741), the window and the template staged in LDS, 16 corners at a time.
-template <typename T>
-__global__ __launch_bounds__(256) void ncc_refine(Level lv, const float *t, int tw, int th, const float *nm,
-                                                  const vtg_cell *cells, int step, Aux *aux)
+// One workgroup per top cell of each search (grid: 8, searches): every half-pixel corner within around + 1 of twice
+// its best corner (the inner 2 around + 1 square is search_slot's refinement, track.c:1697-1711; the ring gives
+// take_back's neighbours, track.c:1737-1741), the window and the template staged in LDS, 16 corners at a time.
+template <typename T, int TWQ>
+__global__ __launch_bounds__(256) void ncc_refine(SBatch b)
 {
     extern __shared__ float sm[];
+    const SJob &a = b.j[blockIdx.y];
     const int c = blockIdx.x;
-    if (c >= aux->ntop)
+    if (a.rows == 0 || c >= a.aux->ntop)
         return;
+    constexpr int tw = 16 * TWQ;
+    const int th = b.th, step = b.step;
+    const Level lv = a.lv;
     const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 4;
     const int around = step > 1 ? 2 * step - 1 : 1, R = around + 1, side = 2 * R + 1;
-    const vtg_cell cl = cells[aux->tops[c]];
+    const vtg_cell cl = a.cells[a.aux->tops[c]];
     const int cx = 2 * cl.x, cy = 2 * cl.y;
     const int rx = (cx - R) >> 1, ry = (cy - R) >> 1;
     const int rw = ((cx + R) >> 1) - rx + tw + 2, rh = ((cy + R) >> 1) - ry + th + 2;
@@ -579,7 +662,7 @@ __global__ __launch_bounds__(256) void ncc_refine(Level lv, const float *t, int 
 #pragma unroll
         for (int k = 0; k < 16; k++) {
             const int i = tid + 256 * k;
-            tv[k] = t[i < nt ? i : nt - 1];
+            tv[k] = a.t[i < nt ? i : nt - 1];
         }
 #pragma unroll
         for (int k = 0; k < 24; k++) {
@@ -599,70 +682,85 @@ __global__ __launch_bounds__(256) void ncc_refine(Level lv, const float *t, int 
                 reg[tid + 256 * k] = rv[k];
     }
     __syncthreads();
-    const double norm = (double)nm[0];
+    const double norm = (double)a.nm[0];
     for (int q0 = 0; q0 < side * side; q0 += 16) {
         const int q = q0 + grp;
         const int hy = cy - R + q / side, hx = cx - R + q % side;
-        const double v = score16_lds(reg, rx, ry, rw, tl, tw, th, norm, lv.w, lv.h, q < side * side ? hx : -64,
-                                     hy, lane);
+        const double v =
+            score16_lds<TWQ>(reg, rx, ry, rw, tl, th, norm, lv.w, lv.h, q < side * side ? hx : -64, hy, lane);
         if (q < side * side && (lane & 15) == 0)
-            aux->hs[c][q] = v;
+            a.aux->hs[c][q] = v;
     }
 }
 
 // The pick, the second peak clear of it by half a template (search_slot, track.c:1704-1725) and the parabola offsets
 // from the pick's ring (take_back, track.c:1737-1741; peak_offset, 1650-1657). Selection only: no scoring here.
-__global__ __launch_bounds__(64) void ncc_final(const vtg_cell *cells, int ncell, int tw, int th, int step,
-                                                const Aux *aux_g, vtg_found *found)
+// Grid: one 64-lane workgroup per search. A search with nothing to search gets the empty record (score -2).
+__global__ __launch_bounds__(64) void ncc_final(SBatch b)
 {
     __shared__ Aux ax;
     __shared__ vtg_cell tc[8];
+    __shared__ double scs[8];
+    __shared__ int sxs[8], sys[8];
+    const SJob &a = b.j[blockIdx.x];
+    const int ncell = a.cols * a.rows, tw = b.tw, th = b.th, step = b.step;
     {
-        // the record into LDS with every lane, then one lane decides
-        const int nwords = (int)(sizeof(Aux) / 4);
-        const unsigned int *src = (const unsigned int *)aux_g;
+        // the record into LDS, every load issued before the first store
+        constexpr int NW = (int)(sizeof(Aux) / 4), PER = (NW + 63) / 64;
+        const unsigned int *src = (const unsigned int *)a.aux;
         unsigned int *dst = (unsigned int *)&ax;
-        for (int i = threadIdx.x; i < nwords; i += 64)
-            dst[i] = src[i];
-        __syncthreads();
-        if (threadIdx.x < 8 && (int)threadIdx.x < ax.ntop)
-            tc[threadIdx.x] = cells[ax.tops[threadIdx.x]];
-        __syncthreads();
+        unsigned int v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = threadIdx.x + 64 * k;
+            v[k] = i < NW ? src[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = threadIdx.x + 64 * k;
+            if (i < NW)
+                dst[i] = v[k];
+        }
     }
-    if (threadIdx.x != 0)
-        return;
-    const Aux *aux = &ax;
-    const int ntop = aux->ntop;
+    __syncthreads();
+    const int ntop = a.rows == 0 ? 0 : ax.ntop;
     const int around = step > 1 ? 2 * step - 1 : 1, R = around + 1, side = 2 * R + 1;
-    double fn = -2.0, sec = -2.0;
-    int fhx = 0, fhy = 0, fc = -1;
-    double sc[8];
-    int shx[8], shy[8];
-    for (int c = 0; c < ntop; c++) {
-        const vtg_cell cl = tc[c];
+    if ((int)threadIdx.x < ntop) {
+        // lane c: candidate c's best half-pixel corner, first in row-major order
+        const int c = threadIdx.x;
+        const vtg_cell cl = a.cells[ax.tops[c]];
+        tc[c] = cl;
         const int cx = 2 * cl.x, cy = 2 * cl.y;
-        sc[c] = -2.0;
-        shx[c] = cx;
-        shy[c] = cy;
+        double s = -2.0;
+        int bx = cx, by = cy;
         for (int hy = cy - around; hy <= cy + around; hy++)
             for (int hx = cx - around; hx <= cx + around; hx++) {
-                const double v = aux->hs[c][(hy - cy + R) * side + hx - cx + R];
-                if (v > sc[c])
-                    sc[c] = v, shx[c] = hx, shy[c] = hy;
+                const double v = ax.hs[c][(hy - cy + R) * side + hx - cx + R];
+                if (v > s)
+                    s = v, bx = hx, by = hy;
             }
-        if (sc[c] > fn)
-            fn = sc[c], fhx = shx[c], fhy = shy[c], fc = c;
+        scs[c] = s;
+        sxs[c] = bx;
+        sys[c] = by;
     }
+    __syncthreads();
+    if (threadIdx.x != 0)
+        return;
+    double fn = -2.0, sec = -2.0;
+    int fhx = 0, fhy = 0, fc = -1;
+    for (int c = 0; c < ntop; c++)
+        if (scs[c] > fn)
+            fn = scs[c], fhx = sxs[c], fhy = sys[c], fc = c;
     for (int c = 0; c < ntop; c++) {
-        const int dx = shx[c] - fhx, dy = shy[c] - fhy;
-        if ((dx * dx > tw * tw || dy * dy > th * th) && sc[c] > sec)
-            sec = sc[c];
+        const int dx = sxs[c] - fhx, dy = sys[c] - fhy;
+        if ((dx * dx > tw * tw || dy * dy > th * th) && scs[c] > sec)
+            sec = scs[c];
     }
-    auto peak = [](double a, double m, double b) {
-        const double den = a - 2.0 * m + b;
+    auto peak = [](double l, double m, double r) {
+        const double den = l - 2.0 * m + r;
         if (den >= 0.0)
             return 0.0;
-        const double o = 0.5 * (a - b) / den;
+        const double o = 0.5 * (l - r) / den;
         return o < -0.5 ? -0.5 : o > 0.5 ? 0.5 : o;
     };
     vtg_found f;
@@ -677,15 +775,15 @@ __global__ __launch_bounds__(64) void ncc_final(const vtg_cell *cells, int ncell
     if (fc >= 0) {
         const vtg_cell cl = tc[fc];
         const int ix = fhx - 2 * cl.x + R, iy = fhy - 2 * cl.y + R;
-        const double *g = aux->hs[fc];
+        const double *g = ax.hs[fc];
         f.ox = peak(g[iy * side + ix - 1], fn, g[iy * side + ix + 1]);
         f.oy = peak(g[(iy - 1) * side + ix], fn, g[(iy + 1) * side + ix]);
-        const vtg_cell b = tc[0];
-        f.best_x = b.x;
-        f.best_y = b.y;
-        f.best_v = b.v;
+        const vtg_cell bc = tc[0];
+        f.best_x = bc.x;
+        f.best_y = bc.y;
+        f.best_v = bc.v;
     }
-    *found = f;
+    *a.found = f;
 }
 
 template <typename F>
@@ -736,73 +834,157 @@ int launch_ncc_score(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int n, con
     });
 }
 
-size_t search_scratch_bytes(int ncell)
-{
-    return (((size_t)ncell * sizeof(vtg_cell) + 255) & ~(size_t)255) + 256 + ((sizeof(Aux) + 255) & ~(size_t)255);
-}
+static size_t aux_bytes() { return (sizeof(Aux) + 255) & ~(size_t)255; }
 
-// cells: the map; found, aux: device records after it in the same scratch.
-int launch_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
-                      vtg_cell *cells, vtg_found *found, void *aux_mem)
+// One launch per stage for up to SMAX searches that share template size, level element type and step.
+static int launch_batch(vtg_ctx *c, const vtg_pyr *p0, int level, bool fast, int wpr, SBatch &b)
 {
-    const Level lv = p->lv[t->level];
-    const int cols = (x1 - x0) / 16 + 1, rows = (y1 - y0) / 16 + 1, ncell = cols * rows;
-    const float *nm = (const float *)t->block;
-    const long long *su = (const long long *)((const char *)t->block + 16);
-    Aux *aux = (Aux *)aux_mem;
-    const bool fast = p->elem[t->level] == 1 && t->words && (t->th % YB) == 0 && t->th >= YB &&
-                      (t->words_per_row == 17 || t->words_per_row == 9 || t->words_per_row == 5);
+    int maxcols = 0, maxrows = 0, maxcell = 0;
+    for (int k = 0; k < b.n; k++) {
+        maxcols = std::max(maxcols, b.j[k].cols);
+        maxrows = std::max(maxrows, b.j[k].rows);
+        maxcell = std::max(maxcell, b.j[k].cols * b.j[k].rows);
+    }
     hipEvent_t ev;
-    if (fast) {
-        FastArgs a{(const uint8_t *)lv.p, lv.w, lv.h, lv.pitch, t->words, t->tw, t->th, x0, x1, y0, y1, step, cols,
-                   rows, nm, su, cells};
-        const dim3 grid((cols + 15) / 16, rows);
-        const int lw = NX / 4 + t->words_per_row + 1;
-        const size_t lds = (size_t)(NY + t->th - 1) * lw * sizeof(uint32_t);
-        timer_begin(c, "ncc_cells", &ev);
-        if (t->words_per_row == 17)
-            hipLaunchKernelGGL(ncc_cells_fast<17>, grid, dim3(256), lds, c->stream, a);
-        else if (t->words_per_row == 9)
-            hipLaunchKernelGGL(ncc_cells_fast<9>, grid, dim3(256), lds, c->stream, a);
-        else
-            hipLaunchKernelGGL(ncc_cells_fast<5>, grid, dim3(256), lds, c->stream, a);
-        VTG_HIP(c, hipGetLastError());
-        timer_end(c, "ncc_cells", ev);
-    } else {
-        int rc = by_level_type(c, p, t->level, [&](auto *tag) -> int {
-            using T = std::remove_pointer_t<decltype(tag)>;
-            timer_begin(c, "ncc_cells_generic", &ev);
-            hipLaunchKernelGGL(ncc_cells_generic<T>, dim3((ncell + 3) / 4), dim3(256), 0, c->stream, lv, t->t, t->tw,
-                               t->th, nm, x0, x1, y0, y1, step, cols, ncell, cells);
+    if (maxcell > 0) {
+        if (fast) {
+            const dim3 grid((maxcols + 15) / 16, maxrows, b.n);
+            const int lw = NX / 4 + wpr + 1;
+            const size_t lds = (size_t)(NY + b.th - 1) * lw * sizeof(uint32_t);
+            timer_begin(c, "ncc_cells", &ev);
+            if (wpr == 17)
+                hipLaunchKernelGGL(ncc_cells_fast<17>, grid, dim3(256), lds, c->stream, b);
+            else if (wpr == 9)
+                hipLaunchKernelGGL(ncc_cells_fast<9>, grid, dim3(256), lds, c->stream, b);
+            else
+                hipLaunchKernelGGL(ncc_cells_fast<5>, grid, dim3(256), lds, c->stream, b);
             VTG_HIP(c, hipGetLastError());
-            timer_end(c, "ncc_cells_generic", ev);
+            timer_end(c, "ncc_cells", ev);
+        } else {
+            int rc = by_level_type(c, p0, level, [&](auto *tag) -> int {
+                using T = std::remove_pointer_t<decltype(tag)>;
+                timer_begin(c, "ncc_cells_generic", &ev);
+                hipLaunchKernelGGL(ncc_cells_generic<T>, dim3((maxcell + 3) / 4, b.n), dim3(256), 0, c->stream, b);
+                VTG_HIP(c, hipGetLastError());
+                timer_end(c, "ncc_cells_generic", ev);
+                return VTG_OK;
+            });
+            if (rc)
+                return rc;
+        }
+    }
+    timer_begin(c, "ncc_top8", &ev);
+    hipLaunchKernelGGL(ncc_top8, dim3(b.n), dim3(1024), 0, c->stream, b);
+    VTG_HIP(c, hipGetLastError());
+    timer_end(c, "ncc_top8", ev);
+    if (maxcell > 0) {
+        const int around = b.step > 1 ? 2 * b.step - 1 : 1, R = around + 1;
+        const int rmax = R + 1 + 2; // region extent beyond the template, whole pixels, each way
+        const size_t lds =
+            ((size_t)b.tw * b.th + (size_t)(b.tw + rmax + 1) * (b.th + rmax + 1)) * sizeof(float);
+        int rc = by_level_type(c, p0, level, [&](auto *tag) -> int {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            const dim3 grid(8, b.n);
+            timer_begin(c, "ncc_refine", &ev);
+            switch (b.tw / 16) {
+            case 1: hipLaunchKernelGGL((ncc_refine<T, 1>), grid, dim3(256), lds, c->stream, b); break;
+            case 2: hipLaunchKernelGGL((ncc_refine<T, 2>), grid, dim3(256), lds, c->stream, b); break;
+            case 3: hipLaunchKernelGGL((ncc_refine<T, 3>), grid, dim3(256), lds, c->stream, b); break;
+            default: hipLaunchKernelGGL((ncc_refine<T, 4>), grid, dim3(256), lds, c->stream, b); break;
+            }
+            VTG_HIP(c, hipGetLastError());
+            timer_end(c, "ncc_refine", ev);
             return VTG_OK;
         });
         if (rc)
             return rc;
     }
-    timer_begin(c, "ncc_top8", &ev);
-    hipLaunchKernelGGL(ncc_top8, dim3(1), dim3(1024), 0, c->stream, cells, ncell, aux);
-    VTG_HIP(c, hipGetLastError());
-    timer_end(c, "ncc_top8", ev);
-    const int around = step > 1 ? 2 * step - 1 : 1, R = around + 1;
-    const int rmax = R + 1 + 2; // region extent beyond the template, whole pixels, each way
-    const size_t lds = ((size_t)t->tw * t->th + (size_t)(t->tw + rmax + 1) * (t->th + rmax + 1)) * sizeof(float);
-    int rc = by_level_type(c, p, t->level, [&](auto *tag) -> int {
-        using T = std::remove_pointer_t<decltype(tag)>;
-        timer_begin(c, "ncc_refine", &ev);
-        hipLaunchKernelGGL(ncc_refine<T>, dim3(8), dim3(256), lds, c->stream, lv, t->t, t->tw, t->th, nm, cells, step,
-                           aux);
-        VTG_HIP(c, hipGetLastError());
-        timer_end(c, "ncc_refine", ev);
-        return VTG_OK;
-    });
-    if (rc)
-        return rc;
     timer_begin(c, "ncc_final", &ev);
-    hipLaunchKernelGGL(ncc_final, dim3(1), dim3(64), 0, c->stream, cells, ncell, t->tw, t->th, step, aux, found);
+    hipLaunchKernelGGL(ncc_final, dim3(b.n), dim3(64), 0, c->stream, b);
     VTG_HIP(c, hipGetLastError());
     timer_end(c, "ncc_final", ev);
+    return VTG_OK;
+}
+
+// n searches into device records: each window clipped to where the template fits (as search_slot clips), maps in
+// cells[k] or the context's scratch, picks in found_dev[k] or, when found_dev is null, in the scratch (returned in
+// *found_scratch). Stream-ordered; no allocation once the scratch has grown to the batch's size.
+static int search_many(vtg_ctx *c, int n, const vtg_pyr *const *p, const vtg_tmpl *const *t, const int *region, int step,
+                vtg_cell *const *cells, vtg_found *found_dev, vtg_found **found_scratch)
+{
+    if (!c)
+        return VTG_EINVAL;
+    if (n < 1 || !p || !t || !region || (step != 1 && step != 2))
+        return fail(c, VTG_EINVAL, "vtg_ncc_search: bad argument");
+    const vtg_tmpl *t0 = t[0];
+    if (!p[0] || !t0 || t0->level >= p[0]->levels)
+        return fail(c, VTG_EINVAL, "vtg_ncc_search: bad pyramid or template");
+    const int level0 = t0->level;
+    for (int k = 0; k < n; k++) {
+        if (!p[k] || !t[k] || t[k]->level >= p[k]->levels)
+            return fail(c, VTG_EINVAL, "vtg_ncc_search: bad pyramid or template %d", k);
+        if (t[k]->tw != t0->tw || t[k]->th != t0->th || (t[k]->words != nullptr) != (t0->words != nullptr) ||
+            p[k]->storage != p[0]->storage || p[k]->elem[t[k]->level] != p[0]->elem[level0])
+            return fail(c, VTG_EINVAL, "vtg_ncc_search: search %d differs from the first in template size or level "
+                                       "type", k);
+    }
+    const bool fast = p[0]->elem[level0] == 1 && t0->words && (t0->th % YB) == 0 && t0->th >= YB &&
+                      (t0->words_per_row == 17 || t0->words_per_row == 9 || t0->words_per_row == 5);
+    // the scratch: aux records, then found records (when the caller has none), then maps (for searches without one)
+    std::vector<int> win((size_t)4 * n), ncell(n);
+    const size_t auxb = aux_bytes();
+    size_t bytes = (size_t)n * auxb + (found_dev ? 0 : (size_t)n * 256);
+    for (int k = 0; k < n; k++) {
+        const Level &lv = p[k]->lv[t[k]->level];
+        const int hi_x = lv.w - t[k]->tw - 2, hi_y = lv.h - t[k]->th - 2;
+        int x0 = region[4 * k], x1 = region[4 * k + 1], y0 = region[4 * k + 2], y1 = region[4 * k + 3];
+        x0 = x0 < 1 ? 1 : x0;
+        x1 = x1 > hi_x ? hi_x : x1;
+        y0 = y0 < 1 ? 1 : y0;
+        y1 = y1 > hi_y ? hi_y : y1;
+        win[4 * k] = x0, win[4 * k + 1] = x1, win[4 * k + 2] = y0, win[4 * k + 3] = y1;
+        ncell[k] = (x1 < x0 || y1 < y0) ? 0 : ((x1 - x0) / 16 + 1) * ((y1 - y0) / 16 + 1);
+        if (!(cells && cells[k]))
+            bytes += ((size_t)ncell[k] * sizeof(vtg_cell) + 255) & ~(size_t)255;
+    }
+    void *sc = nullptr;
+    int rc = scratch(c, bytes, &sc);
+    if (rc)
+        return rc;
+    char *aux0 = (char *)sc, *fnd0 = aux0 + (size_t)n * auxb;
+    char *cel = fnd0 + (found_dev ? 0 : (size_t)n * 256);
+    if (found_scratch)
+        *found_scratch = found_dev ? nullptr : (vtg_found *)fnd0;
+    for (int k0 = 0; k0 < n; k0 += SMAX) {
+        SBatch b{};
+        b.n = std::min(SMAX, n - k0);
+        b.tw = t0->tw;
+        b.th = t0->th;
+        b.step = step;
+        for (int i = 0; i < b.n; i++) {
+            const int k = k0 + i;
+            SJob &j = b.j[i];
+            const int *w = &win[4 * k];
+            j.lv = p[k]->lv[t[k]->level];
+            j.t = t[k]->t;
+            j.nm = (const float *)t[k]->block;
+            j.sum_t = (const long long *)((const char *)t[k]->block + 16);
+            j.words = t[k]->words;
+            j.x0 = w[0], j.x1 = w[1], j.y0 = w[2], j.y1 = w[3];
+            j.cols = ncell[k] ? (w[1] - w[0]) / 16 + 1 : 0;
+            j.rows = ncell[k] ? (w[3] - w[2]) / 16 + 1 : 0;
+            if (cells && cells[k]) {
+                j.cells = cells[k];
+            } else {
+                j.cells = (vtg_cell *)cel;
+                cel += ((size_t)ncell[k] * sizeof(vtg_cell) + 255) & ~(size_t)255;
+            }
+            j.found = found_dev ? found_dev + k : (vtg_found *)(fnd0 + (size_t)k * 256);
+            j.aux = (Aux *)(aux0 + (size_t)k * auxb);
+        }
+        if ((rc = launch_batch(c, p[0], level0, fast, t0->words_per_row, b)))
+            return rc;
+    }
     return VTG_OK;
 }
 
@@ -886,31 +1068,12 @@ int vtg_ncc_template_read(vtg_ctx *c, const vtg_tmpl *t, float *app, float *norm
     return VTG_OK;
 }
 
-// The search into device records: the clipped window, the map in `cells` (or the context's scratch), the pick in
-// found_dev; stream-ordered, no host wait, no allocation once the scratch has grown to the map's size.
-static int search_dev(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
-                      struct vtg_cell *cells, struct vtg_found *found_dev, bool *empty)
+int vtg_ncc_search_many(vtg_ctx *c, int n, const vtg_pyr *const *p, const vtg_tmpl *const *t, const int *region,
+                        int step, struct vtg_cell *const *cells, struct vtg_found *found_dev)
 {
-    if (!c || !p || !t || (step != 1 && step != 2) || t->level >= p->levels)
-        return fail(c, VTG_EINVAL, "vtg_ncc_search: bad argument");
-    const Level &lv = p->lv[t->level];
-    const int hi_x = lv.w - t->tw - 2, hi_y = lv.h - t->th - 2;
-    x0 = x0 < 1 ? 1 : x0;
-    x1 = x1 > hi_x ? hi_x : x1;
-    y0 = y0 < 1 ? 1 : y0;
-    y1 = y1 > hi_y ? hi_y : y1;
-    *empty = x1 < x0 || y1 < y0;
-    if (*empty)
-        return VTG_OK;
-    const int ncell = ((x1 - x0) / 16 + 1) * ((y1 - y0) / 16 + 1);
-    void *sc = nullptr;
-    const size_t cell_bytes = ((size_t)ncell * sizeof(vtg_cell) + 255) & ~(size_t)255;
-    int rc = scratch(c, search_scratch_bytes(ncell), &sc);
-    if (rc)
-        return rc;
-    vtg_cell *cl = cells ? cells : (vtg_cell *)sc;
-    vtg_found *fd = found_dev ? found_dev : (vtg_found *)((char *)sc + cell_bytes);
-    return launch_ncc_search(c, p, t, x0, x1, y0, y1, step, cl, fd, (char *)sc + cell_bytes + 256);
+    if (!found_dev)
+        return fail(c, VTG_EINVAL, "vtg_ncc_search_many: null found records");
+    return search_many(c, n, p, t, region, step, cells, found_dev, nullptr);
 }
 
 int vtg_ncc_search_async(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
@@ -918,19 +1081,9 @@ int vtg_ncc_search_async(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0
 {
     if (!found_dev)
         return fail(c, VTG_EINVAL, "vtg_ncc_search_async: null found record");
-    bool empty = false;
-    int rc = search_dev(c, p, t, x0, x1, y0, y1, step, cells, found_dev, &empty);
-    if (rc || !empty)
-        return rc;
-    // nothing to search: the record says so (stream-ordered, no memset)
-    void *host = nullptr;
-    if ((rc = pinned(c, sizeof(vtg_found), &host)))
-        return rc;
-    VTG_HIP(c, hipStreamSynchronize(c->stream));
-    *(vtg_found *)host = vtg_found{0, 0, -2.0, -2.0, 0.0, 0.0, 0, 0, 0, -2.0f};
-    VTG_HIP(c, hipMemcpyAsync(found_dev, host, sizeof(vtg_found), hipMemcpyHostToDevice, c->stream));
-    VTG_HIP(c, hipStreamSynchronize(c->stream));
-    return VTG_OK;
+    const int region[4] = {x0, x1, y0, y1};
+    vtg_cell *cl[1] = {cells};
+    return search_many(c, 1, &p, &t, region, step, cl, found_dev, nullptr);
 }
 
 int vtg_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
@@ -938,17 +1091,12 @@ int vtg_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int 
 {
     if (!found)
         return fail(c, VTG_EINVAL, "vtg_ncc_search: null found record");
-    *found = vtg_found{0, 0, -2.0, -2.0, 0.0, 0.0, 0, 0, 0, -2.0f};
-    bool empty = false;
-    int rc = search_dev(c, p, t, x0, x1, y0, y1, step, cells, nullptr, &empty);
-    if (rc || empty)
+    const int region[4] = {x0, x1, y0, y1};
+    vtg_cell *cl[1] = {cells};
+    vtg_found *fd = nullptr;
+    int rc = search_many(c, 1, &p, &t, region, step, cl, nullptr, &fd);
+    if (rc)
         return rc;
-    const Level &lv = p->lv[t->level];
-    const int hx1 = std::min(x1, lv.w - t->tw - 2), hy1 = std::min(y1, lv.h - t->th - 2);
-    const int cx0 = std::max(x0, 1), cy0 = std::max(y0, 1);
-    const int ncell = ((hx1 - cx0) / 16 + 1) * ((hy1 - cy0) / 16 + 1);
-    const size_t cell_bytes = ((size_t)ncell * sizeof(vtg_cell) + 255) & ~(size_t)255;
-    vtg_found *fd = (vtg_found *)((char *)c->scratch + cell_bytes);
     void *host = nullptr;
     if ((rc = pinned(c, sizeof(vtg_found), &host)))
         return rc;
@@ -957,7 +1105,6 @@ int vtg_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int 
     *found = *(vtg_found *)host;
     return VTG_OK;
 }
-
 int vtg_ncc_score(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out)
 {
     if (!c || !p || !t || n < 0 || (n > 0 && (!xy || !out)) || t->level >= p->levels)

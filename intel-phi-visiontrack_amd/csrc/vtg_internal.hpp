@@ -107,9 +107,6 @@ int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *c
                    const int *off, const lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
                    int solve_mode, lk_out *out);
 int launch_ncc_template(vtg_ctx *c, const vtg_pyr *p, int level, int x0, int y0, vtg_tmpl *t);
-size_t search_scratch_bytes(int ncell);
-int launch_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
-                      vtg_cell *cells, vtg_found *found, void *aux);
 int launch_ncc_score(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out);
 
 }  // namespace vtg

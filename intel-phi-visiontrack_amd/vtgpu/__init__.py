@@ -75,6 +75,7 @@ def lib():
         "vtg_ncc_template_read": [V, V, V, C.POINTER(F)],
         "vtg_ncc_search": [V, V, V, I, I, I, I, I, V, C.POINTER(Found)],
         "vtg_ncc_search_async": [V, V, V, I, I, I, I, I, V, V],
+        "vtg_ncc_search_many": [V, I, V, V, V, I, V, V],
         "vtg_graph_begin": [V], "vtg_graph_end": [V, C.POINTER(V)], "vtg_graph_launch": [V, V],
         "vtg_graph_destroy": [V, V],
         "vtg_ncc_score": [V, V, V, I, V, I, V],
@@ -100,7 +101,8 @@ EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", 
            "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_timing_enable", "vtg_timing_read",
            "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_level",
            "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_margin", "vtg_ncc_template", "vtg_ncc_template_destroy",
-           "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_score",
+           "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_many",
+           "vtg_ncc_score",
            "vtg_graph_begin", "vtg_graph_end", "vtg_graph_launch", "vtg_graph_destroy", "vtg_img_u8_to_q6", "vtg_img_reduce",
            "vtg_lk_track_planes", "vtg_correlate_planes"]
 
@@ -307,6 +309,16 @@ class Context:
         self.check(self.L.vtg_ncc_search_async(self.h, pyr.h, tmpl.h, x0, x1, y0, y1, step,
                                                cells.ptr if cells is not None else None, found_dev.ptr),
                    "ncc_search_async")
+
+    def ncc_search_many(self, pyrs, tmpls, regions, step, found_dev, cells=None):
+        """Searches k over regions[k] = (x0, x1, y0, y1) of pyrs[k] with tmpls[k]; picks to the device records
+        found_dev (a DevBuf of len(pyrs) Found records). Stream-ordered."""
+        n = len(pyrs)
+        P = (C.c_void_p * n)(*[p.h for p in pyrs])
+        T = (C.c_void_p * n)(*[t.h for t in tmpls])
+        R = (C.c_int * (4 * n))(*[int(v) for r in regions for v in r])
+        CL = None if cells is None else (C.c_void_p * n)(*[c.ptr if c is not None else None for c in cells])
+        self.check(self.L.vtg_ncc_search_many(self.h, n, P, T, R, step, CL, found_dev.ptr), "ncc_search_many")
 
     # graphs
     def graph_begin(self):

@@ -48,8 +48,12 @@ def test_graph_replays_match_eager(ctx):
                 tm.append(sctx.ncc_template(pyrs[k][0], 0, w // 3, h // 3, 32, 32))
             else:
                 tm[k].update(pyrs[k][0], 0, w // 3, h // 3)
-            sctx.check(L.vtg_ncc_search_async(sctx.h, pyrs[k][1].h, tm[k].h, 1, w, 1, h, 1, cells[k].ptr,
-                                              C.c_void_p(found.ptr.value + fs * k)))
+        # the first two searches in one call, the last alone (both forms in the captured graph)
+        sctx.ncc_search_many([p[1] for p in pyrs[:2]], tm[:2], [(1, w, 1, h) for w, h, _, _ in sizes[:2]], 1, found,
+                             cells=cells[:2])
+        w, h = sizes[2][:2]
+        sctx.check(L.vtg_ncc_search_async(sctx.h, pyrs[2][1].h, tm[2].h, 1, w, 1, h, 1, cells[2].ptr,
+                                          C.c_void_p(found.ptr.value + fs * 2)))
         ctx.wait(sctx)
 
     def results():

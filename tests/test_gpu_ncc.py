@@ -129,3 +129,45 @@ def test_search_1080_full(ctx, oracle):
     assert abs(f.hx / 2 - (x0 + tr[0])) <= 5 and abs(f.hy / 2 - (y0 + tr[1])) <= 5
     cells.free()
     t.destroy()
+
+
+@pytest.mark.parametrize("tw,th,level,step", [(64, 64, 0, 1), (32, 21, 1, 2), (32, 24, 0, 2)])
+def test_search_many_matches_single(ctx, tw, th, level, step):
+    """Several searches in one call (frames of different sizes, partial windows, one window that clips to nothing and
+    more searches than one launch holds) give the records of the single-search call, field for field."""
+    import ctypes as C
+    sizes = [(640, 480), (800, 600), (1280, 720), (640, 480), (960, 540), (640, 480), (720, 400), (1024, 768),
+             (640, 480), (800, 600)]
+    pyrs, tms, regions = [], [], []
+    for k, (w, h) in enumerate(sizes):
+        f0, f1, _ = vtgpu.frame_pair(w, h, 500 + k)
+        p0, p1 = ctx.pyramid(w, h, 2), ctx.pyramid(w, h, 2)
+        p0.build(f0)
+        p1.build(f1)
+        lw, lh = (w, h) if level == 0 else ((w + 1) // 2, (h + 1) // 2)
+        tms.append(ctx.ncc_template(p0, level, lw // 3, lh // 4, tw, th))
+        pyrs.append((p0, p1))
+        regions.append((1, lw, 1, lh) if k % 3 else (lw // 4, lw // 2 + 7, lh // 5, lh // 2 + 3))
+    regions[4] = (lw * 4, lw * 5, 1, 10)  # clips to nothing
+    fs = C.sizeof(vtgpu.Found)
+    found = ctx.alloc(fs * len(sizes))
+    say("ncc search_many", tw, th, level, step)
+    ctx.ncc_search_many([p[1] for p in pyrs], tms, regions, step, found)
+    ctx.sync()
+    got = (vtgpu.Found * len(sizes))()
+    ctx.check(vtgpu.lib().vtg_download(ctx.h, C.cast(got, C.c_void_p), found.ptr, fs * len(sizes)))
+    for k in range(len(sizes)):
+        x0, x1, y0, y1 = regions[k]
+        one = ctx.ncc_search(pyrs[k][1], tms[k], x0, x1, y0, y1, step)
+        g = got[k]
+        assert bytes(one) == bytes(g), (k, (one.hx, one.hy, one.ncc), (g.hx, g.hy, g.ncc))
+        if k == 4:
+            assert g.cells == 0 and g.ncc == -2.0
+        else:
+            assert g.cells > 0
+    found.free()
+    for t in tms:
+        t.destroy()
+    for p0, p1 in pyrs:
+        p0.destroy()
+        p1.destroy()
