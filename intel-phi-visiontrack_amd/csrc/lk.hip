@@ -3,18 +3,30 @@
 //
 // Lane layout. A w x h window is cut into G = 64 / w row groups of w lanes each (21 x 21: 3 groups of 7 rows, 63 lanes;
 // 15 x 15: 4 groups of 4 rows, 60 lanes). Lane (k, g) owns window column k and rows [g * chunk, (g + 1) * chunk): its
-// template values T, Tx, Ty live in registers for the whole level, and each iteration it walks down its rows carrying
-// the x-interpolated row, as the reference's vector code carries a row (lk_vx.c:100-126).
-// Memory. At the start of a level the wave issues, in one batch, the loads of the template's (w + 3) x (h + 3) patch of I
-// and of J's neighbourhood of the level's starting estimate (the window plus a margin of m pixels), into its own LDS
-// slice: one round trip to memory per level. The bilinear window S, the Sobel gradients and every iteration's samples
-// of J then come from LDS; only an estimate that leaves the margin reloads J's neighbourhood.
+// template values T, Tx, Ty live in registers for the whole level. A lane walks MAXR consecutive window rows (a
+// compile-time count no smaller than the chunk); a group whose chunk would run past the window's last row walks the
+// last MAXR rows instead, and rows that are not the lane's own carry zero gradients, so they add nothing to any sum.
+// Direct schedule (the default). The window build reads the level in memory straight into registers: per image row,
+// four taps and three x-interpolations give the lane's three columns of the bilinear window S, and consecutive rows
+// give S rows and the Sobel gradients, with no staging and no index division. Each iteration reads two taps per row of
+// J, x-interpolated once per row and y-interpolated between rows, as the reference's vector code carries a row
+// (lk_vx.c:100-126). The level bounds test (a window m = w / 2 + 1 pixels clear of the left/top edge and m + 1 of the
+// right/bottom edge) keeps every pixel read inside the level, so no coordinate is clamped.
+// Staged schedule (margin > 0). At the start of a level the wave loads, in one batch, the template's (w + 3) x (h + 3)
+// patch of I and J's neighbourhood of the level's starting estimate (the window plus a margin of m pixels) into its
+// LDS slice; S, the gradients and every iteration's samples then come from LDS. The arithmetic of both schedules is
+// the same operation for operation, so their results agree bit for bit.
+// Element types. Level values are read in their own units (u8: grey, u16: Q6, f16/bf16: grey) and carried at that
+// scale through the interpolation; the Q6 factor is applied where it is exact (a power of two), so the results equal
+// those of interpolating Q6 values.
 // Sums over the window (G, b, the residual) reduce across the wave with DPP (quad permutes, row mirrors, row
 // broadcasts) and one readlane, so every lane holds the same total. The per-point driver (level walk, eigenvalue test,
 // 2x2 solve, convergence, bounds) runs in fp64 on those uniform values, the reference's being x87 long double; the
 // patch arithmetic is fp32 with fused multiply-adds where the reference's vector code fuses them.
-// Batches. A launch covers points of up to 64 frame pairs of any sizes: a table maps each pair to its pyramids'
-// device descriptors, its top level and its slice of the point array.
+// Batches. A launch covers points of up to 64 frame pairs of any sizes: a table in the kernel arguments maps each pair
+// to its pyramids' device descriptors, its top level and its slice of the point array. Workgroups are mapped to points
+// so that each of the 8 XCDs (which take workgroups in turn) works on one contiguous run of points, keeping
+// neighbouring windows in one L2.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
@@ -32,13 +44,14 @@ struct Cfg {
     float eps, min_eig, max_res;
 };
 
-constexpr int BMAX = 8; // pairs per launch (inline in the kernel arguments)
+constexpr int BMAX = 64; // pairs per launch (inline in the kernel arguments)
+constexpr int NXCD = 8;  // XCDs of the device; workgroup i runs on XCD i mod 8
 
 struct LkArgs {
     const PyrView *I[BMAX], *J[BMAX];
     int top[BMAX];
     int off[BMAX + 1];
-    int npairs, n;
+    int npairs, n, per_xcd;
     Cfg cfg;
     int margin, lds_floats;
     const float *xy, *guess;
@@ -46,21 +59,27 @@ struct LkArgs {
     lk_out *out;
 };
 
+// A level element in its own units, and how many Q6 units one of them is.
 template <typename T> struct Ld;
 template <> struct Ld<uint8_t> {
-    static __device__ __forceinline__ float get(uint8_t v) { return (float)v * 64.0f; }
+    static constexpr float scale = 64.0f;
+    static __device__ __forceinline__ float get(uint8_t v) { return (float)v; }
 };
 template <> struct Ld<uint16_t> {
+    static constexpr float scale = 1.0f;
     static __device__ __forceinline__ float get(uint16_t v) { return (float)v; }
 };
 template <> struct Ld<float> {
+    static constexpr float scale = 1.0f;
     static __device__ __forceinline__ float get(float v) { return v; }
 };
 template <> struct Ld<__half> {
-    static __device__ __forceinline__ float get(__half v) { return __half2float(v) * 64.0f; }
+    static constexpr float scale = 64.0f;
+    static __device__ __forceinline__ float get(__half v) { return __half2float(v); }
 };
 template <> struct Ld<__hip_bfloat16> {
-    static __device__ __forceinline__ float get(__hip_bfloat16 v) { return __bfloat162float(v) * 64.0f; }
+    static constexpr float scale = 64.0f;
+    static __device__ __forceinline__ float get(__hip_bfloat16 v) { return __bfloat162float(v); }
 };
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -92,11 +111,14 @@ __device__ __forceinline__ float wave_sum(float v)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
-// This lane's place in the window.
+// This lane's place in the window: column k, its own rows [r0, r0 + nr), the MAXR rows it walks from re (inside the
+// window when h >= MAXR), and which of those are its own (bit i: row re + i).
 struct Geo {
-    int W, H, k, r0, nr;
+    int W, H, k, r0, nr, re;
+    unsigned on;
 };
 
+template <int MAXR>
 __device__ __forceinline__ Geo geometry(const Cfg &c, int lane)
 {
     Geo g;
@@ -109,12 +131,132 @@ __device__ __forceinline__ Geo geometry(const Cfg &c, int lane)
     int nr = c.h - g.r0;
     nr = nr < chunk ? nr : chunk;
     g.nr = grp < groups && nr > 0 ? nr : 0;
+    const int last = c.h - MAXR > 0 ? c.h - MAXR : 0;
+    g.re = g.nr == 0 ? 0 : (g.r0 < last ? g.r0 : last);
+    g.on = 0;
+#pragma unroll
+    for (int i = 0; i < MAXR; i++)
+        if (g.re + i >= g.r0 && g.re + i < g.r0 + g.nr)
+            g.on |= 1u << i;
     return g;
 }
 
 template <int MAXR> struct Tile {
     float t[MAXR], tx[MAXR], ty[MAXR];
 };
+
+// Sobel gradients of window rows re .. re + MAXR - 1 at the lane's column from the lane's three columns of S rows
+// re .. re + MAXR + 1 (values in units of `ks` Q6), with T = S interior; returns this lane's part of G. Rows that are not
+// the lane's own get zero gradients (build_vx, lk_vx.c:36-97).
+template <int MAXR>
+__device__ __forceinline__ void sobel_rows(const float (&S)[MAXR + 2][3], float ks, const Geo &g, Tile<MAXR> &tl,
+                                           float &sxx, float &sxy, float &syy)
+{
+    const float kg = ks * (1.0f / 512.0f);
+#pragma unroll
+    for (int i = 0; i < MAXR; i++) {
+        const float *u = S[i], *m = S[i + 1], *d = S[i + 2];
+        const float du = u[2] - u[0], dm = m[2] - m[0], dd = d[2] - d[0];
+        float tx = fmaf(dm, 2.0f, du + dd) * kg;
+        const float vl = d[0] - u[0], vc = d[1] - u[1], vr = d[2] - u[2];
+        float ty = fmaf(vc, 2.0f, vl + vr) * kg;
+        const bool on = (g.on >> i) & 1u;
+        tx = on ? tx : 0.0f;
+        ty = on ? ty : 0.0f;
+        tl.t[i] = m[1] * ks;
+        tl.tx[i] = tx;
+        tl.ty[i] = ty;
+        sxx = fmaf(tx, tx, sxx);
+        sxy = fmaf(tx, ty, sxy);
+        syy = fmaf(ty, ty, syy);
+    }
+}
+
+// One iteration's (RES = false: b = totals of It Tx, It Ty; iterate_vx, lk_vx.c:100-140) or the residual's (RES = true:
+// totals of |It| / 64 and (It / 64)^2 over the lane's own rows; residual_vx, lk_vx.c:142-175) sums from the lane's
+// x-interpolated J rows A[0 .. MAXR] (units of `ks` Q6): J = y-interpolation of consecutive rows, It = T - J.
+template <int MAXR, bool RES>
+__device__ __forceinline__ void mismatch(const float (&A)[MAXR + 1], float fy, float ks, const Geo &g,
+                                         const Tile<MAXR> &tl, float *b)
+{
+    float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < MAXR; i++) {
+        const float jv = fmaf(A[i + 1] - A[i], fy, A[i]);
+        const float it = fmaf(jv, -ks, tl.t[i]); // T - ks J, the product exact
+        if (RES) {
+            const bool on = (g.on >> i) & 1u;
+            const float q = it * (1.0f / 64.0f);
+            s0 = on ? s0 + fabsf(q) : s0;
+            s1 = on ? fmaf(q, q, s1) : s1;
+        } else {
+            s0 = fmaf(it, tl.tx[i], s0);
+            s1 = fmaf(it, tl.ty[i], s1);
+        }
+    }
+    b[0] = wave_sum(s0);
+    b[1] = wave_sum(s1);
+}
+
+// ---- the direct schedule
+
+// The template at (px, py), straight from the level into registers: returns the wave totals of Tx^2, Tx Ty, Ty^2.
+template <int MAXR, typename T>
+__device__ __forceinline__ void build_direct(const Level &lv, double px, double py, const Geo &g, Tile<MAXR> &tl,
+                                             float *G)
+{
+    const double flx = floor(px), fly = floor(py);
+    const float fx = (float)(px - flx), fy = (float)(py - fly);
+    // S row 0, column 0 samples level pixel (flx - w/2 - 1 + fx, fly - h/2 - 1 + fy); this lane needs S columns
+    // k .. k + 2 of S rows re .. re + MAXR + 1, i.e. image rows re .. re + MAXR + 2 (at most h + 2) and 4 columns
+    const T *base = (const T *)lv.p;
+    const unsigned x0 = (unsigned)((int)flx - g.W / 2 - 1 + g.k);
+    const int y0 = (int)fly - g.H / 2 - 1;
+    float X[MAXR + 3][3];
+#pragma unroll
+    for (int i = 0; i < MAXR + 3; i++) {
+        const int j = g.re + i < g.H + 2 ? g.re + i : g.H + 2;
+        const T *p = base + (unsigned)(y0 + j) * (unsigned)lv.pitch + x0;
+        const float a0 = Ld<T>::get(p[0]), a1 = Ld<T>::get(p[1]), a2 = Ld<T>::get(p[2]), a3 = Ld<T>::get(p[3]);
+        X[i][0] = fmaf(a1 - a0, fx, a0);
+        X[i][1] = fmaf(a2 - a1, fx, a1);
+        X[i][2] = fmaf(a3 - a2, fx, a2);
+    }
+    float S[MAXR + 2][3];
+#pragma unroll
+    for (int i = 0; i < MAXR + 2; i++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            S[i][c] = fmaf(X[i + 1][c] - X[i][c], fy, X[i][c]);
+    float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+    sobel_rows<MAXR>(S, Ld<T>::scale, g, tl, sxx, sxy, syy);
+    G[0] = wave_sum(sxx);
+    G[1] = wave_sum(sxy);
+    G[2] = wave_sum(syy);
+}
+
+// J at (qx, qy) straight from the level: rows re .. re + MAXR (at most h) of the window at floor(q), 2 taps each.
+template <int MAXR, typename T, bool RES>
+__device__ __forceinline__ void sample_direct(const Level &lv, double qx, double qy, const Geo &g,
+                                              const Tile<MAXR> &tl, float *b)
+{
+    const double flx = floor(qx), fly = floor(qy);
+    const float fx = (float)(qx - flx), fy = (float)(qy - fly);
+    const T *base = (const T *)lv.p;
+    const unsigned x = (unsigned)((int)flx - g.W / 2 + g.k);
+    const int y0 = (int)fly - g.H / 2;
+    float A[MAXR + 1];
+#pragma unroll
+    for (int i = 0; i <= MAXR; i++) {
+        const int j = g.re + i < g.H ? g.re + i : g.H;
+        const T *p = base + (unsigned)(y0 + j) * (unsigned)lv.pitch + x;
+        const float a0 = Ld<T>::get(p[0]), a1 = Ld<T>::get(p[1]);
+        A[i] = fmaf(a1 - a0, fx, a0);
+    }
+    mismatch<MAXR, RES>(A, fy, Ld<T>::scale, g, tl, b);
+}
+
+// ---- the staged schedule
 
 // A rectangle of a level copied into LDS: rows ry .. ry + rh - 1, columns rx .. rx + rw - 1, coordinates clamped to the
 // level (the reference's replicated border).
@@ -128,10 +270,10 @@ struct Slice {
     float *P, *S, *R;
 };
 
-// The wave's loads of one level, issued together: rows 0 .. np - 1 of the combined region come from level li at
-// (rp.rx, rp.ry) (the template patch, np = 0 to skip it), the next rg.rh rows from level lj at (rg.rx, rg.ry) (J's
-// neighbourhood); every row is rg.rw wide. Element i = lane + 64 k sits at row i / rw, column i % rw, stepped without
-// a division per element. Coordinates are clamped to the level (the replicated border).
+// The wave's loads of one level, issued together, as Q6 values: rows 0 .. np - 1 of the combined region come from
+// level li at (rp.rx, rp.ry) (the template patch, np = 0 to skip it), the next rg.rh rows from level lj at
+// (rg.rx, rg.ry) (J's neighbourhood); every row is rg.rw wide. Element i = lane + 64 k sits at row i / rw, column
+// i % rw, stepped without a division per element.
 template <typename T, int BATCH>
 __device__ __forceinline__ void load_regions(const Level &li, const Region &rp, int np, const Level &lj,
                                              const Region &rg, float *dst, int lane)
@@ -149,7 +291,7 @@ __device__ __forceinline__ void load_regions(const Level &li, const Region &rp, 
             const bool fromI = rr < np;
             const int y = fromI ? clampi(rp.ry + rr, 0, li.h - 1) : clampi(rg.ry + rr - np, 0, lj.h - 1);
             const int x = fromI ? clampi(rp.rx + c2, 0, li.w - 1) : clampi(rg.rx + c2, 0, lj.w - 1);
-            v[k] = Ld<T>::get(fromI ? pi[(size_t)y * li.pitch + x] : pj[(size_t)y * lj.pitch + x]);
+            v[k] = Ld<T>::get(fromI ? pi[(size_t)y * li.pitch + x] : pj[(size_t)y * lj.pitch + x]) * Ld<T>::scale;
             c2 += dr;
             rr += dq;
             if (c2 >= rw)
@@ -167,9 +309,8 @@ __device__ __forceinline__ void load_regions(const Level &li, const Region &rp, 
     wave_sync();
 }
 
-// The template at (px, py) from the staged patch P (origin = the window's corner minus one row and column): S =
-// bilinear window over rows -1..h, columns -1..w; T = S interior, Tx, Ty = Sobel(S) / 512; returns the wave totals of
-// Tx^2, Tx Ty, Ty^2 (build_vx, lk_vx.c:36-97).
+// The template at (px, py) from the staged patch P (origin = the window's corner minus one row and column): S into
+// LDS, then the lane's rows as in the direct schedule.
 template <int MAXR>
 __device__ __forceinline__ void build(const Slice &sl, int PW, float fx, float fy, const Geo &g, int lane,
                                       Tile<MAXR> &tl, float *G)
@@ -182,23 +323,16 @@ __device__ __forceinline__ void build(const Slice &sl, int PW, float fx, float f
         sl.S[i] = fmaf(bot - top, fy, top);
     }
     wave_sync();
-    float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+    float S[MAXR + 2][3];
 #pragma unroll
-    for (int i = 0; i < MAXR; i++) {
-        if (i < g.nr) {
-            const float *u = sl.S + (g.r0 + i) * SW + g.k, *m = u + SW, *d = m + SW;
-            const float du = u[2] - u[0], dm = m[2] - m[0], dd = d[2] - d[0];
-            const float tx = fmaf(dm, 2.0f, du + dd) * (1.0f / 512.0f);
-            const float vl = d[0] - u[0], vc = d[1] - u[1], vr = d[2] - u[2];
-            const float ty = fmaf(vc, 2.0f, vl + vr) * (1.0f / 512.0f);
-            tl.t[i] = m[1];
-            tl.tx[i] = tx;
-            tl.ty[i] = ty;
-            sxx = fmaf(tx, tx, sxx);
-            sxy = fmaf(tx, ty, sxy);
-            syy = fmaf(ty, ty, syy);
-        }
+    for (int i = 0; i < MAXR + 2; i++) {
+        const int j = g.re + i < SH - 1 ? g.re + i : SH - 1;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            S[i][c] = sl.S[j * SW + g.k + c];
     }
+    float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+    sobel_rows<MAXR>(S, 1.0f, g, tl, sxx, sxy, syy);
     G[0] = wave_sum(sxx);
     G[1] = wave_sum(sxy);
     G[2] = wave_sum(syy);
@@ -210,8 +344,7 @@ __device__ __forceinline__ bool covers(const Region &rg, int x0, int y0, int W, 
     return x0 >= rg.rx && y0 >= rg.ry && x0 + W <= rg.rx + rg.rw - 1 && y0 + H <= rg.ry + rg.rh - 1;
 }
 
-// J sampled at (qx, qy) from the staged neighbourhood: It = T - J; RES = false: totals of It Tx and It Ty
-// (iterate_vx, lk_vx.c:100-140); RES = true: totals of |It| / 64 and (It / 64)^2 (residual_vx, lk_vx.c:142-175).
+// J sampled at (qx, qy) from the staged neighbourhood R (Q6 values).
 template <int MAXR, bool RES>
 __device__ __forceinline__ void sample(const float *R, const Region &rg, double qx, double qy, const Geo &g,
                                        const Tile<MAXR> &tl, float *b)
@@ -219,118 +352,15 @@ __device__ __forceinline__ void sample(const float *R, const Region &rg, double 
     const double flx = floor(qx), fly = floor(qy);
     const float fx = (float)(qx - flx), fy = (float)(qy - fly);
     const int x = clampi((int)flx - g.W / 2 + g.k - rg.rx, 0, rg.rw - 2);
-    const int y0 = (int)fly - g.H / 2 + g.r0 - rg.ry;
-    float a0[MAXR + 1], a1[MAXR + 1];
+    const int y0 = (int)fly - g.H / 2 - rg.ry;
+    float A[MAXR + 1];
 #pragma unroll
     for (int i = 0; i <= MAXR; i++) {
-        const float *p = R + clampi(y0 + i, 0, rg.rh - 1) * rg.rw + x;
-        a0[i] = p[0];
-        a1[i] = p[1];
+        const int j = g.re + i < g.H ? g.re + i : g.H;
+        const float *p = R + clampi(y0 + j, 0, rg.rh - 1) * rg.rw + x;
+        A[i] = fmaf(p[1] - p[0], fx, p[0]);
     }
-    float s0 = 0.0f, s1 = 0.0f;
-    float above = fmaf(a1[0] - a0[0], fx, a0[0]);
-#pragma unroll
-    for (int i = 0; i < MAXR; i++) {
-        const float here = fmaf(a1[i + 1] - a0[i + 1], fx, a0[i + 1]);
-        const float it = tl.t[i] - fmaf(here - above, fy, above);
-        const bool on = i < g.nr;
-        if (RES) {
-            const float q = it * (1.0f / 64.0f);
-            s0 = on ? s0 + fabsf(q) : s0;
-            s1 = on ? fmaf(q, q, s1) : s1;
-        } else {
-            s0 = on ? fmaf(it, tl.tx[i], s0) : s0;
-            s1 = on ? fmaf(it, tl.ty[i], s1) : s1;
-        }
-        above = here;
-    }
-    b[0] = wave_sum(s0);
-    b[1] = wave_sum(s1);
-}
-
-// ---- the direct schedule: no staging; the window build and every iteration read the level in memory
-
-template <int MAXR, typename T>
-__device__ __forceinline__ void build_direct(const Level &lv, double px, double py, const Geo &g, int lane, float *S,
-                                             Tile<MAXR> &tl, float *G)
-{
-    const double flx = floor(px), fly = floor(py);
-    const float fx = (float)(px - flx), fy = (float)(py - fly);
-    const int SW = g.W + 2, SH = g.H + 2;
-    const int xs = (int)flx - g.W / 2 - 1, ys = (int)fly - g.H / 2 - 1;
-    const T *base = (const T *)lv.p;
-    const int nS = SW * SH, nb = (nS + 63) / 64;
-    wave_sync();
-#pragma unroll 1
-    for (int blk = 0; blk < nb; blk++) {
-        const int i = lane + 64 * blk, ic = i < nS ? i : nS - 1;
-        const int r = ic / SW, cc = ic - r * SW;
-        const unsigned o = (unsigned)clampi(ys + r, 0, lv.h - 2) * (unsigned)lv.pitch +
-                           (unsigned)clampi(xs + cc, 0, lv.w - 2);
-        const float a0 = Ld<T>::get(base[o]), a1 = Ld<T>::get(base[o + 1]);
-        const float b0 = Ld<T>::get(base[o + lv.pitch]), b1 = Ld<T>::get(base[o + lv.pitch + 1]);
-        const float top = fmaf(a1 - a0, fx, a0), bot = fmaf(b1 - b0, fx, b0);
-        if (i < nS)
-            S[i] = fmaf(bot - top, fy, top);
-    }
-    wave_sync();
-    float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
-#pragma unroll
-    for (int i = 0; i < MAXR; i++) {
-        if (i < g.nr) {
-            const float *u = S + (g.r0 + i) * SW + g.k, *m = u + SW, *d = m + SW;
-            const float du = u[2] - u[0], dm = m[2] - m[0], dd = d[2] - d[0];
-            const float tx = fmaf(dm, 2.0f, du + dd) * (1.0f / 512.0f);
-            const float vl = d[0] - u[0], vc = d[1] - u[1], vr = d[2] - u[2];
-            const float ty = fmaf(vc, 2.0f, vl + vr) * (1.0f / 512.0f);
-            tl.t[i] = m[1];
-            tl.tx[i] = tx;
-            tl.ty[i] = ty;
-            sxx = fmaf(tx, tx, sxx);
-            sxy = fmaf(tx, ty, sxy);
-            syy = fmaf(ty, ty, syy);
-        }
-    }
-    G[0] = wave_sum(sxx);
-    G[1] = wave_sum(sxy);
-    G[2] = wave_sum(syy);
-}
-
-template <int MAXR, typename T, bool RES>
-__device__ __forceinline__ void sample_direct(const Level &lv, double qx, double qy, const Geo &g,
-                                              const Tile<MAXR> &tl, float *b)
-{
-    const double flx = floor(qx), fly = floor(qy);
-    const float fx = (float)(qx - flx), fy = (float)(qy - fly);
-    const unsigned x = (unsigned)clampi((int)flx - g.W / 2 + g.k, 0, lv.w - 2);
-    const int y0 = (int)fly - g.H / 2 + g.r0;
-    const T *base = (const T *)lv.p; // uniform: 32-bit per-lane offsets against it
-    float a0[MAXR + 1], a1[MAXR + 1];
-#pragma unroll
-    for (int i = 0; i <= MAXR; i++) {
-        const unsigned o = (unsigned)clampi(y0 + i, 0, lv.h - 1) * (unsigned)lv.pitch + x;
-        a0[i] = Ld<T>::get(base[o]);
-        a1[i] = Ld<T>::get(base[o + 1]);
-    }
-    float s0 = 0.0f, s1 = 0.0f;
-    float above = fmaf(a1[0] - a0[0], fx, a0[0]);
-#pragma unroll
-    for (int i = 0; i < MAXR; i++) {
-        const float here = fmaf(a1[i + 1] - a0[i + 1], fx, a0[i + 1]);
-        const float it = tl.t[i] - fmaf(here - above, fy, above);
-        const bool on = i < g.nr;
-        if (RES) {
-            const float q = it * (1.0f / 64.0f);
-            s0 = on ? s0 + fabsf(q) : s0;
-            s1 = on ? fmaf(q, q, s1) : s1;
-        } else {
-            s0 = on ? fmaf(it, tl.tx[i], s0) : s0;
-            s1 = on ? fmaf(it, tl.ty[i], s1) : s1;
-        }
-        above = here;
-    }
-    b[0] = wave_sum(s0);
-    b[1] = wave_sum(s1);
+    mismatch<MAXR, RES>(A, fy, 1.0f, g, tl, b);
 }
 
 struct Track {
@@ -365,9 +395,10 @@ __device__ __forceinline__ Solve solve2(double gxx, double gxy, double gyy, doub
     s.det = gxx * gyy - gxy * gxy;
     const double tr = gxx + gyy, df = gxx - gyy;
     s.eig = (tr - sqrt(df * df + 4.0 * gxy * gxy)) * 0.5 / area;
-    s.ia = gyy / s.det;
-    s.ib = -gxy / s.det;
-    s.ic = gxx / s.det;
+    const double inv = 1.0 / s.det;
+    s.ia = gyy * inv;
+    s.ib = -gxy * inv;
+    s.ic = gxx * inv;
     return s;
 }
 
@@ -426,9 +457,9 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
                 level_loads<T0, BATCH>(li, rp, lj, rg, sl, true, lane);
             build<MAXR>(sl, rg.rw, (float)(px - flx), (float)(py - fly), g, lane, tl, G);
         } else if (L > 0) {
-            build_direct<MAXR, T1>(li, px, py, g, lane, sl.S, tl, G);
+            build_direct<MAXR, T1>(li, px, py, g, tl, G);
         } else {
-            build_direct<MAXR, T0>(li, px, py, g, lane, sl.S, tl, G);
+            build_direct<MAXR, T0>(li, px, py, g, tl, G);
         }
         const double gxx = G[0], gxy = G[1], gyy = G[2];
         const Solve s = solve2<SOLVE>(gxx, gxy, gyy, area);
@@ -529,23 +560,28 @@ __global__ __launch_bounds__(256) void lk_kernel(LkArgs a)
     extern __shared__ float smem[];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int pt = blockIdx.x * 4 + wave;
+    // workgroup b runs on XCD b mod 8; XCD x takes the contiguous run of workgroups [x per_xcd, (x + 1) per_xcd)
+    const int wg = (int)(blockIdx.x % NXCD) * a.per_xcd + (int)(blockIdx.x / NXCD);
+    const int pt = wg * 4 + wave;
     if (pt >= a.n)
         return;
-    // this point's pair: static indices only, so the table stays in the kernel arguments
-    const PyrView *Ip = a.I[0], *Jp = a.J[0];
-    int top = a.top[0];
-#pragma unroll
-    for (int q = 1; q < BMAX; q++)
-        if (q < a.npairs && pt >= a.off[q])
-            Ip = a.I[q], Jp = a.J[q], top = a.top[q];
-    const PyrView &I = *Ip, &J = *Jp;
+    // this point's pair: the last whose slice starts at or before it (uniform per wave)
+    int lo = 0, hi = a.npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pt >= a.off[mid])
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const PyrView &I = *a.I[lo], &J = *a.J[lo];
+    const int top = a.top[lo];
     const int rw = a.cfg.w + 1 + 2 * a.margin, ph = a.cfg.h + 3;
     float *base = smem + wave * a.lds_floats;
     // layout: S first, then P and R contiguous (one combined region)
     const int sS = (a.cfg.w + 2) * (a.cfg.h + 2);
     const Slice sl{base + sS, base, base + sS + rw * ph};
-    const Geo g = geometry(a.cfg, lane);
+    const Geo g = geometry<MAXR>(a.cfg, lane);
     const float x = a.xy[2 * pt], y = a.xy[2 * pt + 1];
     const float gx = a.guess ? a.guess[2 * pt] : 0.0f, gy = a.guess ? a.guess[2 * pt + 1] : 0.0f;
     Track o;
@@ -554,8 +590,7 @@ __global__ __launch_bounds__(256) void lk_kernel(LkArgs a)
         // back from J to I, the template rebuilt from J (lk_driver.inc:111-121)
         Track b;
         track<MAXR, T0, T1, SOLVE, STAGED>(J, I, top, a.cfg, g, lane, sl, a.margin, o.x, o.y,
-                                           (float)((double)x - (double)o.x),
-                                   (float)((double)y - (double)o.y), b);
+                                           (float)((double)x - (double)o.x), (float)((double)y - (double)o.y), b);
         const double ex = (double)b.x - (double)x, ey = (double)b.y - (double)y;
         if (b.status || ex * ex + ey * ey > (double)a.fb_max * (double)a.fb_max)
             o.status |= LK_FLOWBACK;
@@ -599,11 +634,14 @@ int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, size_t
     return VTG_OK;
 }
 
+// Rows per lane: the smallest compiled count no smaller than the window's chunk.
 template <int SOLVE, bool STAGED>
 int dispatch_rows(vtg_ctx *c, const LkArgs &a, int storage, int chunk, dim3 grid, size_t lds)
 {
     if (chunk <= 4)
         return dispatch_storage<4, SOLVE, STAGED>(c, a, storage, grid, lds);
+    if (chunk <= 7)
+        return dispatch_storage<7, SOLVE, STAGED>(c, a, storage, grid, lds);
     if (chunk <= 8)
         return dispatch_storage<8, SOLVE, STAGED>(c, a, storage, grid, lds);
     if (chunk <= 16)
@@ -656,15 +694,18 @@ int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *c
         return VTG_OK;
     a.cfg = Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->eps, cfg->min_eig, cfg->max_residual};
     a.margin = c->lk_margin > 0 ? c->lk_margin : 1;
+    // the staged schedule's slice per wave: S, then the patch and J's neighbourhood; the direct one uses no LDS
     a.lds_floats = c->lk_margin > 0 ? (cfg->w + 2) * (cfg->h + 2) +
                                           (cfg->w + 1 + 2 * a.margin) * (cfg->h + 3 + cfg->h + 1 + 2 * a.margin)
-                                    : (cfg->w + 2) * (cfg->h + 2);
+                                    : 0;
     a.xy = xy + 2 * (size_t)off[0];
     a.guess = guess ? guess + 2 * (size_t)off[0] : nullptr;
     a.fb_max = fb_max;
     a.out = out + off[0];
     const int groups = 64 / cfg->w, chunk = (cfg->h + groups - 1) / groups;
-    const dim3 grid((a.n + 3) / 4);
+    const int wgs = (a.n + 3) / 4;
+    a.per_xcd = (wgs + NXCD - 1) / NXCD;
+    const dim3 grid(a.per_xcd * NXCD);
     const size_t lds = (size_t)4 * a.lds_floats * sizeof(float);
     hipEvent_t ev;
     timer_begin(c, "lk_track", &ev);
