@@ -11,13 +11,16 @@
 // at most 64 * 255 * 2^-11 per pixel, random in sign; it is left out, which moves a score by ~1e-8 (bounded by
 // 1.3e-5 only if every rounding had the same sign and the window were at the energy floor).
 // The fast search computes sum u u_t with v_dot4_u32_u8 (4 byte products per lane per instruction, exact u32), and
-// sum u, sum u^2 per row with v_sad_u8 / dot4 as each frame row enters the wave's registers; the finish is in fp64.
+// the box sums sum u, sum u^2 of every corner's window by sliding sums over the tile in LDS (down the columns, then
+// along the rows); the finish is in fp64.
 //
 // Fast kernel layout: a workgroup owns 256 x 16 template corners = 16 cells of the map. Its frame tile (corners plus
 // the template's extent) sits in LDS as 32-bit words. Wave s (0..3) takes corners x = X0 + 4 q + s for lane q, so all
 // lanes of a wave share one byte alignment: the template is pre-shifted by that alignment (4 copies, tw/4 + 1 words per
 // row) and read as uniform scalar loads, and each lane reads tw/4 + 1 aligned words per frame row. A lane keeps 8 frame
 // rows in registers and slides them down, so each row read from LDS feeds 8 corners; two passes of 8 rows make 16.
+// The products of one template row run word-outer over the 8 corners' accumulators (independent chains), and the next
+// row's template words are requested before them.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
@@ -345,6 +348,15 @@ __device__ __forceinline__ bool better(float v, int y, int x, float v2, int y2, 
     return v > v2 || (v == v2 && (y < y2 || (y == y2 && x < x2)));
 }
 
+// The frame tile of a workgroup sits in LDS as 32-bit words: LR = NY + th - 1 rows of LW = NX / 4 + WPR + 1 words,
+// word 0 of a row holding the 4 bytes from the aligned column A = X0 & ~3. After it: the vertical window sums V1, V2
+// (sum of u and u^2 over th rows) of the pass's YB corner rows for each of the VB = NX + tw - 1 byte columns, then the
+// box sums B1, B2 of the pass's YB x NX corners.
+__host__ __device__ constexpr size_t fast_lds_words(int th, int tw, int wpr)
+{
+    return (size_t)(NY + th - 1) * (NX / 4 + wpr + 1) + (size_t)2 * YB * (NX + tw - 1) + (size_t)2 * YB * NX;
+}
+
 // Grid: (column blocks of 16 cells, cell rows, searches); blocks past a search's own map leave at once.
 template <int WPR>
 __global__ __launch_bounds__(256) void ncc_cells_fast(SBatch b)
@@ -358,10 +370,12 @@ __global__ __launch_bounds__(256) void ncc_cells_fast(SBatch b)
     const uint8_t *img = (const uint8_t *)a.lv.p;
     const int ih = a.lv.h, pitch = a.lv.pitch, tw = b.tw, th = b.th, stp = b.step;
     constexpr int LW = NX / 4 + WPR + 1;
-    const int LR = NY + th - 1;
+    const int LR = NY + th - 1, VB = NX + tw - 1;
+    uint32_t *V1 = lds + LR * LW, *V2 = V1 + YB * VB, *B1 = V2 + YB * VB, *B2 = B1 + YB * NX;
     const int X0 = a.x0 + blockIdx.x * NX, Y0 = a.y0 + blockIdx.y * NY;
     const int A = X0 & ~3, off = X0 - A;
-    for (int i = threadIdx.x; i < LR * LW; i += 256) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < LR * LW; i += 256) {
         const int r = i / LW, c = i - r * LW;
         const int gy = Y0 + r < ih ? Y0 + r : ih - 1;
         const int gx = A + 4 * c;
@@ -371,86 +385,115 @@ __global__ __launch_bounds__(256) void ncc_cells_fast(SBatch b)
         lds[i] = v;
     }
     __syncthreads();
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const uint8_t *tb = (const uint8_t *)lds; // the tile as bytes: row r, column b of the corners at r * 4 LW + off + b
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int sh = (off + wave) & 3, wb = (off + wave) >> 2;
     const uint32_t *T = a.words + (size_t)sh * th * WPR;
-    const uint32_t mask0 = 0xffffffffu << (8 * sh), maskl = sh ? (0xffffffffu >> (32 - 8 * sh)) : 0u;
-    const int x = X0 + 4 * lane + wave;
+    const int x = X0 + 4 * lane + wave, c = 4 * lane + wave;
     const bool x_ok = x <= a.x1 && ((x - a.x0) % stp) == 0;
     const double n = (double)(tw * th), norm = (double)a.nm[0], mean = (double)a.nm[1];
     const double sum_t = (double)a.sum_t[0];
     float best_v = -3.0f;
     int best_x = 0, best_y = 0;
     const uint32_t *lbase = lds + lane + wb;
+    // running vertical sums of byte columns tid and tid + 256, carried from pass to pass
+    uint32_t vs1[2] = {0u, 0u}, vs2[2] = {0u, 0u};
     for (int sp = 0; sp < NY / YB; sp++) {
+        // vertical sums of the pass's corner rows d = sp YB .. sp YB + YB - 1, sliding down one row at a time
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int bc = tid + 256 * q;
+            if (bc < VB) {
+                const uint8_t *col = tb + off + bc;
+                uint32_t s1 = vs1[q], s2 = vs2[q];
+                for (int d = 0; d < YB; d++) {
+                    const int rr = sp * YB + d;
+                    if (rr == 0) {
+                        for (int r = 0; r < th; r++) {
+                            const uint32_t u = col[r * 4 * LW];
+                            s1 += u;
+                            s2 += u * u;
+                        }
+                    } else {
+                        const uint32_t u_in = col[(rr + th - 1) * 4 * LW], u_out = col[(rr - 1) * 4 * LW];
+                        s1 += u_in - u_out;
+                        s2 += u_in * u_in - u_out * u_out;
+                    }
+                    V1[d * VB + bc] = s1;
+                    V2[d * VB + bc] = s2;
+                }
+                vs1[q] = s1;
+                vs2[q] = s2;
+            }
+        }
+        __syncthreads();
+        {
+            // box sums: task tid = (quantity, corner row, 16-corner segment), sliding right
+            const int row = tid >> 4, seg = tid & 15, d = row & (YB - 1);
+            const uint32_t *src = (row < YB ? V1 : V2) + d * VB;
+            uint32_t *dst = (row < YB ? B1 : B2) + d * NX;
+            const int c0 = seg * 16;
+            uint32_t s = 0;
+            for (int k = 0; k < tw; k++)
+                s += src[c0 + k];
+            dst[c0] = s;
+            for (int k = 1; k < 16; k++) {
+                s += src[c0 + k + tw - 1] - src[c0 + k - 1];
+                dst[c0 + k] = s;
+            }
+        }
+        __syncthreads();
         uint32_t F[YB][WPR];
         uint32_t P[YB];
-        uint32_t c1 = 0, c2 = 0, s1[YB], s2[YB], e1[YB], e2[YB];
 #pragma unroll
         for (int d = 0; d < YB; d++)
             P[d] = 0;
-        s1[0] = s2[0] = 0;
-        // rows 0..YB-2 of the pass: into the ring, their row sums into the running totals
+        // rows 0 .. YB - 2 of the pass into the ring
 #pragma unroll
         for (int j = 0; j < YB - 1; j++) {
             const uint32_t *src = lbase + (sp * YB + j) * LW;
 #pragma unroll
             for (int m = 0; m < WPR; m++)
                 F[j][m] = src[m];
-#pragma unroll
-            for (int m = 0; m < WPR; m++) {
-                const uint32_t w = m == 0 ? (F[j][m] & mask0) : (m == WPR - 1 ? (F[j][m] & maskl) : F[j][m]);
-                c1 = __builtin_amdgcn_sad_u8(w, 0u, c1);
-                c2 = __builtin_amdgcn_udot4(w, w, c2, false);
-            }
-            s1[j + 1] = c1;
-            s2[j + 1] = c2;
         }
-        // template rows r = 0 .. th-1; before step r the ring holds rows r .. r+6, step r brings in row r+7
-        auto step = [&](const int r, const int j, const bool last) __attribute__((always_inline)) {
+        uint32_t tr[WPR];
+#pragma unroll
+        for (int m = 0; m < WPR; m++)
+            tr[m] = T[m];
+        // template row r: the ring holds frame rows r .. r + YB - 2 and takes row r + YB - 1; the next template row's
+        // words are requested before this row's products, so their latency hides behind them
+        auto step = [&](const int r, const int j) __attribute__((always_inline)) {
             const int slot = (j + YB - 1) % YB;
             const uint32_t *src = lbase + (sp * YB + r + YB - 1) * LW;
 #pragma unroll
             for (int m = 0; m < WPR; m++)
                 F[slot][m] = src[m];
-            uint32_t tr[WPR];
+            const int rn = r + 1 < th ? r + 1 : r;
+            uint32_t tn[WPR];
 #pragma unroll
             for (int m = 0; m < WPR; m++)
-                tr[m] = T[r * WPR + m];
+                tn[m] = T[rn * WPR + m];
 #pragma unroll
-            for (int d = 0; d < YB; d++) {
+            for (int m = 0; m < WPR; m++) {
 #pragma unroll
-                for (int m = 0; m < WPR; m++)
+                for (int d = 0; d < YB; d++)
                     P[d] = __builtin_amdgcn_udot4(F[(j + d) % YB][m], tr[m], P[d], false);
             }
 #pragma unroll
-            for (int m = 0; m < WPR; m++) {
-                const uint32_t w =
-                    m == 0 ? (F[slot][m] & mask0) : (m == WPR - 1 ? (F[slot][m] & maskl) : F[slot][m]);
-                c1 = __builtin_amdgcn_sad_u8(w, 0u, c1);
-                c2 = __builtin_amdgcn_udot4(w, w, c2, false);
-            }
-            if (last) {
-                e1[j] = c1;
-                e2[j] = c2;
-            }
+            for (int m = 0; m < WPR; m++)
+                tr[m] = tn[m];
         };
-        int r = 0;
-        for (; r + YB < th; r += YB) {
+        for (int r = 0; r < th; r += YB) {
 #pragma unroll
             for (int j = 0; j < YB; j++)
-                step(r + j, j, false);
+                step(r + j, j);
         }
-#pragma unroll
-        for (int j = 0; j < YB; j++)
-            step(r + j, j, true);
-        // scores of the pass's 8 corners of this lane
+        // scores of the pass's YB corners of this lane
 #pragma unroll
         for (int d = 0; d < YB; d++) {
             const int y = Y0 + sp * YB + d;
-            const uint8_t *rowb = (const uint8_t *)(lds + (sp * YB + d) * LW + lane + wb);
-            const int u0 = rowb[sh];
-            const double su = (double)(e1[d] - s1[d]), sq = (double)(e2[d] - s2[d]);
+            const int u0 = tb[(sp * YB + d) * 4 * LW + off + c];
+            const double su = (double)B1[d * NX + c], sq = (double)B2[d * NX + c];
             const double pc = (double)P[d] - (double)u0 * sum_t;
             const double sv = su - n * (double)u0;
             const double st = 4096.0 * pc - 64.0 * mean * sv;
@@ -478,14 +521,14 @@ __global__ __launch_bounds__(256) void ncc_cells_fast(SBatch b)
         cyy[wave][lane >> 2] = best_y;
     }
     __syncthreads();
-    if (threadIdx.x < 16) {
-        const int c = threadIdx.x;
-        float v = cv[0][c];
-        int bx = cx[0][c], by = cyy[0][c];
+    if (tid < 16) {
+        const int cc = tid;
+        float v = cv[0][cc];
+        int bx = cx[0][cc], by = cyy[0][cc];
         for (int w = 1; w < 4; w++)
-            if (better(cv[w][c], cyy[w][c], cx[w][c], v, by, bx))
-                v = cv[w][c], bx = cx[w][c], by = cyy[w][c];
-        const int col = blockIdx.x * 16 + c;
+            if (better(cv[w][cc], cyy[w][cc], cx[w][cc], v, by, bx))
+                v = cv[w][cc], bx = cx[w][cc], by = cyy[w][cc];
+        const int col = blockIdx.x * 16 + cc;
         if (col < a.cols)
             a.cells[(size_t)blockIdx.y * a.cols + col] = vtg_cell{v, bx, by};
     }
@@ -849,8 +892,7 @@ static int launch_batch(vtg_ctx *c, const vtg_pyr *p0, int level, bool fast, int
     if (maxcell > 0) {
         if (fast) {
             const dim3 grid((maxcols + 15) / 16, maxrows, b.n);
-            const int lw = NX / 4 + wpr + 1;
-            const size_t lds = (size_t)(NY + b.th - 1) * lw * sizeof(uint32_t);
+            const size_t lds = fast_lds_words(b.th, b.tw, wpr) * sizeof(uint32_t);
             timer_begin(c, "ncc_cells", &ev);
             if (wpr == 17)
                 hipLaunchKernelGGL(ncc_cells_fast<17>, grid, dim3(256), lds, c->stream, b);
