@@ -15,7 +15,7 @@ Two forms run, on the same inputs:
 A step of the batched form is one batch of B pairs; `value` is pairs per second. Inputs rotate over more than the
 256 MB Infinity Cache. Prints one JSON line; details go to stderr.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-graph] [--cpu-seconds S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-graph] [--cpu-seconds S] [--batch-steps N]
 """
 import argparse
 import ctypes as C
@@ -170,12 +170,74 @@ def plain_rate(vtgpu, d_frames, corners, xs, ys, steps):
     return steps / (time.perf_counter() - t)
 
 
+def batching(vtgpu, steps, seed=2024):
+    """The uneven batch: 64 pairs (640x480 .. 3840x2160, 500-20,000 points, +-24 px, 3-5 levels), both pyramids of
+    every pair and all points in one LK launch per schedule, captured as one HIP graph and replayed; first under the
+    default schedules, then after the autotuner has picked per size class. Per form: the median and p99 of a whole
+    batch (graph launch to LK records on the host) and the rates."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from workloads import Batch, batch_spec
+    ctx = vtgpu.Context(0)
+    spec = batch_spec(seed)
+    b = Batch(vtgpu, ctx, spec)
+    cfg = vtgpu.LkCfg(WIN, WIN, 10, 0.01, 1e-4, 40.0, 0)
+    h_out = np.zeros(b.npoints, vtgpu.LK_OUT_DTYPE)
+
+    def timed(label):
+        b.enqueue(cfg)  # eager once: scratch and schedules settle
+        ctx.sync()
+        ctx.graph_begin()
+        b.enqueue(cfg)
+        g = ctx.graph_end()
+        lat = []
+        for i in range(steps + 2):
+            t = time.perf_counter()
+            ctx.graph_launch(g)
+            b.out.get(h_out)
+            if i >= 2:
+                lat.append(time.perf_counter() - t)
+        ctx.graph_destroy(g)
+        ctx.timing(True)
+        ctx.timing_reset()
+        b.enqueue(cfg)
+        ctx.sync()
+        kern = {k: ctx.timing_read(k) for k in ("pyr_reduce", "lk_track")}
+        ctx.timing(False)
+        ms = np.array(lat) * 1e3
+        r = {"batch_ms_p50": round(float(np.percentile(ms, 50)), 3),
+             "batch_ms_p99": round(float(np.percentile(ms, 99)), 3),
+             "pairs_per_s": round(len(spec) / (np.median(ms) * 1e-3), 1),
+             "lk_pts_per_s": round(b.npoints / (kern["lk_track"][0] * 1e-3), 1),
+             "kernel_ms": {k: round(v[0], 4) for k, v in kern.items()},
+             "launches": {k: int(v[1]) for k, v in kern.items()}}
+        log(f"batching {label}: {r}")
+        return r
+
+    res = {"pairs": len(spec), "points": b.npoints, "megapixels_per_frame_set": round(b.pixels() / 1e6, 2),
+           "levels": {str(L): sum(1 for x in spec if x[2] == L) for L in (3, 4, 5)},
+           "sizes": {f"{w}x{h}": sum(1 for x in spec if x[0] == w) for (w, h) in
+                     ((640, 480), (1280, 720), (1920, 1080), (3840, 2160))},
+           "graph": True, "storage": "q6"}
+    res["default"] = timed("default")
+    picks = {}
+    for (w, h, L), k in sorted(b.classes().items()):
+        n = b.offsets[k + 1] - b.offsets[k]
+        picks[f"{w}x{h}/L{L}"] = ctx.tune_pair(b.pyrs[k][0], b.pyrs[k][1], b.frames[k][0], b.frames[k][1], L - 1, cfg,
+                                               b.xy.view(8 * b.offsets[k], 8 * n))
+    res["autotune"] = picks
+    res["tuned"] = timed("tuned")
+    ctx.tune_reset()
+    b.destroy()
+    ctx.close()
+    return res
+
+
 def cpu_baseline(f0, f1, xs, ys, budget_s, gpu_out):
     """The oracle's reference-faithful tier (C, pthreads) on this host: pyramids of both frames, LK on a sample of the
     grid, the NCC map on a band of cell rows; scaled to frame pairs per second. Also the GPU's flow error against the
     FP64 oracle on the sample, over its well-conditioned points."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
-    from gpu_util import conditioning
+    from gpu_util import conditioning, precision_eval
     from oracle_ffi import Oracle, lk_cfg
     o = Oracle()
     cores = min(os.cpu_count() or 1, 16)
@@ -209,6 +271,11 @@ def cpu_baseline(f0, f1, xs, ys, budget_s, gpu_out):
               f"{band} of {rows} cell rows x{rows / band:.1f}; lane-model tier, {cores} threads")
     log(f"cpu: {cores} threads: pyramids {t_pyr * 1e3:.1f} ms, LK {t_lk * 1e3:.0f} ms/16,200 pts, NCC "
         f"{t_ncc * 1e3:.0f} ms -> {1 / t_pair:.3f} pairs/s")
+    # storage precision against the FP64 oracle (pyramids fp16 / bf16 / exact, fp64 or mixed solve)
+    import vtgpu
+    prec = precision_eval(vtgpu, o, vtgpu.Context(0), threads=cores)
+    log("precision: " + ", ".join(f"{k} p99 {v['textured']['p99_px']:.2e} px" for k, v in prec.items()
+                                  if isinstance(v, dict)))
     return ({"value": round(1.0 / t_pair, 4), "unit": "frame-pairs/s", "cores": cores, "kind": "port",
              "sample": sample, "lk_pts_per_s": round(NPTS / t_lk, 1), "stage_ms": {
                  "pyramids": round(t_pyr * 1e3, 2), "lk": round(t_lk * 1e3, 1), "ncc": round(t_ncc * 1e3, 1)}},
@@ -217,7 +284,7 @@ def cpu_baseline(f0, f1, xs, ys, budget_s, gpu_out):
              "vs_f64_max_px": float(err.max()),
              "share_within_0.05px_all_converged": float((e_all[both] <= 0.05).mean()),
              "status_equal_vs_reference_tier": same_status,
-             "vs_reference_tier_max_px": float(dref.max()) if len(dref) else 0.0})
+             "vs_reference_tier_max_px": float(dref.max()) if len(dref) else 0.0}, prec)
 
 
 def main():
@@ -229,6 +296,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=72, help="device-resident frame pairs rotated through")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
     ap.add_argument("--plain-steps", type=int, default=8)
+    ap.add_argument("--batch-steps", type=int, default=20, help="graph replays of the uneven 64-pair batch (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -304,9 +372,10 @@ def main():
 
     plain = plain_rate(vtgpu, d_frames, corners, xs, ys, a.plain_steps)
     log(f"plain: {plain:.2f} pairs/s")
-    cpu, err = ({}, {})
+    batch = batching(vtgpu, a.batch_steps) if a.batch_steps > 0 else {}
+    cpu, err, prec = ({}, {}, {})
     if a.cpu_seconds > 0:
-        cpu, err = cpu_baseline(f0s[0], f1s[0], xs, ys, a.cpu_seconds, gpu_out0)
+        cpu, err, prec = cpu_baseline(f0s[0], f1s[0], xs, ys, a.cpu_seconds, gpu_out0)
     line = {
         "metric": METRIC, "value": round(pairs_per_s, 2), "unit": "frame-pairs/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(1e3 * total / a.steps, 4), "higher_is_better": True,
@@ -326,6 +395,8 @@ def main():
         "roofline": {"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
                      "frac": round(achieved / peak, 4), "traffic": None},
         "flow_error": err,
+        "batching": batch,
+        "precision": prec,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

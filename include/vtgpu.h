@@ -127,6 +127,30 @@ int vtg_lk_track_many(vtg_ctx *ctx, int npairs, const vtg_pyr *const *I, const v
  * bit for bit either way; only time differs (default 0). */
 int vtg_lk_set_margin(vtg_ctx *ctx, int margin);
 
+/* ------------------------------------------------------------------ autotuner (no counterpart in the reference, whose
+ * schedule is fixed at build time: one thread per row range or point, track.c:210-292) */
+
+/* What the autotuner measured and keeps for a size class (level-0 size, levels, storage; for LK also top level and
+ * window). */
+struct vtg_tune {
+    int pyr_rows;                         /* pyramid reduce tile height: 8, 16 or 32 output rows */
+    int lk_margin, lk_waves;              /* LK schedule: margin 0 = direct, else staged; waves (points) per workgroup */
+    float pyr_us, lk_us;                  /* median times of the picks: both pyramids; the given points */
+    float pyr_us_default, lk_us_default;  /* the same under the defaults (16 rows; direct, 4 waves) */
+    int candidates;                       /* schedules timed */
+};
+
+/* Time every candidate schedule of the pyramid reduce and of the LK kernel on this pair (device frames f0, f1 of
+ * `pitch` bytes per row, built into I and J) and these n device points, and keep the fastest for the pair's size
+ * class in the context: later builds and LK launches of pyramids of that class use them (the context's margin,
+ * vtg_lk_set_margin, applies to classes not tuned). Waits; allocates a temporary output. */
+int vtg_tune_pair(vtg_ctx *ctx, vtg_pyr *I, vtg_pyr *J, const uint8_t *f0, const uint8_t *f1, int pitch, int top,
+                  const struct lk_cfg *cfg, int n, const float *xy, int solve_mode, struct vtg_tune *res);
+/* The picks in force for a size class (the defaults when it was not tuned; times left 0). */
+int vtg_tune_get(vtg_ctx *ctx, const vtg_pyr *I, int top, const struct lk_cfg *cfg, struct vtg_tune *res);
+/* Forget every pick. */
+int vtg_tune_reset(vtg_ctx *ctx);
+
 /* ------------------------------------------------------------------ NCC search (src/kern/track.c:184-208, 301-329, 935-974, 1615-1734) */
 
 typedef struct vtg_tmpl vtg_tmpl;

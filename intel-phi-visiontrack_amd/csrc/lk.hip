@@ -47,11 +47,14 @@ struct Cfg {
 constexpr int BMAX = 64; // pairs per launch (inline in the kernel arguments)
 constexpr int NXCD = 8;  // XCDs of the device; workgroup i runs on XCD i mod 8
 
+// A launch's pairs: pair q's points are xy[start[q] ..] (global indices), numbered pre[q] .. pre[q + 1] - 1 in the
+// launch.
 struct LkArgs {
     const PyrView *I[BMAX], *J[BMAX];
     int top[BMAX];
-    int off[BMAX + 1];
-    int npairs, n, per_xcd;
+    int start[BMAX];
+    int pre[BMAX + 1];
+    int npairs, n, per_xcd, waves;
     Cfg cfg;
     int margin, lds_floats;
     const float *xy, *guess;
@@ -555,25 +558,26 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
 }
 
 template <int MAXR, typename T0, typename T1, int SOLVE, bool STAGED>
-__global__ __launch_bounds__(256) void lk_kernel(LkArgs a)
+__global__ __launch_bounds__(512) void lk_kernel(LkArgs a)
 {
     extern __shared__ float smem[];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     // workgroup b runs on XCD b mod 8; XCD x takes the contiguous run of workgroups [x per_xcd, (x + 1) per_xcd)
     const int wg = (int)(blockIdx.x % NXCD) * a.per_xcd + (int)(blockIdx.x / NXCD);
-    const int pt = wg * 4 + wave;
-    if (pt >= a.n)
+    const int lp = wg * a.waves + wave;
+    if (lp >= a.n)
         return;
-    // this point's pair: the last whose slice starts at or before it (uniform per wave)
+    // this point's pair: the last whose numbering starts at or before it (uniform per wave)
     int lo = 0, hi = a.npairs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (pt >= a.off[mid])
+        if (lp >= a.pre[mid])
             lo = mid;
         else
             hi = mid - 1;
     }
+    const int pt = a.start[lo] + (lp - a.pre[lo]);
     const PyrView &I = *a.I[lo], &J = *a.J[lo];
     const int top = a.top[lo];
     const int rw = a.cfg.w + 1 + 2 * a.margin, ph = a.cfg.h + 3;
@@ -611,20 +615,20 @@ __global__ __launch_bounds__(256) void lk_kernel(LkArgs a)
 }
 
 template <int MAXR, int SOLVE, bool STAGED>
-int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, size_t lds)
+int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 block, size_t lds)
 {
     switch (storage) {
     case VTG_STORE_Q6:
-        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, uint16_t, SOLVE, STAGED>), grid, dim3(256), lds, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, uint16_t, SOLVE, STAGED>), grid, block, lds, c->stream, a);
         break;
     case VTG_STORE_F32:
-        hipLaunchKernelGGL((lk_kernel<MAXR, float, float, SOLVE, STAGED>), grid, dim3(256), lds, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<MAXR, float, float, SOLVE, STAGED>), grid, block, lds, c->stream, a);
         break;
     case VTG_STORE_F16:
-        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __half, SOLVE, STAGED>), grid, dim3(256), lds, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __half, SOLVE, STAGED>), grid, block, lds, c->stream, a);
         break;
     case VTG_STORE_BF16:
-        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __hip_bfloat16, SOLVE, STAGED>), grid, dim3(256), lds, c->stream,
+        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __hip_bfloat16, SOLVE, STAGED>), grid, block, lds, c->stream,
                            a);
         break;
     default:
@@ -636,17 +640,17 @@ int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, size_t
 
 // Rows per lane: the smallest compiled count no smaller than the window's chunk.
 template <int SOLVE, bool STAGED>
-int dispatch_rows(vtg_ctx *c, const LkArgs &a, int storage, int chunk, dim3 grid, size_t lds)
+int dispatch_rows(vtg_ctx *c, const LkArgs &a, int storage, int chunk, dim3 grid, dim3 block, size_t lds)
 {
     if (chunk <= 4)
-        return dispatch_storage<4, SOLVE, STAGED>(c, a, storage, grid, lds);
+        return dispatch_storage<4, SOLVE, STAGED>(c, a, storage, grid, block, lds);
     if (chunk <= 7)
-        return dispatch_storage<7, SOLVE, STAGED>(c, a, storage, grid, lds);
+        return dispatch_storage<7, SOLVE, STAGED>(c, a, storage, grid, block, lds);
     if (chunk <= 8)
-        return dispatch_storage<8, SOLVE, STAGED>(c, a, storage, grid, lds);
+        return dispatch_storage<8, SOLVE, STAGED>(c, a, storage, grid, block, lds);
     if (chunk <= 16)
-        return dispatch_storage<16, SOLVE, STAGED>(c, a, storage, grid, lds);
-    return dispatch_storage<32, SOLVE, STAGED>(c, a, storage, grid, lds);
+        return dispatch_storage<16, SOLVE, STAGED>(c, a, storage, grid, block, lds);
+    return dispatch_storage<32, SOLVE, STAGED>(c, a, storage, grid, block, lds);
 }
 
 int check_cfg(vtg_ctx *c, const lk_cfg *cfg, int solve_mode)
@@ -675,50 +679,100 @@ int check_pair(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top)
 
 }  // namespace
 
-// Pairs [0, np) with points off[k] .. off[k+1] of xy; np <= BMAX.
-int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
-                   const int *off, const lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
-                   int solve_mode, lk_out *out)
+// One launch for pairs sel[0 .. m) (m <= BMAX; indices into the caller's arrays) under schedule s.
+static int launch_group(vtg_ctx *c, const std::vector<int> &sel, const vtg_pyr *const *I, const vtg_pyr *const *J,
+                        const int *top, const int *off, const lk_cfg *cfg, const float *xy, const float *guess,
+                        float fb_max, int solve_mode, lk_out *out, LkSched s)
 {
     LkArgs a;
-    a.npairs = np;
-    for (int k = 0; k < np; k++) {
-        a.I[k] = I[k]->dview;
-        a.J[k] = J[k]->dview;
-        a.top[k] = top[k];
-        a.off[k] = off[k] - off[0];
+    a.npairs = (int)sel.size();
+    int acc = 0;
+    for (int q = 0; q < a.npairs; q++) {
+        const int k = sel[q];
+        a.I[q] = I[k]->dview;
+        a.J[q] = J[k]->dview;
+        a.top[q] = top[k];
+        a.start[q] = off[k];
+        a.pre[q] = acc;
+        acc += off[k + 1] - off[k];
     }
-    a.off[np] = off[np] - off[0];
-    a.n = off[np] - off[0];
+    a.pre[a.npairs] = acc;
+    a.n = acc;
     if (a.n <= 0)
         return VTG_OK;
     a.cfg = Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->eps, cfg->min_eig, cfg->max_residual};
-    a.margin = c->lk_margin > 0 ? c->lk_margin : 1;
+    a.margin = s.margin > 0 ? s.margin : 1;
     // the staged schedule's slice per wave: S, then the patch and J's neighbourhood; the direct one uses no LDS
-    a.lds_floats = c->lk_margin > 0 ? (cfg->w + 2) * (cfg->h + 2) +
-                                          (cfg->w + 1 + 2 * a.margin) * (cfg->h + 3 + cfg->h + 1 + 2 * a.margin)
-                                    : 0;
-    a.xy = xy + 2 * (size_t)off[0];
-    a.guess = guess ? guess + 2 * (size_t)off[0] : nullptr;
+    a.lds_floats = s.margin > 0 ? (cfg->w + 2) * (cfg->h + 2) +
+                                      (cfg->w + 1 + 2 * a.margin) * (cfg->h + 3 + cfg->h + 1 + 2 * a.margin)
+                                : 0;
+    a.xy = xy;
+    a.guess = guess;
     a.fb_max = fb_max;
-    a.out = out + off[0];
+    a.out = out;
+    a.waves = s.waves;
     const int groups = 64 / cfg->w, chunk = (cfg->h + groups - 1) / groups;
-    const int wgs = (a.n + 3) / 4;
+    const int wgs = (a.n + a.waves - 1) / a.waves;
     a.per_xcd = (wgs + NXCD - 1) / NXCD;
-    const dim3 grid(a.per_xcd * NXCD);
-    const size_t lds = (size_t)4 * a.lds_floats * sizeof(float);
+    const dim3 grid(a.per_xcd * NXCD), block(64 * a.waves);
+    const size_t lds = (size_t)a.waves * a.lds_floats * sizeof(float);
+    if (s.waves < 1 || s.waves > 8 || lds > 65536)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: schedule margin %d x %d waves needs %zu B of LDS (at most 65536)",
+                    s.margin, s.waves, lds);
     hipEvent_t ev;
     timer_begin(c, "lk_track", &ev);
-    const bool staged = c->lk_margin > 0;
+    const bool staged = s.margin > 0;
     int rc;
     if (solve_mode == VTG_SOLVE_MIXED)
-        rc = staged ? dispatch_rows<VTG_SOLVE_MIXED, true>(c, a, I[0]->storage, chunk, grid, lds)
-                    : dispatch_rows<VTG_SOLVE_MIXED, false>(c, a, I[0]->storage, chunk, grid, lds);
+        rc = staged ? dispatch_rows<VTG_SOLVE_MIXED, true>(c, a, I[sel[0]]->storage, chunk, grid, block, lds)
+                    : dispatch_rows<VTG_SOLVE_MIXED, false>(c, a, I[sel[0]]->storage, chunk, grid, block, lds);
     else
-        rc = staged ? dispatch_rows<VTG_SOLVE_F64, true>(c, a, I[0]->storage, chunk, grid, lds)
-                    : dispatch_rows<VTG_SOLVE_F64, false>(c, a, I[0]->storage, chunk, grid, lds);
+        rc = staged ? dispatch_rows<VTG_SOLVE_F64, true>(c, a, I[sel[0]]->storage, chunk, grid, block, lds)
+                    : dispatch_rows<VTG_SOLVE_F64, false>(c, a, I[sel[0]]->storage, chunk, grid, block, lds);
     timer_end(c, "lk_track", ev);
     return rc;
+}
+
+uint64_t lk_class(const vtg_pyr *I, int top, const lk_cfg *cfg)
+{
+    return ((uint64_t)I->w << 48) ^ ((uint64_t)I->h << 32) ^ ((uint64_t)top << 28) ^ ((uint64_t)I->storage << 24) ^
+           ((uint64_t)cfg->w << 12) ^ (uint64_t)cfg->h;
+}
+
+LkSched tuned_lk(const vtg_ctx *c, const vtg_pyr *I, int top, const lk_cfg *cfg)
+{
+    const auto it = c->lk_sched.find(lk_class(I, top, cfg));
+    if (it != c->lk_sched.end())
+        return it->second;
+    return LkSched{c->lk_margin, 4};
+}
+
+// Pairs [0, np) with points off[k] .. off[k+1] - 1 of xy: grouped by schedule, one launch per group of up to BMAX.
+int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
+                   const int *off, const lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
+                   int solve_mode, lk_out *out, const LkSched *sched)
+{
+    std::vector<LkSched> sc(np);
+    for (int k = 0; k < np; k++)
+        sc[k] = sched ? *sched : tuned_lk(c, I[k], top[k], cfg);
+    std::vector<bool> done(np, false);
+    for (int k = 0; k < np; k++) {
+        if (done[k])
+            continue;
+        std::vector<int> sel;
+        for (int q = k; q < np; q++)
+            if (!done[q] && sc[q].margin == sc[k].margin && sc[q].waves == sc[k].waves) {
+                sel.push_back(q);
+                done[q] = true;
+            }
+        for (size_t b = 0; b < sel.size(); b += BMAX) {
+            const std::vector<int> part(sel.begin() + b, sel.begin() + std::min(sel.size(), b + BMAX));
+            int rc = launch_group(c, part, I, J, top, off, cfg, xy, guess, fb_max, solve_mode, out, sc[k]);
+            if (rc)
+                return rc;
+        }
+    }
+    return VTG_OK;
 }
 
 }  // namespace vtg
@@ -738,7 +792,7 @@ int vtg_lk_track_ex(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top, con
     if (rc || n == 0)
         return rc;
     const int off[2] = {0, n};
-    return launch_lk_many(c, 1, &I, &J, &top, off, cfg, xy, guess, fb_max, solve_mode, out);
+    return launch_lk_many(c, 1, &I, &J, &top, off, cfg, xy, guess, fb_max, solve_mode, out, nullptr);
 }
 
 int vtg_lk_track(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top, const struct lk_cfg *cfg, int n,
@@ -766,11 +820,7 @@ int vtg_lk_track_many(vtg_ctx *c, int npairs, const vtg_pyr *const *I, const vtg
         return VTG_OK;
     if (!xy || !out)
         return fail(c, VTG_EINVAL, "vtg_lk_track_many: null points or output");
-    for (int k = 0; k < npairs && !rc; k += BMAX) {
-        const int m = std::min(BMAX, npairs - k);
-        rc = launch_lk_many(c, m, I + k, J + k, top + k, offsets + k, cfg, xy, guess, fb_max, solve_mode, out);
-    }
-    return rc;
+    return launch_lk_many(c, npairs, I, J, top, offsets, cfg, xy, guess, fb_max, solve_mode, out, nullptr);
 }
 
 int vtg_lk_set_margin(vtg_ctx *c, int margin)

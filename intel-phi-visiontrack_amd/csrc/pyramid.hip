@@ -2,9 +2,10 @@
 // blur of level l-1 sampled at even pixels, (sum + 128) >> 8 in exact integers (img_reduce_ref, src/kern/img_ref.c:98-114),
 // with the border replicated (the reference's padding, src/kern/img_ref.c:9-30, is a clamp of the coordinates).
 //
-// One launch per level. A workgroup makes a 64 x 16 tile of the output: it stages the 131 x 35 input tile (with the 2-pixel
-// halo, coordinates clamped) in LDS as integers, sums the 5 rows at each even row, then the 5 columns at each even column.
-// The level-0 launch also writes the frame into the pyramid's level 0 from the same tile, so the frame is read once.
+// One launch per level. A workgroup makes a 64 x TH tile of the output (TH = 8, 16 or 32 rows, picked per size class by
+// the autotuner; 16 by default): it stages the 131 x (2 TH + 3) input tile (with the 2-pixel halo, coordinates clamped)
+// in LDS as integers, sums the 5 rows at each even row, then the 5 columns at each even column. The level-0 launch also
+// writes the frame into the pyramid's level 0 from the same tile, so the frame is read once.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
@@ -16,7 +17,7 @@ namespace vtg {
 
 namespace {
 
-constexpr int TW = 64, TH = 16, IW = 2 * TW + 3, IH = 2 * TH + 3;
+constexpr int TW = 64, IW = 2 * TW + 3;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -50,7 +51,7 @@ struct RJob {
     int sw, sh, sp, dw, dh, dp, cp;
     int tiles_x, t0; // tiles across the output, first linear tile of the job in the launch
 };
-constexpr int JMAX = 16;
+constexpr int JMAX = 32;
 struct RJobs {
     RJob j[JMAX];
     int n;
@@ -70,9 +71,10 @@ template <> struct AccT<true> { using type = int; };
 // coordinates clamped: the replicated border) is read with every load of a thread issued before the first is used,
 // then summed in LDS: 5 rows at each even row, then 5 columns at each even column. EXACT: integer sums and one
 // (h + 128) >> 8, bit-exact to the reference; else fp32 without rounding (fp16 / bf16 levels).
-template <typename TS, typename TD, typename TC, bool COPY, bool EXACT>
+template <typename TS, typename TD, typename TC, bool COPY, bool EXACT, int TH>
 __global__ __launch_bounds__(256) void reduce_batch(RJobs jobs)
 {
+    constexpr int IH = 2 * TH + 3;
     using Acc = typename AccT<EXACT>::type;
     __shared__ Acc tin[IH][IW + 1];
     __shared__ Acc tv[TH][IW + 1];
@@ -162,8 +164,8 @@ __global__ __launch_bounds__(256) void from_plane(const float *__restrict__ src,
 
 size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-template <typename TS, typename TD, typename TC, bool COPY, bool EXACT>
-int launch_jobs(vtg_ctx *c, std::vector<RJob> &all)
+template <typename TS, typename TD, typename TC, bool COPY, bool EXACT, int TH>
+int launch_jobs_th(vtg_ctx *c, const std::vector<RJob> &all)
 {
     for (size_t b = 0; b < all.size(); b += JMAX) {
         RJobs js{};
@@ -178,9 +180,29 @@ int launch_jobs(vtg_ctx *c, std::vector<RJob> &all)
         }
         hipEvent_t ev;
         timer_begin(c, "pyr_reduce", &ev);
-        hipLaunchKernelGGL((reduce_batch<TS, TD, TC, COPY, EXACT>), dim3(tiles), dim3(256), 0, c->stream, js);
+        hipLaunchKernelGGL((reduce_batch<TS, TD, TC, COPY, EXACT, TH>), dim3(tiles), dim3(256), 0, c->stream, js);
         VTG_HIP(c, hipGetLastError());
         timer_end(c, "pyr_reduce", ev);
+    }
+    return VTG_OK;
+}
+
+// jobs[i] with tile rows rows[i]: one launch (per JMAX jobs) for each tile height present.
+template <typename TS, typename TD, typename TC, bool COPY, bool EXACT>
+int launch_jobs(vtg_ctx *c, const std::vector<RJob> &jobs, const std::vector<int> &rows)
+{
+    for (int th : {8, 16, 32}) {
+        std::vector<RJob> sel;
+        for (size_t i = 0; i < jobs.size(); i++)
+            if (rows[i] == th)
+                sel.push_back(jobs[i]);
+        if (sel.empty())
+            continue;
+        int rc = th == 8    ? launch_jobs_th<TS, TD, TC, COPY, EXACT, 8>(c, sel)
+                 : th == 16 ? launch_jobs_th<TS, TD, TC, COPY, EXACT, 16>(c, sel)
+                            : launch_jobs_th<TS, TD, TC, COPY, EXACT, 32>(c, sel);
+        if (rc)
+            return rc;
     }
     return VTG_OK;
 }
@@ -188,30 +210,35 @@ int launch_jobs(vtg_ctx *c, std::vector<RJob> &all)
 // Level 1 of every pyramid from its frame (writing level 0 on the way), then each higher level from the one below,
 // one launch per level for all pyramids together.
 template <typename T1, typename TC, bool EXACT>
-int build_levels(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch)
+int build_levels(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch,
+                 const int *rows)
 {
     int maxl = 0;
     for (int i = 0; i < n; i++)
         maxl = std::max(maxl, ps[i]->levels);
     std::vector<RJob> jobs;
+    std::vector<int> th;
     for (int i = 0; i < n; i++) {
         const Level *L = ps[i]->lv;
         if (ps[i]->levels < 2)
             continue;
         jobs.push_back(RJob{frames[i], (void *)L[1].p, (void *)L[0].p, ps[i]->w, ps[i]->h, pitch[i], L[1].w, L[1].h,
                             L[1].pitch, L[0].pitch, 0, 0});
+        th.push_back(rows[i]);
     }
-    int rc = jobs.empty() ? VTG_OK : launch_jobs<uint8_t, T1, TC, true, EXACT>(c, jobs);
+    int rc = jobs.empty() ? VTG_OK : launch_jobs<uint8_t, T1, TC, true, EXACT>(c, jobs, th);
     for (int l = 2; l < maxl && !rc; l++) {
         jobs.clear();
+        th.clear();
         for (int i = 0; i < n; i++) {
             if (ps[i]->levels <= l)
                 continue;
             const Level *L = ps[i]->lv;
             jobs.push_back(RJob{L[l - 1].p, (void *)L[l].p, nullptr, L[l - 1].w, L[l - 1].h, L[l - 1].pitch, L[l].w,
                                 L[l].h, L[l].pitch, 0, 0, 0});
+            th.push_back(rows[i]);
         }
-        rc = launch_jobs<T1, T1, T1, false, EXACT>(c, jobs);
+        rc = launch_jobs<T1, T1, T1, false, EXACT>(c, jobs, th);
     }
     return rc;
 }
@@ -228,9 +255,14 @@ PyrView view(const vtg_pyr *p)
     return v;
 }
 
-// frames[i]: pyramid i's u8 frame on the device, pitch[i] bytes per row. All pyramids of one storage.
-int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch)
+// frames[i]: pyramid i's u8 frame on the device, pitch[i] bytes per row. All pyramids of one storage. rows: each
+// pyramid's reduce tile height (8, 16 or 32), or null for the context's choice for its size class.
+int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch,
+                          const int *rows_in)
 {
+    std::vector<int> rows(n);
+    for (int i = 0; i < n; i++)
+        rows[i] = rows_in ? rows_in[i] : tuned_pyr_rows(c, ps[i]);
     // single-level pyramids: level 0 is the frame (converted for F32)
     for (int i = 0; i < n; i++) {
         if (ps[i]->levels != 1)
@@ -246,17 +278,17 @@ int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *
         }
     }
     switch (ps[0]->storage) {
-    case VTG_STORE_Q6: return build_levels<uint16_t, uint8_t, true>(c, ps, n, frames, pitch);
-    case VTG_STORE_F32: return build_levels<float, float, true>(c, ps, n, frames, pitch);
-    case VTG_STORE_F16: return build_levels<__half, uint8_t, false>(c, ps, n, frames, pitch);
-    case VTG_STORE_BF16: return build_levels<__hip_bfloat16, uint8_t, false>(c, ps, n, frames, pitch);
+    case VTG_STORE_Q6: return build_levels<uint16_t, uint8_t, true>(c, ps, n, frames, pitch, rows.data());
+    case VTG_STORE_F32: return build_levels<float, float, true>(c, ps, n, frames, pitch, rows.data());
+    case VTG_STORE_F16: return build_levels<__half, uint8_t, false>(c, ps, n, frames, pitch, rows.data());
+    case VTG_STORE_BF16: return build_levels<__hip_bfloat16, uint8_t, false>(c, ps, n, frames, pitch, rows.data());
     }
     return fail(c, VTG_EINVAL, "pyramid storage %d", ps[0]->storage);
 }
 
 int launch_pyr_build(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame, int pitch)
 {
-    return launch_pyr_build_many(c, &p, 1, &frame, &pitch);
+    return launch_pyr_build_many(c, &p, 1, &frame, &pitch, nullptr);
 }
 
 int pyr_from_planes(vtg_ctx *c, const vt_plane *levels, int n, vtg_pyr **out)
@@ -415,7 +447,7 @@ int vtg_pyr_build_many(vtg_ctx *c, int n, vtg_pyr *const *pyrs, const uint8_t *c
         if (!pyrs[i] || !frames[i] || pitches[i] < pyrs[i]->w || pyrs[i]->storage != pyrs[0]->storage)
             return fail(c, VTG_EINVAL, "vtg_pyr_build_many: pyramid %d: bad frame, pitch or storage", i);
     if (on_device)
-        return launch_pyr_build_many(c, pyrs, n, frames, pitches);
+        return launch_pyr_build_many(c, pyrs, n, frames, pitches, nullptr);
     // host frames: into level 0 where it is u8, else into scratch, then the same batch
     std::vector<const uint8_t *> dev(n);
     std::vector<int> dp(n);
@@ -443,7 +475,7 @@ int vtg_pyr_build_many(vtg_ctx *c, int n, vtg_pyr *const *pyrs, const uint8_t *c
         VTG_HIP(c, hipMemcpy2DAsync((void *)dev[i], dp[i], frames[i], pitches[i], p->w, p->h, hipMemcpyHostToDevice,
                                     c->stream));
     }
-    return launch_pyr_build_many(c, pyrs, n, dev.data(), dp.data());
+    return launch_pyr_build_many(c, pyrs, n, dev.data(), dp.data(), nullptr);
 }
 
 int vtg_pyr_level(const vtg_pyr *p, int level, void **dptr, int *w, int *h, size_t *pitch_bytes, int *elem)
@@ -568,7 +600,7 @@ int vtg_img_reduce(vtg_ctx *c, const struct vt_plane *dst, const struct vt_plane
         {
             std::vector<RJob> jobs{RJob{L[0].p, (void *)L[1].p, nullptr, L[0].w, L[0].h, L[0].pitch, L[1].w, L[1].h,
                                         L[1].pitch, 0, 0, 0}};
-            rc = launch_jobs<float, float, float, false, true>(c, jobs);
+            rc = launch_jobs<float, float, float, false, true>(c, jobs, std::vector<int>{16});
         }
         if (e == hipSuccess && !rc) {
             const dim3 grid((dst->stride + 255) / 256, dst->h + 2 * VT_PAD_Y);

@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -35,6 +36,12 @@ struct Timer {
     hipEvent_t a, b;
 };
 
+// An LK launch schedule: margin 0 = direct (the level read in memory), > 0 = staged with that margin; waves per
+// workgroup 2, 4 or 8.
+struct LkSched {
+    int margin, waves;
+};
+
 }  // namespace vtg
 
 struct vtg_ctx {
@@ -53,6 +60,9 @@ struct vtg_ctx {
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;
     int lk_margin = 0;                       // LK schedule: 0 direct, > 0 stage J with this margin
+    // the autotuner's picks per size class (keys from pyr_class / lk_class)
+    std::map<uint64_t, int> pyr_rows;
+    std::map<uint64_t, vtg::LkSched> lk_sched;
 };
 
 struct vtg_pyr {
@@ -102,10 +112,18 @@ int pyr_from_planes(vtg_ctx *c, const vt_plane *levels, int n, vtg_pyr **out);
 
 // kernels (one per translation unit)
 int launch_pyr_build(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame_dev, int pitch);
-int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch);
+int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch,
+                          const int *rows);
+// sched: one schedule for every pair, or null for each pair's size-class pick (the context default otherwise)
 int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
                    const int *off, const lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
-                   int solve_mode, lk_out *out);
+                   int solve_mode, lk_out *out, const LkSched *sched);
+
+// Size classes of the autotuner and the picks kept for them.
+uint64_t pyr_class(const vtg_pyr *p);
+uint64_t lk_class(const vtg_pyr *I, int top, const lk_cfg *cfg);
+int tuned_pyr_rows(const vtg_ctx *c, const vtg_pyr *p);
+LkSched tuned_lk(const vtg_ctx *c, const vtg_pyr *I, int top, const lk_cfg *cfg);
 int launch_ncc_template(vtg_ctx *c, const vtg_pyr *p, int level, int x0, int y0, vtg_tmpl *t);
 int launch_ncc_score(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out);
 

@@ -40,6 +40,15 @@ class Found(C.Structure):
                 ("best_y", C.c_int), ("best_v", C.c_float)]
 
 
+class Tune(C.Structure):
+    _fields_ = [("pyr_rows", C.c_int), ("lk_margin", C.c_int), ("lk_waves", C.c_int), ("pyr_us", C.c_float),
+                ("lk_us", C.c_float), ("pyr_us_default", C.c_float), ("lk_us_default", C.c_float),
+                ("candidates", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class VtgError(RuntimeError):
     pass
 
@@ -76,6 +85,8 @@ def lib():
         "vtg_ncc_search": [V, V, V, I, I, I, I, I, V, C.POINTER(Found)],
         "vtg_ncc_search_async": [V, V, V, I, I, I, I, I, V, V],
         "vtg_ncc_search_many": [V, I, V, V, V, I, V, V],
+        "vtg_tune_pair": [V, V, V, V, V, I, I, C.POINTER(LkCfg), I, V, I, C.POINTER(Tune)],
+        "vtg_tune_get": [V, V, I, C.POINTER(LkCfg), C.POINTER(Tune)], "vtg_tune_reset": [V],
         "vtg_graph_begin": [V], "vtg_graph_end": [V, C.POINTER(V)], "vtg_graph_launch": [V, V],
         "vtg_graph_destroy": [V, V],
         "vtg_ncc_score": [V, V, V, I, V, I, V],
@@ -102,6 +113,7 @@ EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", 
            "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_level",
            "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_margin", "vtg_ncc_template", "vtg_ncc_template_destroy",
            "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_many",
+           "vtg_tune_pair", "vtg_tune_get", "vtg_tune_reset",
            "vtg_ncc_score",
            "vtg_graph_begin", "vtg_graph_end", "vtg_graph_launch", "vtg_graph_destroy", "vtg_img_u8_to_q6", "vtg_img_reduce",
            "vtg_lk_track_planes", "vtg_correlate_planes"]
@@ -320,6 +332,23 @@ class Context:
         CL = None if cells is None else (C.c_void_p * n)(*[c.ptr if c is not None else None for c in cells])
         self.check(self.L.vtg_ncc_search_many(self.h, n, P, T, R, step, CL, found_dev.ptr), "ncc_search_many")
 
+    # autotuner
+    def tune_pair(self, I, J, f0, f1, top, cfg, xy, solve=SOLVE_F64):
+        """Time the schedules on this pair (device frames f0, f1: DevBufs, pitch = width) and points (DevBuf) and keep
+        the fastest for its size class; returns what was measured."""
+        t = Tune()
+        self.check(self.L.vtg_tune_pair(self.h, I.h, J.h, f0.ptr, f1.ptr, I.w, top, C.byref(cfg), xy.nbytes // 8,
+                                        xy.ptr, solve, C.byref(t)), "tune_pair")
+        return t.as_dict()
+
+    def tune_get(self, I, top, cfg):
+        t = Tune()
+        self.check(self.L.vtg_tune_get(self.h, I.h, top, C.byref(cfg), C.byref(t)), "tune_get")
+        return t.as_dict()
+
+    def tune_reset(self):
+        self.check(self.L.vtg_tune_reset(self.h), "tune_reset")
+
     # graphs
     def graph_begin(self):
         self.check(self.L.vtg_graph_begin(self.h), "graph_begin")
@@ -363,6 +392,16 @@ class DevBuf:
         if self.ptr:
             self.ctx.L.vtg_free(self.ctx.h, self.ptr)
             self.ptr = None
+
+    def view(self, offset, nbytes):
+        """Bytes [offset, offset + nbytes) of this buffer (not owned: never freed through the view)."""
+        assert 0 <= offset and offset + nbytes <= self.nbytes
+        return _View(self.ctx, C.c_void_p(self.ptr.value + offset), nbytes)
+
+
+class _View(DevBuf):
+    def free(self):
+        self.ptr = None
 
 
 class Pyramid:

@@ -96,3 +96,65 @@ def test_graph_replays_match_eager(ctx):
         p0.destroy()
         p1.destroy()
     sctx.close()
+
+
+def test_uneven_batch(ctx, oracle):
+    """Pairs of four frame sizes with 3-5 levels and +-24 px motion, their pyramids built together and all their points
+    in one LK launch: every pair's records equal its own single-pair call, and two pairs match the oracle (pyramids bit
+    for bit, LK statuses and flow within 0.01 px on the well-conditioned points). Then the autotuner picks a schedule
+    per size class, and the batch under the picks gives the same bytes."""
+    import numpy as np
+    from gpu_util import conditioning, flow_diff
+    from oracle_ffi import lk_cfg
+    from workloads import Batch, batch_spec
+    spec = batch_spec(11, npairs=12, nmin=500, nmax=3000)
+    sizes = {w for (w, _, _, _, _) in spec}
+    assert len(sizes) >= 3 and {L for (_, _, L, _, _) in spec} >= {3, 4}
+    b = Batch(vtgpu, ctx, spec)
+    cfg = vtgpu.lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0)
+    say("uneven batch", len(spec), "pairs", b.npoints, "points")
+    b.enqueue(cfg)
+    ctx.sync()
+    got = b.out.get(np.zeros(b.npoints, vtgpu.LK_OUT_DTYPE))
+    for k, (w, h, L, n, _) in enumerate(spec):
+        xy = b.xy_host[2 * b.offsets[k]:2 * b.offsets[k + 1]]
+        one = ctx.lk_track_host(b.pyrs[k][0], b.pyrs[k][1], L - 1, cfg, xy[0::2], xy[1::2])
+        assert one.tobytes() == got[b.offsets[k]:b.offsets[k + 1]].tobytes(), k
+    for k in (min(range(len(spec)), key=lambda i: spec[i][0] * spec[i][1]),
+              max(range(len(spec)), key=lambda i: spec[i][0] * spec[i][1])):
+        w, h, L, n, _ = spec[k]
+        say("uneven batch oracle pair", k, w, h, L)
+        f0, f1 = b.host[k]
+        k0, I = oracle.pyramid(f0, w, h, L, threads=16)
+        k1, J = oracle.pyramid(f1, w, h, L, threads=16)
+        for lv in range(L):
+            assert np.array_equal(b.pyrs[k][1].read_level(lv).view(), k1[lv].view()), (k, lv)
+        xy = b.xy_host[2 * b.offsets[k]:2 * b.offsets[k + 1]]
+        xs, ys = xy[0::2], xy[1::2]
+        ref = oracle.lk_batch(0, I, J, L - 1, lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0), xs, ys, threads=16)
+        well, _ = conditioning(oracle, I, J, L - 1, lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0), xs, ys)
+        g = got[b.offsets[k]:b.offsets[k + 1]]
+        assert (g["status"] == ref["status"])[well].all(), k
+        ok = well & (ref["status"] == 0)
+        assert flow_diff(g, ref)[ok].max() <= 0.01, k
+    # the autotuner: one pair per size class, then the whole batch under the picks
+    for (w, h, L), k in b.classes().items():
+        say("autotune class", w, h, L)
+        n = b.offsets[k + 1] - b.offsets[k]
+        r = ctx.tune_pair(b.pyrs[k][0], b.pyrs[k][1], b.frames[k][0], b.frames[k][1], L - 1, cfg,
+                          b.xy.view(8 * b.offsets[k], 8 * n))
+        assert r["candidates"] >= 10 and r["pyr_rows"] in (8, 16, 32) and r["lk_waves"] in (2, 4, 8)
+        assert ctx.tune_get(b.pyrs[k][0], L - 1, cfg)["lk_margin"] == r["lk_margin"]
+    b.enqueue(cfg)
+    ctx.sync()
+    tuned = b.out.get(np.zeros(b.npoints, vtgpu.LK_OUT_DTYPE))
+    assert tuned.tobytes() == got.tobytes()
+    for k in range(len(spec)):
+        w, h, L, _, _ = spec[k]
+        f0, f1 = b.host[k]
+        if k < 3:
+            # the picked tile height builds the same levels
+            assert np.array_equal(b.pyrs[k][0].read_level(L - 1).view(),
+                                  oracle.pyramid(f0, w, h, L, threads=16)[0][L - 1].view())
+    ctx.tune_reset()
+    b.destroy()
