@@ -15,7 +15,7 @@ Two forms run, on the same inputs:
 A step of the batched form is one batch of B pairs; `value` is pairs per second. Inputs rotate over more than the
 256 MB Infinity Cache. Prints one JSON line; details go to stderr.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-graph] [--cpu-seconds S] [--batch-steps N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-graph] [--cpu-seconds S] [--batch-steps N] [--serve-frames F]
 """
 import argparse
 import ctypes as C
@@ -232,6 +232,67 @@ def batching(vtgpu, steps, seed=2024):
     return res
 
 
+def serving(vtgpu, frames_per_stream, seed=31, budget=2 << 30):
+    """The serving loop under 8 streams (3 x 1080p60, 3 x 720p30, 2 x 480p60; grids of 16,200 / 8,160 / 4,080
+    points; a random walk of 1.5 px per frame; an NCC search on 2% of frames), first with Poisson arrivals at the
+    streams' rates (latency from submit to results on the host, p50 / p99 per class and overall), then saturated: every
+    frame queued at once, which gives the server's capacity in frames per second."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from workloads import STREAMS, StreamFrames
+    rng = np.random.default_rng(seed)
+    frames = [StreamFrames(vtgpu, w, h, 100 + i, k=24, nx=nx, ny=ny) for i, (w, h, _, _, nx, ny) in enumerate(STREAMS)]
+    cfg = vtgpu.LkCfg(WIN, WIN, 10, 0.01, 1e-4, 40.0, 0)
+
+    def run(poisson):
+        srv = vtgpu.Server(budget, lanes=2, max_batch=16, cfg=cfg)
+        ids = [srv.open(w, h, L, nx * ny) for (w, h, _, L, nx, ny) in STREAMS]
+        events = []
+        for s, (w, h, fps, _, _, _) in enumerate(STREAMS):
+            times = np.cumsum(rng.exponential(1.0 / fps, frames_per_stream)) if poisson else np.zeros(frames_per_stream)
+            ncc = rng.random(frames_per_stream) < 0.02
+            events += [(float(times[i]), s, i, bool(ncc[i])) for i in range(frames_per_stream)]
+        events.sort()
+        pending, lat = {}, []
+
+        def collect(block=False):
+            for tk in list(pending):
+                r = srv.wait(tk) if block else srv.poll(tk)
+                if r is not None:
+                    lat.append((pending.pop(tk), r[2]))
+
+        t0 = time.perf_counter()
+        for (t, s, i, ncc) in events:
+            while poisson and time.perf_counter() - t0 < t:
+                collect()
+                time.sleep(0.0002)
+            w, h = STREAMS[s][0], STREAMS[s][1]
+            tk = srv.submit(ids[s], frames[s][i], frames[s].xy, ncc=(w // 2 - 32, h // 2 - 32) if ncc else None)
+            pending[tk] = s
+            if poisson:
+                collect()
+        collect(block=True)
+        elapsed = time.perf_counter() - t0
+        st = srv.stats()
+        srv.destroy()
+        ms = np.array([x[1] for x in lat])
+        cls = np.array([STREAMS[x[0]][1] for x in lat])
+        r = {"frames": len(lat), "seconds": round(elapsed, 3), "frames_per_s": round(len(lat) / elapsed, 1),
+             "p50_ms": round(float(np.percentile(ms, 50)), 3), "p99_ms": round(float(np.percentile(ms, 99)), 3),
+             "per_class": {f"{h}p": {"p50_ms": round(float(np.percentile(ms[cls == h], 50)), 3),
+                                      "p99_ms": round(float(np.percentile(ms[cls == h], 99)), 3)}
+                           for h in (1080, 720, 480)},
+             "lk_points_per_s": round(st["lk_points"] / elapsed, 1), "batches": st["batches"],
+             "mean_batch": round(st["completed"] / max(1, st["batches"]), 2), "max_batch": st["max_batch"],
+             "max_batches_in_flight": st["max_inflight"], "hbm_budget": st["hbm_budget"],
+             "hbm_resident": st["hbm_resident"], "hbm_peak": st["hbm_peak"]}
+        log(f"serving {'poisson' if poisson else 'saturated'}: {r}")
+        return r
+
+    return {"streams": "3 x 1920x1080@60, 3 x 1280x720@30, 2 x 640x480@60", "offered_frames_per_s": 390,
+            "frames_per_stream": frames_per_stream, "lanes": 2, "max_batch": 16, "ncc_share": 0.02,
+            "poisson": run(True), "saturated": run(False)}
+
+
 def cpu_baseline(f0, f1, xs, ys, budget_s, gpu_out):
     """The oracle's reference-faithful tier (C, pthreads) on this host: pyramids of both frames, LK on a sample of the
     grid, the NCC map on a band of cell rows; scaled to frame pairs per second. Also the GPU's flow error against the
@@ -297,6 +358,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
     ap.add_argument("--plain-steps", type=int, default=8)
     ap.add_argument("--batch-steps", type=int, default=20, help="graph replays of the uneven 64-pair batch (0: skip)")
+    ap.add_argument("--serve-frames", type=int, default=300, help="frames per stream of the serving runs (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -373,6 +435,7 @@ def main():
     plain = plain_rate(vtgpu, d_frames, corners, xs, ys, a.plain_steps)
     log(f"plain: {plain:.2f} pairs/s")
     batch = batching(vtgpu, a.batch_steps) if a.batch_steps > 0 else {}
+    serve = serving(vtgpu, a.serve_frames) if a.serve_frames > 0 else {}
     cpu, err, prec = ({}, {}, {})
     if a.cpu_seconds > 0:
         cpu, err, prec = cpu_baseline(f0s[0], f1s[0], xs, ys, a.cpu_seconds, gpu_out0)
@@ -396,6 +459,7 @@ def main():
                      "frac": round(achieved / peak, 4), "traffic": None},
         "flow_error": err,
         "batching": batch,
+        "serving": serve,
         "precision": prec,
         "cpu_baseline": cpu,
     }

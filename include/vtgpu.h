@@ -29,6 +29,7 @@ enum {
     VTG_ENOMEM = -12, /* device or host allocation failed */
     VTG_EIO = -5,     /* a HIP call failed (launch, copy, device lost) */
     VTG_ERANGE = -34, /* a size beyond what the kernels support */
+    VTG_EAGAIN = -11, /* a served request still in flight */
 };
 
 /* ------------------------------------------------------------------ context and memory */
@@ -136,7 +137,7 @@ struct vtg_tune {
     int pyr_rows;                         /* pyramid reduce tile height: 8, 16 or 32 output rows */
     int lk_margin, lk_waves;              /* LK schedule: margin 0 = direct, else staged; waves (points) per workgroup */
     float pyr_us, lk_us;                  /* median times of the picks: both pyramids; the given points */
-    float pyr_us_default, lk_us_default;  /* the same under the defaults (16 rows; direct, 4 waves) */
+    float pyr_us_default, lk_us_default;  /* the same under the defaults (16 rows; direct, 2 waves) */
     int candidates;                       /* schedules timed */
 };
 
@@ -198,6 +199,52 @@ int vtg_ncc_search_many(vtg_ctx *ctx, int n, const vtg_pyr *const *p, const vtg_
 /* Scores at n corners (tk_correlate, track.h:57-59; half != 0: half-pixel corners, search_half): xy device int
  * pairs, out device doubles: -1 outside, 0 for a window under 1/16 of the template's energy. */
 int vtg_ncc_score(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out);
+
+/* ------------------------------------------------------------------ serving (the OP_LK / OP_SEARCH job ops of
+ * tk_build / tk_track, track.c:210-429, 584-629, driven as a request loop over several streams instead of one
+ * tracker's frame sequence) */
+
+typedef struct vtg_srv vtg_srv;
+
+struct vtg_srv_cfg {
+    int device;
+    size_t hbm_budget;     /* device bytes the server may hold: every stream's resident pyramids and template, every
+                            * in-flight batch's frames, points and results */
+    int lanes;             /* HIP streams batches are spread over, 1 .. 8 */
+    int max_batch;         /* frames per batch, 1 .. 64 */
+    struct lk_cfg lk;      /* the window and stopping rule of every request */
+    int solve_mode;        /* VTG_SOLVE_F64 or VTG_SOLVE_MIXED */
+    float fb_max;          /* > 0: flow-back check (lk_flowback) */
+    int tmpl_w, tmpl_h;    /* NCC template size (level 0) */
+};
+
+struct vtg_srv_stats {
+    long submitted, completed, batches, lk_points;
+    int max_batch, max_inflight;  /* largest batch; most batches on the device at once */
+    size_t hbm_budget, hbm_resident, hbm_in_use, hbm_peak;
+};
+
+int vtg_srv_create(const struct vtg_srv_cfg *cfg, vtg_srv **out);
+int vtg_srv_destroy(vtg_srv *srv);
+const char *vtg_srv_error(vtg_srv *srv);
+/* A stream of w x h frames with `levels`-level pyramids and up to max_points points per frame. Its two pyramids and
+ * template stay resident; VTG_ENOMEM when they do not fit the budget left. */
+int vtg_srv_open(vtg_srv *srv, int w, int h, int levels, int max_points, int *stream);
+/* Waits for the stream's requests, then releases its memory. */
+int vtg_srv_close(vtg_srv *srv, int stream);
+/* A frame of a stream (host bytes, `pitch` per row) with n points (host x, y pairs; optional guesses) tracked from the
+ * stream's previous frame to this one (none for its first frame); ncc_x0 >= 0: also cut the template at (ncc_x0,
+ * ncc_y0) of the previous frame and search this whole frame for it. Copies its inputs and returns a ticket at once. */
+int vtg_srv_submit(vtg_srv *srv, int stream, const uint8_t *frame, int pitch, int n, const float *xy,
+                   const float *guess, int ncc_x0, int ncc_y0, uint64_t *ticket);
+/* 0 with the results when the ticket is done (out: room for the request's n records; *nrec: how many were written, n
+ * when the frame had a previous one, else 0; found: the pick, or score -2 when no search ran), VTG_EAGAIN while it is
+ * in flight. latency_ms: submit to results on the host. A ticket's results are delivered once. */
+int vtg_srv_poll(vtg_srv *srv, uint64_t ticket, struct lk_out *out, int *nrec, struct vtg_found *found,
+                 double *latency_ms);
+int vtg_srv_wait(vtg_srv *srv, uint64_t ticket, struct lk_out *out, int *nrec, struct vtg_found *found,
+                 double *latency_ms);
+int vtg_srv_get_stats(vtg_srv *srv, struct vtg_srv_stats *stats);
 
 /* ------------------------------------------------------------------ reference-shaped calls over host planes */
 

@@ -744,7 +744,7 @@ LkSched tuned_lk(const vtg_ctx *c, const vtg_pyr *I, int top, const lk_cfg *cfg)
     const auto it = c->lk_sched.find(lk_class(I, top, cfg));
     if (it != c->lk_sched.end())
         return it->second;
-    return LkSched{c->lk_margin, 4};
+    return LkSched{c->lk_margin, 2};
 }
 
 // Pairs [0, np) with points off[k] .. off[k+1] - 1 of xy: grouped by schedule, one launch per group of up to BMAX.

@@ -1030,6 +1030,38 @@ static int search_many(vtg_ctx *c, int n, const vtg_pyr *const *p, const vtg_tmp
     return VTG_OK;
 }
 
+size_t tmpl_bytes(int tw, int th, bool u8)
+{
+    const size_t fbytes = ((size_t)tw * th * sizeof(float) + 255) & ~(size_t)255;
+    return 256 + 2 * fbytes + (u8 ? (size_t)4 * th * (tw / 4 + 1) * sizeof(uint32_t) : 0);
+}
+
+int tmpl_new(vtg_ctx *c, int tw, int th, int level, bool u8, void *mem, vtg_tmpl **out)
+{
+    *out = nullptr;
+    vtg_tmpl *t = new vtg_tmpl();
+    t->tw = tw;
+    t->th = th;
+    t->level = level;
+    t->words_per_row = tw / 4 + 1;
+    t->owned = mem == nullptr;
+    if (mem) {
+        t->block = mem;
+    } else {
+        hipError_t e = hipMalloc(&t->block, tmpl_bytes(tw, th, u8));
+        if (e != hipSuccess) {
+            delete t;
+            return hip_fail(c, e, "vtg_ncc_template: hipMalloc");
+        }
+    }
+    const size_t fbytes = ((size_t)tw * th * sizeof(float) + 255) & ~(size_t)255;
+    t->t = (float *)((char *)t->block + 256);
+    t->raw = (float *)((char *)t->t + fbytes);
+    t->words = u8 ? (uint32_t *)((char *)t->raw + fbytes) : nullptr;
+    *out = t;
+    return VTG_OK;
+}
+
 }  // namespace vtg
 
 using namespace vtg;
@@ -1047,23 +1079,11 @@ int vtg_ncc_template(vtg_ctx *c, const vtg_pyr *p, int level, int x0, int y0, in
     if (x0 < 1 || y0 < 1 || x0 + tw >= lv.w - 1 || y0 + th >= lv.h - 1)
         return fail(c, VTG_EINVAL, "vtg_ncc_template: window at (%d,%d) does not fit level %d (%dx%d)", x0, y0, level,
                     lv.w, lv.h);
-    vtg_tmpl *t = new vtg_tmpl();
-    t->tw = tw;
-    t->th = th;
-    t->level = level;
-    t->words_per_row = tw / 4 + 1;
-    const size_t head = 256, fbytes = (size_t)tw * th * sizeof(float);
-    const size_t wbytes = (size_t)4 * th * t->words_per_row * sizeof(uint32_t);
-    const size_t bytes = head + 2 * ((fbytes + 255) & ~(size_t)255) + wbytes;
-    hipError_t e = hipMalloc(&t->block, bytes);
-    if (e != hipSuccess) {
-        delete t;
-        return hip_fail(c, e, "vtg_ncc_template: hipMalloc");
-    }
-    t->t = (float *)((char *)t->block + head);
-    t->raw = (float *)((char *)t->t + ((fbytes + 255) & ~(size_t)255));
-    t->words = p->elem[level] == 1 ? (uint32_t *)((char *)t->raw + ((fbytes + 255) & ~(size_t)255)) : nullptr;
-    int rc = launch_ncc_template(c, p, level, x0, y0, t);
+    vtg_tmpl *t = nullptr;
+    int rc = tmpl_new(c, tw, th, level, p->elem[level] == 1, nullptr, &t);
+    if (rc)
+        return rc;
+    rc = launch_ncc_template(c, p, level, x0, y0, t);
     if (rc) {
         (void)hipFree(t->block);
         delete t;
@@ -1091,7 +1111,8 @@ int vtg_ncc_template_destroy(vtg_ctx *c, vtg_tmpl *t)
     if (!c || !t)
         return VTG_EINVAL;
     (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(t->block);
+    if (t->owned)
+        (void)hipFree(t->block);
     delete t;
     return VTG_OK;
 }

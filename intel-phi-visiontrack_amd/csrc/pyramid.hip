@@ -255,6 +255,89 @@ PyrView view(const vtg_pyr *p)
     return v;
 }
 
+static int elem_bytes(int storage, int level)
+{
+    switch (storage) {
+    case VTG_STORE_Q6: return level == 0 ? 1 : 2;
+    case VTG_STORE_F32: return 4;
+    case VTG_STORE_F16:
+    case VTG_STORE_BF16: return level == 0 ? 1 : 2;
+    }
+    return 0;
+}
+
+// The level layout of a pyramid (pitches rounded to 256 B, levels to 4 KiB, then the device descriptor).
+static size_t pyr_layout(vtg_pyr *p)
+{
+    size_t off = 0;
+    int lw = p->w, lh = p->h;
+    for (int l = 0; l < p->levels; l++) {
+        const int eb = elem_bytes(p->storage, l);
+        const size_t pitch_b = round_up((size_t)lw * eb, 256);
+        p->lv[l].w = lw;
+        p->lv[l].h = lh;
+        p->lv[l].pitch = (int)(pitch_b / eb);
+        p->offset[l] = off;
+        p->elem[l] = eb;
+        off += round_up(pitch_b * lh, 4096);
+        lw = (lw + 1) / 2;
+        lh = (lh + 1) / 2;
+    }
+    return off;
+}
+
+size_t pyr_bytes(int w, int h, int levels, int storage)
+{
+    vtg_pyr p;
+    p.w = w;
+    p.h = h;
+    p.levels = levels;
+    p.storage = storage;
+    return pyr_layout(&p) + round_up(sizeof(PyrView), 256);
+}
+
+int pyr_new(vtg_ctx *c, int w, int h, int levels, int storage, void *mem, vtg_pyr **out)
+{
+    *out = nullptr;
+    if (w < 8 || h < 8 || w > 16384 || h > 16384)
+        return fail(c, VTG_ERANGE, "vtg_pyr_create: frame %dx%d outside 8..16384", w, h);
+    if (levels < 1 || levels > VTG_MAX_LEVELS)
+        return fail(c, VTG_EINVAL, "vtg_pyr_create: %d levels (1..%d)", levels, VTG_MAX_LEVELS);
+    if (storage < VTG_STORE_Q6 || storage > VTG_STORE_BF16)
+        return fail(c, VTG_EINVAL, "vtg_pyr_create: storage %d", storage);
+    vtg_pyr *p = new vtg_pyr();
+    p->w = w;
+    p->h = h;
+    p->levels = levels;
+    p->storage = storage;
+    const size_t view_off = pyr_layout(p);
+    p->bytes = view_off + round_up(sizeof(PyrView), 256);
+    p->owned = mem == nullptr;
+    if (mem) {
+        p->block = mem;
+    } else {
+        hipError_t e = hipMalloc(&p->block, p->bytes);
+        if (e != hipSuccess) {
+            delete p;
+            return hip_fail(c, e, "vtg_pyr_create: hipMalloc");
+        }
+    }
+    for (int l = 0; l < levels; l++)
+        p->lv[l].p = (char *)p->block + p->offset[l];
+    // the descriptor batched kernels read, written once
+    p->dview = (PyrView *)((char *)p->block + view_off);
+    const PyrView v = view(p);
+    hipError_t e = hipMemcpy(p->dview, &v, sizeof v, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (p->owned)
+            (void)hipFree(p->block);
+        delete p;
+        return hip_fail(c, e, "vtg_pyr_create: descriptor");
+    }
+    *out = p;
+    return VTG_OK;
+}
+
 // frames[i]: pyramid i's u8 frame on the device, pitch[i] bytes per row. All pyramids of one storage. rows: each
 // pyramid's reduce tile height (8, 16 or 32), or null for the context's choice for its size class.
 int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch,
@@ -333,17 +416,6 @@ int pyr_from_planes(vtg_ctx *c, const vt_plane *levels, int n, vtg_pyr **out)
     return VTG_OK;
 }
 
-static int elem_bytes(int storage, int level)
-{
-    switch (storage) {
-    case VTG_STORE_Q6: return level == 0 ? 1 : 2;
-    case VTG_STORE_F32: return 4;
-    case VTG_STORE_F16:
-    case VTG_STORE_BF16: return level == 0 ? 1 : 2;
-    }
-    return 0;
-}
-
 }  // namespace vtg
 
 using namespace vtg;
@@ -354,53 +426,7 @@ int vtg_pyr_create(vtg_ctx *c, int w, int h, int levels, int storage, vtg_pyr **
 {
     if (!c || !out)
         return VTG_EINVAL;
-    *out = nullptr;
-    if (w < 8 || h < 8 || w > 16384 || h > 16384)
-        return fail(c, VTG_ERANGE, "vtg_pyr_create: frame %dx%d outside 8..16384", w, h);
-    if (levels < 1 || levels > VTG_MAX_LEVELS)
-        return fail(c, VTG_EINVAL, "vtg_pyr_create: %d levels (1..%d)", levels, VTG_MAX_LEVELS);
-    if (storage < VTG_STORE_Q6 || storage > VTG_STORE_BF16)
-        return fail(c, VTG_EINVAL, "vtg_pyr_create: storage %d", storage);
-    vtg_pyr *p = new vtg_pyr();
-    p->w = w;
-    p->h = h;
-    p->levels = levels;
-    p->storage = storage;
-    size_t off = 0;
-    int lw = w, lh = h;
-    for (int l = 0; l < levels; l++) {
-        const int eb = elem_bytes(storage, l);
-        const size_t pitch_b = round_up((size_t)lw * eb, 256);
-        p->lv[l].w = lw;
-        p->lv[l].h = lh;
-        p->lv[l].pitch = (int)(pitch_b / eb);
-        p->offset[l] = off;
-        p->elem[l] = eb;
-        off += round_up(pitch_b * lh, 4096);
-        lw = (lw + 1) / 2;
-        lh = (lh + 1) / 2;
-    }
-    const size_t view_off = off;
-    off += round_up(sizeof(PyrView), 256);
-    p->bytes = off;
-    hipError_t e = hipMalloc(&p->block, off);
-    if (e != hipSuccess) {
-        delete p;
-        return hip_fail(c, e, "vtg_pyr_create: hipMalloc");
-    }
-    for (int l = 0; l < levels; l++)
-        p->lv[l].p = (char *)p->block + p->offset[l];
-    // the descriptor batched kernels read, written once
-    p->dview = (PyrView *)((char *)p->block + view_off);
-    const PyrView v = view(p);
-    e = hipMemcpy(p->dview, &v, sizeof v, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        (void)hipFree(p->block);
-        delete p;
-        return hip_fail(c, e, "vtg_pyr_create: descriptor");
-    }
-    *out = p;
-    return VTG_OK;
+    return pyr_new(c, w, h, levels, storage, nullptr, out);
 }
 
 int vtg_pyr_destroy(vtg_ctx *c, vtg_pyr *p)
@@ -408,7 +434,8 @@ int vtg_pyr_destroy(vtg_ctx *c, vtg_pyr *p)
     if (!c || !p)
         return VTG_EINVAL;
     (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(p->block);
+    if (p->owned)
+        (void)hipFree(p->block);
     delete p;
     return VTG_OK;
 }

@@ -117,7 +117,7 @@ int vtg_tune_pair(vtg_ctx *c, vtg_pyr *I, vtg_pyr *J, const uint8_t *f0, const u
             if (rc)
                 break;
             res->candidates++;
-            if (margin == 0 && waves == 4)
+            if (margin == 0 && waves == 2)
                 res->lk_us_default = us;
             if (us < best)
                 best = us, res->lk_margin = margin, res->lk_waves = waves, res->lk_us = us;

@@ -73,6 +73,7 @@ struct vtg_pyr {
     size_t offset[vtg::kMaxLevels];
     int elem[vtg::kMaxLevels];
     vtg::PyrView *dview = nullptr;           // the levels' descriptor in device memory (read by batched kernels)
+    bool owned = true;                       // block from hipMalloc (else placed in a caller's memory)
 };
 
 struct vtg_tmpl {
@@ -84,6 +85,7 @@ struct vtg_tmpl {
     float *raw = nullptr;
     uint32_t *words = nullptr;  // 4 shifts x th rows x (tw/4 + 1) words, level 0 only
     int words_per_row = 0;
+    bool owned = true;          // block from hipMalloc (else placed in a caller's memory)
 };
 
 namespace vtg {
@@ -107,6 +109,13 @@ int scratch(vtg_ctx *c, size_t bytes, void **p);
 int pinned(vtg_ctx *c, size_t bytes, void **p);
 
 PyrView view(const vtg_pyr *p);
+// A pyramid laid out in `mem` (pyr_bytes of device memory), or in its own allocation when mem is null.
+size_t pyr_bytes(int w, int h, int levels, int storage);
+int pyr_new(vtg_ctx *c, int w, int h, int levels, int storage, void *mem, vtg_pyr **out);
+// A template record of tw x th for a level of element size 1 (u8) or more, laid out in `mem` (tmpl_bytes), or in its
+// own allocation when mem is null; its contents come from vtg_ncc_template_update.
+size_t tmpl_bytes(int tw, int th, bool u8);
+int tmpl_new(vtg_ctx *c, int tw, int th, int level, bool u8, void *mem, vtg_tmpl **out);
 // An F32 pyramid holding copies of host planes levels[0..n-1] (their visible pixels).
 int pyr_from_planes(vtg_ctx *c, const vt_plane *levels, int n, vtg_pyr **out);
 

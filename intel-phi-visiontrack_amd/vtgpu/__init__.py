@@ -49,6 +49,18 @@ class Tune(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SrvCfg(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget", C.c_size_t), ("lanes", C.c_int), ("max_batch", C.c_int),
+                ("lk", LkCfg), ("solve_mode", C.c_int), ("fb_max", C.c_float), ("tmpl_w", C.c_int),
+                ("tmpl_h", C.c_int)]
+
+
+class SrvStats(C.Structure):
+    _fields_ = [("submitted", C.c_long), ("completed", C.c_long), ("batches", C.c_long), ("lk_points", C.c_long),
+                ("max_batch", C.c_int), ("max_inflight", C.c_int), ("hbm_budget", C.c_size_t),
+                ("hbm_resident", C.c_size_t), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t)]
+
+
 class VtgError(RuntimeError):
     pass
 
@@ -87,6 +99,12 @@ def lib():
         "vtg_ncc_search_many": [V, I, V, V, V, I, V, V],
         "vtg_tune_pair": [V, V, V, V, V, I, I, C.POINTER(LkCfg), I, V, I, C.POINTER(Tune)],
         "vtg_tune_get": [V, V, I, C.POINTER(LkCfg), C.POINTER(Tune)], "vtg_tune_reset": [V],
+        "vtg_srv_create": [C.POINTER(SrvCfg), C.POINTER(V)], "vtg_srv_destroy": [V],
+        "vtg_srv_open": [V, I, I, I, I, C.POINTER(I)], "vtg_srv_close": [V, I],
+        "vtg_srv_submit": [V, I, V, I, I, V, V, I, I, C.POINTER(C.c_uint64)],
+        "vtg_srv_poll": [V, C.c_uint64, V, C.POINTER(I), C.POINTER(Found), C.POINTER(C.c_double)],
+        "vtg_srv_wait": [V, C.c_uint64, V, C.POINTER(I), C.POINTER(Found), C.POINTER(C.c_double)],
+        "vtg_srv_get_stats": [V, C.POINTER(SrvStats)],
         "vtg_graph_begin": [V], "vtg_graph_end": [V, C.POINTER(V)], "vtg_graph_launch": [V, V],
         "vtg_graph_destroy": [V, V],
         "vtg_ncc_score": [V, V, V, I, V, I, V],
@@ -103,6 +121,8 @@ def lib():
     L.vtg_ctx_error.restype = C.c_char_p
     L.vtg_ctx_stream.argtypes = [V]
     L.vtg_ctx_stream.restype = V
+    L.vtg_srv_error.argtypes = [V]
+    L.vtg_srv_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -113,7 +133,8 @@ EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", 
            "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_level",
            "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_margin", "vtg_ncc_template", "vtg_ncc_template_destroy",
            "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_many",
-           "vtg_tune_pair", "vtg_tune_get", "vtg_tune_reset",
+           "vtg_tune_pair", "vtg_tune_get", "vtg_tune_reset", "vtg_srv_create", "vtg_srv_destroy", "vtg_srv_error",
+           "vtg_srv_open", "vtg_srv_close", "vtg_srv_submit", "vtg_srv_poll", "vtg_srv_wait", "vtg_srv_get_stats",
            "vtg_ncc_score",
            "vtg_graph_begin", "vtg_graph_end", "vtg_graph_launch", "vtg_graph_destroy", "vtg_img_u8_to_q6", "vtg_img_reduce",
            "vtg_lk_track_planes", "vtg_correlate_planes"]
@@ -169,6 +190,14 @@ def frame_pair(w, h, seed, max_shift=8.0, max_rot=2.0, noise=2.0, motion=None, f
     S.vts_frame_pair(f0.ctypes.data, f1.ctypes.data, w, h, w, seed, C.byref(BLOB_SCENE), C.byref(motion), noise,
                      flat_amp, None if fl is None else fl.ctypes.data, 0 if fl is None else len(fl) // 4, threads)
     return f0, f1, motion
+
+
+def frame(w, h, seed, dx, dy, theta_deg=0.0, noise=2.0, noise_seed=0, threads=8):
+    """One frame of a sequence: the seed's blob scene moved by (dx, dy, theta) about the centre, plus noise."""
+    f = np.zeros((h, w), np.uint8)
+    m = Motion(dx, dy, theta_deg)
+    synth().vts_frame(f.ctypes.data, w, h, w, seed, C.byref(BLOB_SCENE), C.byref(m), noise, noise_seed, threads)
+    return f
 
 
 def truth(motion, w, h, xs, ys):
@@ -480,3 +509,75 @@ class HostPlane:
 
     def view(self):
         return self.padded()[PAD_Y:PAD_Y + self.h, PAD_X:PAD_X + self.w]
+
+
+EAGAIN = -11
+
+
+class Server:
+    """A vtg_srv: streams of frames served by a scheduler thread in continuous batches under a device budget."""
+
+    def __init__(self, budget, lanes=2, max_batch=16, cfg=None, solve=SOLVE_F64, fb_max=0.0, tmpl=(64, 64), device=0):
+        self.L = lib()
+        c = SrvCfg(device, int(budget), lanes, max_batch, cfg if cfg is not None else lk_cfg(), solve, fb_max,
+                   tmpl[0], tmpl[1])
+        h = C.c_void_p()
+        rc = self.L.vtg_srv_create(C.byref(c), C.byref(h))
+        if rc:
+            raise VtgError(f"vtg_srv_create = {rc}")
+        self.h, self.n = h, {}
+
+    def check(self, rc, what):
+        if rc:
+            raise VtgError(f"{what} failed ({rc}): {self.L.vtg_srv_error(self.h).decode()}")
+
+    def open(self, w, h, levels, max_points):
+        sid = C.c_int()
+        self.check(self.L.vtg_srv_open(self.h, w, h, levels, max_points, C.byref(sid)), "srv_open")
+        return sid.value
+
+    def close(self, stream):
+        self.check(self.L.vtg_srv_close(self.h, stream), "srv_close")
+
+    def submit(self, stream, frame, xy=None, guess=None, ncc=None):
+        frame = np.ascontiguousarray(frame, np.uint8)
+        n = 0 if xy is None else len(xy) // 2
+        xyp = None if xy is None else np.ascontiguousarray(xy, np.float32)
+        gp = None if guess is None else np.ascontiguousarray(guess, np.float32)
+        x0, y0 = ncc if ncc is not None else (-1, -1)
+        t = C.c_uint64()
+        self.check(self.L.vtg_srv_submit(self.h, stream, frame.ctypes.data, frame.strides[0], n,
+                                         None if xyp is None else xyp.ctypes.data,
+                                         None if gp is None else gp.ctypes.data, x0, y0, C.byref(t)), "srv_submit")
+        self.n[t.value] = n
+        return t.value
+
+    def _take(self, fn, ticket):
+        n = self.n[ticket]
+        out = np.zeros(n, LK_OUT_DTYPE)
+        f = Found()
+        lat = C.c_double()
+        nrec = C.c_int()
+        rc = fn(self.h, ticket, out.ctypes.data if n else None, C.byref(nrec), C.byref(f), C.byref(lat))
+        if rc == EAGAIN:
+            return None
+        self.check(rc, "srv_take")
+        del self.n[ticket]
+        return out[:nrec.value], f, lat.value
+
+    def poll(self, ticket):
+        """(records, found, latency_ms) once done, else None."""
+        return self._take(self.L.vtg_srv_poll, ticket)
+
+    def wait(self, ticket):
+        return self._take(self.L.vtg_srv_wait, ticket)
+
+    def stats(self):
+        st = SrvStats()
+        self.check(self.L.vtg_srv_get_stats(self.h, C.byref(st)), "srv_stats")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def destroy(self):
+        if self.h:
+            self.L.vtg_srv_destroy(self.h)
+            self.h = None

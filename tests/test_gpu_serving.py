@@ -1,0 +1,109 @@
+"""The serving loop: results of a stream served alone equal those served in a mix with other streams (bit for bit),
+equal the direct calls on the same pyramids, and a device budget too small for everything at once still serves every
+request (in smaller batches) without exceeding it."""
+import numpy as np
+import pytest
+
+import vtgpu
+from gpu_util import say
+from workloads import StreamFrames
+
+pytestmark = pytest.mark.gpu
+
+CFG = vtgpu.lk_cfg(15, 15, 10, 0.01, 1e-4, 40.0)
+
+
+def _serve(srv, streams, frames, nframes, ncc_at=(), burst=True):
+    """Submit nframes of every stream (interleaved, all at once when burst) and collect results per stream."""
+    ids = [srv.open(sf.w, sf.h, L, len(sf.xy) // 2) for sf, L in zip(frames, streams)]
+    res = [[None] * nframes for _ in ids]
+    tickets = []
+    for i in range(nframes):
+        for s, sf in enumerate(frames):
+            w, h = sf.w, sf.h
+            ncc = (w // 2 - 32, h // 2 - 32) if (s, i) in ncc_at else None
+            t = srv.submit(ids[s], sf[i], sf.xy, ncc=ncc)
+            if burst:
+                tickets.append((s, i, t))
+            else:
+                res[s][i] = srv.wait(t)
+    for s, i, t in tickets:
+        res[s][i] = srv.wait(t)
+    for sid in ids:
+        srv.close(sid)
+    return res
+
+
+def test_alone_equals_mix_equals_direct(ctx):
+    a = StreamFrames(vtgpu, 640, 480, 5, k=6, nx=40, ny=30)
+    others = [StreamFrames(vtgpu, 1280, 720, 6, k=6, nx=60, ny=30), StreamFrames(vtgpu, 1920, 1080, 7, k=6,
+                                                                                 nx=80, ny=45),
+              StreamFrames(vtgpu, 640, 480, 8, k=6, nx=40, ny=30)]
+    nf = 6
+    say("serving alone")
+    srv = vtgpu.Server(1 << 30, lanes=2, max_batch=8, cfg=CFG)
+    alone = _serve(srv, [3], [a], nf, ncc_at={(0, 3)}, burst=False)[0]
+    srv.destroy()
+    say("serving mix")
+    srv = vtgpu.Server(1 << 30, lanes=2, max_batch=8, cfg=CFG)
+    mix = _serve(srv, [3, 4, 4, 3], [a] + others, nf, ncc_at={(0, 3), (2, 4)}, burst=True)
+    st = srv.stats()
+    srv.destroy()
+    assert st["completed"] == 4 * nf and st["max_batch"] > 1, st
+    for i in range(nf):
+        assert alone[i][0].tobytes() == mix[0][i][0].tobytes(), i
+        assert bytes(alone[i][1]) == bytes(mix[0][i][1]), i
+    assert len(alone[0][0]) == 0 and alone[0][1].ncc == -2.0  # the first frame only builds its pyramid
+    assert alone[3][1].ncc > 0.9 and mix[2][4][1].ncc > 0.9     # the searches ran
+    # the direct calls on the same frames
+    L = 3
+    for i in range(1, nf):
+        p0, p1 = ctx.pyramid(a.w, a.h, L), ctx.pyramid(a.w, a.h, L)
+        p0.build(a[i - 1])
+        p1.build(a[i])
+        d = ctx.lk_track_host(p0, p1, L - 1, CFG, a.xy[0::2], a.xy[1::2])
+        assert d.tobytes() == alone[i][0].tobytes(), i
+        if i == 3:
+            t = ctx.ncc_template(p0, 0, a.w // 2 - 32, a.h // 2 - 32, 64, 64)
+            f = ctx.ncc_search(p1, t, 1, a.w, 1, a.h, 1)
+            assert bytes(f) == bytes(alone[3][1])
+            t.destroy()
+        p0.destroy()
+        p1.destroy()
+
+
+def _r(n, a):
+    return (n + a - 1) // a * a
+
+
+def _resident(w, h, levels):
+    """Device bytes a stream keeps: two Q6 pyramids (u8 level 0, u16 above; rows of 256 B, levels of 4 KiB, a
+    256 B descriptor) and a 64 x 64 template."""
+    b, lw, lh = 0, w, h
+    for lv in range(levels):
+        b += _r(_r(lw * (1 if lv == 0 else 2), 256) * lh, 4096)
+        lw, lh = (lw + 1) // 2, (lh + 1) // 2
+    return 2 * _r(b + 256, 256) + _r(256 + 2 * 64 * 64 * 4 + 4 * 64 * 17 * 4, 256)
+
+
+def test_tight_budget_admission():
+    a = StreamFrames(vtgpu, 640, 480, 9, k=4, nx=40, ny=30)
+    b = StreamFrames(vtgpu, 1280, 720, 10, k=4, nx=40, ny=30)
+    # room for both streams' resident pyramids and one 1280 x 720 frame in flight, not both frames of a round
+    budget = _resident(640, 480, 3) + _resident(1280, 720, 4) + 1280 * 720 + (192 << 10)
+    srv = vtgpu.Server(budget, lanes=2, max_batch=8, cfg=CFG)
+    say("serving tight budget", budget)
+    res = _serve(srv, [3, 4], [a, b], 4, burst=True)
+    st = srv.stats()
+    assert st["completed"] == 8 and st["hbm_peak"] <= budget and st["max_batch"] == 1, st
+    # the same frames with room to spare give the same records
+    srv2 = vtgpu.Server(1 << 30, lanes=2, max_batch=8, cfg=CFG)
+    ref = _serve(srv2, [3, 4], [a, b], 4, burst=True)
+    srv2.destroy()
+    for s in range(2):
+        for i in range(4):
+            assert res[s][i][0].tobytes() == ref[s][i][0].tobytes()
+    # a stream whose resident pyramids do not fit is refused
+    with pytest.raises(vtgpu.VtgError):
+        srv.open(3840, 2160, 5, 100)
+    srv.destroy()

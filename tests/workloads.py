@@ -79,3 +79,32 @@ class Batch:
         for p0, p1 in self.pyrs:
             p0.destroy()
             p1.destroy()
+
+
+# Serving: (w, h, fps, levels, grid nx x ny) per stream class; 3 x 1080p60, 3 x 720p30, 2 x 480p60.
+STREAMS = [(1920, 1080, 60.0, 4, 180, 90)] * 3 + [(1280, 720, 30.0, 4, 120, 68)] * 3 + [(640, 480, 60.0, 3, 68, 60)] * 2
+
+
+class StreamFrames:
+    """A stream's frames: K distinct frames of one scene whose position walks randomly (sigma px per frame and axis),
+    served in ping-pong order 0, 1, .., K-1, K-2, .., 1, 0, 1, .. so consecutive frames are always neighbours of the
+    walk; its tracking points are a grid."""
+
+    def __init__(self, vtgpu, w, h, seed, k=24, sigma=1.5, nx=None, ny=None, noise=2.0):
+        rng = np.random.default_rng(seed)
+        steps = rng.normal(0.0, sigma, (k, 2))
+        steps[0] = 0.0
+        walk = np.cumsum(steps, 0)
+        self.w, self.h, self.k = w, h, k
+        self.frames = [vtgpu.frame(w, h, seed, float(walk[i, 0]), float(walk[i, 1]), 0.0, noise, seed * 1000 + i,
+                                   threads=16) for i in range(k)]
+        xs, ys = vtgpu.grid(nx or 16, ny or 12, w, h)
+        self.xy = np.stack([xs, ys], 1).ravel().astype(np.float32)
+
+    def index(self, i):
+        period = 2 * (self.k - 1)
+        j = i % period
+        return j if j < self.k else period - j
+
+    def __getitem__(self, i):
+        return self.frames[self.index(i)]
