@@ -34,7 +34,7 @@ W, H, LEVELS, NX, NY, WIN, TPL = 1920, 1080, 4, 180, 90, 21, 64
 NPTS = NX * NY
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, published (dual-issue packed FP32)
-PEAK_DOT4_TOPS = 314.6     # u8 dot4 on the vector ALUs: 256 CU x 4 SIMD x 16 lanes x 4 MAC x 2 x 2.4 GHz (derived)
+PEAK_INT8_MFMA_TOPS = 5000.0  # MI355X dense int8 matrix peak, published (5.0 POP/s)
 UNIQ = 8                   # distinct seeded pairs; device copies rotate through more than the Infinity Cache
 
 
@@ -93,8 +93,9 @@ class Batched:
         self.h_found = (vtgpu.Found * batch)()
         self.graphs = {}
 
-    def enqueue(self, g):
-        """All device work of group g's batch, stream-ordered on A and S (A waits for S at the end)."""
+    def enqueue(self, g, serial=False):
+        """All device work of group g's batch, stream-ordered on A and S (A waits for S at the end); serial: the
+        search starts after LK instead of beside it."""
         A, S, B = self.A, self.S, self.B
         pyrs = self.pyrs[g if self.graph else 0]
         fr = self.frames[g * B:(g + 1) * B]
@@ -102,12 +103,13 @@ class Batched:
         S.wait(A)
         A.lk_track_many([p[0] for p in pyrs], [p[1] for p in pyrs], [LEVELS - 1] * B, self.offs, self.cfg, self.xy,
                         out=self.out)
-        for k in range(B):
-            cx, cy = self.corners[(g * B + k) % UNIQ]
-            if self.tmpl[k] is None:
-                self.tmpl[k] = S.ncc_template(pyrs[k][0], 0, cx, cy, TPL, TPL)
-            else:
-                self.tmpl[k].update(pyrs[k][0], 0, cx, cy)
+        if serial:
+            S.wait(A)  # the search after LK: each kernel timed alone on the device
+        corners = [self.corners[(g * B + k) % UNIQ] for k in range(B)]
+        if self.tmpl[0] is None:
+            self.tmpl = [S.ncc_template(pyrs[k][0], 0, *corners[k], TPL, TPL) for k in range(B)]
+        else:  # the B templates re-cut in one launch
+            S.ncc_templates_update(self.tmpl, [p[0] for p in pyrs], [0] * B, corners)
         # the B exhaustive searches: one launch per stage for all of them
         S.ncc_search_many([p[1] for p in pyrs], self.tmpl, [(1, W, 1, H)] * B, 1, self.found, cells=self.cells)
         A.wait(S)
@@ -400,16 +402,18 @@ def main():
     log(f"batched: {pairs_per_s:.1f} pairs/s, batch p50 {np.percentile(lat_ms, 50):.3f} ms p99 "
         f"{np.percentile(lat_ms, 99):.3f} ms; pair 0 NCC pick ({found0.hx / 2},{found0.hy / 2}) {found0.ncc:.4f}")
 
-    # per-kernel times: the same batched steps launched eagerly, HIP events around every launch on its own stream
+    # per-kernel times: the same batched steps launched eagerly with the two streams serialised (each kernel alone on
+    # the device), HIP events around every launch on its own stream
     bt.A.timing(True)
     bt.S.timing(True)
     ksteps = min(a.steps, 10)
     for i in range(ksteps):
-        bt.enqueue(i % bt.groups)
+        bt.enqueue(i % bt.groups, serial=True)
     bt.A.sync()
     kernels = {}
     for ctx in (bt.A, bt.S):
-        for k in ("pyr_reduce", "lk_track", "ncc_template", "ncc_cells", "ncc_top8", "ncc_refine", "ncc_final"):
+        for k in ("pyr_reduce", "lk_track", "ncc_template", "ncc_xcorr", "ncc_cells", "ncc_top8", "ncc_refine",
+                  "ncc_final"):
             ms, n = ctx.timing_read(k)
             if n:
                 kernels[k] = {"avg_us": round(1e3 * ms / n, 2), "launches": n, "ms_per_step": round(ms / ksteps, 4)}
@@ -417,9 +421,14 @@ def main():
     bt.S.timing(False)
     npos = (W - TPL - 2) * (H - TPL - 2)
     lk_flop_step = float(np.sum(flops))
-    work = {  # per step (one batch): fp32 flops, u8 MAC ops, HBM bytes
+    ncol, nrow = (W - TPL - 2 - 1) // 16 + 1, (H - TPL - 2 - 1) // 16 + 1
+    work = {  # per step (one batch): fp32 flops, int8 MAC ops, HBM bytes
         "lk_track": ("valu", lk_flop_step / 1e12, PEAK_FP32_TFLOPS, "TFLOP/s"),
-        "ncc_cells": ("valu", a.batch * 2.0 * TPL * TPL * npos / 1e12, PEAK_DOT4_TOPS, "TOP/s"),
+        # the cross term: 2 tw th ops per corner (the band's zero products are not counted)
+        "ncc_xcorr": ("mfma", a.batch * 2.0 * TPL * TPL * npos / 1e12, PEAK_INT8_MFMA_TOPS, "TOP/s"),
+        # the scoring pass: reads each corner's cross term (4 B) and the frame once, writes the cell map
+        "ncc_cells": ("hbm", a.batch * (256 * 4 * ncol * nrow + W * H + 12 * ncol * nrow) / 1e9, PEAK_HBM_GBS,
+                      "GB/s"),
         "pyr_reduce": ("hbm", a.batch * 2 * pyr_bytes(W, H, LEVELS, 2) / 1e9, PEAK_HBM_GBS, "GB/s"),
     }
     for k, (b, wk, pk, u) in work.items():

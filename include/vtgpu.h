@@ -178,6 +178,11 @@ int vtg_ncc_template_destroy(vtg_ctx *ctx, vtg_tmpl *t);
 /* Re-cut an existing template of the same size from another window or pyramid (store_appearance into a slot that
  * already exists, track.c:935-974), stream-ordered, no allocation. */
 int vtg_ncc_template_update(vtg_ctx *ctx, vtg_tmpl *t, const vtg_pyr *p, int level, int x0, int y0);
+/* n re-cuts at once (store_appearance for several slots or streams in one round, track.c:935-974): template t[k] from
+ * level[k] of p[k] at corner (x0[k], y0[k]), host arrays; one launch per run of up to 8 that share template size and
+ * level element type. Stream-ordered, no allocation. */
+int vtg_ncc_template_update_many(vtg_ctx *ctx, int n, vtg_tmpl *const *t, const vtg_pyr *const *p, const int *level,
+                                 const int *x0, const int *y0);
 /* The template's norm (sum of squares of its zero-mean values) and fp32 pixels (zero-mean, row-major) to host. */
 int vtg_ncc_template_read(vtg_ctx *ctx, const vtg_tmpl *t, float *app_host, float *norm);
 /* Search corners [x0, x1] x [y0, y1] at `step` (clipped to where the template fits, as search_slot clips), on the

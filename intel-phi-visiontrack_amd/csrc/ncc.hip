@@ -10,9 +10,11 @@
 // The bracketed term is the fp32 rounding of the stored template (|t - 64 u_t + mean| <= 2^-11 for |t| < 2^14):
 // at most 64 * 255 * 2^-11 per pixel, random in sign; it is left out, which moves a score by ~1e-8 (bounded by
 // 1.3e-5 only if every rounding had the same sign and the window were at the energy floor).
-// The fast search computes sum u u_t with v_dot4_u32_u8 (4 byte products per lane per instruction, exact u32), and
-// the box sums sum u, sum u^2 of every corner's window by sliding sums over the tile in LDS (down the columns, then
-// along the rows); the finish is in fp64.
+// The search of a 64-wide template computes sum u u_t on the int8 matrix cores (ncc_xcorr_mfma: the cross term of
+// every corner as a banded matrix product, exact i32), then ncc_cells_score forms the box sums sum u, sum u^2 of every
+// corner's window by sliding sums over a tile in LDS (down the columns, then along the rows) and scores in fp64.
+// Narrower templates (16, 32) take the dot4 kernel below, which computes sum u u_t with v_dot4_u32_u8 (4 byte
+// products per lane per instruction, exact u32) beside the same box sums.
 //
 // Fast kernel layout: a workgroup owns 256 x 16 template corners = 16 cells of the map. Its frame tile (corners plus
 // the template's extent) sits in LDS as 32-bit words. Wave s (0..3) takes corners x = X0 + 4 q + s for lane q, so all
@@ -85,10 +87,25 @@ __device__ __forceinline__ float tree16(const float *p)
 // row in row order, then the tree (store_appearance, track.c:948-966). For integer-valued levels the lane sums are
 // exact integers (at most 256 x 16,320 < 2^24), so they are formed in any order; the sum of squares of the zero-mean
 // values is not, and runs as 16 sequential chains.
+// One template of a batch: the level and corner it is cut from and where its records go.
+struct TJob {
+    Level lv;
+    int x0, y0;
+    float *t, *raw, *nm;
+    long long *su;
+    uint32_t *words; // level 0 (u8) only
+};
+
+constexpr int TMAX = 8; // templates per launch
+
+struct TBatch {
+    TJob j[TMAX];
+    int n, tw, th, wpr;
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void ncc_template_kernel(Level lv, int x0, int y0, int tw, int th, float *t, float *raw,
-                                                           float *out_norm_mean, long long *out_sum_u,
-                                                           uint32_t *words, int wpr)
+__device__ __forceinline__ void template_one(Level lv, int x0, int y0, int tw, int th, float *t, float *raw,
+                                             float *out_norm_mean, long long *out_sum_u, uint32_t *words, int wpr)
 {
     __shared__ float px[64 * 64];
     __shared__ int ipart[256];
@@ -205,6 +222,14 @@ __global__ __launch_bounds__(256) void ncc_template_kernel(Level lv, int x0, int
     }
 }
 
+// Grid: one workgroup per template of the batch (store_appearance for several slots or streams in one round).
+template <typename T>
+__global__ __launch_bounds__(256) void ncc_template_kernel(TBatch b)
+{
+    const TJob &j = b.j[blockIdx.x];
+    template_one<T>(j.lv, j.x0, j.y0, b.tw, b.th, j.t, j.raw, j.nm, j.su, j.words, b.wpr);
+}
+
 // ---------------------------------------------------------------- scores at given corners, the reference's arithmetic
 
 // The score of one corner by 16 lanes that stand for the reference's 16 vector lanes: lane j sums columns j, j+16, ...
@@ -297,6 +322,7 @@ struct SJob {
     vtg_cell *cells;
     vtg_found *found;
     Aux *aux;
+    int *prod;               // MFMA path: the cross term of every corner of the padded map, 16 cols x 16 rows wide
 };
 
 constexpr int SMAX = 8; // searches per launch
@@ -532,6 +558,260 @@ __global__ __launch_bounds__(256) void ncc_cells_fast(SBatch b)
         if (col < a.cols)
             a.cells[(size_t)blockIdx.y * a.cols + col] = vtg_cell{v, bx, by};
     }
+}
+
+// ---------------------------------------------------------------- cell map, MFMA: u8 level, 64-wide template
+
+// The cross term of a 64-wide template as a banded matrix product on the int8 matrix cores. For a 16 x 16 tile of
+// corners (x across, y down) and image row R = y_t + d of the tile's reach (d = 0 .. th + 14):
+//     P[x][y] += sum_k a[R][x + k] * b[d - (y - y_t)][k]          (k = 0 .. 63; b rows outside 0 .. th-1 are zero)
+// i.e. one v_mfma_i32_16x16x64_i8 with A = 16 x 64 bytes of image row R sliding in x and B = 64 x 16 template bytes,
+// column y holding template row d - y. Both operands are biased to signed bytes (a = u - 128, b = u_t - 128), which
+// the scoring kernel undoes exactly: sum u u_t = sum a b + 128 (sum u + sum u_t) - 16384 n. 64 of the th + 15 bands
+// of a tile are full, so th / (th + 15) of the matrix work is useful (81 % at th = 64).
+// A lane's A operand (row m = lane % 16, bytes 16 (lane / 16) .. + 15 of the window) is 16 unaligned bytes of an LDS
+// row: 5 aligned words funnel-shifted by the lane's byte offset. Its B operand (column n = lane % 16, the same 16 k)
+// is one aligned 16-byte read of a template row. The result tile lands as lane n + 16 q holding corners x = 4 q .. 4 q
+// + 3 of row y = n, stored as one 16-byte write per tile. Each wave owns 4 x 4 tiles (64 x 64 corners), so every
+// A operand feeds up to 4 products and every B operand 4.
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int XT = 4, YT = 4;           // tiles per wave, across and down
+constexpr int XB = 2 * 16 * XT;         // corners of a workgroup (2 x 2 waves) across
+constexpr int YBK = 2 * 16 * YT;        // and down
+constexpr int XS = XB + 80;             // LDS image row stride in bytes: the block, the template's reach, alignment
+constexpr int TS = 80;                  // LDS template row stride in bytes (64 + 16 spreads a wave's rows over banks)
+
+__host__ __device__ constexpr size_t xcorr_lds_bytes(int th)
+{
+    return (size_t)(YBK + th - 1) * XS + (size_t)(th + 30) * TS;
+}
+
+// Grid: (column blocks of XB corners, row blocks of YBK corners, searches) over each search's padded map
+// (16 cols x 16 rows corners); blocks past a search's own map leave at once.
+__global__ __launch_bounds__(256) void ncc_xcorr_mfma(SBatch b)
+{
+    extern __shared__ uint32_t lds[];
+    const SJob &a = b.j[blockIdx.z];
+    const int pcols = 16 * a.cols, prows = 16 * a.rows;
+    const int bx0 = blockIdx.x * XB, by0 = blockIdx.y * YBK; // block origin in map corners
+    if (bx0 >= pcols || by0 >= prows)
+        return; // the whole workgroup: no barrier is pending
+    const uint8_t *img = (const uint8_t *)a.lv.p;
+    const int ih = a.lv.h, pitch = a.lv.pitch, th = b.th;
+    const int X0 = a.x0 + bx0, Y0 = a.y0 + by0;
+    const int A = X0 & ~3, off = X0 - A;
+    constexpr int XW = XS / 4, TW = TS / 4;
+    const int TR = YBK + th - 1;
+    uint32_t *tile = lds, *tpl = lds + TR * XW;
+    const int tid = threadIdx.x;
+    {
+        // the image rows Y0 .. Y0 + TR - 1 from column A, biased to signed bytes; rows past the level repeat its
+        // last row and words past the pitch read 0 (only corners outside the search window see either)
+        constexpr int U = 8;
+        const int nw = TR * XW;
+        for (int i0 = 0; i0 < nw; i0 += 256 * U) {
+            uint32_t v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + tid + 256 * u;
+                const int r = i / XW, c = i - r * XW;
+                const int gy = Y0 + r < ih ? Y0 + r : ih - 1;
+                const int gx = A + 4 * c;
+                v[u] = (i < nw && gx + 3 < pitch) ? *(const uint32_t *)(img + (size_t)gy * pitch + gx) : 0x80808080u;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + tid + 256 * u;
+                if (i < nw)
+                    tile[i] = v[u] ^ 0x80808080u;
+            }
+        }
+        // the template, rows -15 .. th + 14 (zero outside 0 .. th - 1), biased; from the unshifted byte copy
+        const int ntw = (th + 30) * TW, wpr = 17;
+        for (int i = tid; i < ntw; i += 256) {
+            const int j = i / TW, m = i - j * TW, r = j - 15;
+            tpl[i] = (r >= 0 && r < th && m < 16) ? a.words[r * wpr + m] ^ 0x80808080u : 0u;
+        }
+    }
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int wx = wave & 1, wy = wave >> 1;
+    const int n16 = lane & 15, g = lane >> 4;
+    const int o0 = off + 64 * wx + n16 + 16 * g; // byte of this lane's x-tile 0 operand in a tile row
+    const int sh = o0 & 3;
+    const uint32_t *arow = tile + (64 * wy) * XW + (o0 >> 2);
+    const char *brow = (const char *)tpl + (15 - n16) * TS + 16 * g; // + d TS: template row d - n16 (+15)
+    v4i acc[YT][XT];
+#pragma unroll
+    for (int t = 0; t < YT; t++)
+#pragma unroll
+        for (int mx = 0; mx < XT; mx++)
+            acc[t][mx] = v4i{0, 0, 0, 0};
+    const int nR = 16 * YT + th - 1, band = th + 15;
+    for (int R = 0; R < nR; R++) {
+        v4i af[XT];
+#pragma unroll
+        for (int mx = 0; mx < XT; mx++) {
+            const uint32_t *p = arow + R * XW + 4 * mx;
+            const uint32_t w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3], w4 = p[4];
+            af[mx] = v4i{(int)__builtin_amdgcn_alignbyte(w1, w0, sh), (int)__builtin_amdgcn_alignbyte(w2, w1, sh),
+                         (int)__builtin_amdgcn_alignbyte(w3, w2, sh), (int)__builtin_amdgcn_alignbyte(w4, w3, sh)};
+        }
+#pragma unroll
+        for (int t = 0; t < YT; t++) {
+            const int d = R - 16 * t;
+            if (d >= 0 && d < band) {
+                const v4i bf = *(const v4i *)(brow + d * TS);
+#pragma unroll
+                for (int mx = 0; mx < XT; mx++)
+                    acc[t][mx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mx], bf, acc[t][mx], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < YT; t++)
+#pragma unroll
+        for (int mx = 0; mx < XT; mx++) {
+            const int tx = bx0 + 64 * wx + 16 * mx, ty = by0 + 64 * wy + 16 * t;
+            if (tx < pcols && ty < prows)
+                *(v4i *)(a.prod + (size_t)(ty + n16) * pcols + tx + 4 * g) = acc[t][mx];
+        }
+}
+
+// The cell map from the MFMA cross terms: the frame tile in LDS, the window sums sum u, sum u^2 of every corner by
+// sliding sums (down the columns, then along the rows, as in ncc_cells_fast), the score in fp64 and the first best
+// corner of each 16 x 16 cell (OP_SEARCH, track.c:301-329). Thread c owns corner column X0 + c; the 16 lanes of a cell
+// column reduce by shuffles. Grid: (column blocks of 16 cells, cell rows, searches).
+__host__ __device__ constexpr size_t score_lds_words(int th, int tw)
+{
+    return (size_t)(NY + th - 1) * (NX / 4 + tw / 4 + 2) + (size_t)2 * YB * (NX + tw - 1) + (size_t)2 * YB * NX;
+}
+
+__global__ __launch_bounds__(256) void ncc_cells_score(SBatch b)
+{
+    extern __shared__ uint32_t lds[];
+    const SJob &a = b.j[blockIdx.z];
+    if ((int)blockIdx.x * 16 >= a.cols || (int)blockIdx.y >= a.rows)
+        return; // the whole workgroup: no barrier is pending
+    const uint8_t *img = (const uint8_t *)a.lv.p;
+    const int ih = a.lv.h, pitch = a.lv.pitch, tw = b.tw, th = b.th, stp = b.step;
+    const int LW = NX / 4 + tw / 4 + 2, LR = NY + th - 1, VB = NX + tw - 1;
+    uint32_t *V1 = lds + LR * LW, *V2 = V1 + YB * VB, *B1 = V2 + YB * VB, *B2 = B1 + YB * NX;
+    const int X0 = a.x0 + blockIdx.x * NX, Y0 = a.y0 + blockIdx.y * NY;
+    const int A = X0 & ~3, off = X0 - A;
+    const int tid = threadIdx.x;
+    {
+        constexpr int U = 8;
+        const int nw = LR * LW;
+        for (int i0 = 0; i0 < nw; i0 += 256 * U) {
+            uint32_t v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + tid + 256 * u;
+                const int r = i / LW, c = i - r * LW;
+                const int gy = Y0 + r < ih ? Y0 + r : ih - 1;
+                const int gx = A + 4 * c;
+                v[u] = (i < nw && gx + 3 < pitch) ? *(const uint32_t *)(img + (size_t)gy * pitch + gx) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + tid + 256 * u;
+                if (i < nw)
+                    lds[i] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+    const uint8_t *tb = (const uint8_t *)lds; // row r, column c of the corners at r * 4 LW + off + c
+    const int pcols = 16 * a.cols;
+    const int cx = blockIdx.x * NX + tid;     // this thread's corner column in the map
+    const int x = X0 + tid;
+    const bool x_ok = cx < pcols && x <= a.x1 && ((x - a.x0) % stp) == 0;
+    const int *prow = a.prod + (size_t)(blockIdx.y * NY) * pcols + (cx < pcols ? cx : 0);
+    const double n = (double)(tw * th), norm = (double)a.nm[0], mean = (double)a.nm[1];
+    const double sum_t = (double)a.sum_t[0];
+    float best_v = -3.0f;
+    int best_x = 0, best_y = 0;
+    uint32_t vs1[2] = {0u, 0u}, vs2[2] = {0u, 0u};
+    for (int sp = 0; sp < NY / YB; sp++) {
+        int pv[YB];
+#pragma unroll
+        for (int d = 0; d < YB; d++)
+            pv[d] = cx < pcols ? prow[(size_t)(sp * YB + d) * pcols] : 0;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int bc = tid + 256 * q;
+            if (bc < VB) {
+                const uint8_t *col = tb + off + bc;
+                uint32_t s1 = vs1[q], s2 = vs2[q];
+                for (int d = 0; d < YB; d++) {
+                    const int rr = sp * YB + d;
+                    if (rr == 0) {
+                        for (int r = 0; r < th; r++) {
+                            const uint32_t u = col[r * 4 * LW];
+                            s1 += u;
+                            s2 += u * u;
+                        }
+                    } else {
+                        const uint32_t u_in = col[(rr + th - 1) * 4 * LW], u_out = col[(rr - 1) * 4 * LW];
+                        s1 += u_in - u_out;
+                        s2 += u_in * u_in - u_out * u_out;
+                    }
+                    V1[d * VB + bc] = s1;
+                    V2[d * VB + bc] = s2;
+                }
+                vs1[q] = s1;
+                vs2[q] = s2;
+            }
+        }
+        __syncthreads();
+        {
+            // box sums: task tid = (quantity, corner row, 16-corner segment), sliding right
+            const int row = tid >> 4, seg = tid & 15, d = row & (YB - 1);
+            const uint32_t *src = (row < YB ? V1 : V2) + d * VB;
+            uint32_t *dst = (row < YB ? B1 : B2) + d * NX;
+            const int c0 = seg * 16;
+            uint32_t s = 0;
+            for (int k = 0; k < tw; k++)
+                s += src[c0 + k];
+            dst[c0] = s;
+            for (int k = 1; k < 16; k++) {
+                s += src[c0 + k + tw - 1] - src[c0 + k - 1];
+                dst[c0 + k] = s;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < YB; d++) {
+            const int y = Y0 + sp * YB + d;
+            const int u0 = tb[(sp * YB + d) * 4 * LW + off + tid];
+            const double su = (double)B1[d * NX + tid], sq = (double)B2[d * NX + tid];
+            const double uut = (double)pv[d] + 128.0 * (su + sum_t) - 16384.0 * n; // sum u u_t, exact
+            const double pc = uut - (double)u0 * sum_t;
+            const double sv = su - n * (double)u0;
+            const double st = 4096.0 * pc - 64.0 * mean * sv;
+            const double sj = 64.0 * sv;
+            const double sjj = 4096.0 * (sq - 2.0 * (double)u0 * su + n * (double)u0 * (double)u0);
+            const double var = sjj - sj * sj / n;
+            const double sc = (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : st / sqrt(norm * var);
+            const float v = (float)sc;
+            const bool ok = x_ok && y <= a.y1 && ((y - a.y0) % stp) == 0;
+            if (ok && better(v, y, x, best_v, best_y, best_x))
+                best_v = v, best_x = x, best_y = y;
+        }
+    }
+    // the cell of this thread: lanes 16 c .. 16 c + 15 of a wave hold cell column c of the wave
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        const float v2 = __shfl_xor(best_v, o);
+        const int x2 = __shfl_xor(best_x, o), y2 = __shfl_xor(best_y, o);
+        if (better(v2, y2, x2, best_v, best_y, best_x))
+            best_v = v2, best_x = x2, best_y = y2;
+    }
+    const int col = blockIdx.x * 16 + (tid >> 4);
+    if ((tid & 15) == 0 && col < a.cols)
+        a.cells[(size_t)blockIdx.y * a.cols + col] = vtg_cell{best_v, best_x, best_y};
 }
 
 // ---------------------------------------------------------------- finish: top 8, half-pixel refinement, peaks
@@ -844,22 +1124,52 @@ int by_level_type(vtg_ctx *c, const vtg_pyr *p, int level, F &&f)
 
 }  // namespace
 
+// n templates, cut from pyramids p[k] at level[k], corner (x0[k], y0[k]), into t[k]: one launch per run of up to TMAX
+// consecutive templates that share size and level element type. Arguments were checked by the caller.
+int launch_ncc_templates(vtg_ctx *c, int n, vtg_tmpl *const *t, const vtg_pyr *const *p, const int *level,
+                         const int *x0, const int *y0)
+{
+    for (int k0 = 0; k0 < n;) {
+        const vtg_pyr *p0 = p[k0];
+        const int l0 = level[k0];
+        TBatch b{};
+        b.tw = t[k0]->tw;
+        b.th = t[k0]->th;
+        b.wpr = t[k0]->words_per_row;
+        int k = k0;
+        for (; k < n && b.n < TMAX; k++) {
+            if (t[k]->tw != b.tw || t[k]->th != b.th || p[k]->storage != p0->storage ||
+                p[k]->elem[level[k]] != p0->elem[l0])
+                break;
+            TJob &j = b.j[b.n++];
+            j.lv = p[k]->lv[level[k]];
+            j.x0 = x0[k];
+            j.y0 = y0[k];
+            j.t = t[k]->t;
+            j.raw = t[k]->raw;
+            j.nm = (float *)t[k]->block;
+            j.su = (long long *)((char *)t[k]->block + 16);
+            j.words = p[k]->elem[level[k]] == 1 ? t[k]->words : nullptr;
+        }
+        const int rc = by_level_type(c, p0, l0, [&](auto *tag) -> int {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            hipEvent_t ev;
+            timer_begin(c, "ncc_template", &ev);
+            hipLaunchKernelGGL(ncc_template_kernel<T>, dim3(b.n), dim3(256), 0, c->stream, b);
+            VTG_HIP(c, hipGetLastError());
+            timer_end(c, "ncc_template", ev);
+            return VTG_OK;
+        });
+        if (rc)
+            return rc;
+        k0 = k;
+    }
+    return VTG_OK;
+}
+
 int launch_ncc_template(vtg_ctx *c, const vtg_pyr *p, int level, int x0, int y0, vtg_tmpl *t)
 {
-    const Level lv = p->lv[level];
-    const bool u8 = p->elem[level] == 1;
-    float *nm = (float *)((char *)t->block + 0);
-    long long *su = (long long *)((char *)t->block + 16);
-    return by_level_type(c, p, level, [&](auto *tag) -> int {
-        using T = std::remove_pointer_t<decltype(tag)>;
-        hipEvent_t ev;
-        timer_begin(c, "ncc_template", &ev);
-        hipLaunchKernelGGL(ncc_template_kernel<T>, dim3(1), dim3(256), 0, c->stream, lv, x0, y0, t->tw, t->th, t->t,
-                           t->raw, nm, su, u8 ? t->words : nullptr, t->words_per_row);
-        VTG_HIP(c, hipGetLastError());
-        timer_end(c, "ncc_template", ev);
-        return VTG_OK;
-    });
+    return launch_ncc_templates(c, 1, &t, &p, &level, &x0, &y0);
 }
 
 int launch_ncc_score(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out)
@@ -880,7 +1190,9 @@ int launch_ncc_score(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int n, con
 static size_t aux_bytes() { return (sizeof(Aux) + 255) & ~(size_t)255; }
 
 // One launch per stage for up to SMAX searches that share template size, level element type and step.
-static int launch_batch(vtg_ctx *c, const vtg_pyr *p0, int level, bool fast, int wpr, SBatch &b)
+enum { MAP_GENERIC = 0, MAP_DOT4 = 1, MAP_MFMA = 2 };
+
+static int launch_batch(vtg_ctx *c, const vtg_pyr *p0, int level, int mode, int wpr, SBatch &b)
 {
     int maxcols = 0, maxrows = 0, maxcell = 0;
     for (int k = 0; k < b.n; k++) {
@@ -890,7 +1202,19 @@ static int launch_batch(vtg_ctx *c, const vtg_pyr *p0, int level, bool fast, int
     }
     hipEvent_t ev;
     if (maxcell > 0) {
-        if (fast) {
+        if (mode == MAP_MFMA) {
+            const dim3 gx((16 * maxcols + XB - 1) / XB, (16 * maxrows + YBK - 1) / YBK, b.n);
+            timer_begin(c, "ncc_xcorr", &ev);
+            hipLaunchKernelGGL(ncc_xcorr_mfma, gx, dim3(256), xcorr_lds_bytes(b.th), c->stream, b);
+            VTG_HIP(c, hipGetLastError());
+            timer_end(c, "ncc_xcorr", ev);
+            const dim3 gs((maxcols + 15) / 16, maxrows, b.n);
+            timer_begin(c, "ncc_cells", &ev);
+            hipLaunchKernelGGL(ncc_cells_score, gs, dim3(256), score_lds_words(b.th, b.tw) * sizeof(uint32_t),
+                               c->stream, b);
+            VTG_HIP(c, hipGetLastError());
+            timer_end(c, "ncc_cells", ev);
+        } else if (mode == MAP_DOT4) {
             const dim3 grid((maxcols + 15) / 16, maxrows, b.n);
             const size_t lds = fast_lds_words(b.th, b.tw, wpr) * sizeof(uint32_t);
             timer_begin(c, "ncc_cells", &ev);
@@ -970,8 +1294,13 @@ static int search_many(vtg_ctx *c, int n, const vtg_pyr *const *p, const vtg_tmp
             return fail(c, VTG_EINVAL, "vtg_ncc_search: search %d differs from the first in template size or level "
                                        "type", k);
     }
-    const bool fast = p[0]->elem[level0] == 1 && t0->words && (t0->th % YB) == 0 && t0->th >= YB &&
-                      (t0->words_per_row == 17 || t0->words_per_row == 9 || t0->words_per_row == 5);
+    // u8 level: 64-wide templates go to the matrix cores, 16- and 32-wide ones to the dot4 kernel
+    const bool u8 = p[0]->elem[level0] == 1 && t0->words;
+    const int mode = u8 && t0->tw == 64 ? MAP_MFMA
+                     : u8 && (t0->th % YB) == 0 && t0->th >= YB &&
+                               (t0->words_per_row == 17 || t0->words_per_row == 9 || t0->words_per_row == 5)
+                         ? MAP_DOT4
+                         : MAP_GENERIC;
     // the scratch: aux records, then found records (when the caller has none), then maps (for searches without one)
     std::vector<int> win((size_t)4 * n), ncell(n);
     const size_t auxb = aux_bytes();
@@ -988,6 +1317,8 @@ static int search_many(vtg_ctx *c, int n, const vtg_pyr *const *p, const vtg_tmp
         ncell[k] = (x1 < x0 || y1 < y0) ? 0 : ((x1 - x0) / 16 + 1) * ((y1 - y0) / 16 + 1);
         if (!(cells && cells[k]))
             bytes += ((size_t)ncell[k] * sizeof(vtg_cell) + 255) & ~(size_t)255;
+        if (mode == MAP_MFMA)
+            bytes += (size_t)ncell[k] * 256 * sizeof(int); // the cross terms of the padded map
     }
     void *sc = nullptr;
     int rc = scratch(c, bytes, &sc);
@@ -1023,8 +1354,13 @@ static int search_many(vtg_ctx *c, int n, const vtg_pyr *const *p, const vtg_tmp
             }
             j.found = found_dev ? found_dev + k : (vtg_found *)(fnd0 + (size_t)k * 256);
             j.aux = (Aux *)(aux0 + (size_t)k * auxb);
+            j.prod = nullptr;
+            if (mode == MAP_MFMA) {
+                j.prod = (int *)cel;
+                cel += (size_t)ncell[k] * 256 * sizeof(int);
+            }
         }
-        if ((rc = launch_batch(c, p[0], level0, fast, t0->words_per_row, b)))
+        if ((rc = launch_batch(c, p[0], level0, mode, t0->words_per_row, b)))
             return rc;
     }
     return VTG_OK;
@@ -1104,6 +1440,29 @@ int vtg_ncc_template_update(vtg_ctx *c, vtg_tmpl *t, const vtg_pyr *p, int level
         return fail(c, VTG_EINVAL, "vtg_ncc_template_update: level element size differs from the template's");
     t->level = level;
     return launch_ncc_template(c, p, level, x0, y0, t);
+}
+
+int vtg_ncc_template_update_many(vtg_ctx *c, int n, vtg_tmpl *const *t, const vtg_pyr *const *p, const int *level,
+                                 const int *x0, const int *y0)
+{
+    if (!c)
+        return VTG_EINVAL;
+    if (n < 0 || (n > 0 && (!t || !p || !level || !x0 || !y0)))
+        return fail(c, VTG_EINVAL, "vtg_ncc_template_update_many: bad argument");
+    for (int k = 0; k < n; k++) {
+        if (!t[k] || !p[k] || level[k] < 0 || level[k] >= p[k]->levels)
+            return fail(c, VTG_EINVAL, "vtg_ncc_template_update_many: bad template, pyramid or level %d", k);
+        const Level &lv = p[k]->lv[level[k]];
+        if (x0[k] < 1 || y0[k] < 1 || x0[k] + t[k]->tw >= lv.w - 1 || y0[k] + t[k]->th >= lv.h - 1)
+            return fail(c, VTG_EINVAL, "vtg_ncc_template_update_many: window %d at (%d,%d) does not fit level %d", k,
+                        x0[k], y0[k], level[k]);
+        if ((p[k]->elem[level[k]] == 1) != (t[k]->words != nullptr))
+            return fail(c, VTG_EINVAL, "vtg_ncc_template_update_many: level element size of %d differs from its "
+                                       "template's", k);
+    }
+    for (int k = 0; k < n; k++)
+        t[k]->level = level[k];
+    return launch_ncc_templates(c, n, t, p, level, x0, y0);
 }
 
 int vtg_ncc_template_destroy(vtg_ctx *c, vtg_tmpl *t)

@@ -92,7 +92,7 @@ def lib():
         "vtg_lk_track_ex": [V, V, V, I, C.POINTER(LkCfg), I, V, V, F, I, V],
         "vtg_lk_track_many": [V, I, V, V, V, V, C.POINTER(LkCfg), V, V, F, I, V], "vtg_lk_set_margin": [V, I],
         "vtg_ncc_template": [V, V, I, I, I, I, I, C.POINTER(V)], "vtg_ncc_template_destroy": [V, V],
-        "vtg_ncc_template_update": [V, V, V, I, I, I],
+        "vtg_ncc_template_update": [V, V, V, I, I, I], "vtg_ncc_template_update_many": [V, I, V, V, V, V, V],
         "vtg_ncc_template_read": [V, V, V, C.POINTER(F)],
         "vtg_ncc_search": [V, V, V, I, I, I, I, I, V, C.POINTER(Found)],
         "vtg_ncc_search_async": [V, V, V, I, I, I, I, I, V, V],
@@ -132,7 +132,7 @@ EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", 
            "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_timing_enable", "vtg_timing_read",
            "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_level",
            "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_margin", "vtg_ncc_template", "vtg_ncc_template_destroy",
-           "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_many",
+           "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_template_update_many", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_many",
            "vtg_tune_pair", "vtg_tune_get", "vtg_tune_reset", "vtg_srv_create", "vtg_srv_destroy", "vtg_srv_error",
            "vtg_srv_open", "vtg_srv_close", "vtg_srv_submit", "vtg_srv_poll", "vtg_srv_wait", "vtg_srv_get_stats",
            "vtg_ncc_score",
@@ -339,6 +339,18 @@ class Context:
     # NCC
     def ncc_template(self, pyr, level, x0, y0, tw, th):
         return Template(self, pyr, level, x0, y0, tw, th)
+
+    def ncc_templates_update(self, tmpls, pyrs, levels, corners):
+        """Re-cut tmpls[k] from levels[k] of pyrs[k] at corners[k] = (x0, y0), one launch per run of up to 8."""
+        n = len(tmpls)
+        T = (C.c_void_p * n)(*[t.h for t in tmpls])
+        P = (C.c_void_p * n)(*[p.h for p in pyrs])
+        Lv = (C.c_int * n)(*levels)
+        X = (C.c_int * n)(*[int(c[0]) for c in corners])
+        Y = (C.c_int * n)(*[int(c[1]) for c in corners])
+        self.check(self.L.vtg_ncc_template_update_many(self.h, n, T, P, Lv, X, Y), "ncc_template_update_many")
+        for t, lv in zip(tmpls, levels):
+            t.level = lv
 
     def ncc_search(self, pyr, tmpl, x0, x1, y0, y1, step=1, cells=None):
         f = Found()

@@ -78,7 +78,8 @@ def _check_cells(gcells, ocells, score_at):
 
 
 @pytest.mark.parametrize("w,h,tw,th,step", [(400, 300, 64, 64, 1), (400, 300, 32, 24, 1), (400, 300, 16, 16, 2),
-                                            (400, 300, 32, 21, 1), (400, 300, 64, 64, 2)])
+                                            (400, 300, 32, 21, 1), (400, 300, 64, 64, 2), (400, 300, 64, 40, 1),
+                                            (333, 257, 64, 21, 2), (517, 203, 64, 7, 1)])
 def test_search_matches_oracle(ctx, oracle, w, h, tw, th, step):
     f0, f1, mo, p0, p1, k0, k1 = _setup(ctx, oracle, w, h, 31 + tw + step)
     x0, y0 = _textured_corner(f0, tw, th, tw + th)
@@ -171,3 +172,34 @@ def test_search_many_matches_single(ctx, tw, th, level, step):
     for p0, p1 in pyrs:
         p0.destroy()
         p1.destroy()
+
+
+def test_template_update_many_matches_single(ctx):
+    """Re-cutting several templates in one call (mixed levels, so the u8 and u16 runs launch apart, and more than one
+    launch holds) gives each template the bits of its own single cut."""
+    sizes = [(640, 480), (800, 600), (1280, 720), (640, 480), (960, 540), (720, 400), (1024, 768), (640, 480),
+             (800, 600), (640, 480), (1280, 720)]
+    pyrs, levels, corners = [], [], []
+    for k, (w, h) in enumerate(sizes):
+        f0, _, _ = vtgpu.frame_pair(w, h, 700 + k)
+        p = ctx.pyramid(w, h, 2)
+        p.build(f0)
+        pyrs.append(p)
+        lv = 1 if k in (3, 4, 9) else 0
+        levels.append(lv)
+        lw, lh = (w, h) if lv == 0 else ((w + 1) // 2, (h + 1) // 2)
+        corners.append((lw // 3 + k, lh // 4 + 2 * k))
+    tw, th = 32, 21
+    many = [ctx.ncc_template(pyrs[k], levels[k], 5, 5, tw, th) for k in range(len(sizes))]
+    say("ncc template update_many")
+    ctx.ncc_templates_update(many, pyrs, levels, corners)
+    for k in range(len(sizes)):
+        one = ctx.ncc_template(pyrs[k], levels[k], corners[k][0], corners[k][1], tw, th)
+        a1, n1 = one.read()
+        a2, n2 = many[k].read()
+        assert np.array_equal(a1, a2) and n1 == n2, k
+        one.destroy()
+    for t in many:
+        t.destroy()
+    for p in pyrs:
+        p.destroy()
