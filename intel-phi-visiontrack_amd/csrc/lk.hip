@@ -557,8 +557,10 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
     o.status = status;
 }
 
+// The direct schedule of windows up to 7 rows per lane fits 128 VGPRs without spilling: bound it to 4 waves per SIMD
+// (latency hiding is what the iterations need); the other forms keep the compiler's allocation.
 template <int MAXR, typename T0, typename T1, int SOLVE, bool STAGED>
-__global__ __launch_bounds__(512) void lk_kernel(LkArgs a)
+__global__ __launch_bounds__(512, (!STAGED && MAXR <= 7) ? 4 : 1) void lk_kernel(LkArgs a)
 {
     extern __shared__ float smem[];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

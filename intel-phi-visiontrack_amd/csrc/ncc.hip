@@ -581,6 +581,7 @@ constexpr int XB = 2 * 16 * XT;         // corners of a workgroup (2 x 2 waves) 
 constexpr int YBK = 2 * 16 * YT;        // and down
 constexpr int XS = XB + 80;             // LDS image row stride in bytes: the block, the template's reach, alignment
 constexpr int TS = 80;                  // LDS template row stride in bytes (64 + 16 spreads a wave's rows over banks)
+constexpr int XC_LOADS = ((YBK + 63) * (XS / 4) + 255) / 256; // tile words per thread at th = 64
 
 __host__ __device__ constexpr size_t xcorr_lds_bytes(int th)
 {
@@ -608,24 +609,22 @@ __global__ __launch_bounds__(256) void ncc_xcorr_mfma(SBatch b)
     {
         // the image rows Y0 .. Y0 + TR - 1 from column A, biased to signed bytes; rows past the level repeat its
         // last row and words past the pitch read 0 (only corners outside the search window see either)
-        constexpr int U = 8;
+        constexpr int U = XC_LOADS; // every load of the tile in flight at once
         const int nw = TR * XW;
-        for (int i0 = 0; i0 < nw; i0 += 256 * U) {
-            uint32_t v[U];
+        uint32_t v[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int i = i0 + tid + 256 * u;
-                const int r = i / XW, c = i - r * XW;
-                const int gy = Y0 + r < ih ? Y0 + r : ih - 1;
-                const int gx = A + 4 * c;
-                v[u] = (i < nw && gx + 3 < pitch) ? *(const uint32_t *)(img + (size_t)gy * pitch + gx) : 0x80808080u;
-            }
+        for (int u = 0; u < U; u++) {
+            const int i = tid + 256 * u;
+            const int r = i / XW, c = i - r * XW;
+            const int gy = Y0 + r < ih ? Y0 + r : ih - 1;
+            const int gx = A + 4 * c;
+            v[u] = (i < nw && gx + 3 < pitch) ? *(const uint32_t *)(img + (size_t)gy * pitch + gx) : 0x80808080u;
+        }
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int i = i0 + tid + 256 * u;
-                if (i < nw)
-                    tile[i] = v[u] ^ 0x80808080u;
-            }
+        for (int u = 0; u < U; u++) {
+            const int i = tid + 256 * u;
+            if (i < nw)
+                tile[i] = v[u] ^ 0x80808080u;
         }
         // the template, rows -15 .. th + 14 (zero outside 0 .. th - 1), biased; from the unshifted byte copy
         const int ntw = (th + 30) * TW, wpr = 17;
@@ -679,13 +678,28 @@ __global__ __launch_bounds__(256) void ncc_xcorr_mfma(SBatch b)
         }
 }
 
-// The cell map from the MFMA cross terms: the frame tile in LDS, the window sums sum u, sum u^2 of every corner by
-// sliding sums (down the columns, then along the rows, as in ncc_cells_fast), the score in fp64 and the first best
-// corner of each 16 x 16 cell (OP_SEARCH, track.c:301-329). Thread c owns corner column X0 + c; the 16 lanes of a cell
-// column reduce by shuffles. Grid: (column blocks of 16 cells, cell rows, searches).
-__host__ __device__ constexpr size_t score_lds_words(int th, int tw)
+// The cell map from the MFMA cross terms: the frame tile in LDS; for each corner row, the window sums of every byte
+// column (sum u, sum u^2 over th rows, slid down by one row at a time) and their prefix sums along the row (a scan
+// per wave, the waves' totals through LDS), so a corner's box sums are two differences of prefixes; the score in fp64
+// and the first best corner of each 16 x 16 cell (OP_SEARCH, track.c:301-329). Thread c owns corner column X0 + c
+// and byte column c (threads 0 .. 62 also column 256 + c); the 16 lanes of a cell column reduce by shuffles.
+// Grid: (column blocks of 16 cells, cell rows, searches).
+constexpr int SC_LW = NX / 4 + 64 / 4 + 2;   // tile row in words: the 256 corners, the template's reach, alignment
+constexpr int SC_LOADS = 26;                 // tile words per thread: (16 + 64 - 1) rows x SC_LW <= 256 x 26
+
+__host__ __device__ constexpr size_t score_lds_words(int th)
 {
-    return (size_t)(NY + th - 1) * (NX / 4 + tw / 4 + 2) + (size_t)2 * YB * (NX + tw - 1) + (size_t)2 * YB * NX;
+    return (size_t)(NY + th - 1) * SC_LW + 2 * 2 * 320 + 2 * 16;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o);
+        v += lane >= o ? u : 0u;
+    }
+    return v;
 }
 
 __global__ __launch_bounds__(256) void ncc_cells_score(SBatch b)
@@ -696,110 +710,117 @@ __global__ __launch_bounds__(256) void ncc_cells_score(SBatch b)
         return; // the whole workgroup: no barrier is pending
     const uint8_t *img = (const uint8_t *)a.lv.p;
     const int ih = a.lv.h, pitch = a.lv.pitch, tw = b.tw, th = b.th, stp = b.step;
-    const int LW = NX / 4 + tw / 4 + 2, LR = NY + th - 1, VB = NX + tw - 1;
-    uint32_t *V1 = lds + LR * LW, *V2 = V1 + YB * VB, *B1 = V2 + YB * VB, *B2 = B1 + YB * NX;
+    const int LR = NY + th - 1, VB = NX + tw - 1;
+    uint32_t *E = lds + LR * SC_LW; // [row parity][quantity][320]: exclusive prefix sums of the column sums
+    uint32_t *tot = E + 2 * 2 * 320; // [row parity][quantity][8]: each wave's total
     const int X0 = a.x0 + blockIdx.x * NX, Y0 = a.y0 + blockIdx.y * NY;
     const int A = X0 & ~3, off = X0 - A;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int pcols = 16 * a.cols;
+    const int cx = blockIdx.x * NX + tid; // this thread's corner column in the map
+    const bool in_map = cx < pcols;
+    int pv[NY];
     {
-        constexpr int U = 8;
-        const int nw = LR * LW;
-        for (int i0 = 0; i0 < nw; i0 += 256 * U) {
-            uint32_t v[U];
+        // every load (the tile, this column's 16 cross terms) issued before the first store
+        const int nw = LR * SC_LW;
+        uint32_t v[SC_LOADS];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int i = i0 + tid + 256 * u;
-                const int r = i / LW, c = i - r * LW;
-                const int gy = Y0 + r < ih ? Y0 + r : ih - 1;
-                const int gx = A + 4 * c;
-                v[u] = (i < nw && gx + 3 < pitch) ? *(const uint32_t *)(img + (size_t)gy * pitch + gx) : 0u;
-            }
+        for (int u = 0; u < SC_LOADS; u++) {
+            const int i = tid + 256 * u;
+            const int r = i / SC_LW, c = i - r * SC_LW;
+            const int gy = Y0 + r < ih ? Y0 + r : ih - 1;
+            const int gx = A + 4 * c;
+            v[u] = (i < nw && gx + 3 < pitch) ? *(const uint32_t *)(img + (size_t)gy * pitch + gx) : 0u;
+        }
+        const int *prow = a.prod + (size_t)(blockIdx.y * NY) * pcols + (in_map ? cx : 0);
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int i = i0 + tid + 256 * u;
-                if (i < nw)
-                    lds[i] = v[u];
-            }
+        for (int d = 0; d < NY; d++)
+            pv[d] = in_map ? prow[(size_t)d * pcols] : 0;
+#pragma unroll
+        for (int u = 0; u < SC_LOADS; u++) {
+            const int i = tid + 256 * u;
+            if (i < nw)
+                lds[i] = v[u];
         }
     }
     __syncthreads();
-    const uint8_t *tb = (const uint8_t *)lds; // row r, column c of the corners at r * 4 LW + off + c
-    const int pcols = 16 * a.cols;
-    const int cx = blockIdx.x * NX + tid;     // this thread's corner column in the map
+    const uint8_t *tb = (const uint8_t *)lds;
+    constexpr int RB = 4 * SC_LW; // tile row in bytes
+    const uint8_t *col0 = tb + off + tid, *col1 = col0 + 256;
+    const bool two = tid < VB - 256; // this thread also owns byte column 256 + tid
     const int x = X0 + tid;
-    const bool x_ok = cx < pcols && x <= a.x1 && ((x - a.x0) % stp) == 0;
-    const int *prow = a.prod + (size_t)(blockIdx.y * NY) * pcols + (cx < pcols ? cx : 0);
+    const bool x_ok = in_map && x <= a.x1 && ((x - a.x0) % stp) == 0;
     const double n = (double)(tw * th), norm = (double)a.nm[0], mean = (double)a.nm[1];
     const double sum_t = (double)a.sum_t[0];
     float best_v = -3.0f;
     int best_x = 0, best_y = 0;
-    uint32_t vs1[2] = {0u, 0u}, vs2[2] = {0u, 0u};
-    for (int sp = 0; sp < NY / YB; sp++) {
-        int pv[YB];
-#pragma unroll
-        for (int d = 0; d < YB; d++)
-            pv[d] = cx < pcols ? prow[(size_t)(sp * YB + d) * pcols] : 0;
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int bc = tid + 256 * q;
-            if (bc < VB) {
-                const uint8_t *col = tb + off + bc;
-                uint32_t s1 = vs1[q], s2 = vs2[q];
-                for (int d = 0; d < YB; d++) {
-                    const int rr = sp * YB + d;
-                    if (rr == 0) {
-                        for (int r = 0; r < th; r++) {
-                            const uint32_t u = col[r * 4 * LW];
-                            s1 += u;
-                            s2 += u * u;
-                        }
-                    } else {
-                        const uint32_t u_in = col[(rr + th - 1) * 4 * LW], u_out = col[(rr - 1) * 4 * LW];
-                        s1 += u_in - u_out;
-                        s2 += u_in * u_in - u_out * u_out;
-                    }
-                    V1[d * VB + bc] = s1;
-                    V2[d * VB + bc] = s2;
-                }
-                vs1[q] = s1;
-                vs2[q] = s2;
+    uint32_t s1a = 0, s2a = 0, s1b = 0, s2b = 0;
+    for (int d = 0; d < NY; d++) {
+        // the column sums of corner row d: rows d .. d + th - 1 of byte columns tid (and 256 + tid)
+        if (d == 0) {
+            for (int r = 0; r < th; r++) {
+                const uint32_t ua = col0[r * RB], ub = two ? col1[r * RB] : 0u;
+                s1a += ua;
+                s2a += ua * ua;
+                s1b += ub;
+                s2b += ub * ub;
+            }
+        } else {
+            const uint32_t ia = col0[(d + th - 1) * RB], oa = col0[(d - 1) * RB];
+            s1a += ia - oa;
+            s2a += ia * ia - oa * oa;
+            if (two) {
+                const uint32_t ib = col1[(d + th - 1) * RB], ob = col1[(d - 1) * RB];
+                s1b += ib - ob;
+                s2b += ib * ib - ob * ob;
+            }
+        }
+        const uint32_t i1a = wave_incl_scan(s1a, lane), i2a = wave_incl_scan(s2a, lane);
+        uint32_t i1b = 0, i2b = 0;
+        if (wave == 0) {
+            i1b = wave_incl_scan(s1b, lane);
+            i2b = wave_incl_scan(s2b, lane);
+        }
+        uint32_t *Eb = E + (d & 1) * 640, *Tb = tot + (d & 1) * 16;
+        if (lane == 63) {
+            Tb[wave] = i1a;
+            Tb[8 + wave] = i2a;
+            if (wave == 0) {
+                Tb[4] = i1b;
+                Tb[12] = i2b;
             }
         }
         __syncthreads();
-        {
-            // box sums: task tid = (quantity, corner row, 16-corner segment), sliding right
-            const int row = tid >> 4, seg = tid & 15, d = row & (YB - 1);
-            const uint32_t *src = (row < YB ? V1 : V2) + d * VB;
-            uint32_t *dst = (row < YB ? B1 : B2) + d * NX;
-            const int c0 = seg * 16;
-            uint32_t s = 0;
-            for (int k = 0; k < tw; k++)
-                s += src[c0 + k];
-            dst[c0] = s;
-            for (int k = 1; k < 16; k++) {
-                s += src[c0 + k + tw - 1] - src[c0 + k - 1];
-                dst[c0 + k] = s;
-            }
+        uint32_t o1 = 0, o2 = 0;
+        for (int w2 = 0; w2 < wave; w2++) {
+            o1 += Tb[w2];
+            o2 += Tb[8 + w2];
+        }
+        Eb[tid] = o1 + i1a - s1a;
+        Eb[320 + tid] = o2 + i2a - s2a;
+        if (wave == 0) {
+            const uint32_t t1 = Tb[0] + Tb[1] + Tb[2] + Tb[3], t2 = Tb[8] + Tb[9] + Tb[10] + Tb[11];
+            Eb[256 + lane] = t1 + i1b - s1b;
+            Eb[576 + lane] = t2 + i2b - s2b;
         }
         __syncthreads();
-#pragma unroll
-        for (int d = 0; d < YB; d++) {
-            const int y = Y0 + sp * YB + d;
-            const int u0 = tb[(sp * YB + d) * 4 * LW + off + tid];
-            const double su = (double)B1[d * NX + tid], sq = (double)B2[d * NX + tid];
-            const double uut = (double)pv[d] + 128.0 * (su + sum_t) - 16384.0 * n; // sum u u_t, exact
-            const double pc = uut - (double)u0 * sum_t;
-            const double sv = su - n * (double)u0;
-            const double st = 4096.0 * pc - 64.0 * mean * sv;
-            const double sj = 64.0 * sv;
-            const double sjj = 4096.0 * (sq - 2.0 * (double)u0 * su + n * (double)u0 * (double)u0);
-            const double var = sjj - sj * sj / n;
-            const double sc = (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : st / sqrt(norm * var);
-            const float v = (float)sc;
-            const bool ok = x_ok && y <= a.y1 && ((y - a.y0) % stp) == 0;
-            if (ok && better(v, y, x, best_v, best_y, best_x))
-                best_v = v, best_x = x, best_y = y;
-        }
+        // corner (x, y): box sums over columns tid .. tid + tw - 1 of the column sums
+        const uint32_t box1 = Eb[tid + tw] - Eb[tid], box2 = Eb[320 + tid + tw] - Eb[320 + tid];
+        const int y = Y0 + d;
+        const int u0 = col0[d * RB];
+        const double su = (double)box1, sq = (double)box2;
+        const double uut = (double)pv[d] + 128.0 * (su + sum_t) - 16384.0 * n; // sum u u_t, exact
+        const double pc = uut - (double)u0 * sum_t;
+        const double sv = su - n * (double)u0;
+        const double st = 4096.0 * pc - 64.0 * mean * sv;
+        const double sj = 64.0 * sv;
+        const double sjj = 4096.0 * (sq - 2.0 * (double)u0 * su + n * (double)u0 * (double)u0);
+        const double var = sjj - sj * sj / n;
+        const double sc = (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : st / sqrt(norm * var);
+        const float v = (float)sc;
+        const bool ok = x_ok && y <= a.y1 && ((y - a.y0) % stp) == 0;
+        if (ok && better(v, y, x, best_v, best_y, best_x))
+            best_v = v, best_x = x, best_y = y;
     }
     // the cell of this thread: lanes 16 c .. 16 c + 15 of a wave hold cell column c of the wave
 #pragma unroll
@@ -1210,7 +1231,7 @@ static int launch_batch(vtg_ctx *c, const vtg_pyr *p0, int level, int mode, int 
             timer_end(c, "ncc_xcorr", ev);
             const dim3 gs((maxcols + 15) / 16, maxrows, b.n);
             timer_begin(c, "ncc_cells", &ev);
-            hipLaunchKernelGGL(ncc_cells_score, gs, dim3(256), score_lds_words(b.th, b.tw) * sizeof(uint32_t),
+            hipLaunchKernelGGL(ncc_cells_score, gs, dim3(256), score_lds_words(b.th) * sizeof(uint32_t),
                                c->stream, b);
             VTG_HIP(c, hipGetLastError());
             timer_end(c, "ncc_cells", ev);
