@@ -362,6 +362,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20, help="graph replays of the uneven 64-pair batch (0: skip)")
     ap.add_argument("--serve-frames", type=int, default=300, help="frames per stream of the serving runs (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample (0: skip)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="one device-to-device copy of this many MiB first: a plain copy of known size that calibrates "
+                         "the HBM byte counters of a profiled run (0: none)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (streams shard with no "
@@ -378,6 +381,13 @@ def main():
     rng = np.random.default_rng(2)
     corners = [(int(rng.integers(100, W - 200)), int(rng.integers(100, H - 200))) for _ in range(UNIQ)]
     up = vtgpu.Context(0)
+    if a.copy_mib > 0:
+        src, dst = up.alloc(a.copy_mib << 20), up.alloc(a.copy_mib << 20)
+        up.check(vtgpu.lib().vtg_copy_d2d(up.h, dst.ptr, src.ptr, a.copy_mib << 20), "copy_d2d")
+        up.sync()
+        log(f"calibration copy: {a.copy_mib} MiB device to device")
+        src.free()
+        dst.free()
     d_frames = [(up.to_device(f0s[i % UNIQ]), up.to_device(f1s[i % UNIQ])) for i in range(pairs)]
     up.sync()
 

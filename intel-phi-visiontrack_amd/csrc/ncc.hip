@@ -583,9 +583,40 @@ constexpr int XS = XB + 80;             // LDS image row stride in bytes: the bl
 constexpr int TS = 80;                  // LDS template row stride in bytes (64 + 16 spreads a wave's rows over banks)
 constexpr int XC_LOADS = ((YBK + 63) * (XS / 4) + 255) / 256; // tile words per thread at th = 64
 
+// Templates shorter than 33 rows run as 33 rows whose extra rows are zero, so the band of every tile is at least 48
+// rows and the phases of the product loop come in one order.
+__host__ __device__ constexpr int xcorr_th(int th) { return th < 33 ? 33 : th; }
+
 __host__ __device__ constexpr size_t xcorr_lds_bytes(int th)
 {
-    return (size_t)(YBK + th - 1) * XS + (size_t)(th + 30) * TS;
+    return (size_t)(YBK + xcorr_th(th) - 1) * XS + (size_t)(xcorr_th(th) + 30) * TS;
+}
+
+// Rows r0 .. r1 - 1 of a phase in which y-tiles TLO .. THI are active: each row's A operands (XT x-tiles) and, per
+// active tile, its B operand and XT products.
+template <int TLO, int THI>
+__device__ __forceinline__ void xcorr_phase(int r0, int r1, const uint32_t *arow, const char *brow, int sh,
+                                            v4i (&acc)[YT][XT])
+{
+    constexpr int XW = XS / 4;
+#pragma unroll 2
+    for (int R = r0; R < r1; R++) {
+        v4i af[XT];
+#pragma unroll
+        for (int mx = 0; mx < XT; mx++) {
+            const uint32_t *p = arow + R * XW + 4 * mx;
+            const uint32_t w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3], w4 = p[4];
+            af[mx] = v4i{(int)__builtin_amdgcn_alignbyte(w1, w0, sh), (int)__builtin_amdgcn_alignbyte(w2, w1, sh),
+                         (int)__builtin_amdgcn_alignbyte(w3, w2, sh), (int)__builtin_amdgcn_alignbyte(w4, w3, sh)};
+        }
+#pragma unroll
+        for (int t = TLO; t <= THI; t++) {
+            const v4i bf = *(const v4i *)(brow + (R - 16 * t) * TS);
+#pragma unroll
+            for (int mx = 0; mx < XT; mx++)
+                acc[t][mx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mx], bf, acc[t][mx], 0, 0, 0);
+        }
+    }
 }
 
 // Grid: (column blocks of XB corners, row blocks of YBK corners, searches) over each search's padded map
@@ -599,11 +630,11 @@ __global__ __launch_bounds__(256) void ncc_xcorr_mfma(SBatch b)
     if (bx0 >= pcols || by0 >= prows)
         return; // the whole workgroup: no barrier is pending
     const uint8_t *img = (const uint8_t *)a.lv.p;
-    const int ih = a.lv.h, pitch = a.lv.pitch, th = b.th;
+    const int ih = a.lv.h, pitch = a.lv.pitch, th = b.th, the = xcorr_th(th);
     const int X0 = a.x0 + bx0, Y0 = a.y0 + by0;
     const int A = X0 & ~3, off = X0 - A;
     constexpr int XW = XS / 4, TW = TS / 4;
-    const int TR = YBK + th - 1;
+    const int TR = YBK + the - 1;
     uint32_t *tile = lds, *tpl = lds + TR * XW;
     const int tid = threadIdx.x;
     {
@@ -626,8 +657,8 @@ __global__ __launch_bounds__(256) void ncc_xcorr_mfma(SBatch b)
             if (i < nw)
                 tile[i] = v[u] ^ 0x80808080u;
         }
-        // the template, rows -15 .. th + 14 (zero outside 0 .. th - 1), biased; from the unshifted byte copy
-        const int ntw = (th + 30) * TW, wpr = 17;
+        // the template, rows -15 .. the + 14 (zero outside 0 .. th - 1), biased; from the unshifted byte copy
+        const int ntw = (the + 30) * TW, wpr = 17;
         for (int i = tid; i < ntw; i += 256) {
             const int j = i / TW, m = i - j * TW, r = j - 15;
             tpl[i] = (r >= 0 && r < th && m < 16) ? a.words[r * wpr + m] ^ 0x80808080u : 0u;
@@ -647,27 +678,16 @@ __global__ __launch_bounds__(256) void ncc_xcorr_mfma(SBatch b)
 #pragma unroll
         for (int mx = 0; mx < XT; mx++)
             acc[t][mx] = v4i{0, 0, 0, 0};
-    const int nR = 16 * YT + th - 1, band = th + 15;
-    for (int R = 0; R < nR; R++) {
-        v4i af[XT];
-#pragma unroll
-        for (int mx = 0; mx < XT; mx++) {
-            const uint32_t *p = arow + R * XW + 4 * mx;
-            const uint32_t w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3], w4 = p[4];
-            af[mx] = v4i{(int)__builtin_amdgcn_alignbyte(w1, w0, sh), (int)__builtin_amdgcn_alignbyte(w2, w1, sh),
-                         (int)__builtin_amdgcn_alignbyte(w3, w2, sh), (int)__builtin_amdgcn_alignbyte(w4, w3, sh)};
-        }
-#pragma unroll
-        for (int t = 0; t < YT; t++) {
-            const int d = R - 16 * t;
-            if (d >= 0 && d < band) {
-                const v4i bf = *(const v4i *)(brow + d * TS);
-#pragma unroll
-                for (int mx = 0; mx < XT; mx++)
-                    acc[t][mx] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mx], bf, acc[t][mx], 0, 0, 0);
-            }
-        }
-    }
+    // y-tile t takes rows R = 16 t .. 16 t + band - 1 (band = th_eff + 15 >= 48): seven phases, each a loop over a
+    // fixed run of active tiles, in the same order for every template height
+    const int band = the + 15;
+    xcorr_phase<0, 0>(0, 16, arow, brow, sh, acc);
+    xcorr_phase<0, 1>(16, 32, arow, brow, sh, acc);
+    xcorr_phase<0, 2>(32, 48, arow, brow, sh, acc);
+    xcorr_phase<0, 3>(48, band, arow, brow, sh, acc);
+    xcorr_phase<1, 3>(band, band + 16, arow, brow, sh, acc);
+    xcorr_phase<2, 3>(band + 16, band + 32, arow, brow, sh, acc);
+    xcorr_phase<3, 3>(band + 32, band + 48, arow, brow, sh, acc);
 #pragma unroll
     for (int t = 0; t < YT; t++)
 #pragma unroll
