@@ -70,48 +70,56 @@ def pyr_bytes(w, h, levels, elem):
 
 
 class Batched:
-    """B pairs per step on two contexts (A: pyramids and LK, S: the searches)."""
+    """B pairs per step on two contexts (A: pyramids, cross terms and LK; S: the searches' scoring and finish), the
+    results of each step downloaded on one of two copy contexts while the next step runs (double-buffered pinned
+    host records)."""
 
     def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph):
         self.v, self.B, self.graph = vtgpu, batch, graph
         self.A, self.S = vtgpu.Context(0), vtgpu.Context(0)
+        self.copy = [vtgpu.Context(0), vtgpu.Context(0)]
         self.frames, self.corners = d_frames, corners
         self.groups = len(d_frames) // batch
         A = self.A
         self.pyrs = [[(A.pyramid(W, H, LEVELS), A.pyramid(W, H, LEVELS)) for _ in range(batch)]
                      for _ in range(self.groups if graph else 1)]
         self.xy = A.to_device(np.tile(np.stack([xs, ys], 1).ravel(), batch))
-        self.out = A.alloc(32 * NPTS * batch)
+        # each group's own records, so a step's download never races the next step's writes
+        self.outs = [A.alloc(32 * NPTS * batch) for _ in range(self.groups)]
         self.cfg = vtgpu.LkCfg(WIN, WIN, 10, 0.01, 1e-4, 40.0, 0)
         self.offs = [k * NPTS for k in range(batch + 1)]
         ncell = ((W - 1) // 16 + 1) * ((H - 1) // 16 + 1)   # the header's bound for corners [1, W] x [1, H]
-        self.cells = [self.S.alloc(12 * ncell) for _ in range(batch)]
+        self.cells = [self.A.alloc(12 * ncell) for _ in range(batch)]
         self.fs = C.sizeof(vtgpu.Found)
-        self.found = self.S.alloc(self.fs * batch)
+        self.founds = [A.alloc(self.fs * batch) for _ in range(self.groups)]
         self.tmpl = [None] * batch
-        self.h_out = np.zeros(NPTS * batch, vtgpu.LK_OUT_DTYPE)
-        self.h_found = (vtgpu.Found * batch)()
+        self.h_out = [vtgpu.PinnedArray(A, (NPTS * batch,), vtgpu.LK_OUT_DTYPE) for _ in range(2)]
+        self.h_found = [vtgpu.PinnedArray(A, (self.fs * batch,), np.uint8) for _ in range(2)]
         self.graphs = {}
 
     def enqueue(self, g, serial=False):
-        """All device work of group g's batch, stream-ordered on A and S (A waits for S at the end); serial: the
-        search starts after LK instead of beside it."""
+        """All device work of group g's batch. Context A: the 2B pyramids, the B template cuts, the B searches' cross
+        terms (matrix cores), then LK of all pairs; context S, after the cross terms: the searches' scoring and finish,
+        beside LK. A waits for S at the end. serial: everything on A, one kernel after another (for timing each
+        kernel alone)."""
         A, S, B = self.A, self.S, self.B
         pyrs = self.pyrs[g if self.graph else 0]
         fr = self.frames[g * B:(g + 1) * B]
         A.build_many([p for pair in pyrs for p in pair], [f for pair in fr for f in pair])
-        S.wait(A)
-        A.lk_track_many([p[0] for p in pyrs], [p[1] for p in pyrs], [LEVELS - 1] * B, self.offs, self.cfg, self.xy,
-                        out=self.out)
-        if serial:
-            S.wait(A)  # the search after LK: each kernel timed alone on the device
         corners = [self.corners[(g * B + k) % UNIQ] for k in range(B)]
         if self.tmpl[0] is None:
-            self.tmpl = [S.ncc_template(pyrs[k][0], 0, *corners[k], TPL, TPL) for k in range(B)]
+            self.tmpl = [A.ncc_template(pyrs[k][0], 0, *corners[k], TPL, TPL) for k in range(B)]
         else:  # the B templates re-cut in one launch
-            S.ncc_templates_update(self.tmpl, [p[0] for p in pyrs], [0] * B, corners)
+            A.ncc_templates_update(self.tmpl, [p[0] for p in pyrs], [0] * B, corners)
         # the B exhaustive searches: one launch per stage for all of them
-        S.ncc_search_many([p[1] for p in pyrs], self.tmpl, [(1, W, 1, H)] * B, 1, self.found, cells=self.cells)
+        regions = [(1, W, 1, H)] * B
+        if serial:
+            A.ncc_search_many([p[1] for p in pyrs], self.tmpl, regions, 1, self.founds[g], cells=self.cells)
+        else:
+            A.ncc_search_many_split(S, [p[1] for p in pyrs], self.tmpl, regions, 1, self.founds[g],
+                                    cells=self.cells)
+        A.lk_track_many([p[0] for p in pyrs], [p[1] for p in pyrs], [LEVELS - 1] * B, self.offs, self.cfg, self.xy,
+                        out=self.outs[g])
         A.wait(S)
 
     def prepare(self):
@@ -126,17 +134,24 @@ class Batched:
                 self.enqueue(g)
                 self.graphs[g] = self.A.graph_end()
 
-    def step(self, i):
+    def launch(self, i):
+        """Step i: its group's work, then its results (every LK record, every pick) downloaded on copy context i % 2
+        into pinned buffer i % 2, no host wait."""
         g = i % self.groups
         if self.graph:
             self.A.graph_launch(self.graphs[g])
         else:
             self.enqueue(g)
-        # results to the host: every LK record and every pick
-        self.out.get(self.h_out)
-        self.A.check(self.v.lib().vtg_download(self.A.h, C.cast(self.h_found, C.c_void_p), self.found.ptr,
-                                               self.fs * self.B), "download")
-        return self.h_out
+        cp = self.copy[i % 2]
+        cp.wait(self.A)
+        self.outs[g].get_async(cp, self.h_out[i % 2].arr)
+        self.founds[g].get_async(cp, self.h_found[i % 2].arr)
+
+    def finish(self, i):
+        """Wait for step i's results on the host; returns its LK records and picks."""
+        self.copy[i % 2].sync()
+        found = (self.v.Found * self.B).from_buffer_copy(self.h_found[i % 2].arr.tobytes())
+        return self.h_out[i % 2].arr, found
 
 
 def plain_rate(vtgpu, d_frames, corners, xs, ys, steps):
@@ -394,21 +409,30 @@ def main():
     bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=not a.no_graph)
     bt.prepare()
     for i in range(a.warmup):
-        bt.step(i)
+        bt.launch(i)
+        bt.finish(i)
+    # timed: step i+1 is launched before step i's results are waited for, so downloads overlap the next step;
+    # latency = launch of a step to its results on the host
     lat = []
     t0 = time.perf_counter()
+    ts = []
     for i in range(a.steps):
-        ts = time.perf_counter()
-        bt.step(i)
-        lat.append(time.perf_counter() - ts)
+        ts.append(time.perf_counter())
+        bt.launch(i)
+        if i >= 1:
+            bt.finish(i - 1)
+            lat.append(time.perf_counter() - ts[i - 1])
+    bt.finish(a.steps - 1)
+    lat.append(time.perf_counter() - ts[a.steps - 1])
     total = time.perf_counter() - t0
     pairs_per_s = a.steps * a.batch / total
     lat_ms = np.array(lat) * 1e3
     # per-pair algorithmic counts from each pair's own LK records (group 0 holds pairs 0..B-1)
-    bt.step(0)
-    flops = [lk_flops(bt.h_out[k * NPTS:(k + 1) * NPTS], xs, ys, LEVELS - 1, W, H, WIN) for k in range(a.batch)]
-    gpu_out0 = bt.h_out[:NPTS].copy()
-    found0 = bt.h_found[0]
+    bt.launch(0)
+    h_out, h_found = bt.finish(0)
+    flops = [lk_flops(h_out[k * NPTS:(k + 1) * NPTS], xs, ys, LEVELS - 1, W, H, WIN) for k in range(a.batch)]
+    gpu_out0 = h_out[:NPTS].copy()
+    found0 = h_found[0]
     log(f"batched: {pairs_per_s:.1f} pairs/s, batch p50 {np.percentile(lat_ms, 50):.3f} ms p99 "
         f"{np.percentile(lat_ms, 99):.3f} ms; pair 0 NCC pick ({found0.hx / 2},{found0.hy / 2}) {found0.ncc:.4f}")
 

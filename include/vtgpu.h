@@ -55,6 +55,13 @@ int vtg_free(vtg_ctx *ctx, void *dptr);
 /* Stream-ordered copies; these two wait for completion (the host buffer may be pageable). */
 int vtg_upload(vtg_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
 int vtg_download(vtg_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+/* Stream-ordered download with no wait: dst_host must be pinned (vtg_host_alloc) and stay untouched until the
+ * context's work up to this call is done (vtg_ctx_sync, or a later context that waited for this one). */
+int vtg_download_async(vtg_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+/* Pinned (page-locked) host memory: the target of vtg_download_async and a faster source/target of the waiting
+ * copies. */
+int vtg_host_alloc(vtg_ctx *ctx, size_t bytes, void **host);
+int vtg_host_free(vtg_ctx *ctx, void *host);
 /* Stream-ordered device-to-device copy and fill (no wait). */
 int vtg_copy_d2d(vtg_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
 
@@ -201,6 +208,13 @@ int vtg_ncc_search_async(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int 
  * Stream-ordered, no host wait; a search whose window clips to nothing gets score -2 and cells 0. */
 int vtg_ncc_search_many(vtg_ctx *ctx, int n, const vtg_pyr *const *p, const vtg_tmpl *const *t, const int *region,
                         int step, struct vtg_cell *const *cells, struct vtg_found *found_dev);
+/* vtg_ncc_search_many with its stages on two contexts of one device: the cross terms (the map's matrix products) on
+ * map_ctx, then, ordered after them, the scoring, top 8, refinement and pick on finish_ctx. A caller overlaps the
+ * finish with other work on map_ctx (e.g. LK), joining finish_ctx with vtg_ctx_wait before reading found_dev. The
+ * scratch is map_ctx's; finish_ctx must not be destroyed while the search is in flight. */
+int vtg_ncc_search_many_split(vtg_ctx *map_ctx, vtg_ctx *finish_ctx, int n, const vtg_pyr *const *p,
+                              const vtg_tmpl *const *t, const int *region, int step, struct vtg_cell *const *cells,
+                              struct vtg_found *found_dev);
 /* Scores at n corners (tk_correlate, track.h:57-59; half != 0: half-pixel corners, search_half): xy device int
  * pairs, out device doubles: -1 outside, 0 for a window under 1/16 of the template's energy. */
 int vtg_ncc_score(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out);

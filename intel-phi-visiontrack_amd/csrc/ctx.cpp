@@ -83,6 +83,8 @@ int scratch(vtg_ctx *c, size_t bytes, void **p)
     if (bytes > c->scratch_bytes) {
         if (c->scratch) {
             VTG_HIP(c, hipStreamSynchronize(c->stream));
+            if (c->peer)
+                VTG_HIP(c, hipStreamSynchronize(c->peer->stream));
             VTG_HIP(c, hipFree(c->scratch));
             c->scratch = nullptr;
             c->scratch_bytes = 0;
@@ -220,6 +222,34 @@ int vtg_download(vtg_ctx *c, void *dst, const void *src, size_t bytes)
         return fail(c, VTG_EINVAL, "vtg_download: bad argument");
     VTG_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     VTG_HIP(c, hipStreamSynchronize(c->stream));
+    return VTG_OK;
+}
+
+int vtg_download_async(vtg_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c || (!dst && bytes) || (!src && bytes))
+        return fail(c, VTG_EINVAL, "vtg_download_async: bad argument");
+    VTG_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    return VTG_OK;
+}
+
+int vtg_host_alloc(vtg_ctx *c, size_t bytes, void **host)
+{
+    if (!c || !host)
+        return fail(c, VTG_EINVAL, "vtg_host_alloc: bad argument");
+    *host = nullptr;
+    const hipError_t e = hipHostMalloc(host, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess)
+        return fail(c, VTG_ENOMEM, "vtg_host_alloc: %zu bytes: %s", bytes, hipGetErrorString(e));
+    return VTG_OK;
+}
+
+int vtg_host_free(vtg_ctx *c, void *host)
+{
+    if (!c)
+        return VTG_EINVAL;
+    if (host)
+        VTG_HIP(c, hipHostFree(host));
     return VTG_OK;
 }
 

@@ -87,6 +87,94 @@ template <> struct Ld<__hip_bfloat16> {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// A level read through a buffer resource: the base and size are wave-uniform (scalar registers), each load is the
+// resource plus a lane's 32-bit byte offset o plus a uniform offset so (a row step, kept in a scalar register), and a
+// read past the level's bytes returns 0 instead of touching memory. Two or four neighbouring elements of a row come
+// in one load.
+template <typename T> struct Row;
+template <> struct Row<uint8_t> {
+    static __device__ __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float &a0, float &a1)
+    {
+        const unsigned v = __builtin_amdgcn_raw_buffer_load_b16(r, o, so, 0);
+        a0 = (float)(v & 0xffu);
+        a1 = (float)(v >> 8);
+    }
+    static __device__ __forceinline__ void quad(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float *a)
+    {
+        const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0);
+        a[0] = (float)(v & 0xffu);
+        a[1] = (float)((v >> 8) & 0xffu);
+        a[2] = (float)((v >> 16) & 0xffu);
+        a[3] = (float)(v >> 24);
+    }
+};
+template <> struct Row<uint16_t> {
+    static __device__ __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float &a0, float &a1)
+    {
+        const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0);
+        a0 = (float)(v & 0xffffu);
+        a1 = (float)(v >> 16);
+    }
+    static __device__ __forceinline__ void quad(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float *a)
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
+        a[0] = (float)(v[0] & 0xffffu);
+        a[1] = (float)(v[0] >> 16);
+        a[2] = (float)(v[1] & 0xffffu);
+        a[3] = (float)(v[1] >> 16);
+    }
+};
+template <> struct Row<float> {
+    static __device__ __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float &a0, float &a1)
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
+        a0 = __uint_as_float(v[0]);
+        a1 = __uint_as_float(v[1]);
+    }
+    static __device__ __forceinline__ void quad(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float *a)
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, o, so, 0);
+        for (int i = 0; i < 4; i++)
+            a[i] = __uint_as_float(v[i]);
+    }
+};
+template <typename H> struct Row16 {
+    static __device__ __forceinline__ float cvt(unsigned short b)
+    {
+        H h;
+        __builtin_memcpy(&h, &b, 2);
+        return Ld<H>::get(h);
+    }
+    static __device__ __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float &a0, float &a1)
+    {
+        const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0);
+        a0 = cvt((unsigned short)(v & 0xffffu));
+        a1 = cvt((unsigned short)(v >> 16));
+    }
+    static __device__ __forceinline__ void quad(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float *a)
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
+        a[0] = cvt((unsigned short)(v[0] & 0xffffu));
+        a[1] = cvt((unsigned short)(v[0] >> 16));
+        a[2] = cvt((unsigned short)(v[1] & 0xffffu));
+        a[3] = cvt((unsigned short)(v[1] >> 16));
+    }
+};
+template <> struct Row<__half> : Row16<__half> {};
+template <> struct Row<__hip_bfloat16> : Row16<__hip_bfloat16> {};
+
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t level_rsrc(const Level &lv, unsigned &pitch_bytes)
+{
+    const uint64_t v = (uint64_t)lv.p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const int pitch = __builtin_amdgcn_readfirstlane(lv.pitch), h = __builtin_amdgcn_readfirstlane(lv.h);
+    pitch_bytes = (unsigned)pitch * (unsigned)sizeof(T);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(((uint64_t)hi << 32) | lo), 0,
+                                             (int)((unsigned)pitch * (unsigned)h * (unsigned)sizeof(T)), 0x00020000);
+}
+
 // LDS written by one lane and read by another lane of the same wave: order the accesses, no workgroup barrier.
 __device__ __forceinline__ void wave_sync()
 {
@@ -212,18 +300,32 @@ __device__ __forceinline__ void build_direct(const Level &lv, double px, double 
     const float fx = (float)(px - flx), fy = (float)(py - fly);
     // S row 0, column 0 samples level pixel (flx - w/2 - 1 + fx, fly - h/2 - 1 + fy); this lane needs S columns
     // k .. k + 2 of S rows re .. re + MAXR + 1, i.e. image rows re .. re + MAXR + 2 (at most h + 2) and 4 columns
-    const T *base = (const T *)lv.p;
-    const unsigned x0 = (unsigned)((int)flx - g.W / 2 - 1 + g.k);
+    unsigned pitch;
+    const __amdgpu_buffer_rsrc_t rs = level_rsrc<T>(lv, pitch);
+    const unsigned x0 = (unsigned)((int)flx - g.W / 2 - 1 + g.k) * (unsigned)sizeof(T);
     const int y0 = (int)fly - g.H / 2 - 1;
     float X[MAXR + 3][3];
+    if (g.H >= MAXR) {
+        // rows re .. re + MAXR + 2 lie inside the window's h + 3 rows: one lane offset, the row steps uniform
+        const unsigned o = (unsigned)(y0 + g.re) * pitch + x0;
 #pragma unroll
-    for (int i = 0; i < MAXR + 3; i++) {
-        const int j = g.re + i < g.H + 2 ? g.re + i : g.H + 2;
-        const T *p = base + (unsigned)(y0 + j) * (unsigned)lv.pitch + x0;
-        const float a0 = Ld<T>::get(p[0]), a1 = Ld<T>::get(p[1]), a2 = Ld<T>::get(p[2]), a3 = Ld<T>::get(p[3]);
-        X[i][0] = fmaf(a1 - a0, fx, a0);
-        X[i][1] = fmaf(a2 - a1, fx, a1);
-        X[i][2] = fmaf(a3 - a2, fx, a2);
+        for (int i = 0; i < MAXR + 3; i++) {
+            float a[4];
+            Row<T>::quad(rs, o, (unsigned)i * pitch, a);
+            X[i][0] = fmaf(a[1] - a[0], fx, a[0]);
+            X[i][1] = fmaf(a[2] - a[1], fx, a[1]);
+            X[i][2] = fmaf(a[3] - a[2], fx, a[2]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < MAXR + 3; i++) {
+            const int j = g.re + i < g.H + 2 ? g.re + i : g.H + 2;
+            float a[4];
+            Row<T>::quad(rs, (unsigned)(y0 + j) * pitch + x0, 0u, a);
+            X[i][0] = fmaf(a[1] - a[0], fx, a[0]);
+            X[i][1] = fmaf(a[2] - a[1], fx, a[1]);
+            X[i][2] = fmaf(a[3] - a[2], fx, a[2]);
+        }
     }
     float S[MAXR + 2][3];
 #pragma unroll
@@ -245,16 +347,28 @@ __device__ __forceinline__ void sample_direct(const Level &lv, double qx, double
 {
     const double flx = floor(qx), fly = floor(qy);
     const float fx = (float)(qx - flx), fy = (float)(qy - fly);
-    const T *base = (const T *)lv.p;
-    const unsigned x = (unsigned)((int)flx - g.W / 2 + g.k);
+    unsigned pitch;
+    const __amdgpu_buffer_rsrc_t rs = level_rsrc<T>(lv, pitch);
+    const unsigned x = (unsigned)((int)flx - g.W / 2 + g.k) * (unsigned)sizeof(T);
     const int y0 = (int)fly - g.H / 2;
     float A[MAXR + 1];
+    if (g.H >= MAXR) {
+        // rows re .. re + MAXR lie inside the window's h + 1 rows: one lane offset, the row steps uniform
+        const unsigned o = (unsigned)(y0 + g.re) * pitch + x;
 #pragma unroll
-    for (int i = 0; i <= MAXR; i++) {
-        const int j = g.re + i < g.H ? g.re + i : g.H;
-        const T *p = base + (unsigned)(y0 + j) * (unsigned)lv.pitch + x;
-        const float a0 = Ld<T>::get(p[0]), a1 = Ld<T>::get(p[1]);
-        A[i] = fmaf(a1 - a0, fx, a0);
+        for (int i = 0; i <= MAXR; i++) {
+            float a0, a1;
+            Row<T>::pair(rs, o, (unsigned)i * pitch, a0, a1);
+            A[i] = fmaf(a1 - a0, fx, a0);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i <= MAXR; i++) {
+            const int j = g.re + i < g.H ? g.re + i : g.H;
+            float a0, a1;
+            Row<T>::pair(rs, (unsigned)(y0 + j) * pitch + x, 0u, a0, a1);
+            A[i] = fmaf(a1 - a0, fx, a0);
+        }
     }
     mismatch<MAXR, RES>(A, fy, Ld<T>::scale, g, tl, b);
 }

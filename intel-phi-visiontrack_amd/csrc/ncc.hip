@@ -1233,7 +1233,9 @@ static size_t aux_bytes() { return (sizeof(Aux) + 255) & ~(size_t)255; }
 // One launch per stage for up to SMAX searches that share template size, level element type and step.
 enum { MAP_GENERIC = 0, MAP_DOT4 = 1, MAP_MFMA = 2 };
 
-static int launch_batch(vtg_ctx *c, const vtg_pyr *p0, int level, int mode, int wpr, SBatch &b)
+// The map's products run on c's stream; the scoring pass (MFMA form) and the finish (top 8, refinement, pick) on f's,
+// after the products (f may be c).
+static int launch_batch(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, int mode, int wpr, SBatch &b)
 {
     int maxcols = 0, maxrows = 0, maxcell = 0;
     for (int k = 0; k < b.n; k++) {
@@ -1249,12 +1251,17 @@ static int launch_batch(vtg_ctx *c, const vtg_pyr *p0, int level, int mode, int 
             hipLaunchKernelGGL(ncc_xcorr_mfma, gx, dim3(256), xcorr_lds_bytes(b.th), c->stream, b);
             VTG_HIP(c, hipGetLastError());
             timer_end(c, "ncc_xcorr", ev);
+            if (f != c) {
+                int rc = vtg_ctx_wait(f, c);
+                if (rc)
+                    return fail(c, rc, "vtg_ncc_search: %s", vtg_ctx_error(f));
+            }
             const dim3 gs((maxcols + 15) / 16, maxrows, b.n);
-            timer_begin(c, "ncc_cells", &ev);
+            timer_begin(f, "ncc_cells", &ev);
             hipLaunchKernelGGL(ncc_cells_score, gs, dim3(256), score_lds_words(b.th) * sizeof(uint32_t),
-                               c->stream, b);
+                               f->stream, b);
             VTG_HIP(c, hipGetLastError());
-            timer_end(c, "ncc_cells", ev);
+            timer_end(f, "ncc_cells", ev);
         } else if (mode == MAP_DOT4) {
             const dim3 grid((maxcols + 15) / 16, maxrows, b.n);
             const size_t lds = fast_lds_words(b.th, b.tw, wpr) * sizeof(uint32_t);
@@ -1280,10 +1287,15 @@ static int launch_batch(vtg_ctx *c, const vtg_pyr *p0, int level, int mode, int 
                 return rc;
         }
     }
-    timer_begin(c, "ncc_top8", &ev);
-    hipLaunchKernelGGL(ncc_top8, dim3(b.n), dim3(1024), 0, c->stream, b);
+    if (f != c && mode != MAP_MFMA) {
+        int rc = vtg_ctx_wait(f, c);
+        if (rc)
+            return fail(c, rc, "vtg_ncc_search: %s", vtg_ctx_error(f));
+    }
+    timer_begin(f, "ncc_top8", &ev);
+    hipLaunchKernelGGL(ncc_top8, dim3(b.n), dim3(1024), 0, f->stream, b);
     VTG_HIP(c, hipGetLastError());
-    timer_end(c, "ncc_top8", ev);
+    timer_end(f, "ncc_top8", ev);
     if (maxcell > 0) {
         const int around = b.step > 1 ? 2 * b.step - 1 : 1, R = around + 1;
         const int rmax = R + 1 + 2; // region extent beyond the template, whole pixels, each way
@@ -1292,35 +1304,38 @@ static int launch_batch(vtg_ctx *c, const vtg_pyr *p0, int level, int mode, int 
         int rc = by_level_type(c, p0, level, [&](auto *tag) -> int {
             using T = std::remove_pointer_t<decltype(tag)>;
             const dim3 grid(8, b.n);
-            timer_begin(c, "ncc_refine", &ev);
+            timer_begin(f, "ncc_refine", &ev);
             switch (b.tw / 16) {
-            case 1: hipLaunchKernelGGL((ncc_refine<T, 1>), grid, dim3(256), lds, c->stream, b); break;
-            case 2: hipLaunchKernelGGL((ncc_refine<T, 2>), grid, dim3(256), lds, c->stream, b); break;
-            case 3: hipLaunchKernelGGL((ncc_refine<T, 3>), grid, dim3(256), lds, c->stream, b); break;
-            default: hipLaunchKernelGGL((ncc_refine<T, 4>), grid, dim3(256), lds, c->stream, b); break;
+            case 1: hipLaunchKernelGGL((ncc_refine<T, 1>), grid, dim3(256), lds, f->stream, b); break;
+            case 2: hipLaunchKernelGGL((ncc_refine<T, 2>), grid, dim3(256), lds, f->stream, b); break;
+            case 3: hipLaunchKernelGGL((ncc_refine<T, 3>), grid, dim3(256), lds, f->stream, b); break;
+            default: hipLaunchKernelGGL((ncc_refine<T, 4>), grid, dim3(256), lds, f->stream, b); break;
             }
             VTG_HIP(c, hipGetLastError());
-            timer_end(c, "ncc_refine", ev);
+            timer_end(f, "ncc_refine", ev);
             return VTG_OK;
         });
         if (rc)
             return rc;
     }
-    timer_begin(c, "ncc_final", &ev);
-    hipLaunchKernelGGL(ncc_final, dim3(b.n), dim3(64), 0, c->stream, b);
+    timer_begin(f, "ncc_final", &ev);
+    hipLaunchKernelGGL(ncc_final, dim3(b.n), dim3(64), 0, f->stream, b);
     VTG_HIP(c, hipGetLastError());
-    timer_end(c, "ncc_final", ev);
+    timer_end(f, "ncc_final", ev);
     return VTG_OK;
 }
 
 // n searches into device records: each window clipped to where the template fits (as search_slot clips), maps in
 // cells[k] or the context's scratch, picks in found_dev[k] or, when found_dev is null, in the scratch (returned in
 // *found_scratch). Stream-ordered; no allocation once the scratch has grown to the batch's size.
-static int search_many(vtg_ctx *c, int n, const vtg_pyr *const *p, const vtg_tmpl *const *t, const int *region, int step,
-                vtg_cell *const *cells, vtg_found *found_dev, vtg_found **found_scratch)
+static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, const vtg_tmpl *const *t,
+                       const int *region, int step, vtg_cell *const *cells, vtg_found *found_dev,
+                       vtg_found **found_scratch)
 {
-    if (!c)
+    if (!c || !f)
         return VTG_EINVAL;
+    if (f != c && f->device != c->device)
+        return fail(c, VTG_EINVAL, "vtg_ncc_search: contexts on different devices");
     if (n < 1 || !p || !t || !region || (step != 1 && step != 2))
         return fail(c, VTG_EINVAL, "vtg_ncc_search: bad argument");
     const vtg_tmpl *t0 = t[0];
@@ -1362,6 +1377,7 @@ static int search_many(vtg_ctx *c, int n, const vtg_pyr *const *p, const vtg_tmp
             bytes += (size_t)ncell[k] * 256 * sizeof(int); // the cross terms of the padded map
     }
     void *sc = nullptr;
+    c->peer = f != c ? f : nullptr; // the scratch is read on f's stream too: growing it waits for f as well
     int rc = scratch(c, bytes, &sc);
     if (rc)
         return rc;
@@ -1401,7 +1417,7 @@ static int search_many(vtg_ctx *c, int n, const vtg_pyr *const *p, const vtg_tmp
                 cel += (size_t)ncell[k] * 256 * sizeof(int);
             }
         }
-        if ((rc = launch_batch(c, p[0], level0, mode, t0->words_per_row, b)))
+        if ((rc = launch_batch(c, f, p[0], level0, mode, t0->words_per_row, b)))
             return rc;
     }
     return VTG_OK;
@@ -1536,7 +1552,18 @@ int vtg_ncc_search_many(vtg_ctx *c, int n, const vtg_pyr *const *p, const vtg_tm
 {
     if (!found_dev)
         return fail(c, VTG_EINVAL, "vtg_ncc_search_many: null found records");
-    return search_many(c, n, p, t, region, step, cells, found_dev, nullptr);
+    return search_many(c, c, n, p, t, region, step, cells, found_dev, nullptr);
+}
+
+int vtg_ncc_search_many_split(vtg_ctx *map_ctx, vtg_ctx *finish_ctx, int n, const vtg_pyr *const *p,
+                              const vtg_tmpl *const *t, const int *region, int step, struct vtg_cell *const *cells,
+                              struct vtg_found *found_dev)
+{
+    if (!map_ctx || !finish_ctx)
+        return VTG_EINVAL;
+    if (!found_dev)
+        return fail(map_ctx, VTG_EINVAL, "vtg_ncc_search_many_split: null found records");
+    return search_many(map_ctx, finish_ctx, n, p, t, region, step, cells, found_dev, nullptr);
 }
 
 int vtg_ncc_search_async(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
@@ -1546,7 +1573,7 @@ int vtg_ncc_search_async(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0
         return fail(c, VTG_EINVAL, "vtg_ncc_search_async: null found record");
     const int region[4] = {x0, x1, y0, y1};
     vtg_cell *cl[1] = {cells};
-    return search_many(c, 1, &p, &t, region, step, cl, found_dev, nullptr);
+    return search_many(c, c, 1, &p, &t, region, step, cl, found_dev, nullptr);
 }
 
 int vtg_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
@@ -1557,7 +1584,7 @@ int vtg_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int 
     const int region[4] = {x0, x1, y0, y1};
     vtg_cell *cl[1] = {cells};
     vtg_found *fd = nullptr;
-    int rc = search_many(c, 1, &p, &t, region, step, cl, nullptr, &fd);
+    int rc = search_many(c, c, 1, &p, &t, region, step, cl, nullptr, &fd);
     if (rc)
         return rc;
     void *host = nullptr;

@@ -56,6 +56,7 @@ struct vtg_ctx {
     // scratch owned by the context (grown on demand, freed on destroy)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    vtg_ctx *peer = nullptr;                 // another context whose stream also reads the scratch (split search)
     hipEvent_t join = nullptr;               // for vtg_ctx_wait
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;

@@ -81,7 +81,8 @@ def lib():
     sig = {
         "vtg_ctx_create": [I, C.POINTER(V)], "vtg_ctx_destroy": [V], "vtg_ctx_sync": [V], "vtg_ctx_wait": [V, V],
         "vtg_alloc": [V, S, C.POINTER(V)], "vtg_free": [V, V], "vtg_upload": [V, V, V, S],
-        "vtg_download": [V, V, V, S], "vtg_copy_d2d": [V, V, V, S],
+        "vtg_download": [V, V, V, S], "vtg_copy_d2d": [V, V, V, S], "vtg_download_async": [V, V, V, S],
+        "vtg_host_alloc": [V, S, C.POINTER(V)], "vtg_host_free": [V, V],
         "vtg_timing_enable": [V, I], "vtg_timing_read": [V, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "vtg_timing_reset": [V],
         "vtg_pyr_create": [V, I, I, I, I, C.POINTER(V)], "vtg_pyr_destroy": [V, V],
@@ -96,7 +97,7 @@ def lib():
         "vtg_ncc_template_read": [V, V, V, C.POINTER(F)],
         "vtg_ncc_search": [V, V, V, I, I, I, I, I, V, C.POINTER(Found)],
         "vtg_ncc_search_async": [V, V, V, I, I, I, I, I, V, V],
-        "vtg_ncc_search_many": [V, I, V, V, V, I, V, V],
+        "vtg_ncc_search_many": [V, I, V, V, V, I, V, V], "vtg_ncc_search_many_split": [V, V, I, V, V, V, I, V, V],
         "vtg_tune_pair": [V, V, V, V, V, I, I, C.POINTER(LkCfg), I, V, I, C.POINTER(Tune)],
         "vtg_tune_get": [V, V, I, C.POINTER(LkCfg), C.POINTER(Tune)], "vtg_tune_reset": [V],
         "vtg_srv_create": [C.POINTER(SrvCfg), C.POINTER(V)], "vtg_srv_destroy": [V],
@@ -129,10 +130,10 @@ def lib():
 
 # every symbol include/vtgpu.h declares
 EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", "vtg_ctx_error", "vtg_ctx_stream", "vtg_alloc",
-           "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_timing_enable", "vtg_timing_read",
+           "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_download_async", "vtg_host_alloc", "vtg_host_free", "vtg_timing_enable", "vtg_timing_read",
            "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_level",
            "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_margin", "vtg_ncc_template", "vtg_ncc_template_destroy",
-           "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_template_update_many", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_many",
+           "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_template_update_many", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_many", "vtg_ncc_search_many_split",
            "vtg_tune_pair", "vtg_tune_get", "vtg_tune_reset", "vtg_srv_create", "vtg_srv_destroy", "vtg_srv_error",
            "vtg_srv_open", "vtg_srv_close", "vtg_srv_submit", "vtg_srv_poll", "vtg_srv_wait", "vtg_srv_get_stats",
            "vtg_ncc_score",
@@ -373,6 +374,16 @@ class Context:
         CL = None if cells is None else (C.c_void_p * n)(*[c.ptr if c is not None else None for c in cells])
         self.check(self.L.vtg_ncc_search_many(self.h, n, P, T, R, step, CL, found_dev.ptr), "ncc_search_many")
 
+    def ncc_search_many_split(self, finish, pyrs, tmpls, regions, step, found_dev, cells=None):
+        """ncc_search_many with the cross terms on this context and the scoring and finish on context `finish`."""
+        n = len(pyrs)
+        P = (C.c_void_p * n)(*[p.h for p in pyrs])
+        T = (C.c_void_p * n)(*[t.h for t in tmpls])
+        R = (C.c_int * (4 * n))(*[int(v) for r in regions for v in r])
+        CL = None if cells is None else (C.c_void_p * n)(*[c.ptr if c is not None else None for c in cells])
+        self.check(self.L.vtg_ncc_search_many_split(self.h, finish.h, n, P, T, R, step, CL, found_dev.ptr),
+                   "ncc_search_many_split")
+
     # autotuner
     def tune_pair(self, I, J, f0, f1, top, cfg, xy, solve=SOLVE_F64):
         """Time the schedules on this pair (device frames f0, f1: DevBufs, pitch = width) and points (DevBuf) and keep
@@ -417,12 +428,36 @@ class Context:
         return r
 
 
+class PinnedArray:
+    """A numpy array over pinned host memory (vtg_host_alloc); .arr is the array, free() releases it."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = C.c_void_p()
+        ctx.check(ctx.L.vtg_host_alloc(ctx.h, nbytes, C.byref(p)), "host_alloc")
+        self.ptr = p
+        buf = (C.c_char * nbytes).from_address(p.value)
+        self.arr = np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.arr = None
+            self.ctx.L.vtg_host_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
 class DevBuf:
     def __init__(self, ctx, ptr, nbytes):
         self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
 
     def get(self, arr):
         self.ctx.check(self.ctx.L.vtg_download(self.ctx.h, arr.ctypes.data, self.ptr, arr.nbytes), "download")
+        return arr
+
+    def get_async(self, ctx, arr):
+        """Download into `arr` (a PinnedArray's array) on ctx's stream, no wait."""
+        ctx.check(ctx.L.vtg_download_async(ctx.h, arr.ctypes.data, self.ptr, arr.nbytes), "download_async")
         return arr
 
     def put(self, arr):

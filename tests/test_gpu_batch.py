@@ -158,3 +158,21 @@ def test_uneven_batch(ctx, oracle):
                                   oracle.pyramid(f0, w, h, L, threads=16)[0][L - 1].view())
     ctx.tune_reset()
     b.destroy()
+
+
+def test_download_async_into_pinned(ctx):
+    """vtg_download_async into vtg_host_alloc memory, ordered after work on another context, gives the bytes a waiting
+    download gives."""
+    other = vtgpu.Context(0)
+    rng = np.random.default_rng(5)
+    src = rng.integers(0, 256, 3 << 20, dtype=np.uint8)
+    d = ctx.to_device(src)
+    pin = vtgpu.PinnedArray(ctx, (src.size,), np.uint8)
+    say("download_async")
+    other.wait(ctx)
+    d.get_async(other, pin.arr)
+    other.sync()
+    assert np.array_equal(pin.arr, src)
+    pin.free()
+    d.free()
+    other.close()

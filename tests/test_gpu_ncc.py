@@ -203,3 +203,43 @@ def test_template_update_many_matches_single(ctx):
         t.destroy()
     for p in pyrs:
         p.destroy()
+
+
+@pytest.mark.parametrize("tw,th", [(64, 64), (32, 21)])
+def test_search_split_matches_single_context(ctx, tw, th):
+    """The search with its cross terms on one context and its scoring and finish on another (overlapping other work
+    on the first) gives the records of the one-context call, field for field."""
+    import ctypes as C
+    other = vtgpu.Context(0)
+    sizes = [(1920, 1080), (640, 480), (1280, 720), (800, 600), (1920, 1080), (640, 480), (960, 540), (1024, 768),
+             (640, 480)]
+    pyrs, tms, regions = [], [], []
+    for k, (w, h) in enumerate(sizes):
+        f0, f1, _ = vtgpu.frame_pair(w, h, 900 + k)
+        p0, p1 = ctx.pyramid(w, h, 1), ctx.pyramid(w, h, 1)
+        p0.build(f0)
+        p1.build(f1)
+        tms.append(ctx.ncc_template(p0, 0, w // 3, h // 4, tw, th))
+        pyrs.append((p0, p1))
+        regions.append((1, w, 1, h) if k % 2 else (w // 5, w // 2, h // 6, h // 2))
+    fs = C.sizeof(vtgpu.Found)
+    a, b = ctx.alloc(fs * len(sizes)), ctx.alloc(fs * len(sizes))
+    say("ncc search_many_split", tw, th)
+    ctx.ncc_search_many([p[1] for p in pyrs], tms, regions, 1, a)
+    ctx.ncc_search_many_split(other, [p[1] for p in pyrs], tms, regions, 1, b)
+    ctx.wait(other)
+    ctx.sync()
+    ga, gb = (vtgpu.Found * len(sizes))(), (vtgpu.Found * len(sizes))()
+    ctx.check(vtgpu.lib().vtg_download(ctx.h, C.cast(ga, C.c_void_p), a.ptr, fs * len(sizes)))
+    ctx.check(vtgpu.lib().vtg_download(ctx.h, C.cast(gb, C.c_void_p), b.ptr, fs * len(sizes)))
+    for k in range(len(sizes)):
+        assert bytes(ga[k]) == bytes(gb[k]), k
+        assert ga[k].cells > 0 and ga[k].ncc > -1.0
+    a.free()
+    b.free()
+    for t in tms:
+        t.destroy()
+    for p0, p1 in pyrs:
+        p0.destroy()
+        p1.destroy()
+    other.close()
