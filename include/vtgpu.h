@@ -85,7 +85,15 @@ int vtg_graph_destroy(vtg_ctx *ctx, void *graph);
  * u8 level 0 (the frame itself, Q6 = 64 x byte) and u16 upper levels hold the same values in 1 and 2 bytes. F32 is
  * the reference's own fp32 Q6 (4 B per pixel, the plain port). F16 and BF16 hold grey = Q6 / 64 above level 0 and
  * are not exact. */
-enum { VTG_STORE_Q6 = 0, VTG_STORE_F32 = 1, VTG_STORE_F16 = 2, VTG_STORE_BF16 = 3 };
+enum { VTG_STORE_Q6 = 0, VTG_STORE_F32 = 1, VTG_STORE_F16 = 2, VTG_STORE_BF16 = 3, VTG_STORE_Q6W = 4 };
+/* VTG_STORE_Q6W: every level u16 Q6, exact; the storage for frames whose level 0 is filtered (below). */
+
+/* Level-0 filters of tk_build (src/kern/track.c:600-623; the tracker's `contrast` and `pre_gauss` options,
+ * src/common/vt_proto.h:89-90): VTG_BUILD_CONTRAST stretches the frame's Q6 values to 0 .. 16320 by its own min and
+ * max (img_minmax + img_norm, gain 16320 / span and offset -min * gain in long double, as tk_build computes them);
+ * VTG_BUILD_PRE_GAUSS then blurs level 0 with the 5 x 5 binomial (img_gauss5). Both in the reference's arithmetic,
+ * bit for bit. Level 0 is then no longer 64 x a byte: the pyramid must be VTG_STORE_Q6W or VTG_STORE_F32. */
+enum { VTG_BUILD_CONTRAST = 1, VTG_BUILD_PRE_GAUSS = 2 };
 
 typedef struct vtg_pyr vtg_pyr;
 
@@ -102,6 +110,10 @@ int vtg_pyr_build(vtg_ctx *ctx, vtg_pyr *p, const uint8_t *frame, int pitch, int
  * covers every pyramid that has it. frames[i], pitches[i] as for vtg_pyr_build. */
 int vtg_pyr_build_many(vtg_ctx *ctx, int n, vtg_pyr *const *pyrs, const uint8_t *const *frames, const int *pitches,
                        int on_device);
+/* vtg_pyr_build / vtg_pyr_build_many with the level-0 filters `flags` (VTG_BUILD_*). */
+int vtg_pyr_build_ex(vtg_ctx *ctx, vtg_pyr *p, const uint8_t *frame, int pitch, int on_device, int flags);
+int vtg_pyr_build_many_ex(vtg_ctx *ctx, int n, vtg_pyr *const *pyrs, const uint8_t *const *frames,
+                          const int *pitches, int on_device, int flags);
 /* Level l's device pointer, size, pitch in bytes and element size. */
 int vtg_pyr_level(const vtg_pyr *p, int level, void **dptr, int *w, int *h, size_t *pitch_bytes, int *elem_bytes);
 /* Level l into a host fp32 Q6 plane, with the reference's replicated padding filled (img_pad_rows, img_pad_tb,
@@ -272,6 +284,15 @@ int vtg_img_u8_to_q6(vtg_ctx *ctx, const struct vt_plane *dst, const uint8_t *sr
 /* img_reduce_* (src/kern/img.h:30-33): dst rows y0..y1 from src (whose padding need not be current: the border is
  * replicated by the kernel), padding of those rows filled. */
 int vtg_img_reduce(vtg_ctx *ctx, const struct vt_plane *dst, const struct vt_plane *src, int y0, int y1);
+/* img_gauss5_* (src/kern/img.h, img_ref.c:83-96): dst rows y0..y1 = the 5 x 5 binomial of src, (h + 128) >> 8 (the
+ * border replicated), padding of those rows filled. */
+int vtg_img_gauss5(vtg_ctx *ctx, const struct vt_plane *dst, const struct vt_plane *src, int y0, int y1);
+/* img_norm_* (img_ref.c:42-55): dst rows y0..y1 = clamp(rint(src * gain_offset[0] + gain_offset[1]), 0, 16320) with the
+ * reference's long double sum reproduced exactly, padding of those rows filled. */
+int vtg_img_norm(vtg_ctx *ctx, const struct vt_plane *dst, const struct vt_plane *src, const float *gain_offset,
+                 int y0, int y1);
+/* img_minmax_* (img_ref.c:57-71): out[0], out[1] = min and max of rows y0..y1 of src (host floats). */
+int vtg_img_minmax(vtg_ctx *ctx, const struct vt_plane *src, float *out, int y0, int y1);
 /* lk_track_* / lk_flowback_* (src/kern/lk.h:45-53) for n points over host pyramids I[0..top], J[0..top]:
  * gx, gy may be NULL (zero guess); fb_max > 0 runs the flowback check. */
 int vtg_lk_track_planes(vtg_ctx *ctx, const struct vt_plane *I, const struct vt_plane *J, int top,

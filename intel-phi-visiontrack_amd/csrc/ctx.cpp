@@ -154,6 +154,8 @@ int vtg_ctx_destroy(vtg_ctx *c)
         (void)hipEventDestroy(e);
     if (c->scratch)
         (void)hipFree(c->scratch);
+    if (c->filt)
+        (void)hipFree(c->filt);
     if (c->pinned)
         (void)hipHostFree(c->pinned);
     if (c->join)

@@ -1159,6 +1159,7 @@ int by_level_type(vtg_ctx *c, const vtg_pyr *p, int level, F &&f)
     case VTG_STORE_Q6: return eb == 1 ? f((uint8_t *)nullptr) : f((uint16_t *)nullptr);
     case VTG_STORE_F16: return eb == 1 ? f((uint8_t *)nullptr) : f((__half *)nullptr);
     case VTG_STORE_BF16: return eb == 1 ? f((uint8_t *)nullptr) : f((__hip_bfloat16 *)nullptr);
+    case VTG_STORE_Q6W: return f((uint16_t *)nullptr);
     }
     return fail(c, VTG_EINVAL, "storage %d", p->storage);
 }

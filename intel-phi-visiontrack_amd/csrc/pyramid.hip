@@ -262,6 +262,7 @@ static int elem_bytes(int storage, int level)
     case VTG_STORE_F32: return 4;
     case VTG_STORE_F16:
     case VTG_STORE_BF16: return level == 0 ? 1 : 2;
+    case VTG_STORE_Q6W: return 2;
     }
     return 0;
 }
@@ -303,7 +304,7 @@ int pyr_new(vtg_ctx *c, int w, int h, int levels, int storage, void *mem, vtg_py
         return fail(c, VTG_ERANGE, "vtg_pyr_create: frame %dx%d outside 8..16384", w, h);
     if (levels < 1 || levels > VTG_MAX_LEVELS)
         return fail(c, VTG_EINVAL, "vtg_pyr_create: %d levels (1..%d)", levels, VTG_MAX_LEVELS);
-    if (storage < VTG_STORE_Q6 || storage > VTG_STORE_BF16)
+    if (storage < VTG_STORE_Q6 || storage > VTG_STORE_Q6W)
         return fail(c, VTG_EINVAL, "vtg_pyr_create: storage %d", storage);
     vtg_pyr *p = new vtg_pyr();
     p->w = w;
@@ -338,14 +339,48 @@ int pyr_new(vtg_ctx *c, int w, int h, int levels, int storage, void *mem, vtg_py
     return VTG_OK;
 }
 
+// Levels 1 .. of pyramids whose level 0 is written: each from the one below, one launch per level for all.
+template <typename T>
+static int levels_from_l0(vtg_ctx *c, vtg_pyr *const *ps, int n, const int *rows)
+{
+    int maxl = 0;
+    for (int i = 0; i < n; i++)
+        maxl = std::max(maxl, ps[i]->levels);
+    std::vector<RJob> jobs;
+    std::vector<int> th;
+    int rc = VTG_OK;
+    for (int l = 1; l < maxl && !rc; l++) {
+        jobs.clear();
+        th.clear();
+        for (int i = 0; i < n; i++) {
+            if (ps[i]->levels <= l)
+                continue;
+            const Level *L = ps[i]->lv;
+            jobs.push_back(RJob{L[l - 1].p, (void *)L[l].p, nullptr, L[l - 1].w, L[l - 1].h, L[l - 1].pitch, L[l].w,
+                                L[l].h, L[l].pitch, 0, 0, 0});
+            th.push_back(rows[i]);
+        }
+        rc = launch_jobs<T, T, T, false, true>(c, jobs, th);
+    }
+    return rc;
+}
+
 // frames[i]: pyramid i's u8 frame on the device, pitch[i] bytes per row. All pyramids of one storage. rows: each
-// pyramid's reduce tile height (8, 16 or 32), or null for the context's choice for its size class.
+// pyramid's reduce tile height (8, 16 or 32), or null for the context's choice for its size class. flags: the level-0
+// filters (storage Q6W or F32 only; Q6W always builds level 0 through the filter kernel, 64 x byte without flags).
 int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch,
-                          const int *rows_in)
+                          const int *rows_in, int flags)
 {
     std::vector<int> rows(n);
     for (int i = 0; i < n; i++)
         rows[i] = rows_in ? rows_in[i] : tuned_pyr_rows(c, ps[i]);
+    if (flags || ps[0]->storage == VTG_STORE_Q6W) {
+        int rc = launch_level0_many(c, ps, n, frames, pitch, flags);
+        if (rc)
+            return rc;
+        return ps[0]->storage == VTG_STORE_F32 ? levels_from_l0<float>(c, ps, n, rows.data())
+                                               : levels_from_l0<uint16_t>(c, ps, n, rows.data());
+    }
     // single-level pyramids: level 0 is the frame (converted for F32)
     for (int i = 0; i < n; i++) {
         if (ps[i]->levels != 1)
@@ -440,15 +475,33 @@ int vtg_pyr_destroy(vtg_ctx *c, vtg_pyr *p)
     return VTG_OK;
 }
 
+static int check_flags(vtg_ctx *c, const vtg_pyr *p, int flags)
+{
+    if (flags & ~(VTG_BUILD_CONTRAST | VTG_BUILD_PRE_GAUSS))
+        return fail(c, VTG_EINVAL, "vtg_pyr_build: unknown flags 0x%x", flags);
+    if (flags && p->storage != VTG_STORE_Q6W && p->storage != VTG_STORE_F32)
+        return fail(c, VTG_EINVAL, "vtg_pyr_build: the level-0 filters need storage VTG_STORE_Q6W or VTG_STORE_F32 "
+                                   "(storage %d keeps level 0 as the u8 frame)", p->storage);
+    return VTG_OK;
+}
+
 int vtg_pyr_build(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame, int pitch, int on_device)
+{
+    return vtg_pyr_build_ex(c, p, frame, pitch, on_device, 0);
+}
+
+int vtg_pyr_build_ex(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame, int pitch, int on_device, int flags)
 {
     if (!c || !p || !frame || pitch < p->w)
         return fail(c, VTG_EINVAL, "vtg_pyr_build: bad argument");
+    int rc = check_flags(c, p, flags);
+    if (rc)
+        return rc;
     if (!on_device) {
         // the host frame goes straight into level 0's bytes when that level is u8, else into scratch
         void *dst = nullptr;
         size_t dp = (size_t)p->w;
-        if (p->storage == VTG_STORE_F32) {
+        if (p->storage == VTG_STORE_F32 || p->storage == VTG_STORE_Q6W) {
             int rc = scratch(c, (size_t)p->w * p->h, &dst);
             if (rc)
                 return rc;
@@ -460,27 +513,38 @@ int vtg_pyr_build(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame, int pitch, int o
         frame = (const uint8_t *)dst;
         pitch = (int)dp;
     }
-    return launch_pyr_build(c, p, frame, pitch);
+    return launch_pyr_build_many(c, &p, 1, &frame, &pitch, nullptr, flags);
 }
 
 int vtg_pyr_build_many(vtg_ctx *c, int n, vtg_pyr *const *pyrs, const uint8_t *const *frames, const int *pitches,
                        int on_device)
 {
+    return vtg_pyr_build_many_ex(c, n, pyrs, frames, pitches, on_device, 0);
+}
+
+int vtg_pyr_build_many_ex(vtg_ctx *c, int n, vtg_pyr *const *pyrs, const uint8_t *const *frames, const int *pitches,
+                          int on_device, int flags)
+{
     if (!c || n < 0 || (n > 0 && (!pyrs || !frames || !pitches)))
         return fail(c, VTG_EINVAL, "vtg_pyr_build_many: bad argument");
     if (n == 0)
         return VTG_OK;
-    for (int i = 0; i < n; i++)
+    for (int i = 0; i < n; i++) {
         if (!pyrs[i] || !frames[i] || pitches[i] < pyrs[i]->w || pyrs[i]->storage != pyrs[0]->storage)
             return fail(c, VTG_EINVAL, "vtg_pyr_build_many: pyramid %d: bad frame, pitch or storage", i);
+        int rc = check_flags(c, pyrs[i], flags);
+        if (rc)
+            return rc;
+    }
     if (on_device)
-        return launch_pyr_build_many(c, pyrs, n, frames, pitches, nullptr);
+        return launch_pyr_build_many(c, pyrs, n, frames, pitches, nullptr, flags);
     // host frames: into level 0 where it is u8, else into scratch, then the same batch
     std::vector<const uint8_t *> dev(n);
     std::vector<int> dp(n);
     size_t need = 0;
+    const bool wide = pyrs[0]->storage == VTG_STORE_F32 || pyrs[0]->storage == VTG_STORE_Q6W;
     for (int i = 0; i < n; i++)
-        if (pyrs[i]->storage == VTG_STORE_F32)
+        if (wide)
             need += round_up((size_t)pyrs[i]->w * pyrs[i]->h, 256);
     void *tmp = nullptr;
     if (need) {
@@ -491,7 +555,7 @@ int vtg_pyr_build_many(vtg_ctx *c, int n, vtg_pyr *const *pyrs, const uint8_t *c
     size_t off = 0;
     for (int i = 0; i < n; i++) {
         vtg_pyr *p = pyrs[i];
-        if (p->storage == VTG_STORE_F32) {
+        if (wide) {
             dev[i] = (const uint8_t *)tmp + off;
             dp[i] = p->w;
             off += round_up((size_t)p->w * p->h, 256);
@@ -502,7 +566,7 @@ int vtg_pyr_build_many(vtg_ctx *c, int n, vtg_pyr *const *pyrs, const uint8_t *c
         VTG_HIP(c, hipMemcpy2DAsync((void *)dev[i], dp[i], frames[i], pitches[i], p->w, p->h, hipMemcpyHostToDevice,
                                     c->stream));
     }
-    return launch_pyr_build_many(c, pyrs, n, dev.data(), dp.data(), nullptr);
+    return launch_pyr_build_many(c, pyrs, n, dev.data(), dp.data(), nullptr, flags);
 }
 
 int vtg_pyr_level(const vtg_pyr *p, int level, void **dptr, int *w, int *h, size_t *pitch_bytes, int *elem)
@@ -543,7 +607,7 @@ int vtg_pyr_read_level(vtg_ctx *c, const vtg_pyr *p, int level, const struct vt_
     else if (es == 1)
         hipLaunchKernelGGL(to_plane<uint8_t>, grid, dim3(256), 0, c->stream, (const uint8_t *)L.p, L.pitch, L.w, L.h,
                            (float *)tmp, dst->stride);
-    else if (p->storage == VTG_STORE_Q6)
+    else if (p->storage == VTG_STORE_Q6 || p->storage == VTG_STORE_Q6W)
         hipLaunchKernelGGL(to_plane<uint16_t>, grid, dim3(256), 0, c->stream, (const uint16_t *)L.p, L.pitch, L.w,
                            L.h, (float *)tmp, dst->stride);
     else if (p->storage == VTG_STORE_F16)

@@ -57,6 +57,7 @@ struct vtg_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     vtg_ctx *peer = nullptr;                 // another context whose stream also reads the scratch (split search)
+    void *filt = nullptr;                    // the contrast stretch's tables and per-frame tables (built on first use)
     hipEvent_t join = nullptr;               // for vtg_ctx_wait
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;
@@ -123,7 +124,10 @@ int pyr_from_planes(vtg_ctx *c, const vt_plane *levels, int n, vtg_pyr **out);
 // kernels (one per translation unit)
 int launch_pyr_build(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame_dev, int pitch);
 int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch,
-                          const int *rows);
+                          const int *rows, int flags = 0);
+// Level 0 of pyramids of storage Q6W or F32 from device u8 frames, filtered as `flags` asks (filters.hip).
+int launch_level0_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch,
+                       int flags);
 // sched: one schedule for every pair, or null for each pair's size-class pick (the context default otherwise)
 int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
                    const int *off, const lk_cfg *cfg, const float *xy, const float *guess, float fb_max,

@@ -15,7 +15,8 @@ LIB_GPU = os.path.join(HERE, "libvtgpu.so")
 LIB_SYNTH = os.path.join(HERE, "libvtsynth.so")
 
 PAD_X, PAD_Y, Q6 = 32, 4, 64
-STORE_Q6, STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2, 3
+STORE_Q6, STORE_F32, STORE_F16, STORE_BF16, STORE_Q6W = 0, 1, 2, 3, 4
+BUILD_CONTRAST, BUILD_PRE_GAUSS = 1, 2
 SOLVE_F64, SOLVE_MIXED = 0, 1
 LK_OK, LK_OUT, LK_LOWEIG, LK_RESIDUAL, LK_FLOWBACK = 0, 1, 2, 4, 8
 
@@ -87,6 +88,10 @@ def lib():
         "vtg_timing_reset": [V],
         "vtg_pyr_create": [V, I, I, I, I, C.POINTER(V)], "vtg_pyr_destroy": [V, V],
         "vtg_pyr_build": [V, V, V, I, I], "vtg_pyr_build_many": [V, I, V, V, V, I],
+        "vtg_pyr_build_ex": [V, V, V, I, I, I], "vtg_pyr_build_many_ex": [V, I, V, V, V, I, I],
+        "vtg_img_gauss5": [V, C.POINTER(Plane), C.POINTER(Plane), I, I],
+        "vtg_img_norm": [V, C.POINTER(Plane), C.POINTER(Plane), V, I, I],
+        "vtg_img_minmax": [V, C.POINTER(Plane), V, I, I],
         "vtg_pyr_level": [V, I, C.POINTER(V), C.POINTER(I), C.POINTER(I), C.POINTER(S), C.POINTER(I)],
         "vtg_pyr_read_level": [V, V, I, C.POINTER(Plane)],
         "vtg_lk_track": [V, V, V, I, C.POINTER(LkCfg), I, V, V, F, V],
@@ -131,7 +136,8 @@ def lib():
 # every symbol include/vtgpu.h declares
 EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", "vtg_ctx_error", "vtg_ctx_stream", "vtg_alloc",
            "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_download_async", "vtg_host_alloc", "vtg_host_free", "vtg_timing_enable", "vtg_timing_read",
-           "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_level",
+           "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_build_ex",
+           "vtg_pyr_build_many_ex", "vtg_img_gauss5", "vtg_img_norm", "vtg_img_minmax", "vtg_pyr_level",
            "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_margin", "vtg_ncc_template", "vtg_ncc_template_destroy",
            "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_template_update_many", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_many", "vtg_ncc_search_many_split",
            "vtg_tune_pair", "vtg_tune_get", "vtg_tune_reset", "vtg_srv_create", "vtg_srv_destroy", "vtg_srv_error",
@@ -291,13 +297,13 @@ class Context:
     def pyramid(self, w, h, levels, storage=STORE_Q6):
         return Pyramid(self, w, h, levels, storage)
 
-    def build_many(self, pyrs, frames):
+    def build_many(self, pyrs, frames, flags=0):
         """Build several pyramids in one launch per level: frames are DevBufs (pitch = width)."""
         n = len(pyrs)
         P = (C.c_void_p * n)(*[p.h for p in pyrs])
         F = (C.c_void_p * n)(*[f.ptr for f in frames])
         pitch = (C.c_int * n)(*[p.w for p in pyrs])
-        self.check(self.L.vtg_pyr_build_many(self.h, n, P, F, pitch, 1), "pyr_build_many")
+        self.check(self.L.vtg_pyr_build_many_ex(self.h, n, P, F, pitch, 1, flags), "pyr_build_many")
 
     # LK
     def lk_track(self, I, J, top, cfg, xy, guess=None, fb_max=0.0, out=None, solve=SOLVE_F64):
@@ -487,14 +493,15 @@ class Pyramid:
         ctx.check(ctx.L.vtg_pyr_create(ctx.h, w, h, levels, storage, C.byref(p)), "pyr_create")
         self.h = p
 
-    def build(self, frame, on_device=False, pitch=None):
-        """frame: host uint8 (h, w) array, or a DevBuf with on_device=True."""
+    def build(self, frame, on_device=False, pitch=None, flags=0):
+        """frame: host uint8 (h, w) array, or a DevBuf with on_device=True. flags: BUILD_CONTRAST | BUILD_PRE_GAUSS."""
         if on_device:
-            self.ctx.check(self.ctx.L.vtg_pyr_build(self.ctx.h, self.h, frame.ptr, pitch or self.w, 1), "pyr_build")
+            self.ctx.check(self.ctx.L.vtg_pyr_build_ex(self.ctx.h, self.h, frame.ptr, pitch or self.w, 1, flags),
+                           "pyr_build")
         else:
             frame = np.ascontiguousarray(frame, np.uint8)
-            self.ctx.check(self.ctx.L.vtg_pyr_build(self.ctx.h, self.h, frame.ctypes.data, frame.strides[0], 0),
-                           "pyr_build")
+            self.ctx.check(self.ctx.L.vtg_pyr_build_ex(self.ctx.h, self.h, frame.ctypes.data, frame.strides[0], 0,
+                                                       flags), "pyr_build")
 
     def level_info(self, level):
         p, w, h, pb, eb = C.c_void_p(), C.c_int(), C.c_int(), C.c_size_t(), C.c_int()
