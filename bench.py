@@ -202,23 +202,25 @@ def plain_rate(vtgpu, d_frames, corners, xs, ys, steps):
 
 
 def batching(vtgpu, steps, seed=2024):
-    """The uneven batch: 64 pairs (640x480 .. 3840x2160, 500-20,000 points, +-24 px, 3-5 levels), both pyramids of
-    every pair and all points in one LK launch per schedule, captured as one HIP graph and replayed; first under the
-    default schedules, then after the autotuner has picked per size class. Per form: the median and p99 of a whole
-    batch (graph launch to LK records on the host) and the rates."""
+    """The uneven batch: 64 pairs (640x480 .. 3840x2160, 500-20,000 points, +-24 px, 3-5 levels), captured as one HIP
+    graph and replayed, in three forms: one launch per pyramid level for all 128 pyramids then one LK launch, under the
+    default schedules; the same after the autotuner has picked per size class; and the fused form, every pyramid and
+    every point in ONE persistent launch (work items in dependency order, a pair's LK overlapping later pyramids).
+    Per form: the median and p99 of a whole batch (graph launch to LK records on the host) and the rates."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
-    from workloads import Batch, batch_spec
+    from workloads import Batch, batch_spec, enqueue_fused
     ctx = vtgpu.Context(0)
     spec = batch_spec(seed)
     b = Batch(vtgpu, ctx, spec)
     cfg = vtgpu.LkCfg(WIN, WIN, 10, 0.01, 1e-4, 40.0, 0)
     h_out = np.zeros(b.npoints, vtgpu.LK_OUT_DTYPE)
 
-    def timed(label):
-        b.enqueue(cfg)  # eager once: scratch and schedules settle
+    def timed(label, fused=False):
+        enq = (lambda: enqueue_fused(b, cfg)) if fused else (lambda: b.enqueue(cfg))
+        enq()  # eager once: scratch and schedules settle
         ctx.sync()
         ctx.graph_begin()
-        b.enqueue(cfg)
+        enq()
         g = ctx.graph_end()
         lat = []
         for i in range(steps + 2):
@@ -230,15 +232,15 @@ def batching(vtgpu, steps, seed=2024):
         ctx.graph_destroy(g)
         ctx.timing(True)
         ctx.timing_reset()
-        b.enqueue(cfg)
+        enq()
         ctx.sync()
-        kern = {k: ctx.timing_read(k) for k in ("pyr_reduce", "lk_track")}
+        kern = {k: ctx.timing_read(k) for k in (("pyr_lk_fused",) if fused else ("pyr_reduce", "lk_track"))}
         ctx.timing(False)
         ms = np.array(lat) * 1e3
         r = {"batch_ms_p50": round(float(np.percentile(ms, 50)), 3),
              "batch_ms_p99": round(float(np.percentile(ms, 99)), 3),
              "pairs_per_s": round(len(spec) / (np.median(ms) * 1e-3), 1),
-             "lk_pts_per_s": round(b.npoints / (kern["lk_track"][0] * 1e-3), 1),
+             "lk_pts_per_s": round(b.npoints / (kern["pyr_lk_fused" if fused else "lk_track"][0] * 1e-3), 1),
              "kernel_ms": {k: round(v[0], 4) for k, v in kern.items()},
              "launches": {k: int(v[1]) for k, v in kern.items()}}
         log(f"batching {label}: {r}")
@@ -258,6 +260,9 @@ def batching(vtgpu, steps, seed=2024):
     res["autotune"] = picks
     res["tuned"] = timed("tuned")
     ctx.tune_reset()
+    # one persistent launch for all 128 pyramids and all points (items in dependency order)
+    res["fused"] = timed("fused", fused=True)
+    res["fused"]["device_error"] = ctx.device_error()
     b.destroy()
     ctx.close()
     return res

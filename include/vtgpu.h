@@ -142,6 +142,20 @@ int vtg_lk_track_ex(vtg_ctx *ctx, const vtg_pyr *I, const vtg_pyr *J, int top, c
 int vtg_lk_track_many(vtg_ctx *ctx, int npairs, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
                       const int *offsets, const struct lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
                       int solve_mode, struct lk_out *out);
+/* Both pyramids of npairs frame pairs built AND every pair's points tracked in ONE persistent launch (the batched
+ * tk_build + OP_LK, track.c:584-629 and 275-292): pair k builds I[k] from device frame0[k] and J[k] from frame1[k]
+ * (u8, pitch0[k] / pitch1[k] bytes per row), then tracks points offsets[k] .. offsets[k+1] - 1 of xy from I[k] to J[k]
+ * from level top[k]. Workgroups take work items in dependency order (a pair's reduce tiles level by level, then its
+ * LK blocks of 4 points) and an item waits only for the items it reads, so one pair's LK overlaps later pairs'
+ * pyramids. Results equal vtg_pyr_build_many + vtg_lk_track_many bit for bit. Q6 pyramids with at least 2 levels,
+ * up to 64 pairs. Stream-ordered; graph-capturable (no allocation after the first call). */
+int vtg_pyr_lk_fused(vtg_ctx *ctx, int npairs, vtg_pyr *const *I, vtg_pyr *const *J, const uint8_t *const *frame0,
+                     const uint8_t *const *frame1, const int *pitch0, const int *pitch1, const int *top,
+                     const int *offsets, const struct lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
+                     int solve_mode, struct lk_out *out);
+/* The context's device error word (waits for the context): nonzero when a fused launch's bounded wait gave up, whose
+ * results are then not valid; reading clears it. */
+int vtg_ctx_device_error(vtg_ctx *ctx, int *code);
 /* The LK schedule: margin > 0 stages each level's template patch and J's neighbourhood (the window plus `margin`
  * pixels) in LDS with one batch of loads; 0 reads the pyramid in memory at every iteration. Results are the same
  * bit for bit either way; only time differs (default 0). */

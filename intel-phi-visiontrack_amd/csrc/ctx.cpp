@@ -156,6 +156,8 @@ int vtg_ctx_destroy(vtg_ctx *c)
         (void)hipFree(c->scratch);
     if (c->filt)
         (void)hipFree(c->filt);
+    if (c->fused)
+        (void)hipFree(c->fused);
     if (c->pinned)
         (void)hipHostFree(c->pinned);
     if (c->join)

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <climits>
 
+#include "reduce_tile.hpp"
 #include "vtg_internal.hpp"
 
 namespace vtg {
@@ -757,6 +758,208 @@ int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 b
     return VTG_OK;
 }
 
+// ---------------------------------------------------------------- fused pyramid + LK, one persistent launch
+
+// The batched tk_build + OP_LK (track.c:584-629, 275-292) as one launch: work items in the order of their
+// dependencies, pair by pair: the reduce tiles of I's and J's level 1 (which also write the frames into level 0), of
+// level 2, ..., then the pair's LK blocks (4 points, one per wave). Workgroups take items from a counter in that order;
+// a level-l tile waits until its pyramid's level l - 1 is complete, an LK block until both pyramids are complete up to
+// its top level. Every item waited for was taken before the waiter's own, by a workgroup that is running and waits
+// only for earlier items, so the earliest unfinished item always progresses: no deadlock, whatever the residency.
+// A tile publishes with an agent-scope release (every thread) before its count, a waiter reads the count relaxed and
+// then acquires (every thread), so levels written on any XCD are seen fresh. A wait gives up after a bounded number of
+// sleeps, setting the context's error word (the results are then not valid), so every wave reaches the exit.
+constexpr int FB = 64;          // pairs per fused launch
+constexpr int FP = 4;           // points per LK item (one per wave)
+constexpr int FCTR = 4;         // counter words before the done counts: [0] next item, [1] error word
+
+struct FusedArgs {
+    LkArgs lk;                  // pairs' pyramids, top levels, point slices, window configuration, output
+    const uint8_t *f0[FB], *f1[FB];
+    int p0[FB], p1[FB];         // frame pitches (bytes)
+    int item0[FB + 1];          // first item of each pair; item0[npairs] = items in all
+    int *ctr;                   // FCTR words, then done[pair][side][level]
+};
+
+__device__ __forceinline__ int level_tiles(const Level &L) { return ((L.w + 63) / 64) * ((L.h + 15) / 16); }
+
+// Wait until *p >= need: sleeps between reads; gives up (setting the error word) after 2^24 sleeps, or at once once
+// another wait has given up, so a launch with a broken wait still drains in about a second.
+__device__ __forceinline__ void wait_count(int *p, int need, int *err)
+{
+    for (int spins = 0; __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need; spins++) {
+        __builtin_amdgcn_s_sleep(2);
+        if ((spins & 1023) == 1023 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            break;
+        if (spins == (1 << 24)) {
+            __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+    }
+}
+
+// The counters of a fused launch to zero; the error word only when clear_err (the context's first launch): it stays
+// set until the host reads it.
+__global__ __launch_bounds__(256) void fused_reset(int *ctr, int n, int clear_err)
+{
+    for (int i = threadIdx.x; i < n; i += 256)
+        if (i != 1 || clear_err)
+            ctr[i] = 0;
+}
+
+template <int MAXR, int SOLVE>
+__global__ __launch_bounds__(256) void fused_kernel(FusedArgs a)
+{
+    __shared__ int s_item;
+    __shared__ int lds[rt::tile_lds_elems<16>()];
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int np = a.lk.npairs, total = a.item0[np];
+    int *err = a.ctr + 1, *done = a.ctr + FCTR;
+    for (;;) {
+        if (tid == 0)
+            s_item = __hip_atomic_fetch_add(a.ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads(); // s_item is rewritten next round
+        if (item >= total)
+            break;
+        int lo = 0, hi = np - 1; // the pair: the last whose first item is at or before this one
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (item >= a.item0[mid])
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int k = lo;
+        const PyrView &I = *a.lk.I[k], &J = *a.lk.J[k];
+        int loc = item - a.item0[k];
+        bool pyr_item = false;
+        for (int l = 1; l < I.levels && !pyr_item; l++) {
+            for (int sd = 0; sd < 2; sd++) {
+                const PyrView &P = sd ? J : I;
+                const int tn = level_tiles(P.l[l]);
+                if (loc >= tn) {
+                    loc -= tn;
+                    continue;
+                }
+                int *cnt = done + (k * 2 + sd) * kMaxLevels;
+                if (l >= 2) {
+                    if (tid == 0)
+                        wait_count(cnt + l - 1, level_tiles(P.l[l - 1]), err);
+                    __syncthreads();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                }
+                rt::RJob job;
+                job.dst = (void *)P.l[l].p;
+                job.dw = P.l[l].w;
+                job.dh = P.l[l].h;
+                job.dp = P.l[l].pitch;
+                job.tiles_x = (P.l[l].w + 63) / 64;
+                job.t0 = 0;
+                if (l == 1) {
+                    job.src = sd ? a.f1[k] : a.f0[k];
+                    job.sp = sd ? a.p1[k] : a.p0[k];
+                    job.sw = P.l[0].w;
+                    job.sh = P.l[0].h;
+                    job.copy = (void *)P.l[0].p;
+                    job.cp = P.l[0].pitch;
+                    rt::reduce_tile<uint8_t, uint16_t, uint8_t, true, true, 16>(job, loc, lds);
+                } else {
+                    job.src = P.l[l - 1].p;
+                    job.sp = P.l[l - 1].pitch;
+                    job.sw = P.l[l - 1].w;
+                    job.sh = P.l[l - 1].h;
+                    job.copy = nullptr;
+                    job.cp = 0;
+                    rt::reduce_tile<uint16_t, uint16_t, uint16_t, false, true, 16>(job, loc, lds);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __syncthreads();
+                if (tid == 0)
+                    __hip_atomic_fetch_add(cnt + l, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                pyr_item = true;
+                break;
+            }
+        }
+        if (pyr_item)
+            continue;
+        // an LK item: both pyramids complete up to level max(top, 1) (which completes every level below it)
+        const int top = a.lk.top[k], tl = top > 1 ? top : 1;
+        if (tid == 0) {
+            wait_count(done + (k * 2) * kMaxLevels + tl, level_tiles(I.l[tl]), err);
+            wait_count(done + (k * 2 + 1) * kMaxLevels + tl, level_tiles(J.l[tl]), err);
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const int lp = loc * FP + wave, npk = a.lk.pre[k + 1] - a.lk.pre[k];
+        if (lp < npk) {
+            const Geo g = geometry<MAXR>(a.lk.cfg, lane);
+            const int pt = a.lk.start[k] + lp;
+            const float x = a.lk.xy[2 * pt], y = a.lk.xy[2 * pt + 1];
+            const float gx = a.lk.guess ? a.lk.guess[2 * pt] : 0.0f, gy = a.lk.guess ? a.lk.guess[2 * pt + 1] : 0.0f;
+            const Slice sl{nullptr, nullptr, nullptr};
+            Track o;
+            track<MAXR, uint8_t, uint16_t, SOLVE, false>(I, J, top, a.lk.cfg, g, lane, sl, 1, x, y, gx, gy, o);
+            if (a.lk.fb_max > 0.0f && o.status == 0) {
+                Track b;
+                track<MAXR, uint8_t, uint16_t, SOLVE, false>(J, I, top, a.lk.cfg, g, lane, sl, 1, o.x, o.y,
+                                                             (float)((double)x - (double)o.x),
+                                                             (float)((double)y - (double)o.y), b);
+                const double ex = (double)b.x - (double)x, ey = (double)b.y - (double)y;
+                if (b.status || ex * ex + ey * ey > (double)a.lk.fb_max * (double)a.lk.fb_max)
+                    o.status |= LK_FLOWBACK;
+            }
+            if (lane == 0) {
+                lk_out r;
+                r.x = o.x;
+                r.y = o.y;
+                r.min_eig = o.min_eig;
+                r.residual = o.residual;
+                r.cov_xx = o.cxx;
+                r.cov_xy = o.cxy;
+                r.cov_yy = o.cyy;
+                r.iters = (uint16_t)o.iters;
+                r.status = (uint16_t)o.status;
+                a.lk.out[pt] = r;
+            }
+        }
+    }
+}
+
+template <int MAXR, int SOLVE>
+int launch_fused_t(vtg_ctx *c, const FusedArgs &a, int total)
+{
+    static int per_cu = 0, cus = 0; // resident workgroups per CU of this kernel, CUs of device 0 class
+    if (!per_cu) {
+        VTG_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_kernel<MAXR, SOLVE>, 256, 0));
+        VTG_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        if (per_cu < 1)
+            per_cu = 1;
+    }
+    const int grid = std::min(total, per_cu * cus);
+    hipEvent_t ev;
+    timer_begin(c, "pyr_lk_fused", &ev);
+    hipLaunchKernelGGL((fused_kernel<MAXR, SOLVE>), dim3(grid), dim3(256), 0, c->stream, a);
+    VTG_HIP(c, hipGetLastError());
+    timer_end(c, "pyr_lk_fused", ev);
+    return VTG_OK;
+}
+
+template <int SOLVE>
+int launch_fused_rows(vtg_ctx *c, const FusedArgs &a, int total, int chunk)
+{
+    if (chunk <= 4)
+        return launch_fused_t<4, SOLVE>(c, a, total);
+    if (chunk <= 7)
+        return launch_fused_t<7, SOLVE>(c, a, total);
+    if (chunk <= 8)
+        return launch_fused_t<8, SOLVE>(c, a, total);
+    if (chunk <= 16)
+        return launch_fused_t<16, SOLVE>(c, a, total);
+    return launch_fused_t<32, SOLVE>(c, a, total);
+}
+
 // Rows per lane: the smallest compiled count no smaller than the window's chunk.
 template <int SOLVE, bool STAGED>
 int dispatch_rows(vtg_ctx *c, const LkArgs &a, int storage, int chunk, dim3 grid, dim3 block, size_t lds)
@@ -940,6 +1143,84 @@ int vtg_lk_track_many(vtg_ctx *c, int npairs, const vtg_pyr *const *I, const vtg
     if (!xy || !out)
         return fail(c, VTG_EINVAL, "vtg_lk_track_many: null points or output");
     return launch_lk_many(c, npairs, I, J, top, offsets, cfg, xy, guess, fb_max, solve_mode, out, nullptr);
+}
+
+int vtg_pyr_lk_fused(vtg_ctx *c, int npairs, vtg_pyr *const *I, vtg_pyr *const *J, const uint8_t *const *frame0,
+                     const uint8_t *const *frame1, const int *pitch0, const int *pitch1, const int *top,
+                     const int *offsets, const struct lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
+                     int solve_mode, struct lk_out *out)
+{
+    if (!c || !cfg || npairs < 0 || npairs > FB ||
+        (npairs > 0 && (!I || !J || !frame0 || !frame1 || !pitch0 || !pitch1 || !top || !offsets)))
+        return fail(c, VTG_EINVAL, "vtg_pyr_lk_fused: bad argument (at most %d pairs)", FB);
+    int rc = check_cfg(c, cfg, solve_mode);
+    if (rc || npairs == 0)
+        return rc;
+    FusedArgs a;
+    a.lk.npairs = npairs;
+    int acc = 0, items = 0;
+    for (int k = 0; k < npairs; k++) {
+        if ((rc = check_pair(c, I[k], J[k], top[k])))
+            return rc;
+        if (I[k]->storage != VTG_STORE_Q6 || I[k]->levels < 2 || J[k]->levels != I[k]->levels || !frame0[k] ||
+            !frame1[k] || pitch0[k] < I[k]->w || pitch1[k] < J[k]->w || offsets[k + 1] < offsets[k])
+            return fail(c, VTG_EINVAL, "vtg_pyr_lk_fused: pair %d: Q6 pyramids of one level count >= 2, frames, "
+                                       "pitches and offsets", k);
+        a.lk.I[k] = I[k]->dview;
+        a.lk.J[k] = J[k]->dview;
+        a.lk.top[k] = top[k];
+        a.lk.start[k] = offsets[k];
+        a.lk.pre[k] = acc;
+        acc += offsets[k + 1] - offsets[k];
+        a.f0[k] = frame0[k];
+        a.f1[k] = frame1[k];
+        a.p0[k] = pitch0[k];
+        a.p1[k] = pitch1[k];
+        a.item0[k] = items;
+        for (int l = 1; l < I[k]->levels; l++)
+            items += ((I[k]->lv[l].w + 63) / 64) * ((I[k]->lv[l].h + 15) / 16) +
+                     ((J[k]->lv[l].w + 63) / 64) * ((J[k]->lv[l].h + 15) / 16);
+        items += (offsets[k + 1] - offsets[k] + FP - 1) / FP;
+    }
+    a.lk.pre[npairs] = acc;
+    a.item0[npairs] = items;
+    a.lk.n = acc;
+    if (acc > 0 && (!xy || !out))
+        return fail(c, VTG_EINVAL, "vtg_pyr_lk_fused: null points or output");
+    a.lk.cfg = Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->eps, cfg->min_eig, cfg->max_residual};
+    a.lk.margin = 1;
+    a.lk.lds_floats = 0;
+    a.lk.xy = xy;
+    a.lk.guess = guess;
+    a.lk.fb_max = fb_max;
+    a.lk.out = out;
+    a.lk.waves = FP;
+    a.lk.per_xcd = 0;
+    const int nctr = FCTR + FB * 2 * kMaxLevels;
+    if (!c->fused)
+        VTG_HIP(c, hipMalloc(&c->fused, nctr * sizeof(int) + 256));
+    a.ctr = (int *)c->fused;
+    // the counters cleared by a kernel (no memset node, so the launch can be captured in a graph)
+    hipLaunchKernelGGL(fused_reset, dim3(1), dim3(256), 0, c->stream, a.ctr, nctr, c->fused_init ? 0 : 1);
+    c->fused_init = true;
+    VTG_HIP(c, hipGetLastError());
+    const int groups = 64 / cfg->w, chunk = (cfg->h + groups - 1) / groups;
+    return solve_mode == VTG_SOLVE_MIXED ? launch_fused_rows<VTG_SOLVE_MIXED>(c, a, items, chunk)
+                                         : launch_fused_rows<VTG_SOLVE_F64>(c, a, items, chunk);
+}
+
+int vtg_ctx_device_error(vtg_ctx *c, int *code)
+{
+    if (!c || !code)
+        return VTG_EINVAL;
+    *code = 0;
+    if (!c->fused)
+        return VTG_OK;
+    VTG_HIP(c, hipStreamSynchronize(c->stream));
+    VTG_HIP(c, hipMemcpy(code, (int *)c->fused + 1, sizeof(int), hipMemcpyDeviceToHost));
+    if (*code)
+        VTG_HIP(c, hipMemset((int *)c->fused + 1, 0, sizeof(int)));
+    return VTG_OK;
 }
 
 int vtg_lk_set_margin(vtg_ctx *c, int margin)

@@ -58,6 +58,8 @@ struct vtg_ctx {
     size_t scratch_bytes = 0;
     vtg_ctx *peer = nullptr;                 // another context whose stream also reads the scratch (split search)
     void *filt = nullptr;                    // the contrast stretch's tables and per-frame tables (built on first use)
+    void *fused = nullptr;                   // the fused launch's work counter, error word and per-level done counts
+    bool fused_init = false;
     hipEvent_t join = nullptr;               // for vtg_ctx_wait
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;

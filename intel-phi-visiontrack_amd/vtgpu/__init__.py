@@ -97,6 +97,8 @@ def lib():
         "vtg_lk_track": [V, V, V, I, C.POINTER(LkCfg), I, V, V, F, V],
         "vtg_lk_track_ex": [V, V, V, I, C.POINTER(LkCfg), I, V, V, F, I, V],
         "vtg_lk_track_many": [V, I, V, V, V, V, C.POINTER(LkCfg), V, V, F, I, V], "vtg_lk_set_margin": [V, I],
+        "vtg_pyr_lk_fused": [V, I, V, V, V, V, V, V, V, V, C.POINTER(LkCfg), V, V, F, I, V],
+        "vtg_ctx_device_error": [V, C.POINTER(I)],
         "vtg_ncc_template": [V, V, I, I, I, I, I, C.POINTER(V)], "vtg_ncc_template_destroy": [V, V],
         "vtg_ncc_template_update": [V, V, V, I, I, I], "vtg_ncc_template_update_many": [V, I, V, V, V, V, V],
         "vtg_ncc_template_read": [V, V, V, C.POINTER(F)],
@@ -138,7 +140,7 @@ EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", 
            "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_download_async", "vtg_host_alloc", "vtg_host_free", "vtg_timing_enable", "vtg_timing_read",
            "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_build_ex",
            "vtg_pyr_build_many_ex", "vtg_img_gauss5", "vtg_img_norm", "vtg_img_minmax", "vtg_pyr_level",
-           "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_margin", "vtg_ncc_template", "vtg_ncc_template_destroy",
+           "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_margin", "vtg_pyr_lk_fused", "vtg_ctx_device_error", "vtg_ncc_template", "vtg_ncc_template_destroy",
            "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_template_update_many", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_many", "vtg_ncc_search_many_split",
            "vtg_tune_pair", "vtg_tune_get", "vtg_tune_reset", "vtg_srv_create", "vtg_srv_destroy", "vtg_srv_error",
            "vtg_srv_open", "vtg_srv_close", "vtg_srv_submit", "vtg_srv_poll", "vtg_srv_wait", "vtg_srv_get_stats",
@@ -304,6 +306,26 @@ class Context:
         F = (C.c_void_p * n)(*[f.ptr for f in frames])
         pitch = (C.c_int * n)(*[p.w for p in pyrs])
         self.check(self.L.vtg_pyr_build_many_ex(self.h, n, P, F, pitch, 1, flags), "pyr_build_many")
+
+    def pyr_lk_fused(self, Is, Js, frames0, frames1, tops, offsets, cfg, xy, out, guess=None, fb_max=0.0,
+                     solve=SOLVE_F64):
+        """Build I[k] from frames0[k] and J[k] from frames1[k] (DevBufs, pitch = width) and track every pair's points
+        in one persistent launch; out: DevBuf of lk_out records."""
+        n = len(Is)
+        A = lambda xs: (C.c_void_p * n)(*xs)
+        pitch0 = (C.c_int * n)(*[p.w for p in Is])
+        pitch1 = (C.c_int * n)(*[p.w for p in Js])
+        T = (C.c_int * n)(*tops)
+        O = (C.c_int * (n + 1))(*offsets)
+        self.check(self.L.vtg_pyr_lk_fused(self.h, n, A([p.h for p in Is]), A([p.h for p in Js]),
+                                           A([f.ptr for f in frames0]), A([f.ptr for f in frames1]), pitch0, pitch1,
+                                           T, O, C.byref(cfg), xy.ptr, guess.ptr if guess is not None else None,
+                                           fb_max, solve, out.ptr), "pyr_lk_fused")
+
+    def device_error(self):
+        code = C.c_int()
+        self.check(self.L.vtg_ctx_device_error(self.h, C.byref(code)), "device_error")
+        return code.value
 
     # LK
     def lk_track(self, I, J, top, cfg, xy, guess=None, fb_max=0.0, out=None, solve=SOLVE_F64):

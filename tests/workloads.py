@@ -108,3 +108,10 @@ class StreamFrames:
 
     def __getitem__(self, i):
         return self.frames[self.index(i)]
+
+
+def enqueue_fused(batch, cfg, solve=0, fb_max=0.0):
+    """The same batch as Batch.enqueue in ONE persistent launch (pyramids and LK fused)."""
+    b = batch
+    b.ctx.pyr_lk_fused([p[0] for p in b.pyrs], [p[1] for p in b.pyrs], [f[0] for f in b.frames],
+                       [f[1] for f in b.frames], b.tops, b.offsets, cfg, b.xy, b.out, fb_max=fb_max, solve=solve)
