@@ -761,35 +761,50 @@ int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 b
 // ---------------------------------------------------------------- fused pyramid + LK, one persistent launch
 
 // The batched tk_build + OP_LK (track.c:584-629, 275-292) as one launch: work items in the order of their
-// dependencies, pair by pair: the reduce tiles of I's and J's level 1 (which also write the frames into level 0), of
-// level 2, ..., then the pair's LK blocks (4 points, one per wave). Workgroups take items from a counter in that order;
-// a level-l tile waits until its pyramid's level l - 1 is complete, an LK block until both pyramids are complete up to
-// its top level. Every item waited for was taken before the waiter's own, by a workgroup that is running and waits
-// only for earlier items, so the earliest unfinished item always progresses: no deadlock, whatever the residency.
-// A tile publishes with an agent-scope release (every thread) before its count, a waiter reads the count relaxed and
-// then acquires (every thread), so levels written on any XCD are seen fresh. A wait gives up after a bounded number of
-// sleeps, setting the context's error word (the results are then not valid), so every wave reaches the exit.
+// dependencies: bands of 16 output rows of every pair's level 1 (which also write the frames into level 0), then of
+// every pair's level 2, ..., then each pair's points, pair by pair. Workgroups take items from a counter in that order
+// (a workgroup's first point turns its four waves into independent takers of one point each, as no band follows); a
+// level-l band waits until its pyramid's level l - 1 is complete, a point until both pyramids are complete up to its
+// top level. Every item waited for was taken before the waiter's own, by a workgroup that is running and waits only
+// for earlier items, so the earliest unfinished item always progresses: no deadlock, whatever the residency.
+// There is one queue per XCD (workgroup w serves queue w mod 8, the XCD it runs on), each in that same order of
+// classes (level 1, level 2, ..., then the pairs' points): a level's bands are dealt round-robin over the queues and a
+// pair's points split into 8 contiguous chunks, one per queue, so an XCD's LK windows stay in one region of the frame.
+// The earliest unfinished class has all its dependencies finished and its items in every queue either finished or
+// held by running workgroups, so the queues cannot deadlock either. Building every pyramid level by level first keeps
+// the reduce work wide (a pair's own levels form a chain a few bands long); points then wait at most for the last
+// levels.
+// A band publishes with an agent-scope release (every thread) before its count, a waiter reads the count relaxed and
+// then acquires, so levels written on any XCD are seen fresh; those fences write back and invalidate caches, so bands
+// are coarse and a wave that already saw a pair's pyramids complete skips the wait (and its fence) for the pair's next
+// points. A wait gives up after a bounded number of sleeps, setting the context's error word (the results are then not
+// valid), so every wave reaches the exit.
 constexpr int FB = 64;          // pairs per fused launch
-constexpr int FP = 4;           // points per LK item (one per wave)
-constexpr int FCTR = 4;         // counter words before the done counts: [0] next item, [1] error word
+constexpr int FBAND = 16;       // output rows per pyramid item (a band across the whole level)
+constexpr int FQ = NXCD;        // work queues: one per XCD (workgroup w serves queue w mod 8, the XCD it runs on)
+constexpr int FERR = FQ;        // the error word
+constexpr int FCTR = 16;        // counter words before the done counts: [0, FQ) each queue's next item, [FERR] error
 
 struct FusedArgs {
     LkArgs lk;                  // pairs' pyramids, top levels, point slices, window configuration, output
     const uint8_t *f0[FB], *f1[FB];
     int p0[FB], p1[FB];         // frame pitches (bytes)
-    int item0[FB + 1];          // first item of each pair; item0[npairs] = items in all
     int *ctr;                   // FCTR words, then done[pair][side][level]
 };
 
-__device__ __forceinline__ int level_tiles(const Level &L) { return ((L.w + 63) / 64) * ((L.h + 15) / 16); }
+__device__ __forceinline__ int level_bands(const Level &L) { return (L.h + FBAND - 1) / FBAND; }
 
-// Wait until *p >= need: sleeps between reads; gives up (setting the error word) after 2^24 sleeps, or at once once
-// another wait has given up, so a launch with a broken wait still drains in about a second.
+// Wait until *p >= need, called by a whole wave (every lane reads the same word; the value each test uses is lane 0's,
+// so the loop and its exit are uniform): sleeps between reads; gives up (setting the error word) after 2^24 sleeps, or
+// at once once another wait has given up, so a launch with a broken wait still ends.
 __device__ __forceinline__ void wait_count(int *p, int need, int *err)
 {
-    for (int spins = 0; __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need; spins++) {
+    for (int spins = 0;
+         __builtin_amdgcn_readfirstlane(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need;
+         spins++) {
         __builtin_amdgcn_s_sleep(2);
-        if ((spins & 1023) == 1023 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        if ((spins & 1023) == 1023 &&
+            __builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
             break;
         if (spins == (1 << 24)) {
             __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -798,104 +813,156 @@ __device__ __forceinline__ void wait_count(int *p, int need, int *err)
     }
 }
 
+// Add `add` to *p once for the calling wave (lane 0's operand; the others add 0) and return the old value, uniform.
+// Called by whole waves only, so no branch on the lane index sits in the persistent loops: a loop whose back edge
+// passes a lane-0-only block can be compiled into one that lane 0 leaves alone while the other lanes go round.
+__device__ __forceinline__ int wave_fetch_add(int *p, int add, int lane)
+{
+    return __builtin_amdgcn_readfirstlane(
+        __hip_atomic_fetch_add(p, lane == 0 ? add : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
 // The counters of a fused launch to zero; the error word only when clear_err (the context's first launch): it stays
 // set until the host reads it.
 __global__ __launch_bounds__(256) void fused_reset(int *ctr, int n, int clear_err)
 {
     for (int i = threadIdx.x; i < n; i += 256)
-        if (i != 1 || clear_err)
+        if (i != FERR || clear_err)
             ctr[i] = 0;
 }
 
+// Bands of a level of one side in queue q: b = q, q + 8, ...
+__device__ __forceinline__ int queue_bands(int nb, int q) { return nb > q ? (nb - q + FQ - 1) / FQ : 0; }
+
+
 template <int MAXR, int SOLVE>
-__global__ __launch_bounds__(256) void fused_kernel(FusedArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fused_kernel(FusedArgs a)
 {
     __shared__ int s_item;
-    __shared__ int lds[rt::tile_lds_elems<16>()];
+    // this workgroup's queue: first the bands of every pair's level 1, then of level 2, ... (qa[(l - 1) np + k]: the
+    // first item of level l of pair k, both sides), then the q-th of 8 contiguous runs of all points (pair by pair)
+    __shared__ int qa[(kMaxLevels - 1) * FB + 1];
+    __shared__ int lds[rt::tile_lds_elems<FBAND>()];
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int np = a.lk.npairs, total = a.item0[np];
-    int *err = a.ctr + 1, *done = a.ctr + FCTR;
+    const int np = a.lk.npairs, q = blockIdx.x % FQ;
+    int *err = a.ctr + FERR, *done = a.ctr + FCTR;
+    int nl = 1; // the most levels of any pair
+    for (int k = 0; k < np; k++)
+        nl = a.lk.I[k]->levels > nl ? a.lk.I[k]->levels : nl;
+    if (tid == 0) {
+        int acc = 0;
+        for (int l = 1; l < nl; l++)
+            for (int k = 0; k < np; k++) {
+                qa[(l - 1) * np + k] = acc;
+                const PyrView &I = *a.lk.I[k], &J = *a.lk.J[k];
+                if (l < I.levels)
+                    acc += queue_bands(level_bands(I.l[l]), q) + queue_bands(level_bands(J.l[l]), q);
+            }
+        qa[(nl - 1) * np] = acc;
+    }
+    __syncthreads();
+    const int npa = (nl - 1) * np, nbands = __builtin_amdgcn_readfirstlane(qa[npa]);
+    const int run = (a.lk.n + FQ - 1) / FQ, pfirst = q * run;
+    const int plast = pfirst + run < a.lk.n ? pfirst + run : a.lk.n;
+    const int total = nbands + (plast > pfirst ? plast - pfirst : 0);
     for (;;) {
-        if (tid == 0)
-            s_item = __hip_atomic_fetch_add(a.ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (wave == 0) // uniform: wave 0 takes the next item and every lane of it stores the same value
+            s_item = wave_fetch_add(a.ctr + q, 1, lane);
         __syncthreads();
-        const int item = s_item;
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads(); // s_item is rewritten next round
         if (item >= total)
             break;
-        int lo = 0, hi = np - 1; // the pair: the last whose first item is at or before this one
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (item >= a.item0[mid])
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const int k = lo;
-        const PyrView &I = *a.lk.I[k], &J = *a.lk.J[k];
-        int loc = item - a.item0[k];
-        bool pyr_item = false;
-        for (int l = 1; l < I.levels && !pyr_item; l++) {
-            for (int sd = 0; sd < 2; sd++) {
-                const PyrView &P = sd ? J : I;
-                const int tn = level_tiles(P.l[l]);
-                if (loc >= tn) {
-                    loc -= tn;
-                    continue;
-                }
-                int *cnt = done + (k * 2 + sd) * kMaxLevels;
-                if (l >= 2) {
-                    if (tid == 0)
-                        wait_count(cnt + l - 1, level_tiles(P.l[l - 1]), err);
-                    __syncthreads();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                }
-                rt::RJob job;
-                job.dst = (void *)P.l[l].p;
-                job.dw = P.l[l].w;
-                job.dh = P.l[l].h;
-                job.dp = P.l[l].pitch;
-                job.tiles_x = (P.l[l].w + 63) / 64;
-                job.t0 = 0;
-                if (l == 1) {
-                    job.src = sd ? a.f1[k] : a.f0[k];
-                    job.sp = sd ? a.p1[k] : a.p0[k];
-                    job.sw = P.l[0].w;
-                    job.sh = P.l[0].h;
-                    job.copy = (void *)P.l[0].p;
-                    job.cp = P.l[0].pitch;
-                    rt::reduce_tile<uint8_t, uint16_t, uint8_t, true, true, 16>(job, loc, lds);
-                } else {
-                    job.src = P.l[l - 1].p;
-                    job.sp = P.l[l - 1].pitch;
-                    job.sw = P.l[l - 1].w;
-                    job.sh = P.l[l - 1].h;
-                    job.copy = nullptr;
-                    job.cp = 0;
-                    rt::reduce_tile<uint16_t, uint16_t, uint16_t, false, true, 16>(job, loc, lds);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                __syncthreads();
-                if (tid == 0)
-                    __hip_atomic_fetch_add(cnt + l, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                pyr_item = true;
-                break;
+        if (item < nbands) {
+            // a band: its level and pair from the last entry of qa at or before it, then its side
+            int lo = 0, hi = npa - 1;
+            while (lo < hi) { // values read from LDS are per-lane to the compiler: keep the search in scalars
+                const int mid = (lo + hi + 1) >> 1;
+                if (item >= __builtin_amdgcn_readfirstlane(qa[mid]))
+                    lo = mid;
+                else
+                    hi = mid - 1;
             }
-        }
-        if (pyr_item)
+            const int k = lo % np, l = lo / np + 1;
+            int loc = item - __builtin_amdgcn_readfirstlane(qa[lo]);
+            const int ni = queue_bands(level_bands(a.lk.I[k]->l[l]), q), sd = loc >= ni ? 1 : 0;
+            const PyrView &P = sd ? *a.lk.J[k] : *a.lk.I[k];
+            loc = q + FQ * (loc - sd * ni); // the band of the level
+            int *cnt = done + (k * 2 + sd) * kMaxLevels;
+            if (l >= 2) {
+                if (wave == 0)
+                    wait_count(cnt + l - 1, level_bands(P.l[l - 1]), err);
+                __syncthreads();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            rt::RJob job;
+            job.dst = (void *)P.l[l].p;
+            job.dw = P.l[l].w;
+            job.dh = P.l[l].h;
+            job.dp = P.l[l].pitch;
+            job.tiles_x = (P.l[l].w + 63) / 64;
+            job.t0 = 0;
+            // the band's tiles, left to right (tile rows are FBAND = 16 output rows)
+            if (l == 1) {
+                job.src = sd ? a.f1[k] : a.f0[k];
+                job.sp = sd ? a.p1[k] : a.p0[k];
+                job.sw = P.l[0].w;
+                job.sh = P.l[0].h;
+                job.copy = (void *)P.l[0].p;
+                job.cp = P.l[0].pitch;
+                for (int tx = 0; tx < job.tiles_x; tx++) {
+                    rt::reduce_tile<uint8_t, uint16_t, uint8_t, true, true, FBAND>(job, loc * job.tiles_x + tx, lds);
+                    __syncthreads(); // the next tile reuses the LDS
+                }
+            } else {
+                job.src = P.l[l - 1].p;
+                job.sp = P.l[l - 1].pitch;
+                job.sw = P.l[l - 1].w;
+                job.sh = P.l[l - 1].h;
+                job.copy = nullptr;
+                job.cp = 0;
+                for (int tx = 0; tx < job.tiles_x; tx++) {
+                    rt::reduce_tile<uint16_t, uint16_t, uint16_t, false, true, FBAND>(job, loc * job.tiles_x + tx,
+                                                                                      lds);
+                    __syncthreads();
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            if (wave == 0)
+                wave_fetch_add(cnt + l, 1, lane);
             continue;
-        // an LK item: both pyramids complete up to level max(top, 1) (which completes every level below it)
-        const int top = a.lk.top[k], tl = top > 1 ? top : 1;
-        if (tid == 0) {
-            wait_count(done + (k * 2) * kMaxLevels + tl, level_tiles(I.l[tl]), err);
-            wait_count(done + (k * 2 + 1) * kMaxLevels + tl, level_tiles(J.l[tl]), err);
         }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const int lp = loc * FP + wave, npk = a.lk.pre[k + 1] - a.lk.pre[k];
-        if (lp < npk) {
+        // Every later item of this queue is an LK item (one point): the waves now take them one at a time, each on its
+        // own (wave 0 starts with the item just taken), and no longer meet at barriers.
+        int it = wave == 0 ? item : -1, ready_pair = -1;
+        for (;;) {
+            if (it < 0)
+                it = wave_fetch_add(a.ctr + q, 1, lane);
+            if (it >= total)
+                return;
+            const int gp = pfirst + (it - nbands); // the point in the numbering of all pairs' points
+            int lo = 0, hi = np - 1; // its pair: the last whose numbering starts at or before it
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (gp >= a.lk.pre[mid])
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int k = lo;
+            const PyrView &I = *a.lk.I[k], &J = *a.lk.J[k];
+            // both pyramids complete up to level max(top, 1) (which completes every level below it)
+            const int top = a.lk.top[k], tl = top > 1 ? top : 1;
+            if (ready_pair != k) {
+                wait_count(done + (k * 2) * kMaxLevels + tl, level_bands(I.l[tl]), err);
+                wait_count(done + (k * 2 + 1) * kMaxLevels + tl, level_bands(J.l[tl]), err);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                ready_pair = k;
+            }
+            const int pt = a.lk.start[k] + (gp - a.lk.pre[k]);
+            it = -1;
             const Geo g = geometry<MAXR>(a.lk.cfg, lane);
-            const int pt = a.lk.start[k] + lp;
             const float x = a.lk.xy[2 * pt], y = a.lk.xy[2 * pt + 1];
             const float gx = a.lk.guess ? a.lk.guess[2 * pt] : 0.0f, gy = a.lk.guess ? a.lk.guess[2 * pt + 1] : 0.0f;
             const Slice sl{nullptr, nullptr, nullptr};
@@ -937,7 +1004,8 @@ int launch_fused_t(vtg_ctx *c, const FusedArgs &a, int total)
         if (per_cu < 1)
             per_cu = 1;
     }
-    const int grid = std::min(total, per_cu * cus);
+    // every queue needs workgroups: a multiple of FQ, at least FQ
+    const int grid = std::max(FQ, std::min((total + FQ - 1) / FQ * FQ, per_cu * cus / FQ * FQ));
     hipEvent_t ev;
     timer_begin(c, "pyr_lk_fused", &ev);
     hipLaunchKernelGGL((fused_kernel<MAXR, SOLVE>), dim3(grid), dim3(256), 0, c->stream, a);
@@ -1176,14 +1244,11 @@ int vtg_pyr_lk_fused(vtg_ctx *c, int npairs, vtg_pyr *const *I, vtg_pyr *const *
         a.f1[k] = frame1[k];
         a.p0[k] = pitch0[k];
         a.p1[k] = pitch1[k];
-        a.item0[k] = items;
         for (int l = 1; l < I[k]->levels; l++)
-            items += ((I[k]->lv[l].w + 63) / 64) * ((I[k]->lv[l].h + 15) / 16) +
-                     ((J[k]->lv[l].w + 63) / 64) * ((J[k]->lv[l].h + 15) / 16);
-        items += (offsets[k + 1] - offsets[k] + FP - 1) / FP;
+            items += (I[k]->lv[l].h + FBAND - 1) / FBAND + (J[k]->lv[l].h + FBAND - 1) / FBAND;
+        items += (offsets[k + 1] - offsets[k] + 3) / 4; // points, four waves to a workgroup
     }
     a.lk.pre[npairs] = acc;
-    a.item0[npairs] = items;
     a.lk.n = acc;
     if (acc > 0 && (!xy || !out))
         return fail(c, VTG_EINVAL, "vtg_pyr_lk_fused: null points or output");
@@ -1194,7 +1259,7 @@ int vtg_pyr_lk_fused(vtg_ctx *c, int npairs, vtg_pyr *const *I, vtg_pyr *const *
     a.lk.guess = guess;
     a.lk.fb_max = fb_max;
     a.lk.out = out;
-    a.lk.waves = FP;
+    a.lk.waves = 4;
     a.lk.per_xcd = 0;
     const int nctr = FCTR + FB * 2 * kMaxLevels;
     if (!c->fused)
@@ -1217,9 +1282,9 @@ int vtg_ctx_device_error(vtg_ctx *c, int *code)
     if (!c->fused)
         return VTG_OK;
     VTG_HIP(c, hipStreamSynchronize(c->stream));
-    VTG_HIP(c, hipMemcpy(code, (int *)c->fused + 1, sizeof(int), hipMemcpyDeviceToHost));
+    VTG_HIP(c, hipMemcpy(code, (int *)c->fused + FERR, sizeof(int), hipMemcpyDeviceToHost));
     if (*code)
-        VTG_HIP(c, hipMemset((int *)c->fused + 1, 0, sizeof(int)));
+        VTG_HIP(c, hipMemset((int *)c->fused + FERR, 0, sizeof(int)));
     return VTG_OK;
 }
 
