@@ -64,4 +64,46 @@ struct lk_out {
 };
 #endif
 
+/* Still-cell vectors (src/kern/still.h:19-25, src/common/vt_proto.h:385-416). Named vtg_* so a translation unit may
+ * also include the reference's vt_proto.h; the layouts are the reference's, so pointers convert both ways. */
+#ifndef VTG_STILL_TYPES
+#define VTG_STILL_TYPES
+#define VTG_CELL_W 16                /* a cell: 16 x 8 pixels of the level read (VT_CELL_W, VT_CELL_H) */
+#define VTG_CELL_H 8
+#define VTG_CELL_LK 0xffu            /* state: the low byte is LK's status, 0 = a good vector */
+#define VTG_CELL_CHANGED 0x100u      /* this cell's own evidence said it moved */
+#define VTG_CELL_MOVING 0x200u       /* it, or a neighbour, moved: its mean starts again */
+#define VTG_CELL_COUNT_SHIFT 16      /* the high half: frames in its mean */
+
+struct vtg_still_cfg { /* = struct still_cfg */
+    int w, h;          /* of the level read */
+    int avg;           /* frames at most in a cell's mean */
+    float motion_px;   /* a good vector longer than this: moved */
+    float motion_grey; /* mean |frame - mean| over a cell above this, 8-bit grey: moved */
+    int threads;       /* the reference's team size; unused here */
+};
+
+struct vtg_still_cell { /* = struct vt_cell, 24 B */
+    float u, v;       /* pixels of the level; meaningful when LK's status is 0 */
+    float diff;       /* mean |frame - mean| over the cell, 8-bit grey */
+    uint32_t state;   /* VTG_CELL_* */
+    float sigma;      /* sqrt(cov_xx + cov_yy) of LK's own covariance (0 unless LK's status is 0) */
+    float pad;
+};
+
+struct vtg_motion { /* = the leading fields of struct vt_motion, through frames */
+    uint32_t cols, rows;
+    uint32_t cell_at;   /* not filled (the reference's transport offset) */
+    uint32_t tracked;   /* cells with a good vector */
+    uint32_t moving;    /* cells that moved at the newest frame */
+    float count_mean;   /* frames in a cell's mean */
+    float drift_u, drift_v, drift_rms; /* mean vector (and scatter) over cells that did not move */
+    float diff_median;  /* the camera's noise: the median cell difference, 8-bit grey */
+    float diff_limit;   /* what a cell's difference had to exceed */
+    float pad;
+    uint64_t cycles_vectors, cycles_mean; /* not filled */
+    uint64_t frames;
+};
+#endif
+
 #endif

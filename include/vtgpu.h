@@ -317,6 +317,33 @@ int vtg_lk_track_planes(vtg_ctx *ctx, const struct vt_plane *I, const struct vt_
 int vtg_correlate_planes(vtg_ctx *ctx, const struct vt_plane *pl, const float *app, int tw, int th, double norm,
                          int n, const int *x, const int *y, double *out);
 
+/* --- still-cell vectors (src/kern/still.h:18-53, still.c): one camera's running-mean picture of the level it reads,
+ * and per 16 x 8 cell, LK (15 x 15, one level, flowback 0.25) from that mean to each new frame. A frame is the
+ * reference's four phases in order; `vtg_still_frame` runs them in one call. The frame is a level of a device pyramid
+ * (any storage, read as Q6 fp32) or, in the _plane forms, a padded host plane. Cells, motion and hash match the
+ * reference's: u, v within LK's tolerance of the reference's LK; the differences, decisions, counts, weights and the
+ * mean bit for bit given the same vectors. */
+typedef struct vtg_still vtg_still;
+int vtg_still_create(vtg_ctx *ctx, const struct vtg_still_cfg *cfg, vtg_still **out); /* still_create, still.c:60-96 */
+int vtg_still_destroy(vtg_ctx *ctx, vtg_still *s);
+int vtg_still_first(const vtg_still *s);                                            /* still_first, still.c:97 */
+int vtg_still_size(const vtg_still *s, int *cols, int *rows);
+/* still_vectors (still.c:113-138): the cells' vectors, differences and sigmas against `level` of `pyr` */
+int vtg_still_vectors(vtg_ctx *ctx, vtg_still *s, const vtg_pyr *pyr, int level);
+int vtg_still_vectors_plane(vtg_ctx *ctx, vtg_still *s, const struct vt_plane *cur);
+/* still_decide (still.c:140-237): noise median, evidence, 3 x 3 vote, counts, weights, drift; waits for the vectors */
+int vtg_still_decide(vtg_ctx *ctx, vtg_still *s);
+/* still_update (still.c:239-262): the weighted mean with this frame */
+int vtg_still_update(vtg_ctx *ctx, vtg_still *s, const vtg_pyr *pyr, int level);
+int vtg_still_update_plane(vtg_ctx *ctx, vtg_still *s, const struct vt_plane *cur);
+int vtg_still_finish(vtg_ctx *ctx, vtg_still *s);                                   /* still_finish, still.c:264-270 */
+/* the four phases in order (vectors skipped on the first frame, as the reference's caller does: m3_ref.c:273-277) */
+int vtg_still_frame(vtg_ctx *ctx, vtg_still *s, const vtg_pyr *pyr, int level);
+int vtg_still_cells(vtg_ctx *ctx, vtg_still *s, struct vtg_still_cell *out);             /* still_cells: cols x rows */
+int vtg_still_mean(vtg_ctx *ctx, const vtg_still *s, const struct vt_plane *out);  /* still_mean, into a host plane */
+int vtg_still_motion(const vtg_still *s, struct vtg_motion *m);                    /* still_motion, still.c:280-293 */
+int vtg_still_hash(vtg_ctx *ctx, vtg_still *s, uint64_t h, uint64_t *out);         /* still_hash, still.c:272-278 */
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,4 +122,86 @@ __host__ __device__ inline float norm_exact(float s, float g, float o)
     return clamp_rint(neg, (uint64_t)mag, e);
 }
 
+// Exact long double arithmetic on non-negative values m * 2^e (m < 2^64), each result rounded as x87 rounds it: the
+// still-cell differences and sigmas (src/kern/still.c:99-111, 136).
+
+struct XL {
+    uint64_t m;
+    int e;
+};
+
+__host__ __device__ inline XL xl_add(XL a, float f)
+{
+    const FDec d = fdec(f); // f >= 0
+    if (d.m == 0)
+        return a;
+    if (a.m == 0)
+        return XL{(uint64_t)d.m, d.e};
+    const int top = (a.e + 64 > d.e + 24 ? a.e + 64 : d.e + 24), base = top - 126;
+    unsigned __int128 mag = shift_jam((unsigned __int128)a.m, a.e - base) + shift_jam((unsigned __int128)d.m, d.e - base);
+    int e = base;
+    round_sig(mag, e, 64);
+    return XL{(uint64_t)mag, e};
+}
+
+// a / n (n >= 1), rounded to 64 bits and then to float: (float)(long double / n)
+__host__ __device__ inline float xl_div_to_float(XL a, int n)
+{
+    if (a.m == 0)
+        return 0.0f;
+    const int lz = __builtin_clzll(a.m); // the top bit to bit 63, so the quotient keeps more than 64 bits
+    a.m <<= lz;
+    a.e -= lz;
+    const unsigned __int128 num = (unsigned __int128)a.m << 60;
+    const unsigned __int128 q = num / (unsigned __int128)n, r = num - q * (unsigned __int128)n;
+    unsigned __int128 mag = (q << 1) | (r != 0 ? 1 : 0); // a sticky bit below every rounding position
+    int e = a.e - 61;
+    round_sig(mag, e, 64);
+    round_sig(mag, e, 24);
+    return ldexpf((float)(uint64_t)mag, e);
+}
+
+__host__ __device__ inline uint64_t isqrt128(unsigned __int128 v)
+{
+    unsigned __int128 r = 0, bit = (unsigned __int128)1 << 126;
+    while (bit > v)
+        bit >>= 2;
+    while (bit) {
+        if (v >= r + bit) {
+            v -= r + bit;
+            r = (r >> 1) + bit;
+        } else {
+            r >>= 1;
+        }
+        bit >>= 2;
+    }
+    return (uint64_t)r;
+}
+
+// (float)sqrtl((long double)a + (long double)b) for a, b >= 0: the sum rounded to 64 bits, its root to 64, then to 24
+__host__ __device__ inline float sqrt_sum(float a, float b)
+{
+    if (!(a >= 0.0f && b >= 0.0f)) // a covariance is never negative; keep whatever a negative sum gives
+        return (float)sqrt((double)a + (double)b);
+    XL s = xl_add(XL{0, 0}, a);
+    s = xl_add(s, b);
+    if (s.m == 0)
+        return 0.0f;
+    // s.m * 2^e with an even exponent and the integer part in [2^126, 2^128), so the root has 64 bits
+    const int lz = __builtin_clzll(s.m); // s.m < 2^64
+    int k = 64 + lz;                      // shift that puts the top bit at 127
+    if (((s.e - k) & 1) != 0)
+        k--;                              // top bit at 126
+    const unsigned __int128 M = (unsigned __int128)s.m << k;
+    uint64_t r = isqrt128(M);
+    const unsigned __int128 rem = M - (unsigned __int128)r * r;
+    unsigned __int128 mag = r;
+    int e = (s.e - k) / 2;
+    if (rem > (unsigned __int128)r) // the root's fraction is above one half (it is never exactly one half)
+        mag += 1;
+    round_sig(mag, e, 64);
+    round_sig(mag, e, 24);
+    return ldexpf((float)(uint64_t)mag, e);
+}
+
 }  // namespace vtg

@@ -50,6 +50,30 @@ class Tune(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class StillCfg(C.Structure):
+    """struct vtg_still_cfg (= the reference's struct still_cfg)."""
+    _fields_ = [("w", C.c_int), ("h", C.c_int), ("avg", C.c_int), ("motion_px", C.c_float),
+                ("motion_grey", C.c_float), ("threads", C.c_int)]
+
+
+STILL_CELL_DTYPE = np.dtype([("u", "<f4"), ("v", "<f4"), ("diff", "<f4"), ("state", "<u4"), ("sigma", "<f4"),
+                             ("pad", "<f4")])
+
+STILL_MOTION_FIELDS = ["cols", "rows", "cell_at", "tracked", "moving", "count_mean", "drift_u", "drift_v",
+                       "drift_rms", "diff_median", "diff_limit", "pad", "cycles_vectors", "cycles_mean", "frames"]
+
+
+class StillMotion(C.Structure):
+    """struct vtg_motion (= the leading fields of the reference's struct vt_motion)."""
+    _fields_ = [(n, C.c_uint32) for n in STILL_MOTION_FIELDS[:5]] + \
+               [(n, C.c_float) for n in STILL_MOTION_FIELDS[5:12]] + \
+               [(n, C.c_uint64) for n in STILL_MOTION_FIELDS[12:]]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k in ("tracked", "moving", "count_mean", "drift_u", "drift_v", "drift_rms",
+                                              "diff_median", "diff_limit", "frames")}
+
+
 class SrvCfg(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_size_t), ("lanes", C.c_int), ("max_batch", C.c_int),
                 ("lk", LkCfg), ("solve_mode", C.c_int), ("fb_max", C.c_float), ("tmpl_w", C.c_int),
@@ -120,6 +144,14 @@ def lib():
         "vtg_img_reduce": [V, C.POINTER(Plane), C.POINTER(Plane), I, I],
         "vtg_lk_track_planes": [V, V, V, I, C.POINTER(LkCfg), I, V, V, V, V, F, V],
         "vtg_correlate_planes": [V, C.POINTER(Plane), V, I, I, C.c_double, I, V, V, V],
+        "vtg_still_create": [V, C.POINTER(StillCfg), C.POINTER(V)], "vtg_still_destroy": [V, V],
+        "vtg_still_first": [V], "vtg_still_size": [V, C.POINTER(I), C.POINTER(I)],
+        "vtg_still_vectors": [V, V, V, I], "vtg_still_vectors_plane": [V, V, C.POINTER(Plane)],
+        "vtg_still_decide": [V, V], "vtg_still_update": [V, V, V, I],
+        "vtg_still_update_plane": [V, V, C.POINTER(Plane)], "vtg_still_finish": [V, V],
+        "vtg_still_frame": [V, V, V, I], "vtg_still_cells": [V, V, V],
+        "vtg_still_mean": [V, V, C.POINTER(Plane)], "vtg_still_motion": [V, C.POINTER(StillMotion)],
+        "vtg_still_hash": [V, V, C.c_uint64, C.POINTER(C.c_uint64)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -146,7 +178,11 @@ EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", 
            "vtg_srv_open", "vtg_srv_close", "vtg_srv_submit", "vtg_srv_poll", "vtg_srv_wait", "vtg_srv_get_stats",
            "vtg_ncc_score",
            "vtg_graph_begin", "vtg_graph_end", "vtg_graph_launch", "vtg_graph_destroy", "vtg_img_u8_to_q6", "vtg_img_reduce",
-           "vtg_lk_track_planes", "vtg_correlate_planes"]
+           "vtg_lk_track_planes", "vtg_correlate_planes",
+           "vtg_still_create", "vtg_still_destroy", "vtg_still_first", "vtg_still_size", "vtg_still_vectors",
+           "vtg_still_vectors_plane", "vtg_still_decide", "vtg_still_update", "vtg_still_update_plane",
+           "vtg_still_finish", "vtg_still_frame", "vtg_still_cells", "vtg_still_mean", "vtg_still_motion",
+           "vtg_still_hash"]
 
 
 def synth():
@@ -298,6 +334,9 @@ class Context:
     # pyramids
     def pyramid(self, w, h, levels, storage=STORE_Q6):
         return Pyramid(self, w, h, levels, storage)
+
+    def still(self, w, h, avg=16, motion_px=0.5, motion_grey=4.0):
+        return Still(self, StillCfg(w, h, avg, motion_px, motion_grey, 1))
 
     def build_many(self, pyrs, frames, flags=0):
         """Build several pyramids in one launch per level: frames are DevBufs (pitch = width)."""
@@ -569,6 +608,72 @@ class Template:
 
 def plane_stride(w):
     return ((w + 15) & ~15) + 2 * PAD_X
+
+
+class Still:
+    """A vtg_still: one camera's running mean of the level it reads and its 16 x 8 cells' vectors (still.h)."""
+
+    def __init__(self, ctx, cfg):
+        self.ctx, self.cfg, self.L = ctx, cfg, ctx.L
+        h = C.c_void_p()
+        ctx.check(self.L.vtg_still_create(ctx.h, C.byref(cfg), C.byref(h)), "still_create")
+        self.h = h
+        c, r = C.c_int(), C.c_int()
+        ctx.check(self.L.vtg_still_size(h, C.byref(c), C.byref(r)), "still_size")
+        self.cols, self.rows = c.value, r.value
+
+    @property
+    def ncells(self):
+        return self.cols * self.rows
+
+    def first(self):
+        return bool(self.L.vtg_still_first(self.h))
+
+    def vectors(self, pyr=None, level=0, plane=None):
+        if plane is not None:
+            self.ctx.check(self.L.vtg_still_vectors_plane(self.ctx.h, self.h, C.byref(plane.c)), "still_vectors")
+        else:
+            self.ctx.check(self.L.vtg_still_vectors(self.ctx.h, self.h, pyr.h, level), "still_vectors")
+
+    def decide(self):
+        self.ctx.check(self.L.vtg_still_decide(self.ctx.h, self.h), "still_decide")
+
+    def update(self, pyr=None, level=0, plane=None):
+        if plane is not None:
+            self.ctx.check(self.L.vtg_still_update_plane(self.ctx.h, self.h, C.byref(plane.c)), "still_update")
+        else:
+            self.ctx.check(self.L.vtg_still_update(self.ctx.h, self.h, pyr.h, level), "still_update")
+
+    def finish(self):
+        self.ctx.check(self.L.vtg_still_finish(self.ctx.h, self.h), "still_finish")
+
+    def frame(self, pyr, level=0):
+        self.ctx.check(self.L.vtg_still_frame(self.ctx.h, self.h, pyr.h, level), "still_frame")
+
+    def cells(self):
+        out = np.zeros(self.ncells, STILL_CELL_DTYPE)
+        self.ctx.check(self.L.vtg_still_cells(self.ctx.h, self.h, out.ctypes.data), "still_cells")
+        return out
+
+    def mean(self):
+        hp = HostPlane(self.cfg.w, self.cfg.h)
+        self.ctx.check(self.L.vtg_still_mean(self.ctx.h, self.h, C.byref(hp.c)), "still_mean")
+        return hp
+
+    def motion(self):
+        m = StillMotion()
+        self.ctx.check(self.L.vtg_still_motion(self.h, C.byref(m)), "still_motion")
+        return m.as_dict()
+
+    def hash(self, h=0xcbf29ce484222325):
+        out = C.c_uint64()
+        self.ctx.check(self.L.vtg_still_hash(self.ctx.h, self.h, h, C.byref(out)), "still_hash")
+        return out.value
+
+    def destroy(self):
+        if self.h:
+            self.L.vtg_still_destroy(self.ctx.h, self.h)
+            self.h = None
 
 
 class HostPlane:

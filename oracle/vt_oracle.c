@@ -948,3 +948,215 @@ void vo_ncc_search(const struct vt_plane *pl, const float *app, int tw, int th, 
     if (!cells_out)
         free(sc);
 }
+
+/* ------------------------------------------------------------------ still-cell vectors (src/kern/still.c)
+ * Pinned against the reference's own still.c built from source in oracle/_ref (tests/test_oracle.py: the hash of
+ * every cell and the mean after each of a run of frames, and the motion summary, equal bit for bit), and by closed
+ * forms (identical frames: zero vectors and differences; a constant offset: the exact difference). */
+#define ST_PATCH 15          /* still.c:12-18: the LK patch and its limits */
+#define ST_ITER 6
+#define ST_EPS 0.01f
+#define ST_MIN_EIG 0.1f
+#define ST_MAX_RES 16.0f
+#define ST_FB 0.25f
+#define ST_VEC_SIGMA 12.0L   /* still.c:19-22: how far past LK's own sigma a vector must reach, the noise rules */
+#define ST_DIFF_NOISE 3.0L
+#define ST_NOISE_FRAMES 32.0L
+#define ST_EVIDENCE 2
+#define ST_MARGIN 9.0f       /* still.c:23 */
+#define ST_BINS 256          /* still.c:29-30 */
+#define ST_PER_GREY 16
+
+struct vo_still *vo_still_new(const struct vtg_still_cfg *c)
+{
+    struct vo_still *s = calloc(1, sizeof *s);
+    s->w = c->w;
+    s->h = c->h;
+    s->avg = c->avg;
+    s->motion_px = c->motion_px;
+    s->motion_grey = c->motion_grey;
+    s->cols = (c->w + VTG_CELL_W - 1) / VTG_CELL_W;
+    s->rows = (c->h + VTG_CELL_H - 1) / VTG_CELL_H;
+    s->ncells = s->cols * s->rows;
+    s->mean = vo_plane_new(c->w, c->h);
+    s->cell = calloc((size_t)s->ncells, sizeof *s->cell);
+    s->raw = calloc((size_t)(s->cols + 2) * (size_t)(s->rows + 2), 1);
+    s->count = calloc((size_t)s->ncells, sizeof *s->count);
+    s->weight = calloc((size_t)s->ncells, sizeof *s->weight);
+    return s;
+}
+
+void vo_still_free(struct vo_still *s)
+{
+    if (!s)
+        return;
+    vo_plane_free(&s->mean);
+    free(s->cell);
+    free(s->raw);
+    free(s->count);
+    free(s->weight);
+    free(s);
+}
+
+struct lk_cfg vo_still_lk_cfg(void)
+{
+    struct lk_cfg c = {ST_PATCH, ST_PATCH, ST_ITER, ST_EPS, ST_MIN_EIG, ST_MAX_RES, 0};
+    return c;
+}
+
+void vo_still_centre(const struct vo_still *s, int i, float *x, float *y)
+{
+    const float xm = (float)s->w - 1.0f - ST_MARGIN, ym = (float)s->h - 1.0f - ST_MARGIN;
+    float cx = (float)((i % s->cols) * VTG_CELL_W + VTG_CELL_W / 2), cy = (float)((i / s->cols) * VTG_CELL_H + VTG_CELL_H / 2);
+    *x = cx < ST_MARGIN ? ST_MARGIN : cx > xm ? xm : cx;
+    *y = cy < ST_MARGIN ? ST_MARGIN : cy > ym ? ym : cy;
+}
+
+/* cell_diff (still.c:99-111): per row of the cell, the 16 lanes' |frame - mean| summed in the lane order; the rows
+ * summed in long double; the mean over rows x 16 (the lanes past the plane's edge read its padding). */
+float vo_still_cell_diff(const struct vo_still *s, const struct vt_plane *cur, int col, int row)
+{
+    const int x = col * VTG_CELL_W, y0 = row * VTG_CELL_H, y1 = y0 + VTG_CELL_H < s->h ? y0 + VTG_CELL_H : s->h;
+    long double sum = 0.0L;
+    for (int y = y0; y < y1; y++) {
+        float d[NL];
+        for (int i = 0; i < NL; i++)
+            d[i] = fabsf(plane_row(cur, y)[x + i] - plane_row(&s->mean, y)[x + i]);
+        sum += (long double)lane_hsum(d);
+    }
+    return (float)(sum / (long double)((y1 - y0) * VTG_CELL_W));
+}
+
+/* still_vectors (still.c:113-138): LK (lane tier) from the mean to the frame at each cell's centre, with the flowback
+ * check; the vector, the difference in 8-bit grey, LK's status and sigma. */
+void vo_still_vectors(struct vo_still *s, const struct vt_plane *cur, int threads)
+{
+    const int n = s->ncells;
+    float *xs = malloc(sizeof(float) * (size_t)n), *ys = malloc(sizeof(float) * (size_t)n);
+    struct lk_out *o = malloc(sizeof *o * (size_t)n);
+    for (int i = 0; i < n; i++)
+        vo_still_centre(s, i, &xs[i], &ys[i]);
+    const struct lk_cfg cfg = vo_still_lk_cfg();
+    vo_lk_batch(VO_LANE, &s->mean, cur, 0, &cfg, n, xs, ys, NULL, NULL, ST_FB, o, threads);
+    for (int i = 0; i < n; i++) {
+        struct vtg_still_cell *c = &s->cell[i];
+        c->u = (float)((long double)o[i].x - (long double)xs[i]);
+        c->v = (float)((long double)o[i].y - (long double)ys[i]);
+        c->diff = (float)((long double)vo_still_cell_diff(s, cur, i % s->cols, i / s->cols) / (long double)VT_Q6);
+        c->state = o[i].status;
+        c->sigma = o[i].status ? 0.0f : (float)sqrtl((long double)o[i].cov_xx + (long double)o[i].cov_yy);
+    }
+    free(xs);
+    free(ys);
+    free(o);
+}
+
+/* still_decide (still.c:140-237): the camera's noise as the median cell difference (a 1/16-grey histogram), the
+ * usual noise followed slowly unless everything changed, each cell's own evidence (moved past max(motion_px,
+ * 12 sigma), LK lost its patch, or the difference over the limit), a 3 x 3 vote of at least two, the cell's frame
+ * count and weight, its state, and the drift of the cells that held still. */
+void vo_still_decide(struct vo_still *s)
+{
+    const int cols = s->cols, rows = s->rows, wide = cols + 2, n = s->ncells;
+    uint32_t hist[ST_BINS] = {0};
+    for (int i = 0; i < n; i++) {
+        const long b = (long)((long double)s->cell[i].diff * (long double)ST_PER_GREY);
+        hist[b < ST_BINS - 1 ? b : ST_BINS - 1]++;
+    }
+    int med = 0;
+    uint32_t below = 0;
+    while (med < ST_BINS - 1 && below + hist[med] < (uint32_t)n / 2u)
+        below += hist[med++];
+    const long double noise = (long double)med / (long double)ST_PER_GREY, three = noise * ST_DIFF_NOISE;
+    const long double limit = three > (long double)s->motion_grey ? three : (long double)s->motion_grey;
+    s->diff_median = (float)noise;
+    s->diff_limit = (float)limit;
+    const long double usual3 = s->noise_usual * ST_DIFF_NOISE;
+    const long double usual = usual3 > (long double)s->motion_grey ? usual3 : (long double)s->motion_grey;
+    const int everything = s->frames >= 2 && noise > usual;
+    if (!everything && s->frames >= 1)
+        s->noise_usual = s->frames == 1 ? noise : s->noise_usual + (noise - s->noise_usual) / ST_NOISE_FRAMES;
+    const long double px2 = (long double)s->motion_px * (long double)s->motion_px;
+    for (int i = 0; i < n; i++) {
+        const struct vtg_still_cell *c = &s->cell[i];
+        const uint32_t lk = c->state & VTG_CELL_LK;
+        const long double u = c->u, v = c->v, sg = (long double)c->sigma * ST_VEC_SIGMA;
+        const long double need = sg * sg > px2 ? sg * sg : px2;
+        const int moved = lk == 0 && u * u + v * v > need;
+        const int lost = (lk & LK_RESIDUAL) != 0;
+        const int changed = (long double)c->diff > limit;
+        s->raw[(size_t)(i / cols + 1) * (size_t)wide + (size_t)(i % cols + 1)] = (uint8_t)(moved || lost || changed);
+    }
+    uint32_t tracked = 0, moving = 0, quiet = 0;
+    long double su = 0.0L, sv = 0.0L, suu = 0.0L, counts = 0.0L;
+    for (int r = 0; r < rows; r++)
+        for (int q = 0; q < cols; q++) {
+            const int i = r * cols + q;
+            uint32_t votes = 0;
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++)
+                    votes += s->raw[(size_t)(r + 1 + dy) * (size_t)wide + (size_t)(q + 1 + dx)];
+            const int any = votes >= ST_EVIDENCE || everything;
+            const int own = s->raw[(size_t)(r + 1) * (size_t)wide + (size_t)(q + 1)];
+            const uint32_t grown = (uint32_t)s->count[i] + 1u;
+            const uint32_t k = any ? 1u : grown < (uint32_t)s->avg ? grown : (uint32_t)s->avg;
+            s->count[i] = (uint16_t)k;
+            s->weight[i] = (float)(1.0L / (long double)k);
+            struct vtg_still_cell *c = &s->cell[i];
+            const uint32_t lk = c->state & VTG_CELL_LK;
+            c->state = lk | (own ? VTG_CELL_CHANGED : 0u) | (any ? VTG_CELL_MOVING : 0u) | (k << VTG_CELL_COUNT_SHIFT);
+            tracked += lk == 0;
+            moving += any != 0;
+            counts += (long double)k;
+            if (lk == 0 && !any) {
+                const long double u = c->u, v = c->v;
+                su += u;
+                sv += v;
+                suu += u * u + v * v;
+                quiet++;
+            }
+        }
+    s->tracked = tracked;
+    s->moving = moving;
+    s->count_mean = (float)(counts / (long double)n);
+    const long double mu = quiet ? su / (long double)quiet : 0.0L, mv = quiet ? sv / (long double)quiet : 0.0L;
+    s->drift_u = (float)mu;
+    s->drift_v = (float)mv;
+    const long double var = quiet ? suu / (long double)quiet - mu * mu - mv * mv : 0.0L;
+    s->drift_rms = var > 0.0L ? (float)sqrtl(var) : 0.0f;
+}
+
+/* still_update (still.c:239-262): each 16-lane group of a row takes its cell's weight: a weight of 1 copies the
+ * frame, otherwise mean + (frame - mean) * weight in one fused multiply-add; the rows' padding renewed. */
+void vo_still_update(struct vo_still *s, const struct vt_plane *cur)
+{
+    for (int r = 0; r < s->rows; r++) {
+        const int y0 = r * VTG_CELL_H, y1 = y0 + VTG_CELL_H < s->h ? y0 + VTG_CELL_H : s->h;
+        for (int y = y0; y < y1; y++) {
+            const float *in = plane_row(cur, y);
+            float *m = plane_row(&s->mean, y);
+            for (int x = 0; x < s->w; x += VTG_CELL_W) {
+                const float wt = s->weight[r * s->cols + x / VTG_CELL_W];
+                for (int i = 0; i < NL; i++)
+                    m[x + i] = wt == 1.0f ? in[x + i] : fmaf(in[x + i] - m[x + i], wt, m[x + i]);
+            }
+        }
+        vo_pad_rows(&s->mean, y0, y1);
+    }
+}
+
+/* still_finish (still.c:264-270) */
+void vo_still_finish(struct vo_still *s)
+{
+    vo_pad_tb(&s->mean);
+    s->frames++;
+}
+
+/* still_hash (still.c:272-278) */
+uint64_t vo_still_hash(const struct vo_still *s, uint64_t h)
+{
+    h = vo_fnv(h, s->cell, (size_t)s->ncells * sizeof(struct vtg_still_cell));
+    for (int y = 0; y < s->h; y++)
+        h = vo_fnv(h, plane_row(&s->mean, y), (size_t)s->w * sizeof(float));
+    return h;
+}

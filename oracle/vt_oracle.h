@@ -81,6 +81,31 @@ struct vo_found { int hx, hy; long double ncc, second; int cells; };
 void vo_ncc_search(const struct vt_plane *pl, const float *app, int tw, int th, float norm, int x0, int x1, int y0,
                    int y1, int step, struct vo_found *f, struct vo_cell *cells_out, int threads);
 
+/* --- still-cell vectors (src/kern/still.c): the state held explicitly so a test can set it from the GPU's --- */
+struct vo_still {
+    int w, h, cols, rows, ncells, avg;
+    float motion_px, motion_grey;
+    struct vt_plane mean;    /* padded, zeroed at creation */
+    struct vtg_still_cell *cell;   /* ncells */
+    uint8_t *raw;            /* (cols + 2) x (rows + 2): this frame's own evidence, a zero border */
+    uint16_t *count;         /* frames in a cell's mean */
+    float *weight;           /* this frame's weight of a cell */
+    uint64_t frames;
+    uint32_t tracked, moving;
+    float count_mean, drift_u, drift_v, drift_rms, diff_median, diff_limit;
+    long double noise_usual;
+};
+struct vo_still *vo_still_new(const struct vtg_still_cfg *c);
+void vo_still_free(struct vo_still *s);
+struct lk_cfg vo_still_lk_cfg(void);
+void vo_still_centre(const struct vo_still *s, int i, float *x, float *y);
+float vo_still_cell_diff(const struct vo_still *s, const struct vt_plane *cur, int col, int row);
+void vo_still_vectors(struct vo_still *s, const struct vt_plane *cur, int threads);
+void vo_still_decide(struct vo_still *s);
+void vo_still_update(struct vo_still *s, const struct vt_plane *cur);
+void vo_still_finish(struct vo_still *s);
+uint64_t vo_still_hash(const struct vo_still *s, uint64_t h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -152,3 +152,97 @@ class Oracle:
 
 def lk_cfg(w=21, h=21, max_iter=10, eps=0.01, min_eig=1e-4, max_residual=40.0):
     return LkCfg(w, h, max_iter, eps, min_eig, max_residual, 0)
+
+
+# --- still-cell vectors (struct vtg_still_cfg / vtg_cell / vtg_motion in include/vt_types.h; struct vo_still)
+class StillCfg(C.Structure):
+    _fields_ = [("w", C.c_int), ("h", C.c_int), ("avg", C.c_int), ("motion_px", C.c_float),
+                ("motion_grey", C.c_float), ("threads", C.c_int)]
+
+
+CELL_STILL_DTYPE = np.dtype([("u", "<f4"), ("v", "<f4"), ("diff", "<f4"), ("state", "<u4"), ("sigma", "<f4"),
+                             ("pad", "<f4")])
+assert CELL_STILL_DTYPE.itemsize == 24
+
+MOTION_FIELDS = ["cols", "rows", "cell_at", "tracked", "moving", "count_mean", "drift_u", "drift_v", "drift_rms",
+                 "diff_median", "diff_limit", "pad", "cycles_vectors", "cycles_mean", "frames"]
+
+
+class Motion(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in MOTION_FIELDS[:5]] + [(n, C.c_float) for n in MOTION_FIELDS[5:12]] + \
+               [(n, C.c_uint64) for n in MOTION_FIELDS[12:]]
+
+
+class VoStill(C.Structure):
+    _fields_ = [("w", C.c_int), ("h", C.c_int), ("cols", C.c_int), ("rows", C.c_int), ("ncells", C.c_int),
+                ("avg", C.c_int), ("motion_px", C.c_float), ("motion_grey", C.c_float), ("mean", Plane),
+                ("cell", C.c_void_p), ("raw", C.c_void_p), ("count", C.c_void_p), ("weight", C.c_void_p),
+                ("frames", C.c_uint64), ("tracked", C.c_uint32), ("moving", C.c_uint32),
+                ("count_mean", C.c_float), ("drift_u", C.c_float), ("drift_v", C.c_float),
+                ("drift_rms", C.c_float), ("diff_median", C.c_float), ("diff_limit", C.c_float),
+                ("noise_usual", C.c_longdouble)]
+
+
+class OracleStill:
+    """The oracle's still-cell state (vo_still_*), with numpy views of its arrays."""
+
+    def __init__(self, oracle, cfg):
+        L = self.L = oracle.lib
+        P = C.POINTER(VoStill)
+        L.vo_still_new.restype = P
+        L.vo_still_new.argtypes = [C.POINTER(StillCfg)]
+        L.vo_still_free.argtypes = [P]
+        L.vo_still_vectors.argtypes = [P, C.POINTER(Plane), C.c_int]
+        L.vo_still_decide.argtypes = [P]
+        L.vo_still_update.argtypes = [P, C.POINTER(Plane)]
+        L.vo_still_finish.argtypes = [P]
+        L.vo_still_hash.restype = C.c_uint64
+        L.vo_still_hash.argtypes = [P, C.c_uint64]
+        L.vo_still_centre.argtypes = [P, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        self.p = L.vo_still_new(C.byref(cfg))
+        s = self.s = self.p.contents
+        n = s.ncells
+        self.cells = np.ctypeslib.as_array(C.cast(s.cell, C.POINTER(C.c_uint8)), (n * 24,)).view(CELL_STILL_DTYPE)
+        self.count = np.ctypeslib.as_array(C.cast(s.count, C.POINTER(C.c_uint16)), (n,))
+        self.weight = np.ctypeslib.as_array(C.cast(s.weight, C.POINTER(C.c_float)), (n,))
+        st = s.mean.stride
+        base = C.cast(s.mean.p, C.c_void_p).value - (PAD_Y * st + PAD_X) * 4
+        self.mean_padded = np.ctypeslib.as_array(C.cast(base, C.POINTER(C.c_float)),
+                                                 ((s.h + 2 * PAD_Y) * st,)).reshape(s.h + 2 * PAD_Y, st)
+
+    def mean(self):
+        s = self.s
+        return self.mean_padded[PAD_Y:PAD_Y + s.h, PAD_X:PAD_X + s.w]
+
+    def vectors(self, cur, threads=8):
+        self.L.vo_still_vectors(self.p, C.byref(cur.c), threads)
+
+    def decide(self):
+        self.L.vo_still_decide(self.p)
+
+    def update(self, cur):
+        self.L.vo_still_update(self.p, C.byref(cur.c))
+
+    def finish(self):
+        self.L.vo_still_finish(self.p)
+
+    def hash(self, h=0xcbf29ce484222325):
+        return self.L.vo_still_hash(self.p, h)
+
+    def motion(self):
+        s = self.s
+        return {k: getattr(s, k) for k in ("tracked", "moving", "count_mean", "drift_u", "drift_v", "drift_rms",
+                                          "diff_median", "diff_limit", "frames")}
+
+    def centres(self):
+        n = self.s.ncells
+        xs, ys = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        x, y = C.c_float(), C.c_float()
+        for i in range(n):
+            self.L.vo_still_centre(self.p, i, C.byref(x), C.byref(y))
+            xs[i], ys[i] = x.value, y.value
+        return xs, ys
+
+    def free(self):
+        self.L.vo_still_free(self.p)
+        self.p = None
