@@ -329,6 +329,66 @@ def serving(vtgpu, frames_per_stream, seed=31, budget=2 << 30):
             "poisson": run(True), "saturated": run(False)}
 
 
+def still_bench(vtgpu, nframes, cpu_frames=4):
+    """Still-cell vectors (still.c) at the reference's live loads: 1080p frames read at level 1 (960 x 540, 4,080 cells)
+    and at level 0 (1920 x 1080, 16,200 cells); the camera jitters by up to 1 px (a pair's two frames alternate). A frame
+    is vtg_still_frame (per-cell LK and differences on the GPU, the decision on this thread, the mean update) with the
+    stream drained after it: p50 / p99 ms per frame and cells per second. Beside it, the oracle's still phases
+    (lane-tier LK on 16 threads) on the same frames, a few frames each."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from oracle_ffi import Oracle, OracleStill, StillCfg
+    f0, f1, _ = vtgpu.frame_pair(W, H, 77, max_shift=1.0, max_rot=0.0)
+    ctx = vtgpu.Context(0)
+    oracle = Oracle()
+    out = {}
+    for lv in (1, 0):
+        w, h = W, H
+        for _ in range(lv):
+            w, h = (w + 1) // 2, (h + 1) // 2
+        pyrs = []
+        for f in (f0, f1):
+            pyr = ctx.pyramid(W, H, lv + 1)
+            pyr.build(f)
+            pyrs.append(pyr)
+        st = ctx.still(w, h, 16, 0.5, 4.0)
+        for i in range(3):
+            st.frame(pyrs[i % 2], lv)
+        ctx.sync()
+        ts = []
+        for i in range(nframes):
+            t = time.perf_counter()
+            st.frame(pyrs[i % 2], lv)
+            ctx.sync()
+            ts.append(time.perf_counter() - t)
+        ms = np.array(ts) * 1e3
+        m = st.motion()
+        planes = [oracle.pyramid(f, W, H, lv + 1)[0][lv] for f in (f0, f1)]
+        o = OracleStill(oracle, StillCfg(w, h, 16, 0.5, 4.0, 1))
+        o.decide()
+        o.update(planes[0])
+        o.finish()
+        t = time.perf_counter()
+        for i in range(cpu_frames):
+            o.vectors(planes[(i + 1) % 2], threads=16)
+            o.decide()
+            o.update(planes[(i + 1) % 2])
+            o.finish()
+        cpu_ms = (time.perf_counter() - t) * 1e3 / cpu_frames
+        o.free()
+        key = f"{w}x{h}"
+        out[key] = {"camera": f"{W}x{H} read at level {lv}", "cells": st.ncells, "frames": nframes,
+                    "gpu_ms_p50": round(float(np.percentile(ms, 50)), 4),
+                    "gpu_ms_p99": round(float(np.percentile(ms, 99)), 4),
+                    "cells_per_s": round(st.ncells / (float(np.percentile(ms, 50)) * 1e-3), 1),
+                    "tracked": m["tracked"], "moving": m["moving"], "diff_median": m["diff_median"],
+                    "cpu_ms_per_frame": round(cpu_ms, 3), "cpu_threads": 16, "cpu_kind": "port (oracle lane tier)"}
+        log(f"still {key}: {out[key]}")
+        st.destroy()
+        for pyr in pyrs:
+            pyr.destroy()
+    return out
+
+
 def cpu_baseline(f0s, f1s, corners, xs, ys, budget_s, gpu_out):
     """The oracle's reference-faithful tier (C, the reference's lane arithmetic) on this host, on the same workload:
     whole 1080p pairs (both pyramids, LK of all 16,200 points, the 64 x 64 template's exhaustive NCC map). All cores
@@ -427,6 +487,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20, help="graph replays of the uneven 64-pair batch (0: skip)")
     ap.add_argument("--serve-frames", type=int, default=300, help="frames per stream of the serving runs (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample (0: skip)")
+    ap.add_argument("--still-frames", type=int, default=60, help="timed frames of each still-cell camera (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one device-to-device copy of this many MiB first: a plain copy of known size that calibrates "
                          "the HBM byte counters of a profiled run (0: none)")
@@ -529,6 +590,7 @@ def main():
     log(f"plain: {plain:.2f} pairs/s")
     batch = batching(vtgpu, a.batch_steps) if a.batch_steps > 0 else {}
     serve = serving(vtgpu, a.serve_frames) if a.serve_frames > 0 else {}
+    still = still_bench(vtgpu, a.still_frames) if a.still_frames > 0 else {}
     cpu, err, prec = ({}, {}, {})
     if a.cpu_seconds > 0:
         cpu, err, prec = cpu_baseline(f0s, f1s, corners, xs, ys, a.cpu_seconds, gpu_out0)
@@ -555,6 +617,7 @@ def main():
         "batching": batch,
         "serving": serve,
         "precision": prec,
+        "still_cells": still,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
