@@ -104,6 +104,12 @@ struct vtg_motion { /* = the leading fields of struct vt_motion, through frames 
     uint64_t cycles_vectors, cycles_mean; /* not filled */
     uint64_t frames;
 };
+
+struct vtg_stereo_scan { /* = struct stereo_scan (src/kern/stereo.h:12-16) */
+    float disparity;   /* of the best peak, in pixels of the level; negative when no candidate fits */
+    float ncc;         /* the best peak's correlation */
+    float second;      /* the best correlation at least 3 candidates away from it */
+};
 #endif
 
 #endif

@@ -344,6 +344,15 @@ int vtg_still_mean(vtg_ctx *ctx, const vtg_still *s, const struct vt_plane *out)
 int vtg_still_motion(const vtg_still *s, struct vtg_motion *m);                    /* still_motion, still.c:280-293 */
 int vtg_still_hash(vtg_ctx *ctx, vtg_still *s, uint64_t h, uint64_t *out);         /* still_hash, still.c:272-278 */
 
+/* --- stereo row scan (src/kern/stereo.c:36-80 stereo_scan_vx, called per feature by the tracker's stereo step,
+ * track.c:1866-1895): a 16 x 11 zero-mean template of `level` of the left pyramid at each feature against integer
+ * disparities dmin .. dmax (at most 512) of the right pyramid's row. n features in one launch; xy (2 floats a feature)
+ * and out are device buffers. The _planes form takes the reference's host planes and host arrays. */
+int vtg_stereo_scan_many(vtg_ctx *ctx, const vtg_pyr *left, const vtg_pyr *right, int level, int n, const float *xy,
+                         int dmin, int dmax, struct vtg_stereo_scan *out);
+int vtg_stereo_scan_planes(vtg_ctx *ctx, const struct vt_plane *left, const struct vt_plane *right, int n,
+                           const float *x, const float *y, int dmin, int dmax, struct vtg_stereo_scan *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,8 @@ class StillCfg(C.Structure):
 STILL_CELL_DTYPE = np.dtype([("u", "<f4"), ("v", "<f4"), ("diff", "<f4"), ("state", "<u4"), ("sigma", "<f4"),
                              ("pad", "<f4")])
 
+STEREO_SCAN_DTYPE = np.dtype([("disparity", "<f4"), ("ncc", "<f4"), ("second", "<f4")])
+
 STILL_MOTION_FIELDS = ["cols", "rows", "cell_at", "tracked", "moving", "count_mean", "drift_u", "drift_v",
                        "drift_rms", "diff_median", "diff_limit", "pad", "cycles_vectors", "cycles_mean", "frames"]
 
@@ -152,6 +154,8 @@ def lib():
         "vtg_still_frame": [V, V, V, I], "vtg_still_cells": [V, V, V],
         "vtg_still_mean": [V, V, C.POINTER(Plane)], "vtg_still_motion": [V, C.POINTER(StillMotion)],
         "vtg_still_hash": [V, V, C.c_uint64, C.POINTER(C.c_uint64)],
+        "vtg_stereo_scan_many": [V, V, V, I, I, V, I, I, V],
+        "vtg_stereo_scan_planes": [V, C.POINTER(Plane), C.POINTER(Plane), I, V, V, I, I, V],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -182,7 +186,7 @@ EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", 
            "vtg_still_create", "vtg_still_destroy", "vtg_still_first", "vtg_still_size", "vtg_still_vectors",
            "vtg_still_vectors_plane", "vtg_still_decide", "vtg_still_update", "vtg_still_update_plane",
            "vtg_still_finish", "vtg_still_frame", "vtg_still_cells", "vtg_still_mean", "vtg_still_motion",
-           "vtg_still_hash"]
+           "vtg_still_hash", "vtg_stereo_scan_many", "vtg_stereo_scan_planes"]
 
 
 def synth():
@@ -334,6 +338,24 @@ class Context:
     # pyramids
     def pyramid(self, w, h, levels, storage=STORE_Q6):
         return Pyramid(self, w, h, levels, storage)
+
+    def stereo_scan_many(self, left, right, level, xy, dmin, dmax):
+        """The row scan of n features (xy: DevBuf of 2 n floats) at `level`; returns (n,) STEREO_SCAN_DTYPE."""
+        n = xy.nbytes // 8
+        out = self.alloc(max(12 * n, 16))
+        self.check(self.L.vtg_stereo_scan_many(self.h, left.h, right.h, level, n, xy.ptr, dmin, dmax, out.ptr),
+                   "stereo_scan_many")
+        res = out.get(np.zeros(n, STEREO_SCAN_DTYPE)).copy()
+        out.free()
+        return res
+
+    def stereo_scan_planes(self, left, right, xs, ys, dmin, dmax):
+        xs = np.ascontiguousarray(xs, np.float32)
+        ys = np.ascontiguousarray(ys, np.float32)
+        out = np.zeros(len(xs), STEREO_SCAN_DTYPE)
+        self.check(self.L.vtg_stereo_scan_planes(self.h, C.byref(left.c), C.byref(right.c), len(xs), xs.ctypes.data,
+                                                 ys.ctypes.data, dmin, dmax, out.ctypes.data), "stereo_scan_planes")
+        return out
 
     def still(self, w, h, avg=16, motion_px=0.5, motion_grey=4.0):
         return Still(self, StillCfg(w, h, avg, motion_px, motion_grey, 1))

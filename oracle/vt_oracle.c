@@ -1160,3 +1160,73 @@ uint64_t vo_still_hash(const struct vo_still *s, uint64_t h)
         h = vo_fnv(h, plane_row(&s->mean, y), (size_t)s->w * sizeof(float));
     return h;
 }
+
+/* ------------------------------------------------------------------ stereo row scan (src/kern/stereo.c:1-80)
+ * Pinned against the reference's own stereo_scan_vx from oracle/_ref (tests/test_oracle_stereo.py: every field bit
+ * for bit over random features, disparity ranges and edge cases), and by closed forms (a right image that is the left
+ * shifted by d: peak at d with correlation 1; a flat template: no correlation). */
+#define SC_W 16
+#define SC_H 11
+#define SC_MAX 512
+
+static long double sc_ncc(float st, float sj, float sjj, long double norm_t, long double n)
+{
+    const long double var = (long double)sjj - (long double)sj * (long double)sj / n;
+    if (var <= 0 || norm_t <= 0)
+        return 0;
+    return (long double)st / sqrtl(norm_t * var);
+}
+
+void vo_stereo_scan(const struct vt_plane *left, const struct vt_plane *right, float x, float y, int dmin, int dmax,
+                    struct vo_stereo_scan *out)
+{
+    const int x0 = (int)x - SC_W / 2, y0 = (int)y - SC_H / 2;
+    float t[SC_H][SC_W], sum[SC_W] = {0};
+    for (int r = 0; r < SC_H; r++)
+        for (int k = 0; k < SC_W; k++) {
+            t[r][k] = plane_row(left, y0 + r)[x0 + k];
+            sum[k] += t[r][k];
+        }
+    const long double n = (long double)(SC_W * SC_H);
+    const float mean = (float)((long double)lane_hsum(sum) / n);
+    float stt[SC_W] = {0};
+    for (int r = 0; r < SC_H; r++)
+        for (int k = 0; k < SC_W; k++) {
+            t[r][k] -= mean;
+            stt[k] = fmaf(t[r][k], t[r][k], stt[k]);
+        }
+    const long double norm_t = lane_hsum(stt);
+    long double curve[SC_MAX];
+    int count = 0;
+    for (int d = dmin; d <= dmax && count < SC_MAX; d++, count++) {
+        const int xr = x0 - d;
+        if (xr < 1 || xr + SC_W >= right->w - 1) {
+            curve[count] = -1;
+            continue;
+        }
+        float st[SC_W] = {0}, sj[SC_W] = {0}, sjj[SC_W] = {0};
+        for (int r = 0; r < SC_H; r++)
+            for (int k = 0; k < SC_W; k++) {
+                const float j = plane_row(right, y0 + r)[xr + k];
+                st[k] = fmaf(j, t[r][k], st[k]);
+                sj[k] += j;
+                sjj[k] = fmaf(j, j, sjj[k]);
+            }
+        curve[count] = sc_ncc(lane_hsum(st), lane_hsum(sj), lane_hsum(sjj), norm_t, n);
+    }
+    /* the first highest candidate, then the highest at least 3 candidates from it (stereo.c:17-34) */
+    int bi = -1;
+    long double bv = -2;
+    for (int i = 0; i < count; i++)
+        if (curve[i] > bv) {
+            bv = curve[i];
+            bi = i;
+        }
+    long double sv = -1;
+    for (int i = 0; i < count; i++)
+        if ((i < bi - 2 || i > bi + 2) && curve[i] > sv)
+            sv = curve[i];
+    out->disparity = bi < 0 ? -1.0f : (float)(dmin + bi);
+    out->ncc = bi < 0 ? 0.0f : (float)bv;
+    out->second = (float)sv;
+}

@@ -106,6 +106,12 @@ void vo_still_update(struct vo_still *s, const struct vt_plane *cur);
 void vo_still_finish(struct vo_still *s);
 uint64_t vo_still_hash(const struct vo_still *s, uint64_t h);
 
+/* --- stereo row scan (src/kern/stereo.h:9-20): a 16 x 11 zero-mean template of the left level against integer
+ * disparities dmin .. dmax of the right row --- */
+struct vo_stereo_scan { float disparity, ncc, second; }; /* = struct stereo_scan */
+void vo_stereo_scan(const struct vt_plane *left, const struct vt_plane *right, float x, float y, int dmin, int dmax,
+                    struct vo_stereo_scan *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -246,3 +246,18 @@ class OracleStill:
     def free(self):
         self.L.vo_still_free(self.p)
         self.p = None
+
+
+class StereoScan(C.Structure):
+    """struct stereo_scan (src/kern/stereo.h:12-16) = struct vo_stereo_scan = struct vtg_stereo_scan."""
+    _fields_ = [("disparity", C.c_float), ("ncc", C.c_float), ("second", C.c_float)]
+
+    def tup(self):
+        return (self.disparity, self.ncc, self.second)
+
+
+def bind_stereo(lib, name):
+    fn = getattr(lib, name)
+    fn.argtypes = [C.POINTER(Plane), C.POINTER(Plane), C.c_float, C.c_float, C.c_int, C.c_int,
+                   C.POINTER(StereoScan)]
+    return fn

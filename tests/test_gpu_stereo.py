@@ -1,0 +1,91 @@
+"""The stereo row scan on the GPU (vtg_stereo_scan_many / _planes, src/kern/stereo.c:36-80) against the oracle's
+restatement (itself pinned to the reference's stereo_scan_vx in test_oracle_stereo.py): for every feature the same
+disparity, and the correlation and second peak within 1e-5 (the three sums are the reference's bits; the correlation
+from them is double here, long double there). Features over the whole level, at its edges, at level 0 and 1; ranges
+past the 512-candidate cap, off the right image, and empty; Q6, F32 and host-plane inputs agree."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import vtgpu
+from gpu_util import say
+from oracle_ffi import StereoScan, bind_stereo
+from test_oracle_stereo import stereo_pair
+
+pytestmark = pytest.mark.gpu
+
+RANGES = [(0, 40), (-5, 30), (0, 700), (10, 10), (5, 4), (150, 260)]
+
+
+def _features(rng, w, h, n):
+    xs = rng.uniform(0.0, w - 1.0, n).astype(np.float32)
+    ys = rng.uniform(1.0, h - 2.0, n).astype(np.float32)
+    xs[:8] = [0.0, 3.5, 7.9, w - 1.0, w - 4.2, 20.0, 40.0, 60.0]   # template off the left and right edges
+    ys[:8] = [1.0, 2.5, 4.0, h - 2.0, h - 3.5, 1.0, h - 2.0, 6.0]
+    return xs, ys
+
+
+def _oracle_scans(oracle, L, R, xs, ys, dmin, dmax):
+    fn = bind_stereo(oracle.lib, "vo_stereo_scan")
+    out = np.zeros((len(xs), 3), np.float32)
+    for i in range(len(xs)):
+        s = StereoScan()
+        fn(C.byref(L.c), C.byref(R.c), float(xs[i]), float(ys[i]), dmin, dmax, C.byref(s))
+        out[i] = s.tup()
+    return out
+
+
+@pytest.mark.parametrize("lv", [0, 1])
+def test_stereo_scan_matches_oracle(ctx, oracle, lv):
+    W, H = 640, 400
+    lft, rgt = stereo_pair(oracle, W, H, 24, 6)
+    pl, pr = ctx.pyramid(W, H, lv + 1), ctx.pyramid(W, H, lv + 1)
+    pl.build(lft)
+    pr.build(rgt)
+    Lo, _ = oracle.pyramid(lft, W, H, lv + 1)
+    Ro, _ = oracle.pyramid(rgt, W, H, lv + 1)
+    L, R = Lo[lv], Ro[lv]
+    rng = np.random.default_rng(12 + lv)
+    xs, ys = _features(rng, L.w, L.h, 400)
+    xy = ctx.to_device(np.stack([xs, ys], 1).ravel())
+    agree = 0
+    for dmin, dmax in RANGES:
+        say("stereo scan", lv, dmin, dmax)
+        got = ctx.stereo_scan_many(pl, pr, lv, xy, dmin, dmax)
+        ref = _oracle_scans(oracle, L, R, xs, ys, dmin, dmax)
+        assert np.array_equal(got["disparity"], ref[:, 0]), (dmin, dmax)
+        assert np.abs(got["ncc"] - ref[:, 1]).max() <= 1e-5
+        assert np.abs(got["second"] - ref[:, 2]).max() <= 1e-5
+        agree += int((got["ncc"] == ref[:, 1]).sum())
+        if (dmin, dmax) == (0, 40):
+            peak = got["ncc"] > 0.9
+            assert peak.mean() > 0.5 and np.median(got["disparity"][peak]) == round(24 / 2 ** lv)
+    say(f"stereo scan lv{lv}: {len(xs)} features x {len(RANGES)} ranges, correlation bit-equal on {agree}")
+    xy.free()
+    pl.destroy()
+    pr.destroy()
+
+
+def test_stereo_scan_storages_and_planes(ctx, oracle):
+    """F32 pyramids and the reference's host planes give the Q6 pyramid's results bit for bit."""
+    W, H = 320, 240
+    lft, rgt = stereo_pair(oracle, W, H, 9, 7)
+    q = [ctx.pyramid(W, H, 1) for _ in range(2)]
+    f = [ctx.pyramid(W, H, 1, vtgpu.STORE_F32) for _ in range(2)]
+    for p, img in zip(q + f, [lft, rgt, lft, rgt]):
+        p.build(img)
+    rng = np.random.default_rng(3)
+    xs, ys = _features(rng, W, H, 200)
+    xy = ctx.to_device(np.stack([xs, ys], 1).ravel())
+    a = ctx.stereo_scan_many(q[0], q[1], 0, xy, -3, 30)
+    b = ctx.stereo_scan_many(f[0], f[1], 0, xy, -3, 30)
+    Lo, _ = oracle.pyramid(lft, W, H, 1)
+    Ro, _ = oracle.pyramid(rgt, W, H, 1)
+    c = ctx.stereo_scan_planes(Lo[0], Ro[0], xs, ys, -3, 30)
+    assert a.tobytes() == b.tobytes() == c.tobytes()
+    with pytest.raises(vtgpu.VtgError):
+        ctx.stereo_scan_many(q[0], f[1], 0, xy, 0, 10)   # storages differ
+    xy.free()
+    for p in q + f:
+        p.destroy()
