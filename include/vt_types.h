@@ -105,6 +105,33 @@ struct vtg_motion { /* = the leading fields of struct vt_motion, through frames 
     uint64_t frames;
 };
 
+#define VTG_FF_STEREO 1u    /* a feature's flags (VT_FF_*, src/common/vt_proto.h:324-327): xr, yr, disparity hold a match */
+#define VTG_FF_3D 2u        /* its 3D point passed every check */
+#define VTG_FF_AMBIGUOUS 4u /* the scan's second peak was too close to its first */
+#define VTG_FF_NEW 8u       /* detected at this frame (kept through the match) */
+
+struct vtg_stereo_geom { long double fx, fy, cx, cy, baseline; }; /* = struct stereo_geom (stereo.h:22) */
+
+struct vtg_stereo_cfg { /* the tracker's stereo settings (vt_proto.h:97-99, track.c:488-490) */
+    int stereo_level;          /* the scan's level: the tracker takes the lowest with max_disp / 2^level <= 64 */
+    float min_disp, max_disp;  /* pixels of level 0 */
+    float min_ncc, ambiguity;  /* the scan's peak must reach min_ncc; its second peak must not exceed ambiguity x peak */
+    float fb_max;              /* the flowback limit, pixels (0: no flowback) */
+    float max_reproj;          /* pixels */
+    struct lk_cfg lk;
+    struct vtg_stereo_geom geom;
+};
+
+struct vtg_stereo_feature { /* one feature of the tracker's stereo step (track.c:1859-1914) */
+    float x, y;         /* in: the feature in the left image, pixels of level 0 */
+    float disparity;    /* in: a prior (> 0) or none (<= 0); out: x - xr, or -1 when there is no match */
+    uint32_t flags;     /* in: VTG_FF_NEW is kept; out: VTG_FF_STEREO, VTG_FF_3D, VTG_FF_AMBIGUOUS */
+    float xr, yr;       /* out: the match in the right image (0 when none) */
+    float z;            /* out: the 3D point's depth (0 unless VTG_FF_3D) */
+    float pad;
+    long double p[3];   /* out: the triangulated point, written when triangulation succeeds */
+};
+
 struct vtg_stereo_scan { /* = struct stereo_scan (src/kern/stereo.h:12-16) */
     float disparity;   /* of the best peak, in pixels of the level; negative when no candidate fits */
     float ncc;         /* the best peak's correlation */

@@ -352,6 +352,12 @@ int vtg_stereo_scan_many(vtg_ctx *ctx, const vtg_pyr *left, const vtg_pyr *right
                          int dmin, int dmax, struct vtg_stereo_scan *out);
 int vtg_stereo_scan_planes(vtg_ctx *ctx, const struct vt_plane *left, const struct vt_plane *right, int n,
                            const float *x, const float *y, int dmin, int dmax, struct vtg_stereo_scan *out);
+/* The tracker's stereo step (stereo_one, track.c:1859-1914) for n features (a host array, read and written): the scan
+ * at cfg->stereo_level for features without a prior disparity, LK from the left to the right pyramid with the disparity
+ * as x guess (flowback from level 0), the match, and the 3D point where it triangulates and reprojects within
+ * max_reproj. Waits for its results. */
+int vtg_stereo_match(vtg_ctx *ctx, const vtg_pyr *left, const vtg_pyr *right, const struct vtg_stereo_cfg *cfg, int n,
+                     struct vtg_stereo_feature *feat);
 
 #ifdef __cplusplus
 }

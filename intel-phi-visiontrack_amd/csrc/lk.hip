@@ -60,6 +60,7 @@ struct LkArgs {
     int margin, lds_floats;
     const float *xy, *guess;
     float fb_max;
+    int fb_top;                 // the flowback's top level (-1: the pair's own)
     lk_out *out;
 };
 
@@ -710,7 +711,8 @@ __global__ __launch_bounds__(512, (!STAGED && MAXR <= 7) ? 4 : 1) void lk_kernel
     if (a.fb_max > 0.0f && o.status == 0) {
         // back from J to I, the template rebuilt from J (lk_driver.inc:111-121)
         Track b;
-        track<MAXR, T0, T1, SOLVE, STAGED>(J, I, top, a.cfg, g, lane, sl, a.margin, o.x, o.y,
+        const int fb_top = a.fb_top >= 0 ? a.fb_top : top;
+        track<MAXR, T0, T1, SOLVE, STAGED>(J, I, fb_top, a.cfg, g, lane, sl, a.margin, o.x, o.y,
                                            (float)((double)x - (double)o.x), (float)((double)y - (double)o.y), b);
         const double ex = (double)b.x - (double)x, ey = (double)b.y - (double)y;
         if (b.status || ex * ex + ey * ey > (double)a.fb_max * (double)a.fb_max)
@@ -1072,7 +1074,7 @@ int check_pair(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top)
 // One launch for pairs sel[0 .. m) (m <= BMAX; indices into the caller's arrays) under schedule s.
 static int launch_group(vtg_ctx *c, const std::vector<int> &sel, const vtg_pyr *const *I, const vtg_pyr *const *J,
                         const int *top, const int *off, const lk_cfg *cfg, const float *xy, const float *guess,
-                        float fb_max, int solve_mode, lk_out *out, LkSched s)
+                        float fb_max, int solve_mode, lk_out *out, LkSched s, int fb_top)
 {
     LkArgs a;
     a.npairs = (int)sel.size();
@@ -1099,6 +1101,7 @@ static int launch_group(vtg_ctx *c, const std::vector<int> &sel, const vtg_pyr *
     a.xy = xy;
     a.guess = guess;
     a.fb_max = fb_max;
+    a.fb_top = fb_top;
     a.out = out;
     a.waves = s.waves;
     const int groups = 64 / cfg->w, chunk = (cfg->h + groups - 1) / groups;
@@ -1140,7 +1143,7 @@ LkSched tuned_lk(const vtg_ctx *c, const vtg_pyr *I, int top, const lk_cfg *cfg)
 // Pairs [0, np) with points off[k] .. off[k+1] - 1 of xy: grouped by schedule, one launch per group of up to BMAX.
 int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
                    const int *off, const lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
-                   int solve_mode, lk_out *out, const LkSched *sched)
+                   int solve_mode, lk_out *out, const LkSched *sched, int fb_top)
 {
     std::vector<LkSched> sc(np);
     for (int k = 0; k < np; k++)
@@ -1157,7 +1160,7 @@ int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *c
             }
         for (size_t b = 0; b < sel.size(); b += BMAX) {
             const std::vector<int> part(sel.begin() + b, sel.begin() + std::min(sel.size(), b + BMAX));
-            int rc = launch_group(c, part, I, J, top, off, cfg, xy, guess, fb_max, solve_mode, out, sc[k]);
+            int rc = launch_group(c, part, I, J, top, off, cfg, xy, guess, fb_max, solve_mode, out, sc[k], fb_top);
             if (rc)
                 return rc;
         }
@@ -1258,6 +1261,7 @@ int vtg_pyr_lk_fused(vtg_ctx *c, int npairs, vtg_pyr *const *I, vtg_pyr *const *
     a.lk.xy = xy;
     a.lk.guess = guess;
     a.lk.fb_max = fb_max;
+    a.lk.fb_top = -1;
     a.lk.out = out;
     a.lk.waves = 4;
     a.lk.per_xcd = 0;

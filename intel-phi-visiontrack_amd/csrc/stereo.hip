@@ -10,6 +10,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "ldnorm.hpp"
@@ -167,6 +168,8 @@ template <typename T> void launch_t(vtg_ctx *c, const ScanArgs &a)
     hipLaunchKernelGGL(stereo_scan<T>, dim3((a.n + 3) / 4), dim3(256), 0, c->stream, a);
 }
 
+size_t up(size_t v) { return (v + 255) / 256 * 256; }
+
 int launch_scan(vtg_ctx *c, const vtg_pyr *left, const vtg_pyr *right, int level, int n, const float *xy, int dmin,
                 int dmax, vtg_stereo_scan *out)
 {
@@ -199,6 +202,29 @@ int launch_scan(vtg_ctx *c, const vtg_pyr *left, const vtg_pyr *right, int level
     VTG_HIP(c, hipGetLastError());
     timer_end(c, "stereo_scan", ev);
     return VTG_OK;
+}
+
+// stereo.c:124-141, in long double on this thread as the reference computes them on one.
+int triangulate(const vtg_stereo_geom &g, long double xl, long double yl, long double xr, long double yr,
+                long double *p)
+{
+    const long double d = xl - xr;
+    if (d <= 0.0L)
+        return 0;
+    const long double z = g.fx * g.baseline / d;
+    p[0] = (xl - g.cx) * z / g.fx;
+    p[1] = (0.5L * (yl + yr) - g.cy) * z / g.fy;
+    p[2] = z;
+    return 1;
+}
+
+long double reprojection(const vtg_stereo_geom &g, const long double *p, long double xl, long double yl,
+                         long double xr, long double yr)
+{
+    const long double ul = g.fx * p[0] / p[2] + g.cx, vl = g.fy * p[1] / p[2] + g.cy;
+    const long double ur = g.fx * (p[0] - g.baseline) / p[2] + g.cx;
+    const long double el = ul - xl, fl = vl - yl, er = ur - xr, fr = vl - yr;
+    return sqrtl((el * el + fl * fl + er * er + fr * fr) * 0.5L);
 }
 
 }  // namespace
@@ -262,6 +288,129 @@ int vtg_stereo_scan_planes(vtg_ctx *c, const struct vt_plane *left, const struct
     if (pr)
         vtg_pyr_destroy(c, pr);
     return rc;
+}
+
+// The tracker's stereo step (stereo_one, track.c:1859-1914) for n features: the scan on the GPU for features without a
+// prior, LK on the GPU for the rest (the disparity as x guess; from the stereo level after a scan, level 0 with a
+// prior; the flowback from level 0), then each feature's decision and 3D point here, in long double.
+int vtg_stereo_match(vtg_ctx *c, const vtg_pyr *left, const vtg_pyr *right, const struct vtg_stereo_cfg *cfg, int n,
+                     struct vtg_stereo_feature *f)
+{
+    if (!c || !left || !right || !cfg || n < 0 || (n > 0 && !f))
+        return VTG_EINVAL;
+    const int sl = cfg->stereo_level;
+    const lk_cfg &lk = cfg->lk;
+    if (sl < 0 || sl >= left->levels || sl >= right->levels || sl > 30 || lk.w < 5 || lk.w > LK_MAX_W || lk.h < 5 ||
+        lk.h > LK_MAX_H || lk.max_iter < 0)
+        return fail(c, VTG_EINVAL, "vtg_stereo_match: stereo level %d of %d / %d levels, window %dx%d", sl,
+                    left->levels, right->levels, lk.w, lk.h);
+    if (left->storage != right->storage)
+        return fail(c, VTG_EINVAL, "vtg_stereo_match: pyramids of different storage");
+    for (int l = 0; l <= sl; l++)
+        if (left->lv[l].w != right->lv[l].w || left->lv[l].h != right->lv[l].h)
+            return fail(c, VTG_EINVAL, "vtg_stereo_match: level %d sizes differ", l);
+    if (n == 0)
+        return VTG_OK;
+    // device scratch: scan points, scans, LK points, guesses, LK records
+    const size_t o_sc = up(sizeof(float) * 2 * n), o_lxy = o_sc + up(sizeof(vtg_stereo_scan) * n),
+                 o_g = o_lxy + up(sizeof(float) * 2 * n), o_out = o_g + up(sizeof(float) * 2 * n),
+                 bytes = o_out + up(sizeof(lk_out) * n);
+    void *buf = nullptr;
+    int rc = scratch(c, bytes, &buf);
+    if (rc)
+        return rc;
+    char *d = (char *)buf;
+    const long double s = 1.0L / (long double)(1 << sl);
+    std::vector<int> scanned;
+    std::vector<float> sxy;
+    for (int i = 0; i < n; i++) {
+        f[i].xr = f[i].yr = 0.0f;
+        f[i].z = 0.0f;
+        if (f[i].disparity <= 0.0f) {
+            scanned.push_back(i);
+            sxy.push_back((float)((long double)f[i].x * s));
+            sxy.push_back((float)((long double)f[i].y * s));
+        }
+    }
+    std::vector<vtg_stereo_scan> sc(scanned.size());
+    if (!scanned.empty()) {
+        const int dmin = (int)((long double)cfg->min_disp * s), dmax = (int)((long double)cfg->max_disp * s);
+        VTG_HIP(c, hipMemcpyAsync(d, sxy.data(), sizeof(float) * sxy.size(), hipMemcpyHostToDevice, c->stream));
+        rc = launch_scan(c, left, right, sl, (int)scanned.size(), (const float *)d, dmin, dmax,
+                         (vtg_stereo_scan *)(d + o_sc));
+        if (rc)
+            return rc;
+        VTG_HIP(c, hipMemcpyAsync(sc.data(), d + o_sc, sizeof(vtg_stereo_scan) * sc.size(), hipMemcpyDeviceToHost,
+                                  c->stream));
+        VTG_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    // the points LK takes: those with a prior (from level 0), then the scanned ones that passed (from the stereo level)
+    std::vector<int> idx0, idx1;
+    std::vector<float> guess(n, 0.0f);
+    for (int i = 0; i < n; i++)
+        if (f[i].disparity > 0.0f) {
+            guess[i] = f[i].disparity;
+            idx0.push_back(i);
+        }
+    for (size_t k = 0; k < scanned.size(); k++) {
+        vtg_stereo_feature &q = f[scanned[k]];
+        const uint32_t fl = q.flags & VTG_FF_NEW;
+        if (sc[k].disparity < 0.0f || sc[k].ncc < cfg->min_ncc) {
+            q.flags = fl;
+            q.disparity = -1.0f;
+        } else if ((long double)sc[k].second > (long double)sc[k].ncc * (long double)cfg->ambiguity) {
+            q.flags = fl | VTG_FF_AMBIGUOUS;
+            q.disparity = -1.0f;
+        } else {
+            guess[scanned[k]] = (float)((long double)sc[k].disparity * (long double)(1 << sl));
+            idx1.push_back(scanned[k]);
+        }
+    }
+    const int m0 = (int)idx0.size(), m1 = (int)idx1.size(), m = m0 + m1;
+    if (m > 0) {
+        std::vector<float> lxy(2 * (size_t)m), lg(2 * (size_t)m, 0.0f);
+        for (int k = 0; k < m; k++) {
+            const int i = k < m0 ? idx0[k] : idx1[k - m0];
+            lxy[2 * k] = f[i].x;
+            lxy[2 * k + 1] = f[i].y;
+            lg[2 * k] = (float)(-(long double)guess[i]);
+        }
+        VTG_HIP(c, hipMemcpyAsync(d + o_lxy, lxy.data(), sizeof(float) * lxy.size(), hipMemcpyHostToDevice,
+                                  c->stream));
+        VTG_HIP(c, hipMemcpyAsync(d + o_g, lg.data(), sizeof(float) * lg.size(), hipMemcpyHostToDevice, c->stream));
+        const vtg_pyr *I[2] = {left, left}, *J[2] = {right, right};
+        const int top[2] = {0, sl}, off[3] = {0, m0, m};
+        rc = launch_lk_many(c, 2, I, J, top, off, &lk, (const float *)(d + o_lxy), (const float *)(d + o_g),
+                            cfg->fb_max, VTG_SOLVE_F64, (lk_out *)(d + o_out), nullptr, 0);
+        if (rc)
+            return rc;
+        std::vector<lk_out> out(m);
+        VTG_HIP(c, hipMemcpyAsync(out.data(), d + o_out, sizeof(lk_out) * m, hipMemcpyDeviceToHost, c->stream));
+        VTG_HIP(c, hipStreamSynchronize(c->stream));
+        for (int k = 0; k < m; k++) {
+            vtg_stereo_feature &q = f[k < m0 ? idx0[k] : idx1[k - m0]];
+            uint32_t fl = q.flags & VTG_FF_NEW;
+            const lk_out &o = out[k];
+            if (o.status) {
+                q.flags = fl;
+                q.disparity = -1.0f;
+                continue;
+            }
+            q.xr = o.x;
+            q.yr = o.y;
+            const long double disparity = (long double)q.x - (long double)o.x;
+            q.disparity = (float)disparity;
+            fl |= VTG_FF_STEREO;
+            if (disparity >= (long double)cfg->min_disp && disparity <= (long double)cfg->max_disp &&
+                triangulate(cfg->geom, q.x, q.y, o.x, o.y, q.p) &&
+                reprojection(cfg->geom, q.p, q.x, q.y, o.x, o.y) <= (long double)cfg->max_reproj) {
+                fl |= VTG_FF_3D;
+                q.z = (float)q.p[2];
+            }
+            q.flags = fl;
+        }
+    }
+    return VTG_OK;
 }
 
 }  // extern "C"

@@ -130,10 +130,11 @@ int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *
 // Level 0 of pyramids of storage Q6W or F32 from device u8 frames, filtered as `flags` asks (filters.hip).
 int launch_level0_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *frames, const int *pitch,
                        int flags);
-// sched: one schedule for every pair, or null for each pair's size-class pick (the context default otherwise)
+// sched: one schedule for every pair, or null for each pair's size-class pick (the context default otherwise);
+// fb_top: the flowback's top level (-1: each pair's own top)
 int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
                    const int *off, const lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
-                   int solve_mode, lk_out *out, const LkSched *sched);
+                   int solve_mode, lk_out *out, const LkSched *sched, int fb_top = -1);
 
 // Size classes of the autotuner and the picks kept for them.
 uint64_t pyr_class(const vtg_pyr *p);

@@ -59,6 +59,27 @@ class StillCfg(C.Structure):
 STILL_CELL_DTYPE = np.dtype([("u", "<f4"), ("v", "<f4"), ("diff", "<f4"), ("state", "<u4"), ("sigma", "<f4"),
                              ("pad", "<f4")])
 
+class StereoGeom(C.Structure):
+    _fields_ = [(n, C.c_longdouble) for n in ("fx", "fy", "cx", "cy", "baseline")]
+
+
+class StereoCfg(C.Structure):
+    """struct vtg_stereo_cfg (the tracker's stereo settings)"""
+    _fields_ = [("stereo_level", C.c_int), ("min_disp", C.c_float), ("max_disp", C.c_float),
+                ("min_ncc", C.c_float), ("ambiguity", C.c_float), ("fb_max", C.c_float),
+                ("max_reproj", C.c_float), ("lk", LkCfg), ("geom", StereoGeom)]
+
+
+class StereoFeature(C.Structure):
+    """struct vtg_stereo_feature (one feature of the stereo step, in and out)"""
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("disparity", C.c_float), ("flags", C.c_uint32),
+                ("xr", C.c_float), ("yr", C.c_float), ("z", C.c_float), ("pad", C.c_float),
+                ("p", C.c_longdouble * 3)]
+
+    def key(self):
+        return (self.x, self.y, self.disparity, self.flags, self.xr, self.yr, self.z)
+
+
 STEREO_SCAN_DTYPE = np.dtype([("disparity", "<f4"), ("ncc", "<f4"), ("second", "<f4")])
 
 STILL_MOTION_FIELDS = ["cols", "rows", "cell_at", "tracked", "moving", "count_mean", "drift_u", "drift_v",
@@ -156,6 +177,7 @@ def lib():
         "vtg_still_hash": [V, V, C.c_uint64, C.POINTER(C.c_uint64)],
         "vtg_stereo_scan_many": [V, V, V, I, I, V, I, I, V],
         "vtg_stereo_scan_planes": [V, C.POINTER(Plane), C.POINTER(Plane), I, V, V, I, I, V],
+        "vtg_stereo_match": [V, V, V, C.POINTER(StereoCfg), I, V],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -186,7 +208,8 @@ EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", 
            "vtg_still_create", "vtg_still_destroy", "vtg_still_first", "vtg_still_size", "vtg_still_vectors",
            "vtg_still_vectors_plane", "vtg_still_decide", "vtg_still_update", "vtg_still_update_plane",
            "vtg_still_finish", "vtg_still_frame", "vtg_still_cells", "vtg_still_mean", "vtg_still_motion",
-           "vtg_still_hash", "vtg_stereo_scan_many", "vtg_stereo_scan_planes"]
+           "vtg_still_hash", "vtg_stereo_scan_many", "vtg_stereo_scan_planes",
+           "vtg_stereo_match"]
 
 
 def synth():
@@ -356,6 +379,12 @@ class Context:
         self.check(self.L.vtg_stereo_scan_planes(self.h, C.byref(left.c), C.byref(right.c), len(xs), xs.ctypes.data,
                                                  ys.ctypes.data, dmin, dmax, out.ctypes.data), "stereo_scan_planes")
         return out
+
+    def stereo_match(self, left, right, cfg, feats):
+        """The tracker's stereo step over `feats` (a ctypes array of StereoFeature, read and written)."""
+        self.check(self.L.vtg_stereo_match(self.h, left.h, right.h, C.byref(cfg), len(feats), C.addressof(feats)),
+                   "stereo_match")
+        return feats
 
     def still(self, w, h, avg=16, motion_px=0.5, motion_grey=4.0):
         return Still(self, StillCfg(w, h, avg, motion_px, motion_grey, 1))

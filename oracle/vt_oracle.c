@@ -1230,3 +1230,89 @@ void vo_stereo_scan(const struct vt_plane *left, const struct vt_plane *right, f
     out->ncc = bi < 0 ? 0.0f : (float)bv;
     out->second = (float)sv;
 }
+
+/* ------------------------------------------------------------------ stereo geometry and match
+ * (src/kern/stereo.c:124-141, src/kern/track.c:1859-1914). The geometry is pinned bit for bit against the reference's
+ * own stereo_triangulate / stereo_reprojection from oracle/_ref; the match, against the reference's own scan, LK and
+ * geometry composed in the tracker's order (tests/test_oracle_stereo.py), and by a closed form (a shifted copy: every
+ * textured feature matched at the shift, its depth f B / d). */
+int vo_stereo_triangulate(const struct vtg_stereo_geom *g, long double xl, long double yl, long double xr,
+                          long double yr, long double *p)
+{
+    const long double d = xl - xr;
+    if (d <= 0.0L)
+        return 0;
+    const long double z = g->fx * g->baseline / d;
+    p[0] = (xl - g->cx) * z / g->fx;
+    p[1] = (0.5L * (yl + yr) - g->cy) * z / g->fy;
+    p[2] = z;
+    return 1;
+}
+
+long double vo_stereo_reprojection(const struct vtg_stereo_geom *g, const long double *p, long double xl,
+                                   long double yl, long double xr, long double yr)
+{
+    const long double ul = g->fx * p[0] / p[2] + g->cx, vl = g->fy * p[1] / p[2] + g->cy;
+    const long double ur = g->fx * (p[0] - g->baseline) / p[2] + g->cx;
+    const long double el = ul - xl, fl = vl - yl, er = ur - xr, fr = vl - yr;
+    return sqrtl((el * el + fl * fl + er * er + fr * fr) * 0.5L);
+}
+
+/* The tracker's stereo step for n features (stereo_one, track.c:1859-1914): a feature without a prior disparity is
+ * scanned at the stereo level (rejected below min_ncc, or marked ambiguous when the second peak is too close); LK (lane
+ * tier) from the left to the right pyramid with the disparity as its x guess, from the stereo level (or level 0 with a
+ * prior), the flowback from level 0; the match, its disparity, and the 3D point where the disparity is in range and it
+ * reprojects within max_reproj. */
+void vo_stereo_match(const struct vt_plane *L, const struct vt_plane *R, const struct vtg_stereo_cfg *c, int n,
+                     struct vtg_stereo_feature *f)
+{
+    for (int i = 0; i < n; i++) {
+        struct vtg_stereo_feature *q = &f[i];
+        uint32_t fl = q->flags & VTG_FF_NEW;
+        q->xr = q->yr = 0.0f;
+        q->z = 0.0f;
+        float guess = q->disparity;
+        int top = 0;
+        if (guess <= 0.0f) {
+            const int sl = c->stereo_level;
+            const long double s = 1.0L / (long double)(1 << sl);
+            const int dmin = (int)((long double)c->min_disp * s), dmax = (int)((long double)c->max_disp * s);
+            struct vo_stereo_scan sc;
+            vo_stereo_scan(&L[sl], &R[sl], (float)((long double)q->x * s), (float)((long double)q->y * s), dmin, dmax,
+                           &sc);
+            if (sc.disparity < 0.0f || sc.ncc < c->min_ncc) {
+                q->flags = fl;
+                q->disparity = -1.0f;
+                continue;
+            }
+            if ((long double)sc.second > (long double)sc.ncc * (long double)c->ambiguity) {
+                q->flags = fl | VTG_FF_AMBIGUOUS;
+                q->disparity = -1.0f;
+                continue;
+            }
+            guess = (float)((long double)sc.disparity * (long double)(1 << sl));
+            top = sl;
+        }
+        struct lk_out m;
+        vo_lk_track(VO_LANE, L, R, top, &c->lk, q->x, q->y, (float)(-(long double)guess), 0.0f, &m);
+        if (c->fb_max > 0.0f)
+            vo_lk_flowback(VO_LANE, L, R, 0, &c->lk, q->x, q->y, c->fb_max, &m);
+        if (m.status) {
+            q->flags = fl;
+            q->disparity = -1.0f;
+            continue;
+        }
+        q->xr = m.x;
+        q->yr = m.y;
+        const long double disparity = (long double)q->x - (long double)m.x;
+        q->disparity = (float)disparity;
+        fl |= VTG_FF_STEREO;
+        if (disparity >= (long double)c->min_disp && disparity <= (long double)c->max_disp &&
+            vo_stereo_triangulate(&c->geom, q->x, q->y, m.x, m.y, q->p) &&
+            vo_stereo_reprojection(&c->geom, q->p, q->x, q->y, m.x, m.y) <= (long double)c->max_reproj) {
+            fl |= VTG_FF_3D;
+            q->z = (float)q->p[2];
+        }
+        q->flags = fl;
+    }
+}

@@ -112,6 +112,14 @@ struct vo_stereo_scan { float disparity, ncc, second; }; /* = struct stereo_scan
 void vo_stereo_scan(const struct vt_plane *left, const struct vt_plane *right, float x, float y, int dmin, int dmax,
                     struct vo_stereo_scan *out);
 
+int vo_stereo_triangulate(const struct vtg_stereo_geom *g, long double xl, long double yl, long double xr,
+                          long double yr, long double *p);
+long double vo_stereo_reprojection(const struct vtg_stereo_geom *g, const long double *p, long double xl,
+                                   long double yl, long double xr, long double yr);
+/* the tracker's stereo step over n features; L, R: the two pyramids' levels 0 .. max(stereo_level, 0) */
+void vo_stereo_match(const struct vt_plane *L, const struct vt_plane *R, const struct vtg_stereo_cfg *c, int n,
+                     struct vtg_stereo_feature *f);
+
 #ifdef __cplusplus
 }
 #endif

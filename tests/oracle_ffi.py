@@ -261,3 +261,12 @@ def bind_stereo(lib, name):
     fn.argtypes = [C.POINTER(Plane), C.POINTER(Plane), C.c_float, C.c_float, C.c_int, C.c_int,
                    C.POINTER(StereoScan)]
     return fn
+
+
+from vtgpu import StereoCfg, StereoFeature, StereoGeom  # noqa: E402  (one ctypes class per C struct)
+
+
+def stereo_cfg(w, h, level=1, min_disp=2.0, max_disp=100.0, min_ncc=0.8, ambiguity=0.9, fb_max=0.5, max_reproj=1.0):
+    lk = LkCfg(15, 15, 10, 0.01, 1e-4, 40.0, 0)
+    return StereoCfg(level, min_disp, max_disp, min_ncc, ambiguity, fb_max, max_reproj, lk,
+                     StereoGeom(500.0, 500.0, w / 2.0, h / 2.0, 0.12))

@@ -89,3 +89,47 @@ def test_stereo_scan_storages_and_planes(ctx, oracle):
     xy.free()
     for p in q + f:
         p.destroy()
+
+
+@pytest.mark.parametrize("level,fb", [(1, 0.5), (0, 0.0)])
+def test_stereo_match_matches_oracle(ctx, oracle, level, fb):
+    """vtg_stereo_match against vo_stereo_match (pinned to the reference's functions in the tracker's order): the same
+    flags on every feature (a prior on a quarter of them, NEW kept), the match within 0.01 px, depth within 1e-3
+    relative; scan rejections and ambiguity decisions identical."""
+    from oracle_ffi import Plane, stereo_cfg
+    from test_oracle_stereo import _match_inputs
+    W, H, d0 = 640, 400, 24
+    lft, rgt, feats = _match_inputs(oracle, W, H, d0, 23, 300)
+    cfg = stereo_cfg(W, H, level=level, max_disp=60.0 if level == 0 else 100.0, fb_max=fb)
+    levels = level + 1
+    pl, pr = ctx.pyramid(W, H, levels), ctx.pyramid(W, H, levels)
+    pl.build(lft)
+    pr.build(rgt)
+    _, L = oracle.pyramid(lft, W, H, levels)
+    _, R = oracle.pyramid(rgt, W, H, levels)
+    mine = type(feats).from_buffer_copy(feats)
+    theirs = type(feats).from_buffer_copy(feats)
+    say("stereo match", level, fb)
+    ctx.stereo_match(pl, pr, cfg, mine)
+    oracle.lib.vo_stereo_match.argtypes = [C.POINTER(Plane), C.POINTER(Plane), C.POINTER(type(cfg)), C.c_int,
+                                           C.c_void_p]
+    oracle.lib.vo_stereo_match(L, R, C.byref(cfg), len(theirs), C.addressof(theirs))
+    same = [mine[i].flags == theirs[i].flags for i in range(len(feats))]
+    assert np.mean(same) >= 0.99, np.mean(same)
+    worst, n3 = 0.0, 0
+    for i in range(len(feats)):
+        a, b = mine[i], theirs[i]
+        if not same[i]:
+            continue
+        if a.flags & 1:
+            worst = max(worst, abs(a.xr - b.xr), abs(a.yr - b.yr), abs(a.disparity - b.disparity))
+        else:
+            assert a.disparity == b.disparity == -1.0 and a.xr == b.xr == 0.0
+        if a.flags & 2:
+            n3 += 1
+            assert abs(a.z - b.z) <= 1e-3 * abs(b.z)
+    assert worst <= 0.01 and n3 > 150
+    say(f"stereo match level {level}: {len(feats)} features, flags equal on {np.mean(same):.4f}, 3D {n3}, "
+        f"max |d| {worst:.2e} px")
+    pl.destroy()
+    pr.destroy()
