@@ -389,6 +389,66 @@ def still_bench(vtgpu, nframes, cpu_frames=4):
     return out
 
 
+def stereo_bench(vtgpu, calls, nfeat=512, d0=40):
+    """The tracker's stereo step at 1080p (stereo.c, track.c:1859-1914) with its dataset settings (dataset.c:154-157:
+    disparities 2 .. w/8, min_ncc 0.6, ambiguity 0.85; the scan at level 2, the lowest with w/8 / 2^level <= 64) on a
+    rectified pair (the right frame is the left moved d0 px, fresh noise): VT_MAX_FEATURES = 512 features, first all
+    without a prior (scan + LK), then all with one (LK only). ms per call p50 / p99 (features in, results on the host),
+    and the oracle's same step on one thread."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from oracle_ffi import Oracle, Plane, stereo_cfg
+    f0, _, _ = vtgpu.frame_pair(W, H, 91, max_shift=0.0, max_rot=0.0)
+    rng = np.random.default_rng(5)
+    right = np.clip(np.roll(f0, -d0, axis=1).astype(np.int32) + rng.normal(0, 1.5, f0.shape).round().astype(np.int32),
+                    0, 255).astype(np.uint8)
+    ctx = vtgpu.Context(0)
+    levels = 3
+    pl, pr = ctx.pyramid(W, H, levels), ctx.pyramid(W, H, levels)
+    pl.build(f0)
+    pr.build(right)
+    cfg = stereo_cfg(W, H, level=2, min_disp=2.0, max_disp=W / 8.0, min_ncc=0.6, ambiguity=0.85, fb_max=0.5,
+                     max_reproj=1.0)
+    base = (vtgpu.StereoFeature * nfeat)()
+    for i in range(nfeat):
+        base[i].x = float(rng.uniform(W / 8.0 + 20, W - 20))
+        base[i].y = float(rng.uniform(20, H - 20))
+    out = {"pair": f"{W}x{H}, right = left moved {d0} px", "features": nfeat, "stereo_level": 2,
+           "disparities": [2, W // 8]}
+    for label, prior in (("scan_and_lk", 0.0), ("lk_with_prior", float(d0) + 0.3)):
+        for i in range(nfeat):
+            base[i].disparity = prior
+        ts = []
+        for k in range(calls + 3):
+            f = type(base).from_buffer_copy(base)
+            t = time.perf_counter()
+            ctx.stereo_match(pl, pr, cfg, f)
+            if k >= 3:
+                ts.append(time.perf_counter() - t)
+        ms = np.array(ts) * 1e3
+        fl = np.array([q.flags for q in f])
+        out[label] = {"gpu_ms_p50": round(float(np.percentile(ms, 50)), 4),
+                      "gpu_ms_p99": round(float(np.percentile(ms, 99)), 4),
+                      "features_per_s": round(nfeat / (float(np.percentile(ms, 50)) * 1e-3), 1),
+                      "matched": int(((fl & 1) > 0).sum()), "3d": int(((fl & 2) > 0).sum()),
+                      "ambiguous": int(((fl & 4) > 0).sum())}
+    oracle = Oracle()
+    _, L = oracle.pyramid(f0, W, H, levels)
+    _, R = oracle.pyramid(right, W, H, levels)
+    oracle.lib.vo_stereo_match.argtypes = [C.POINTER(Plane), C.POINTER(Plane), C.POINTER(type(cfg)), C.c_int,
+                                           C.c_void_p]
+    for i in range(nfeat):
+        base[i].disparity = 0.0
+    f = type(base).from_buffer_copy(base)
+    t = time.perf_counter()
+    oracle.lib.vo_stereo_match(L, R, C.byref(cfg), nfeat, C.addressof(f))
+    out["cpu_ms_scan_and_lk"] = round((time.perf_counter() - t) * 1e3, 3)
+    out["cpu_threads"] = 1
+    log(f"stereo: {out}")
+    pl.destroy()
+    pr.destroy()
+    return out
+
+
 def cpu_baseline(f0s, f1s, corners, xs, ys, budget_s, gpu_out):
     """The oracle's reference-faithful tier (C, the reference's lane arithmetic) on this host, on the same workload:
     whole 1080p pairs (both pyramids, LK of all 16,200 points, the 64 x 64 template's exhaustive NCC map). All cores
@@ -488,6 +548,7 @@ def main():
     ap.add_argument("--serve-frames", type=int, default=300, help="frames per stream of the serving runs (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample (0: skip)")
     ap.add_argument("--still-frames", type=int, default=60, help="timed frames of each still-cell camera (0: skip)")
+    ap.add_argument("--stereo-calls", type=int, default=30, help="timed calls of the 1080p stereo step (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one device-to-device copy of this many MiB first: a plain copy of known size that calibrates "
                          "the HBM byte counters of a profiled run (0: none)")
@@ -591,6 +652,7 @@ def main():
     batch = batching(vtgpu, a.batch_steps) if a.batch_steps > 0 else {}
     serve = serving(vtgpu, a.serve_frames) if a.serve_frames > 0 else {}
     still = still_bench(vtgpu, a.still_frames) if a.still_frames > 0 else {}
+    stereo = stereo_bench(vtgpu, a.stereo_calls) if a.stereo_calls > 0 else {}
     cpu, err, prec = ({}, {}, {})
     if a.cpu_seconds > 0:
         cpu, err, prec = cpu_baseline(f0s, f1s, corners, xs, ys, a.cpu_seconds, gpu_out0)
@@ -618,6 +680,7 @@ def main():
         "serving": serve,
         "precision": prec,
         "still_cells": still,
+        "stereo": stereo,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
