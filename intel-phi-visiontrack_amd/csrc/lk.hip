@@ -764,33 +764,33 @@ int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 b
 
 // The batched tk_build + OP_LK (track.c:584-629, 275-292) as one launch: work items in the order of their
 // dependencies: bands of 16 output rows of every pair's level 1 (which also write the frames into level 0), then of
-// every pair's level 2, ..., then each pair's points, pair by pair. Workgroups take items from a counter in that order
-// (a workgroup's first point turns its four waves into independent takers of one point each, as no band follows); a
-// level-l band waits until its pyramid's level l - 1 is complete, a point until both pyramids are complete up to its
-// top level. Every item waited for was taken before the waiter's own, by a workgroup that is running and waits only
-// for earlier items, so the earliest unfinished item always progresses: no deadlock, whatever the residency.
-// There is one queue per XCD (workgroup w serves queue w mod 8, the XCD it runs on), each in that same order of
-// classes (level 1, level 2, ..., then the pairs' points): a level's bands are dealt round-robin over the queues and a
-// pair's points split into 8 contiguous chunks, one per queue, so an XCD's LK windows stay in one region of the frame.
-// The earliest unfinished class has all its dependencies finished and its items in every queue either finished or
-// held by running workgroups, so the queues cannot deadlock either. Building every pyramid level by level first keeps
-// the reduce work wide (a pair's own levels form a chain a few bands long); points then wait at most for the last
-// levels.
-// A band publishes with an agent-scope release (every thread) before its count, a waiter reads the count relaxed and
-// then acquires, so levels written on any XCD are seen fresh; those fences write back and invalidate caches, so bands
-// are coarse and a wave that already saw a pair's pyramids complete skips the wait (and its fence) for the pair's next
-// points. A wait gives up after a bounded number of sleeps, setting the context's error word (the results are then not
-// valid), so every wave reaches the exit.
+// every pair's level 2, ..., then the points. Workgroups take items from a counter in that order (a workgroup's first
+// point turns its four waves into independent takers of one point each, as no band follows). A level-l band waits until
+// its pyramid's level l - 1 is complete; a wave's first point waits until every pyramid is complete, then the wave
+// acquires once for all its points (an agent-scope acquire invalidates the XCD's L2, which LK's overlapping windows
+// live in, so one per pair change costs far more than the wait). Every item waited for was taken before the waiter's
+// own, by a workgroup that is running and waits only for earlier items, so the earliest unfinished item always
+// progresses. There is one queue per XCD (workgroup w serves queue w mod 8, the XCD it runs on), each in that same order
+// of classes (level 1, level 2, ..., then the points): a level's bands are dealt round-robin over the queues and the
+// queue's points are the q-th of 8 contiguous runs of all points, so an XCD's LK windows stay in a few pairs. The
+// earliest unfinished class has all its dependencies finished and its items in every queue either finished or held by
+// running workgroups (the grid is at most what is resident at once), so the queues cannot deadlock either.
+// A band publishes with an agent-scope release (every thread) before its counts, a waiter reads a count relaxed and then
+// acquires, so levels written on any XCD are seen fresh. A wait gives up after a bounded number of sleeps, setting the
+// context's error word (the results are then not valid), so every wave reaches the exit.
 constexpr int FB = 64;          // pairs per fused launch
 constexpr int FBAND = 16;       // output rows per pyramid item (a band across the whole level)
 constexpr int FQ = NXCD;        // work queues: one per XCD (workgroup w serves queue w mod 8, the XCD it runs on)
 constexpr int FERR = FQ;        // the error word
-constexpr int FCTR = 16;        // counter words before the done counts: [0, FQ) each queue's next item, [FERR] error
+constexpr int FTOT = FQ + 1;    // bands finished, all pyramids
+constexpr int FCTR = 16;        // counter words before the done counts: [0, FQ) each queue's next item, [FERR] error,
+                                // [FTOT] bands finished
 
 struct FusedArgs {
     LkArgs lk;                  // pairs' pyramids, top levels, point slices, window configuration, output
     const uint8_t *f0[FB], *f1[FB];
     int p0[FB], p1[FB];         // frame pitches (bytes)
+    int total_bands;            // bands of every pyramid's levels 1 ..
     int *ctr;                   // FCTR words, then done[pair][side][level]
 };
 
@@ -931,13 +931,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __syncthreads();
-            if (wave == 0)
+            if (wave == 0) {
                 wave_fetch_add(cnt + l, 1, lane);
+                wave_fetch_add(a.ctr + FTOT, 1, lane);
+            }
             continue;
         }
         // Every later item of this queue is an LK item (one point): the waves now take them one at a time, each on its
         // own (wave 0 starts with the item just taken), and no longer meet at barriers.
-        int it = wave == 0 ? item : -1, ready_pair = -1;
+        int it = wave == 0 ? item : -1;
+        bool ready = false;
         for (;;) {
             if (it < 0)
                 it = wave_fetch_add(a.ctr + q, 1, lane);
@@ -954,13 +957,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
             }
             const int k = lo;
             const PyrView &I = *a.lk.I[k], &J = *a.lk.J[k];
-            // both pyramids complete up to level max(top, 1) (which completes every level below it)
-            const int top = a.lk.top[k], tl = top > 1 ? top : 1;
-            if (ready_pair != k) {
-                wait_count(done + (k * 2) * kMaxLevels + tl, level_bands(I.l[tl]), err);
-                wait_count(done + (k * 2 + 1) * kMaxLevels + tl, level_bands(J.l[tl]), err);
+            const int top = a.lk.top[k];
+            if (!ready) {
+                // every pyramid complete, then one acquire for all the wave's points (an agent-scope acquire
+                // invalidates the XCD's L2, which LK's overlapping windows live in)
+                wait_count(a.ctr + FTOT, a.total_bands, err);
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                ready_pair = k;
+                ready = true;
             }
             const int pt = a.lk.start[k] + (gp - a.lk.pre[k]);
             it = -1;
@@ -1228,6 +1231,7 @@ int vtg_pyr_lk_fused(vtg_ctx *c, int npairs, vtg_pyr *const *I, vtg_pyr *const *
     if (rc || npairs == 0)
         return rc;
     FusedArgs a;
+    a.total_bands = 0;
     a.lk.npairs = npairs;
     int acc = 0, items = 0;
     for (int k = 0; k < npairs; k++) {
@@ -1247,8 +1251,11 @@ int vtg_pyr_lk_fused(vtg_ctx *c, int npairs, vtg_pyr *const *I, vtg_pyr *const *
         a.f1[k] = frame1[k];
         a.p0[k] = pitch0[k];
         a.p1[k] = pitch1[k];
-        for (int l = 1; l < I[k]->levels; l++)
-            items += (I[k]->lv[l].h + FBAND - 1) / FBAND + (J[k]->lv[l].h + FBAND - 1) / FBAND;
+        for (int l = 1; l < I[k]->levels; l++) {
+            const int b = (I[k]->lv[l].h + FBAND - 1) / FBAND + (J[k]->lv[l].h + FBAND - 1) / FBAND;
+            items += b;
+            a.total_bands += b;
+        }
         items += (offsets[k + 1] - offsets[k] + 3) / 4; // points, four waves to a workgroup
     }
     a.lk.pre[npairs] = acc;
