@@ -137,6 +137,33 @@ struct vtg_stereo_scan { /* = struct stereo_scan (src/kern/stereo.h:12-16) */
     float ncc;         /* the best peak's correlation */
     float second;      /* the best correlation at least 3 candidates away from it */
 };
+
+/* The front door's message (src/common/vt_proto.h:143-197): the host's descriptor of one frame in one of its slots,
+ * read from a ring of VTG_DESC_RING; same layout as struct vt_msg (88 bytes). */
+#define VTG_MSG_MAGIC 0x4d54u /* "TM" */
+#define VTG_DESC_RING 256
+enum { VTG_MSG_NONE, VTG_MSG_FRAME, VTG_MSG_RESULT };
+#define VTG_PIPE_PYR 1u
+#define VTG_PIPE_FLOW 2u
+#define VTG_PIPE_STEREO 4u
+#define VTG_PIPE_EST 8u
+enum { VTG_PRIO_ORDERED, VTG_PRIO_LIVE /* frames waiting ahead of this one are skipped */ };
+enum { VTG_EXC_NONE, VTG_EXC_BAD_CHECK, VTG_EXC_BAD_TYPE, VTG_EXC_BAD_SLOT, VTG_EXC_BAD_LENGTH,
+       VTG_EXC_INSTANCE_ORDER };  /* = enum vt_exception's ingest cases */
+
+struct vtg_msg {
+    uint16_t magic, type;       /* VTG_MSG_MAGIC, VTG_MSG_FRAME */
+    uint16_t dest, priority;    /* VTG_PIPE_ mask (must hold VTG_PIPE_PYR), VTG_PRIO_ */
+    uint32_t algorithm, length; /* the bytes of the frame in its slot */
+    uint64_t instance;          /* the source's frame number: strictly increasing per camera */
+    uint64_t acq_ns, pub_ns;    /* acquisition and publication times, host clock */
+    uint32_t slot, cmd;         /* the slot holding the frame; a command (not acted on here) */
+    float roi[4];
+    uint32_t sc_key;
+    float sc_value;
+    uint32_t camera, generated; /* which camera: 0 .. cameras - 1 */
+    uint64_t check;             /* vtg_msg_check of everything before it */
+};
 #endif
 
 #endif

@@ -291,6 +291,50 @@ int vtg_srv_wait(vtg_srv *srv, uint64_t ticket, struct lk_out *out, int *nrec, s
                  double *latency_ms);
 int vtg_srv_get_stats(vtg_srv *srv, struct vtg_srv_stats *stats);
 
+/* --- the front door (src/common/vt_proto.h:143-197; src/card/session_track.c:76-139, 188-222): frames arrive as the
+ * reference's wire format describes them, a ring of VTG_DESC_RING descriptors (struct vtg_msg = struct vt_msg)
+ * published by a count, each naming a host slot holding its frame. vtg_door_ingest takes the descriptors published
+ * so far and checks each before acting on it, in the reference's order (magic and check word, type / destination /
+ * camera, slot, length, the camera's instance order): a bad one counts as an exception (the last kept) and its slot is
+ * free at once. At most dev_slots frames are taken and unanswered (backpressure: the rest wait for the next call);
+ * when the newest frame a call takes asks for VTG_PRIO_LIVE, the frames taken before it in that call are skipped.
+ * Good frames go to the serving loop with their camera's points; vtg_door_result answers them in order. */
+uint64_t vtg_msg_check(const struct vtg_msg *m);                       /* vt_msg_check, vt_proto.h:181-191 */
+struct vtg_door_cfg {
+    int cameras;              /* 1 .. 3 (vt_cameras) */
+    int w[3], h[3], levels[3];/* each camera's frames (u8, pitch = w) and pyramid levels */
+    int npoints[3];           /* each camera's points (xy given at creation) */
+    int max_points;           /* the server streams' capacity */
+    uint32_t slots;           /* the host's frame slots (vt_config.slots) */
+    uint64_t slot_bytes;      /* bytes from one slot to the next */
+    int dev_slots;            /* frames taken and unanswered, at most */
+};
+struct vtg_door_state {       /* what the reference reports back in its control area (vt_ctl) */
+    uint64_t frames_taken;    /* descriptors consumed: their slots are free */
+    uint64_t exceptions;      /* descriptors dropped as malformed */
+    uint32_t last_exception;  /* VTG_EXC_ */
+    uint32_t pad;
+    uint64_t skipped;         /* frames passed over for a LIVE one */
+    uint64_t submitted;       /* frames handed to the serving loop */
+    uint64_t in_flight;       /* of those, not yet answered */
+};
+struct vtg_door_result {
+    uint64_t instance, acq_ns;
+    uint32_t camera, slot;
+    int nrec;                 /* LK records written (0 for a camera's first frame) */
+    int pad;
+    double latency_ms;        /* submit to results on the host */
+};
+typedef struct vtg_door vtg_door;
+/* Opens one server stream per camera; xy[k]: camera k's npoints[k] points (x, y pairs, host). */
+int vtg_door_create(vtg_srv *srv, const struct vtg_door_cfg *cfg, const float *const *xy, vtg_door **out);
+int vtg_door_destroy(vtg_door *d);
+int vtg_door_ingest(vtg_door *d, const struct vtg_msg *desc, uint64_t frames_pub, const uint8_t *slots,
+                    struct vtg_door_state *state);
+/* The oldest unanswered frame's answer (wait = 0: VTG_EAGAIN while it runs or when nothing is in flight); out: room for
+ * its camera's npoints records (max_out). */
+int vtg_door_result(vtg_door *d, int wait, struct vtg_door_result *r, struct lk_out *out, int max_out);
+
 /* ------------------------------------------------------------------ reference-shaped calls over host planes */
 
 /* img_u8_to_q6_* (src/kern/img.h:16-17): dst rows y0..y1 = 64 x src, padding of those rows filled. */

@@ -80,6 +80,38 @@ class StereoFeature(C.Structure):
         return (self.x, self.y, self.disparity, self.flags, self.xr, self.yr, self.z)
 
 
+class Msg(C.Structure):
+    """struct vtg_msg (= the reference's struct vt_msg, 88 bytes)."""
+    _fields_ = [("magic", C.c_uint16), ("type", C.c_uint16), ("dest", C.c_uint16), ("priority", C.c_uint16),
+                ("algorithm", C.c_uint32), ("length", C.c_uint32), ("instance", C.c_uint64),
+                ("acq_ns", C.c_uint64), ("pub_ns", C.c_uint64), ("slot", C.c_uint32), ("cmd", C.c_uint32),
+                ("roi", C.c_float * 4), ("sc_key", C.c_uint32), ("sc_value", C.c_float), ("camera", C.c_uint32),
+                ("generated", C.c_uint32), ("check", C.c_uint64)]
+
+
+MSG_MAGIC, DESC_RING, MSG_FRAME, PIPE_PYR, PIPE_ALL, PRIO_ORDERED, PRIO_LIVE = 0x4d54, 256, 1, 1, 15, 0, 1
+EXC_NONE, EXC_BAD_CHECK, EXC_BAD_TYPE, EXC_BAD_SLOT, EXC_BAD_LENGTH, EXC_INSTANCE_ORDER = range(6)
+
+
+class DoorCfg(C.Structure):
+    _fields_ = [("cameras", C.c_int), ("w", C.c_int * 3), ("h", C.c_int * 3), ("levels", C.c_int * 3),
+                ("npoints", C.c_int * 3), ("max_points", C.c_int), ("slots", C.c_uint32),
+                ("slot_bytes", C.c_uint64), ("dev_slots", C.c_int)]
+
+
+class DoorState(C.Structure):
+    _fields_ = [("frames_taken", C.c_uint64), ("exceptions", C.c_uint64), ("last_exception", C.c_uint32),
+                ("pad", C.c_uint32), ("skipped", C.c_uint64), ("submitted", C.c_uint64), ("in_flight", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class DoorResult(C.Structure):
+    _fields_ = [("instance", C.c_uint64), ("acq_ns", C.c_uint64), ("camera", C.c_uint32), ("slot", C.c_uint32),
+                ("nrec", C.c_int), ("pad", C.c_int), ("latency_ms", C.c_double)]
+
+
 STEREO_SCAN_DTYPE = np.dtype([("disparity", "<f4"), ("ncc", "<f4"), ("second", "<f4")])
 
 STILL_MOTION_FIELDS = ["cols", "rows", "cell_at", "tracked", "moving", "count_mean", "drift_u", "drift_v",
@@ -178,11 +210,16 @@ def lib():
         "vtg_stereo_scan_many": [V, V, V, I, I, V, I, I, V],
         "vtg_stereo_scan_planes": [V, C.POINTER(Plane), C.POINTER(Plane), I, V, V, I, I, V],
         "vtg_stereo_match": [V, V, V, C.POINTER(StereoCfg), I, V],
+        "vtg_door_create": [V, C.POINTER(DoorCfg), V, C.POINTER(V)], "vtg_door_destroy": [V],
+        "vtg_door_ingest": [V, V, C.c_uint64, V, C.POINTER(DoorState)],
+        "vtg_door_result": [V, I, C.POINTER(DoorResult), V, I],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = I
+    L.vtg_msg_check.argtypes = [C.POINTER(Msg)]
+    L.vtg_msg_check.restype = C.c_uint64
     L.vtg_ctx_error.argtypes = [V]
     L.vtg_ctx_error.restype = C.c_char_p
     L.vtg_ctx_stream.argtypes = [V]
@@ -209,7 +246,8 @@ EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", 
            "vtg_still_vectors_plane", "vtg_still_decide", "vtg_still_update", "vtg_still_update_plane",
            "vtg_still_finish", "vtg_still_frame", "vtg_still_cells", "vtg_still_mean", "vtg_still_motion",
            "vtg_still_hash", "vtg_stereo_scan_many", "vtg_stereo_scan_planes",
-           "vtg_stereo_match"]
+           "vtg_stereo_match", "vtg_msg_check", "vtg_door_create", "vtg_door_destroy", "vtg_door_ingest",
+           "vtg_door_result"]
 
 
 def synth():
@@ -744,6 +782,46 @@ class HostPlane:
 
 
 EAGAIN = -11
+
+
+class Door:
+    """A vtg_door in front of a Server: descriptor rings in, answers in order (the reference's wire format)."""
+
+    def __init__(self, server, cfg, xy):
+        self.L, self.server, self.cfg = lib(), server, cfg
+        arr = (C.c_void_p * 3)(*[(x.ctypes.data if x is not None else None) for x in (list(xy) + [None] * 3)[:3]])
+        self._xy = xy
+        h = C.c_void_p()
+        rc = self.L.vtg_door_create(server.h, C.byref(cfg), arr, C.byref(h))
+        if rc:
+            raise VtgError(f"vtg_door_create = {rc}")
+        self.h = h
+
+    def ingest(self, desc, frames_pub, slots):
+        st = DoorState()
+        rc = self.L.vtg_door_ingest(self.h, C.addressof(desc), frames_pub, slots.ctypes.data, C.byref(st))
+        if rc:
+            raise VtgError(f"vtg_door_ingest = {rc}")
+        return st.as_dict()
+
+    def result(self, npoints, wait=True):
+        r = DoorResult()
+        out = np.zeros(max(npoints, 1), LK_OUT_DTYPE)
+        rc = self.L.vtg_door_result(self.h, 1 if wait else 0, C.byref(r), out.ctypes.data, len(out))
+        if rc == EAGAIN:
+            return None
+        if rc:
+            raise VtgError(f"vtg_door_result = {rc}")
+        return r, out[:r.nrec]
+
+    def destroy(self):
+        if self.h:
+            self.L.vtg_door_destroy(self.h)
+            self.h = None
+
+
+def msg_check(m):
+    return lib().vtg_msg_check(C.byref(m))
 
 
 class Server:
