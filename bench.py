@@ -375,8 +375,24 @@ def still_bench(vtgpu, nframes, cpu_frames=4):
             o.finish()
         cpu_ms = (time.perf_counter() - t) * 1e3 / cpu_frames
         o.free()
+        # eight such cameras a frame: one vtg_still_frame_many against eight vtg_still_frame calls
+        cams = [ctx.still(w, h, 16, 0.5, 4.0) for _ in range(8)]
+        for i in range(3):
+            ctx.still_frame_many(cams, [pyrs[i % 2]] * 8, [lv] * 8)
+        ctx.sync()
+        tm = []
+        for i in range(nframes):
+            t = time.perf_counter()
+            ctx.still_frame_many(cams, [pyrs[i % 2]] * 8, [lv] * 8)
+            ctx.sync()
+            tm.append(time.perf_counter() - t)
+        many_ms = float(np.percentile(np.array(tm) * 1e3, 50))
+        for cam in cams:
+            cam.destroy()
         key = f"{w}x{h}"
         out[key] = {"camera": f"{W}x{H} read at level {lv}", "cells": st.ncells, "frames": nframes,
+                    "eight_cameras_ms_p50": round(many_ms, 4),
+                    "eight_cameras_cells_per_s": round(8 * st.ncells / (many_ms * 1e-3), 1),
                     "gpu_ms_p50": round(float(np.percentile(ms, 50)), 4),
                     "gpu_ms_p99": round(float(np.percentile(ms, 99)), 4),
                     "cells_per_s": round(st.ncells / (float(np.percentile(ms, 50)) * 1e-3), 1),

@@ -383,6 +383,9 @@ int vtg_still_update_plane(vtg_ctx *ctx, vtg_still *s, const struct vt_plane *cu
 int vtg_still_finish(vtg_ctx *ctx, vtg_still *s);                                   /* still_finish, still.c:264-270 */
 /* the four phases in order (vectors skipped on the first frame, as the reference's caller does: m3_ref.c:273-277) */
 int vtg_still_frame(vtg_ctx *ctx, vtg_still *s, const vtg_pyr *pyr, int level);
+/* n cameras' frames at once (a camera at most once): every camera's cells in one LK launch, one wait for all cells;
+ * each camera ends as after vtg_still_frame with its own level `level[i]` of pyr[i]. */
+int vtg_still_frame_many(vtg_ctx *ctx, int n, vtg_still *const *s, const vtg_pyr *const *pyr, const int *level);
 int vtg_still_cells(vtg_ctx *ctx, vtg_still *s, struct vtg_still_cell *out);             /* still_cells: cols x rows */
 int vtg_still_mean(vtg_ctx *ctx, const vtg_still *s, const struct vt_plane *out);  /* still_mean, into a host plane */
 int vtg_still_motion(const vtg_still *s, struct vtg_motion *m);                    /* still_motion, still.c:280-293 */

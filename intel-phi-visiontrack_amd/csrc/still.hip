@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "ldnorm.hpp"
@@ -495,6 +496,87 @@ int vtg_still_frame(vtg_ctx *c, vtg_still *s, const vtg_pyr *p, int level)
         rc = run_update(c, s);
     if (!rc)
         s->frames++;
+    return rc;
+}
+
+// Several cameras' frames at once: each camera's level staged, every camera's cells through ONE LK launch (their
+// centres gathered into one array), then each camera's cell kernel; one wait for all the cells; the decisions here, one
+// camera after another; the weights up and each camera's mean update. Each camera is as after vtg_still_frame.
+int vtg_still_frame_many(vtg_ctx *c, int n, vtg_still *const *s, const vtg_pyr *const *p, const int *level)
+{
+    if (!c || n < 0 || (n > 0 && (!s || !p || !level)))
+        return VTG_EINVAL;
+    for (int i = 0; i < n; i++) {
+        if (!s[i])
+            return fail(c, VTG_EINVAL, "vtg_still_frame_many: camera %d is null", i);
+        for (int j = 0; j < i; j++)
+            if (s[j] == s[i])
+                return fail(c, VTG_EINVAL, "vtg_still_frame_many: camera %d given twice", i);
+    }
+    int rc = VTG_OK;
+    for (int i = 0; i < n && !rc; i++)
+        rc = stage_from_pyr(c, s[i], p[i], level[i]);
+    if (rc)
+        return rc;
+    std::vector<const vtg_pyr *> I, J;
+    std::vector<int> top, off{0}, who;
+    for (int i = 0; i < n; i++)
+        if (s[i]->frames > 0) {
+            I.push_back(s[i]->mean);
+            J.push_back(s[i]->cur);
+            top.push_back(0);
+            off.push_back(off.back() + s[i]->ncells);
+            who.push_back(i);
+        }
+    const int m = (int)who.size(), total = off.back();
+    if (m > 0) {
+        void *buf = nullptr;
+        const size_t o_lk = up256(sizeof(float) * 2 * total);
+        rc = scratch(c, o_lk + up256(sizeof(lk_out) * total), &buf);
+        if (rc)
+            return rc;
+        float *xy = (float *)buf;
+        lk_out *lk = (lk_out *)((char *)buf + o_lk);
+        for (int k = 0; k < m; k++)
+            VTG_HIP(c, hipMemcpyAsync(xy + 2 * (size_t)off[k], s[who[k]]->xy, sizeof(float) * 2 * s[who[k]]->ncells,
+                                      hipMemcpyDeviceToDevice, c->stream));
+        const lk_cfg cfg{ST_PATCH, ST_PATCH, ST_ITER, ST_EPS, ST_MIN_EIG, ST_MAX_RES, 0};
+        rc = launch_lk_many(c, m, I.data(), J.data(), top.data(), off.data(), &cfg, xy, nullptr, ST_FB,
+                            VTG_SOLVE_F64, lk, nullptr);
+        if (rc)
+            return rc;
+        for (int k = 0; k < m; k++) {
+            vtg_still *t = s[who[k]];
+            const Level &M = t->mean->lv[0], &C = t->cur->lv[0];
+            CellArgs a{(const float *)C.p, (const float *)M.p, t->cfg.w, t->cfg.h, C.pitch, t->cols, t->ncells, t->xy,
+                       lk + off[k], t->dcell};
+            hipLaunchKernelGGL(still_cells, dim3((t->ncells + 255) / 256), dim3(256), 0, c->stream, a);
+            VTG_HIP(c, hipGetLastError());
+            VTG_HIP(c, hipMemcpyAsync(t->cell.data(), t->dcell, sizeof(vtg_still_cell) * t->ncells,
+                                      hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    VTG_HIP(c, hipStreamSynchronize(c->stream));
+    // the cameras' decisions are independent (each reference camera decides on its own thread): one host thread each
+    {
+        std::vector<std::thread> th;
+        for (int i = 1; i < n; i++)
+            th.emplace_back([&, i] { decide(s[i]); });
+        if (n > 0)
+            decide(s[0]);
+        for (auto &t : th)
+            t.join();
+    }
+    for (int i = 0; i < n; i++) {
+        s[i]->cells_pending = false;
+        VTG_HIP(c, hipMemcpyAsync(s[i]->weight, s[i]->wt.data(), sizeof(float) * s[i]->ncells, hipMemcpyHostToDevice,
+                                  c->stream));
+    }
+    for (int i = 0; i < n && !rc; i++) {
+        rc = run_update(c, s[i]);
+        if (!rc)
+            s[i]->frames++;
+    }
     return rc;
 }
 

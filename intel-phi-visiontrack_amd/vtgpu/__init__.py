@@ -204,7 +204,7 @@ def lib():
         "vtg_still_vectors": [V, V, V, I], "vtg_still_vectors_plane": [V, V, C.POINTER(Plane)],
         "vtg_still_decide": [V, V], "vtg_still_update": [V, V, V, I],
         "vtg_still_update_plane": [V, V, C.POINTER(Plane)], "vtg_still_finish": [V, V],
-        "vtg_still_frame": [V, V, V, I], "vtg_still_cells": [V, V, V],
+        "vtg_still_frame": [V, V, V, I], "vtg_still_frame_many": [V, I, V, V, V], "vtg_still_cells": [V, V, V],
         "vtg_still_mean": [V, V, C.POINTER(Plane)], "vtg_still_motion": [V, C.POINTER(StillMotion)],
         "vtg_still_hash": [V, V, C.c_uint64, C.POINTER(C.c_uint64)],
         "vtg_stereo_scan_many": [V, V, V, I, I, V, I, I, V],
@@ -244,7 +244,7 @@ EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", 
            "vtg_lk_track_planes", "vtg_correlate_planes",
            "vtg_still_create", "vtg_still_destroy", "vtg_still_first", "vtg_still_size", "vtg_still_vectors",
            "vtg_still_vectors_plane", "vtg_still_decide", "vtg_still_update", "vtg_still_update_plane",
-           "vtg_still_finish", "vtg_still_frame", "vtg_still_cells", "vtg_still_mean", "vtg_still_motion",
+           "vtg_still_finish", "vtg_still_frame", "vtg_still_frame_many", "vtg_still_cells", "vtg_still_mean", "vtg_still_motion",
            "vtg_still_hash", "vtg_stereo_scan_many", "vtg_stereo_scan_planes",
            "vtg_stereo_match", "vtg_msg_check", "vtg_door_create", "vtg_door_destroy", "vtg_door_ingest",
            "vtg_door_result"]
@@ -417,6 +417,14 @@ class Context:
         self.check(self.L.vtg_stereo_scan_planes(self.h, C.byref(left.c), C.byref(right.c), len(xs), xs.ctypes.data,
                                                  ys.ctypes.data, dmin, dmax, out.ctypes.data), "stereo_scan_planes")
         return out
+
+    def still_frame_many(self, stills, pyrs, levels):
+        """One frame for each of several still cameras (vtg_still_frame_many)."""
+        n = len(stills)
+        S = (C.c_void_p * n)(*[x.h for x in stills])
+        P = (C.c_void_p * n)(*[x.h for x in pyrs])
+        Lv = (C.c_int * n)(*levels)
+        self.check(self.L.vtg_still_frame_many(self.h, n, S, P, Lv), "still_frame_many")
 
     def stereo_match(self, left, right, cfg, feats):
         """The tracker's stereo step over `feats` (a ctypes array of StereoFeature, read and written)."""

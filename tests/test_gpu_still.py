@@ -166,3 +166,32 @@ def test_still_rejects_bad_sizes(ctx):
     s.vectors(p, 1)   # level 1 is 64 x 64
     s.destroy()
     p.destroy()
+
+
+def test_still_frame_many_equals_single(ctx, oracle):
+    """Five cameras of different sizes and levels through vtg_still_frame_many (one LK launch for all cells) end every
+    frame exactly as five cameras through vtg_still_frame one by one: the same still_hash and motion."""
+    sizes = [(320, 240, 0), (301, 203, 0), (642, 481, 1), (256, 192, 0), (400, 300, 1)]
+    frames = [still_frames(oracle, W, H, 6, 90 + i, jump=12) for i, (W, H, _) in enumerate(sizes)]
+    pyrs = [ctx.pyramid(W, H, lv + 1) for (W, H, lv) in sizes]
+
+    def lvl(W, H, lv):
+        for _ in range(lv):
+            W, H = (W + 1) // 2, (H + 1) // 2
+        return W, H
+
+    many = [ctx.still(*lvl(W, H, lv), 8, 0.5, 3.0) for (W, H, lv) in sizes]
+    one = [ctx.still(*lvl(W, H, lv), 8, 0.5, 3.0) for (W, H, lv) in sizes]
+    for k in range(6):
+        for i, p in enumerate(pyrs):
+            p.build(np.ascontiguousarray(frames[i][k]))
+        ctx.still_frame_many(many, pyrs, [lv for (_, _, lv) in sizes])
+        for i, p in enumerate(pyrs):
+            one[i].frame(p, sizes[i][2])
+        for i in range(len(sizes)):
+            assert many[i].hash() == one[i].hash(), (k, i)
+            assert many[i].motion() == one[i].motion()
+    for s in many + one:
+        s.destroy()
+    for p in pyrs:
+        p.destroy()
