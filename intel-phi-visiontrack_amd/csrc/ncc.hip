@@ -1160,6 +1160,7 @@ int by_level_type(vtg_ctx *c, const vtg_pyr *p, int level, F &&f)
     case VTG_STORE_F16: return eb == 1 ? f((uint8_t *)nullptr) : f((__half *)nullptr);
     case VTG_STORE_BF16: return eb == 1 ? f((uint8_t *)nullptr) : f((__hip_bfloat16 *)nullptr);
     case VTG_STORE_Q6W: return f((uint16_t *)nullptr);
+    case VTG_STORE_Q6F: return eb == 1 ? f((uint8_t *)nullptr) : f((float *)nullptr);
     }
     return fail(c, VTG_EINVAL, "storage %d", p->storage);
 }

@@ -165,6 +165,7 @@ static int elem_bytes(int storage, int level)
     case VTG_STORE_F16:
     case VTG_STORE_BF16: return level == 0 ? 1 : 2;
     case VTG_STORE_Q6W: return 2;
+    case VTG_STORE_Q6F: return level == 0 ? 1 : 4;
     }
     return 0;
 }
@@ -206,7 +207,7 @@ int pyr_new(vtg_ctx *c, int w, int h, int levels, int storage, void *mem, vtg_py
         return fail(c, VTG_ERANGE, "vtg_pyr_create: frame %dx%d outside 8..16384", w, h);
     if (levels < 1 || levels > VTG_MAX_LEVELS)
         return fail(c, VTG_EINVAL, "vtg_pyr_create: %d levels (1..%d)", levels, VTG_MAX_LEVELS);
-    if (storage < VTG_STORE_Q6 || storage > VTG_STORE_Q6W)
+    if (storage < VTG_STORE_Q6 || storage > VTG_STORE_Q6F)
         return fail(c, VTG_EINVAL, "vtg_pyr_create: storage %d", storage);
     vtg_pyr *p = new vtg_pyr();
     p->w = w;
@@ -300,6 +301,7 @@ int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *
     switch (ps[0]->storage) {
     case VTG_STORE_Q6: return build_levels<uint16_t, uint8_t, true>(c, ps, n, frames, pitch, rows.data());
     case VTG_STORE_F32: return build_levels<float, float, true>(c, ps, n, frames, pitch, rows.data());
+    case VTG_STORE_Q6F: return build_levels<float, uint8_t, true>(c, ps, n, frames, pitch, rows.data());
     case VTG_STORE_F16: return build_levels<__half, uint8_t, false>(c, ps, n, frames, pitch, rows.data());
     case VTG_STORE_BF16: return build_levels<__hip_bfloat16, uint8_t, false>(c, ps, n, frames, pitch, rows.data());
     }
@@ -503,7 +505,7 @@ int vtg_pyr_read_level(vtg_ctx *c, const vtg_pyr *p, int level, const struct vt_
         return rc;
     const dim3 grid((dst->stride + 255) / 256, L.h + 2 * VT_PAD_Y);
     const int es = p->elem[level];
-    if (p->storage == VTG_STORE_F32)
+    if (p->storage == VTG_STORE_F32 || (p->storage == VTG_STORE_Q6F && es == 4))
         hipLaunchKernelGGL(to_plane<float>, grid, dim3(256), 0, c->stream, (const float *)L.p, L.pitch, L.w, L.h,
                            (float *)tmp, dst->stride);
     else if (es == 1)

@@ -177,6 +177,14 @@ int stage_from_pyr(vtg_ctx *c, vtg_still *s, const vtg_pyr *p, int level)
         hipLaunchKernelGGL(stage_level<uint16_t>, grid, block, 0, c->stream, (const uint16_t *)L.p, L.pitch, L.w, L.h,
                            dst, D.pitch);
         break;
+    case VTG_STORE_Q6F:
+        if (l0)
+            hipLaunchKernelGGL(stage_level<uint8_t>, grid, block, 0, c->stream, (const uint8_t *)L.p, L.pitch, L.w,
+                               L.h, dst, D.pitch);
+        else
+            hipLaunchKernelGGL(stage_level<float>, grid, block, 0, c->stream, (const float *)L.p, L.pitch, L.w, L.h,
+                               dst, D.pitch);
+        break;
     case VTG_STORE_F32:
         hipLaunchKernelGGL(stage_level<float>, grid, block, 0, c->stream, (const float *)L.p, L.pitch, L.w, L.h, dst,
                            D.pitch);

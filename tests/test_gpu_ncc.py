@@ -243,3 +243,24 @@ def test_search_split_matches_single_context(ctx, tw, th):
         p0.destroy()
         p1.destroy()
     other.close()
+
+
+@pytest.mark.parametrize("tw,th,level", [(64, 64, 0), (32, 21, 1), (32, 24, 0)])
+def test_search_q6f_equals_q6(ctx, tw, th, level):
+    """A Q6F pyramid (u8 level 0, fp32 Q6 above) holds the Q6 pyramid's values: its template and its search give the
+    Q6 pyramid's records, field for field, on the matrix-core path (level 0, 64 x 64) and the generic one (level 1)."""
+    w, h = 800, 600
+    f0, f1, _ = vtgpu.frame_pair(w, h, 612)
+    out = []
+    for st in (vtgpu.STORE_Q6, vtgpu.STORE_Q6F):
+        p0, p1 = ctx.pyramid(w, h, 2, st), ctx.pyramid(w, h, 2, st)
+        p0.build(f0)
+        p1.build(f1)
+        lw, lh = (w, h) if level == 0 else ((w + 1) // 2, (h + 1) // 2)
+        t = ctx.ncc_template(p0, level, lw // 3, lh // 4, tw, th)
+        say("ncc q6f", st, tw, th, level)
+        out.append(bytes(ctx.ncc_search(p1, t, 1, lw, 1, lh, 1)))
+        t.destroy()
+        p0.destroy()
+        p1.destroy()
+    assert out[0] == out[1]
