@@ -85,7 +85,10 @@ int vtg_graph_destroy(vtg_ctx *ctx, void *graph);
  * u8 level 0 (the frame itself, Q6 = 64 x byte) and u16 upper levels hold the same values in 1 and 2 bytes. F32 is
  * the reference's own fp32 Q6 (4 B per pixel, the plain port). F16 and BF16 hold grey = Q6 / 64 above level 0 and
  * are not exact. */
-enum { VTG_STORE_Q6 = 0, VTG_STORE_F32 = 1, VTG_STORE_F16 = 2, VTG_STORE_BF16 = 3, VTG_STORE_Q6W = 4, VTG_STORE_Q6F = 5 };
+enum { VTG_STORE_Q6 = 0, VTG_STORE_F32 = 1, VTG_STORE_F16 = 2, VTG_STORE_BF16 = 3, VTG_STORE_Q6W = 4, VTG_STORE_Q6F = 5,
+       VTG_STORE_F32_U8 = 6 };
+/* VTG_STORE_F32_U8: every level fp32 Q6 (as VTG_STORE_F32) plus the u8 frame kept beside level 0, which the NCC
+ * search's matrix-core path reads: LK reads fp32 at every level. */
 /* VTG_STORE_Q6F: level 0 the u8 frame (as VTG_STORE_Q6, which the NCC search's matrix-core path reads), every level
  * above fp32 Q6 (exact integers, the reference's own plane format): LK on those levels needs no conversions. */
 /* VTG_STORE_Q6W: every level u16 Q6, exact; the storage for frames whose level 0 is filtered (below). */

@@ -749,6 +749,9 @@ int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 b
     case VTG_STORE_Q6F:
         hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, float, SOLVE, STAGED>), grid, block, lds, c->stream, a);
         break;
+    case VTG_STORE_F32_U8:
+        hipLaunchKernelGGL((lk_kernel<MAXR, float, float, SOLVE, STAGED>), grid, block, lds, c->stream, a);
+        break;
     case VTG_STORE_F16:
         hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __half, SOLVE, STAGED>), grid, block, lds, c->stream, a);
         break;

@@ -1153,14 +1153,15 @@ __global__ __launch_bounds__(64) void ncc_final(SBatch b)
 template <typename F>
 int by_level_type(vtg_ctx *c, const vtg_pyr *p, int level, F &&f)
 {
-    const int eb = p->elem[level];
+    const int eb = ncc_elem(p, level);
     switch (p->storage) {
     case VTG_STORE_F32: return f((float *)nullptr);
     case VTG_STORE_Q6: return eb == 1 ? f((uint8_t *)nullptr) : f((uint16_t *)nullptr);
     case VTG_STORE_F16: return eb == 1 ? f((uint8_t *)nullptr) : f((__half *)nullptr);
     case VTG_STORE_BF16: return eb == 1 ? f((uint8_t *)nullptr) : f((__hip_bfloat16 *)nullptr);
     case VTG_STORE_Q6W: return f((uint16_t *)nullptr);
-    case VTG_STORE_Q6F: return eb == 1 ? f((uint8_t *)nullptr) : f((float *)nullptr);
+    case VTG_STORE_Q6F:
+    case VTG_STORE_F32_U8: return eb == 1 ? f((uint8_t *)nullptr) : f((float *)nullptr);
     }
     return fail(c, VTG_EINVAL, "storage %d", p->storage);
 }
@@ -1182,17 +1183,17 @@ int launch_ncc_templates(vtg_ctx *c, int n, vtg_tmpl *const *t, const vtg_pyr *c
         int k = k0;
         for (; k < n && b.n < TMAX; k++) {
             if (t[k]->tw != b.tw || t[k]->th != b.th || p[k]->storage != p0->storage ||
-                p[k]->elem[level[k]] != p0->elem[l0])
+                ncc_elem(p[k], level[k]) != ncc_elem(p0, l0))
                 break;
             TJob &j = b.j[b.n++];
-            j.lv = p[k]->lv[level[k]];
+            j.lv = ncc_level(p[k], level[k]);
             j.x0 = x0[k];
             j.y0 = y0[k];
             j.t = t[k]->t;
             j.raw = t[k]->raw;
             j.nm = (float *)t[k]->block;
             j.su = (long long *)((char *)t[k]->block + 16);
-            j.words = p[k]->elem[level[k]] == 1 ? t[k]->words : nullptr;
+            j.words = ncc_elem(p[k], level[k]) == 1 ? t[k]->words : nullptr;
         }
         const int rc = by_level_type(c, p0, l0, [&](auto *tag) -> int {
             using T = std::remove_pointer_t<decltype(tag)>;
@@ -1219,7 +1220,7 @@ int launch_ncc_score(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int n, con
 {
     if (n <= 0)
         return VTG_OK;
-    const Level lv = p->lv[t->level];
+    const Level lv = ncc_level(p, t->level);
     const float *nm = (const float *)t->block;
     return by_level_type(c, p, t->level, [&](auto *tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
@@ -1348,12 +1349,12 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
         if (!p[k] || !t[k] || t[k]->level >= p[k]->levels)
             return fail(c, VTG_EINVAL, "vtg_ncc_search: bad pyramid or template %d", k);
         if (t[k]->tw != t0->tw || t[k]->th != t0->th || (t[k]->words != nullptr) != (t0->words != nullptr) ||
-            p[k]->storage != p[0]->storage || p[k]->elem[t[k]->level] != p[0]->elem[level0])
+            p[k]->storage != p[0]->storage || ncc_elem(p[k], t[k]->level) != ncc_elem(p[0], level0))
             return fail(c, VTG_EINVAL, "vtg_ncc_search: search %d differs from the first in template size or level "
                                        "type", k);
     }
     // u8 level: 64-wide templates go to the matrix cores, 16- and 32-wide ones to the dot4 kernel
-    const bool u8 = p[0]->elem[level0] == 1 && t0->words;
+    const bool u8 = ncc_elem(p[0], level0) == 1 && t0->words;
     const int mode = u8 && t0->tw == 64 ? MAP_MFMA
                      : u8 && (t0->th % YB) == 0 && t0->th >= YB &&
                                (t0->words_per_row == 17 || t0->words_per_row == 9 || t0->words_per_row == 5)
@@ -1364,7 +1365,7 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
     const size_t auxb = aux_bytes();
     size_t bytes = (size_t)n * auxb + (found_dev ? 0 : (size_t)n * 256);
     for (int k = 0; k < n; k++) {
-        const Level &lv = p[k]->lv[t[k]->level];
+        const Level &lv = ncc_level(p[k], t[k]->level);
         const int hi_x = lv.w - t[k]->tw - 2, hi_y = lv.h - t[k]->th - 2;
         int x0 = region[4 * k], x1 = region[4 * k + 1], y0 = region[4 * k + 2], y1 = region[4 * k + 3];
         x0 = x0 < 1 ? 1 : x0;
@@ -1397,7 +1398,7 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
             const int k = k0 + i;
             SJob &j = b.j[i];
             const int *w = &win[4 * k];
-            j.lv = p[k]->lv[t[k]->level];
+            j.lv = ncc_level(p[k], t[k]->level);
             j.t = t[k]->t;
             j.nm = (const float *)t[k]->block;
             j.sum_t = (const long long *)((const char *)t[k]->block + 16);
@@ -1470,12 +1471,12 @@ int vtg_ncc_template(vtg_ctx *c, const vtg_pyr *p, int level, int x0, int y0, in
     *out = nullptr;
     if (tw < 16 || tw > 64 || (tw & 15) || th < 1 || th > 64)
         return fail(c, VTG_ERANGE, "vtg_ncc_template: %dx%d (width 16..64 in 16s, height 1..64)", tw, th);
-    const Level &lv = p->lv[level];
+    const Level &lv = ncc_level(p, level);
     if (x0 < 1 || y0 < 1 || x0 + tw >= lv.w - 1 || y0 + th >= lv.h - 1)
         return fail(c, VTG_EINVAL, "vtg_ncc_template: window at (%d,%d) does not fit level %d (%dx%d)", x0, y0, level,
                     lv.w, lv.h);
     vtg_tmpl *t = nullptr;
-    int rc = tmpl_new(c, tw, th, level, p->elem[level] == 1, nullptr, &t);
+    int rc = tmpl_new(c, tw, th, level, ncc_elem(p, level) == 1, nullptr, &t);
     if (rc)
         return rc;
     rc = launch_ncc_template(c, p, level, x0, y0, t);
@@ -1492,10 +1493,10 @@ int vtg_ncc_template_update(vtg_ctx *c, vtg_tmpl *t, const vtg_pyr *p, int level
 {
     if (!c || !t || !p || level < 0 || level >= p->levels)
         return fail(c, VTG_EINVAL, "vtg_ncc_template_update: bad argument");
-    const Level &lv = p->lv[level];
+    const Level &lv = ncc_level(p, level);
     if (x0 < 1 || y0 < 1 || x0 + t->tw >= lv.w - 1 || y0 + t->th >= lv.h - 1)
         return fail(c, VTG_EINVAL, "vtg_ncc_template_update: window at (%d,%d) does not fit level %d", x0, y0, level);
-    if ((p->elem[level] == 1) != (t->words != nullptr))
+    if ((ncc_elem(p, level) == 1) != (t->words != nullptr))
         return fail(c, VTG_EINVAL, "vtg_ncc_template_update: level element size differs from the template's");
     t->level = level;
     return launch_ncc_template(c, p, level, x0, y0, t);
@@ -1511,11 +1512,11 @@ int vtg_ncc_template_update_many(vtg_ctx *c, int n, vtg_tmpl *const *t, const vt
     for (int k = 0; k < n; k++) {
         if (!t[k] || !p[k] || level[k] < 0 || level[k] >= p[k]->levels)
             return fail(c, VTG_EINVAL, "vtg_ncc_template_update_many: bad template, pyramid or level %d", k);
-        const Level &lv = p[k]->lv[level[k]];
+        const Level &lv = ncc_level(p[k], level[k]);
         if (x0[k] < 1 || y0[k] < 1 || x0[k] + t[k]->tw >= lv.w - 1 || y0[k] + t[k]->th >= lv.h - 1)
             return fail(c, VTG_EINVAL, "vtg_ncc_template_update_many: window %d at (%d,%d) does not fit level %d", k,
                         x0[k], y0[k], level[k]);
-        if ((p[k]->elem[level[k]] == 1) != (t[k]->words != nullptr))
+        if ((ncc_elem(p[k], level[k]) == 1) != (t[k]->words != nullptr))
             return fail(c, VTG_EINVAL, "vtg_ncc_template_update_many: level element size of %d differs from its "
                                        "template's", k);
     }

@@ -166,6 +166,7 @@ static int elem_bytes(int storage, int level)
     case VTG_STORE_BF16: return level == 0 ? 1 : 2;
     case VTG_STORE_Q6W: return 2;
     case VTG_STORE_Q6F: return level == 0 ? 1 : 4;
+    case VTG_STORE_F32_U8: return 4;
     }
     return 0;
 }
@@ -187,6 +188,12 @@ static size_t pyr_layout(vtg_pyr *p)
         lw = (lw + 1) / 2;
         lh = (lh + 1) / 2;
     }
+    if (p->storage == VTG_STORE_F32_U8) { // the u8 frame beside level 0
+        const size_t pitch_b = round_up((size_t)p->w, 256);
+        p->byte0 = Level{nullptr, p->w, p->h, (int)pitch_b};
+        p->byte0_off = off;
+        off += round_up(pitch_b * p->h, 4096);
+    }
     return off;
 }
 
@@ -207,7 +214,7 @@ int pyr_new(vtg_ctx *c, int w, int h, int levels, int storage, void *mem, vtg_py
         return fail(c, VTG_ERANGE, "vtg_pyr_create: frame %dx%d outside 8..16384", w, h);
     if (levels < 1 || levels > VTG_MAX_LEVELS)
         return fail(c, VTG_EINVAL, "vtg_pyr_create: %d levels (1..%d)", levels, VTG_MAX_LEVELS);
-    if (storage < VTG_STORE_Q6 || storage > VTG_STORE_Q6F)
+    if (storage < VTG_STORE_Q6 || storage > VTG_STORE_F32_U8)
         return fail(c, VTG_EINVAL, "vtg_pyr_create: storage %d", storage);
     vtg_pyr *p = new vtg_pyr();
     p->w = w;
@@ -228,6 +235,8 @@ int pyr_new(vtg_ctx *c, int w, int h, int levels, int storage, void *mem, vtg_py
     }
     for (int l = 0; l < levels; l++)
         p->lv[l].p = (char *)p->block + p->offset[l];
+    if (storage == VTG_STORE_F32_U8)
+        p->byte0.p = (char *)p->block + p->byte0_off;
     // the descriptor batched kernels read, written once
     p->dview = (PyrView *)((char *)p->block + view_off);
     const PyrView v = view(p);
@@ -289,7 +298,7 @@ int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *
         if (ps[i]->levels != 1)
             continue;
         const Level &L0 = ps[i]->lv[0];
-        if (ps[i]->storage == VTG_STORE_F32) {
+        if (ps[i]->storage == VTG_STORE_F32 || ps[i]->storage == VTG_STORE_F32_U8) {
             hipLaunchKernelGGL(u8_to_f32, dim3((ps[i]->w + 255) / 256, ps[i]->h), dim3(256), 0, c->stream, frames[i],
                                pitch[i], (float *)L0.p, L0.pitch, ps[i]->w, ps[i]->h);
             VTG_HIP(c, hipGetLastError());
@@ -302,6 +311,13 @@ int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *
     case VTG_STORE_Q6: return build_levels<uint16_t, uint8_t, true>(c, ps, n, frames, pitch, rows.data());
     case VTG_STORE_F32: return build_levels<float, float, true>(c, ps, n, frames, pitch, rows.data());
     case VTG_STORE_Q6F: return build_levels<float, uint8_t, true>(c, ps, n, frames, pitch, rows.data());
+    case VTG_STORE_F32_U8:
+        // the frame kept beside level 0 (unless it was uploaded there), then the fp32 levels as VTG_STORE_F32
+        for (int i = 0; i < n; i++)
+            if (frames[i] != ps[i]->byte0.p)
+                VTG_HIP(c, hipMemcpy2DAsync((void *)ps[i]->byte0.p, ps[i]->byte0.pitch, frames[i], pitch[i], ps[i]->w,
+                                            ps[i]->h, hipMemcpyDeviceToDevice, c->stream));
+        return build_levels<float, float, true>(c, ps, n, frames, pitch, rows.data());
     case VTG_STORE_F16: return build_levels<__half, uint8_t, false>(c, ps, n, frames, pitch, rows.data());
     case VTG_STORE_BF16: return build_levels<__hip_bfloat16, uint8_t, false>(c, ps, n, frames, pitch, rows.data());
     }
@@ -410,8 +426,9 @@ int vtg_pyr_build_ex(vtg_ctx *c, vtg_pyr *p, const uint8_t *frame, int pitch, in
             if (rc)
                 return rc;
         } else {
-            dst = (void *)p->lv[0].p;
-            dp = (size_t)p->lv[0].pitch;
+            const Level &L0 = p->storage == VTG_STORE_F32_U8 ? p->byte0 : p->lv[0]; // where the u8 frame lives
+            dst = (void *)L0.p;
+            dp = (size_t)L0.pitch;
         }
         VTG_HIP(c, hipMemcpy2DAsync(dst, dp, frame, pitch, p->w, p->h, hipMemcpyHostToDevice, c->stream));
         frame = (const uint8_t *)dst;
@@ -464,8 +481,9 @@ int vtg_pyr_build_many_ex(vtg_ctx *c, int n, vtg_pyr *const *pyrs, const uint8_t
             dp[i] = p->w;
             off += round_up((size_t)p->w * p->h, 256);
         } else {
-            dev[i] = (const uint8_t *)p->lv[0].p;
-            dp[i] = p->lv[0].pitch;
+            const Level &L0 = p->storage == VTG_STORE_F32_U8 ? p->byte0 : p->lv[0]; // where the u8 frame lives
+            dev[i] = (const uint8_t *)L0.p;
+            dp[i] = L0.pitch;
         }
         VTG_HIP(c, hipMemcpy2DAsync((void *)dev[i], dp[i], frames[i], pitches[i], p->w, p->h, hipMemcpyHostToDevice,
                                     c->stream));
@@ -505,7 +523,7 @@ int vtg_pyr_read_level(vtg_ctx *c, const vtg_pyr *p, int level, const struct vt_
         return rc;
     const dim3 grid((dst->stride + 255) / 256, L.h + 2 * VT_PAD_Y);
     const int es = p->elem[level];
-    if (p->storage == VTG_STORE_F32 || (p->storage == VTG_STORE_Q6F && es == 4))
+    if (p->storage == VTG_STORE_F32 || p->storage == VTG_STORE_F32_U8 || (p->storage == VTG_STORE_Q6F && es == 4))
         hipLaunchKernelGGL(to_plane<float>, grid, dim3(256), 0, c->stream, (const float *)L.p, L.pitch, L.w, L.h,
                            (float *)tmp, dst->stride);
     else if (es == 1)

@@ -187,7 +187,7 @@ int launch_scan(vtg_ctx *c, const vtg_pyr *left, const vtg_pyr *right, int level
     hipEvent_t ev;
     timer_begin(c, "stereo_scan", &ev);
     const int st = left->storage;
-    if (st == VTG_STORE_F32 || (st == VTG_STORE_Q6F && level > 0))
+    if (st == VTG_STORE_F32 || st == VTG_STORE_F32_U8 || (st == VTG_STORE_Q6F && level > 0))
         launch_t<float>(c, a);
     else if (level == 0 && st != VTG_STORE_Q6W)
         launch_t<uint8_t>(c, a);

@@ -186,6 +186,7 @@ int stage_from_pyr(vtg_ctx *c, vtg_still *s, const vtg_pyr *p, int level)
                                dst, D.pitch);
         break;
     case VTG_STORE_F32:
+    case VTG_STORE_F32_U8:
         hipLaunchKernelGGL(stage_level<float>, grid, block, 0, c->stream, (const float *)L.p, L.pitch, L.w, L.h, dst,
                            D.pitch);
         break;

@@ -78,7 +78,15 @@ struct vtg_pyr {
     int elem[vtg::kMaxLevels];
     vtg::PyrView *dview = nullptr;           // the levels' descriptor in device memory (read by batched kernels)
     bool owned = true;                       // block from hipMalloc (else placed in a caller's memory)
+    vtg::Level byte0{nullptr, 0, 0, 0};      // VTG_STORE_F32_U8: the u8 frame beside level 0
+    size_t byte0_off = 0;
 };
+
+namespace vtg {
+// The level the NCC search reads: level 0 of a VTG_STORE_F32_U8 pyramid is the u8 frame kept beside it.
+inline const Level &ncc_level(const vtg_pyr *p, int l) { return l == 0 && p->byte0.p ? p->byte0 : p->lv[l]; }
+inline int ncc_elem(const vtg_pyr *p, int l) { return l == 0 && p->byte0.p ? 1 : p->elem[l]; }
+}  // namespace vtg
 
 struct vtg_tmpl {
     int tw = 0, th = 0, level = 0;

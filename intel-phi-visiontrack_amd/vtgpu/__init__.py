@@ -15,7 +15,7 @@ LIB_GPU = os.path.join(HERE, "libvtgpu.so")
 LIB_SYNTH = os.path.join(HERE, "libvtsynth.so")
 
 PAD_X, PAD_Y, Q6 = 32, 4, 64
-STORE_Q6, STORE_F32, STORE_F16, STORE_BF16, STORE_Q6W, STORE_Q6F = 0, 1, 2, 3, 4, 5
+STORE_Q6, STORE_F32, STORE_F16, STORE_BF16, STORE_Q6W, STORE_Q6F, STORE_F32_U8 = 0, 1, 2, 3, 4, 5, 6
 BUILD_CONTRAST, BUILD_PRE_GAUSS = 1, 2
 SOLVE_F64, SOLVE_MIXED = 0, 1
 LK_OK, LK_OUT, LK_LOWEIG, LK_RESIDUAL, LK_FLOWBACK = 0, 1, 2, 4, 8

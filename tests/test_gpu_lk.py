@@ -48,7 +48,7 @@ def test_m2_cases(ctx, oracle):
     ys = np.array([p[1] for p in pts], np.float32)
     i0 = oracle.m2_render(w, h, 2048, 2048)[:, :w]
     kI, I = oracle.pyramid(i0, w, h, L)
-    for storage in (vtgpu.STORE_Q6, vtgpu.STORE_F32, vtgpu.STORE_Q6F):
+    for storage in (vtgpu.STORE_Q6, vtgpu.STORE_F32, vtgpu.STORE_Q6F, vtgpu.STORE_F32_U8):
         PI = _gpu_pyr(ctx, i0, w, h, L, storage)
         for case in g["cases"]:
             i1 = oracle.m2_render(w, h, 2048 - case["dx"], 2048 - case["dy"])[:, :w]

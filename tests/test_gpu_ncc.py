@@ -247,12 +247,13 @@ def test_search_split_matches_single_context(ctx, tw, th):
 
 @pytest.mark.parametrize("tw,th,level", [(64, 64, 0), (32, 21, 1), (32, 24, 0)])
 def test_search_q6f_equals_q6(ctx, tw, th, level):
-    """A Q6F pyramid (u8 level 0, fp32 Q6 above) holds the Q6 pyramid's values: its template and its search give the
-    Q6 pyramid's records, field for field, on the matrix-core path (level 0, 64 x 64) and the generic one (level 1)."""
+    """Q6F pyramids (u8 level 0, fp32 Q6 above) and F32_U8 pyramids (fp32 Q6 levels, the u8 frame kept beside level 0)
+    hold the Q6 pyramid's values: their templates and searches give the Q6 pyramid's records, field for field, on the
+    matrix-core path (level 0, 64 x 64, the kept frame for F32_U8) and the generic one (level 1)."""
     w, h = 800, 600
     f0, f1, _ = vtgpu.frame_pair(w, h, 612)
     out = []
-    for st in (vtgpu.STORE_Q6, vtgpu.STORE_Q6F):
+    for st in (vtgpu.STORE_Q6, vtgpu.STORE_Q6F, vtgpu.STORE_F32_U8):
         p0, p1 = ctx.pyramid(w, h, 2, st), ctx.pyramid(w, h, 2, st)
         p0.build(f0)
         p1.build(f1)
@@ -263,4 +264,4 @@ def test_search_q6f_equals_q6(ctx, tw, th, level):
         t.destroy()
         p0.destroy()
         p1.destroy()
-    assert out[0] == out[1]
+    assert out[0] == out[1] == out[2]

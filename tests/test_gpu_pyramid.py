@@ -16,7 +16,7 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 
 @pytest.mark.parametrize("size", ["640x480", "1280x720", "1920x1080"])
-@pytest.mark.parametrize("storage", [vtgpu.STORE_Q6, vtgpu.STORE_F32, vtgpu.STORE_Q6F])
+@pytest.mark.parametrize("storage", [vtgpu.STORE_Q6, vtgpu.STORE_F32, vtgpu.STORE_Q6F, vtgpu.STORE_F32_U8])
 def test_m1_reduce_lines(ctx, oracle, size, storage):
     gold = json.load(open(os.path.join(GOLD, "m1_hashes.json")))[size]
     w, h = map(int, size.split("x"))
@@ -32,7 +32,7 @@ def test_m1_reduce_lines(ctx, oracle, size, storage):
 
 @pytest.mark.parametrize("w,h,levels", [(640, 480, 3), (1920, 1080, 4), (641, 479, 5), (97, 61, 4), (3840, 2160, 5),
                                         (8, 8, 3)])
-@pytest.mark.parametrize("storage", [vtgpu.STORE_Q6, vtgpu.STORE_F32, vtgpu.STORE_Q6F])
+@pytest.mark.parametrize("storage", [vtgpu.STORE_Q6, vtgpu.STORE_F32, vtgpu.STORE_Q6F, vtgpu.STORE_F32_U8])
 def test_pyramid_bit_exact(ctx, oracle, w, h, levels, storage):
     """Every level, padding included, equals the oracle's padded fp32 Q6 planes bit for bit."""
     f0, _, _ = vtgpu.frame_pair(w, h, w * 7 + h) if w >= 64 else (np.random.default_rng(1).integers(
@@ -64,7 +64,8 @@ def test_pyramid_from_device_frame(ctx, oracle):
     d.free()
 
 
-@pytest.mark.parametrize("storage", [vtgpu.STORE_Q6, vtgpu.STORE_F32, vtgpu.STORE_F16, vtgpu.STORE_Q6F])
+@pytest.mark.parametrize("storage", [vtgpu.STORE_Q6, vtgpu.STORE_F32, vtgpu.STORE_F16, vtgpu.STORE_Q6F,
+                                     vtgpu.STORE_F32_U8])
 def test_build_many_uneven(ctx, oracle, storage):
     """Pyramids of uneven sizes and depths built together (one launch per level) equal the ones built alone."""
     sizes = [(640, 480, 3), (1280, 720, 5), (97, 61, 2), (1920, 1080, 4), (333, 250, 1)]
