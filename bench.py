@@ -95,7 +95,7 @@ class Batched:
         self.frames, self.corners = d_frames, corners
         self.groups = len(d_frames) // batch
         A = self.A
-        self.pyrs = [[(A.pyramid(W, H, LEVELS, vtgpu.STORE_Q6F), A.pyramid(W, H, LEVELS, vtgpu.STORE_Q6F))
+        self.pyrs = [[(A.pyramid(W, H, LEVELS, vtgpu.STORE_F32_U8), A.pyramid(W, H, LEVELS, vtgpu.STORE_F32_U8))
                       for _ in range(batch)]
                      for _ in range(self.groups if graph else 1)]
         self.xy = A.to_device(np.tile(np.stack([xs, ys], 1).ravel(), batch))
@@ -681,7 +681,7 @@ def main():
         "config": {"workload": "1080p frame pairs: 2 x 4-level pyramid + LK 16,200 pts 21x21 + NCC 64x64 exhaustive "
                                "per pair", "w": W, "h": H, "levels": LEVELS, "points_per_pair": NPTS,
                    "window": f"{WIN}x{WIN}", "ncc_template": f"{TPL}x{TPL}", "ncc_corners": npos,
-                   "pairs_per_step": a.batch, "storage": "q6f (u8 level 0, fp32 Q6 above: exact integers)", "graph": not a.no_graph,
+                   "pairs_per_step": a.batch, "storage": "f32_u8 (fp32 Q6 levels: exact integers; the u8 frame kept for the NCC search)", "graph": not a.no_graph,
                    "parallelism": "1 GPU, 2 streams",
                    "input_rotation": f"{pairs} device frame pairs ({pairs * 2 * W * H / 2 ** 20:.0f} MiB)"},
         "plain_pairs_per_s": round(plain, 2), "vs_plain": round(pairs_per_s / plain, 2),
