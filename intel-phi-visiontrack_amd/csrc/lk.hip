@@ -904,6 +904,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             rt::RJob job;
+            job.copy8 = nullptr;
+            job.cp8 = 0;
             job.dst = (void *)P.l[l].p;
             job.dw = P.l[l].w;
             job.dh = P.l[l].h;

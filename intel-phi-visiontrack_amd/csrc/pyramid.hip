@@ -124,8 +124,11 @@ int build_levels(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *const *fr
         const Level *L = ps[i]->lv;
         if (ps[i]->levels < 2)
             continue;
+        // F32_U8: the frame's bytes kept beside level 0 from the same tile (unless the frame already lives there)
+        const bool keep8 = ps[i]->storage == VTG_STORE_F32_U8 && frames[i] != ps[i]->byte0.p;
         jobs.push_back(RJob{frames[i], (void *)L[1].p, (void *)L[0].p, ps[i]->w, ps[i]->h, pitch[i], L[1].w, L[1].h,
-                            L[1].pitch, L[0].pitch, 0, 0});
+                            L[1].pitch, L[0].pitch, 0, 0, keep8 ? (void *)ps[i]->byte0.p : nullptr,
+                            keep8 ? ps[i]->byte0.pitch : 0});
         th.push_back(rows[i]);
     }
     int rc = jobs.empty() ? VTG_OK : launch_jobs<uint8_t, T1, TC, true, EXACT>(c, jobs, th);
@@ -312,9 +315,10 @@ int launch_pyr_build_many(vtg_ctx *c, vtg_pyr *const *ps, int n, const uint8_t *
     case VTG_STORE_F32: return build_levels<float, float, true>(c, ps, n, frames, pitch, rows.data());
     case VTG_STORE_Q6F: return build_levels<float, uint8_t, true>(c, ps, n, frames, pitch, rows.data());
     case VTG_STORE_F32_U8:
-        // the frame kept beside level 0 (unless it was uploaded there), then the fp32 levels as VTG_STORE_F32
+        // the frame kept beside level 0 (unless it was uploaded there): by the level-1 reduce, which reads it anyway,
+        // or by a copy for a single-level pyramid; the fp32 levels as VTG_STORE_F32
         for (int i = 0; i < n; i++)
-            if (frames[i] != ps[i]->byte0.p)
+            if (ps[i]->levels == 1 && frames[i] != ps[i]->byte0.p)
                 VTG_HIP(c, hipMemcpy2DAsync((void *)ps[i]->byte0.p, ps[i]->byte0.pitch, frames[i], pitch[i], ps[i]->w,
                                             ps[i]->h, hipMemcpyDeviceToDevice, c->stream));
         return build_levels<float, float, true>(c, ps, n, frames, pitch, rows.data());

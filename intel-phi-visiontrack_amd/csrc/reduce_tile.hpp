@@ -42,12 +42,15 @@ template <> struct Q6<__hip_bfloat16> {
     static __device__ __forceinline__ __hip_bfloat16 putf(float q) { return __float2bfloat16(q * (1.0f / 64.0f)); }
 };
 
-// One reduce job: a level made from the level below it, possibly writing the frame into level 0 on the way.
+// One reduce job: a level made from the level below it, possibly writing the frame into level 0 on the way (copy, cp
+// elements a row) and, from a u8 frame, its bytes into a second plane as well (copy8, cp8 bytes a row; null: none).
 struct RJob {
     const void *src;
     void *dst, *copy;
     int sw, sh, sp, dw, dh, dp, cp;
     int tiles_x, t0; // tiles across the output, first linear tile of the job in the launch
+    void *copy8;
+    int cp8;
 };
 constexpr int JMAX = 32;
 struct RJobs {
@@ -100,8 +103,12 @@ __device__ __forceinline__ void reduce_tile(const RJob &J, int tile, typename Ac
             tin[r][cc] = (Acc)Q6<TS>::get(raw[k]);
             if (COPY) {
                 const int yy = iy0 + r, xx = ix0 + cc;
-                if (r >= 2 && r < 2 + 2 * TH && cc >= 2 && cc < 2 + 2 * TW && yy < J.sh && xx < J.sw)
+                if (r >= 2 && r < 2 + 2 * TH && cc >= 2 && cc < 2 + 2 * TW && yy < J.sh && xx < J.sw) {
                     ((TC *)J.copy)[(size_t)yy * J.cp + xx] = CopyConv<TC, TS>::get(raw[k]);
+                    if constexpr (sizeof(TS) == 1)
+                        if (J.copy8)
+                            ((uint8_t *)J.copy8)[(size_t)yy * J.cp8 + xx] = (uint8_t)raw[k];
+                }
             }
         }
     }
