@@ -88,8 +88,8 @@ class Batched:
     results of each step downloaded on one of two copy contexts while the next step runs (double-buffered pinned
     host records)."""
 
-    def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph):
-        self.v, self.B, self.graph = vtgpu, batch, graph
+    def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph, one_stream=False):
+        self.v, self.B, self.graph, self.one_stream = vtgpu, batch, graph, one_stream
         self.A, self.S = vtgpu.Context(0), vtgpu.Context(0)
         self.copy = [vtgpu.Context(0), vtgpu.Context(0)]
         self.frames, self.corners = d_frames, corners
@@ -118,6 +118,7 @@ class Batched:
         beside LK. A waits for S at the end. serial: everything on A, one kernel after another (for timing each
         kernel alone)."""
         A, S, B = self.A, self.S, self.B
+        serial = serial or self.one_stream
         pyrs = self.pyrs[g if self.graph else 0]
         fr = self.frames[g * B:(g + 1) * B]
         A.build_many([p for pair in pyrs for p in pair], [f for pair in fr for f in pair])
@@ -560,6 +561,8 @@ def main():
     ap.add_argument("--batch", type=int, default=8, help="frame pairs per step of the batched form")
     ap.add_argument("--pairs", type=int, default=72, help="device-resident frame pairs rotated through")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="the searches' scoring on the LK stream too (no fork and join inside the step)")
     ap.add_argument("--plain-steps", type=int, default=8)
     ap.add_argument("--batch-steps", type=int, default=20, help="graph replays of the uneven 64-pair batch (0: skip)")
     ap.add_argument("--serve-frames", type=int, default=300, help="frames per stream of the serving runs (0: skip)")
@@ -595,7 +598,7 @@ def main():
     d_frames = [(up.to_device(f0s[i % UNIQ]), up.to_device(f1s[i % UNIQ])) for i in range(pairs)]
     up.sync()
 
-    bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=not a.no_graph)
+    bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=not a.no_graph, one_stream=a.one_stream)
     bt.prepare()
     for i in range(a.warmup):
         bt.launch(i)
@@ -682,7 +685,7 @@ def main():
                                "per pair", "w": W, "h": H, "levels": LEVELS, "points_per_pair": NPTS,
                    "window": f"{WIN}x{WIN}", "ncc_template": f"{TPL}x{TPL}", "ncc_corners": npos,
                    "pairs_per_step": a.batch, "storage": "f32_u8 (fp32 Q6 levels: exact integers; the u8 frame kept for the NCC search)", "graph": not a.no_graph,
-                   "parallelism": "1 GPU, 2 streams",
+                   "parallelism": "1 GPU, 1 stream" if a.one_stream else "1 GPU, 2 streams",
                    "input_rotation": f"{pairs} device frame pairs ({pairs * 2 * W * H / 2 ** 20:.0f} MiB)"},
         "plain_pairs_per_s": round(plain, 2), "vs_plain": round(pairs_per_s / plain, 2),
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),

@@ -554,7 +554,7 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
     float eig0 = 0.0f;
     for (int L = top; L >= 0; L--) {
         const double px = ldexp((double)x, -L), py = ldexp((double)y, -L);
-        const Level &li = I.l[L], &lj = J.l[L];
+        const Level li = I.l[L], lj = J.l[L]; // copies: the descriptors stay in registers through the iterations
         if (px < mx || py < my || px >= (double)(li.w - mx - 1) || py >= (double)(li.h - my - 1)) {
             if (L == 0)
                 status |= LK_OUT;
@@ -596,6 +596,8 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             valid0 = true;
         }
         const double lx = (double)(lj.w - mx - 1), ly = (double)(lj.h - my - 1);
+        // the inverse with the sums' 1/64 folded in (a power of two: the products round exactly as b / 64 would)
+        const double ka = s.ia * (1.0 / 64.0), kb = s.ib * (1.0 / 64.0), kc = s.ic * (1.0 / 64.0);
         double vx = 0, vy = 0;
         for (int it = 0; it < c.max_iter; it++) {
             const double qx = px + gx + vx, qy = py + gy + vy;
@@ -621,8 +623,8 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             } else {
                 sample_direct<MAXR, T0, false>(lj, qx, qy, g, tl, b);
             }
-            const double bx = (double)b[0] * (1.0 / 64.0), by = (double)b[1] * (1.0 / 64.0);
-            const double ex = s.ia * bx + s.ib * by, ey = s.ib * bx + s.ic * by;
+            const double bx = (double)b[0], by = (double)b[1];
+            const double ex = ka * bx + kb * by, ey = kb * bx + kc * by;
             vx += ex;
             vy += ey;
             iters++;
