@@ -245,6 +245,34 @@ def test_search_split_matches_single_context(ctx, tw, th):
     other.close()
 
 
+def test_f32u8_kept_frame_build_many(ctx):
+    """F32_U8 pyramids of uneven sizes and depths built together keep each frame beside level 0 (written by the level-1
+    reduce from its tile, or copied for a one-level pyramid): a level-0 search over each gives the record the same
+    template gives over a Q6 pyramid of the frame built alone (the matrix-core path for 64 x 64, the dot4 one for 32 x 21)."""
+    sizes = [(640, 480, 3), (1280, 720, 5), (97, 61, 2), (333, 250, 1)]
+    frames = [vtgpu.frame_pair(w, h, 70 + i) for i, (w, h, _) in enumerate(sizes)]
+    pyrs = [ctx.pyramid(w, h, L, vtgpu.STORE_F32_U8) for w, h, L in sizes]
+    dev = [ctx.to_device(f1) for _, f1, _ in frames]
+    ctx.build_many(pyrs, dev)
+    for (w, h, L), (f0, f1, _), p in zip(sizes, frames, pyrs):
+        tw, th = (64, 64) if w >= 200 else (32, 21)
+        say("f32_u8 kept frame", w, h, L, tw, th)
+        a, b = ctx.pyramid(w, h, L), ctx.pyramid(w, h, L)
+        a.build(f0)
+        b.build(f1)
+        t = ctx.ncc_template(a, 0, w // 3, h // 4, tw, th)
+        ref = ctx.ncc_search(b, t, 1, w, 1, h, 1)
+        got = ctx.ncc_search(p, t, 1, w, 1, h, 1)
+        assert bytes(got) == bytes(ref), (w, h, L)
+        assert ref.ncc > -1.0   # a pick was made
+        t.destroy()
+        a.destroy()
+        b.destroy()
+        p.destroy()
+    for d in dev:
+        d.free()
+
+
 @pytest.mark.parametrize("tw,th,level", [(64, 64, 0), (32, 21, 1), (32, 24, 0)])
 def test_search_q6f_equals_q6(ctx, tw, th, level):
     """Q6F pyramids (u8 level 0, fp32 Q6 above) and F32_U8 pyramids (fp32 Q6 levels, the u8 frame kept beside level 0)
