@@ -712,13 +712,24 @@ __host__ __device__ constexpr size_t score_lds_words(int th)
     return (size_t)(NY + th - 1) * SC_LW + 2 * 2 * 320 + 2 * 16;
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_addu(uint32_t v)
 {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(v, o);
-        v += lane >= o ? u : 0u;
-    }
+    // lanes whose source lies outside the row, and rows the mask leaves out, add 0
+    return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+// Inclusive prefix sum over the 64 lanes (exact integers), all in DPP: shifts by 1, 2, 4, 8 within each row of 16, then
+// row 0's total into row 1 and row 2's into row 3 (row_bcast:15), then rows 0-1's into rows 2 and 3 (row_bcast:31).
+// Every lane must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    v = dpp_addu<0x111, 0xf>(v); // row_shr:1
+    v = dpp_addu<0x112, 0xf>(v); // row_shr:2
+    v = dpp_addu<0x114, 0xf>(v); // row_shr:4
+    v = dpp_addu<0x118, 0xf>(v); // row_shr:8
+    v = dpp_addu<0x142, 0xa>(v); // row_bcast:15
+    v = dpp_addu<0x143, 0xc>(v); // row_bcast:31
     return v;
 }
 
@@ -795,11 +806,11 @@ __global__ __launch_bounds__(256) void ncc_cells_score(SBatch b)
                 s2b += ib * ib - ob * ob;
             }
         }
-        const uint32_t i1a = wave_incl_scan(s1a, lane), i2a = wave_incl_scan(s2a, lane);
+        const uint32_t i1a = wave_incl_scan(s1a), i2a = wave_incl_scan(s2a);
         uint32_t i1b = 0, i2b = 0;
         if (wave == 0) {
-            i1b = wave_incl_scan(s1b, lane);
-            i2b = wave_incl_scan(s2b, lane);
+            i1b = wave_incl_scan(s1b);
+            i2b = wave_incl_scan(s2b);
         }
         uint32_t *Eb = E + (d & 1) * 640, *Tb = tot + (d & 1) * 16;
         if (lane == 63) {
