@@ -886,18 +886,48 @@ __device__ __forceinline__ unsigned long long min64(unsigned long long a, unsign
 
 constexpr int TOP_REG = 32; // keys per thread held in registers: maps of up to 32,768 cells are read once
 
+// One step of a wave-wide minimum of 64-bit keys: both halves moved by the same DPP control (all ones, the identity,
+// where the source lies outside the row or the row mask leaves the row out).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long dpp_min64(unsigned long long m)
+{
+    const unsigned int lo = (unsigned int)__builtin_amdgcn_update_dpp(-1, (int)(unsigned int)m, CTRL, ROW_MASK, 0xf, false);
+    const unsigned int hi =
+        (unsigned int)__builtin_amdgcn_update_dpp(-1, (int)(unsigned int)(m >> 32), CTRL, ROW_MASK, 0xf, false);
+    return min64(m, ((unsigned long long)hi << 32) | lo);
+}
+
+// The smallest key of the wave, the same in every lane: a butterfly within each row of 16 (after each step the lanes
+// of a group hold one value, so the mirrors act as exchanges), then the rows joined into lane 63. Every lane must be
+// active.
+__device__ __forceinline__ unsigned long long wave_min64(unsigned long long m)
+{
+    m = dpp_min64<0xb1, 0xf>(m);  // quad_perm [1,0,3,2]
+    m = dpp_min64<0x4e, 0xf>(m);  // quad_perm [2,3,0,1]
+    m = dpp_min64<0x141, 0xf>(m); // row_half_mirror
+    m = dpp_min64<0x140, 0xf>(m); // row_mirror
+    m = dpp_min64<0x142, 0xa>(m); // row_bcast:15 into rows 1 and 3
+    m = dpp_min64<0x143, 0xc>(m); // row_bcast:31 into rows 2 and 3: lane 63 holds the minimum
+    const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)m, 63);
+    const unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)(m >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 // The 8 best cells, scores above -2 only: 8 rounds of a workgroup-wide minimum of the keys above the last one taken.
-// Grid: one workgroup per search.
+// Keys are unique (the cell index is their low word), so each round's minimum belongs to one thread; with the map in
+// registers each thread keeps its smallest key not yet taken and only the owner of the key just taken looks again.
+// Each round: a DPP minimum per wave, the 16 waves' minima through LDS (double-buffered by round, one barrier), every
+// thread taking the minimum of the 16. Grid: one workgroup per search.
 __global__ __launch_bounds__(1024) void ncc_top8(SBatch b)
 {
-    __shared__ unsigned long long wmin[16];
-    __shared__ unsigned long long taken;
+    __shared__ unsigned long long wmin[2][16];
     const SJob &a = b.j[blockIdx.x];
     const vtg_cell *cells = a.cells;
     const int ncell = a.cols * a.rows;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool inreg = ncell <= 1024 * TOP_REG;
     unsigned long long key[TOP_REG];
+    unsigned long long cur = ~0ull; // with the map in registers: this thread's smallest key not yet taken
     if (inreg) {
         float v[TOP_REG];
 #pragma unroll
@@ -906,18 +936,17 @@ __global__ __launch_bounds__(1024) void ncc_top8(SBatch b)
             v[q] = i < ncell ? cells[i].v : -3.0f;
         }
 #pragma unroll
-        for (int q = 0; q < TOP_REG; q++)
+        for (int q = 0; q < TOP_REG; q++) {
             key[q] = v[q] > -2.0f ? cell_key(v[q], tid + 1024 * q) : ~0ull;
+            cur = min64(cur, key[q]);
+        }
     }
     unsigned long long prev = 0;
     int n = 0;
     for (int round = 0; round < 8; round++) {
         unsigned long long m = ~0ull;
         if (inreg) {
-#pragma unroll
-            for (int q = 0; q < TOP_REG; q++)
-                if (round == 0 || key[q] > prev)
-                    m = min64(m, key[q]);
+            m = cur;
         } else {
             for (int i = tid; i < ncell; i += 1024) {
                 const float v = cells[i].v;
@@ -926,27 +955,28 @@ __global__ __launch_bounds__(1024) void ncc_top8(SBatch b)
                     m = min64(m, k);
             }
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned int lo = __shfl_xor((unsigned int)m, o), hi = __shfl_xor((unsigned int)(m >> 32), o);
-            m = min64(m, ((unsigned long long)hi << 32) | lo);
-        }
+        m = wave_min64(m);
         if (lane == 0)
-            wmin[wave] = m;
-        __syncthreads();
-        if (tid == 0) {
-            unsigned long long t = wmin[0];
-            for (int w = 1; w < 16; w++)
-                t = min64(t, wmin[w]);
-            taken = t;
-        }
-        __syncthreads();
-        prev = taken;
+            wmin[round & 1][wave] = m;
+        __syncthreads(); // the other buffer is rewritten only after the next round's barrier
+        unsigned long long t = wmin[round & 1][0];
+#pragma unroll
+        for (int w = 1; w < 16; w++)
+            t = min64(t, wmin[round & 1][w]);
+        prev = t;
         if (prev == ~0ull)
             break;
         if (tid == 0)
             a.aux->tops[round] = (int)(unsigned int)prev;
         n++;
-        __syncthreads(); // taken is rewritten next round
+        if (inreg && cur == prev) {
+            // this thread's key was taken: its next smallest
+            unsigned long long c2 = ~0ull;
+#pragma unroll
+            for (int q = 0; q < TOP_REG; q++)
+                c2 = key[q] > prev ? min64(c2, key[q]) : c2;
+            cur = c2;
+        }
     }
     if (tid == 0)
         a.aux->ntop = n;

@@ -132,6 +132,25 @@ def test_search_1080_full(ctx, oracle):
     t.destroy()
 
 
+def test_search_large_map(ctx, oracle):
+    """A map of more than 32,768 cells (4200 x 2200, a 16 x 16 template: 262 x 137 cells), where the top-8 selection
+    reads the map from memory each round instead of keeping it in registers: the pick, its score and the second peak
+    equal the oracle's."""
+    w, h, tw, th = 4200, 2200, 16, 16
+    f0, f1, mo, p0, p1, k0, k1 = _setup(ctx, oracle, w, h, 77)
+    x0, y0 = _textured_corner(f0, tw, th, tw + th)
+    t = ctx.ncc_template(p0, 0, x0, y0, tw, th)
+    app, norm = t.read()
+    ncell = ((w - tw - 2 - 1) // 16 + 1) * ((h - th - 2 - 1) // 16 + 1)
+    assert ncell > 32768
+    say("ncc search large map", ncell)
+    f = ctx.ncc_search(p1, t, 1, w, 1, h, 1)
+    of, oc = oracle.ncc_search(k1[0], app, tw, th, norm, 1, w, 1, h, 1, threads=16)
+    assert (f.hx, f.hy) == (of.hx, of.hy)
+    assert abs(f.ncc - float(of.ncc)) <= 1e-5 and abs(f.second - float(of.second)) <= 1e-5
+    t.destroy()
+
+
 @pytest.mark.parametrize("tw,th,level,step", [(64, 64, 0, 1), (32, 21, 1, 2), (32, 24, 0, 2)])
 def test_search_many_matches_single(ctx, tw, th, level, step):
     """Several searches in one call (frames of different sizes, partial windows, one window that clips to nothing and
