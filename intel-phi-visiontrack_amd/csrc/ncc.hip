@@ -699,8 +699,9 @@ __global__ __launch_bounds__(256) void ncc_xcorr_mfma(SBatch b)
 }
 
 // The cell map from the MFMA cross terms: the frame tile in LDS; for each corner row, the window sums of every byte
-// column (sum u, sum u^2 over th rows, slid down by one row at a time) and their prefix sums along the row (a scan
-// per wave, the waves' totals through LDS), so a corner's box sums are two differences of prefixes; the score in fp64
+// column (sum u, sum u^2 over th rows, slid down by one row at a time) and their prefix sums within each 64-column
+// chunk (a DPP scan per wave), so a corner's box sums are two differences of prefixes plus, for a box that ends in the
+// next chunk, its own chunk's total; one barrier a corner row; the score in fp64
 // and the first best corner of each 16 x 16 cell (OP_SEARCH, track.c:301-329). Thread c owns corner column X0 + c
 // and byte column c (threads 0 .. 62 also column 256 + c); the 16 lanes of a cell column reduce by shuffles.
 // Grid: (column blocks of 16 cells, cell rows, searches).
@@ -709,7 +710,7 @@ constexpr int SC_LOADS = 26;                 // tile words per thread: (16 + 64 
 
 __host__ __device__ constexpr size_t score_lds_words(int th)
 {
-    return (size_t)(NY + th - 1) * SC_LW + 2 * 2 * 320 + 2 * 16;
+    return (size_t)(NY + th - 1) * SC_LW + 2 * 2 * 320;
 }
 
 template <int CTRL, int ROW_MASK>
@@ -742,8 +743,7 @@ __global__ __launch_bounds__(256) void ncc_cells_score(SBatch b)
     const uint8_t *img = (const uint8_t *)a.lv.p;
     const int ih = a.lv.h, pitch = a.lv.pitch, tw = b.tw, th = b.th, stp = b.step;
     const int LR = NY + th - 1, VB = NX + tw - 1;
-    uint32_t *E = lds + LR * SC_LW; // [row parity][quantity][320]: exclusive prefix sums of the column sums
-    uint32_t *tot = E + 2 * 2 * 320; // [row parity][quantity][8]: each wave's total
+    uint32_t *E = lds + LR * SC_LW; // [row parity][quantity][320]: each chunk's exclusive prefix sums of the column sums
     const int X0 = a.x0 + blockIdx.x * NX, Y0 = a.y0 + blockIdx.y * NY;
     const int A = X0 & ~3, off = X0 - A;
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -806,37 +806,26 @@ __global__ __launch_bounds__(256) void ncc_cells_score(SBatch b)
                 s2b += ib * ib - ob * ob;
             }
         }
+        // each 64-column chunk's exclusive prefix sums (columns 0-255: wave w's chunk w; columns 256-319: wave 0's
+        // second chunk) into LDS, double-buffered by row; a box of tw <= 64 columns starting in chunk w ends in chunk w
+        // or w + 1, and in the latter case the chunks' offsets differ by chunk w's total, which the wave holds itself:
+        // one barrier a row
         const uint32_t i1a = wave_incl_scan(s1a), i2a = wave_incl_scan(s2a);
-        uint32_t i1b = 0, i2b = 0;
+        uint32_t *Eb = E + (d & 1) * 640;
+        Eb[tid] = i1a - s1a;
+        Eb[320 + tid] = i2a - s2a;
         if (wave == 0) {
-            i1b = wave_incl_scan(s1b);
-            i2b = wave_incl_scan(s2b);
+            const uint32_t i1b = wave_incl_scan(s1b), i2b = wave_incl_scan(s2b);
+            Eb[256 + lane] = i1b - s1b;
+            Eb[576 + lane] = i2b - s2b;
         }
-        uint32_t *Eb = E + (d & 1) * 640, *Tb = tot + (d & 1) * 16;
-        if (lane == 63) {
-            Tb[wave] = i1a;
-            Tb[8 + wave] = i2a;
-            if (wave == 0) {
-                Tb[4] = i1b;
-                Tb[12] = i2b;
-            }
-        }
-        __syncthreads();
-        uint32_t o1 = 0, o2 = 0;
-        for (int w2 = 0; w2 < wave; w2++) {
-            o1 += Tb[w2];
-            o2 += Tb[8 + w2];
-        }
-        Eb[tid] = o1 + i1a - s1a;
-        Eb[320 + tid] = o2 + i2a - s2a;
-        if (wave == 0) {
-            const uint32_t t1 = Tb[0] + Tb[1] + Tb[2] + Tb[3], t2 = Tb[8] + Tb[9] + Tb[10] + Tb[11];
-            Eb[256 + lane] = t1 + i1b - s1b;
-            Eb[576 + lane] = t2 + i2b - s2b;
-        }
+        const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)i1a, 63);
+        const uint32_t t2 = (uint32_t)__builtin_amdgcn_readlane((int)i2a, 63);
         __syncthreads();
         // corner (x, y): box sums over columns tid .. tid + tw - 1 of the column sums
-        const uint32_t box1 = Eb[tid + tw] - Eb[tid], box2 = Eb[320 + tid + tw] - Eb[320 + tid];
+        const bool cross = lane + tw >= 64;
+        const uint32_t box1 = Eb[tid + tw] - Eb[tid] + (cross ? t1 : 0u);
+        const uint32_t box2 = Eb[320 + tid + tw] - Eb[320 + tid] + (cross ? t2 : 0u);
         const int y = Y0 + d;
         const int u0 = col0[d * RB];
         const double su = (double)box1, sq = (double)box2;
