@@ -306,7 +306,8 @@ constexpr int RING_MAX = 9; // half-pixel grid of one candidate: 2 (around + 1) 
 // What the finish kernels pass on: the top cells, then every half-pixel score around each.
 struct Aux {
     int ntop, tops[8];
-    int pad[7];
+    int done; // refinement workgroups of the search finished (cleared by ncc_top8)
+    int pad[6];
     double hs[8][RING_MAX * RING_MAX];
 };
 
@@ -967,8 +968,10 @@ __global__ __launch_bounds__(1024) void ncc_top8(SBatch b)
             cur = c2;
         }
     }
-    if (tid == 0)
+    if (tid == 0) {
         a.aux->ntop = n;
+        a.aux->done = 0;
+    }
 }
 
 // The score at half-pixel corner (hx, hy) from a staged region: pixel (x, y) of the level at reg[(y - ry) * rw + x - rx],
@@ -1026,17 +1029,12 @@ __device__ double score16_lds(const float *reg, int rx, int ry, int rw, const fl
     return (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : a / sqrt(norm * var);
 }
 
-// One workgroup per top cell of each search (grid: 8, searches): every half-pixel corner within around + 1 of twice
-// its best corner (the inner 2 around + 1 square is search_slot's refinement, track.c:1697-1711; the ring gives
-// take_back's neighbours, track.c:1737-1741), the window and the template staged in LDS, 16 corners at a time.
+// Top cell c of a search, by one workgroup of 256: every half-pixel corner within around + 1 of twice its best corner
+// (the inner 2 around + 1 square is search_slot's refinement, track.c:1697-1711; the ring gives take_back's
+// neighbours, track.c:1737-1741), the window and the template staged in LDS (sm), 16 corners at a time.
 template <typename T, int TWQ>
-__global__ __launch_bounds__(256) void ncc_refine(SBatch b)
+__device__ __forceinline__ void refine_one(const SBatch &b, const SJob &a, int c, float *sm)
 {
-    extern __shared__ float sm[];
-    const SJob &a = b.j[blockIdx.y];
-    const int c = blockIdx.x;
-    if (a.rows == 0 || c >= a.aux->ntop)
-        return;
     constexpr int tw = 16 * TWQ;
     const int th = b.th, step = b.step;
     const Level lv = a.lv;
@@ -1088,30 +1086,27 @@ __global__ __launch_bounds__(256) void ncc_refine(SBatch b)
 }
 
 // The pick, the second peak clear of it by half a template (search_slot, track.c:1704-1725) and the parabola offsets
-// from the pick's ring (take_back, track.c:1737-1741; peak_offset, 1650-1657). Selection only: no scoring here.
-// Grid: one 64-lane workgroup per search. A search with nothing to search gets the empty record (score -2).
-__global__ __launch_bounds__(64) void ncc_final(SBatch b)
+// from the pick's ring (take_back, track.c:1737-1741; peak_offset, 1650-1657). Selection only: no scoring here. By the
+// NT threads of one workgroup, with LDS for the search's record (ax) and the candidates' picks. A search with nothing
+// to search gets the empty record (score -2).
+template <int NT>
+__device__ void finish_search(const SBatch &b, const SJob &a, Aux &ax, vtg_cell *tc, double *scs, int *sxs, int *sys)
 {
-    __shared__ Aux ax;
-    __shared__ vtg_cell tc[8];
-    __shared__ double scs[8];
-    __shared__ int sxs[8], sys[8];
-    const SJob &a = b.j[blockIdx.x];
     const int ncell = a.cols * a.rows, tw = b.tw, th = b.th, step = b.step;
     {
         // the record into LDS, every load issued before the first store
-        constexpr int NW = (int)(sizeof(Aux) / 4), PER = (NW + 63) / 64;
+        constexpr int NW = (int)(sizeof(Aux) / 4), PER = (NW + NT - 1) / NT;
         const unsigned int *src = (const unsigned int *)a.aux;
         unsigned int *dst = (unsigned int *)&ax;
         unsigned int v[PER];
 #pragma unroll
         for (int k = 0; k < PER; k++) {
-            const int i = threadIdx.x + 64 * k;
+            const int i = threadIdx.x + NT * k;
             v[k] = i < NW ? src[i] : 0u;
         }
 #pragma unroll
         for (int k = 0; k < PER; k++) {
-            const int i = threadIdx.x + 64 * k;
+            const int i = threadIdx.x + NT * k;
             if (i < NW)
                 dst[i] = v[k];
         }
@@ -1178,6 +1173,35 @@ __global__ __launch_bounds__(64) void ncc_final(SBatch b)
         f.best_v = bc.v;
     }
     *a.found = f;
+}
+
+// Grid: (8, searches), 256 threads. Workgroup c refines top cell c of its search (refine_one) when the search has one;
+// the last workgroup of a search to finish (the count in the search's record) then makes its record (finish_search)
+// from every candidate's scores. Nothing waits: each workgroup counts itself once and leaves.
+template <typename T, int TWQ>
+__global__ __launch_bounds__(256) void ncc_refine_final(SBatch b)
+{
+    extern __shared__ float sm[];
+    __shared__ Aux ax;
+    __shared__ vtg_cell tc[8];
+    __shared__ double scs[8];
+    __shared__ int sxs[8], sys[8];
+    __shared__ int last;
+    const SJob &a = b.j[blockIdx.y];
+    const int c = blockIdx.x;
+    if (a.rows != 0 && c < a.aux->ntop) // the same for the whole workgroup
+        refine_one<T, TWQ>(b, a, c, sm);
+    // this workgroup's scores visible at agent scope before it is counted; the last one counted reads them all
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0)
+        last = __hip_atomic_fetch_add(&a.aux->done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+               (int)gridDim.x - 1;
+    __syncthreads();
+    if (!last)
+        return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    finish_search<256>(b, a, ax, tc, scs, sxs, sys);
 }
 
 template <typename F>
@@ -1329,7 +1353,8 @@ static int launch_batch(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, in
     hipLaunchKernelGGL(ncc_top8, dim3(b.n), dim3(1024), 0, f->stream, b);
     VTG_HIP(c, hipGetLastError());
     timer_end(f, "ncc_top8", ev);
-    if (maxcell > 0) {
+    {
+        // the refinement of every search's top cells and each search's record, one launch
         const int around = b.step > 1 ? 2 * b.step - 1 : 1, R = around + 1;
         const int rmax = R + 1 + 2; // region extent beyond the template, whole pixels, each way
         const size_t lds =
@@ -1337,24 +1362,20 @@ static int launch_batch(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, in
         int rc = by_level_type(c, p0, level, [&](auto *tag) -> int {
             using T = std::remove_pointer_t<decltype(tag)>;
             const dim3 grid(8, b.n);
-            timer_begin(f, "ncc_refine", &ev);
+            timer_begin(f, "ncc_refine_final", &ev);
             switch (b.tw / 16) {
-            case 1: hipLaunchKernelGGL((ncc_refine<T, 1>), grid, dim3(256), lds, f->stream, b); break;
-            case 2: hipLaunchKernelGGL((ncc_refine<T, 2>), grid, dim3(256), lds, f->stream, b); break;
-            case 3: hipLaunchKernelGGL((ncc_refine<T, 3>), grid, dim3(256), lds, f->stream, b); break;
-            default: hipLaunchKernelGGL((ncc_refine<T, 4>), grid, dim3(256), lds, f->stream, b); break;
+            case 1: hipLaunchKernelGGL((ncc_refine_final<T, 1>), grid, dim3(256), lds, f->stream, b); break;
+            case 2: hipLaunchKernelGGL((ncc_refine_final<T, 2>), grid, dim3(256), lds, f->stream, b); break;
+            case 3: hipLaunchKernelGGL((ncc_refine_final<T, 3>), grid, dim3(256), lds, f->stream, b); break;
+            default: hipLaunchKernelGGL((ncc_refine_final<T, 4>), grid, dim3(256), lds, f->stream, b); break;
             }
             VTG_HIP(c, hipGetLastError());
-            timer_end(f, "ncc_refine", ev);
+            timer_end(f, "ncc_refine_final", ev);
             return VTG_OK;
         });
         if (rc)
             return rc;
     }
-    timer_begin(f, "ncc_final", &ev);
-    hipLaunchKernelGGL(ncc_final, dim3(b.n), dim3(64), 0, f->stream, b);
-    VTG_HIP(c, hipGetLastError());
-    timer_end(f, "ncc_final", ev);
     return VTG_OK;
 }
 
