@@ -6,8 +6,8 @@
  *                                (u8_to_q6, norm, minmax, gauss5, reduce, sobel_x/y, absdiff) and the closed form
  *                                (a constant plane stays constant; an integer brute-force 5x5 on tiny planes).
  *   LK lane tier                 logs/m2/host/640x480.txt: ten displacements x (point 0's raw bits + FNV of all 330
- *                                lk_out), multi-level pyramids included; plus oracle/_ref on the BASELINE synthetic
- *                                frames where that build exists (tests/golden/lk_ref_*.json).
+ *                                lk_out), multi-level pyramids included; plus oracle/_ref on the seeded blob-scene
+ *                                frames where that build exists (tests/golden/lk_ref.json).
  *   LK f64 tier                  agreement with the lane tier within 1e-3 px on converged points, and exact
  *                                translation truth on the m2 inputs (|d| <= 14.4 px: EPE within the reference's
  *                                measured 0.0209 px p95).
