@@ -735,6 +735,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
     return v;
 }
 
+// POW2: tw th is a power of two, so sj^2 / n is the exact product sj^2 (1 / n) (the same double, no division).
+template <bool POW2>
 __global__ __launch_bounds__(256) void ncc_cells_score(SBatch b)
 {
     extern __shared__ uint32_t lds[];
@@ -783,6 +785,7 @@ __global__ __launch_bounds__(256) void ncc_cells_score(SBatch b)
     const int x = X0 + tid;
     const bool x_ok = in_map && x <= a.x1 && ((x - a.x0) % stp) == 0;
     const double n = (double)(tw * th), norm = (double)a.nm[0], mean = (double)a.nm[1];
+    const double inv_n = 1.0 / n; // exact when POW2
     const double sum_t = (double)a.sum_t[0];
     float best_v = -3.0f;
     int best_x = 0, best_y = 0;
@@ -836,7 +839,7 @@ __global__ __launch_bounds__(256) void ncc_cells_score(SBatch b)
         const double st = 4096.0 * pc - 64.0 * mean * sv;
         const double sj = 64.0 * sv;
         const double sjj = 4096.0 * (sq - 2.0 * (double)u0 * su + n * (double)u0 * (double)u0);
-        const double var = sjj - sj * sj / n;
+        const double var = sjj - (POW2 ? sj * sj * inv_n : sj * sj / n);
         const double sc = (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : st / sqrt(norm * var);
         const float v = (float)sc;
         const bool ok = x_ok && y <= a.y1 && ((y - a.y0) % stp) == 0;
@@ -1315,8 +1318,13 @@ static int launch_batch(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, in
             }
             const dim3 gs((maxcols + 15) / 16, maxrows, b.n);
             timer_begin(f, "ncc_cells", &ev);
-            hipLaunchKernelGGL(ncc_cells_score, gs, dim3(256), score_lds_words(b.th) * sizeof(uint32_t),
-                               f->stream, b);
+            const int nt = b.tw * b.th;
+            if ((nt & (nt - 1)) == 0)
+                hipLaunchKernelGGL(ncc_cells_score<true>, gs, dim3(256), score_lds_words(b.th) * sizeof(uint32_t),
+                                   f->stream, b);
+            else
+                hipLaunchKernelGGL(ncc_cells_score<false>, gs, dim3(256), score_lds_words(b.th) * sizeof(uint32_t),
+                                   f->stream, b);
             VTG_HIP(c, hipGetLastError());
             timer_end(f, "ncc_cells", ev);
         } else if (mode == MAP_DOT4) {
