@@ -161,10 +161,9 @@ int vtg_pyr_lk_fused(vtg_ctx *ctx, int npairs, vtg_pyr *const *I, vtg_pyr *const
 /* The context's device error word (waits for the context): nonzero when a fused launch's bounded wait gave up, whose
  * results are then not valid; reading clears it. */
 int vtg_ctx_device_error(vtg_ctx *ctx, int *code);
-/* The LK schedule: margin > 0 stages each level's template patch and J's neighbourhood (the window plus `margin`
- * pixels) in LDS with one batch of loads; 0 reads the pyramid in memory at every iteration. Results are the same
- * bit for bit either way; only time differs (default 0). */
-int vtg_lk_set_margin(vtg_ctx *ctx, int margin);
+/* The LK schedule where no autotuner pick applies: wavefronts per workgroup, 1 .. 8 (default 4). A wavefront tracks 4
+ * points, one per row of 16 lanes. Results are the same bit for bit under every schedule; only time differs. */
+int vtg_lk_set_waves(vtg_ctx *ctx, int waves);
 
 /* ------------------------------------------------------------------ autotuner (no counterpart in the reference, whose
  * schedule is fixed at build time: one thread per row range or point, track.c:210-292) */
@@ -173,16 +172,17 @@ int vtg_lk_set_margin(vtg_ctx *ctx, int margin);
  * window). */
 struct vtg_tune {
     int pyr_rows;                         /* pyramid reduce tile height: 8, 16 or 32 output rows */
-    int lk_margin, lk_waves;              /* LK schedule: margin 0 = direct, else staged; waves (points) per workgroup */
+    int lk_waves;                         /* LK schedule: wavefronts (4 points each) per workgroup */
     float pyr_us, lk_us;                  /* median times of the picks: both pyramids; the given points */
-    float pyr_us_default, lk_us_default;  /* the same under the defaults (16 rows; direct, 2 waves) */
+    float pyr_us_default, lk_us_default;  /* the same under the defaults (16 rows; the context's LK waves) */
     int candidates;                       /* schedules timed */
 };
 
 /* Time every candidate schedule of the pyramid reduce and of the LK kernel on this pair (device frames f0, f1 of
  * `pitch` bytes per row, built into I and J) and these n device points, and keep the fastest for the pair's size
- * class in the context: later builds and LK launches of pyramids of that class use them (the context's margin,
- * vtg_lk_set_margin, applies to classes not tuned). Waits; allocates a temporary output. */
+ * class in the context: later builds and LK launches of pyramids of that class use them (the context's waves,
+ * vtg_lk_set_waves, apply to classes not tuned; a batch of several classes takes the pick of its pair with the most
+ * points, so it stays one launch). Waits; allocates a temporary output. */
 int vtg_tune_pair(vtg_ctx *ctx, vtg_pyr *I, vtg_pyr *J, const uint8_t *f0, const uint8_t *f1, int pitch, int top,
                   const struct lk_cfg *cfg, int n, const float *xy, int solve_mode, struct vtg_tune *res);
 /* The picks in force for a size class (the defaults when it was not tuned; times left 0). */

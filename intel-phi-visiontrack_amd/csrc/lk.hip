@@ -1,38 +1,42 @@
-// lk.hip: sparse pyramidal Lucas-Kanade on the GPU, one wavefront per point (lk_driver.inc, src/kern/lk_driver.inc:7-121;
-// patch operations lk_vx.c:36-175; the OP_LK batch loop, src/kern/track.c:275-292).
+// lk.hip: sparse pyramidal Lucas-Kanade on the GPU in the reference's own lane order (lk_driver.inc,
+// src/kern/lk_driver.inc:7-121; patch operations lk_vx.c:36-175; the OP_LK batch loop, src/kern/track.c:275-292).
 //
-// Lane layout. A w x h window is cut into G = 64 / w row groups of w lanes each (21 x 21: 3 groups of 7 rows, 63 lanes;
-// 15 x 15: 4 groups of 4 rows, 60 lanes). Lane (k, g) owns window column k and rows [g * chunk, (g + 1) * chunk): its
-// template values T, Tx, Ty live in registers for the whole level. A lane walks MAXR consecutive window rows (a
-// compile-time count no smaller than the chunk); a group whose chunk would run past the window's last row walks the
-// last MAXR rows instead, and rows that are not the lane's own carry zero gradients, so they add nothing to any sum.
-// Direct schedule (the default). The window build reads the level in memory straight into registers: per image row,
-// four taps and three x-interpolations give the lane's three columns of the bilinear window S, and consecutive rows
-// give S rows and the Sobel gradients, with no staging and no index division. Each iteration reads two taps per row of
-// J, x-interpolated once per row and y-interpolated between rows, as the reference's vector code carries a row
-// (lk_vx.c:100-126). The level bounds test (a window m = w / 2 + 1 pixels clear of the left/top edge and m + 1 of the
-// right/bottom edge) keeps every pixel read inside the level, so no coordinate is clamped.
-// Staged schedule (margin > 0). At the start of a level the wave loads, in one batch, the template's (w + 3) x (h + 3)
-// patch of I and J's neighbourhood of the level's starting estimate (the window plus a margin of m pixels) into its
-// LDS slice; S, the gradients and every iteration's samples then come from LDS. The arithmetic of both schedules is
-// the same operation for operation, so their results agree bit for bit.
-// Element types. Level values are read in their own units (u8: grey, u16: Q6, f16/bf16: grey) and carried at that
-// scale through the interpolation; the Q6 factor is applied where it is exact (a power of two), so the results equal
-// those of interpolating Q6 values.
-// Sums over the window (G, b, the residual) reduce across the wave with DPP (quad permutes, row mirrors, row
-// broadcasts) and one readlane, so every lane holds the same total. The per-point driver (level walk, eigenvalue test,
-// 2x2 solve, convergence, bounds) runs in fp64 on those uniform values, the reference's being x87 long double; the
-// patch arithmetic is fp32 with fused multiply-adds where the reference's vector code fuses them.
-// Batches. A launch covers points of up to 64 frame pairs of any sizes: a table in the kernel arguments maps each pair
-// to its pyramids' device descriptors, its top level and its slice of the point array. Workgroups are mapped to points
-// so that each of the 8 XCDs (which take workgroups in turn) works on one contiguous run of points, keeping
-// neighbouring windows in one L2.
+// Lane layout. A point owns one row of 16 lanes of a wavefront (DPP row), so a wavefront tracks 4 points. Lane j of
+// the row is the reference's vector lane j: for each 16-column block k of the window it owns window column 16k + j
+// and keeps that column's template values T, Tx, Ty (every window row) in registers for the whole level. Its three
+// patch sums run exactly as the reference's lane j runs them: fused multiply-adds over the rows, and within a row over
+// the blocks in order (lk_vx.c:82-94, 119-126, 160-168), and the 16 lane sums meet in the reference's butterfly
+// (partners i^1, i^2, i^4, i^8; vx_hsum, src/common/vx.h:213-221) through four DPP adds inside the row. Every fp32
+// value of the reference's patch arithmetic is produced by the same operations on the same inputs: the bilinear
+// window S (x-interpolation per image row, y-interpolation between rows), the Sobel gradients / 512, It = T - J, the
+// residual's |It| / 64 and (It / 64)^2. So the structure tensor, the mismatch vector and the residual sums equal the
+// reference's bit for bit. Lanes past the window's width (the last block of a window narrower than a whole block) get
+// zero gradients as in the reference (lk_vx.c:82-85) and read a column inside the window, so every value they touch is
+// finite.
+// The driver (level walk, eigenvalue test, cached inverse, 2x2 solve, convergence, bounds, covariance) runs in fp64
+// per lane on those sums; the reference's runs in x87 long double. The positions it produces are converted to the
+// float bilinear weights and integer corners exactly as the reference converts its own, and the residual's means are
+// rounded through a 64-bit significand first, as x87 rounds them (the exact path of ldnorm.hpp when a double
+// quotient is within reach of a float rounding boundary). Where fp64 and x87 would round a driver step differently
+// the records can differ; the tests compare every record with the reference's.
+// All 16 lanes of a row hold identical sums, so a point's control flow is uniform across its row; the 4 points of a
+// wavefront diverge by exec mask (a point done with a level waits for the others of its wavefront).
+// Batches. A launch covers points of up to 64 frame pairs of any sizes; a wavefront's 4 points always belong to one
+// pair (each pair's points are numbered from a multiple of 4), so its pyramid levels are wave-uniform and every level
+// is read through one buffer resource with the row step in a scalar offset. Workgroups are mapped to points so that
+// each of the 8 XCDs (which take workgroups in turn) works on one contiguous run of points, keeping neighbouring
+// windows in one L2.
+// Element types. Level values are read in their own units (u8: grey, u16 and f32: Q6, f16/bf16: grey) and carried at
+// that scale through the interpolation; the Q6 factor (a power of two) is applied where it is exact, so the results
+// equal those of interpolating Q6 values.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
+#include "ldnorm.hpp"
 #include "reduce_tile.hpp"
 #include "vtg_internal.hpp"
 
@@ -47,17 +51,19 @@ struct Cfg {
 
 constexpr int BMAX = 64; // pairs per launch (inline in the kernel arguments)
 constexpr int NXCD = 8;  // XCDs of the device; workgroup i runs on XCD i mod 8
+constexpr int PPW = 4;   // points per wavefront: one per row of 16 lanes
 
-// A launch's pairs: pair q's points are xy[start[q] ..] (global indices), numbered pre[q] .. pre[q + 1] - 1 in the
-// launch.
+// A launch's pairs: pair q's points are xy[start[q] .. start[q] + cnt[q]) (global indices), numbered
+// pre[q] .. pre[q] + cnt[q] - 1 in the launch; pre[q] is a multiple of PPW and slots up to pre[q + 1] are empty.
 struct LkArgs {
     const PyrView *I[BMAX], *J[BMAX];
     int top[BMAX];
     int start[BMAX];
+    int cnt[BMAX];
     int pre[BMAX + 1];
-    int npairs, n, per_xcd, waves;
+    int npairs, slots, per_xcd, waves;
     Cfg cfg;
-    int margin, lds_floats;
+    int solve;                  // VTG_SOLVE_*
     const float *xy, *guess;
     float fb_max;
     int fb_top;                 // the flowback's top level (-1: the pair's own)
@@ -68,15 +74,12 @@ struct LkArgs {
 template <typename T> struct Ld;
 template <> struct Ld<uint8_t> {
     static constexpr float scale = 64.0f;
-    static __device__ __forceinline__ float get(uint8_t v) { return (float)v; }
 };
 template <> struct Ld<uint16_t> {
     static constexpr float scale = 1.0f;
-    static __device__ __forceinline__ float get(uint16_t v) { return (float)v; }
 };
 template <> struct Ld<float> {
     static constexpr float scale = 1.0f;
-    static __device__ __forceinline__ float get(float v) { return v; }
 };
 template <> struct Ld<__half> {
     static constexpr float scale = 64.0f;
@@ -86,8 +89,6 @@ template <> struct Ld<__hip_bfloat16> {
     static constexpr float scale = 64.0f;
     static __device__ __forceinline__ float get(__hip_bfloat16 v) { return __bfloat162float(v); }
 };
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // A level read through a buffer resource: the base and size are wave-uniform (scalar registers), each load is the
 // resource plus a lane's 32-bit byte offset o plus a uniform offset so (a row step, kept in a scalar register), and a
@@ -177,309 +178,199 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t level_rsrc(const Level &lv, un
                                              (int)((unsigned)pitch * (unsigned)h * (unsigned)sizeof(T)), 0x00020000);
 }
 
-// LDS written by one lane and read by another lane of the same wave: order the accesses, no workgroup barrier.
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <int CTRL, int ROW_MASK>
+template <int CTRL>
 __device__ __forceinline__ float dpp_add(float v)
 {
-    const int moved = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false);
+    const int moved = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false);
     return v + __int_as_float(moved);
 }
 
-// Sum over the 64 lanes, the same value in every lane. Every lane must be active.
-__device__ __forceinline__ float wave_sum(float v)
+// The sum of the 16 lanes of the caller's row in the reference's butterfly order (vx_hsum): partners i^1 and i^2 by
+// quad permutes; after those every lane of a quad holds the same value, so the half-row mirror (lane 7 - i) brings the
+// value of lane i^4 and the row mirror (lane 15 - i) that of lane i^8. Each step adds the lane's own value and its
+// partner's, as the reference does, and fp32 addition commutes, so every lane of the row ends with the same bits.
+__device__ __forceinline__ float row_sum(float v)
 {
-    v = dpp_add<0xb1, 0xf>(v);  // quad_perm [1,0,3,2]
-    v = dpp_add<0x4e, 0xf>(v);  // quad_perm [2,3,0,1]
-    v = dpp_add<0x141, 0xf>(v); // row_half_mirror
-    v = dpp_add<0x140, 0xf>(v); // row_mirror: each row of 16 holds its sum
-    v = dpp_add<0x142, 0xa>(v); // row_bcast:15 into rows 1 and 3
-    v = dpp_add<0x143, 0xc>(v); // row_bcast:31 into rows 2 and 3: lane 63 holds the total
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    v = dpp_add<0xb1>(v);  // quad_perm [1,0,3,2]
+    v = dpp_add<0x4e>(v);  // quad_perm [2,3,0,1]
+    v = dpp_add<0x141>(v); // row_half_mirror
+    v = dpp_add<0x140>(v); // row_mirror
+    return v;
 }
 
-// This lane's place in the window: column k, its own rows [r0, r0 + nr), the MAXR rows it walks from re (inside the
-// window when h >= MAXR), and which of those are its own (bit i: row re + i).
-struct Geo {
-    int W, H, k, r0, nr, re;
-    unsigned on;
+// A lane's columns: per block k, the window column it reads (16k + j, or the window's last column for a lane past the
+// width) and whether it is a window column.
+template <int NV> struct Lanes {
+    unsigned col[NV]; // column offset in elements from the window's first column
+    bool on[NV];
 };
 
-template <int MAXR>
-__device__ __forceinline__ Geo geometry(const Cfg &c, int lane)
+template <int NV>
+__device__ __forceinline__ Lanes<NV> lanes(const Cfg &c, int j)
 {
-    Geo g;
-    g.W = c.w;
-    g.H = c.h;
-    const int groups = 64 / c.w, chunk = (c.h + groups - 1) / groups;
-    const int grp = lane / c.w;
-    g.k = lane - grp * c.w;
-    g.r0 = grp * chunk;
-    int nr = c.h - g.r0;
-    nr = nr < chunk ? nr : chunk;
-    g.nr = grp < groups && nr > 0 ? nr : 0;
-    const int last = c.h - MAXR > 0 ? c.h - MAXR : 0;
-    g.re = g.nr == 0 ? 0 : (g.r0 < last ? g.r0 : last);
-    g.on = 0;
+    Lanes<NV> ln;
 #pragma unroll
-    for (int i = 0; i < MAXR; i++)
-        if (g.re + i >= g.r0 && g.re + i < g.r0 + g.nr)
-            g.on |= 1u << i;
-    return g;
+    for (int k = 0; k < NV; k++) {
+        const int cc = 16 * k + j;
+        ln.on[k] = cc < c.w;
+        ln.col[k] = (unsigned)(cc < c.w ? cc : c.w - 1);
+    }
+    return ln;
 }
 
-template <int MAXR> struct Tile {
-    float t[MAXR], tx[MAXR], ty[MAXR];
+// The template of one level in registers: window rows 0 .. HMAX - 1 (the first h used) of the lane's NV columns.
+template <int NV, int HMAX> struct Tmpl {
+    float t[HMAX][NV], tx[HMAX][NV], ty[HMAX][NV];
 };
 
-// Sobel gradients of window rows re .. re + MAXR - 1 at the lane's column from the lane's three columns of S rows
-// re .. re + MAXR + 1 (values in units of `ks` Q6), with T = S interior; returns this lane's part of G. Rows that are not
-// the lane's own get zero gradients (build_vx, lk_vx.c:36-97).
-template <int MAXR>
-__device__ __forceinline__ void sobel_rows(const float (&S)[MAXR + 2][3], float ks, const Geo &g, Tile<MAXR> &tl,
-                                           float &sxx, float &sxy, float &syy)
+// The template at (px, py) (build_vx, lk_vx.c:36-97): the bilinear window S over rows -1 .. h and, per lane and block,
+// columns c - 1, c, c + 1 of its window column c (four taps of each image row, x-interpolated, then y-interpolated
+// with the row above), the Sobel gradients / 512 and T = S's middle; returns G = the butterfly totals of Tx^2,
+// Tx Ty, Ty^2 accumulated over rows, then blocks.
+template <int NV, int HMAX, bool EXACT, typename T>
+__device__ __forceinline__ void build(const Level &lv, double px, double py, const Cfg &c, const Lanes<NV> &ln,
+                                      Tmpl<NV, HMAX> &tp, float *G)
 {
-    const float kg = ks * (1.0f / 512.0f);
-#pragma unroll
-    for (int i = 0; i < MAXR; i++) {
-        const float *u = S[i], *m = S[i + 1], *d = S[i + 2];
-        const float du = u[2] - u[0], dm = m[2] - m[0], dd = d[2] - d[0];
-        float tx = fmaf(dm, 2.0f, du + dd) * kg;
-        const float vl = d[0] - u[0], vc = d[1] - u[1], vr = d[2] - u[2];
-        float ty = fmaf(vc, 2.0f, vl + vr) * kg;
-        const bool on = (g.on >> i) & 1u;
-        tx = on ? tx : 0.0f;
-        ty = on ? ty : 0.0f;
-        tl.t[i] = m[1] * ks;
-        tl.tx[i] = tx;
-        tl.ty[i] = ty;
-        sxx = fmaf(tx, tx, sxx);
-        sxy = fmaf(tx, ty, sxy);
-        syy = fmaf(ty, ty, syy);
-    }
-}
-
-// One iteration's (RES = false: b = totals of It Tx, It Ty; iterate_vx, lk_vx.c:100-140) or the residual's (RES = true:
-// totals of |It| / 64 and (It / 64)^2 over the lane's own rows; residual_vx, lk_vx.c:142-175) sums from the lane's
-// x-interpolated J rows A[0 .. MAXR] (units of `ks` Q6): J = y-interpolation of consecutive rows, It = T - J.
-template <int MAXR, bool RES>
-__device__ __forceinline__ void mismatch(const float (&A)[MAXR + 1], float fy, float ks, const Geo &g,
-                                         const Tile<MAXR> &tl, float *b)
-{
-    float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll
-    for (int i = 0; i < MAXR; i++) {
-        const float jv = fmaf(A[i + 1] - A[i], fy, A[i]);
-        const float it = fmaf(jv, -ks, tl.t[i]); // T - ks J, the product exact
-        if (RES) {
-            const bool on = (g.on >> i) & 1u;
-            const float q = it * (1.0f / 64.0f);
-            s0 = on ? s0 + fabsf(q) : s0;
-            s1 = on ? fmaf(q, q, s1) : s1;
-        } else {
-            s0 = fmaf(it, tl.tx[i], s0);
-            s1 = fmaf(it, tl.ty[i], s1);
-        }
-    }
-    b[0] = wave_sum(s0);
-    b[1] = wave_sum(s1);
-}
-
-// ---- the direct schedule
-
-// The template at (px, py), straight from the level into registers: returns the wave totals of Tx^2, Tx Ty, Ty^2.
-template <int MAXR, typename T>
-__device__ __forceinline__ void build_direct(const Level &lv, double px, double py, const Geo &g, Tile<MAXR> &tl,
-                                             float *G)
-{
+    constexpr float ks = Ld<T>::scale, kg = ks * (1.0f / 512.0f);
     const double flx = floor(px), fly = floor(py);
     const float fx = (float)(px - flx), fy = (float)(py - fly);
-    // S row 0, column 0 samples level pixel (flx - w/2 - 1 + fx, fly - h/2 - 1 + fy); this lane needs S columns
-    // k .. k + 2 of S rows re .. re + MAXR + 1, i.e. image rows re .. re + MAXR + 2 (at most h + 2) and 4 columns
     unsigned pitch;
     const __amdgpu_buffer_rsrc_t rs = level_rsrc<T>(lv, pitch);
-    const unsigned x0 = (unsigned)((int)flx - g.W / 2 - 1 + g.k) * (unsigned)sizeof(T);
-    const int y0 = (int)fly - g.H / 2 - 1;
-    float X[MAXR + 3][3];
-    if (g.H >= MAXR) {
-        // rows re .. re + MAXR + 2 lie inside the window's h + 3 rows: one lane offset, the row steps uniform
-        const unsigned o = (unsigned)(y0 + g.re) * pitch + x0;
+    // image row 0 is the level's row fly - h/2 - 1; tap 0 of window column c is level column flx - w/2 - 1 + c
+    const unsigned o = (unsigned)((int)fly - c.h / 2 - 1) * pitch + (unsigned)((int)flx - c.w / 2 - 1) * sizeof(T);
+    unsigned ob[NV];
 #pragma unroll
-        for (int i = 0; i < MAXR + 3; i++) {
-            float a[4];
-            Row<T>::quad(rs, o, (unsigned)i * pitch, a);
-            X[i][0] = fmaf(a[1] - a[0], fx, a[0]);
-            X[i][1] = fmaf(a[2] - a[1], fx, a[1]);
-            X[i][2] = fmaf(a[3] - a[2], fx, a[2]);
-        }
-    } else {
+    for (int k = 0; k < NV; k++)
+        ob[k] = o + ln.col[k] * (unsigned)sizeof(T);
+    float X[NV][3], S0[NV][3], S1[NV][3], d0[NV], d1[NV];
 #pragma unroll
-        for (int i = 0; i < MAXR + 3; i++) {
-            const int j = g.re + i < g.H + 2 ? g.re + i : g.H + 2;
-            float a[4];
-            Row<T>::quad(rs, (unsigned)(y0 + j) * pitch + x0, 0u, a);
-            X[i][0] = fmaf(a[1] - a[0], fx, a[0]);
-            X[i][1] = fmaf(a[2] - a[1], fx, a[1]);
-            X[i][2] = fmaf(a[3] - a[2], fx, a[2]);
+    for (int k = 0; k < NV; k++) {
+        float a[4];
+        Row<T>::quad(rs, ob[k], 0u, a);
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            X[k][q] = fmaf(a[q + 1] - a[q], fx, a[q]);
+    }
+    float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+    // a constant trip count, so the loop unrolls and the template stays in registers; rows past h (EXACT: h == HMAX,
+    // none) are skipped by a uniform branch
+#pragma unroll
+    for (int i = 1; i <= HMAX + 2; i++) {
+        if (EXACT || i <= c.h + 2) {
+            const int r = i - 1; // the S row this image row completes
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                float a[4], S[3];
+                Row<T>::quad(rs, ob[k], (unsigned)i * pitch, a);
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const float here = fmaf(a[q + 1] - a[q], fx, a[q]);
+                    S[q] = fmaf(here - X[k][q], fy, X[k][q]);
+                    X[k][q] = here;
+                }
+                const float dS = S[2] - S[0];
+                if (r == 0) {
+#pragma unroll
+                    for (int q = 0; q < 3; q++)
+                        S0[k][q] = S[q];
+                    d0[k] = dS;
+                } else if (r == 1) {
+#pragma unroll
+                    for (int q = 0; q < 3; q++)
+                        S1[k][q] = S[q];
+                    d1[k] = dS;
+                } else {
+                    // window row rr = r - 2 from S rows rr (S0), rr + 1 (S1), rr + 2 (S)
+                    const int rr = r - 2;
+                    float tx = fmaf(d1[k], 2.0f, d0[k] + dS) * kg;
+                    const float vl = S[0] - S0[k][0], vc = S[1] - S0[k][1], vr = S[2] - S0[k][2];
+                    float ty = fmaf(vc, 2.0f, vl + vr) * kg;
+                    tx = ln.on[k] ? tx : 0.0f;
+                    ty = ln.on[k] ? ty : 0.0f;
+                    tp.t[rr][k] = S1[k][1] * ks;
+                    tp.tx[rr][k] = tx;
+                    tp.ty[rr][k] = ty;
+                    sxx = fmaf(tx, tx, sxx);
+                    sxy = fmaf(tx, ty, sxy);
+                    syy = fmaf(ty, ty, syy);
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        S0[k][q] = S1[k][q];
+                        S1[k][q] = S[q];
+                    }
+                    d0[k] = d1[k];
+                    d1[k] = dS;
+                }
+            }
         }
     }
-    float S[MAXR + 2][3];
-#pragma unroll
-    for (int i = 0; i < MAXR + 2; i++)
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-            S[i][c] = fmaf(X[i + 1][c] - X[i][c], fy, X[i][c]);
-    float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
-    sobel_rows<MAXR>(S, Ld<T>::scale, g, tl, sxx, sxy, syy);
-    G[0] = wave_sum(sxx);
-    G[1] = wave_sum(sxy);
-    G[2] = wave_sum(syy);
+    G[0] = row_sum(sxx);
+    G[1] = row_sum(sxy);
+    G[2] = row_sum(syy);
 }
 
-// J at (qx, qy) straight from the level: rows re .. re + MAXR (at most h) of the window at floor(q), 2 taps each.
-template <int MAXR, typename T, bool RES>
-__device__ __forceinline__ void sample_direct(const Level &lv, double qx, double qy, const Geo &g,
-                                              const Tile<MAXR> &tl, float *b)
+// J sampled at (qx, qy) over the window (iterate_vx, lk_vx.c:100-140; residual_vx, lk_vx.c:142-175): two taps of each
+// of rows 0 .. h of the window at floor(q), x-interpolated, the row above carried; It = T - J. RES = false: b = the
+// butterfly totals of It Tx and It Ty; RES = true: of |It| / 64 and (It / 64)^2, lanes past the width masked.
+template <int NV, int HMAX, bool EXACT, typename T, bool RES>
+__device__ __forceinline__ void sample(const Level &lv, double qx, double qy, const Cfg &c, const Lanes<NV> &ln,
+                                       const Tmpl<NV, HMAX> &tp, float *b)
 {
+    constexpr float ks = Ld<T>::scale;
     const double flx = floor(qx), fly = floor(qy);
     const float fx = (float)(qx - flx), fy = (float)(qy - fly);
     unsigned pitch;
     const __amdgpu_buffer_rsrc_t rs = level_rsrc<T>(lv, pitch);
-    const unsigned x = (unsigned)((int)flx - g.W / 2 + g.k) * (unsigned)sizeof(T);
-    const int y0 = (int)fly - g.H / 2;
-    float A[MAXR + 1];
-    if (g.H >= MAXR) {
-        // rows re .. re + MAXR lie inside the window's h + 1 rows: one lane offset, the row steps uniform
-        const unsigned o = (unsigned)(y0 + g.re) * pitch + x;
+    const unsigned o = (unsigned)((int)fly - c.h / 2) * pitch + (unsigned)((int)flx - c.w / 2) * sizeof(T);
+    unsigned ob[NV];
+    float prev[NV];
 #pragma unroll
-        for (int i = 0; i <= MAXR; i++) {
-            float a0, a1;
-            Row<T>::pair(rs, o, (unsigned)i * pitch, a0, a1);
-            A[i] = fmaf(a1 - a0, fx, a0);
-        }
-    } else {
+    for (int k = 0; k < NV; k++) {
+        ob[k] = o + ln.col[k] * (unsigned)sizeof(T);
+        float a0, a1;
+        Row<T>::pair(rs, ob[k], 0u, a0, a1);
+        prev[k] = fmaf(a1 - a0, fx, a0);
+    }
+    float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
-        for (int i = 0; i <= MAXR; i++) {
-            const int j = g.re + i < g.H ? g.re + i : g.H;
-            float a0, a1;
-            Row<T>::pair(rs, (unsigned)(y0 + j) * pitch + x, 0u, a0, a1);
-            A[i] = fmaf(a1 - a0, fx, a0);
+    for (int r = 0; r < HMAX; r++) {
+        if (EXACT || r < c.h) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                float a0, a1;
+                Row<T>::pair(rs, ob[k], (unsigned)(r + 1) * pitch, a0, a1);
+                const float cur = fmaf(a1 - a0, fx, a0);
+                const float jv = fmaf(cur - prev[k], fy, prev[k]);
+                const float it = fmaf(jv, -ks, tp.t[r][k]); // T - J, the product by ks exact
+                prev[k] = cur;
+                if (RES) {
+                    const float q = ln.on[k] ? it * (1.0f / 64.0f) : 0.0f;
+                    s0 = s0 + fabsf(q);
+                    s1 = fmaf(q, q, s1);
+                } else {
+                    s0 = fmaf(it, tp.tx[r][k], s0);
+                    s1 = fmaf(it, tp.ty[r][k], s1);
+                }
+            }
         }
     }
-    mismatch<MAXR, RES>(A, fy, Ld<T>::scale, g, tl, b);
+    b[0] = row_sum(s0);
+    b[1] = row_sum(s1);
 }
 
-// ---- the staged schedule
-
-// A rectangle of a level copied into LDS: rows ry .. ry + rh - 1, columns rx .. rx + rw - 1, coordinates clamped to the
-// level (the reference's replicated border).
-struct Region {
-    int rx, ry, rw, rh;
-};
-
-// The wave's LDS slice: P = the template's patch of I ((h + 3) rows), directly followed by R = J's neighbourhood (both
-// rows of the region width w + 1 + 2 m), and S = the bilinear window ((w + 2) x (h + 2)).
-struct Slice {
-    float *P, *S, *R;
-};
-
-// The wave's loads of one level, issued together, as Q6 values: rows 0 .. np - 1 of the combined region come from
-// level li at (rp.rx, rp.ry) (the template patch, np = 0 to skip it), the next rg.rh rows from level lj at
-// (rg.rx, rg.ry) (J's neighbourhood); every row is rg.rw wide. Element i = lane + 64 k sits at row i / rw, column
-// i % rw, stepped without a division per element.
-template <typename T, int BATCH>
-__device__ __forceinline__ void load_regions(const Level &li, const Region &rp, int np, const Level &lj,
-                                             const Region &rg, float *dst, int lane)
+// (float)((long double)s / n) for s >= 0: the quotient rounded to a 64-bit significand, then to float
+// (residual_vx, lk_vx.c:171-174). The double quotient rounds to the same float unless it lies within one of its own
+// ulps of a float rounding boundary; then the exact 128-bit path decides.
+__device__ __forceinline__ float mean_ld(float s, int n)
 {
-    const int rw = rg.rw, n = (np + rg.rh) * rw;
-    const int dq = 64 / rw, dr = 64 - dq * rw;
-    int r = lane / rw, cc = lane - (lane / rw) * rw;
-    const T *pi = (const T *)li.p, *pj = (const T *)lj.p;
-    wave_sync(); // earlier readers of the slice are done
-    for (int base = 0; base < n; base += 64 * BATCH) {
-        float v[BATCH];
-        int rr = r, c2 = cc;
-#pragma unroll
-        for (int k = 0; k < BATCH; k++) {
-            const bool fromI = rr < np;
-            const int y = fromI ? clampi(rp.ry + rr, 0, li.h - 1) : clampi(rg.ry + rr - np, 0, lj.h - 1);
-            const int x = fromI ? clampi(rp.rx + c2, 0, li.w - 1) : clampi(rg.rx + c2, 0, lj.w - 1);
-            v[k] = Ld<T>::get(fromI ? pi[(size_t)y * li.pitch + x] : pj[(size_t)y * lj.pitch + x]) * Ld<T>::scale;
-            c2 += dr;
-            rr += dq;
-            if (c2 >= rw)
-                c2 -= rw, rr++;
-        }
-#pragma unroll
-        for (int k = 0; k < BATCH; k++) {
-            const int i = base + lane + 64 * k;
-            if (i < n)
-                dst[i] = v[k];
-        }
-        r = rr;
-        cc = c2;
-    }
-    wave_sync();
-}
-
-// The template at (px, py) from the staged patch P (origin = the window's corner minus one row and column): S into
-// LDS, then the lane's rows as in the direct schedule.
-template <int MAXR>
-__device__ __forceinline__ void build(const Slice &sl, int PW, float fx, float fy, const Geo &g, int lane,
-                                      Tile<MAXR> &tl, float *G)
-{
-    const int SW = g.W + 2, SH = g.H + 2;
-    for (int i = lane; i < SW * SH; i += 64) {
-        const int r = i / SW, cc = i - r * SW;
-        const float *p = sl.P + r * PW + cc;
-        const float top = fmaf(p[1] - p[0], fx, p[0]), bot = fmaf(p[PW + 1] - p[PW], fx, p[PW]);
-        sl.S[i] = fmaf(bot - top, fy, top);
-    }
-    wave_sync();
-    float S[MAXR + 2][3];
-#pragma unroll
-    for (int i = 0; i < MAXR + 2; i++) {
-        const int j = g.re + i < SH - 1 ? g.re + i : SH - 1;
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-            S[i][c] = sl.S[j * SW + g.k + c];
-    }
-    float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
-    sobel_rows<MAXR>(S, 1.0f, g, tl, sxx, sxy, syy);
-    G[0] = wave_sum(sxx);
-    G[1] = wave_sum(sxy);
-    G[2] = wave_sum(syy);
-}
-
-// Whether the window at integer corner (x0, y0), with its extra column and row, lies inside the region.
-__device__ __forceinline__ bool covers(const Region &rg, int x0, int y0, int W, int H)
-{
-    return x0 >= rg.rx && y0 >= rg.ry && x0 + W <= rg.rx + rg.rw - 1 && y0 + H <= rg.ry + rg.rh - 1;
-}
-
-// J sampled at (qx, qy) from the staged neighbourhood R (Q6 values).
-template <int MAXR, bool RES>
-__device__ __forceinline__ void sample(const float *R, const Region &rg, double qx, double qy, const Geo &g,
-                                       const Tile<MAXR> &tl, float *b)
-{
-    const double flx = floor(qx), fly = floor(qy);
-    const float fx = (float)(qx - flx), fy = (float)(qy - fly);
-    const int x = clampi((int)flx - g.W / 2 + g.k - rg.rx, 0, rg.rw - 2);
-    const int y0 = (int)fly - g.H / 2 - rg.ry;
-    float A[MAXR + 1];
-#pragma unroll
-    for (int i = 0; i <= MAXR; i++) {
-        const int j = g.re + i < g.H ? g.re + i : g.H;
-        const float *p = R + clampi(y0 + j, 0, rg.rh - 1) * rg.rw + x;
-        A[i] = fmaf(p[1] - p[0], fx, p[0]);
-    }
-    mismatch<MAXR, RES>(A, fy, 1.0f, g, tl, b);
+    if (!(s > 0.0f))
+        return 0.0f;
+    const double d = (double)s / (double)n;
+    const uint64_t bits = (uint64_t)__double_as_longlong(d);
+    const int64_t low = (int64_t)(bits & ((1ull << 29) - 1)) - (1ll << 28); // distance to the float midpoint pattern
+    if (d > 0x1p-120 && (low > 2 || low < -2))
+        return (float)d;
+    return xl_div_to_float(xl_add(XL{0, 0}, s), n);
 }
 
 struct Track {
@@ -487,16 +378,15 @@ struct Track {
     int iters, status;
 };
 
-// The 2x2 system of one level: eigenvalue test and inverse. MIXED: fp32 unless G's condition number passes 1e4.
+// The 2x2 system of one level: eigenvalue test and inverse. mixed: fp32 unless G's condition number passes 1e4.
 struct Solve {
     double ia, ib, ic, det, eig;
 };
 
-template <int SOLVE>
-__device__ __forceinline__ Solve solve2(double gxx, double gxy, double gyy, double area)
+__device__ __forceinline__ Solve solve2(double gxx, double gxy, double gyy, double area, bool mixed)
 {
     Solve s;
-    if (SOLVE == VTG_SOLVE_MIXED) {
+    if (mixed) {
         const float fxx = (float)gxx, fxy = (float)gxy, fyy = (float)gyy;
         const float tr = fxx + fyy, df = fxx - fyy, rt = sqrtf(df * df + 4.0f * fxy * fxy);
         const float lo = 0.5f * (tr - rt), hi = 0.5f * (tr + rt);
@@ -514,37 +404,39 @@ __device__ __forceinline__ Solve solve2(double gxx, double gxy, double gyy, doub
     s.det = gxx * gyy - gxy * gxy;
     const double tr = gxx + gyy, df = gxx - gyy;
     s.eig = (tr - sqrt(df * df + 4.0 * gxy * gxy)) * 0.5 / area;
-    const double inv = 1.0 / s.det;
-    s.ia = gyy * inv;
-    s.ib = -gxy * inv;
-    s.ic = gxx * inv;
+    s.ia = gyy / s.det;
+    s.ib = -gxy / s.det;
+    s.ic = gxx / s.det;
     return s;
 }
 
-// Load the level's template patch (with_patch) and J's neighbourhood together (T: the level's element type).
-template <typename T, int BATCH>
-__device__ __forceinline__ void level_loads(const Level &li, const Region &rp, const Level &lj, const Region &rg,
-                                            const Slice &sl, bool with_patch, int lane)
+template <int NV, int HMAX, bool EXACT, typename T0, typename T1>
+__device__ __forceinline__ void build_at(int L, const Level &lv, double px, double py, const Cfg &c,
+                                         const Lanes<NV> &ln, Tmpl<NV, HMAX> &tp, float *G)
 {
-    if (with_patch)
-        load_regions<T, BATCH>(li, rp, rp.rh, lj, rg, sl.P, lane);
+    if (std::is_same<T0, T1>::value || L > 0)
+        build<NV, HMAX, EXACT, T1>(lv, px, py, c, ln, tp, G);
     else
-        load_regions<T, BATCH>(li, rp, 0, lj, rg, sl.R, lane);
+        build<NV, HMAX, EXACT, T0>(lv, px, py, c, ln, tp, G);
 }
 
-// One point from pyramid I to J: the reference's level walk (lk_driver.inc:7-109).
-template <int MAXR, typename T0, typename T1, int SOLVE, bool STAGED>
-__device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c, const Geo &g, int lane,
-                      const Slice &sl, int margin, float x, float y, float gx0, float gy0, Track &o)
+template <int NV, int HMAX, bool EXACT, typename T0, typename T1>
+__device__ __forceinline__ void sample_at(int L, const Level &lv, double qx, double qy, const Cfg &c,
+                                          const Lanes<NV> &ln, const Tmpl<NV, HMAX> &tp, float *b)
 {
-    constexpr int BATCH = MAXR <= 8 ? 26 : 16;
-    Region rg, rp;
-    rg.rw = c.w + 1 + 2 * margin;
-    rg.rh = c.h + 1 + 2 * margin;
-    rg.rx = rg.ry = INT_MIN; // nothing staged
-    rp.rw = rg.rw; // the patch needs w + 3 columns; it shares the region's row width (margin >= 1)
-    rp.rh = c.h + 3;
-    Tile<MAXR> tl;
+    if (std::is_same<T0, T1>::value || L > 0)
+        sample<NV, HMAX, EXACT, T1, false>(lv, qx, qy, c, ln, tp, b);
+    else
+        sample<NV, HMAX, EXACT, T0, false>(lv, qx, qy, c, ln, tp, b);
+}
+
+// One point from pyramid I to J: the reference's level walk (lk_driver.inc:7-109). Every value is the same in the 16
+// lanes of the point's row.
+template <int NV, int HMAX, bool EXACT, typename T0, typename T1>
+__device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c, bool mixed, const Lanes<NV> &ln,
+                      float x, float y, float gx0, float gy0, Track &o)
+{
+    Tmpl<NV, HMAX> tp;
     double gx = ldexp((double)gx0, -top), gy = ldexp((double)gy0, -top);
     const double area = (double)(c.w * c.h), eps2 = (double)c.eps * (double)c.eps;
     const int mx = c.w / 2 + 1, my = c.h / 2 + 1;
@@ -563,25 +455,9 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             continue;
         }
         float G[3];
-        if (STAGED) {
-            // the template patch and J around the level's first estimate, one batch
-            const double flx = floor(px), fly = floor(py);
-            rp.rx = (int)flx - c.w / 2 - 1;
-            rp.ry = (int)fly - c.h / 2 - 1;
-            rg.rx = (int)floor(px + gx) - c.w / 2 - margin;
-            rg.ry = (int)floor(py + gy) - c.h / 2 - margin;
-            if (L > 0)
-                level_loads<T1, BATCH>(li, rp, lj, rg, sl, true, lane);
-            else
-                level_loads<T0, BATCH>(li, rp, lj, rg, sl, true, lane);
-            build<MAXR>(sl, rg.rw, (float)(px - flx), (float)(py - fly), g, lane, tl, G);
-        } else if (L > 0) {
-            build_direct<MAXR, T1>(li, px, py, g, tl, G);
-        } else {
-            build_direct<MAXR, T0>(li, px, py, g, tl, G);
-        }
+        build_at<NV, HMAX, EXACT, T0, T1>(L, li, px, py, c, ln, tp, G);
         const double gxx = G[0], gxy = G[1], gyy = G[2];
-        const Solve s = solve2<SOLVE>(gxx, gxy, gyy, area);
+        const Solve s = solve2(gxx, gxy, gyy, area, mixed);
         if (L == 0)
             eig0 = (float)s.eig;
         if (s.eig < (double)c.min_eig || s.det <= 0) {
@@ -596,35 +472,19 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             valid0 = true;
         }
         const double lx = (double)(lj.w - mx - 1), ly = (double)(lj.h - my - 1);
-        // the inverse with the sums' 1/64 folded in (a power of two: the products round exactly as b / 64 would)
-        const double ka = s.ia * (1.0 / 64.0), kb = s.ib * (1.0 / 64.0), kc = s.ic * (1.0 / 64.0);
+        const double sx = px + gx, sy = py + gy; // the reference adds px + gx first, then the level's step
         double vx = 0, vy = 0;
         for (int it = 0; it < c.max_iter; it++) {
-            const double qx = px + gx + vx, qy = py + gy + vy;
+            const double qx = sx + vx, qy = sy + vy;
             if (qx < mx || qy < my || qx >= lx || qy >= ly) {
                 if (L == 0)
                     status |= LK_OUT, valid0 = false;
                 break;
             }
             float b[2];
-            if (STAGED) {
-                const int wx = (int)floor(qx) - c.w / 2, wy = (int)floor(qy) - c.h / 2;
-                if (!covers(rg, wx, wy, c.w, c.h)) {
-                    rg.rx = wx - margin;
-                    rg.ry = wy - margin;
-                    if (L > 0)
-                        level_loads<T1, BATCH>(li, rp, lj, rg, sl, false, lane);
-                    else
-                        level_loads<T0, BATCH>(li, rp, lj, rg, sl, false, lane);
-                }
-                sample<MAXR, false>(sl.R, rg, qx, qy, g, tl, b);
-            } else if (L > 0) {
-                sample_direct<MAXR, T1, false>(lj, qx, qy, g, tl, b);
-            } else {
-                sample_direct<MAXR, T0, false>(lj, qx, qy, g, tl, b);
-            }
-            const double bx = (double)b[0], by = (double)b[1];
-            const double ex = ka * bx + kb * by, ey = kb * bx + kc * by;
+            sample_at<NV, HMAX, EXACT, T0, T1>(L, lj, qx, qy, c, ln, tp, b);
+            const double bx = (double)b[0] / 64.0, by = (double)b[1] / 64.0;
+            const double ex = s.ia * bx + s.ib * by, ey = s.ib * bx + s.ic * by;
             vx += ex;
             vy += ey;
             iters++;
@@ -649,23 +509,12 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             status |= LK_OUT;
         } else {
             float r[2];
-            if (STAGED) {
-                // level 0 was the last level walked, so rg describes level 0's neighbourhood in R
-                const int wx = (int)floor(qx) - c.w / 2, wy = (int)floor(qy) - c.h / 2;
-                if (!covers(rg, wx, wy, c.w, c.h)) {
-                    rg.rx = wx - margin;
-                    rg.ry = wy - margin;
-                    level_loads<T0, BATCH>(I.l[0], rp, l0, rg, sl, false, lane);
-                }
-                sample<MAXR, true>(sl.R, rg, qx, qy, g, tl, r);
-            } else {
-                sample_direct<MAXR, T0, true>(l0, qx, qy, g, tl, r);
-            }
-            const double n = area;
-            o.residual = (float)((double)r[0] / n);
+            sample<NV, HMAX, EXACT, T0, true>(l0, qx, qy, c, ln, tp, r);
+            const int n = c.w * c.h;
+            o.residual = mean_ld(r[0], n);
             if (o.residual > c.max_res)
                 status |= LK_RESIDUAL;
-            const double s2 = ((double)r[1] / n) * area / (area - 2);
+            const double s2 = (double)mean_ld(r[1], n) * area / (area - 2);
             const double gxx0 = G0[0], gxy0 = G0[1], gyy0 = G0[2], det0 = gxx0 * gyy0 - gxy0 * gxy0;
             o.cxx = (float)(s2 * gyy0 / det0);
             o.cxy = (float)(-s2 * gxy0 / det0);
@@ -675,91 +524,118 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
     o.status = status;
 }
 
-// The direct schedule of windows up to 7 rows per lane fits 128 VGPRs without spilling: bound it to 4 waves per SIMD
-// (latency hiding is what the iterations need); the other forms keep the compiler's allocation.
-template <int MAXR, typename T0, typename T1, int SOLVE, bool STAGED>
-__global__ __launch_bounds__(512, (!STAGED && MAXR <= 7) ? 4 : 1) void lk_kernel(LkArgs a)
+// A point and its flowback (lk_driver.inc:111-121): the second pass tracks the estimate back from J to I, the template
+// rebuilt from J. One call site of track for both passes.
+template <int NV, int HMAX, bool EXACT, typename T0, typename T1>
+__device__ __forceinline__ void track_fb(const PyrView &I, const PyrView &J, int top, int fb_top, const Cfg &c,
+                                         bool mixed, float fb_max, const Lanes<NV> &ln, float x, float y, float gx,
+                                         float gy, Track &o)
 {
-    extern __shared__ float smem[];
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    // workgroup b runs on XCD b mod 8; XCD x takes the contiguous run of workgroups [x per_xcd, (x + 1) per_xcd)
-    const int wg = (int)(blockIdx.x % NXCD) * a.per_xcd + (int)(blockIdx.x / NXCD);
-    const int lp = wg * a.waves + wave;
-    if (lp >= a.n)
-        return;
-    // this point's pair: the last whose numbering starts at or before it (uniform per wave)
-    int lo = 0, hi = a.npairs - 1;
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1 && !(fb_max > 0.0f && o.status == 0))
+            break;
+        Track t;
+        const float sx = pass ? o.x : x, sy = pass ? o.y : y;
+        const float ux = pass ? (float)((double)x - (double)o.x) : gx;
+        const float uy = pass ? (float)((double)y - (double)o.y) : gy;
+        track<NV, HMAX, EXACT, T0, T1>(pass ? J : I, pass ? I : J, pass ? fb_top : top, c, mixed, ln, sx, sy, ux, uy, t);
+        if (pass == 0) {
+            o = t;
+        } else {
+            const double ex = (double)t.x - (double)x, ey = (double)t.y - (double)y;
+            if (t.status || ex * ex + ey * ey > (double)fb_max * (double)fb_max)
+                o.status |= LK_FLOWBACK;
+        }
+    }
+}
+
+__device__ __forceinline__ void store(lk_out *out, const Track &o)
+{
+    lk_out r;
+    r.x = o.x;
+    r.y = o.y;
+    r.min_eig = o.min_eig;
+    r.residual = o.residual;
+    r.cov_xx = o.cxx;
+    r.cov_xy = o.cxy;
+    r.cov_yy = o.cyy;
+    r.iters = (uint16_t)o.iters;
+    r.status = (uint16_t)o.status;
+    *out = r;
+}
+
+// The last pair whose numbering starts at or before slot s (uniform when s is).
+__device__ __forceinline__ int pair_of(const int *pre, int npairs, int s)
+{
+    int lo = 0, hi = npairs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (lp >= a.pre[mid])
+        if (s >= pre[mid])
             lo = mid;
         else
             hi = mid - 1;
     }
-    const int pt = a.start[lo] + (lp - a.pre[lo]);
-    const PyrView &I = *a.I[lo], &J = *a.J[lo];
-    const int top = a.top[lo];
-    const int rw = a.cfg.w + 1 + 2 * a.margin, ph = a.cfg.h + 3;
-    float *base = smem + wave * a.lds_floats;
-    // layout: S first, then P and R contiguous (one combined region)
-    const int sS = (a.cfg.w + 2) * (a.cfg.h + 2);
-    const Slice sl{base + sS, base, base + sS + rw * ph};
-    const Geo g = geometry<MAXR>(a.cfg, lane);
-    const float x = a.xy[2 * pt], y = a.xy[2 * pt + 1];
-    const float gx = a.guess ? a.guess[2 * pt] : 0.0f, gy = a.guess ? a.guess[2 * pt + 1] : 0.0f;
-    Track o;
-    track<MAXR, T0, T1, SOLVE, STAGED>(I, J, top, a.cfg, g, lane, sl, a.margin, x, y, gx, gy, o);
-    if (a.fb_max > 0.0f && o.status == 0) {
-        // back from J to I, the template rebuilt from J (lk_driver.inc:111-121)
-        Track b;
-        const int fb_top = a.fb_top >= 0 ? a.fb_top : top;
-        track<MAXR, T0, T1, SOLVE, STAGED>(J, I, fb_top, a.cfg, g, lane, sl, a.margin, o.x, o.y,
-                                           (float)((double)x - (double)o.x), (float)((double)y - (double)o.y), b);
-        const double ex = (double)b.x - (double)x, ey = (double)b.y - (double)y;
-        if (b.status || ex * ex + ey * ey > (double)a.fb_max * (double)a.fb_max)
-            o.status |= LK_FLOWBACK;
-    }
-    if (lane == 0) {
-        lk_out r;
-        r.x = o.x;
-        r.y = o.y;
-        r.min_eig = o.min_eig;
-        r.residual = o.residual;
-        r.cov_xx = o.cxx;
-        r.cov_xy = o.cxy;
-        r.cov_yy = o.cyy;
-        r.iters = (uint16_t)o.iters;
-        r.status = (uint16_t)o.status;
-        a.out[pt] = r;
-    }
+    return lo;
 }
 
-template <int MAXR, int SOLVE, bool STAGED>
-int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 block, size_t lds)
+template <int NV, int HMAX, bool EXACT, typename T0, typename T1>
+__global__ __launch_bounds__(512) void lk_kernel(LkArgs a)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    // workgroup b runs on XCD b mod 8; XCD x takes the contiguous run of workgroups [x per_xcd, (x + 1) per_xcd)
+    const int wg = (int)(blockIdx.x % NXCD) * a.per_xcd + (int)(blockIdx.x / NXCD);
+    const int s0 = (wg * a.waves + wave) * PPW;
+    if (s0 >= a.slots)
+        return;
+    const int q = __builtin_amdgcn_readfirstlane(pair_of(a.pre, a.npairs, s0));
+    const int local = s0 - a.pre[q] + (lane >> 4);
+    const bool live = local < a.cnt[q];
+    const int pt = a.start[q] + (live ? local : 0);
+    // an empty slot starts far outside every level: it fits none, reads nothing and writes nothing
+    float x = -1e30f, y = -1e30f, gx = 0.0f, gy = 0.0f;
+    if (live) {
+        x = a.xy[2 * pt];
+        y = a.xy[2 * pt + 1];
+        if (a.guess) {
+            gx = a.guess[2 * pt];
+            gy = a.guess[2 * pt + 1];
+        }
+    }
+    const Lanes<NV> ln = lanes<NV>(a.cfg, lane & 15);
+    const int top = a.top[q];
+    Track o;
+    track_fb<NV, HMAX, EXACT, T0, T1>(*a.I[q], *a.J[q], top, a.fb_top >= 0 ? a.fb_top : top, a.cfg,
+                               a.solve == VTG_SOLVE_MIXED, a.fb_max, ln, x, y, gx, gy, o);
+    if (live && (lane & 15) == 0)
+        store(a.out + pt, o);
+}
+
+// The compiled window shapes: NV blocks of 16 columns and HMAX rows. The windows of every workload here, 15 x 15 and
+// 21 x 21, have their own exact shapes (straight-line code); any other window runs the guarded form of its block
+// count with 31 rows.
+template <int NV, int HMAX, bool EXACT>
+int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 block)
 {
     switch (storage) {
     case VTG_STORE_Q6:
-        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, uint16_t, SOLVE, STAGED>), grid, block, lds, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, uint8_t, uint16_t>), grid, block, 0, c->stream, a);
         break;
     case VTG_STORE_F32:
-        hipLaunchKernelGGL((lk_kernel<MAXR, float, float, SOLVE, STAGED>), grid, block, lds, c->stream, a);
+    case VTG_STORE_F32_U8:
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, float, float>), grid, block, 0, c->stream, a);
         break;
     case VTG_STORE_Q6W:
-        hipLaunchKernelGGL((lk_kernel<MAXR, uint16_t, uint16_t, SOLVE, STAGED>), grid, block, lds, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, uint16_t, uint16_t>), grid, block, 0, c->stream, a);
         break;
     case VTG_STORE_Q6F:
-        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, float, SOLVE, STAGED>), grid, block, lds, c->stream, a);
-        break;
-    case VTG_STORE_F32_U8:
-        hipLaunchKernelGGL((lk_kernel<MAXR, float, float, SOLVE, STAGED>), grid, block, lds, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, uint8_t, float>), grid, block, 0, c->stream, a);
         break;
     case VTG_STORE_F16:
-        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __half, SOLVE, STAGED>), grid, block, lds, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, uint8_t, __half>), grid, block, 0, c->stream, a);
         break;
     case VTG_STORE_BF16:
-        hipLaunchKernelGGL((lk_kernel<MAXR, uint8_t, __hip_bfloat16, SOLVE, STAGED>), grid, block, lds, c->stream,
-                           a);
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, uint8_t, __hip_bfloat16>), grid, block, 0, c->stream, a);
         break;
     default:
         return fail(c, VTG_EINVAL, "storage %d", storage);
@@ -768,21 +644,36 @@ int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 b
     return VTG_OK;
 }
 
+int dispatch_shape(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 block)
+{
+    const int nv = (a.cfg.w + 15) / 16;
+    if (nv == 1 && a.cfg.h == 15)
+        return dispatch_storage<1, 15, true>(c, a, storage, grid, block);
+    if (nv == 2 && a.cfg.h == 21)
+        return dispatch_storage<2, 21, true>(c, a, storage, grid, block);
+    if (nv == 1)
+        return dispatch_storage<1, 31, false>(c, a, storage, grid, block);
+    if (nv == 2)
+        return dispatch_storage<2, 31, false>(c, a, storage, grid, block);
+    return dispatch_storage<3, 31, false>(c, a, storage, grid, block);
+}
+
 // ---------------------------------------------------------------- fused pyramid + LK, one persistent launch
 
 // The batched tk_build + OP_LK (track.c:584-629, 275-292) as one launch: work items in the order of their
 // dependencies: bands of 16 output rows of every pair's level 1 (which also write the frames into level 0), then of
-// every pair's level 2, ..., then the points. Workgroups take items from a counter in that order (a workgroup's first
-// point turns its four waves into independent takers of one point each, as no band follows). A level-l band waits until
-// its pyramid's level l - 1 is complete; a wave's first point waits until every pyramid is complete, then the wave
-// acquires once for all its points (an agent-scope acquire invalidates the XCD's L2, which LK's overlapping windows
-// live in, so one per pair change costs far more than the wait). Every item waited for was taken before the waiter's
-// own, by a workgroup that is running and waits only for earlier items, so the earliest unfinished item always
-// progresses. There is one queue per XCD (workgroup w serves queue w mod 8, the XCD it runs on), each in that same order
-// of classes (level 1, level 2, ..., then the points): a level's bands are dealt round-robin over the queues and the
-// queue's points are the q-th of 8 contiguous runs of all points, so an XCD's LK windows stay in a few pairs. The
-// earliest unfinished class has all its dependencies finished and its items in every queue either finished or held by
-// running workgroups (the grid is at most what is resident at once), so the queues cannot deadlock either.
+// every pair's level 2, ..., then the points in blocks of 4 (one wavefront each). Workgroups take items from a counter
+// in that order (a workgroup's first point block turns its four waves into independent takers of one block each, as
+// no band follows). A level-l band waits until its pyramid's level l - 1 is complete; a wave's first block waits
+// until every pyramid is complete, then the wave acquires once for all its blocks (an agent-scope acquire invalidates
+// the XCD's L2, which LK's overlapping windows live in, so one per pair change costs far more than the wait). Every
+// item waited for was taken before the waiter's own, by a workgroup that is running and waits only for earlier items,
+// so the earliest unfinished item always progresses. There is one queue per XCD (workgroup w serves queue w mod 8, the
+// XCD it runs on), each in that same order of classes (level 1, level 2, ..., then the points): a level's bands are
+// dealt round-robin over the queues and the queue's point blocks are the q-th of 8 contiguous runs of all blocks, so
+// an XCD's LK windows stay in a few pairs. The earliest unfinished class has all its dependencies finished and its
+// items in every queue either finished or held by running workgroups (the grid is at most what is resident at once),
+// so the queues cannot deadlock either.
 // A band publishes with an agent-scope release (every thread) before its counts, a waiter reads a count relaxed and then
 // acquires, so levels written on any XCD are seen fresh. A wait gives up after a bounded number of sleeps, setting the
 // context's error word (the results are then not valid), so every wave reaches the exit.
@@ -844,13 +735,12 @@ __global__ __launch_bounds__(256) void fused_reset(int *ctr, int n, int clear_er
 // Bands of a level of one side in queue q: b = q, q + 8, ...
 __device__ __forceinline__ int queue_bands(int nb, int q) { return nb > q ? (nb - q + FQ - 1) / FQ : 0; }
 
-
-template <int MAXR, int SOLVE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fused_kernel(FusedArgs a)
+template <int NV, int HMAX, bool EXACT>
+__global__ __launch_bounds__(256) void fused_kernel(FusedArgs a)
 {
     __shared__ int s_item;
     // this workgroup's queue: first the bands of every pair's level 1, then of level 2, ... (qa[(l - 1) np + k]: the
-    // first item of level l of pair k, both sides), then the q-th of 8 contiguous runs of all points (pair by pair)
+    // first item of level l of pair k, both sides), then the q-th of 8 contiguous runs of all point blocks
     __shared__ int qa[(kMaxLevels - 1) * FB + 1];
     __shared__ int lds[rt::tile_lds_elems<FBAND>()];
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -872,9 +762,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
     }
     __syncthreads();
     const int npa = (nl - 1) * np, nbands = __builtin_amdgcn_readfirstlane(qa[npa]);
-    const int run = (a.lk.n + FQ - 1) / FQ, pfirst = q * run;
-    const int plast = pfirst + run < a.lk.n ? pfirst + run : a.lk.n;
-    const int total = nbands + (plast > pfirst ? plast - pfirst : 0);
+    const int blocks = a.lk.slots / PPW, run = (blocks + FQ - 1) / FQ, bfirst = q * run;
+    const int blast = bfirst + run < blocks ? bfirst + run : blocks;
+    const int total = nbands + (blast > bfirst ? blast - bfirst : 0);
     for (;;) {
         if (wave == 0) // uniform: wave 0 takes the next item and every lane of it stores the same value
             s_item = wave_fetch_add(a.ctr + q, 1, lane);
@@ -947,74 +837,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
             }
             continue;
         }
-        // Every later item of this queue is an LK item (one point): the waves now take them one at a time, each on its
+        // Every later item of this queue is a block of 4 points: the waves now take them one at a time, each on its
         // own (wave 0 starts with the item just taken), and no longer meet at barriers.
         int it = wave == 0 ? item : -1;
         bool ready = false;
+        const Lanes<NV> ln = lanes<NV>(a.lk.cfg, lane & 15);
         for (;;) {
             if (it < 0)
                 it = wave_fetch_add(a.ctr + q, 1, lane);
             if (it >= total)
                 return;
-            const int gp = pfirst + (it - nbands); // the point in the numbering of all pairs' points
-            int lo = 0, hi = np - 1; // its pair: the last whose numbering starts at or before it
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (gp >= a.lk.pre[mid])
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            const int k = lo;
-            const PyrView &I = *a.lk.I[k], &J = *a.lk.J[k];
-            const int top = a.lk.top[k];
+            const int s0 = (bfirst + (it - nbands)) * PPW; // the block's first slot in the numbering of all pairs
+            const int k = __builtin_amdgcn_readfirstlane(pair_of(a.lk.pre, np, s0));
             if (!ready) {
-                // every pyramid complete, then one acquire for all the wave's points (an agent-scope acquire
+                // every pyramid complete, then one acquire for all the wave's blocks (an agent-scope acquire
                 // invalidates the XCD's L2, which LK's overlapping windows live in)
                 wait_count(a.ctr + FTOT, a.total_bands, err);
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 ready = true;
             }
-            const int pt = a.lk.start[k] + (gp - a.lk.pre[k]);
             it = -1;
-            const Geo g = geometry<MAXR>(a.lk.cfg, lane);
-            const float x = a.lk.xy[2 * pt], y = a.lk.xy[2 * pt + 1];
-            const float gx = a.lk.guess ? a.lk.guess[2 * pt] : 0.0f, gy = a.lk.guess ? a.lk.guess[2 * pt + 1] : 0.0f;
-            const Slice sl{nullptr, nullptr, nullptr};
+            const int local = s0 - a.lk.pre[k] + (lane >> 4);
+            const bool live = local < a.lk.cnt[k];
+            const int pt = a.lk.start[k] + (live ? local : 0);
+            float x = -1e30f, y = -1e30f, gx = 0.0f, gy = 0.0f;
+            if (live) {
+                x = a.lk.xy[2 * pt];
+                y = a.lk.xy[2 * pt + 1];
+                if (a.lk.guess) {
+                    gx = a.lk.guess[2 * pt];
+                    gy = a.lk.guess[2 * pt + 1];
+                }
+            }
+            const int top = a.lk.top[k];
             Track o;
-            track<MAXR, uint8_t, uint16_t, SOLVE, false>(I, J, top, a.lk.cfg, g, lane, sl, 1, x, y, gx, gy, o);
-            if (a.lk.fb_max > 0.0f && o.status == 0) {
-                Track b;
-                track<MAXR, uint8_t, uint16_t, SOLVE, false>(J, I, top, a.lk.cfg, g, lane, sl, 1, o.x, o.y,
-                                                             (float)((double)x - (double)o.x),
-                                                             (float)((double)y - (double)o.y), b);
-                const double ex = (double)b.x - (double)x, ey = (double)b.y - (double)y;
-                if (b.status || ex * ex + ey * ey > (double)a.lk.fb_max * (double)a.lk.fb_max)
-                    o.status |= LK_FLOWBACK;
-            }
-            if (lane == 0) {
-                lk_out r;
-                r.x = o.x;
-                r.y = o.y;
-                r.min_eig = o.min_eig;
-                r.residual = o.residual;
-                r.cov_xx = o.cxx;
-                r.cov_xy = o.cxy;
-                r.cov_yy = o.cyy;
-                r.iters = (uint16_t)o.iters;
-                r.status = (uint16_t)o.status;
-                a.lk.out[pt] = r;
-            }
+            track_fb<NV, HMAX, EXACT, uint8_t, uint16_t>(*a.lk.I[k], *a.lk.J[k], top, top, a.lk.cfg,
+                                                  a.lk.solve == VTG_SOLVE_MIXED, a.lk.fb_max, ln, x, y, gx, gy, o);
+            if (live && (lane & 15) == 0)
+                store(a.lk.out + pt, o);
         }
     }
 }
 
-template <int MAXR, int SOLVE>
+template <int NV, int HMAX, bool EXACT>
 int launch_fused_t(vtg_ctx *c, const FusedArgs &a, int total)
 {
     static int per_cu = 0, cus = 0; // resident workgroups per CU of this kernel, CUs of device 0 class
     if (!per_cu) {
-        VTG_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_kernel<MAXR, SOLVE>, 256, 0));
+        VTG_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_kernel<NV, HMAX, EXACT>, 256, 0));
         VTG_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
         if (per_cu < 1)
             per_cu = 1;
@@ -1023,39 +893,24 @@ int launch_fused_t(vtg_ctx *c, const FusedArgs &a, int total)
     const int grid = std::max(FQ, std::min((total + FQ - 1) / FQ * FQ, per_cu * cus / FQ * FQ));
     hipEvent_t ev;
     timer_begin(c, "pyr_lk_fused", &ev);
-    hipLaunchKernelGGL((fused_kernel<MAXR, SOLVE>), dim3(grid), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((fused_kernel<NV, HMAX, EXACT>), dim3(grid), dim3(256), 0, c->stream, a);
     VTG_HIP(c, hipGetLastError());
     timer_end(c, "pyr_lk_fused", ev);
     return VTG_OK;
 }
 
-template <int SOLVE>
-int launch_fused_rows(vtg_ctx *c, const FusedArgs &a, int total, int chunk)
+int launch_fused_shape(vtg_ctx *c, const FusedArgs &a, int total)
 {
-    if (chunk <= 4)
-        return launch_fused_t<4, SOLVE>(c, a, total);
-    if (chunk <= 7)
-        return launch_fused_t<7, SOLVE>(c, a, total);
-    if (chunk <= 8)
-        return launch_fused_t<8, SOLVE>(c, a, total);
-    if (chunk <= 16)
-        return launch_fused_t<16, SOLVE>(c, a, total);
-    return launch_fused_t<32, SOLVE>(c, a, total);
-}
-
-// Rows per lane: the smallest compiled count no smaller than the window's chunk.
-template <int SOLVE, bool STAGED>
-int dispatch_rows(vtg_ctx *c, const LkArgs &a, int storage, int chunk, dim3 grid, dim3 block, size_t lds)
-{
-    if (chunk <= 4)
-        return dispatch_storage<4, SOLVE, STAGED>(c, a, storage, grid, block, lds);
-    if (chunk <= 7)
-        return dispatch_storage<7, SOLVE, STAGED>(c, a, storage, grid, block, lds);
-    if (chunk <= 8)
-        return dispatch_storage<8, SOLVE, STAGED>(c, a, storage, grid, block, lds);
-    if (chunk <= 16)
-        return dispatch_storage<16, SOLVE, STAGED>(c, a, storage, grid, block, lds);
-    return dispatch_storage<32, SOLVE, STAGED>(c, a, storage, grid, block, lds);
+    const int nv = (a.lk.cfg.w + 15) / 16;
+    if (nv == 1 && a.lk.cfg.h == 15)
+        return launch_fused_t<1, 15, true>(c, a, total);
+    if (nv == 2 && a.lk.cfg.h == 21)
+        return launch_fused_t<2, 21, true>(c, a, total);
+    if (nv == 1)
+        return launch_fused_t<1, 31, false>(c, a, total);
+    if (nv == 2)
+        return launch_fused_t<2, 31, false>(c, a, total);
+    return launch_fused_t<3, 31, false>(c, a, total);
 }
 
 int check_cfg(vtg_ctx *c, const lk_cfg *cfg, int solve_mode)
@@ -1082,12 +937,14 @@ int check_pair(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top)
     return VTG_OK;
 }
 
+Cfg to_cfg(const lk_cfg *cfg) { return Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->eps, cfg->min_eig, cfg->max_residual}; }
+
 }  // namespace
 
-// One launch for pairs sel[0 .. m) (m <= BMAX; indices into the caller's arrays) under schedule s.
+// One launch for pairs sel[0 .. m) (m <= BMAX; indices into the caller's arrays) with `waves` wavefronts per workgroup.
 static int launch_group(vtg_ctx *c, const std::vector<int> &sel, const vtg_pyr *const *I, const vtg_pyr *const *J,
                         const int *top, const int *off, const lk_cfg *cfg, const float *xy, const float *guess,
-                        float fb_max, int solve_mode, lk_out *out, LkSched s, int fb_top)
+                        float fb_max, int solve_mode, lk_out *out, int waves, int fb_top)
 {
     LkArgs a;
     a.npairs = (int)sel.size();
@@ -1098,43 +955,30 @@ static int launch_group(vtg_ctx *c, const std::vector<int> &sel, const vtg_pyr *
         a.J[q] = J[k]->dview;
         a.top[q] = top[k];
         a.start[q] = off[k];
+        a.cnt[q] = off[k + 1] - off[k];
         a.pre[q] = acc;
-        acc += off[k + 1] - off[k];
+        acc += (a.cnt[q] + PPW - 1) / PPW * PPW;
     }
     a.pre[a.npairs] = acc;
-    a.n = acc;
-    if (a.n <= 0)
+    a.slots = acc;
+    if (a.slots <= 0)
         return VTG_OK;
-    a.cfg = Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->eps, cfg->min_eig, cfg->max_residual};
-    a.margin = s.margin > 0 ? s.margin : 1;
-    // the staged schedule's slice per wave: S, then the patch and J's neighbourhood; the direct one uses no LDS
-    a.lds_floats = s.margin > 0 ? (cfg->w + 2) * (cfg->h + 2) +
-                                      (cfg->w + 1 + 2 * a.margin) * (cfg->h + 3 + cfg->h + 1 + 2 * a.margin)
-                                : 0;
+    a.cfg = to_cfg(cfg);
+    a.solve = solve_mode;
     a.xy = xy;
     a.guess = guess;
     a.fb_max = fb_max;
     a.fb_top = fb_top;
     a.out = out;
-    a.waves = s.waves;
-    const int groups = 64 / cfg->w, chunk = (cfg->h + groups - 1) / groups;
-    const int wgs = (a.n + a.waves - 1) / a.waves;
+    if (waves < 1 || waves > 8)
+        return fail(c, VTG_EINVAL, "vtg_lk_track: %d waves per workgroup (1..8)", waves);
+    a.waves = waves;
+    const int wgs = (a.slots / PPW + a.waves - 1) / a.waves;
     a.per_xcd = (wgs + NXCD - 1) / NXCD;
     const dim3 grid(a.per_xcd * NXCD), block(64 * a.waves);
-    const size_t lds = (size_t)a.waves * a.lds_floats * sizeof(float);
-    if (s.waves < 1 || s.waves > 8 || lds > 65536)
-        return fail(c, VTG_EINVAL, "vtg_lk_track: schedule margin %d x %d waves needs %zu B of LDS (at most 65536)",
-                    s.margin, s.waves, lds);
     hipEvent_t ev;
     timer_begin(c, "lk_track", &ev);
-    const bool staged = s.margin > 0;
-    int rc;
-    if (solve_mode == VTG_SOLVE_MIXED)
-        rc = staged ? dispatch_rows<VTG_SOLVE_MIXED, true>(c, a, I[sel[0]]->storage, chunk, grid, block, lds)
-                    : dispatch_rows<VTG_SOLVE_MIXED, false>(c, a, I[sel[0]]->storage, chunk, grid, block, lds);
-    else
-        rc = staged ? dispatch_rows<VTG_SOLVE_F64, true>(c, a, I[sel[0]]->storage, chunk, grid, block, lds)
-                    : dispatch_rows<VTG_SOLVE_F64, false>(c, a, I[sel[0]]->storage, chunk, grid, block, lds);
+    const int rc = dispatch_shape(c, a, I[sel[0]]->storage, grid, block);
     timer_end(c, "lk_track", ev);
     return rc;
 }
@@ -1150,33 +994,34 @@ LkSched tuned_lk(const vtg_ctx *c, const vtg_pyr *I, int top, const lk_cfg *cfg)
     const auto it = c->lk_sched.find(lk_class(I, top, cfg));
     if (it != c->lk_sched.end())
         return it->second;
-    return LkSched{c->lk_margin, 2};
+    return LkSched{c->lk_waves};
 }
 
-// Pairs [0, np) with points off[k] .. off[k+1] - 1 of xy: grouped by schedule, one launch per group of up to BMAX.
+// Pairs [0, np) with points off[k] .. off[k+1] - 1 of xy, one launch per BMAX pairs. The workgroup size is `sched`'s
+// when given; otherwise the size-class pick of the pair holding the most points (the context default when its class
+// was not tuned) serves the whole batch, so a batch is never split by schedule.
 int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *const *J, const int *top,
                    const int *off, const lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
                    int solve_mode, lk_out *out, const LkSched *sched, int fb_top)
 {
-    std::vector<LkSched> sc(np);
-    for (int k = 0; k < np; k++)
-        sc[k] = sched ? *sched : tuned_lk(c, I[k], top[k], cfg);
-    std::vector<bool> done(np, false);
-    for (int k = 0; k < np; k++) {
-        if (done[k])
-            continue;
-        std::vector<int> sel;
-        for (int q = k; q < np; q++)
-            if (!done[q] && sc[q].margin == sc[k].margin && sc[q].waves == sc[k].waves) {
-                sel.push_back(q);
-                done[q] = true;
+    int waves = c->lk_waves;
+    if (sched) {
+        waves = sched->waves;
+    } else {
+        int most = -1;
+        for (int k = 0; k < np; k++)
+            if (off[k + 1] - off[k] > most) {
+                most = off[k + 1] - off[k];
+                waves = tuned_lk(c, I[k], top[k], cfg).waves;
             }
-        for (size_t b = 0; b < sel.size(); b += BMAX) {
-            const std::vector<int> part(sel.begin() + b, sel.begin() + std::min(sel.size(), b + BMAX));
-            int rc = launch_group(c, part, I, J, top, off, cfg, xy, guess, fb_max, solve_mode, out, sc[k], fb_top);
-            if (rc)
-                return rc;
-        }
+    }
+    for (int b = 0; b < np; b += BMAX) {
+        std::vector<int> part;
+        for (int k = b; k < std::min(np, b + BMAX); k++)
+            part.push_back(k);
+        const int rc = launch_group(c, part, I, J, top, off, cfg, xy, guess, fb_max, solve_mode, out, waves, fb_top);
+        if (rc)
+            return rc;
     }
     return VTG_OK;
 }
@@ -1255,8 +1100,9 @@ int vtg_pyr_lk_fused(vtg_ctx *c, int npairs, vtg_pyr *const *I, vtg_pyr *const *
         a.lk.J[k] = J[k]->dview;
         a.lk.top[k] = top[k];
         a.lk.start[k] = offsets[k];
+        a.lk.cnt[k] = offsets[k + 1] - offsets[k];
         a.lk.pre[k] = acc;
-        acc += offsets[k + 1] - offsets[k];
+        acc += (a.lk.cnt[k] + PPW - 1) / PPW * PPW;
         a.f0[k] = frame0[k];
         a.f1[k] = frame1[k];
         a.p0[k] = pitch0[k];
@@ -1266,15 +1112,14 @@ int vtg_pyr_lk_fused(vtg_ctx *c, int npairs, vtg_pyr *const *I, vtg_pyr *const *
             items += b;
             a.total_bands += b;
         }
-        items += (offsets[k + 1] - offsets[k] + 3) / 4; // points, four waves to a workgroup
     }
+    items += acc / PPW; // point blocks, one wavefront each
     a.lk.pre[npairs] = acc;
-    a.lk.n = acc;
+    a.lk.slots = acc;
     if (acc > 0 && (!xy || !out))
         return fail(c, VTG_EINVAL, "vtg_pyr_lk_fused: null points or output");
-    a.lk.cfg = Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->eps, cfg->min_eig, cfg->max_residual};
-    a.lk.margin = 1;
-    a.lk.lds_floats = 0;
+    a.lk.cfg = to_cfg(cfg);
+    a.lk.solve = solve_mode;
     a.lk.xy = xy;
     a.lk.guess = guess;
     a.lk.fb_max = fb_max;
@@ -1290,9 +1135,7 @@ int vtg_pyr_lk_fused(vtg_ctx *c, int npairs, vtg_pyr *const *I, vtg_pyr *const *
     hipLaunchKernelGGL(fused_reset, dim3(1), dim3(256), 0, c->stream, a.ctr, nctr, c->fused_init ? 0 : 1);
     c->fused_init = true;
     VTG_HIP(c, hipGetLastError());
-    const int groups = 64 / cfg->w, chunk = (cfg->h + groups - 1) / groups;
-    return solve_mode == VTG_SOLVE_MIXED ? launch_fused_rows<VTG_SOLVE_MIXED>(c, a, items, chunk)
-                                         : launch_fused_rows<VTG_SOLVE_F64>(c, a, items, chunk);
+    return launch_fused_shape(c, a, items);
 }
 
 int vtg_ctx_device_error(vtg_ctx *c, int *code)
@@ -1309,11 +1152,11 @@ int vtg_ctx_device_error(vtg_ctx *c, int *code)
     return VTG_OK;
 }
 
-int vtg_lk_set_margin(vtg_ctx *c, int margin)
+int vtg_lk_set_waves(vtg_ctx *c, int waves)
 {
-    if (!c || margin < 0 || margin > 16)
-        return fail(c, VTG_EINVAL, "vtg_lk_set_margin: %d (0..16)", margin);
-    c->lk_margin = margin;
+    if (!c || waves < 1 || waves > 8)
+        return fail(c, VTG_EINVAL, "vtg_lk_set_waves: %d (1..8)", waves);
+    c->lk_waves = waves;
     return VTG_OK;
 }
 

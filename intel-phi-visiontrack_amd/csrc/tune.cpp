@@ -1,8 +1,8 @@
 // tune.cpp: the autotuner. For a frame pair of a size class it times every candidate schedule of the two launches that
-// carry the batch -- the pyramid reduce (output tiles of 8, 16 or 32 rows) and the LK kernel (direct, or staged with
-// a margin of 2, 4 or 8 pixels; 2, 4 or 8 waves, one point each, per workgroup) -- on the caller's own frames and
-// points, and keeps the fastest for that class in the context. Later builds and LK launches of pyramids of the class
-// use the picks; a batch of several classes becomes one launch per distinct schedule. Every schedule computes the same
+// carry the batch -- the pyramid reduce (output tiles of 8, 16 or 32 rows) and the LK kernel (1, 2, 4 or 8
+// wavefronts of 4 points each per workgroup) -- on the caller's own frames and points, and keeps the fastest for that
+// class in the context. Later builds and LK launches of pyramids of the class use the picks. A batch of several
+// classes stays one LK launch: it takes the pick of the pair holding the most points. Every schedule computes the same
 // bytes (the tests hold them to it), so tuning changes time only.
 #include <algorithm>
 #include <cfloat>
@@ -101,35 +101,25 @@ int vtg_tune_pair(vtg_ctx *c, vtg_pyr *I, vtg_pyr *J, const uint8_t *f0, const u
     const vtg_pyr *Ic = I, *Jc = J;
     best = FLT_MAX;
     int rc = VTG_OK;
-    for (int margin : {0, 2, 4, 8}) {
-        for (int waves : {2, 4, 8}) {
-            const size_t lds = margin == 0 ? 0
-                                           : (size_t)waves * sizeof(float) *
-                                                 ((cfg->w + 2) * (cfg->h + 2) +
-                                                  (cfg->w + 1 + 2 * margin) * (cfg->h + 3 + cfg->h + 1 + 2 * margin));
-            if (lds > 65536)
-                continue;
-            const LkSched s{margin, waves};
-            float us = 0.0f;
-            rc = median_us(c, 3, [&] {
-                return launch_lk_many(c, 1, &Ic, &Jc, &top, off, cfg, xy, nullptr, 0.0f, solve_mode, out, &s);
-            }, &us);
-            if (rc)
-                break;
-            res->candidates++;
-            if (margin == 0 && waves == 2)
-                res->lk_us_default = us;
-            if (us < best)
-                best = us, res->lk_margin = margin, res->lk_waves = waves, res->lk_us = us;
-        }
+    for (int waves : {1, 2, 4, 8}) {
+        const LkSched s{waves};
+        float us = 0.0f;
+        rc = median_us(c, 3, [&] {
+            return launch_lk_many(c, 1, &Ic, &Jc, &top, off, cfg, xy, nullptr, 0.0f, solve_mode, out, &s);
+        }, &us);
         if (rc)
             break;
+        res->candidates++;
+        if (waves == c->lk_waves)
+            res->lk_us_default = us;
+        if (us < best)
+            best = us, res->lk_waves = waves, res->lk_us = us;
     }
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(out);
     if (rc)
         return rc;
-    c->lk_sched[lk_class(I, top, cfg)] = LkSched{res->lk_margin, res->lk_waves};
+    c->lk_sched[lk_class(I, top, cfg)] = LkSched{res->lk_waves};
     return VTG_OK;
 }
 
@@ -140,7 +130,6 @@ int vtg_tune_get(vtg_ctx *c, const vtg_pyr *I, int top, const struct lk_cfg *cfg
     *res = vtg_tune{};
     res->pyr_rows = tuned_pyr_rows(c, I);
     const LkSched s = tuned_lk(c, I, top, cfg);
-    res->lk_margin = s.margin;
     res->lk_waves = s.waves;
     return VTG_OK;
 }

@@ -36,10 +36,9 @@ struct Timer {
     hipEvent_t a, b;
 };
 
-// An LK launch schedule: margin 0 = direct (the level read in memory), > 0 = staged with that margin; waves per
-// workgroup 2, 4 or 8.
+// An LK launch schedule: wavefronts (of 4 points each) per workgroup, 1 .. 8.
 struct LkSched {
-    int margin, waves;
+    int waves;
 };
 
 }  // namespace vtg
@@ -63,7 +62,7 @@ struct vtg_ctx {
     hipEvent_t join = nullptr;               // for vtg_ctx_wait
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;
-    int lk_margin = 0;                       // LK schedule: 0 direct, > 0 stage J with this margin
+    int lk_waves = 4;                        // LK schedule: wavefronts per workgroup where no pick applies
     // the autotuner's picks per size class (keys from pyr_class / lk_class)
     std::map<uint64_t, int> pyr_rows;
     std::map<uint64_t, vtg::LkSched> lk_sched;

@@ -2,7 +2,8 @@
     m1_hashes.json   the reference-held hash lines logs/m1/host/m1-host-*.txt (data, parsed)
     m2_lines.json    logs/m2/host/640x480.txt: per displacement, point 0's raw bits and the FNV of all 330 lk_out
     ncc_ref.json     tk_correlate (src/kern/track.c:208) of oracle/_ref/libvtref.so on seeded 32 x 21 inputs
-    lk_ref.json      lk_track_vx / lk_flowback_vx of oracle/_ref/libvtref.so on the 640 x 480 blob workload
+    lk_ref.json      lk_track_vx / lk_flowback_vx of oracle/_ref/libvtref.so on the 640 x 480 blob workload, and
+                     lk_track_vx on the 1920 x 1080 one (two seeds)
 The reference itself never leaves this container: only these numbers are committed."""
 import ctypes as C
 import json
@@ -71,6 +72,25 @@ def ncc_ref():
     return {"tw": 32, "th": 21, "cases": cases}
 
 
+def _ref_lk_records(ref, o, w, h, levels, seed, nx, ny, win, fb):
+    """lk_track_vx (and lk_flowback_vx when fb > 0) of the reference library on the seeded blob pair, every grid point."""
+    f0, f1, mo = vtgpu.frame_pair(w, h, seed)
+    keep_i, I = o.pyramid(f0, w, h, levels, threads=8)  # the planes' buffers must outlive the calls
+    keep_j, J = o.pyramid(f1, w, h, levels, threads=8)
+    xs, ys = vtgpu.grid(nx, ny, w, h)
+    cfg = LkCfg(win, win, 10, 0.01, 1e-4, 40.0, 0)
+    tile = np.zeros(ref.lk_tile_bytes() // 4 + 64, np.float32)
+    tp = (tile.ctypes.data + 63) & ~63
+    out = np.zeros(len(xs), vtgpu.LK_OUT_DTYPE)
+    outfb = np.zeros(len(xs), vtgpu.LK_OUT_DTYPE)
+    for i in range(len(xs)):
+        ref.lk_track_vx(I, J, levels - 1, C.byref(cfg), xs[i], ys[i], 0.0, 0.0, out[i:i + 1].ctypes.data, tp)
+        if fb:
+            outfb[i] = out[i]
+            ref.lk_flowback_vx(I, J, levels - 1, C.byref(cfg), xs[i], ys[i], fb, outfb[i:i + 1].ctypes.data, tp)
+    return mo, out, outfb
+
+
 def lk_ref():
     ref = C.CDLL(REF_SO)
     ref.lk_track_vx.argtypes = [C.POINTER(Plane), C.POINTER(Plane), C.c_int, C.POINTER(LkCfg), C.c_float, C.c_float,
@@ -80,28 +100,27 @@ def lk_ref():
     ref.lk_tile_bytes.restype = C.c_size_t
     o = Oracle()
     w, h, levels, seed = 640, 480, 3, 1
-    f0, f1, mo = vtgpu.frame_pair(w, h, seed)
-    keep_i, I = o.pyramid(f0, w, h, levels)  # the planes' buffers must outlive the calls
-    keep_j, J = o.pyramid(f1, w, h, levels)
-    xs, ys = vtgpu.grid(40, 30, w, h)
-    cfg = LkCfg(15, 15, 10, 0.01, 1e-4, 40.0, 0)
-    tile = np.zeros(ref.lk_tile_bytes() // 4 + 64, np.float32)
-    tp = (tile.ctypes.data + 63) & ~63
-    out = np.zeros(len(xs), vtgpu.LK_OUT_DTYPE)
-    outfb = np.zeros(len(xs), vtgpu.LK_OUT_DTYPE)
-    for i in range(len(xs)):
-        ref.lk_track_vx(I, J, levels - 1, C.byref(cfg), xs[i], ys[i], 0.0, 0.0, out[i:i + 1].ctypes.data, tp)
-        outfb[i] = out[i]
-        ref.lk_flowback_vx(I, J, levels - 1, C.byref(cfg), xs[i], ys[i], 0.5, outfb[i:i + 1].ctypes.data, tp)
+    mo, out, outfb = _ref_lk_records(ref, o, w, h, levels, seed, 40, 30, 15, 0.5)
     bits = out.view(np.uint32).reshape(-1, 8)
-    return {"w": w, "h": h, "levels": levels, "seed": seed, "grid": [40, 30],
-            "motion": [mo.dx, mo.dy, mo.theta_deg],
-            "cfg": {"w": 15, "h": 15, "max_iter": 10, "eps": 0.01, "min_eig": 1e-4, "max_residual": 40.0},
+    res = {"w": w, "h": h, "levels": levels, "seed": seed, "grid": [40, 30],
+           "motion": [mo.dx, mo.dy, mo.theta_deg],
+           "cfg": {"w": 15, "h": 15, "max_iter": 10, "eps": 0.01, "min_eig": 1e-4, "max_residual": 40.0},
+           "fnv": "%016x" % o.lib.vo_hash_lk(out.ctypes.data, len(out)),
+           "fnv_fb05": "%016x" % o.lib.vo_hash_lk(outfb.ctypes.data, len(outfb)),
+           "first_bits": ["%08x" % b for b in bits[0]],
+           "status_counts": {str(s): int((out["status"] == s).sum()) for s in np.unique(out["status"])},
+           "status_counts_fb05": {str(s): int((outfb["status"] == s).sum()) for s in np.unique(outfb["status"])}}
+    # the native workload: 1920 x 1080, 4 levels, 180 x 90 grid, 21 x 21, two seeds
+    res["blob1080"] = []
+    for seed in (2, 3):
+        mo, out, _ = _ref_lk_records(ref, o, 1920, 1080, 4, seed, 180, 90, 21, 0.0)
+        res["blob1080"].append({
+            "w": 1920, "h": 1080, "levels": 4, "seed": seed, "grid": [180, 90], "motion": [mo.dx, mo.dy, mo.theta_deg],
+            "cfg": {"w": 21, "h": 21, "max_iter": 10, "eps": 0.01, "min_eig": 1e-4, "max_residual": 40.0},
             "fnv": "%016x" % o.lib.vo_hash_lk(out.ctypes.data, len(out)),
-            "fnv_fb05": "%016x" % o.lib.vo_hash_lk(outfb.ctypes.data, len(outfb)),
-            "first_bits": ["%08x" % b for b in bits[0]],
             "status_counts": {str(s): int((out["status"] == s).sum()) for s in np.unique(out["status"])},
-            "status_counts_fb05": {str(s): int((outfb["status"] == s).sum()) for s in np.unique(outfb["status"])}}
+            "iters_sum": int(out["iters"].astype(np.int64).sum())})
+    return res
 
 
 if __name__ == "__main__":

@@ -143,6 +143,25 @@ def test_lk_reference_blob_workload(oracle):
     assert "%016x" % oracle.lib.vo_hash_lk(out.ctypes.data, len(out)) == g["fnv_fb05"]
 
 
+@pytest.mark.parametrize("case", [0, 1])
+def test_lk_reference_blob_1080(oracle, case):
+    """The lane tier equals oracle/_ref's lk_track_vx on every record of the 1920 x 1080 blob workload (4 levels,
+    180 x 90 grid, 21 x 21; fixture written from the reference library)."""
+    import vtgpu
+    g = json.load(open(os.path.join(GOLD, "lk_ref.json")))["blob1080"][case]
+    w, h, L = g["w"], g["h"], g["levels"]
+    f0, f1, mo = vtgpu.frame_pair(w, h, g["seed"])
+    assert np.allclose([mo.dx, mo.dy, mo.theta_deg], g["motion"], rtol=0, atol=1e-12)
+    k0, I = oracle.pyramid(f0, w, h, L, threads=8)
+    k1, J = oracle.pyramid(f1, w, h, L, threads=8)
+    xs, ys = vtgpu.grid(*g["grid"], w, h)
+    c = g["cfg"]
+    cfg = lk_cfg(c["w"], c["h"], c["max_iter"], c["eps"], c["min_eig"], c["max_residual"])
+    out = oracle.lk_batch(0, I, J, L - 1, cfg, xs, ys, threads=8)
+    assert "%016x" % oracle.lib.vo_hash_lk(out.ctypes.data, len(out)) == g["fnv"]
+    assert int(out["iters"].astype(np.int64).sum()) == g["iters_sum"]
+
+
 def _plane_from(img):
     h, w = img.shape
     hp = HostPlane(w, h)
