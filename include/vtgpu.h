@@ -161,7 +161,7 @@ int vtg_pyr_lk_fused(vtg_ctx *ctx, int npairs, vtg_pyr *const *I, vtg_pyr *const
 /* The context's device error word (waits for the context): nonzero when a fused launch's bounded wait gave up, whose
  * results are then not valid; reading clears it. */
 int vtg_ctx_device_error(vtg_ctx *ctx, int *code);
-/* The LK schedule where no autotuner pick applies: wavefronts per workgroup, 1 .. 8 (default 4). A wavefront tracks 4
+/* The LK schedule where no autotuner pick applies: wavefronts per workgroup, 1 .. 8 (default 1). A wavefront tracks 4
  * points, one per row of 16 lanes. Results are the same bit for bit under every schedule; only time differs. */
 int vtg_lk_set_waves(vtg_ctx *ctx, int waves);
 

@@ -21,6 +21,12 @@
 // the records can differ; the tests compare every record with the reference's.
 // All 16 lanes of a row hold identical sums, so a point's control flow is uniform across its row; the 4 points of a
 // wavefront diverge by exec mask (a point done with a level waits for the others of its wavefront).
+// Registers and LDS. The windows of the workloads here (15 x 15, 21 x 21) compile as exact shapes: straight-line code,
+// the template of every full 16-column block in registers, the last, partly used block (5 columns at 21 x 21) in LDS,
+// and each level's J neighbourhood (the window plus 2 pixels each side) staged in LDS once, so the iterations read
+// LDS rather than the level in memory (a window that leaves the neighbourhood stages it again). Any other window runs
+// a guarded form of its block count that reads the level in memory at every iteration. The loads of both forms are
+// pipelined a fixed number of rows ahead of their use.
 // Batches. A launch covers points of up to 64 frame pairs of any sizes; a wavefront's 4 points always belong to one
 // pair (each pair's points are numbered from a multiple of 4), so its pyramid levels are wave-uniform and every level
 // is read through one buffer resource with the row step in a scalar offset. Workgroups are mapped to points so that
@@ -92,19 +98,33 @@ template <> struct Ld<__hip_bfloat16> {
 
 // A level read through a buffer resource: the base and size are wave-uniform (scalar registers), each load is the
 // resource plus a lane's 32-bit byte offset o plus a uniform offset so (a row step, kept in a scalar register), and a
-// read past the level's bytes returns 0 instead of touching memory. Two or four neighbouring elements of a row come
-// in one load.
+// read past the level's bytes returns 0 instead of touching memory. Two (P) or four (Q) neighbouring elements of a row
+// come in one load; the raw bits are converted where they are used, so a load can be issued rows ahead of its use.
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 template <typename T> struct Row;
 template <> struct Row<uint8_t> {
-    static __device__ __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float &a0, float &a1)
+    typedef unsigned P;
+    typedef unsigned Q;
+    static __device__ __forceinline__ float l1(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
     {
-        const unsigned v = __builtin_amdgcn_raw_buffer_load_b16(r, o, so, 0);
-        a0 = (float)(v & 0xffu);
-        a1 = (float)(v >> 8);
+        return (float)__builtin_amdgcn_raw_buffer_load_b8(r, o, so, 0);
     }
-    static __device__ __forceinline__ void quad(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float *a)
+    static __device__ __forceinline__ P lp(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
     {
-        const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0);
+        return __builtin_amdgcn_raw_buffer_load_b16(r, o, so, 0);
+    }
+    static __device__ __forceinline__ void cp(P v, float &a0, float &a1)
+    {
+        a0 = (float)(v & 0xffu);
+        a1 = (float)((v >> 8) & 0xffu);
+    }
+    static __device__ __forceinline__ Q lq(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0);
+    }
+    static __device__ __forceinline__ void cq(Q v, float *a)
+    {
         a[0] = (float)(v & 0xffu);
         a[1] = (float)((v >> 8) & 0xffu);
         a[2] = (float)((v >> 16) & 0xffu);
@@ -112,15 +132,27 @@ template <> struct Row<uint8_t> {
     }
 };
 template <> struct Row<uint16_t> {
-    static __device__ __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float &a0, float &a1)
+    typedef unsigned P;
+    typedef u32x2 Q;
+    static __device__ __forceinline__ float l1(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
     {
-        const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0);
+        return (float)__builtin_amdgcn_raw_buffer_load_b16(r, o, so, 0);
+    }
+    static __device__ __forceinline__ P lp(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0);
+    }
+    static __device__ __forceinline__ void cp(P v, float &a0, float &a1)
+    {
         a0 = (float)(v & 0xffffu);
         a1 = (float)(v >> 16);
     }
-    static __device__ __forceinline__ void quad(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float *a)
+    static __device__ __forceinline__ Q lq(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
     {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
+        return __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
+    }
+    static __device__ __forceinline__ void cq(Q v, float *a)
+    {
         a[0] = (float)(v[0] & 0xffffu);
         a[1] = (float)(v[0] >> 16);
         a[2] = (float)(v[1] & 0xffffu);
@@ -128,41 +160,66 @@ template <> struct Row<uint16_t> {
     }
 };
 template <> struct Row<float> {
-    static __device__ __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float &a0, float &a1)
+    typedef u32x2 P;
+    typedef u32x4 Q;
+    static __device__ __forceinline__ float l1(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
     {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0));
+    }
+    static __device__ __forceinline__ P lp(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
+    }
+    static __device__ __forceinline__ void cp(P v, float &a0, float &a1)
+    {
         a0 = __uint_as_float(v[0]);
         a1 = __uint_as_float(v[1]);
     }
-    static __device__ __forceinline__ void quad(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float *a)
+    static __device__ __forceinline__ Q lq(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
     {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, o, so, 0);
+        return __builtin_amdgcn_raw_buffer_load_b128(r, o, so, 0);
+    }
+    static __device__ __forceinline__ void cq(Q v, float *a)
+    {
         for (int i = 0; i < 4; i++)
             a[i] = __uint_as_float(v[i]);
     }
 };
 template <typename H> struct Row16 {
-    static __device__ __forceinline__ float cvt(unsigned short b)
+    typedef unsigned P;
+    typedef u32x2 Q;
+    static __device__ __forceinline__ float cvt(unsigned short b);
+    static __device__ __forceinline__ float l1(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
     {
-        H h;
-        __builtin_memcpy(&h, &b, 2);
-        return Ld<H>::get(h);
+        return cvt((unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, o, so, 0));
     }
-    static __device__ __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float &a0, float &a1)
+    static __device__ __forceinline__ P lp(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
     {
-        const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0);
+        return __builtin_amdgcn_raw_buffer_load_b32(r, o, so, 0);
+    }
+    static __device__ __forceinline__ void cp(P v, float &a0, float &a1)
+    {
         a0 = cvt((unsigned short)(v & 0xffffu));
         a1 = cvt((unsigned short)(v >> 16));
     }
-    static __device__ __forceinline__ void quad(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so, float *a)
+    static __device__ __forceinline__ Q lq(__amdgpu_buffer_rsrc_t r, unsigned o, unsigned so)
     {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
+        return __builtin_amdgcn_raw_buffer_load_b64(r, o, so, 0);
+    }
+    static __device__ __forceinline__ void cq(Q v, float *a)
+    {
         a[0] = cvt((unsigned short)(v[0] & 0xffffu));
         a[1] = cvt((unsigned short)(v[0] >> 16));
         a[2] = cvt((unsigned short)(v[1] & 0xffffu));
         a[3] = cvt((unsigned short)(v[1] >> 16));
     }
 };
+template <typename H> __device__ __forceinline__ float Row16<H>::cvt(unsigned short b)
+{
+    H h;
+    __builtin_memcpy(&h, &b, 2);
+    return Ld<H>::get(h);
+}
 template <> struct Row<__half> : Row16<__half> {};
 template <> struct Row<__hip_bfloat16> : Row16<__hip_bfloat16> {};
 
@@ -203,12 +260,14 @@ __device__ __forceinline__ float row_sum(float v)
 template <int NV> struct Lanes {
     unsigned col[NV]; // column offset in elements from the window's first column
     bool on[NV];
+    int j;            // the lane in its row
 };
 
 template <int NV>
 __device__ __forceinline__ Lanes<NV> lanes(const Cfg &c, int j)
 {
     Lanes<NV> ln;
+    ln.j = j;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         const int cc = 16 * k + j;
@@ -218,20 +277,67 @@ __device__ __forceinline__ Lanes<NV> lanes(const Cfg &c, int j)
     return ln;
 }
 
-// The template of one level in registers: window rows 0 .. HMAX - 1 (the first h used) of the lane's NV columns.
-template <int NV, int HMAX> struct Tmpl {
-    float t[HMAX][NV], tx[HMAX][NV], ty[HMAX][NV];
+// A compiled window shape: NV blocks of 16 columns, HMAX rows, WX the exact width (0: the launch's width and height,
+// rows past h skipped by uniform branches). An exact shape of two or more blocks keeps its last block, whose lanes are
+// only partly window columns (5 of 16 at 21 x 21), in LDS: a row of TAIL entries (T, Tx, Ty, 0) and one zero entry
+// that the lanes past the width read, so the registers hold only the full blocks.
+template <int NV, int HMAX, int WX> struct Shape {
+    static constexpr bool EXACT = WX > 0;
+    static constexpr bool SPLIT = WX > 0 && NV >= 2;
+    static constexpr int NR = SPLIT ? NV - 1 : NV;                 // blocks in registers
+    static constexpr int TAIL = WX > 0 ? WX - 16 * (NV - 1) : 16;  // window columns of the last block
+    static constexpr int LROW = TAIL + 1;                          // LDS entries per row
+    static constexpr int LDS_POINT = SPLIT ? HMAX * LROW : 0;      // LDS entries (16 B) per point
+    // block k holds only window columns in every lane (known at compile time for an exact width)
+    static constexpr bool full(int k) { return WX > 0 && 16 * k + 15 < WX; }
+    // STG (exact shapes): each level's J neighbourhood is staged in LDS, the window (w + 1 by h + 1 pixels) plus a
+    // margin of M on every side, as floats in the level's own units; the iterations read it there
+    static constexpr bool STG = WX > 0;
+    static constexpr int M = 2;
+    static constexpr int RW = WX + 1 + 2 * M, RH = HMAX + 1 + 2 * M;
+    static constexpr int STAGE = STG ? (RW * RH + 3) / 4 * 4 : 0;    // floats per point, a multiple of 4
+    static constexpr int LDS_BYTES = LDS_POINT * 16 + STAGE * 4;     // LDS per point
 };
+
+// The template of one level: window rows 0 .. HMAX - 1 (the first h used) of the lane's register blocks, and (SPLIT)
+// the point's LDS rows of the last block with the entry this lane reads in each.
+template <int NV, int HMAX, int WX> struct Tmpl {
+    float t[HMAX][Shape<NV, HMAX, WX>::NR], tx[HMAX][Shape<NV, HMAX, WX>::NR], ty[HMAX][Shape<NV, HMAX, WX>::NR];
+    float4 *last;
+    int lc;
+    float *R; // STG: the staged neighbourhood of J, RH rows of RW floats
+};
+
+// v unchanged, but known to the compiler only after `after` exists: a load addressed by the result cannot be issued
+// before the value it follows, which keeps a software pipeline's loads a fixed number of rows ahead of their use (the
+// loads are free of side effects, so without this the compiler issues all of them at once and runs out of registers).
+template <typename V, typename A>
+__device__ __forceinline__ V after(V v, A after)
+{
+    asm volatile("" : "+v"(v) : "v"(after));
+    return v;
+}
+
+// LDS written by one lane and read by another lane of the same wave: order the accesses, no workgroup barrier.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // The template at (px, py) (build_vx, lk_vx.c:36-97): the bilinear window S over rows -1 .. h and, per lane and block,
 // columns c - 1, c, c + 1 of its window column c (four taps of each image row, x-interpolated, then y-interpolated
 // with the row above), the Sobel gradients / 512 and T = S's middle; returns G = the butterfly totals of Tx^2,
 // Tx Ty, Ty^2 accumulated over rows, then blocks.
-template <int NV, int HMAX, bool EXACT, typename T>
+template <int NV, int HMAX, int WX, typename T>
 __device__ __forceinline__ void build(const Level &lv, double px, double py, const Cfg &c, const Lanes<NV> &ln,
-                                      Tmpl<NV, HMAX> &tp, float *G)
+                                      Tmpl<NV, HMAX, WX> &tp, float *G)
 {
+    typedef Shape<NV, HMAX, WX> SH;
+    constexpr bool EXACT = SH::EXACT;
     constexpr float ks = Ld<T>::scale, kg = ks * (1.0f / 512.0f);
+    constexpr int K = NV == 1 ? 8 : 4; // image rows whose loads are in flight ahead of their use
     const double flx = floor(px), fly = floor(py);
     const float fx = (float)(px - flx), fy = (float)(py - fly);
     unsigned pitch;
@@ -242,32 +348,33 @@ __device__ __forceinline__ void build(const Level &lv, double px, double py, con
 #pragma unroll
     for (int k = 0; k < NV; k++)
         ob[k] = o + ln.col[k] * (unsigned)sizeof(T);
+    // image rows 0 .. h + 2; a constant trip count, so the loops unroll and the template stays in registers; rows past
+    // h (EXACT: h == HMAX, none) are skipped by a uniform branch
+    typename Row<T>::Q raw[HMAX + 3][NV];
+#pragma unroll
+    for (int i = 0; i < K; i++)
+        if (EXACT || i <= c.h + 2)
+#pragma unroll
+            for (int k = 0; k < NV; k++)
+                raw[i][k] = Row<T>::lq(rs, ob[k], (unsigned)i * pitch);
     float X[NV][3], S0[NV][3], S1[NV][3], d0[NV], d1[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        float a[4];
-        Row<T>::quad(rs, ob[k], 0u, a);
-#pragma unroll
-        for (int q = 0; q < 3; q++)
-            X[k][q] = fmaf(a[q + 1] - a[q], fx, a[q]);
-    }
     float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
-    // a constant trip count, so the loop unrolls and the template stays in registers; rows past h (EXACT: h == HMAX,
-    // none) are skipped by a uniform branch
 #pragma unroll
-    for (int i = 1; i <= HMAX + 2; i++) {
+    for (int i = 0; i <= HMAX + 2; i++) {
         if (EXACT || i <= c.h + 2) {
             const int r = i - 1; // the S row this image row completes
 #pragma unroll
             for (int k = 0; k < NV; k++) {
                 float a[4], S[3];
-                Row<T>::quad(rs, ob[k], (unsigned)i * pitch, a);
+                Row<T>::cq(raw[i][k], a);
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
                     const float here = fmaf(a[q + 1] - a[q], fx, a[q]);
-                    S[q] = fmaf(here - X[k][q], fy, X[k][q]);
+                    S[q] = i == 0 ? 0.0f : fmaf(here - X[k][q], fy, X[k][q]);
                     X[k][q] = here;
                 }
+                if (i == 0)
+                    continue;
                 const float dS = S[2] - S[0];
                 if (r == 0) {
 #pragma unroll
@@ -285,11 +392,17 @@ __device__ __forceinline__ void build(const Level &lv, double px, double py, con
                     float tx = fmaf(d1[k], 2.0f, d0[k] + dS) * kg;
                     const float vl = S[0] - S0[k][0], vc = S[1] - S0[k][1], vr = S[2] - S0[k][2];
                     float ty = fmaf(vc, 2.0f, vl + vr) * kg;
-                    tx = ln.on[k] ? tx : 0.0f;
-                    ty = ln.on[k] ? ty : 0.0f;
-                    tp.t[rr][k] = S1[k][1] * ks;
-                    tp.tx[rr][k] = tx;
-                    tp.ty[rr][k] = ty;
+                    const bool on = SH::full(k) || ln.on[k];
+                    tx = on ? tx : 0.0f;
+                    ty = on ? ty : 0.0f;
+                    if (SH::SPLIT && k == NV - 1) {
+                        tp.last[rr * SH::LROW + tp.lc] =
+                            on ? make_float4(S1[k][1] * ks, tx, ty, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    } else {
+                        tp.t[rr][k] = S1[k][1] * ks;
+                        tp.tx[rr][k] = tx;
+                        tp.ty[rr][k] = ty;
+                    }
                     sxx = fmaf(tx, tx, sxx);
                     sxy = fmaf(tx, ty, sxy);
                     syy = fmaf(ty, ty, syy);
@@ -302,8 +415,17 @@ __device__ __forceinline__ void build(const Level &lv, double px, double py, con
                     d1[k] = dS;
                 }
             }
+            // the next row's loads: K rows ahead, issued once this row is done
+            if (i + K <= HMAX + 2 && (EXACT || i + K <= c.h + 2))
+#pragma unroll
+                for (int k = 0; k < NV; k++) {
+                    ob[k] = after(ob[k], X[k][0]);
+                    raw[i + K][k] = Row<T>::lq(rs, ob[k], (unsigned)(i + K) * pitch);
+                }
         }
     }
+    if (SH::SPLIT)
+        wave_sync(); // the last block's entries before any lane reads them
     G[0] = row_sum(sxx);
     G[1] = row_sum(sxy);
     G[2] = row_sum(syy);
@@ -312,46 +434,170 @@ __device__ __forceinline__ void build(const Level &lv, double px, double py, con
 // J sampled at (qx, qy) over the window (iterate_vx, lk_vx.c:100-140; residual_vx, lk_vx.c:142-175): two taps of each
 // of rows 0 .. h of the window at floor(q), x-interpolated, the row above carried; It = T - J. RES = false: b = the
 // butterfly totals of It Tx and It Ty; RES = true: of |It| / 64 and (It / 64)^2, lanes past the width masked.
-template <int NV, int HMAX, bool EXACT, typename T, bool RES>
-__device__ __forceinline__ void sample(const Level &lv, double qx, double qy, const Cfg &c, const Lanes<NV> &ln,
-                                       const Tmpl<NV, HMAX> &tp, float *b)
+template <int NV, int HMAX, int WX, typename T, bool RES>
+__device__ __forceinline__ void sample(__amdgpu_buffer_rsrc_t rs, unsigned pitch, double qx, double qy, const Cfg &c,
+                                       const Lanes<NV> &ln, const Tmpl<NV, HMAX, WX> &tp, float *b)
 {
+    typedef Shape<NV, HMAX, WX> SH;
+    constexpr bool EXACT = SH::EXACT;
     constexpr float ks = Ld<T>::scale;
+    constexpr int K = NV == 1 ? 12 : 8; // window rows whose loads are in flight ahead of their use
     const double flx = floor(qx), fly = floor(qy);
     const float fx = (float)(qx - flx), fy = (float)(qy - fly);
-    unsigned pitch;
-    const __amdgpu_buffer_rsrc_t rs = level_rsrc<T>(lv, pitch);
     const unsigned o = (unsigned)((int)fly - c.h / 2) * pitch + (unsigned)((int)flx - c.w / 2) * sizeof(T);
     unsigned ob[NV];
-    float prev[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
+    for (int k = 0; k < NV; k++)
         ob[k] = o + ln.col[k] * (unsigned)sizeof(T);
-        float a0, a1;
-        Row<T>::pair(rs, ob[k], 0u, a0, a1);
-        prev[k] = fmaf(a1 - a0, fx, a0);
-    }
+    typename Row<T>::P raw[HMAX + 1][NV]; // window rows 0 .. h
+#pragma unroll
+    for (int r = 0; r < K; r++)
+        if (EXACT || r <= c.h)
+#pragma unroll
+            for (int k = 0; k < NV; k++)
+                raw[r][k] = Row<T>::lp(rs, ob[k], (unsigned)r * pitch);
+    // SPLIT: the last block's template entries, KL rows ahead
+    constexpr int KL = 3;
+    float4 e[HMAX];
+    int li = tp.lc; // an index, not a pointer, goes through after(): the reads stay LDS reads
+    if (SH::SPLIT)
+#pragma unroll
+        for (int r = 0; r < KL && r < HMAX; r++)
+            e[r] = tp.last[li + r * SH::LROW];
+    float prev[NV];
     float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
-    for (int r = 0; r < HMAX; r++) {
-        if (EXACT || r < c.h) {
+    for (int r = 0; r <= HMAX; r++) {
+        if (EXACT || r <= c.h) {
 #pragma unroll
             for (int k = 0; k < NV; k++) {
                 float a0, a1;
-                Row<T>::pair(rs, ob[k], (unsigned)(r + 1) * pitch, a0, a1);
+                Row<T>::cp(raw[r][k], a0, a1);
                 const float cur = fmaf(a1 - a0, fx, a0);
-                const float jv = fmaf(cur - prev[k], fy, prev[k]);
-                const float it = fmaf(jv, -ks, tp.t[r][k]); // T - J, the product by ks exact
-                prev[k] = cur;
-                if (RES) {
-                    const float q = ln.on[k] ? it * (1.0f / 64.0f) : 0.0f;
-                    s0 = s0 + fabsf(q);
-                    s1 = fmaf(q, q, s1);
-                } else {
-                    s0 = fmaf(it, tp.tx[r][k], s0);
-                    s1 = fmaf(it, tp.ty[r][k], s1);
+                if (r > 0) {
+                    const float jv = fmaf(cur - prev[k], fy, prev[k]);
+                    const bool lds = SH::SPLIT && k == NV - 1;
+                    const int kr = lds ? 0 : k;
+                    const float tt = lds ? e[r - 1].x : tp.t[r - 1][kr], ttx = lds ? e[r - 1].y : tp.tx[r - 1][kr];
+                    const float tty = lds ? e[r - 1].z : tp.ty[r - 1][kr];
+                    const float it = fmaf(jv, -ks, tt); // T - J, the product by ks exact
+                    if (RES) {
+                        const float q = SH::full(k) || ln.on[k] ? it * (1.0f / 64.0f) : 0.0f;
+                        s0 = s0 + fabsf(q);
+                        s1 = fmaf(q, q, s1);
+                    } else {
+                        s0 = fmaf(it, ttx, s0);
+                        s1 = fmaf(it, tty, s1);
+                    }
                 }
+                prev[k] = cur;
             }
+            // the next loads, once this row is done: image rows K ahead, template entries KL ahead
+            if (r + K <= HMAX && (EXACT || r + K <= c.h))
+#pragma unroll
+                for (int k = 0; k < NV; k++) {
+                    ob[k] = after(ob[k], prev[k]);
+                    raw[r + K][k] = Row<T>::lp(rs, ob[k], (unsigned)(r + K) * pitch);
+                }
+            if (SH::SPLIT && r >= 1 && (r - 1) % KL == 0 && r - 1 + KL < HMAX) {
+                li = after(li, s0);
+#pragma unroll
+                for (int q = r - 1 + KL; q < r - 1 + 2 * KL && q < HMAX; q++)
+                    e[q] = tp.last[li + q * SH::LROW];
+            }
+        }
+    }
+    b[0] = row_sum(s0);
+    b[1] = row_sum(s1);
+}
+
+// STG: the neighbourhood of J with corner (rx, ry) into the point's LDS region: RH rows of RW elements as floats in the
+// level's own units. Lane j of the point's row loads columns j, j + 16, ...; the loads go out CH rows at a time.
+// Elements outside the level are never read by a window that passed the bounds test (the margin only), so they are
+// left as the buffer resource returns them.
+template <int NV, int HMAX, int WX, typename T>
+__device__ __forceinline__ void stage(__amdgpu_buffer_rsrc_t rs, unsigned pitch, int rx, int ry, int j, float *R)
+{
+    typedef Shape<NV, HMAX, WX> SH;
+    constexpr int NQ = (SH::RW + 15) / 16, CH = 7;
+    unsigned o = (unsigned)ry * pitch + (unsigned)(rx + j) * (unsigned)sizeof(T);
+    wave_sync(); // every earlier read of the region is done
+#pragma unroll
+    for (int r0 = 0; r0 < SH::RH; r0 += CH) {
+        float v[CH][NQ];
+#pragma unroll
+        for (int r = r0; r < r0 + CH && r < SH::RH; r++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                if (q < NQ - 1 || j + 16 * q < SH::RW)
+                    v[r - r0][q] = Row<T>::l1(rs, o + 16u * q * (unsigned)sizeof(T), (unsigned)r * pitch);
+#pragma unroll
+        for (int r = r0; r < r0 + CH && r < SH::RH; r++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                if (q < NQ - 1 || j + 16 * q < SH::RW)
+                    R[r * SH::RW + j + 16 * q] = v[r - r0][q];
+        o = after(o, v[0][0]); // the next chunk's loads after this chunk's
+    }
+    wave_sync(); // the region before any lane reads it
+}
+
+// iterate_vx (lk_vx.c:100-140) on the staged neighbourhood: the window's corner is (bx, by) in the region.
+template <int NV, int HMAX, int WX>
+__device__ __forceinline__ void sample_st(const float *R, int bx, int by, float fx, float fy, float ks,
+                                          const Lanes<NV> &ln, const Tmpl<NV, HMAX, WX> &tp, float *b)
+{
+    typedef Shape<NV, HMAX, WX> SH;
+    constexpr int K = 4; // rows read ahead of their use
+    int base = by * SH::RW + bx;
+    float raw0[HMAX + 1][NV], raw1[HMAX + 1][NV];
+#pragma unroll
+    for (int r = 0; r < K; r++)
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            raw0[r][k] = R[base + r * SH::RW + (int)ln.col[k]];
+            raw1[r][k] = R[base + r * SH::RW + (int)ln.col[k] + 1];
+        }
+    constexpr int KL = 3;
+    float4 e[HMAX];
+    int li = tp.lc;
+    if (SH::SPLIT)
+#pragma unroll
+        for (int r = 0; r < KL && r < HMAX; r++)
+            e[r] = tp.last[li + r * SH::LROW];
+    float prev[NV];
+    float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+    for (int r = 0; r <= HMAX; r++) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const float a0 = raw0[r][k], a1 = raw1[r][k];
+            const float cur = fmaf(a1 - a0, fx, a0);
+            if (r > 0) {
+                const float jv = fmaf(cur - prev[k], fy, prev[k]);
+                const bool lds = SH::SPLIT && k == NV - 1;
+                const int kr = lds ? 0 : k;
+                const float tt = lds ? e[r - 1].x : tp.t[r - 1][kr], ttx = lds ? e[r - 1].y : tp.tx[r - 1][kr];
+                const float tty = lds ? e[r - 1].z : tp.ty[r - 1][kr];
+                const float it = fmaf(jv, -ks, tt); // T - J, the product by ks exact
+                s0 = fmaf(it, ttx, s0);
+                s1 = fmaf(it, tty, s1);
+            }
+            prev[k] = cur;
+        }
+        if (r + K <= HMAX) {
+            base = after(base, prev[0]);
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                raw0[r + K][k] = R[base + (r + K) * SH::RW + (int)ln.col[k]];
+                raw1[r + K][k] = R[base + (r + K) * SH::RW + (int)ln.col[k] + 1];
+            }
+        }
+        if (SH::SPLIT && r >= 1 && (r - 1) % KL == 0 && r - 1 + KL < HMAX) {
+            li = after(li, s0);
+#pragma unroll
+            for (int q = r - 1 + KL; q < r - 1 + 2 * KL && q < HMAX; q++)
+                e[q] = tp.last[li + q * SH::LROW];
         }
     }
     b[0] = row_sum(s0);
@@ -410,33 +656,55 @@ __device__ __forceinline__ Solve solve2(double gxx, double gxy, double gyy, doub
     return s;
 }
 
-template <int NV, int HMAX, bool EXACT, typename T0, typename T1>
+template <int NV, int HMAX, int WX, typename T0, typename T1>
 __device__ __forceinline__ void build_at(int L, const Level &lv, double px, double py, const Cfg &c,
-                                         const Lanes<NV> &ln, Tmpl<NV, HMAX> &tp, float *G)
+                                         const Lanes<NV> &ln, Tmpl<NV, HMAX, WX> &tp, float *G)
 {
     if (std::is_same<T0, T1>::value || L > 0)
-        build<NV, HMAX, EXACT, T1>(lv, px, py, c, ln, tp, G);
+        build<NV, HMAX, WX, T1>(lv, px, py, c, ln, tp, G);
     else
-        build<NV, HMAX, EXACT, T0>(lv, px, py, c, ln, tp, G);
+        build<NV, HMAX, WX, T0>(lv, px, py, c, ln, tp, G);
 }
 
-template <int NV, int HMAX, bool EXACT, typename T0, typename T1>
-__device__ __forceinline__ void sample_at(int L, const Level &lv, double qx, double qy, const Cfg &c,
-                                          const Lanes<NV> &ln, const Tmpl<NV, HMAX> &tp, float *b)
+// The level J is read at: its buffer resource and pitch in bytes for the element type of level L.
+template <typename T0, typename T1>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_at(int L, const Level &lv, unsigned &pitch)
 {
     if (std::is_same<T0, T1>::value || L > 0)
-        sample<NV, HMAX, EXACT, T1, false>(lv, qx, qy, c, ln, tp, b);
+        return level_rsrc<T1>(lv, pitch);
+    return level_rsrc<T0>(lv, pitch);
+}
+
+template <int NV, int HMAX, int WX, typename T0, typename T1>
+__device__ __forceinline__ void stage_at(int L, __amdgpu_buffer_rsrc_t rs, unsigned pitch, int rx, int ry, int j,
+                                         float *R)
+{
+    if (std::is_same<T0, T1>::value || L > 0)
+        stage<NV, HMAX, WX, T1>(rs, pitch, rx, ry, j, R);
     else
-        sample<NV, HMAX, EXACT, T0, false>(lv, qx, qy, c, ln, tp, b);
+        stage<NV, HMAX, WX, T0>(rs, pitch, rx, ry, j, R);
+}
+
+template <int NV, int HMAX, int WX, typename T0, typename T1>
+__device__ __forceinline__ void sample_at(int L, __amdgpu_buffer_rsrc_t rs, unsigned pitch, double qx, double qy,
+                                          const Cfg &c, const Lanes<NV> &ln, const Tmpl<NV, HMAX, WX> &tp, float *b)
+{
+    if (std::is_same<T0, T1>::value || L > 0)
+        sample<NV, HMAX, WX, T1, false>(rs, pitch, qx, qy, c, ln, tp, b);
+    else
+        sample<NV, HMAX, WX, T0, false>(rs, pitch, qx, qy, c, ln, tp, b);
 }
 
 // One point from pyramid I to J: the reference's level walk (lk_driver.inc:7-109). Every value is the same in the 16
 // lanes of the point's row.
-template <int NV, int HMAX, bool EXACT, typename T0, typename T1>
+template <int NV, int HMAX, int WX, typename T0, typename T1>
 __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c, bool mixed, const Lanes<NV> &ln,
-                      float x, float y, float gx0, float gy0, Track &o)
+                      char *lds, float x, float y, float gx0, float gy0, Track &o)
 {
-    Tmpl<NV, HMAX> tp;
+    Tmpl<NV, HMAX, WX> tp;
+    tp.last = (float4 *)lds;
+    tp.R = (float *)(lds + Shape<NV, HMAX, WX>::LDS_POINT * 16);
+    tp.lc = ln.j < Shape<NV, HMAX, WX>::TAIL ? ln.j : Shape<NV, HMAX, WX>::TAIL;
     double gx = ldexp((double)gx0, -top), gy = ldexp((double)gy0, -top);
     const double area = (double)(c.w * c.h), eps2 = (double)c.eps * (double)c.eps;
     const int mx = c.w / 2 + 1, my = c.h / 2 + 1;
@@ -455,7 +723,7 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             continue;
         }
         float G[3];
-        build_at<NV, HMAX, EXACT, T0, T1>(L, li, px, py, c, ln, tp, G);
+        build_at<NV, HMAX, WX, T0, T1>(L, li, px, py, c, ln, tp, G);
         const double gxx = G[0], gxy = G[1], gyy = G[2];
         const Solve s = solve2(gxx, gxy, gyy, area, mixed);
         if (L == 0)
@@ -473,6 +741,17 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
         }
         const double lx = (double)(lj.w - mx - 1), ly = (double)(lj.h - my - 1);
         const double sx = px + gx, sy = py + gy; // the reference adds px + gx first, then the level's step
+        unsigned pj;
+        const __amdgpu_buffer_rsrc_t rsj = rsrc_at<T0, T1>(L, lj, pj); // once per level, not per iteration
+        typedef Shape<NV, HMAX, WX> SH;
+        int rx = 0, ry = 0; // STG: the staged region's corner
+        if (SH::STG) {
+            // the corner stays inside the level: a negative lane offset would wrap, and rows reached through the
+            // scalar row step past it would be range-checked as out of the level
+            rx = max((int)floor(sx) - c.w / 2 - SH::M, 0);
+            ry = max((int)floor(sy) - c.h / 2 - SH::M, 0);
+            stage_at<NV, HMAX, WX, T0, T1>(L, rsj, pj, rx, ry, ln.j, tp.R);
+        }
         double vx = 0, vy = 0;
         for (int it = 0; it < c.max_iter; it++) {
             const double qx = sx + vx, qy = sy + vy;
@@ -482,7 +761,21 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
                 break;
             }
             float b[2];
-            sample_at<NV, HMAX, EXACT, T0, T1>(L, lj, qx, qy, c, ln, tp, b);
+            if (SH::STG) {
+                const double flx = floor(qx), fly = floor(qy);
+                const int bx = (int)flx - c.w / 2 - rx, by = (int)fly - c.h / 2 - ry;
+                if (bx < 0 || by < 0 || bx + c.w + 1 > SH::RW || by + c.h + 1 > SH::RH) {
+                    // the window left the region: stage again around it
+                    rx = max(rx + bx - SH::M, 0);
+                    ry = max(ry + by - SH::M, 0);
+                    stage_at<NV, HMAX, WX, T0, T1>(L, rsj, pj, rx, ry, ln.j, tp.R);
+                }
+                const float ks = (std::is_same<T0, T1>::value || L > 0) ? Ld<T1>::scale : Ld<T0>::scale;
+                sample_st<NV, HMAX, WX>(tp.R, (int)flx - c.w / 2 - rx, (int)fly - c.h / 2 - ry, (float)(qx - flx),
+                                        (float)(qy - fly), ks, ln, tp, b);
+            } else {
+                sample_at<NV, HMAX, WX, T0, T1>(L, rsj, pj, qx, qy, c, ln, tp, b);
+            }
             const double bx = (double)b[0] / 64.0, by = (double)b[1] / 64.0;
             const double ex = s.ia * bx + s.ib * by, ey = s.ib * bx + s.ic * by;
             vx += ex;
@@ -509,7 +802,9 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             status |= LK_OUT;
         } else {
             float r[2];
-            sample<NV, HMAX, EXACT, T0, true>(l0, qx, qy, c, ln, tp, r);
+            unsigned p0;
+            const __amdgpu_buffer_rsrc_t rs0 = level_rsrc<T0>(l0, p0);
+            sample<NV, HMAX, WX, T0, true>(rs0, p0, qx, qy, c, ln, tp, r);
             const int n = c.w * c.h;
             o.residual = mean_ld(r[0], n);
             if (o.residual > c.max_res)
@@ -526,10 +821,10 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
 
 // A point and its flowback (lk_driver.inc:111-121): the second pass tracks the estimate back from J to I, the template
 // rebuilt from J. One call site of track for both passes.
-template <int NV, int HMAX, bool EXACT, typename T0, typename T1>
+template <int NV, int HMAX, int WX, typename T0, typename T1>
 __device__ __forceinline__ void track_fb(const PyrView &I, const PyrView &J, int top, int fb_top, const Cfg &c,
-                                         bool mixed, float fb_max, const Lanes<NV> &ln, float x, float y, float gx,
-                                         float gy, Track &o)
+                                         bool mixed, float fb_max, const Lanes<NV> &ln, char *lds, float x,
+                                         float y, float gx, float gy, Track &o)
 {
     for (int pass = 0; pass < 2; pass++) {
         if (pass == 1 && !(fb_max > 0.0f && o.status == 0))
@@ -538,7 +833,8 @@ __device__ __forceinline__ void track_fb(const PyrView &I, const PyrView &J, int
         const float sx = pass ? o.x : x, sy = pass ? o.y : y;
         const float ux = pass ? (float)((double)x - (double)o.x) : gx;
         const float uy = pass ? (float)((double)y - (double)o.y) : gy;
-        track<NV, HMAX, EXACT, T0, T1>(pass ? J : I, pass ? I : J, pass ? fb_top : top, c, mixed, ln, sx, sy, ux, uy, t);
+        track<NV, HMAX, WX, T0, T1>(pass ? J : I, pass ? I : J, pass ? fb_top : top, c, mixed, ln, lds, sx, sy, ux, uy,
+                                    t);
         if (pass == 0) {
             o = t;
         } else {
@@ -578,9 +874,10 @@ __device__ __forceinline__ int pair_of(const int *pre, int npairs, int s)
     return lo;
 }
 
-template <int NV, int HMAX, bool EXACT, typename T0, typename T1>
+template <int NV, int HMAX, int WX, typename T0, typename T1>
 __global__ __launch_bounds__(512) void lk_kernel(LkArgs a)
 {
+    extern __shared__ float4 lk_lds[];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     // workgroup b runs on XCD b mod 8; XCD x takes the contiguous run of workgroups [x per_xcd, (x + 1) per_xcd)
@@ -604,38 +901,39 @@ __global__ __launch_bounds__(512) void lk_kernel(LkArgs a)
     }
     const Lanes<NV> ln = lanes<NV>(a.cfg, lane & 15);
     const int top = a.top[q];
+    char *lds = (char *)lk_lds + (wave * PPW + (lane >> 4)) * Shape<NV, HMAX, WX>::LDS_BYTES;
     Track o;
-    track_fb<NV, HMAX, EXACT, T0, T1>(*a.I[q], *a.J[q], top, a.fb_top >= 0 ? a.fb_top : top, a.cfg,
-                               a.solve == VTG_SOLVE_MIXED, a.fb_max, ln, x, y, gx, gy, o);
+    track_fb<NV, HMAX, WX, T0, T1>(*a.I[q], *a.J[q], top, a.fb_top >= 0 ? a.fb_top : top, a.cfg,
+                                   a.solve == VTG_SOLVE_MIXED, a.fb_max, ln, lds, x, y, gx, gy, o);
     if (live && (lane & 15) == 0)
         store(a.out + pt, o);
 }
 
-// The compiled window shapes: NV blocks of 16 columns and HMAX rows. The windows of every workload here, 15 x 15 and
-// 21 x 21, have their own exact shapes (straight-line code); any other window runs the guarded form of its block
-// count with 31 rows.
-template <int NV, int HMAX, bool EXACT>
+// The compiled window shapes: the windows of every workload here, 15 x 15 and 21 x 21, have their own exact shapes
+// (straight-line code); any other window runs the guarded form of its block count with 31 rows.
+template <int NV, int HMAX, int WX>
 int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 block)
 {
+    const size_t lds = (size_t)a.waves * PPW * Shape<NV, HMAX, WX>::LDS_BYTES;
     switch (storage) {
     case VTG_STORE_Q6:
-        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, uint8_t, uint16_t>), grid, block, 0, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, WX, uint8_t, uint16_t>), grid, block, lds, c->stream, a);
         break;
     case VTG_STORE_F32:
     case VTG_STORE_F32_U8:
-        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, float, float>), grid, block, 0, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, WX, float, float>), grid, block, lds, c->stream, a);
         break;
     case VTG_STORE_Q6W:
-        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, uint16_t, uint16_t>), grid, block, 0, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, WX, uint16_t, uint16_t>), grid, block, lds, c->stream, a);
         break;
     case VTG_STORE_Q6F:
-        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, uint8_t, float>), grid, block, 0, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, WX, uint8_t, float>), grid, block, lds, c->stream, a);
         break;
     case VTG_STORE_F16:
-        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, uint8_t, __half>), grid, block, 0, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, WX, uint8_t, __half>), grid, block, lds, c->stream, a);
         break;
     case VTG_STORE_BF16:
-        hipLaunchKernelGGL((lk_kernel<NV, HMAX, EXACT, uint8_t, __hip_bfloat16>), grid, block, 0, c->stream, a);
+        hipLaunchKernelGGL((lk_kernel<NV, HMAX, WX, uint8_t, __hip_bfloat16>), grid, block, lds, c->stream, a);
         break;
     default:
         return fail(c, VTG_EINVAL, "storage %d", storage);
@@ -644,18 +942,28 @@ int dispatch_storage(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 b
     return VTG_OK;
 }
 
+// LDS per point of the shape a window runs in.
+int shape_lds_point(const Cfg &cfg)
+{
+    if (cfg.w == 15 && cfg.h == 15)
+        return Shape<1, 15, 15>::LDS_BYTES;
+    if (cfg.w == 21 && cfg.h == 21)
+        return Shape<2, 21, 21>::LDS_BYTES;
+    return 0;
+}
+
 int dispatch_shape(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 block)
 {
     const int nv = (a.cfg.w + 15) / 16;
-    if (nv == 1 && a.cfg.h == 15)
-        return dispatch_storage<1, 15, true>(c, a, storage, grid, block);
-    if (nv == 2 && a.cfg.h == 21)
-        return dispatch_storage<2, 21, true>(c, a, storage, grid, block);
+    if (a.cfg.w == 15 && a.cfg.h == 15)
+        return dispatch_storage<1, 15, 15>(c, a, storage, grid, block);
+    if (a.cfg.w == 21 && a.cfg.h == 21)
+        return dispatch_storage<2, 21, 21>(c, a, storage, grid, block);
     if (nv == 1)
-        return dispatch_storage<1, 31, false>(c, a, storage, grid, block);
+        return dispatch_storage<1, 31, 0>(c, a, storage, grid, block);
     if (nv == 2)
-        return dispatch_storage<2, 31, false>(c, a, storage, grid, block);
-    return dispatch_storage<3, 31, false>(c, a, storage, grid, block);
+        return dispatch_storage<2, 31, 0>(c, a, storage, grid, block);
+    return dispatch_storage<3, 31, 0>(c, a, storage, grid, block);
 }
 
 // ---------------------------------------------------------------- fused pyramid + LK, one persistent launch
@@ -735,7 +1043,7 @@ __global__ __launch_bounds__(256) void fused_reset(int *ctr, int n, int clear_er
 // Bands of a level of one side in queue q: b = q, q + 8, ...
 __device__ __forceinline__ int queue_bands(int nb, int q) { return nb > q ? (nb - q + FQ - 1) / FQ : 0; }
 
-template <int NV, int HMAX, bool EXACT>
+template <int NV, int HMAX, int WX>
 __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a)
 {
     __shared__ int s_item;
@@ -743,6 +1051,7 @@ __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a)
     // first item of level l of pair k, both sides), then the q-th of 8 contiguous runs of all point blocks
     __shared__ int qa[(kMaxLevels - 1) * FB + 1];
     __shared__ int lds[rt::tile_lds_elems<FBAND>()];
+    extern __shared__ float4 lk_lds[];
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int np = a.lk.npairs, q = blockIdx.x % FQ;
     int *err = a.ctr + FERR, *done = a.ctr + FCTR;
@@ -870,21 +1179,24 @@ __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a)
                 }
             }
             const int top = a.lk.top[k];
+            char *pl = (char *)lk_lds + (wave * PPW + (lane >> 4)) * Shape<NV, HMAX, WX>::LDS_BYTES;
             Track o;
-            track_fb<NV, HMAX, EXACT, uint8_t, uint16_t>(*a.lk.I[k], *a.lk.J[k], top, top, a.lk.cfg,
-                                                  a.lk.solve == VTG_SOLVE_MIXED, a.lk.fb_max, ln, x, y, gx, gy, o);
+            track_fb<NV, HMAX, WX, uint8_t, uint16_t>(*a.lk.I[k], *a.lk.J[k], top, top, a.lk.cfg,
+                                                      a.lk.solve == VTG_SOLVE_MIXED, a.lk.fb_max, ln, pl, x, y, gx,
+                                                      gy, o);
             if (live && (lane & 15) == 0)
                 store(a.lk.out + pt, o);
         }
     }
 }
 
-template <int NV, int HMAX, bool EXACT>
+template <int NV, int HMAX, int WX>
 int launch_fused_t(vtg_ctx *c, const FusedArgs &a, int total)
 {
     static int per_cu = 0, cus = 0; // resident workgroups per CU of this kernel, CUs of device 0 class
+    const size_t lds = (size_t)4 * PPW * Shape<NV, HMAX, WX>::LDS_BYTES;
     if (!per_cu) {
-        VTG_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_kernel<NV, HMAX, EXACT>, 256, 0));
+        VTG_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_kernel<NV, HMAX, WX>, 256, lds));
         VTG_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
         if (per_cu < 1)
             per_cu = 1;
@@ -893,7 +1205,7 @@ int launch_fused_t(vtg_ctx *c, const FusedArgs &a, int total)
     const int grid = std::max(FQ, std::min((total + FQ - 1) / FQ * FQ, per_cu * cus / FQ * FQ));
     hipEvent_t ev;
     timer_begin(c, "pyr_lk_fused", &ev);
-    hipLaunchKernelGGL((fused_kernel<NV, HMAX, EXACT>), dim3(grid), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((fused_kernel<NV, HMAX, WX>), dim3(grid), dim3(256), lds, c->stream, a);
     VTG_HIP(c, hipGetLastError());
     timer_end(c, "pyr_lk_fused", ev);
     return VTG_OK;
@@ -902,15 +1214,15 @@ int launch_fused_t(vtg_ctx *c, const FusedArgs &a, int total)
 int launch_fused_shape(vtg_ctx *c, const FusedArgs &a, int total)
 {
     const int nv = (a.lk.cfg.w + 15) / 16;
-    if (nv == 1 && a.lk.cfg.h == 15)
-        return launch_fused_t<1, 15, true>(c, a, total);
-    if (nv == 2 && a.lk.cfg.h == 21)
-        return launch_fused_t<2, 21, true>(c, a, total);
+    if (a.lk.cfg.w == 15 && a.lk.cfg.h == 15)
+        return launch_fused_t<1, 15, 15>(c, a, total);
+    if (a.lk.cfg.w == 21 && a.lk.cfg.h == 21)
+        return launch_fused_t<2, 21, 21>(c, a, total);
     if (nv == 1)
-        return launch_fused_t<1, 31, false>(c, a, total);
+        return launch_fused_t<1, 31, 0>(c, a, total);
     if (nv == 2)
-        return launch_fused_t<2, 31, false>(c, a, total);
-    return launch_fused_t<3, 31, false>(c, a, total);
+        return launch_fused_t<2, 31, 0>(c, a, total);
+    return launch_fused_t<3, 31, 0>(c, a, total);
 }
 
 int check_cfg(vtg_ctx *c, const lk_cfg *cfg, int solve_mode)
@@ -972,6 +1284,14 @@ static int launch_group(vtg_ctx *c, const std::vector<int> &sel, const vtg_pyr *
     a.out = out;
     if (waves < 1 || waves > 8)
         return fail(c, VTG_EINVAL, "vtg_lk_track: %d waves per workgroup (1..8)", waves);
+    // fewer wavefronts per workgroup where the shape's LDS would pass the device's per-workgroup limit (the schedule
+    // changes, the records do not)
+    static int max_lds = 0;
+    if (!max_lds)
+        VTG_HIP(c, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    const int per_wave = PPW * shape_lds_point(a.cfg);
+    while (waves > 1 && per_wave * waves > max_lds)
+        waves--;
     a.waves = waves;
     const int wgs = (a.slots / PPW + a.waves - 1) / a.waves;
     a.per_xcd = (wgs + NXCD - 1) / NXCD;

@@ -62,7 +62,7 @@ struct vtg_ctx {
     hipEvent_t join = nullptr;               // for vtg_ctx_wait
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;
-    int lk_waves = 4;                        // LK schedule: wavefronts per workgroup where no pick applies
+    int lk_waves = 1;                        // LK schedule: wavefronts per workgroup where no pick applies
     // the autotuner's picks per size class (keys from pyr_class / lk_class)
     std::map<uint64_t, int> pyr_rows;
     std::map<uint64_t, vtg::LkSched> lk_sched;

@@ -307,7 +307,7 @@ def test_schedules_bit_identical_and_batch(ctx, oracle):
     for wv in (1, 2, 8):
         assert results[wv].tobytes() == results[4].tobytes(), wv
     assert Lb.vtg_lk_set_waves(ctx.h, 9) == -22
-    ctx.check(Lb.vtg_lk_set_waves(ctx.h, 4))
+    ctx.check(Lb.vtg_lk_set_waves(ctx.h, 1))
     for k, (PI, PJ, top) in enumerate(pairs):
         one = ctx.lk_track_host(PI, PJ, top, cfg, xys[k][0::2], xys[k][1::2], fb_max=0.5)
         assert one.tobytes() == results[4][offs[k]:offs[k + 1]].tobytes(), k
