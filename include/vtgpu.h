@@ -242,7 +242,9 @@ int vtg_ncc_search_many(vtg_ctx *ctx, int n, const vtg_pyr *const *p, const vtg_
 /* vtg_ncc_search_many with its stages on two contexts of one device: the cross terms (the map's matrix products) on
  * map_ctx, then, ordered after them, the scoring, top 8, refinement and pick on finish_ctx. A caller overlaps the
  * finish with other work on map_ctx (e.g. LK), joining finish_ctx with vtg_ctx_wait before reading found_dev. The
- * scratch is map_ctx's; finish_ctx must not be destroyed while the search is in flight. */
+ * working memory is map_ctx's scratch, and finish_ctx reads it until its work is done: join finish_ctx into map_ctx
+ * (vtg_ctx_wait(map_ctx, finish_ctx)) before the next search on map_ctx, split or not, or the two race. finish_ctx
+ * must not be destroyed while the search is in flight. */
 int vtg_ncc_search_many_split(vtg_ctx *map_ctx, vtg_ctx *finish_ctx, int n, const vtg_pyr *const *p,
                               const vtg_tmpl *const *t, const int *region, int step, struct vtg_cell *const *cells,
                               struct vtg_found *found_dev);
@@ -313,6 +315,10 @@ struct vtg_door_cfg {
     uint32_t slots;           /* the host's frame slots (vt_config.slots) */
     uint64_t slot_bytes;      /* bytes from one slot to the next */
     int dev_slots;            /* frames taken and unanswered, at most */
+    int stereo;               /* camera 0's frames are stereo pairs (vt_config.stereo): two images per slot, the left
+                                 one tracked */
+    int max_inflight;         /* frames with the server at once, at most (0: 1); the other taken frames wait landed,
+                                 where a LIVE frame can skip them */
 };
 struct vtg_door_state {       /* what the reference reports back in its control area (vt_ctl) */
     uint64_t frames_taken;    /* descriptors consumed: their slots are free */

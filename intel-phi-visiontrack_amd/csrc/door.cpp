@@ -2,14 +2,18 @@
 // (src/common/vt_proto.h:143-197), a ring of 88-byte descriptors published by a count, each naming a host slot that
 // holds its frame, and are taken as the reference's ingest takes them (src/card/session_track.c:76-139, 188-222):
 //   - every descriptor is checked before anything acts on it, in the reference's order: magic and check word, then
-//     type / destination / camera, then slot, then length, then the camera's instance order; a bad one is counted as
-//     an exception (the last one kept), its slot is free at once, and the session goes on;
+//     type / destination / camera, then slot, then length (vt_slot_bytes: w x h, twice that for a stereo camera 0,
+//     rounded up to 64 bytes, vt_proto.h:283-287), then the camera's instance order; a bad one is counted as an
+//     exception (the last one kept), its slot is free at once, and the session goes on;
+//   - a good frame lands: its image is copied out of the host slot (the reference's DMA into a device slot), so the
+//     slot is free once the descriptor is taken;
 //   - backpressure: at most dev_slots frames are taken and not yet answered; past that the door stops taking
 //     descriptors, so the host's slots fill and its own policy acts;
-//   - a LIVE frame goes ahead of what waits before it: when the newest frame taken in a call asks for LIVE priority,
-//     the frames taken before it in that call are skipped (their slots free, no results);
-//   - a good frame is handed to the serving loop (vtg_srv_submit copies it, so its slot is free once submitted) with
-//     its camera's points; answers come back in the order the frames were taken.
+//   - landed frames go to the serving loop while fewer than max_inflight of the door's frames run there (the
+//     reference builds a frame's pyramid when a set is free); when the newest landed frame asks for LIVE priority,
+//     every landed frame before it is skipped (no result), whichever call took it (build_one, session_track.c:197-206);
+//   - answers come back in the order the frames went to the server. A submit that fails leaves its frame landed, to
+//     go at the next call.
 // Host code over the serving loop; the GPU work is the server's.
 #include <cstring>
 #include <deque>
@@ -32,6 +36,13 @@ uint64_t msg_check(const vtg_msg *m)
     return h;
 }
 
+// The bytes a frame of camera k takes in its slot (vt_slot_bytes, src/common/vt_proto.h:283-287).
+uint64_t slot_bytes(const vtg_door_cfg &c, int k)
+{
+    const uint64_t one = (uint64_t)c.w[k] * (uint64_t)c.h[k];
+    return ((k == 0 && c.stereo ? 2 : 1) * one + 63) & ~63ull;
+}
+
 }  // namespace
 
 struct vtg_door {
@@ -44,12 +55,57 @@ struct vtg_door {
     bool have_instance[3] = {false, false, false};
     uint64_t exceptions = 0, skipped = 0, submitted = 0, answered = 0;
     uint32_t last_exception = VTG_EXC_NONE;
+    struct Landed {
+        vtg_msg msg;
+        std::vector<uint8_t> image;      // the frame (camera 0's left image when stereo)
+    };
+    std::deque<Landed> landed;           // taken, not yet with the server
     struct Pending {
         vtg_msg msg;
         uint64_t ticket;
     };
-    std::deque<Pending> inflight;
+    std::deque<Pending> inflight;        // with the server, not yet answered
 };
+
+namespace {
+
+// Hand landed frames to the server while fewer than max_inflight run there, the newest LIVE one first.
+int pump(vtg_door *d)
+{
+    const vtg_door_cfg &c = d->cfg;
+    const size_t cap = c.max_inflight > 0 ? (size_t)c.max_inflight : 1;
+    while (!d->landed.empty() && d->inflight.size() < cap) {
+        if (d->landed.size() > 1 && d->landed.back().msg.priority == VTG_PRIO_LIVE) {
+            d->skipped += d->landed.size() - 1;
+            d->landed.erase(d->landed.begin(), d->landed.end() - 1);
+        }
+        vtg_door::Landed &f = d->landed.front();
+        const int k = (int)f.msg.camera;
+        uint64_t ticket = 0;
+        const int rc = vtg_srv_submit(d->srv, d->stream[k], f.image.data(), c.w[k], c.npoints[k],
+                                      d->xy[k].empty() ? nullptr : d->xy[k].data(), nullptr, -1, -1, &ticket);
+        if (rc)
+            return rc; // the frame stays landed
+        d->inflight.push_back(vtg_door::Pending{f.msg, ticket});
+        d->submitted++;
+        d->landed.pop_front();
+    }
+    return VTG_OK;
+}
+
+void report(const vtg_door *d, vtg_door_state *st)
+{
+    if (!st)
+        return;
+    st->frames_taken = d->taken;
+    st->exceptions = d->exceptions;
+    st->last_exception = d->last_exception;
+    st->skipped = d->skipped;
+    st->submitted = d->submitted;
+    st->in_flight = d->inflight.size() + d->landed.size();
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -60,11 +116,12 @@ int vtg_door_create(vtg_srv *srv, const struct vtg_door_cfg *cfg, const float *c
     if (!srv || !cfg || !xy || !out)
         return VTG_EINVAL;
     *out = nullptr;
-    if (cfg->cameras < 1 || cfg->cameras > 3 || cfg->slots < 1 || cfg->dev_slots < 1 || cfg->max_points < 0)
+    if (cfg->cameras < 1 || cfg->cameras > 3 || cfg->slots < 1 || cfg->dev_slots < 1 || cfg->max_points < 0 ||
+        cfg->max_inflight < 0)
         return VTG_EINVAL;
     for (int k = 0; k < cfg->cameras; k++)
         if (cfg->w[k] < 8 || cfg->h[k] < 8 || cfg->npoints[k] < 0 || cfg->npoints[k] > cfg->max_points ||
-            (cfg->npoints[k] > 0 && !xy[k]) || (uint64_t)cfg->w[k] * cfg->h[k] > cfg->slot_bytes)
+            (cfg->npoints[k] > 0 && !xy[k]) || slot_bytes(*cfg, k) > cfg->slot_bytes)
             return VTG_EINVAL;
     vtg_door *d = new vtg_door();
     d->srv = srv;
@@ -99,8 +156,7 @@ int vtg_door_ingest(vtg_door *d, const struct vtg_msg *desc, uint64_t frames_pub
     if (!d || (frames_pub > d->taken && (!desc || !slots)))
         return VTG_EINVAL;
     const vtg_door_cfg &c = d->cfg;
-    std::vector<vtg_msg> landed;
-    while (d->taken < frames_pub && d->inflight.size() + landed.size() < (size_t)c.dev_slots) {
+    while (d->taken < frames_pub && d->inflight.size() + d->landed.size() < (size_t)c.dev_slots) {
         vtg_msg m;
         std::memcpy(&m, &desc[d->taken % VTG_DESC_RING], sizeof m);
         uint32_t ex = VTG_EXC_NONE;
@@ -110,7 +166,7 @@ int vtg_door_ingest(vtg_door *d, const struct vtg_msg *desc, uint64_t frames_pub
             ex = VTG_EXC_BAD_TYPE;
         else if (m.slot >= c.slots)
             ex = VTG_EXC_BAD_SLOT;
-        else if ((uint64_t)m.length != (uint64_t)c.w[m.camera] * (uint64_t)c.h[m.camera])
+        else if ((uint64_t)m.length != slot_bytes(c, (int)m.camera))
             ex = VTG_EXC_BAD_LENGTH;
         else if (d->have_instance[m.camera] && m.instance <= d->last_instance[m.camera])
             ex = VTG_EXC_INSTANCE_ORDER;
@@ -122,33 +178,12 @@ int vtg_door_ingest(vtg_door *d, const struct vtg_msg *desc, uint64_t frames_pub
         }
         d->last_instance[m.camera] = m.instance;
         d->have_instance[m.camera] = true;
-        landed.push_back(m);
-    }
-    size_t first = 0;
-    if (landed.size() > 1 && landed.back().priority == VTG_PRIO_LIVE) {
-        first = landed.size() - 1;
-        d->skipped += first;
-    }
-    int rc = VTG_OK;
-    for (size_t i = first; i < landed.size() && !rc; i++) {
-        const vtg_msg &m = landed[i];
         const int k = (int)m.camera;
-        uint64_t ticket = 0;
-        rc = vtg_srv_submit(d->srv, d->stream[k], slots + (uint64_t)m.slot * c.slot_bytes, c.w[k], c.npoints[k],
-                            d->xy[k].empty() ? nullptr : d->xy[k].data(), nullptr, -1, -1, &ticket);
-        if (!rc) {
-            d->inflight.push_back(vtg_door::Pending{m, ticket});
-            d->submitted++;
-        }
+        const uint8_t *src = slots + (uint64_t)m.slot * c.slot_bytes;
+        d->landed.push_back(vtg_door::Landed{m, std::vector<uint8_t>(src, src + (size_t)c.w[k] * c.h[k])});
     }
-    if (st) {
-        st->frames_taken = d->taken;
-        st->exceptions = d->exceptions;
-        st->last_exception = d->last_exception;
-        st->skipped = d->skipped;
-        st->submitted = d->submitted;
-        st->in_flight = d->inflight.size();
-    }
+    const int rc = pump(d);
+    report(d, st);
     return rc;
 }
 
@@ -156,6 +191,9 @@ int vtg_door_result(vtg_door *d, int wait, struct vtg_door_result *r, struct lk_
 {
     if (!d || !r)
         return VTG_EINVAL;
+    int rc = pump(d);
+    if (rc)
+        return rc;
     if (d->inflight.empty())
         return VTG_EAGAIN;
     const vtg_door::Pending &p = d->inflight.front();
@@ -165,8 +203,8 @@ int vtg_door_result(vtg_door *d, int wait, struct vtg_door_result *r, struct lk_
     int nrec = 0;
     vtg_found found;
     double ms = 0.0;
-    int rc = wait ? vtg_srv_wait(d->srv, p.ticket, out, &nrec, &found, &ms)
-                  : vtg_srv_poll(d->srv, p.ticket, out, &nrec, &found, &ms);
+    rc = wait ? vtg_srv_wait(d->srv, p.ticket, out, &nrec, &found, &ms)
+              : vtg_srv_poll(d->srv, p.ticket, out, &nrec, &found, &ms);
     if (rc)
         return rc;
     r->instance = p.msg.instance;
@@ -177,7 +215,7 @@ int vtg_door_result(vtg_door *d, int wait, struct vtg_door_result *r, struct lk_
     r->latency_ms = ms;
     d->inflight.pop_front();
     d->answered++;
-    return VTG_OK;
+    return pump(d); // the next landed frame takes the freed place
 }
 
 }  // extern "C"

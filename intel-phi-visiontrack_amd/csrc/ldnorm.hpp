@@ -205,3 +205,123 @@ __host__ __device__ inline float sqrt_sum(float a, float b)
 }
 
 }  // namespace vtg
+
+namespace vtg {
+
+// Signed x87 long double values m * 2^e (m < 2^64, 0 for zero) and the operations the LK covariance takes
+// (lk_driver.inc:100-104): products and quotients rounded to a 64-bit significand (nearest, ties to even), as x87
+// rounds them at its default precision.
+struct XS {
+    bool neg;
+    unsigned __int128 m;
+    int e;
+};
+
+__host__ __device__ inline XS xs_from_float(float f)
+{
+    const FDec d = fdec(f);
+    return XS{d.m < 0, (unsigned __int128)(d.m < 0 ? -d.m : d.m), d.e};
+}
+
+__host__ __device__ inline XS xs_from_int(int64_t v)
+{
+    return XS{v < 0, (unsigned __int128)(v < 0 ? -v : v), 0};
+}
+
+__host__ __device__ inline XS xs_mul(XS a, XS b)
+{
+    XS r{a.neg != b.neg, a.m * b.m, a.e + b.e}; // both below 2^64
+    if (r.m)
+        round_sig(r.m, r.e, 64);
+    return r;
+}
+
+__host__ __device__ inline XS xs_div(XS a, XS b)
+{
+    // a / b = (a.m 2^s / b.m) 2^(a.e - b.e - s), with a.m and b.m first brought to 2^63 .. 2^64 and s = 64: a quotient
+    // of 64 or 65 bits, and a sticky bit below it for the remainder
+    if (!a.m)
+        return XS{false, 0, 0};
+    int la = 0, lb = 0;
+    while (!((a.m >> (63 - la)) & 1))
+        la++;
+    while (!((b.m >> (63 - lb)) & 1))
+        lb++;
+    const unsigned __int128 am = a.m << la, bm = b.m << lb;
+    const unsigned __int128 num = am << 64;
+    const unsigned __int128 q = num / bm, rem = num - q * bm;
+    XS r{a.neg != b.neg, (q << 1) | (rem != 0 ? 1 : 0), (a.e - la) - (b.e - lb) - 65};
+    round_sig(r.m, r.e, 64);
+    return r;
+}
+
+// gxx gyy - gxy gxy for floats, the exact difference of the two exact products rounded once to 64 bits.
+__host__ __device__ inline XS xs_det(float gxx, float gyy, float gxy)
+{
+    const XS p = xs_mul(xs_from_float(gxx), xs_from_float(gyy)), q = xs_mul(xs_from_float(gxy), xs_from_float(gxy));
+    // p and q are exact (48 significant bits at most); align both below bit 110 of one accumulator
+    if (!p.m && !q.m)
+        return XS{false, 0, 0};
+    const int tp = p.m ? p.e + 48 : -100000, tq = q.m ? q.e + 48 : -100000;
+    const int base = (tp > tq ? tp : tq) - 110;
+    const unsigned __int128 x = p.m ? shift_jam(p.m, p.e - base) : 0, y = q.m ? shift_jam(q.m, q.e - base) : 0;
+    // p >= 0 (gxx, gyy are sums of squares) and q >= 0
+    XS r;
+    if (x >= y)
+        r = XS{p.neg, x - y, base};
+    else
+        r = XS{!p.neg, y - x, base};
+    if (r.m)
+        round_sig(r.m, r.e, 64);
+    return r;
+}
+
+__host__ __device__ inline float xs_to_float(XS a)
+{
+    if (!a.m)
+        return a.neg ? -0.0f : 0.0f;
+    round_sig(a.m, a.e, 24);
+    const float f = ldexpf((float)(uint64_t)a.m, a.e);
+    return a.neg ? -f : f;
+}
+
+// Whether rounding the double d to float is settled even if d is off by a few of its own ulps: its discarded 29
+// significand bits are more than 16 ulps from the midpoint pattern (and d is a normal float's magnitude).
+__host__ __device__ inline bool float_settled(double d)
+{
+    const double a = d < 0 ? -d : d;
+    if (a == 0.0)
+        return true;
+    if (!(a > 0x1p-120 && a < 0x1p120))
+        return false;
+    uint64_t bits;
+    __builtin_memcpy(&bits, &a, 8);
+    const int64_t low = (int64_t)(bits & ((1ull << 29) - 1)) - (1ll << 28);
+    return low > 16 || low < -16;
+}
+
+// The covariance s^2 G^-1 of lk_driver.inc:100-104 in the reference's x87 arithmetic: s2 = r1 area / (area - 2),
+// then (float)(s2 gyy / det), (float)(-s2 gxy / det), (float)(s2 gxx / det). The fp64 result is taken where fp64 and
+// x87 must round to the same float; elsewhere the exact emulation decides.
+__host__ __device__ inline void cov_ld(float r1, int area, float gxx, float gyy, float gxy, float *cxx, float *cxy,
+                                       float *cyy)
+{
+    const double det = (double)gxx * gyy - (double)gxy * gxy;
+    const double s2 = (double)r1 * area / (area - 2);
+    const double a = s2 * gyy / det, b = -s2 * gxy / det, c = s2 * gxx / det;
+    if (float_settled(a) && float_settled(b) && float_settled(c) && det > 0) {
+        *cxx = (float)a;
+        *cxy = (float)b;
+        *cyy = (float)c;
+        return;
+    }
+    const XS d = xs_det(gxx, gyy, gxy);
+    XS s = xs_div(xs_mul(xs_from_float(r1), xs_from_int(area)), xs_from_int(area - 2));
+    *cxx = xs_to_float(xs_div(xs_mul(s, xs_from_float(gyy)), d));
+    s.neg = !s.neg;
+    *cxy = xs_to_float(xs_div(xs_mul(s, xs_from_float(gxy)), d));
+    s.neg = !s.neg;
+    *cyy = xs_to_float(xs_div(xs_mul(s, xs_from_float(gxx)), d));
+}
+
+}  // namespace vtg

@@ -46,6 +46,11 @@
 #include "reduce_tile.hpp"
 #include "vtg_internal.hpp"
 
+// No contraction of a product and a sum into one fused operation anywhere below: the driver's fp64 expressions round
+// each operation as the reference's x87 does (fused operations only where the reference's vector code fuses them,
+// written as fmaf).
+#pragma clang fp contract(off)
+
 namespace vtg {
 
 namespace {
@@ -809,11 +814,7 @@ __device__ void track(const PyrView &I, const PyrView &J, int top, const Cfg &c,
             o.residual = mean_ld(r[0], n);
             if (o.residual > c.max_res)
                 status |= LK_RESIDUAL;
-            const double s2 = (double)mean_ld(r[1], n) * area / (area - 2);
-            const double gxx0 = G0[0], gxy0 = G0[1], gyy0 = G0[2], det0 = gxx0 * gyy0 - gxy0 * gxy0;
-            o.cxx = (float)(s2 * gyy0 / det0);
-            o.cxy = (float)(-s2 * gxy0 / det0);
-            o.cyy = (float)(s2 * gxx0 / det0);
+            cov_ld(mean_ld(r[1], n), n, G0[0], G0[2], G0[1], &o.cxx, &o.cxy, &o.cyy);
         }
     }
     o.status = status;

@@ -1439,7 +1439,10 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
             bytes += (size_t)ncell[k] * 256 * sizeof(int); // the cross terms of the padded map
     }
     void *sc = nullptr;
-    c->peer = f != c ? f : nullptr; // the scratch is read on f's stream too: growing it waits for f as well
+    // A split search's finish context reads this scratch after the call returns (the caller joins it before the next
+    // search here, as vtgpu.h asks). The peer stays set, so growing the scratch later still waits for it.
+    if (f != c)
+        c->peer = f;
     int rc = scratch(c, bytes, &sc);
     if (rc)
         return rc;

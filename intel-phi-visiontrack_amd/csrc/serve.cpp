@@ -199,6 +199,12 @@ Layout layout(const vtg_srv *s, const std::vector<Req *> &rs)
     return L;
 }
 
+// The most a batch of one frame of a w x h stream with n points can take (points with guesses).
+size_t one_frame_bytes(int w, int h, int n)
+{
+    return r256(r256((size_t)w * h) + 2 * r256((size_t)8 * n) + r256((size_t)32 * n) + 64);
+}
+
 // Stage and launch a batch on lane ln (the server lock held; every call here is asynchronous on the lane's stream).
 // The batch's points are those of its requests with a previous frame, in request order.
 int launch(vtg_srv *s, Lane &ln, const std::vector<Req *> &rs, const Layout &L)
@@ -249,7 +255,6 @@ int launch(vtg_srv *s, Lane &ln, const std::vector<Req *> &rs, const Layout &L)
             region.insert(region.end(), {1, st.w, 1, st.h});
             r->search = nsearch++;
         }
-        st.frames++;
     }
     int rc = launch_pyr_build_many(c, ps.data(), (int)ps.size(), fr.data(), pitch.data(), nullptr);
     if (!rc && !I.empty())
@@ -261,6 +266,9 @@ int launch(vtg_srv *s, Lane &ln, const std::vector<Req *> &rs, const Layout &L)
         rc = vtg_ncc_search_many(c, nsearch, ncc_p.data(), ncc_t.data(), region.data(), 1, nullptr, fd);
     if (rc)
         return rc;
+    // each stream's frame now occupies its pyramid set: its next frame tracks from it (only once the launches went)
+    for (Req *r : rs)
+        s->streams[r->stream].frames++;
     // results to each stream's pinned staging
     pre = 0;
     for (Req *r : rs) {
@@ -346,8 +354,33 @@ void loop(vtg_srv *s)
                 st.queue.pop_front();
                 st.busy = true;
             }
-            if (rs.empty())
+            if (rs.empty()) {
+                // nothing fits: while no lane holds arena memory, a waiting frame that does not fit the whole free
+                // arena never will (other streams' pyramids took the room); finish it with VTG_ENOMEM
+                bool idle = true;
+                for (const Lane &o : s->lanes)
+                    idle &= !o.busy;
+                if (idle)
+                    for (Stream &st : s->streams) {
+                        if (!st.open || st.busy || st.queue.empty())
+                            continue;
+                        Req *r = st.queue.front();
+                        const std::vector<Req *> one{r};
+                        if (layout(s, one).total <= room)
+                            continue;
+                        st.queue.pop_front();
+                        r->rc = VTG_ENOMEM;
+                        r->t_done = Clock::now();
+                        r->done = true;
+                        s->st.completed++;
+                        snprintf(s->err, sizeof s->err, "serving: a %dx%d frame does not fit the free budget", st.w,
+                                 st.h);
+                        progress = true;
+                    }
+                if (progress)
+                    s->fin.notify_all();
                 break;
+            }
             s->stream_rr = (s->stream_rr + 1) % ns;
             const Layout L = layout(s, rs);
             ln.block = s->arena.alloc(L.total);
@@ -477,6 +510,12 @@ int vtg_srv_open(vtg_srv *s, int w, int h, int levels, int max_points, int *stre
     st.resident = s->arena.alloc(st.resident_bytes);
     if (!st.resident)
         return srv_fail(s, VTG_ENOMEM, "vtg_srv_open: the stream's resident pyramids exceed the free budget");
+    if (one_frame_bytes(w, h, max_points) > s->arena.largest()) {
+        // no frame of this stream could ever be admitted: refuse it now rather than queue frames that never run
+        s->arena.release(st.resident, st.resident_bytes);
+        return srv_fail(s, VTG_ENOMEM, "vtg_srv_open: the budget left after the resident pyramids holds no frame of "
+                                       "this stream");
+    }
     char *m = (char *)st.resident;
     int rc = pyr_new(s->main, w, h, levels, VTG_STORE_Q6, m, &st.pyr[0]);
     if (!rc)

@@ -96,7 +96,7 @@ EXC_NONE, EXC_BAD_CHECK, EXC_BAD_TYPE, EXC_BAD_SLOT, EXC_BAD_LENGTH, EXC_INSTANC
 class DoorCfg(C.Structure):
     _fields_ = [("cameras", C.c_int), ("w", C.c_int * 3), ("h", C.c_int * 3), ("levels", C.c_int * 3),
                 ("npoints", C.c_int * 3), ("max_points", C.c_int), ("slots", C.c_uint32),
-                ("slot_bytes", C.c_uint64), ("dev_slots", C.c_int)]
+                ("slot_bytes", C.c_uint64), ("dev_slots", C.c_int), ("stereo", C.c_int), ("max_inflight", C.c_int)]
 
 
 class DoorState(C.Structure):

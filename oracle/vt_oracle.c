@@ -459,93 +459,23 @@ static void lane_residual(const struct vt_plane *pl, long double qx, long double
 
 /* The pyramid walk in long double (lk_driver.inc:7-109): per level, skip if the window does not fit or G is too
  * weak (doubling the guess), else iterate with the level's inverse of G until the step is below eps. */
-static void lane_track(const struct vt_plane *I, const struct vt_plane *J, int top, const struct lk_cfg *c, float x,
-                       float y, float gx0, float gy0, struct lk_out *o)
-{
-    struct lane_tile tile;
-    long double gx = (long double)gx0 * pow2m_ld[top], gy = (long double)gy0 * pow2m_ld[top];
-    const long double area = (long double)(c->w * c->h);
-    const long double eps2 = (long double)c->eps * (long double)c->eps;
-    const int mx = c->w / 2 + 1, my = c->h / 2 + 1;
-    long double det0 = 0, gxx0 = 0, gxy0 = 0, gyy0 = 0;
-    int status = 0, iters = 0, valid0 = 0;
-    float eig0 = 0.0f;
-    for (int L = top; L >= 0; L--) {
-        const long double px = (long double)x * pow2m_ld[L], py = (long double)y * pow2m_ld[L];
-        if (px < mx || py < my || px >= (long double)(I[L].w - mx - 1) || py >= (long double)(I[L].h - my - 1)) {
-            if (L == 0)
-                status |= LK_OUT;
-            else
-                gx *= 2, gy *= 2;
-            continue;
-        }
-        float g[3];
-        lane_build(&I[L], px, py, c, &tile, g);
-        const long double gxx = g[0], gxy = g[1], gyy = g[2];
-        const long double det = gxx * gyy - gxy * gxy, tr = gxx + gyy, df = gxx - gyy;
-        const long double eig = (tr - sqrtl(df * df + 4 * gxy * gxy)) * 0.5L / area;
-        if (L == 0)
-            eig0 = (float)eig;
-        if (eig < (long double)c->min_eig || det <= 0) {
-            if (L == 0)
-                status |= LK_LOWEIG;
-            else
-                gx *= 2, gy *= 2;
-            continue;
-        }
-        if (L == 0) {
-            det0 = det, gxx0 = gxx, gxy0 = gxy, gyy0 = gyy;
-            valid0 = 1;
-        }
-        const long double ia = gyy / det, ib = -gxy / det, ic = gxx / det;
-        const long double lx = (long double)(J[L].w - mx - 1), ly = (long double)(J[L].h - my - 1);
-        long double vx = 0, vy = 0;
-        for (int it = 0; it < c->max_iter; it++) {
-            const long double qx = px + gx + vx, qy = py + gy + vy;
-            if (qx < mx || qy < my || qx >= lx || qy >= ly) {
-                if (L == 0)
-                    status |= LK_OUT, valid0 = 0;
-                break;
-            }
-            float b[2];
-            lane_iterate(&J[L], qx, qy, c, &tile, b);
-            const long double bx = (long double)b[0] / VT_Q6, by = (long double)b[1] / VT_Q6;
-            const long double ex = ia * bx + ib * by, ey = ib * bx + ic * by;
-            vx += ex;
-            vy += ey;
-            iters++;
-            if (ex * ex + ey * ey < eps2)
-                break;
-        }
-        gx += vx;
-        gy += vy;
-        if (L > 0)
-            gx *= 2, gy *= 2;
-    }
-    const long double qx = (long double)x + gx, qy = (long double)y + gy;
-    o->x = (float)qx;
-    o->y = (float)qy;
-    o->min_eig = eig0;
-    o->iters = (uint16_t)iters;
-    o->residual = 0.0f;
-    o->cov_xx = o->cov_xy = o->cov_yy = 0.0f;
-    if (valid0 && !status) {
-        if (qx < mx || qy < my || qx >= (long double)(J[0].w - mx - 1) || qy >= (long double)(J[0].h - my - 1)) {
-            status |= LK_OUT;
-        } else {
-            float r[2];
-            lane_residual(&J[0], qx, qy, c, &tile, r);
-            o->residual = r[0];
-            if (r[0] > c->max_residual)
-                status |= LK_RESIDUAL;
-            const long double s2 = (long double)r[1] * area / (area - 2);
-            o->cov_xx = (float)(s2 * gyy0 / det0);
-            o->cov_xy = (float)(-s2 * gxy0 / det0);
-            o->cov_yy = (float)(s2 * gxx0 / det0);
-        }
-    }
-    o->status = (uint16_t)status;
-}
+#define REAL long double
+#define SQRT sqrtl
+#define LANE_NAME lane_track
+#include "lane_track.inc"
+#undef REAL
+#undef SQRT
+#undef LANE_NAME
+
+/* The same walk with an fp64 driver: the patch arithmetic (and so every fp32 sum) is the lane tier's; only the
+ * driver's steps round to 53 instead of 64 bits. */
+#define REAL double
+#define SQRT sqrt
+#define LANE_NAME lane64_track
+#include "lane_track.inc"
+#undef REAL
+#undef SQRT
+#undef LANE_NAME
 
 /* ------------------------------------------------------------------ LK, f64 tier */
 
@@ -704,6 +634,8 @@ void vo_lk_track(int tier, const struct vt_plane *I, const struct vt_plane *J, i
 {
     if (tier == VO_F64)
         f64_track(I, J, top, c, x, y, gx, gy, o);
+    else if (tier == VO_LANE64)
+        lane64_track(I, J, top, c, x, y, gx, gy, o);
     else
         lane_track(I, J, top, c, x, y, gx, gy, o);
 }
@@ -975,6 +907,7 @@ struct vo_still *vo_still_new(const struct vtg_still_cfg *c)
     s->avg = c->avg;
     s->motion_px = c->motion_px;
     s->motion_grey = c->motion_grey;
+    s->lk_tier = VO_LANE;
     s->cols = (c->w + VTG_CELL_W - 1) / VTG_CELL_W;
     s->rows = (c->h + VTG_CELL_H - 1) / VTG_CELL_H;
     s->ncells = s->cols * s->rows;
@@ -1027,7 +960,7 @@ float vo_still_cell_diff(const struct vo_still *s, const struct vt_plane *cur, i
     return (float)(sum / (long double)((y1 - y0) * VTG_CELL_W));
 }
 
-/* still_vectors (still.c:113-138): LK (lane tier) from the mean to the frame at each cell's centre, with the flowback
+/* still_vectors (still.c:113-138): LK (of the tier s->lk_tier names) from the mean to the frame at each cell's centre, with the flowback
  * check; the vector, the difference in 8-bit grey, LK's status and sigma. */
 void vo_still_vectors(struct vo_still *s, const struct vt_plane *cur, int threads)
 {
@@ -1037,7 +970,7 @@ void vo_still_vectors(struct vo_still *s, const struct vt_plane *cur, int thread
     for (int i = 0; i < n; i++)
         vo_still_centre(s, i, &xs[i], &ys[i]);
     const struct lk_cfg cfg = vo_still_lk_cfg();
-    vo_lk_batch(VO_LANE, &s->mean, cur, 0, &cfg, n, xs, ys, NULL, NULL, ST_FB, o, threads);
+    vo_lk_batch(s->lk_tier, &s->mean, cur, 0, &cfg, n, xs, ys, NULL, NULL, ST_FB, o, threads);
     for (int i = 0; i < n; i++) {
         struct vtg_still_cell *c = &s->cell[i];
         c->u = (float)((long double)o[i].x - (long double)xs[i]);
@@ -1263,8 +1196,8 @@ long double vo_stereo_reprojection(const struct vtg_stereo_geom *g, const long d
  * tier) from the left to the right pyramid with the disparity as its x guess, from the stereo level (or level 0 with a
  * prior), the flowback from level 0; the match, its disparity, and the 3D point where the disparity is in range and it
  * reprojects within max_reproj. */
-void vo_stereo_match(const struct vt_plane *L, const struct vt_plane *R, const struct vtg_stereo_cfg *c, int n,
-                     struct vtg_stereo_feature *f)
+void vo_stereo_match_tier(int tier, const struct vt_plane *L, const struct vt_plane *R, const struct vtg_stereo_cfg *c,
+                          int n, struct vtg_stereo_feature *f)
 {
     for (int i = 0; i < n; i++) {
         struct vtg_stereo_feature *q = &f[i];
@@ -1294,9 +1227,9 @@ void vo_stereo_match(const struct vt_plane *L, const struct vt_plane *R, const s
             top = sl;
         }
         struct lk_out m;
-        vo_lk_track(VO_LANE, L, R, top, &c->lk, q->x, q->y, (float)(-(long double)guess), 0.0f, &m);
+        vo_lk_track(tier, L, R, top, &c->lk, q->x, q->y, (float)(-(long double)guess), 0.0f, &m);
         if (c->fb_max > 0.0f)
-            vo_lk_flowback(VO_LANE, L, R, 0, &c->lk, q->x, q->y, c->fb_max, &m);
+            vo_lk_flowback(tier, L, R, 0, &c->lk, q->x, q->y, c->fb_max, &m);
         if (m.status) {
             q->flags = fl;
             q->disparity = -1.0f;
@@ -1315,4 +1248,10 @@ void vo_stereo_match(const struct vt_plane *L, const struct vt_plane *R, const s
         }
         q->flags = fl;
     }
+}
+
+void vo_stereo_match(const struct vt_plane *L, const struct vt_plane *R, const struct vtg_stereo_cfg *c, int n,
+                     struct vtg_stereo_feature *f)
+{
+    vo_stereo_match_tier(VO_LANE, L, R, c, n, f);
 }

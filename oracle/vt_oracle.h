@@ -7,6 +7,9 @@
  *   lane  (reference-faithful)  fp32 patch arithmetic in the reference's 16-lane order (fmaf per lane, the
  *                               i^1,i^2,i^4,i^8 horizontal sum), long double driver: reproduces the reference's
  *                               committed hash lines bit for bit (src/kern/lk_vx.c, src/common/vx.h:158-252).
+ *   lane64                      the lane tier's fp32 patch arithmetic with the driver in double instead of x87
+ *                               long double: the arithmetic of the GPU's LK. Equal to the lane tier except on the
+ *                               rare records where a 53-bit driver step rounds a position differently.
  *   f64   (FP64 oracle)         every patch and driver operation in double: the yardstick each flow result's
  *                               error is reported against. The reference has no such tier.
  *   mt    (threaded baseline)   pthreads over rows (pyramid) and over points / cells (LK, NCC) for timing.
@@ -54,7 +57,7 @@ void vo_pyr_build(struct vt_plane *lv, const uint8_t *img, int stride, int level
 void vo_pyr_build_mt(struct vt_plane *lv, const uint8_t *img, int stride, int levels, int threads);
 
 /* --- LK (src/kern/lk_driver.inc, lk_vx.c, lk_ref.c) --- */
-enum { VO_LANE = 0, VO_F64 = 1 };
+enum { VO_LANE = 0, VO_F64 = 1, VO_LANE64 = 2 };
 void vo_lk_track(int tier, const struct vt_plane *I, const struct vt_plane *J, int top, const struct lk_cfg *c,
                  float x, float y, float gx, float gy, struct lk_out *o);
 void vo_lk_flowback(int tier, const struct vt_plane *I, const struct vt_plane *J, int top, const struct lk_cfg *c,
@@ -94,6 +97,7 @@ struct vo_still {
     uint32_t tracked, moving;
     float count_mean, drift_u, drift_v, drift_rms, diff_median, diff_limit;
     long double noise_usual;
+    int lk_tier;             /* the LK tier of still_vectors: VO_LANE (the reference's x87 driver) at creation */
 };
 struct vo_still *vo_still_new(const struct vtg_still_cfg *c);
 void vo_still_free(struct vo_still *s);
@@ -119,6 +123,9 @@ long double vo_stereo_reprojection(const struct vtg_stereo_geom *g, const long d
 /* the tracker's stereo step over n features; L, R: the two pyramids' levels 0 .. max(stereo_level, 0) */
 void vo_stereo_match(const struct vt_plane *L, const struct vt_plane *R, const struct vtg_stereo_cfg *c, int n,
                      struct vtg_stereo_feature *f);
+/* the same with the LK of the given tier (VO_LANE in vo_stereo_match) */
+void vo_stereo_match_tier(int tier, const struct vt_plane *L, const struct vt_plane *R, const struct vtg_stereo_cfg *c,
+                          int n, struct vtg_stereo_feature *f);
 
 #ifdef __cplusplus
 }

@@ -180,7 +180,7 @@ class VoStill(C.Structure):
                 ("frames", C.c_uint64), ("tracked", C.c_uint32), ("moving", C.c_uint32),
                 ("count_mean", C.c_float), ("drift_u", C.c_float), ("drift_v", C.c_float),
                 ("drift_rms", C.c_float), ("diff_median", C.c_float), ("diff_limit", C.c_float),
-                ("noise_usual", C.c_longdouble)]
+                ("noise_usual", C.c_longdouble), ("lk_tier", C.c_int)]
 
 
 class OracleStill:

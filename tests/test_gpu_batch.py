@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import vtgpu
-from gpu_util import say
+from gpu_util import lane_ref, say
 
 pytestmark = pytest.mark.gpu
 
@@ -131,7 +131,7 @@ def test_uneven_batch(ctx, oracle):
             assert np.array_equal(b.pyrs[k][1].read_level(lv).view(), k1[lv].view()), (k, lv)
         xy = b.xy_host[2 * b.offsets[k]:2 * b.offsets[k + 1]]
         xs, ys = xy[0::2], xy[1::2]
-        ref = oracle.lk_batch(0, I, J, L - 1, lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0), xs, ys, threads=16)
+        ref = lane_ref(oracle, I, J, L - 1, lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0), xs, ys, threads=16)
         _assert_same(got[b.offsets[k]:b.offsets[k + 1]], ref, f"uneven batch pair {k}")
     # the autotuner: one pair per size class, then the whole batch under the picks
     for (w, h, L), k in b.classes().items():

@@ -136,3 +136,71 @@ def test_door_live_and_backpressure():
     assert door.result(n, wait=False) is None
     door.destroy()
     srv.destroy()
+
+
+def _msg(cam, instance, slot, length, priority=vtgpu.PRIO_ORDERED):
+    m = vtgpu.Msg()
+    m.magic, m.type, m.dest, m.priority = vtgpu.MSG_MAGIC, vtgpu.MSG_FRAME, vtgpu.PIPE_ALL, priority
+    m.length, m.instance, m.slot, m.camera, m.acq_ns = length, instance, slot, cam, 1000 + instance
+    m.check = vtgpu.msg_check(m)
+    return m
+
+
+@pytest.mark.parametrize("stereo", [0, 1])
+def test_door_live_across_calls_and_slot_length(stereo):
+    """A 350 x 250 camera (w x h = 87,500, not a multiple of 64): a descriptor of length w x h is BAD_LENGTH and one of
+    vt_slot_bytes (rounded up to 64, twice w x h first for a stereo camera 0) is taken, as the reference's ingest
+    checks it (session_track.c:82, 107). With one frame at a time at the server, a LIVE frame published in a later
+    ingest call than an ordered frame still waiting skips it (build_one, session_track.c:197-206): the answers are the
+    first frame and the LIVE one, the waiting frame is counted as skipped."""
+    w, h = 350, 250
+    one = w * h
+    sb = ((2 if stereo else 1) * one + 63) & ~63
+    sf = StreamFrames(vtgpu, w, h, 90, k=6, nx=12, ny=10)
+    n = len(sf.xy) // 2
+    srv = vtgpu.Server(1 << 30, lanes=2, max_batch=8, cfg=vtgpu.lk_cfg(15, 15))
+    cfg = vtgpu.DoorCfg()
+    cfg.cameras, cfg.w[0], cfg.h[0], cfg.levels[0], cfg.npoints[0] = 1, w, h, L, n
+    cfg.max_points, cfg.slots, cfg.slot_bytes, cfg.dev_slots = 400, 8, sb, 8
+    cfg.stereo, cfg.max_inflight = stereo, 1
+    door = vtgpu.Door(srv, cfg, [sf.xy])
+    desc = (vtgpu.Msg * vtgpu.DESC_RING)()
+    slots = np.zeros(8 * sb, np.uint8)
+    pub = [0]
+
+    def publish(instance, frame, length=sb, priority=vtgpu.PRIO_ORDERED):
+        slot = pub[0] % 8
+        slots[slot * sb:slot * sb + one] = frame.ravel()
+        if stereo:
+            slots[slot * sb + one:slot * sb + 2 * one] = 255 - frame.ravel()
+        desc[pub[0] % vtgpu.DESC_RING] = _msg(0, instance, slot, length, priority)
+        pub[0] += 1
+
+    say("door live across calls, stereo", stereo)
+    publish(1, sf[0])
+    st = door.ingest(desc, pub[0], slots)
+    assert st["submitted"] == 1
+    publish(2, sf[1])
+    st = door.ingest(desc, pub[0], slots)
+    assert st["submitted"] == 1 and st["in_flight"] == 2   # frame 2 waits landed behind frame 1
+    publish(3, sf[2], length=one)                             # the unrounded size: dropped
+    publish(4, sf[3], priority=vtgpu.PRIO_LIVE)
+    st = door.ingest(desc, pub[0], slots)
+    assert st["exceptions"] == 1 and st["last_exception"] == vtgpu.EXC_BAD_LENGTH and st["frames_taken"] == 4
+    answers = []
+    while True:
+        r = door.result(n, wait=True)
+        if r is None:
+            break
+        answers.append(r)
+    st = door.ingest(desc, pub[0], slots)
+    assert [int(a.instance) for a, _ in answers] == [1, 4] and st["skipped"] == 1 and st["in_flight"] == 0
+    # frame 4 tracked from frame 1, the last one the stream built: the same records as a direct server run
+    direct = vtgpu.Server(1 << 30, lanes=2, max_batch=8, cfg=vtgpu.lk_cfg(15, 15))
+    sid = direct.open(w, h, L, 400)
+    direct.wait(direct.submit(sid, sf[0], sf.xy))
+    ref = direct.wait(direct.submit(sid, sf[3], sf.xy))[0]
+    assert answers[1][1].tobytes() == ref.tobytes()
+    direct.destroy()
+    door.destroy()
+    srv.destroy()

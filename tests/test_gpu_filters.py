@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import vtgpu
-from gpu_util import conditioning, say
+from gpu_util import conditioning, lane_ref, say
 from oracle_ffi import HostPlane, lk_cfg
 
 pytestmark = pytest.mark.gpu
@@ -165,7 +165,7 @@ def test_lk_on_filtered_pyramids(ctx, oracle):
     cfg = lk_cfg(15, 15, 10, 0.01, 1e-4, 40.0)
     say("lk filtered")
     got = ctx.lk_track_host(p0, p1, L - 1, vtgpu.LkCfg(15, 15, 10, 0.01, 1e-4, 40.0, 0), xs, ys)
-    ref = oracle.lk_batch(0, I, J, L - 1, cfg, xs, ys)
+    ref = lane_ref(oracle, I, J, L - 1, cfg, xs, ys)
     well, _ = conditioning(oracle, I, J, L - 1, cfg, xs, ys, threads=8)
     assert (got["status"] == ref["status"])[well].all()
     ok = well & (ref["status"] == 0)

@@ -264,6 +264,55 @@ def test_search_split_matches_single_context(ctx, tw, th):
     other.close()
 
 
+def test_split_searches_in_a_row(ctx):
+    """Split searches back to back on one map context, each joined as vtgpu.h asks before the next search there (the
+    second over smaller windows), then a one-context search needing more scratch than before (the scratch grows while
+    the earlier finish context may still hold it): every record equals the one-context call's."""
+    import ctypes as C
+    other = vtgpu.Context(0)
+    sizes = [(1920, 1080), (640, 480), (1280, 720)]
+    pyrs, tms = [], []
+    for k, (w, h) in enumerate(sizes):
+        f0, f1, _ = vtgpu.frame_pair(w, h, 950 + k)
+        p0, p1 = ctx.pyramid(w, h, 1), ctx.pyramid(w, h, 1)
+        p0.build(f0)
+        p1.build(f1)
+        tms.append(ctx.ncc_template(p0, 0, w // 3, h // 4, 64, 64))
+        pyrs.append((p0, p1))
+    big = [(1, w, 1, h) for (w, h) in sizes]
+    small = [(w // 4, w // 2, h // 4, h // 2) for (w, h) in sizes]
+    fs, n = C.sizeof(vtgpu.Found), len(sizes)
+    bufs = [ctx.alloc(fs * n) for _ in range(5)]
+    say("ncc split searches in a row")
+    ctx.ncc_search_many_split(other, [p[1] for p in pyrs], tms, big, 1, bufs[0])
+    ctx.wait(other)
+    ctx.ncc_search_many_split(other, [p[1] for p in pyrs], tms, small, 1, bufs[1])
+    ctx.wait(other)
+    many = [p[1] for p in pyrs] * 3
+    grow = ctx.alloc(fs * 3 * n)
+    ctx.ncc_search_many(many, tms * 3, big * 3, 1, grow)  # grows the scratch
+    ctx.ncc_search_many([p[1] for p in pyrs], tms, big, 1, bufs[2])
+    ctx.ncc_search_many([p[1] for p in pyrs], tms, small, 1, bufs[3])
+    ctx.sync()
+    got = []
+    for b in bufs[:4]:
+        g = (vtgpu.Found * n)()
+        ctx.check(vtgpu.lib().vtg_download(ctx.h, C.cast(g, C.c_void_p), b.ptr, fs * n))
+        got.append(g)
+    for k in range(n):
+        assert bytes(got[0][k]) == bytes(got[2][k]), ("big", k)
+        assert bytes(got[1][k]) == bytes(got[3][k]), ("small", k)
+        assert got[1][k].cells < got[0][k].cells
+    for b in bufs + [grow]:
+        b.free()
+    for t in tms:
+        t.destroy()
+    for p0, p1 in pyrs:
+        p0.destroy()
+        p1.destroy()
+    other.close()
+
+
 def test_f32u8_kept_frame_build_many(ctx):
     """F32_U8 pyramids of uneven sizes and depths built together keep each frame beside level 0 (written by the level-1
     reduce from its tile, or copied for a one-level pyramid): a level-0 search over each gives the record the same

@@ -107,3 +107,20 @@ def test_tight_budget_admission():
     with pytest.raises(vtgpu.VtgError):
         srv.open(3840, 2160, 5, 100)
     srv.destroy()
+
+
+def test_budget_too_small_for_any_frame():
+    """A stream whose resident pyramids fit the budget but whose frames never could (what is left is smaller than one
+    frame and its points) is refused at open with VTG_ENOMEM instead of queueing frames that never run; a stream that
+    fits is served as usual on the same server."""
+    srv = vtgpu.Server(8 << 20, lanes=1, max_batch=4, cfg=CFG)
+    say("serving budget too small")
+    with pytest.raises(vtgpu.VtgError) as e:
+        srv.open(1920, 1080, 4, 16200)   # 2 x 3.4 MB resident, then 2.9 MB per frame: 8 MB cannot hold both
+    assert "(-12)" in str(e.value)
+    sf = StreamFrames(vtgpu, 320, 240, 9, k=3, nx=10, ny=8)
+    sid = srv.open(320, 240, 3, len(sf.xy) // 2)
+    outs = [srv.wait(srv.submit(sid, sf[i], sf.xy)) for i in range(3)]
+    assert len(outs[0][0]) == 0 and len(outs[2][0]) == len(sf.xy) // 2
+    srv.close(sid)
+    srv.destroy()

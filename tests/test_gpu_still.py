@@ -1,8 +1,9 @@
 """Still-cell vectors on the GPU (vtg_still_*, src/kern/still.c) against the oracle's restatement (itself pinned to the
 reference's still.c in test_oracle_still.py), phase by phase over runs of frames with noise, a walking square and a
 jump in brightness:
-  vectors  each cell's mean difference bit for bit; LK's status equal and u, v within 0.01 px of the lane tier (the LK
-           tolerance), sigma within 1e-3 relative;
+  vectors  every cell bit for bit: u, v, state (LK's status), the mean difference and sigma, against the oracle with
+           its lane64 LK (the reference's fp32 lane arithmetic under the fp64 driver the GPU runs); the x87 lane tier
+           gives the same cells but for at most 1 in 1,000 where the two drivers round apart;
   decide   given the same vectors (the GPU's, fed to the oracle too): every cell's state, count and the motion summary
            bit for bit;
   update   the new mean bit for bit; and the still_hash over cells and mean equal.
@@ -25,16 +26,14 @@ def _level_plane(oracle, img, w, h, lv):
 
 
 def _compare_vectors(gc, oc, k):
-    assert np.array_equal(gc["diff"].view(np.uint32), oc["diff"].view(np.uint32)), f"frame {k}: diff"
-    same = gc["state"] == oc["state"]
-    assert same.mean() >= 0.99, f"frame {k}: status equal on {same.mean():.4f}"
-    good = same & (oc["state"] == 0)
-    du = np.abs(gc["u"] - oc["u"])[good]
-    dv = np.abs(gc["v"] - oc["v"])[good]
-    assert good.sum() > 0 and max(du.max(), dv.max()) <= 0.01, f"frame {k}: |d| {max(du.max(), dv.max())}"
-    rel = np.abs(gc["sigma"] - oc["sigma"])[good] / np.maximum(oc["sigma"][good], 1e-6)
-    assert rel.max() <= 1e-3, f"frame {k}: sigma {rel.max()}"
-    return int((~same).sum()), float(max(du.max(), dv.max()))
+    """Every cell equal bit for bit: the per-cell LK is the reference's lane arithmetic under an fp64 driver, and the
+    differences and sigmas follow the reference's x87 rounding exactly."""
+    for f in ("u", "v", "diff", "sigma"):
+        bad = np.nonzero(gc[f].view(np.uint32) != oc[f].view(np.uint32))[0]
+        assert len(bad) == 0, f"frame {k}: {f} differs on {len(bad)} cells, first {bad[:8]}"
+    bad = np.nonzero(gc["state"] != oc["state"])[0]
+    assert len(bad) == 0, f"frame {k}: state differs on {len(bad)} cells, first {bad[:8]}"
+    return 0, 0.0
 
 
 def run_phases(ctx, oracle, W, H, lv, cfg_t, nframes, seed, jump, storage=vtgpu.STORE_Q6, use_plane=False):
@@ -46,7 +45,7 @@ def run_phases(ctx, oracle, W, H, lv, cfg_t, nframes, seed, jump, storage=vtgpu.
     o = OracleStill(oracle, StillCfg(w, h, *cfg_t, 1))
     assert (g.cols, g.rows) == (o.s.cols, o.s.rows)
     pyr = ctx.pyramid(W, H, lv + 1, storage)
-    worst, flips, moved = 0.0, 0, 0
+    worst, flips, moved, lane_apart = 0.0, 0, 0, 0
     for k, img in enumerate(imgs):
         cur = _level_plane(oracle, img, W, H, lv)
         pyr.build(np.ascontiguousarray(img))
@@ -55,7 +54,15 @@ def run_phases(ctx, oracle, W, H, lv, cfg_t, nframes, seed, jump, storage=vtgpu.
                 g.vectors(plane=cur)
             else:
                 g.vectors(pyr, lv)
+            o.s.lk_tier = 0
             o.vectors(cur)
+            c87 = o.cells.copy()
+            o.s.lk_tier = 2
+            o.vectors(cur)
+            nd = int((c87.view(np.uint32).reshape(len(c87), -1) !=
+                      o.cells.view(np.uint32).reshape(len(c87), -1)).any(1).sum())
+            assert nd <= max(1, len(c87) // 1000), f"frame {k}: lane64 vs x87 lane tier differ on {nd} cells"
+            lane_apart += nd
             f, d = _compare_vectors(g.cells(), o.cells.copy(), k)
             flips += f
             worst = max(worst, d)
@@ -77,7 +84,7 @@ def run_phases(ctx, oracle, W, H, lv, cfg_t, nframes, seed, jump, storage=vtgpu.
         assert g.hash() == o.hash(), f"frame {k}: hash"
         moved += gm["moving"]
     say(f"still {w}x{h} lv{lv}: {g.ncells} cells, {nframes} frames, status flips {flips}, max |d| {worst:.2e} px, "
-        f"moving cells summed {moved}")
+        f"moving cells summed {moved}, cells where the x87 and fp64 drivers round apart {lane_apart}")
     assert moved > 0
     g.destroy()
     o.free()
