@@ -232,6 +232,16 @@ int vtg_ncc_search(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int x0, in
  * wait, no allocation once the context's scratch has grown to the map's size, so it can be captured in a graph). */
 int vtg_ncc_search_async(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
                          struct vtg_cell *cells, struct vtg_found *found_dev);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid it). vtg_ncc_search_async with every byte of working memory (the per-search
+ * record, the cell map when cells is NULL, the matrix-core cross terms) in a fixed device array of 9 MiB, one per
+ * device, looked up when a context is created and held until the process ends: enough for one exhaustive 64 x 64
+ * search of a 1920 x 1080 level. Allocates nothing, ever; a search that needs more is refused with VTG_ERANGE. The
+ * caller passes the pyramid, the template and the outputs. Every call on a device shares that device's array, which
+ * keeps the last call's contents: calls that overlap on one device (another context's stream, another host thread)
+ * race, so order them. Results equal vtg_ncc_search_async's bit for bit. */
+int vtg_ncc_search_fixed(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
+                         struct vtg_cell *cells, struct vtg_found *found_dev);
 /* n searches at once, one launch per stage for each group of 8 (OP_SEARCH over several slots or streams in one
  * round, track.c:301-329 and search_slot, track.c:1661-1726): search k covers corners region[4k .. 4k+3] =
  * x0, x1, y0, y1 of pyramid p[k] with template t[k]. All templates have one size, all searched levels one element

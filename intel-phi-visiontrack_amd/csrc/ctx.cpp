@@ -136,6 +136,11 @@ int vtg_ctx_create(int device, vtg_ctx **out)
         delete c;
         return VTG_EIO;
     }
+    if (fixed_scratch_init(c)) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return VTG_EIO;
+    }
     *out = c;
     return VTG_OK;
 }

@@ -27,6 +27,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "vtg_internal.hpp"
@@ -1390,9 +1391,10 @@ static int launch_batch(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, in
 // n searches into device records: each window clipped to where the template fits (as search_slot clips), maps in
 // cells[k] or the context's scratch, picks in found_dev[k] or, when found_dev is null, in the scratch (returned in
 // *found_scratch). Stream-ordered; no allocation once the scratch has grown to the batch's size.
+// fixed: working memory of fixed_bytes to use instead of the context's scratch (nothing is allocated then).
 static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, const vtg_tmpl *const *t,
                        const int *region, int step, vtg_cell *const *cells, vtg_found *found_dev,
-                       vtg_found **found_scratch)
+                       vtg_found **found_scratch, void *fixed = nullptr, size_t fixed_bytes = 0)
 {
     if (!c || !f)
         return VTG_EINVAL;
@@ -1439,13 +1441,20 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
             bytes += (size_t)ncell[k] * 256 * sizeof(int); // the cross terms of the padded map
     }
     void *sc = nullptr;
-    // A split search's finish context reads this scratch after the call returns (the caller joins it before the next
-    // search here, as vtgpu.h asks). The peer stays set, so growing the scratch later still waits for it.
-    if (f != c)
-        c->peer = f;
-    int rc = scratch(c, bytes, &sc);
-    if (rc)
-        return rc;
+    int rc;
+    if (fixed) {
+        if (bytes > fixed_bytes)
+            return fail(c, VTG_ERANGE, "vtg_ncc_search_fixed: the search needs %zu bytes of working memory, the fixed "
+                                       "array holds %zu", bytes, fixed_bytes);
+        sc = fixed;
+    } else {
+        // A split search's finish context reads this scratch after the call returns (the caller joins it before the
+        // next search here, as vtgpu.h asks). The peer stays set, so growing the scratch later still waits for it.
+        if (f != c)
+            c->peer = f;
+        if ((rc = scratch(c, bytes, &sc)))
+            return rc;
+    }
     char *aux0 = (char *)sc, *fnd0 = aux0 + (size_t)n * auxb;
     char *cel = fnd0 + (found_dev ? 0 : (size_t)n * 256);
     if (found_scratch)
@@ -1485,6 +1494,34 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
         if ((rc = launch_batch(c, f, p[0], level0, mode, t0->words_per_row, b)))
             return rc;
     }
+    return VTG_OK;
+}
+
+}  // namespace vtg
+
+// The working memory of vtg_ncc_search_fixed: a fixed array, one copy per device, held until the process ends. Host
+// code reaches it only through the address hipGetSymbolAddress gives for the current device (the name itself is a
+// host-side stand-in, not a device address). Enough for one exhaustive 64 x 64 search of a 1920 x 1080 level.
+static constexpr size_t kFixedBytes = (size_t)9 << 20;
+static __device__ __align__(256) unsigned char scratch[kFixedBytes];
+
+namespace vtg {
+
+static void *g_fixed[64]; // scratch's address on each device, once looked up
+static std::mutex g_fixed_mu;
+
+int fixed_scratch_init(vtg_ctx *c)
+{
+    if (c->device < 0 || c->device >= 64)
+        return fail(c, VTG_ERANGE, "device %d: the fixed search array is kept for devices 0..63", c->device);
+    std::lock_guard<std::mutex> lk(g_fixed_mu);
+    if (g_fixed[c->device])
+        return VTG_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(::scratch)); // the context's device is current
+    if (e != hipSuccess)
+        return hip_fail(c, e, "hipGetSymbolAddress(scratch)");
+    g_fixed[c->device] = ptr;
     return VTG_OK;
 }
 
@@ -1639,6 +1676,21 @@ int vtg_ncc_search_async(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0
     const int region[4] = {x0, x1, y0, y1};
     vtg_cell *cl[1] = {cells};
     return search_many(c, c, 1, &p, &t, region, step, cl, found_dev, nullptr);
+}
+
+int vtg_ncc_search_fixed(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,
+                         struct vtg_cell *cells, struct vtg_found *found_dev)
+{
+    if (!c)
+        return VTG_EINVAL;
+    if (!found_dev)
+        return fail(c, VTG_EINVAL, "vtg_ncc_search_fixed: null found record");
+    void *fixed = c->device >= 0 && c->device < 64 ? g_fixed[c->device] : nullptr;
+    if (!fixed)
+        return fail(c, VTG_EIO, "vtg_ncc_search_fixed: the fixed array of device %d was not looked up", c->device);
+    const int region[4] = {x0, x1, y0, y1};
+    vtg_cell *cl[1] = {cells};
+    return search_many(c, c, 1, &p, &t, region, step, cl, found_dev, nullptr, fixed, kFixedBytes);
 }
 
 int vtg_ncc_search(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,

@@ -149,6 +149,8 @@ uint64_t lk_class(const vtg_pyr *I, int top, const lk_cfg *cfg);
 int tuned_pyr_rows(const vtg_ctx *c, const vtg_pyr *p);
 LkSched tuned_lk(const vtg_ctx *c, const vtg_pyr *I, int top, const lk_cfg *cfg);
 int launch_ncc_template(vtg_ctx *c, const vtg_pyr *p, int level, int x0, int y0, vtg_tmpl *t);
+// Look up the fixed search array's address on the context's device (current), once per device (ncc.hip).
+int fixed_scratch_init(vtg_ctx *c);
 int launch_ncc_score(vtg_ctx *c, const vtg_pyr *p, const vtg_tmpl *t, int n, const int *xy, int half, double *out);
 
 }  // namespace vtg
