@@ -223,7 +223,8 @@ int vtg_ncc_template_update_many(vtg_ctx *ctx, int n, vtg_tmpl *const *t, const 
                                  const int *x0, const int *y0);
 /* The template's norm (sum of squares of its zero-mean values) and fp32 pixels (zero-mean, row-major) to host. */
 int vtg_ncc_template_read(vtg_ctx *ctx, const vtg_tmpl *t, float *app_host, float *norm);
-/* Search corners [x0, x1] x [y0, y1] at `step` (clipped to where the template fits, as search_slot clips), on the
+/* Search corners [x0, x1] x [y0, y1] at `step` 1 or 2 (the tracker's steps, track.c:1796, 1813, 1836; others are
+ * VTG_EINVAL), clipped to where the template fits as search_slot clips, on the
  * template's level of pyramid p. cells: device buffer of at least ((x1-x0)/16+1) * ((y1-y0)/16+1) records, or NULL.
  * found: host record, written on return (this call waits for the search). */
 int vtg_ncc_search(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int x0, int x1, int y0, int y1, int step,

@@ -4,6 +4,8 @@
     ncc_ref.json     tk_correlate (src/kern/track.c:208) of oracle/_ref/libvtref.so on seeded 32 x 21 inputs
     lk_ref.json      lk_track_vx / lk_flowback_vx of oracle/_ref/libvtref.so on the 640 x 480 blob workload, and
                      lk_track_vx on the 1920 x 1080 one (two seeds)
+    ncc_refine.json  store_appearance, search_half, peak_offset and search_slot (src/kern/track.c:935-974,
+                     1615-1726) of oracle/_ref/libvtsearch.so on the planes of tests/ncc_cases.py
 The reference itself never leaves this container: only these numbers are committed."""
 import ctypes as C
 import json
@@ -123,10 +125,76 @@ def lk_ref():
     return res
 
 
+SEARCH_SO = os.path.join(REPO, "oracle", "_ref", "libvtsearch.so")
+
+
+class RefFound(C.Structure):
+    _fields_ = [("hx", C.c_int), ("hy", C.c_int), ("ncc", C.c_longdouble), ("second", C.c_longdouble)]
+
+
+def _ld_hex(v):
+    """A long double returned through ctypes (a Python float: the value rounded to double) as a hex string."""
+    return float(v).hex()
+
+
+def ncc_refine():
+    import hashlib
+    import ncc_cases as nc
+    ref = C.CDLL(SEARCH_SO)
+    P = C.POINTER(Plane)
+    ref.ref_store_appearance.argtypes = [P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_void_p,
+                                         C.POINTER(C.c_float)]
+    ref.ref_search_half.restype = C.c_longdouble
+    ref.ref_search_half.argtypes = [P, C.c_void_p, C.c_float, C.c_int, C.c_int]
+    ref.ref_peak_offset.restype = C.c_longdouble
+    ref.ref_peak_offset.argtypes = [C.c_longdouble] * 3
+    ref.ref_search_slot.argtypes = [P, C.c_int, C.c_void_p, C.c_float] + [C.c_int] * 5 + \
+        [C.POINTER(RefFound), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    o = Oracle()
+    rng = np.random.default_rng(20261204)
+    out = {"tw": nc.TW, "th": nc.TH, "region": list(nc.REGION), "cases": []}
+    for name, img, (cx, cy) in nc.cases(o):
+        h, w = img.shape
+        hp = nc.host_plane(o, img)
+        pix = np.zeros(nc.TW * nc.TH, np.float32)
+        norm = C.c_float()
+        assert ref.ref_store_appearance(C.byref(hp.c), cx, cy, *nc.REGION, pix.ctypes.data, C.byref(norm)) == 0
+        case = {"name": name, "w": w, "h": h, "centre": [cx, cy],
+                "template_sha256": hashlib.sha256(pix.tobytes()).hexdigest(),
+                "norm": float(norm.value).hex()}
+        x0, y0 = int(cx) - nc.TW // 2, int(cy) - nc.TH // 2
+        corners = [(2 * x0 + dx, 2 * y0 + dy) for dy in range(-3, 4) for dx in range(-3, 4)]
+        corners += [(int(rng.integers(-4, 2 * w)), int(rng.integers(-4, 2 * h))) for _ in range(60)]
+        corners += [(1, 1), (2, 2), (2 * (w - nc.TW - 3), 2), (2 * (w - nc.TW - 3) + 1, 3),
+                    (2 * (w - nc.TW - 2), 4), (4, 2 * (h - nc.TH - 3) + 1), (4, 2 * (h - nc.TH - 2))]
+        case["half"] = [[hx, hy, _ld_hex(ref.ref_search_half(C.byref(hp.c), pix.ctypes.data, norm, hx, hy))]
+                        for hx, hy in corners]
+        cap = (w // 16 + 2) * (h // 16 + 2)
+        sv, sx, sy = np.zeros(cap, np.float32), np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        case["searches"] = []
+        for (a, b, c, d, step) in nc.windows(w, h, (x0, y0)):
+            f, n = RefFound(), C.c_int()
+            assert ref.ref_search_slot(C.byref(hp.c), 0, pix.ctypes.data, norm, a, b, c, d, step, C.byref(f),
+                                       sv.ctypes.data, sx.ctypes.data, sy.ctypes.data, cap, C.byref(n)) == 0
+            m = n.value
+            cells = np.concatenate([sv[:m].view(np.uint32), sx[:m].view(np.uint32), sy[:m].view(np.uint32)])
+            case["searches"].append({"window": [a, b, c, d, step], "hx": f.hx, "hy": f.hy, "ncc": _ld_hex(f.ncc),
+                                     "second": _ld_hex(f.second), "cells": m,
+                                     "cells_sha256": hashlib.sha256(cells.tobytes()).hexdigest()})
+        out["cases"].append(case)
+    triples = [(float(rng.normal()), float(rng.normal()), float(rng.normal())) for _ in range(150)]
+    triples += [(0.2, 1.0, 0.6), (0.5, 1.0, 0.5), (1.0, 1.0, 1.0), (0.0, 1.0, 0.99), (0.99, 1.0, 0.0),
+                (0.3, 0.5, 0.9), (0.9, 0.5, 0.3), (-1.0, 0.0, 1.0), (0.75, 1.0, 0.25), (0.25, 1.0, 0.75)]
+    out["peak_offset"] = [[l.hex(), c.hex(), r.hex(), _ld_hex(ref.ref_peak_offset(l, c, r))] for l, c, r in triples]
+    return out
+
+
 if __name__ == "__main__":
     json.dump(m1(), open(os.path.join(HERE, "m1_hashes.json"), "w"), indent=1)
     json.dump(m2(), open(os.path.join(HERE, "m2_lines.json"), "w"), indent=1)
     if os.path.exists(REF_SO):
         json.dump(ncc_ref(), open(os.path.join(HERE, "ncc_ref.json"), "w"))
         json.dump(lk_ref(), open(os.path.join(HERE, "lk_ref.json"), "w"), indent=1)
+    if os.path.exists(SEARCH_SO):
+        json.dump(ncc_refine(), open(os.path.join(HERE, "ncc_refine.json"), "w"))
     print("fixtures written")

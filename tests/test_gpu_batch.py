@@ -156,6 +156,47 @@ def test_uneven_batch(ctx, oracle):
     b.destroy()
 
 
+def test_uneven_batch_full_shape(ctx, oracle):
+    """The uneven batch at its full shape: 64 pairs of 640x480 .. 3840x2160 with 500-20,000 points each, +-24 px, 3-5
+    levels, all pyramids built together and all points in one LK launch. Every pair's records equal its own
+    single-pair call, and one pair of every size class matches the oracle: its pyramids bit for bit and every LK record
+    equal to the lane tier's."""
+    import numpy as np
+    from oracle_ffi import lk_cfg
+    from test_gpu_lk import _assert_same
+    from workloads import Batch, batch_spec
+    spec = batch_spec(12, npairs=64, nmin=500, nmax=20000)
+    assert {w for (w, _, _, _, _) in spec} == {640, 1280, 1920, 3840}
+    b = Batch(vtgpu, ctx, spec)
+    cfg = vtgpu.lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0)
+    say("uneven batch full shape", len(spec), "pairs", b.npoints, "points")
+    b.enqueue(cfg)
+    ctx.sync()
+    got = b.out.get(np.zeros(b.npoints, vtgpu.LK_OUT_DTYPE))
+    for k, (w, h, L, n, _) in enumerate(spec):
+        xy = b.xy_host[2 * b.offsets[k]:2 * b.offsets[k + 1]]
+        one = ctx.lk_track_host(b.pyrs[k][0], b.pyrs[k][1], L - 1, cfg, xy[0::2], xy[1::2])
+        assert one.tobytes() == got[b.offsets[k]:b.offsets[k + 1]].tobytes(), k
+    firsts = {}
+    for k, (w, h, L, n, _) in enumerate(spec):
+        firsts.setdefault(w, k)
+    assert len(firsts) == 4
+    for w, k in sorted(firsts.items()):
+        w, h, L, n, _ = spec[k]
+        say("uneven batch full shape: oracle pair", k, w, h, L, n)
+        f0, f1 = b.host[k]
+        k0, I = oracle.pyramid(f0, w, h, L, threads=16)
+        k1, J = oracle.pyramid(f1, w, h, L, threads=16)
+        for lv in range(L):
+            assert np.array_equal(b.pyrs[k][0].read_level(lv).view(), k0[lv].view()), (k, lv)
+            assert np.array_equal(b.pyrs[k][1].read_level(lv).view(), k1[lv].view()), (k, lv)
+        xy = b.xy_host[2 * b.offsets[k]:2 * b.offsets[k + 1]]
+        ref = lane_ref(oracle, I, J, L - 1, lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0), xy[0::2], xy[1::2], threads=16,
+                       label=f"full batch pair {k}")
+        _assert_same(got[b.offsets[k]:b.offsets[k + 1]], ref, f"full batch pair {k} ({w}x{h}, {n} points)")
+    b.destroy()
+
+
 def test_download_async_into_pinned(ctx):
     """vtg_download_async into vtg_host_alloc memory, ordered after work on another context, gives the bytes a waiting
     download gives."""

@@ -53,6 +53,48 @@ def test_scores_whole_and_half(ctx, oracle, level, tw, th):
     t.destroy()
 
 
+def test_refine_reference_fixture(ctx, oracle):
+    """The GPU against the reference's own store_appearance, search_half and search_slot (tests/golden/ncc_refine.json,
+    written from src/kern/track.c compiled from source): the template bit for bit, half-pixel scores within 1e-5 (-1
+    exactly where the window leaves the plane), and every search's pick equal, its score and second peak within 1e-5,
+    over the fixture's four planes and the windows at steps 1 and 2 (whole, inner, clipped, empty, one coarse corner
+    a pixel off); the steps above 2, which the tracker never uses, are refused with VTG_EINVAL."""
+    import hashlib
+    import json
+    import os
+    import ncc_cases as nc
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ncc_refine.json")))
+    byname = {c["name"]: c for c in g["cases"]}
+    for name, img, (cx, cy) in nc.cases(oracle):
+        case = byname[name]
+        h, w = img.shape
+        x0, y0 = int(cx) - nc.TW // 2, int(cy) - nc.TH // 2
+        say("ncc refine fixture", name, w, h)
+        p = ctx.pyramid(w, h, 1)
+        p.build(np.ascontiguousarray(img))
+        t = ctx.ncc_template(p, 0, x0, y0, nc.TW, nc.TH)
+        app, norm = t.read()
+        assert hashlib.sha256(app.tobytes()).hexdigest() == case["template_sha256"], name
+        assert float(norm).hex() == case["norm"], name
+        hxy = np.array([[a, b] for a, b, _ in case["half"]], np.int32)
+        want = np.array([float.fromhex(v) for _, _, v in case["half"]])
+        got = ctx.ncc_score(p, t, hxy, half=True)
+        assert np.array_equal(got == -1, want == -1), name
+        assert np.abs(got - want).max() <= 1e-5, (name, np.abs(got - want).max())
+        for srch in case["searches"]:
+            a, b, c, d, step = srch["window"]
+            if step > 2:   # the tracker searches at steps 1 and 2 only (track.c:1796, 1813, 1836); others refused
+                with pytest.raises(vtgpu.VtgError, match=r"\(-22\)"):
+                    ctx.ncc_search(p, t, a, b, c, d, step)
+                continue
+            f = ctx.ncc_search(p, t, a, b, c, d, step)
+            assert (f.hx, f.hy, f.cells) == (srch["hx"], srch["hy"], srch["cells"]), (name, srch["window"])
+            assert abs(f.ncc - float.fromhex(srch["ncc"])) <= 1e-5, (name, srch["window"])
+            assert abs(f.second - float.fromhex(srch["second"])) <= 1e-5, (name, srch["window"])
+        t.destroy()
+        p.destroy()
+
+
 def _textured_corner(img, tw, th, seed):
     """A corner whose tw x th window has strong texture (the searches are only interesting there)."""
     rng = np.random.default_rng(seed)

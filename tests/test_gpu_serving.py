@@ -124,3 +124,55 @@ def test_budget_too_small_for_any_frame():
     assert len(outs[0][0]) == 0 and len(outs[2][0]) == len(sf.xy) // 2
     srv.close(sid)
     srv.destroy()
+
+
+def test_config5_full_shape(ctx, oracle):
+    """The serving workload at its full shape over a bounded run: 8 streams (3 x 1920x1080 with 16,200 points, 3 x
+    1280x720 with 8,160, 2 x 640x480 with 4,080; 21 x 21 LK; a random walk of 1.5 px per frame), 10 frames each, an
+    NCC search on 2 of the 80 frames (2.5%), a 2 GiB budget, continuous batching (lanes 2, batches up to 16). Every
+    stream's records served in the mix equal the same stream served alone, bit for bit; one frame of every resolution
+    class matches the oracle (every LK record equal to the lane tier's), and both searches match the oracle's (same
+    pick, score and second peak within 1e-5)."""
+    from gpu_util import lane_ref
+    from oracle_ffi import lk_cfg
+    from test_gpu_lk import _assert_same
+    from workloads import STREAMS
+    cfg = vtgpu.lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0)
+    nf = 10
+    frames = [StreamFrames(vtgpu, w, h, 200 + i, k=6, nx=nx, ny=ny) for i, (w, h, _, _, nx, ny) in enumerate(STREAMS)]
+    levels = [L for (_, _, _, L, _, _) in STREAMS]
+    ncc_at = {(0, 5), (6, 7)}
+    say("config 5: mix of 8 streams")
+    srv = vtgpu.Server(2 << 30, lanes=2, max_batch=16, cfg=cfg)
+    mix = _serve(srv, levels, frames, nf, ncc_at=ncc_at, burst=True)
+    st = srv.stats()
+    srv.destroy()
+    assert st["completed"] == 8 * nf and st["max_batch"] > 1 and st["hbm_peak"] <= 2 << 30, st
+    for s in range(8):
+        say("config 5: stream alone", s)
+        srv = vtgpu.Server(2 << 30, lanes=2, max_batch=16, cfg=cfg)
+        alone = _serve(srv, [levels[s]], [frames[s]], nf, ncc_at={(0, i) for (t, i) in ncc_at if t == s},
+                       burst=False)[0]
+        srv.destroy()
+        for i in range(nf):
+            assert alone[i][0].tobytes() == mix[s][i][0].tobytes(), (s, i)
+            assert bytes(alone[i][1]) == bytes(mix[s][i][1]), (s, i)
+    cfgo = lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0)
+    for s, i in ((0, 4), (3, 6), (6, 3)):
+        sf, L = frames[s], levels[s]
+        say("config 5: oracle frame", s, i)
+        k0, I = oracle.pyramid(sf[i - 1], sf.w, sf.h, L, threads=16)
+        k1, J = oracle.pyramid(sf[i], sf.w, sf.h, L, threads=16)
+        ref = lane_ref(oracle, I, J, L - 1, cfgo, sf.xy[0::2], sf.xy[1::2], threads=16, label=f"stream {s} frame {i}")
+        _assert_same(mix[s][i][0], ref, f"config 5 stream {s} frame {i}")
+    for s, i in sorted(ncc_at):
+        sf, L = frames[s], levels[s]
+        say("config 5: oracle search", s, i)
+        k0, I = oracle.pyramid(sf[i - 1], sf.w, sf.h, 1, threads=16)
+        k1, J = oracle.pyramid(sf[i], sf.w, sf.h, 1, threads=16)
+        app, norm = oracle.ncc_template(k0[0], sf.w // 2 - 32, sf.h // 2 - 32, 64, 64)
+        of, _ = oracle.ncc_search(k1[0], app, 64, 64, norm, 1, sf.w, 1, sf.h, 1, threads=16)
+        f = mix[s][i][1]
+        assert (f.hx, f.hy) == (of.hx, of.hy), (s, i)
+        assert abs(f.ncc - float(of.ncc)) <= 1e-5 and abs(f.second - float(of.second)) <= 1e-5, (s, i)
+        assert f.ncc > 0.8   # the thing was found (noise and a sub-pixel walk keep it under 1)

@@ -227,6 +227,99 @@ def test_ncc_reference_vectors(oracle):
         assert got == case["ncc"]
 
 
+def test_ncc_refine_reference(oracle):
+    """The template, the half-pixel scores, the parabola and the whole search pinned to the reference's own functions
+    (store_appearance, search_half, peak_offset and search_slot of src/kern/track.c, compiled from source by
+    oracle/ref_track.c; fixture tests/golden/ncc_refine.json): vo_ncc_template's bits and norm, vo_ncc_half at 74
+    corners a case (the 7 x 7 half-pixel neighbourhood of the template's own corner, random corners and both edges),
+    vo_peak_offset on 160 triples, and vo_ncc_search's pick, score, second peak and whole cell map over 10 windows (whole
+    plane at steps 1-3, inner, clipped at both corners, empty, and three whose one coarse corner is step - 1 pixels
+    from the thing at steps 2-4) on four planes: textured, a planted copy, a dimmed noisy
+    copy, and a repeating pattern."""
+    import hashlib
+    import ncc_cases as nc
+    g = json.load(open(os.path.join(GOLD, "ncc_refine.json")))
+    byname = {c["name"]: c for c in g["cases"]}
+    for name, img, (cx, cy) in nc.cases(oracle):
+        case = byname[name]
+        h, w = img.shape
+        hp = nc.host_plane(oracle, img)
+        app, norm = oracle.ncc_template(hp, int(cx) - nc.TW // 2, int(cy) - nc.TH // 2, nc.TW, nc.TH)
+        assert hashlib.sha256(app.tobytes()).hexdigest() == case["template_sha256"], name
+        assert float(norm).hex() == case["norm"], name
+        for hx, hy, want in case["half"]:
+            got = oracle.lib.vo_ncc_half(C.byref(hp.c), app.ctypes.data, nc.TW, nc.TH, norm, hx, hy)
+            assert float(got).hex() == want, (name, hx, hy)
+        x0, y0 = int(cx) - nc.TW // 2, int(cy) - nc.TH // 2
+        assert [tuple(s["window"]) for s in case["searches"]] == nc.windows(w, h, (x0, y0))
+        for srch in case["searches"]:
+            a, b, c, d, step = srch["window"]
+            f, cells = oracle.ncc_search(hp, app, nc.TW, nc.TH, norm, a, b, c, d, step)
+            assert (f.hx, f.hy, float(f.ncc).hex(), float(f.second).hex(), f.cells) == \
+                (srch["hx"], srch["hy"], srch["ncc"], srch["second"], srch["cells"]), (name, srch["window"])
+            bits = np.concatenate([np.ascontiguousarray(cells[k]).view(np.uint32) for k in ("v", "x", "y")])
+            assert hashlib.sha256(bits.tobytes()).hexdigest() == srch["cells_sha256"], (name, srch["window"])
+    for l, c, r, want in g["peak_offset"]:
+        got = oracle.lib.vo_peak_offset(float.fromhex(l), float.fromhex(c), float.fromhex(r))
+        assert float(got).hex() == want, (l, c, r)
+
+
+def test_ncc_refine_closed_forms(oracle):
+    """What the mathematics fixes about the refinement, independent of the reference:
+    - the parabola through (-1, f(-1)), (0, f(0)), (1, f(1)) of a downward quadratic peaks at its vertex v, clamped to
+      +-0.5; a flat or upward one gives 0;
+    - a template cut from an image moved right by half a pixel scores 1 at the odd half-pixel corner (the 2-tap
+      average restores the original exactly), and one cut moved down by half a pixel at the odd row;
+    - a second, clear copy of the thing (more than half a template away) is the second peak, scoring what the plain
+      score gives at that copy; a copy nearer than that is not counted;
+    - corners whose half-pixel window leaves the plane score -1."""
+    import ncc_cases as nc
+    rng = np.random.default_rng(11)
+    po = oracle.lib.vo_peak_offset
+    for _ in range(200):
+        a, v, c0 = -float(rng.uniform(0.1, 5)), float(rng.uniform(-0.9, 0.9)), float(rng.normal())
+        f = [a * (x - v) ** 2 + c0 for x in (-1, 0, 1)]
+        want = min(0.5, max(-0.5, v))
+        assert abs(po(*f) - want) < 1e-12, (a, v)
+        assert po(*[-y for y in f]) == 0.0 and po(c0, c0, c0) == 0.0
+    # half-pixel copies
+    img, sh = nc.half_shifted(200, 120, 3)
+    hp, hs = nc.host_plane(oracle, img), nc.host_plane(oracle, sh)
+    x0, y0 = 70, 40
+    app, norm = oracle.ncc_template(hs, x0, y0, nc.TW, nc.TH)
+    v = oracle.lib.vo_ncc_half(C.byref(hp.c), app.ctypes.data, nc.TW, nc.TH, norm, 2 * x0 + 1, 2 * y0)
+    assert v > 1 - 1e-6
+    assert oracle.lib.vo_ncc_half(C.byref(hp.c), app.ctypes.data, nc.TW, nc.TH, norm, 2 * x0, 2 * y0) < v
+    f, _ = oracle.ncc_search(hp, app, nc.TW, nc.TH, norm, 1, 200, 1, 120, 1)
+    assert (f.hx, f.hy) == (2 * x0 + 1, 2 * y0) and f.ncc > 1 - 1e-6
+    imgT, shT = img.T.copy(), sh.T.copy()     # the same, down by half a pixel
+    hpT, hsT = nc.host_plane(oracle, imgT), nc.host_plane(oracle, shT)
+    app, norm = oracle.ncc_template(hsT, y0, x0, nc.TW, nc.TH)
+    v = oracle.lib.vo_ncc_half(C.byref(hpT.c), app.ctypes.data, nc.TW, nc.TH, norm, 2 * y0, 2 * x0 + 1)
+    assert v > 1 - 1e-6
+    # a planted clear copy is the second peak; a near one is not
+    far = nc.planted(288, 176, 21, (60, 50), (180, 100), size=(32, 21), gain=0.9, noise=4.0)
+    hp = nc.host_plane(oracle, far)
+    app, norm = oracle.ncc_template(hp, 60, 50, nc.TW, nc.TH)
+    f, _ = oracle.ncc_search(hp, app, nc.TW, nc.TH, norm, 1, 288, 1, 176, 1)
+    at_copy = max(oracle.lib.vo_ncc_half(C.byref(hp.c), app.ctypes.data, nc.TW, nc.TH, norm, 360 + dx, 200 + dy)
+                  for dx in (-1, 0, 1) for dy in (-1, 0, 1))
+    assert (f.hx, f.hy) == (120, 100) and f.ncc > 1 - 1e-6
+    assert float(f.second) == float(at_copy) and 0.9 < f.second < 1 - 1e-4
+    near = nc.planted(288, 176, 21, (60, 50), (60 + 12, 50), size=(32, 21))
+    hp = nc.host_plane(oracle, near)
+    app, norm = oracle.ncc_template(hp, 60, 50, nc.TW, nc.TH)
+    f, _ = oracle.ncc_search(hp, app, nc.TW, nc.TH, norm, 1, 288, 1, 176, 1)
+    assert f.second < 0.9, float(f.second)
+    # a window whose only coarse corner is step - 1 pixels off in both axes still finds the exact copy
+    for s in (2, 3, 4):
+        f, _ = oracle.ncc_search(hp, app, nc.TW, nc.TH, norm, 60 - s + 1, 60, 50 - s + 1, 50, s)
+        assert (f.hx, f.hy) == (120, 100) and f.ncc > 1 - 1e-6, s
+    # off the plane
+    for hx, hy in [(0, 10), (10, 0), (2 * (288 - nc.TW - 2), 10), (10, 2 * (176 - nc.TH - 2))]:
+        assert oracle.lib.vo_ncc_half(C.byref(hp.c), app.ctypes.data, nc.TW, nc.TH, norm, hx, hy) == -1.0
+
+
 def _textbook_zncc(win, tmpl):
     a = win.astype(np.float64) - win.mean()
     t = tmpl.astype(np.float64) - tmpl.mean()
