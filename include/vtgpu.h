@@ -164,6 +164,10 @@ int vtg_ctx_device_error(vtg_ctx *ctx, int *code);
 /* The LK schedule where no autotuner pick applies: wavefronts per workgroup, 1 .. 8 (default 1). A wavefront tracks 4
  * points, one per row of 16 lanes. Results are the same bit for bit under every schedule; only time differs. */
 int vtg_lk_set_waves(vtg_ctx *ctx, int waves);
+/* The LK points per wavefront where no autotuner pick applies: 4 (one point per row of 16 lanes), 3 (21 x 21
+ * windows: three points' vector lanes in three rows, the fourth row computing their second block's 5 columns), or 0
+ * (default: 3 where the window allows it, else 4). Results are the same bit for bit; only time differs. */
+int vtg_lk_set_ppw(vtg_ctx *ctx, int ppw);
 
 /* ------------------------------------------------------------------ autotuner (no counterpart in the reference, whose
  * schedule is fixed at build time: one thread per row range or point, track.c:210-292) */
@@ -172,10 +176,11 @@ int vtg_lk_set_waves(vtg_ctx *ctx, int waves);
  * window). */
 struct vtg_tune {
     int pyr_rows;                         /* pyramid reduce tile height: 8, 16 or 32 output rows */
-    int lk_waves;                         /* LK schedule: wavefronts (4 points each) per workgroup */
+    int lk_waves;                         /* LK schedule: wavefronts per workgroup */
     float pyr_us, lk_us;                  /* median times of the picks: both pyramids; the given points */
     float pyr_us_default, lk_us_default;  /* the same under the defaults (16 rows; the context's LK waves) */
     int candidates;                       /* schedules timed */
+    int lk_ppw;                           /* LK schedule: points per wavefront (3 or 4) */
 };
 
 /* Time every candidate schedule of the pyramid reduce and of the LK kernel on this pair (device frames f0, f1 of

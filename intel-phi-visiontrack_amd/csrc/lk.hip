@@ -910,6 +910,513 @@ __global__ __launch_bounds__(512) void lk_kernel(LkArgs a)
         store(a.out + pt, o);
 }
 
+// ---------------------------------------------------------------- three points per wavefront (windows 17 .. 21 wide)
+//
+// A window wider than 16 columns and at most 21 (the 1080p workload's 21 x 21) leaves 16 - (w - 16) lanes of a
+// point's second block idle in the layout above (11 of 16 at 21 x 21). Here a wavefront tracks 3 points and every lane
+// works: lanes 16p .. 16p + 15 (p = 0, 1, 2) are point p's reference vector lanes, lane j owning window column j and
+// keeping that column's template in registers; lanes 48 .. 48 + 3 TAIL - 1 take the second block's TAIL = w - 16
+// columns of the three points (lane 48 + TAIL p + t: point p, column 16 + t; the zero lane follows them, and any lanes
+// after that repeat point 2's last column and feed no one). A second-block column's gradients (at each build) and its
+// T - J (at each sample) reach vector lane t of its point by ds_bpermute; that lane keeps the gradients in registers
+// and adds the products to its sums right after its own column of the same row, where the reference's lane t adds its
+// second block (lk_vx.c:82-94, 119-126, 160-168). So every sum, and every record, is the same bits as the reference's
+// and as the layout above. A point's totals reach its second-block lanes the same way, so all lanes of a point run the
+// same level walk on the same values.
+// Per point and iteration both layouts issue about the same VALU work (the second block's idle lanes there, the
+// hand-off and the extra chain products here); this one builds templates with about a fifth fewer instructions.
+template <int HMAX, int WX> struct Shape3 {
+    static_assert(WX > 16 && WX <= 21, "three points per wavefront: windows 17 .. 21 wide");
+    static constexpr int TAIL = WX - 16;
+    static constexpr int M = 2; // the staged neighbourhood's margin, as in Shape
+    static constexpr int RW = WX + 1 + 2 * M, RH = HMAX + 1 + 2 * M;
+    static constexpr int STAGE = (RW * RH + 3) / 4 * 4; // floats per point
+    static constexpr int WAVE_BYTES = 3 * STAGE * 4;    // LDS per wavefront
+};
+constexpr int PPW3 = 3;
+
+struct Lane3 {
+    int p;     // the wavefront's point this lane works for
+    int c;     // its window column
+    int j;     // lanes 0 .. 47: the reference vector lane; lanes 48 .. 63: the column's place in the second block
+    bool vec;  // a vector lane (lanes 0 .. 47)
+    bool tail; // a vector lane whose reference lane also owns second-block column 16 + j
+    bool zero; // the zero lane (lane 48 + 3 TAIL): its level reads fall outside the level and return 0
+    int src;   // ds_bpermute address of the lane computing this point's column 16 + j; for a vector lane without a
+               // second-block column, the zero lane, whose gradients and (read from the level) T - J are 0
+    int head;  // ds_bpermute address of a vector lane of this point (the lanes holding its totals)
+};
+
+template <int WX>
+__device__ __forceinline__ Lane3 lane3(int lane)
+{
+    constexpr int TAIL = WX - 16;
+    static_assert(3 * TAIL < 16, "a spare lane for the zero lane");
+    const int row = lane >> 4, j = lane & 15;
+    Lane3 l;
+    l.vec = row < 3;
+    l.zero = lane == 48 + 3 * TAIL;
+    if (l.vec) {
+        l.p = row;
+        l.j = j;
+        l.c = j;
+        l.tail = j < TAIL;
+        l.src = 4 * (j < TAIL ? 48 + TAIL * row + j : 48 + 3 * TAIL);
+    } else {
+        l.p = min(j / TAIL, 2);
+        l.j = min(j - TAIL * l.p, TAIL - 1);
+        l.c = 16 + l.j;
+        l.tail = false;
+        l.src = 4 * lane;
+    }
+    l.head = 4 * 16 * l.p;
+    return l;
+}
+
+// LDS reads by byte address: an address carried in a register (the loop's ordering goes through it) plus a constant
+// offset, which the reads take as their immediate offset.
+typedef __attribute__((address_space(3))) float lds_float;
+__device__ __forceinline__ unsigned lds_addr(const void *p) { return (unsigned)(uintptr_t)(const lds_float *)p; }
+__device__ __forceinline__ float lds_f(unsigned a) { return *(const lds_float *)(uintptr_t)a; }
+
+// v of the lane at byte address a (ds_bpermute: every lane read here is active when its reader is, as both work for
+// the same point).
+__device__ __forceinline__ float bperm(int a, float v)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(a, __float_as_int(v)));
+}
+
+// A level's template in one lane: its own column's T, Tx, Ty and (vector lanes) the gradients of its second-block column,
+// zero for a lane without one (zero gradients add nothing to a sum: the reference's zeroed lanes, lk_vx.c:82-85).
+template <int HMAX> struct Tmpl3 {
+    float t[HMAX], tx[HMAX], ty[HMAX], ux[HMAX], uy[HMAX];
+};
+
+// build_vx (lk_vx.c:36-97) in this layout: the same per-column arithmetic as build, one column per lane.
+template <int HMAX, int WX, typename T>
+__device__ __forceinline__ void build3(const Level &lv, double px, double py, const Lane3 &ln, Tmpl3<HMAX> &tp,
+                                       float *G)
+{
+    constexpr float ks = Ld<T>::scale, kg = ks * (1.0f / 512.0f);
+    constexpr int K = 4; // image rows whose loads are in flight ahead of their use
+    const double flx = floor(px), fly = floor(py);
+    const float fx = (float)(px - flx), fy = (float)(py - fly);
+    unsigned pitch;
+    const __amdgpu_buffer_rsrc_t rs = level_rsrc<T>(lv, pitch);
+    // image row 0 is the level's row fly - h/2 - 1; tap 0 of window column c is level column flx - w/2 - 1 + c
+    unsigned o = (unsigned)((int)fly - HMAX / 2 - 1) * pitch + (unsigned)((int)flx - WX / 2 - 1 + ln.c) * sizeof(T);
+    o = ln.zero ? 0x80000000u : o; // past every level: the zero lane reads 0
+    typename Row<T>::Q raw[HMAX + 3];
+#pragma unroll
+    for (int i = 0; i < K; i++)
+        raw[i] = Row<T>::lq(rs, o, (unsigned)i * pitch);
+    float X[3], S0[3], S1[3], d0 = 0.0f, d1 = 0.0f;
+    float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+#pragma unroll
+    for (int i = 0; i <= HMAX + 2; i++) {
+        const int r = i - 1; // the S row this image row completes
+        float a[4], S[3];
+        Row<T>::cq(raw[i], a);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const float here = fmaf(a[q + 1] - a[q], fx, a[q]);
+            S[q] = i == 0 ? 0.0f : fmaf(here - X[q], fy, X[q]);
+            X[q] = here;
+        }
+        if (i > 0) {
+            const float dS = S[2] - S[0];
+            if (r == 0) {
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+                    S0[q] = S[q];
+                d0 = dS;
+            } else if (r == 1) {
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+                    S1[q] = S[q];
+                d1 = dS;
+            } else {
+                // window row rr = r - 2 from S rows rr (S0), rr + 1 (S1), rr + 2 (S)
+                const int rr = r - 2;
+                const float tx = fmaf(d1, 2.0f, d0 + dS) * kg;
+                const float vl = S[0] - S0[0], vc = S[1] - S0[1], vr = S[2] - S0[2];
+                const float ty = fmaf(vc, 2.0f, vl + vr) * kg;
+                tp.t[rr] = S1[1] * ks;
+                tp.tx[rr] = tx;
+                tp.ty[rr] = ty;
+                const float ux = bperm(ln.src, tx), uy = bperm(ln.src, ty); // 0 from the zero lane
+                tp.ux[rr] = ux;
+                tp.uy[rr] = uy;
+                sxx = fmaf(tx, tx, sxx);
+                sxy = fmaf(tx, ty, sxy);
+                syy = fmaf(ty, ty, syy);
+                sxx = fmaf(ux, ux, sxx);
+                sxy = fmaf(ux, uy, sxy);
+                syy = fmaf(uy, uy, syy);
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    S0[q] = S1[q];
+                    S1[q] = S[q];
+                }
+                d0 = d1;
+                d1 = dS;
+            }
+        }
+        // the next row's loads: K rows ahead, issued once this row is done
+        if (i + K <= HMAX + 2) {
+            o = after(o, X[0]);
+            raw[i + K] = Row<T>::lq(rs, o, (unsigned)(i + K) * pitch);
+        }
+    }
+    G[0] = bperm(ln.head, row_sum(sxx));
+    G[1] = bperm(ln.head, row_sum(sxy));
+    G[2] = bperm(ln.head, row_sum(syy));
+}
+
+// iterate_vx (lk_vx.c:100-140; RES = false) and residual_vx (lk_vx.c:142-175; RES = true) in this layout, J read from
+// the level: two taps of each of window rows 0 .. h at floor(q), x-interpolated, the row above carried; It = T - J.
+template <int HMAX, int WX, typename T, bool RES>
+__device__ __forceinline__ void sample3(__amdgpu_buffer_rsrc_t rs, unsigned pitch, double qx, double qy,
+                                        const Lane3 &ln, const Tmpl3<HMAX> &tp, float *b)
+{
+    constexpr float ks = Ld<T>::scale;
+    constexpr int K = 8; // window rows whose loads are in flight ahead of their use
+    const double flx = floor(qx), fly = floor(qy);
+    const float fx = (float)(qx - flx), fy = (float)(qy - fly);
+    unsigned o = (unsigned)((int)fly - HMAX / 2) * pitch + (unsigned)((int)flx - WX / 2 + ln.c) * sizeof(T);
+    o = ln.zero ? 0x80000000u : o; // past every level: the zero lane's T - J is 0
+    typename Row<T>::P raw[HMAX + 1];
+#pragma unroll
+    for (int r = 0; r < K; r++)
+        raw[r] = Row<T>::lp(rs, o, (unsigned)r * pitch);
+    float prev = 0.0f, s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+    for (int r = 0; r <= HMAX; r++) {
+        float a0, a1;
+        Row<T>::cp(raw[r], a0, a1);
+        const float cur = fmaf(a1 - a0, fx, a0);
+        if (r > 0) {
+            const float jv = fmaf(cur - prev, fy, prev);
+            const float it = fmaf(jv, -ks, tp.t[r - 1]); // T - J, the product by ks exact
+            const float iu = bperm(ln.src, it);
+            if (RES) {
+                const float q = it * (1.0f / 64.0f), qu = iu * (1.0f / 64.0f); // 0 from the zero lane
+                s0 = s0 + fabsf(q);
+                s1 = fmaf(q, q, s1);
+                s0 = s0 + fabsf(qu);
+                s1 = fmaf(qu, qu, s1);
+            } else {
+                s0 = fmaf(it, tp.tx[r - 1], s0);
+                s1 = fmaf(it, tp.ty[r - 1], s1);
+                s0 = fmaf(iu, tp.ux[r - 1], s0);
+                s1 = fmaf(iu, tp.uy[r - 1], s1);
+            }
+        }
+        prev = cur;
+        if (r + K <= HMAX) {
+            o = after(o, prev);
+            raw[r + K] = Row<T>::lp(rs, o, (unsigned)(r + K) * pitch);
+        }
+    }
+    b[0] = bperm(ln.head, row_sum(s0));
+    b[1] = bperm(ln.head, row_sum(s1));
+}
+
+// The neighbourhood of J with corner (rx, ry) into the point's LDS region (as stage; the vector lanes load it).
+template <int HMAX, int WX, typename T>
+__device__ __forceinline__ void stage3(__amdgpu_buffer_rsrc_t rs, unsigned pitch, int rx, int ry, const Lane3 &ln,
+                                       float *R)
+{
+    typedef Shape3<HMAX, WX> SH;
+    constexpr int NQ = (SH::RW + 15) / 16, CH = 7;
+    wave_sync(); // every earlier read of the region is done
+    if (ln.vec) {
+        const int j = ln.j;
+        unsigned o = (unsigned)ry * pitch + (unsigned)(rx + j) * (unsigned)sizeof(T);
+#pragma unroll
+        for (int r0 = 0; r0 < SH::RH; r0 += CH) {
+            float v[CH][NQ];
+#pragma unroll
+            for (int r = r0; r < r0 + CH && r < SH::RH; r++)
+#pragma unroll
+                for (int q = 0; q < NQ; q++)
+                    if (q < NQ - 1 || j + 16 * q < SH::RW)
+                        v[r - r0][q] = Row<T>::l1(rs, o + 16u * q * (unsigned)sizeof(T), (unsigned)r * pitch);
+#pragma unroll
+            for (int r = r0; r < r0 + CH && r < SH::RH; r++)
+#pragma unroll
+                for (int q = 0; q < NQ; q++)
+                    if (q < NQ - 1 || j + 16 * q < SH::RW)
+                        R[r * SH::RW + j + 16 * q] = v[r - r0][q];
+            o = after(o, v[0][0]); // the next chunk's loads after this chunk's
+        }
+    }
+    wave_sync(); // the region before any lane reads it
+}
+
+// iterate_vx (lk_vx.c:100-140) on the staged neighbourhood: the window's corner is (bx, by) in the region.
+template <int HMAX, int WX>
+__device__ __forceinline__ void sample_st3(const float *R, int bx, int by, float fx, float fy, float ks,
+                                           const Lane3 &ln, const Tmpl3<HMAX> &tp, float *b)
+{
+    typedef Shape3<HMAX, WX> SH;
+    constexpr int K = 4; // rows read ahead of their use
+    // the region's rows from three byte addresses, rows 0-9, 10-19 and 20-, each within the reads' immediate offsets
+    constexpr int RB = 10;
+    static_assert(HMAX + 1 <= 3 * RB, "");
+    unsigned A[3];
+    A[0] = lds_addr(R) + 4u * (unsigned)(by * SH::RW + bx + ln.c);
+    A[1] = A[0] + RB * SH::RW * 4;
+    A[2] = A[1] + RB * SH::RW * 4;
+    float raw0[HMAX + 1], raw1[HMAX + 1];
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        raw0[r] = lds_f(A[r / RB] + (r % RB) * SH::RW * 4);
+        raw1[r] = lds_f(A[r / RB] + (r % RB) * SH::RW * 4 + 4);
+    }
+
+    // the sums run D window rows behind the values they add, so a row's T - J has come back from the second-block
+    // lane (ds_bpermute) by the time its sums need it
+    constexpr int D = 3;
+    float it[HMAX], iu[HMAX];
+    float prev = 0.0f, s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+    for (int r = 0; r <= HMAX + D; r++) {
+        if (r <= HMAX) {
+            const float cur = fmaf(raw1[r] - raw0[r], fx, raw0[r]);
+            if (r > 0) {
+                const float jv = fmaf(cur - prev, fy, prev);
+                it[r - 1] = fmaf(jv, -ks, tp.t[r - 1]); // T - J, the product by ks exact
+                iu[r - 1] = bperm(ln.src, it[r - 1]);
+            }
+            prev = cur;
+            if (r + K <= HMAX) {
+                const int n = r + K;
+                A[n / RB] = after(A[n / RB], prev);
+                raw0[n] = lds_f(A[n / RB] + (n % RB) * SH::RW * 4);
+                raw1[n] = lds_f(A[n / RB] + (n % RB) * SH::RW * 4 + 4);
+            }
+        }
+        const int w = r - 1 - D; // the window row whose sums this step adds
+        if (w >= 0 && w < HMAX) {
+            s0 = fmaf(it[w], tp.tx[w], s0);
+            s1 = fmaf(it[w], tp.ty[w], s1);
+            s0 = fmaf(iu[w], tp.ux[w], s0);
+            s1 = fmaf(iu[w], tp.uy[w], s1);
+        }
+    }
+    b[0] = bperm(ln.head, row_sum(s0));
+    b[1] = bperm(ln.head, row_sum(s1));
+}
+
+// One point from pyramid I to J in this layout: the level walk of track (lk_driver.inc:7-109), every value the same in
+// all lanes of the point.
+template <int HMAX, int WX, typename T0, typename T1>
+__device__ __forceinline__ void track3(const PyrView &I, const PyrView &J, int top, const Cfg c, bool mixed,
+                                       const Lane3 &ln, float *R, float x, float y, float gx0, float gy0,
+                                       Track &o)
+{
+    typedef Shape3<HMAX, WX> SH;
+    Tmpl3<HMAX> tp;
+    double gx = ldexp((double)gx0, -top), gy = ldexp((double)gy0, -top);
+    const double area = (double)(WX * HMAX), eps2 = (double)c.eps * (double)c.eps;
+    constexpr int mx = WX / 2 + 1, my = HMAX / 2 + 1;
+    float G0[3] = {0.0f, 0.0f, 0.0f}; // level 0's structure tensor, for the covariance
+    int status = 0, iters = 0;
+    bool valid0 = false;
+    float eig0 = 0.0f;
+    for (int L = top; L >= 0; L--) {
+        const double px = ldexp((double)x, -L), py = ldexp((double)y, -L);
+        const Level li = I.l[L], lj = J.l[L]; // copies: the descriptors stay in registers through the iterations
+        if (px < mx || py < my || px >= (double)(li.w - mx - 1) || py >= (double)(li.h - my - 1)) {
+            if (L == 0)
+                status |= LK_OUT;
+            else
+                gx *= 2, gy *= 2;
+            continue;
+        }
+        float G[3];
+        if (std::is_same<T0, T1>::value || L > 0)
+            build3<HMAX, WX, T1>(li, px, py, ln, tp, G);
+        else
+            build3<HMAX, WX, T0>(li, px, py, ln, tp, G);
+        const double gxx = G[0], gxy = G[1], gyy = G[2];
+        const Solve s = solve2(gxx, gxy, gyy, area, mixed);
+        if (L == 0)
+            eig0 = (float)s.eig;
+        if (s.eig < (double)c.min_eig || s.det <= 0) {
+            if (L == 0)
+                status |= LK_LOWEIG;
+            else
+                gx *= 2, gy *= 2;
+            continue;
+        }
+        if (L == 0) {
+            G0[0] = G[0], G0[1] = G[1], G0[2] = G[2];
+            valid0 = true;
+        }
+        const double lx = (double)(lj.w - mx - 1), ly = (double)(lj.h - my - 1);
+        const double sx = px + gx, sy = py + gy; // the reference adds px + gx first, then the level's step
+        unsigned pj;
+        const __amdgpu_buffer_rsrc_t rsj = rsrc_at<T0, T1>(L, lj, pj); // once per level, not per iteration
+        const float ks = (std::is_same<T0, T1>::value || L > 0) ? Ld<T1>::scale : Ld<T0>::scale;
+        // the staged region's corner stays inside the level: a negative lane offset would wrap, and rows reached
+        // through the scalar row step past it would be range-checked as out of the level
+        int rx = max((int)floor(sx) - WX / 2 - SH::M, 0);
+        int ry = max((int)floor(sy) - HMAX / 2 - SH::M, 0);
+        if (std::is_same<T0, T1>::value || L > 0)
+            stage3<HMAX, WX, T1>(rsj, pj, rx, ry, ln, R);
+        else
+            stage3<HMAX, WX, T0>(rsj, pj, rx, ry, ln, R);
+        double vx = 0, vy = 0;
+        for (int it = 0; it < c.max_iter; it++) {
+            const double qx = sx + vx, qy = sy + vy;
+            if (qx < mx || qy < my || qx >= lx || qy >= ly) {
+                if (L == 0)
+                    status |= LK_OUT, valid0 = false;
+                break;
+            }
+            const double flx = floor(qx), fly = floor(qy);
+            int bx = (int)flx - WX / 2 - rx, by = (int)fly - HMAX / 2 - ry;
+            if (bx < 0 || by < 0 || bx + WX + 1 > SH::RW || by + HMAX + 1 > SH::RH) {
+                // the window left the region: stage again around it
+                rx = max(rx + bx - SH::M, 0);
+                ry = max(ry + by - SH::M, 0);
+                if (std::is_same<T0, T1>::value || L > 0)
+                    stage3<HMAX, WX, T1>(rsj, pj, rx, ry, ln, R);
+                else
+                    stage3<HMAX, WX, T0>(rsj, pj, rx, ry, ln, R);
+                bx = (int)flx - WX / 2 - rx;
+                by = (int)fly - HMAX / 2 - ry;
+            }
+            float b[2];
+            sample_st3<HMAX, WX>(R, bx, by, (float)(qx - flx), (float)(qy - fly), ks, ln, tp, b);
+            const double bxx = (double)b[0] / 64.0, byy = (double)b[1] / 64.0;
+            const double ex = s.ia * bxx + s.ib * byy, ey = s.ib * bxx + s.ic * byy;
+            vx += ex;
+            vy += ey;
+            iters++;
+            if (ex * ex + ey * ey < eps2)
+                break;
+        }
+        gx += vx;
+        gy += vy;
+        if (L > 0)
+            gx *= 2, gy *= 2;
+    }
+    const double qx = (double)x + gx, qy = (double)y + gy;
+    o.x = (float)qx;
+    o.y = (float)qy;
+    o.min_eig = eig0;
+    o.iters = iters;
+    o.residual = 0.0f;
+    o.cxx = o.cxy = o.cyy = 0.0f;
+    if (valid0 && !status) {
+        const Level &l0 = J.l[0];
+        if (qx < mx || qy < my || qx >= (double)(l0.w - mx - 1) || qy >= (double)(l0.h - my - 1)) {
+            status |= LK_OUT;
+        } else {
+            float r[2];
+            unsigned p0;
+            const __amdgpu_buffer_rsrc_t rs0 = level_rsrc<T0>(l0, p0);
+            sample3<HMAX, WX, T0, true>(rs0, p0, qx, qy, ln, tp, r);
+            constexpr int n = WX * HMAX;
+            o.residual = mean_ld(r[0], n);
+            if (o.residual > c.max_res)
+                status |= LK_RESIDUAL;
+            cov_ld(mean_ld(r[1], n), n, G0[0], G0[2], G0[1], &o.cxx, &o.cxy, &o.cyy);
+        }
+    }
+    o.status = status;
+}
+
+template <int HMAX, int WX, typename T0, typename T1>
+__global__ __launch_bounds__(512) void lk3_kernel(LkArgs a)
+{
+    extern __shared__ float4 lk_lds[];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    // workgroup b runs on XCD b mod 8; XCD x takes the contiguous run of workgroups [x per_xcd, (x + 1) per_xcd)
+    const int wg = (int)(blockIdx.x % NXCD) * a.per_xcd + (int)(blockIdx.x / NXCD);
+    const int s0 = (wg * a.waves + wave) * PPW3;
+    if (s0 >= a.slots)
+        return;
+    const int q = __builtin_amdgcn_readfirstlane(pair_of(a.pre, a.npairs, s0));
+    const Lane3 ln = lane3<WX>(lane);
+    const int local = s0 - a.pre[q] + ln.p;
+    const bool live = local < a.cnt[q];
+    const int pt = a.start[q] + (live ? local : 0);
+    // an empty slot starts far outside every level: it fits none, reads nothing and writes nothing
+    float x = -1e30f, y = -1e30f, gx = 0.0f, gy = 0.0f;
+    if (live) {
+        x = a.xy[2 * pt];
+        y = a.xy[2 * pt + 1];
+        if (a.guess) {
+            gx = a.guess[2 * pt];
+            gy = a.guess[2 * pt + 1];
+        }
+    }
+    const int top = a.top[q], fb_top = a.fb_top >= 0 ? a.fb_top : top;
+    typedef Shape3<HMAX, WX> SH;
+    float *R = (float *)((char *)lk_lds + wave * SH::WAVE_BYTES) + ln.p * SH::STAGE;
+    const bool mixed = a.solve == VTG_SOLVE_MIXED;
+    const Cfg cfg = a.cfg;
+    const float fb_max = a.fb_max;
+    const PyrView *pI = a.I[q], *pJ = a.J[q];
+    // the point, then its flowback (lk_driver.inc:111-121): one call site of track3 for both passes
+    Track o;
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1 && !(fb_max > 0.0f && o.status == 0))
+            break;
+        Track t;
+        const float sx = pass ? o.x : x, sy = pass ? o.y : y;
+        const float ux = pass ? (float)((double)x - (double)o.x) : gx;
+        const float uy = pass ? (float)((double)y - (double)o.y) : gy;
+        track3<HMAX, WX, T0, T1>(pass ? *pJ : *pI, pass ? *pI : *pJ, pass ? fb_top : top, cfg, mixed, ln, R, sx, sy,
+                                 ux, uy, t);
+        if (pass == 0) {
+            o = t;
+        } else {
+            const double ex = (double)t.x - (double)x, ey = (double)t.y - (double)y;
+            if (t.status || ex * ex + ey * ey > (double)fb_max * (double)fb_max)
+                o.status |= LK_FLOWBACK;
+        }
+    }
+    if (live && ln.vec && ln.j == 0)
+        store(a.out + pt, o);
+}
+
+template <int HMAX, int WX>
+int dispatch3(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 block)
+{
+    const size_t lds = (size_t)a.waves * Shape3<HMAX, WX>::WAVE_BYTES;
+    switch (storage) {
+    case VTG_STORE_Q6:
+        hipLaunchKernelGGL((lk3_kernel<HMAX, WX, uint8_t, uint16_t>), grid, block, lds, c->stream, a);
+        break;
+    case VTG_STORE_F32:
+    case VTG_STORE_F32_U8:
+        hipLaunchKernelGGL((lk3_kernel<HMAX, WX, float, float>), grid, block, lds, c->stream, a);
+        break;
+    case VTG_STORE_Q6W:
+        hipLaunchKernelGGL((lk3_kernel<HMAX, WX, uint16_t, uint16_t>), grid, block, lds, c->stream, a);
+        break;
+    case VTG_STORE_Q6F:
+        hipLaunchKernelGGL((lk3_kernel<HMAX, WX, uint8_t, float>), grid, block, lds, c->stream, a);
+        break;
+    case VTG_STORE_F16:
+        hipLaunchKernelGGL((lk3_kernel<HMAX, WX, uint8_t, __half>), grid, block, lds, c->stream, a);
+        break;
+    case VTG_STORE_BF16:
+        hipLaunchKernelGGL((lk3_kernel<HMAX, WX, uint8_t, __hip_bfloat16>), grid, block, lds, c->stream, a);
+        break;
+    default:
+        return fail(c, VTG_EINVAL, "storage %d", storage);
+    }
+    VTG_HIP(c, hipGetLastError());
+    return VTG_OK;
+}
+
 // The compiled window shapes: the windows of every workload here, 15 x 15 and 21 x 21, have their own exact shapes
 // (straight-line code); any other window runs the guarded form of its block count with 31 rows.
 template <int NV, int HMAX, int WX>
@@ -952,6 +1459,10 @@ int shape_lds_point(const Cfg &cfg)
         return Shape<2, 21, 21>::LDS_BYTES;
     return 0;
 }
+
+// The window shapes compiled in the three-points-per-wavefront layout.
+bool three_shape(const Cfg &cfg) { return cfg.w == 21 && cfg.h == 21; }
+bool three_shape(const lk_cfg &cfg) { return cfg.w == 21 && cfg.h == 21; }
 
 int dispatch_shape(vtg_ctx *c, const LkArgs &a, int storage, dim3 grid, dim3 block)
 {
@@ -1257,10 +1768,13 @@ Cfg to_cfg(const lk_cfg *cfg) { return Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->e
 // One launch for pairs sel[0 .. m) (m <= BMAX; indices into the caller's arrays) with `waves` wavefronts per workgroup.
 static int launch_group(vtg_ctx *c, const std::vector<int> &sel, const vtg_pyr *const *I, const vtg_pyr *const *J,
                         const int *top, const int *off, const lk_cfg *cfg, const float *xy, const float *guess,
-                        float fb_max, int solve_mode, lk_out *out, int waves, int fb_top)
+                        float fb_max, int solve_mode, lk_out *out, int waves, int ppw, int fb_top)
 {
     LkArgs a;
     a.npairs = (int)sel.size();
+    // points per wavefront: 3 (one lane per window column) where the window's shape is compiled in that layout, else 4
+    const bool three = ppw != PPW && three_shape(*cfg);
+    const int per = three ? PPW3 : PPW;
     int acc = 0;
     for (int q = 0; q < a.npairs; q++) {
         const int k = sel[q];
@@ -1270,7 +1784,7 @@ static int launch_group(vtg_ctx *c, const std::vector<int> &sel, const vtg_pyr *
         a.start[q] = off[k];
         a.cnt[q] = off[k + 1] - off[k];
         a.pre[q] = acc;
-        acc += (a.cnt[q] + PPW - 1) / PPW * PPW;
+        acc += (a.cnt[q] + per - 1) / per * per;
     }
     a.pre[a.npairs] = acc;
     a.slots = acc;
@@ -1290,19 +1804,22 @@ static int launch_group(vtg_ctx *c, const std::vector<int> &sel, const vtg_pyr *
     static int max_lds = 0;
     if (!max_lds)
         VTG_HIP(c, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-    const int per_wave = PPW * shape_lds_point(a.cfg);
+    const int per_wave = three ? Shape3<21, 21>::WAVE_BYTES : PPW * shape_lds_point(a.cfg);
     while (waves > 1 && per_wave * waves > max_lds)
         waves--;
     a.waves = waves;
-    const int wgs = (a.slots / PPW + a.waves - 1) / a.waves;
+    const int wgs = (a.slots / per + a.waves - 1) / a.waves;
     a.per_xcd = (wgs + NXCD - 1) / NXCD;
     const dim3 grid(a.per_xcd * NXCD), block(64 * a.waves);
     hipEvent_t ev;
     timer_begin(c, "lk_track", &ev);
-    const int rc = dispatch_shape(c, a, I[sel[0]]->storage, grid, block);
+    const int rc = three ? dispatch3<21, 21>(c, a, I[sel[0]]->storage, grid, block)
+                         : dispatch_shape(c, a, I[sel[0]]->storage, grid, block);
     timer_end(c, "lk_track", ev);
     return rc;
 }
+
+int lk_default_ppw(const lk_cfg *cfg) { return three_shape(*cfg) ? PPW3 : PPW; }
 
 uint64_t lk_class(const vtg_pyr *I, int top, const lk_cfg *cfg)
 {
@@ -1315,7 +1832,7 @@ LkSched tuned_lk(const vtg_ctx *c, const vtg_pyr *I, int top, const lk_cfg *cfg)
     const auto it = c->lk_sched.find(lk_class(I, top, cfg));
     if (it != c->lk_sched.end())
         return it->second;
-    return LkSched{c->lk_waves};
+    return LkSched{c->lk_waves, c->lk_ppw};
 }
 
 // Pairs [0, np) with points off[k] .. off[k+1] - 1 of xy, one launch per BMAX pairs. The workgroup size is `sched`'s
@@ -1325,22 +1842,23 @@ int launch_lk_many(vtg_ctx *c, int np, const vtg_pyr *const *I, const vtg_pyr *c
                    const int *off, const lk_cfg *cfg, const float *xy, const float *guess, float fb_max,
                    int solve_mode, lk_out *out, const LkSched *sched, int fb_top)
 {
-    int waves = c->lk_waves;
+    LkSched s{c->lk_waves, c->lk_ppw};
     if (sched) {
-        waves = sched->waves;
+        s = *sched;
     } else {
         int most = -1;
         for (int k = 0; k < np; k++)
             if (off[k + 1] - off[k] > most) {
                 most = off[k + 1] - off[k];
-                waves = tuned_lk(c, I[k], top[k], cfg).waves;
+                s = tuned_lk(c, I[k], top[k], cfg);
             }
     }
     for (int b = 0; b < np; b += BMAX) {
         std::vector<int> part;
         for (int k = b; k < std::min(np, b + BMAX); k++)
             part.push_back(k);
-        const int rc = launch_group(c, part, I, J, top, off, cfg, xy, guess, fb_max, solve_mode, out, waves, fb_top);
+        const int rc =
+            launch_group(c, part, I, J, top, off, cfg, xy, guess, fb_max, solve_mode, out, s.waves, s.ppw, fb_top);
         if (rc)
             return rc;
     }
@@ -1478,6 +1996,14 @@ int vtg_lk_set_waves(vtg_ctx *c, int waves)
     if (!c || waves < 1 || waves > 8)
         return fail(c, VTG_EINVAL, "vtg_lk_set_waves: %d (1..8)", waves);
     c->lk_waves = waves;
+    return VTG_OK;
+}
+
+int vtg_lk_set_ppw(vtg_ctx *c, int ppw)
+{
+    if (!c || (ppw != 0 && ppw != 3 && ppw != 4))
+        return fail(c, VTG_EINVAL, "vtg_lk_set_ppw: %d (0, 3 or 4)", ppw);
+    c->lk_ppw = ppw;
     return VTG_OK;
 }
 

@@ -1,7 +1,7 @@
 // tune.cpp: the autotuner. For a frame pair of a size class it times every candidate schedule of the two launches that
-// carry the batch -- the pyramid reduce (output tiles of 8, 16 or 32 rows) and the LK kernel (1, 2, 4 or 8
-// wavefronts of 4 points each per workgroup) -- on the caller's own frames and points, and keeps the fastest for that
-// class in the context. Later builds and LK launches of pyramids of the class use the picks. A batch of several
+// carry the batch -- the pyramid reduce (output tiles of 8, 16 or 32 rows) and the LK kernel (3 or 4 points per
+// wavefront, where the window allows 3; 1, 2, 4 or 8 wavefronts per workgroup) -- on the caller's own frames and
+// points, and keeps the fastest for that class in the context. Later builds and LK launches of pyramids of the class use the picks. A batch of several
 // classes stays one LK launch: it takes the pick of the pair holding the most points. Every schedule computes the same
 // bytes (the tests hold them to it), so tuning changes time only.
 #include <algorithm>
@@ -101,25 +101,32 @@ int vtg_tune_pair(vtg_ctx *c, vtg_pyr *I, vtg_pyr *J, const uint8_t *f0, const u
     const vtg_pyr *Ic = I, *Jc = J;
     best = FLT_MAX;
     int rc = VTG_OK;
-    for (int waves : {1, 2, 4, 8}) {
-        const LkSched s{waves};
-        float us = 0.0f;
-        rc = median_us(c, 3, [&] {
-            return launch_lk_many(c, 1, &Ic, &Jc, &top, off, cfg, xy, nullptr, 0.0f, solve_mode, out, &s);
-        }, &us);
-        if (rc)
-            break;
-        res->candidates++;
-        if (waves == c->lk_waves)
-            res->lk_us_default = us;
-        if (us < best)
-            best = us, res->lk_waves = waves, res->lk_us = us;
+    const LkSched dflt = LkSched{c->lk_waves, c->lk_ppw ? c->lk_ppw : lk_default_ppw(cfg)};
+    for (int ppw : {3, 4}) {
+        if (ppw == 3 && lk_default_ppw(cfg) != 3)
+            continue; // the window has no three-point form
+        for (int waves : {1, 2, 4, 8}) {
+            if (rc)
+                break;
+            const LkSched s{waves, ppw};
+            float us = 0.0f;
+            rc = median_us(c, 3, [&] {
+                return launch_lk_many(c, 1, &Ic, &Jc, &top, off, cfg, xy, nullptr, 0.0f, solve_mode, out, &s);
+            }, &us);
+            if (rc)
+                break;
+            res->candidates++;
+            if (waves == dflt.waves && ppw == dflt.ppw)
+                res->lk_us_default = us;
+            if (us < best)
+                best = us, res->lk_waves = waves, res->lk_ppw = ppw, res->lk_us = us;
+        }
     }
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(out);
     if (rc)
         return rc;
-    c->lk_sched[lk_class(I, top, cfg)] = LkSched{res->lk_waves};
+    c->lk_sched[lk_class(I, top, cfg)] = LkSched{res->lk_waves, res->lk_ppw};
     return VTG_OK;
 }
 
@@ -131,6 +138,7 @@ int vtg_tune_get(vtg_ctx *c, const vtg_pyr *I, int top, const struct lk_cfg *cfg
     res->pyr_rows = tuned_pyr_rows(c, I);
     const LkSched s = tuned_lk(c, I, top, cfg);
     res->lk_waves = s.waves;
+    res->lk_ppw = s.ppw ? s.ppw : lk_default_ppw(cfg);
     return VTG_OK;
 }
 

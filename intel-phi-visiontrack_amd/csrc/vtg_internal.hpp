@@ -39,6 +39,7 @@ struct Timer {
 // An LK launch schedule: wavefronts (of 4 points each) per workgroup, 1 .. 8.
 struct LkSched {
     int waves;
+    int ppw; // points per wavefront: 0 (3 where the window's shape allows, else 4), 3 or 4
 };
 
 }  // namespace vtg
@@ -62,6 +63,7 @@ struct vtg_ctx {
     hipEvent_t join = nullptr;               // for vtg_ctx_wait
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;
+    int lk_ppw = 0;                          // LK points per wavefront where no pick applies (0: the shape's best)
     int lk_waves = 1;                        // LK schedule: wavefronts per workgroup where no pick applies
     // the autotuner's picks per size class (keys from pyr_class / lk_class)
     std::map<uint64_t, int> pyr_rows;
@@ -148,6 +150,8 @@ uint64_t pyr_class(const vtg_pyr *p);
 uint64_t lk_class(const vtg_pyr *I, int top, const lk_cfg *cfg);
 int tuned_pyr_rows(const vtg_ctx *c, const vtg_pyr *p);
 LkSched tuned_lk(const vtg_ctx *c, const vtg_pyr *I, int top, const lk_cfg *cfg);
+// Points per wavefront of the LK layout a window runs in by default: 3 where that layout is compiled, else 4.
+int lk_default_ppw(const lk_cfg *cfg);
 int launch_ncc_template(vtg_ctx *c, const vtg_pyr *p, int level, int x0, int y0, vtg_tmpl *t);
 // Look up the fixed search array's address on the context's device (current), once per device (ncc.hip).
 int fixed_scratch_init(vtg_ctx *c);

@@ -139,8 +139,10 @@ def test_uneven_batch(ctx, oracle):
         n = b.offsets[k + 1] - b.offsets[k]
         r = ctx.tune_pair(b.pyrs[k][0], b.pyrs[k][1], b.frames[k][0], b.frames[k][1], L - 1, cfg,
                           b.xy.view(8 * b.offsets[k], 8 * n))
-        assert r["candidates"] == 7 and r["pyr_rows"] in (8, 16, 32) and r["lk_waves"] in (1, 2, 4, 8)
-        assert ctx.tune_get(b.pyrs[k][0], L - 1, cfg)["lk_waves"] == r["lk_waves"]
+        assert r["candidates"] == 3 + 8 and r["pyr_rows"] in (8, 16, 32) and r["lk_waves"] in (1, 2, 4, 8)
+        assert r["lk_ppw"] in (3, 4)
+        got_pick = ctx.tune_get(b.pyrs[k][0], L - 1, cfg)
+        assert (got_pick["lk_waves"], got_pick["lk_ppw"]) == (r["lk_waves"], r["lk_ppw"])
     b.enqueue(cfg)
     ctx.sync()
     tuned = b.out.get(np.zeros(b.npoints, vtgpu.LK_OUT_DTYPE))

@@ -283,8 +283,9 @@ def test_precision_variants_vs_f64(ctx, oracle, storage, solve, tol):
 
 
 def test_schedules_bit_identical_and_batch(ctx, oracle):
-    """Every LK schedule (1, 2, 4, 8 wavefronts per workgroup) gives the same bytes, and one batched launch over pairs
-    of uneven sizes (point counts not multiples of 4) gives the same bytes as one launch per pair and as the lane tier."""
+    """Every LK schedule (3 or 4 points per wavefront, 1, 2, 4, 8 wavefronts per workgroup) gives the same bytes, and one
+    batched launch over pairs of uneven sizes (point counts not multiples of 3 or 4) gives the same bytes as one launch
+    per pair and as the lane tier."""
     sizes = [(640, 480, 3, 401), (1280, 720, 4, 902), (333, 250, 2, 151), (1920, 1080, 4, 1203)]
     cfg = lk_cfg(21, 21, 10, 0.01, 1e-4, 40.0)
     pairs, xys, offs, frames = [], [], [0], []
@@ -300,20 +301,24 @@ def test_schedules_bit_identical_and_batch(ctx, oracle):
     dxy = ctx.to_device(xy)
     Lb = vtgpu.lib()
     results = {}
-    for waves in (1, 2, 4, 8):
-        say("lk schedule waves", waves)
-        ctx.check(Lb.vtg_lk_set_waves(ctx.h, waves))
-        out = ctx.lk_track_many([p[0] for p in pairs], [p[1] for p in pairs], [p[2] for p in pairs], offs, cfg, dxy,
-                                fb_max=0.5)
-        results[waves] = out.get(np.zeros(offs[-1], vtgpu.LK_OUT_DTYPE))
-        out.free()
-    for wv in (1, 2, 8):
-        assert results[wv].tobytes() == results[4].tobytes(), wv
+    for ppw in (3, 4):
+        ctx.check(Lb.vtg_lk_set_ppw(ctx.h, ppw))
+        for waves in (1, 2, 4, 8):
+            say("lk schedule ppw", ppw, "waves", waves)
+            ctx.check(Lb.vtg_lk_set_waves(ctx.h, waves))
+            out = ctx.lk_track_many([p[0] for p in pairs], [p[1] for p in pairs], [p[2] for p in pairs], offs, cfg,
+                                    dxy, fb_max=0.5)
+            results[ppw, waves] = out.get(np.zeros(offs[-1], vtgpu.LK_OUT_DTYPE))
+            out.free()
+    for key in results:
+        assert results[key].tobytes() == results[3, 4].tobytes(), key
     assert Lb.vtg_lk_set_waves(ctx.h, 9) == -22
+    assert Lb.vtg_lk_set_ppw(ctx.h, 5) == -22
     ctx.check(Lb.vtg_lk_set_waves(ctx.h, 1))
+    ctx.check(Lb.vtg_lk_set_ppw(ctx.h, 0))
     for k, (PI, PJ, top) in enumerate(pairs):
         one = ctx.lk_track_host(PI, PJ, top, cfg, xys[k][0::2], xys[k][1::2], fb_max=0.5)
-        assert one.tobytes() == results[4][offs[k]:offs[k + 1]].tobytes(), k
+        assert one.tobytes() == results[3, 4][offs[k]:offs[k + 1]].tobytes(), k
         f0, f1, w, h, L = frames[k]
         kI, I = oracle.pyramid(f0, w, h, L, threads=8)
         kJ, J = oracle.pyramid(f1, w, h, L, threads=8)
