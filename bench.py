@@ -200,7 +200,26 @@ def plain_rate(vtgpu, d_frames, corners, xs, ys, steps):
     for i in range(steps):
         one(i)
     ctx.sync()
-    return steps / (time.perf_counter() - t)
+    rate = steps / (time.perf_counter() - t)
+    # the same pairs once more with HIP events around every launch: milliseconds per pair by stage
+    ctx.timing(True)
+    ctx.timing_reset()
+    for i in range(steps):
+        one(i)
+    ctx.sync()
+    names = {"pyramids": ("pyr_level0", "pyr_reduce"), "lk": ("lk_track",),
+             "ncc": ("ncc_template", "ncc_cells_generic", "ncc_cells", "ncc_top8", "ncc_refine_final")}
+    stage = {s: sum(ctx.timing_read(k)[0] for k in ks) / steps for s, ks in names.items()}
+    cells_ms = ctx.timing_read("ncc_cells_generic")[0] / steps
+    ctx.timing(False)
+    npos = (W - TPL - 2) * (H - TPL - 2)
+    ncc_tflops = 2.0 * TPL * TPL * npos / (cells_ms * 1e-3) / 1e12 if cells_ms > 0 else None
+    for b in (p0, p1):
+        b.destroy()
+    return rate, {"stage_ms_per_pair": {k: round(v, 4) for k, v in stage.items()},
+                  "ncc_map_kernel": {"name": "ncc_cells_generic (the reference's fp32 lane arithmetic)",
+                                     "ms_per_search": round(cells_ms, 3),
+                                     "achieved_tflops": round(ncc_tflops, 2) if ncc_tflops else None}}
 
 
 def batching(vtgpu, steps, seed=2024):
@@ -467,6 +486,119 @@ def stereo_bench(vtgpu, calls, nfeat=512, d0=40):
     return out
 
 
+def host_cpu():
+    """The host's processor model and logical CPU count, and the threads this run uses (a one-GPU box's share)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "threads_used": min(os.cpu_count() or 1, 16),
+            "note": "a one-GPU box's CPU share is 16 threads; the baseline uses that many"}
+
+
+def config1(o, cores, reps=7):
+    """The reference's plain CPU case (640 x 480 pair, 3 levels, 1,200 points on a 40 x 30 grid, 15 x 15, 10 iterations,
+    eps 0.01, min_eig 1e-4) on the oracle's lane tier: both pyramids and the LK of all points, median of `reps` pairs,
+    one thread and all threads; the GPU's plain form on the same pair beside it."""
+    import vtgpu
+    from oracle_ffi import lk_cfg
+    w, h, L = 640, 480, 3
+    f0, f1, _ = vtgpu.frame_pair(w, h, 1)
+    xs1, ys1 = vtgpu.grid(40, 30, w, h)
+    cfg = lk_cfg(15, 15, 10, 0.01, 1e-4, 40.0)
+    res = {}
+    for threads in (1, cores):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            k0, I = o.pyramid(f0, w, h, L, threads=threads)
+            k1, J = o.pyramid(f1, w, h, L, threads=threads)
+            o.lk_batch(0, I, J, L - 1, cfg, xs1, ys1, threads=threads)
+            ts.append(time.perf_counter() - t)
+        ms = float(np.median(ts)) * 1e3
+        res[f"cpu_{threads}_threads"] = {"ms_per_pair": round(ms, 3), "lk_pts_per_s": round(len(xs1) / (ms * 1e-3), 1)}
+    ctx = vtgpu.Context(0)
+    p0, p1 = ctx.pyramid(w, h, L), ctx.pyramid(w, h, L)
+    d0, d1 = ctx.to_device(f0), ctx.to_device(f1)
+    xy = ctx.to_device(np.stack([xs1, ys1], 1).ravel())
+    gcfg = vtgpu.LkCfg(15, 15, 10, 0.01, 1e-4, 40.0, 0)
+    out = ctx.alloc(32 * len(xs1))
+    h_out = np.zeros(len(xs1), vtgpu.LK_OUT_DTYPE)
+    ts = []
+    for i in range(reps + 3):
+        t = time.perf_counter()
+        p0.build(d0, on_device=True)
+        p1.build(d1, on_device=True)
+        ctx.lk_track(p0, p1, L - 1, gcfg, xy, None, 0.0, out)
+        out.get(h_out)
+        if i >= 3:
+            ts.append(time.perf_counter() - t)
+    ms = float(np.median(ts)) * 1e3
+    res["gpu_plain_one_pair"] = {"ms_per_pair": round(ms, 3), "lk_pts_per_s": round(len(xs1) / (ms * 1e-3), 1),
+                                 "note": "host to results on the host, one pair per call (launch-bound)"}
+    res["workload"] = "640x480 pair, 3 levels, 1,200 points (40x30 grid), 15x15, 10 iterations, eps 0.01"
+    for b in (out, d0, d1, xy):
+        b.free()
+    for b in (p0, p1):
+        b.destroy()
+    ctx.close()
+    log(f"config 1: {res}")
+    return res
+
+
+def pcie_rate(vtgpu, bt, f0s, f1s, steps):
+    """The batched step with its frames arriving in pinned host memory: each step's 2B frames are uploaded over PCIe on
+    an upload context while the previous step computes, and the step's graph waits for its upload. Pairs per second
+    including the uploads."""
+    B = bt.B
+    up = [vtgpu.Context(0), vtgpu.Context(0)]
+    host = [vtgpu.PinnedArray(up[0], f0s[k].shape, np.uint8) for k in range(UNIQ)] + \
+           [vtgpu.PinnedArray(up[0], f1s[k].shape, np.uint8) for k in range(UNIQ)]
+    for k in range(UNIQ):
+        host[k].arr[...] = f0s[k]
+        host[UNIQ + k].arr[...] = f1s[k]
+
+    def upload(i):
+        g = i % bt.groups
+        c = up[i % 2]
+        for k in range(B):
+            j = g * B + k
+            a0, a1 = bt.frames[j]
+            a0.put_async(c, host[j % UNIQ].arr)
+            a1.put_async(c, host[UNIQ + j % UNIQ].arr)
+
+    nbytes = 2 * B * W * H
+    upload(0)
+    up[0].sync()
+    for i in range(3):  # warm
+        bt.A.wait(up[i % 2])
+        bt.launch(i)
+        upload(i + 1)
+        bt.finish(i)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        bt.A.wait(up[i % 2])
+        bt.launch(i)
+        upload(i + 1)
+        if i >= 1:
+            bt.finish(i - 1)
+    bt.finish(steps - 1)
+    total = time.perf_counter() - t0
+    for c in up:
+        c.sync()
+    for hp in host:
+        hp.free()
+    r = {"pairs_per_s": round(steps * B / total, 2), "ms_per_step": round(1e3 * total / steps, 4),
+         "upload_bytes_per_step": nbytes, "upload_gb_per_s": round(nbytes * steps / total / 1e9, 2),
+         "note": "frames uploaded from pinned host memory each step, overlapped with the previous step's compute"}
+    log(f"pcie-inclusive: {r}")
+    return r
+
+
 def cpu_baseline(f0s, f1s, corners, xs, ys, budget_s, gpu_out):
     """The oracle's reference-faithful tier (C, the reference's lane arithmetic) on this host, on the same workload:
     whole 1080p pairs (both pyramids, LK of all 16,200 points, the 64 x 64 template's exhaustive NCC map). All cores
@@ -523,17 +655,17 @@ def cpu_baseline(f0s, f1s, corners, xs, ys, budget_s, gpu_out):
     log(f"cpu: {cores} threads: pyramids {stage[0] * 1e3:.1f} ms, LK {stage[1] * 1e3:.0f} ms, NCC "
         f"{stage[2] * 1e3:.0f} ms -> {1 / med:.3f} pairs/s ({len(rounds)} pairs); 1 thread: {1 / one:.4f} pairs/s "
         f"({len(ones)} pairs); {time.perf_counter() - t_start:.1f} s")
-    # the GPU's flow against the FP64 oracle and the reference tier on pair 0 (every 8th point)
-    stride = 8
-    sx, sy = xs[::stride], ys[::stride]
-    well, f64 = conditioning(o, I0, J0, LEVELS - 1, cfg, sx, sy, threads=cores)
-    ref = ref0[::stride]
-    g = gpu_out[::stride]
+    # the GPU's records of pair 0 against the reference tier (the reference's lane arithmetic and x87 driver) and the
+    # FP64 oracle, over every point
+    well, f64 = conditioning(o, I0, J0, LEVELS - 1, cfg, xs, ys, threads=cores)
+    ref, g = ref0, gpu_out
     both = (g["status"] == 0) & (f64["status"] == 0)
     e_all = np.maximum(np.abs(g["x"] - f64["x"]), np.abs(g["y"] - f64["y"]))
-    err = e_all[both & well]
-    same_status = float((g["status"] == ref["status"])[well].mean())
-    ok_ref = (g["status"] == 0) & (ref["status"] == 0) & well
+    err = e_all[both]
+    same_status = float((g["status"] == ref["status"]).mean())
+    same_record = float((np.ascontiguousarray(g).view(np.uint32).reshape(-1, 8) ==
+                         np.ascontiguousarray(ref).view(np.uint32).reshape(-1, 8)).all(1).mean())
+    ok_ref = (g["status"] == 0) & (ref["status"] == 0)
     dref = np.maximum(np.abs(g["x"] - ref["x"]), np.abs(g["y"] - ref["y"]))[ok_ref]
     # storage precision against the FP64 oracle (pyramids fp16 / bf16 / exact, fp64 or mixed solve)
     import vtgpu
@@ -541,15 +673,20 @@ def cpu_baseline(f0s, f1s, corners, xs, ys, budget_s, gpu_out):
     log("precision: " + ", ".join(f"{k} p99 {v['textured']['p99_px']:.2e} px" for k, v in prec.items()
                                   if isinstance(v, dict)))
     return ({"value": round(1.0 / med, 4), "unit": "frame-pairs/s", "cores": cores, "kind": "port",
-             "sample": sample, "one_thread": {"value": round(1.0 / one, 5), "unit": "frame-pairs/s", "cores": 1},
+             "sample": sample, "host": host_cpu(), "one_thread": {"value": round(1.0 / one, 5), "unit": "frame-pairs/s",
+                                                                  "cores": 1},
+             "config1_640x480": config1(o, cores),
              "lk_pts_per_s": round(NPTS / stage[1], 1), "stage_ms": {
                  "pyramids": round(stage[0] * 1e3, 2), "lk": round(stage[1] * 1e3, 1), "ncc": round(stage[2] * 1e3, 1)},
              "cpu_seconds": round(time.perf_counter() - t_start, 2)},
-            {"sample_points": int(len(sx)), "well_conditioned": int(well.sum()),
+            {"points": int(len(xs)), "converged_in_both_vs_f64": int(both.sum()),
+             "well_conditioned": int(well.sum()),
              "vs_f64_median_px": float(np.median(err)), "vs_f64_p99_px": float(np.percentile(err, 99)),
              "vs_f64_max_px": float(err.max()),
-             "share_within_0.05px_all_converged": float((e_all[both] <= 0.05).mean()),
+             "share_within_0.05px_all_converged": float((err <= 0.05).mean()),
+             "vs_f64_max_px_well_conditioned": float(e_all[both & well].max()),
              "status_equal_vs_reference_tier": same_status,
+             "records_bit_equal_vs_reference_tier": same_record,
              "vs_reference_tier_max_px": float(dref.max()) if len(dref) else 0.0}, prec)
 
 
@@ -569,6 +706,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample (0: skip)")
     ap.add_argument("--still-frames", type=int, default=60, help="timed frames of each still-cell camera (0: skip)")
     ap.add_argument("--stereo-calls", type=int, default=30, help="timed calls of the 1080p stereo step (0: skip)")
+    ap.add_argument("--pcie-steps", type=int, default=1, help="1: also time the step with its frames uploaded from "
+                    "pinned host memory each step (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one device-to-device copy of this many MiB first: a plain copy of known size that calibrates "
                          "the HBM byte counters of a profiled run (0: none)")
@@ -666,8 +805,17 @@ def main():
     achieved = per_step / (kernels[dom]["ms_per_step"] * 1e-3)
     lk_us = kernels["lk_track"]["avg_us"]
 
-    plain = plain_rate(vtgpu, d_frames, corners, xs, ys, a.plain_steps)
-    log(f"plain: {plain:.2f} pairs/s")
+    plain, plain_detail = plain_rate(vtgpu, d_frames, corners, xs, ys, max(1, a.plain_steps))
+    log(f"plain: {plain:.2f} pairs/s {plain_detail}")
+    # the batched form's stages per pair (serial kernel pass) beside the plain form's
+    bstage = {"pyramids": kernels.get("pyr_reduce", {}).get("ms_per_step", 0.0) / a.batch,
+              "lk": kernels["lk_track"]["ms_per_step"] / a.batch,
+              "ncc": sum(kernels[k]["ms_per_step"] for k in kernels if k.startswith("ncc_")) / a.batch}
+    by_stage = {s: {"plain_ms_per_pair": plain_detail["stage_ms_per_pair"][s],
+                    "batched_ms_per_pair": round(v, 4),
+                    "ratio": round(plain_detail["stage_ms_per_pair"][s] / v, 2) if v > 0 else None}
+                for s, v in bstage.items()}
+    pcie = pcie_rate(vtgpu, bt, f0s, f1s, a.steps) if a.pcie_steps else {}
     batch = batching(vtgpu, a.batch_steps) if a.batch_steps > 0 else {}
     serve = serving(vtgpu, a.serve_frames) if a.serve_frames > 0 else {}
     still = still_bench(vtgpu, a.still_frames) if a.still_frames > 0 else {}
@@ -678,15 +826,22 @@ def main():
     line = {
         "metric": METRIC, "value": round(pairs_per_s, 2), "unit": "frame-pairs/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(1e3 * total / a.steps, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u8/u16 int + fp32 (fp64 per-point solve)",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8 frames, fp32 Q6 levels (exact integers), fp32 patch sums, fp64 per-point driver",
         "data": "synthetic (seeded 200-blob frames, rigid motion + noise)",
         "config": {"workload": "1080p frame pairs: 2 x 4-level pyramid + LK 16,200 pts 21x21 + NCC 64x64 exhaustive "
                                "per pair", "w": W, "h": H, "levels": LEVELS, "points_per_pair": NPTS,
                    "window": f"{WIN}x{WIN}", "ncc_template": f"{TPL}x{TPL}", "ncc_corners": npos,
-                   "pairs_per_step": a.batch, "storage": "f32_u8 (fp32 Q6 levels: exact integers; the u8 frame kept for the NCC search)", "graph": not a.no_graph,
+                   "pairs_per_step": a.batch,
+                   "storage": "f32_u8 (fp32 Q6 levels: exact integers; the u8 frame kept for the NCC search; the "
+                              "level-1 reduce also writes the fp32 level 0 and the kept frame: 5 B per level-0 pixel "
+                              "beyond the pyramid's algorithmic bytes, which LK's fp32 reads repay)",
+                   "lk_points_per_wavefront": 3, "graph": not a.no_graph,
                    "parallelism": "1 GPU, 1 stream" if a.one_stream else "1 GPU, 2 streams",
                    "input_rotation": f"{pairs} device frame pairs ({pairs * 2 * W * H / 2 ** 20:.0f} MiB)"},
         "plain_pairs_per_s": round(plain, 2), "vs_plain": round(pairs_per_s / plain, 2),
+        "vs_plain_by_stage": by_stage, "plain_ncc_map_kernel": plain_detail["ncc_map_kernel"],
+        "pcie_inclusive": pcie,
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "latency_unit": f"one step = {a.batch} pairs, submit to results on the host",
         "lk_pts_per_s": round(a.batch * NPTS / (lk_us * 1e-6), 1),

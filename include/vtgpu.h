@@ -58,6 +58,9 @@ int vtg_download(vtg_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes
 /* Stream-ordered download with no wait: dst_host must be pinned (vtg_host_alloc) and stay untouched until the
  * context's work up to this call is done (vtg_ctx_sync, or a later context that waited for this one). */
 int vtg_download_async(vtg_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+/* Stream-ordered upload with no wait: src_host must be pinned (vtg_host_alloc) and stay unchanged until the
+ * context's work up to this call is done. */
+int vtg_upload_async(vtg_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
 /* Pinned (page-locked) host memory: the target of vtg_download_async and a faster source/target of the waiting
  * copies. */
 int vtg_host_alloc(vtg_ctx *ctx, size_t bytes, void **host);

@@ -242,6 +242,14 @@ int vtg_download_async(vtg_ctx *c, void *dst, const void *src, size_t bytes)
     return VTG_OK;
 }
 
+int vtg_upload_async(vtg_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c || (!dst && bytes) || (!src && bytes))
+        return fail(c, VTG_EINVAL, "vtg_upload_async: bad argument");
+    VTG_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return VTG_OK;
+}
+
 int vtg_host_alloc(vtg_ctx *c, size_t bytes, void **host)
 {
     if (!c || !host)

@@ -162,6 +162,7 @@ def lib():
         "vtg_ctx_create": [I, C.POINTER(V)], "vtg_ctx_destroy": [V], "vtg_ctx_sync": [V], "vtg_ctx_wait": [V, V],
         "vtg_alloc": [V, S, C.POINTER(V)], "vtg_free": [V, V], "vtg_upload": [V, V, V, S],
         "vtg_download": [V, V, V, S], "vtg_copy_d2d": [V, V, V, S], "vtg_download_async": [V, V, V, S],
+        "vtg_upload_async": [V, V, V, S],
         "vtg_host_alloc": [V, S, C.POINTER(V)], "vtg_host_free": [V, V],
         "vtg_timing_enable": [V, I], "vtg_timing_read": [V, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "vtg_timing_reset": [V],
@@ -233,7 +234,7 @@ def lib():
 
 # every symbol include/vtgpu.h declares
 EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", "vtg_ctx_error", "vtg_ctx_stream", "vtg_alloc",
-           "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_download_async", "vtg_host_alloc", "vtg_host_free", "vtg_timing_enable", "vtg_timing_read",
+           "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_download_async", "vtg_upload_async", "vtg_host_alloc", "vtg_host_free", "vtg_timing_enable", "vtg_timing_read",
            "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_build_ex",
            "vtg_pyr_build_many_ex", "vtg_img_gauss5", "vtg_img_norm", "vtg_img_minmax", "vtg_pyr_level",
            "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_waves", "vtg_lk_set_ppw", "vtg_pyr_lk_fused", "vtg_ctx_device_error", "vtg_ncc_template", "vtg_ncc_template_destroy",
@@ -628,6 +629,10 @@ class DevBuf:
     def put(self, arr):
         arr = np.ascontiguousarray(arr)
         self.ctx.check(self.ctx.L.vtg_upload(self.ctx.h, self.ptr, arr.ctypes.data, arr.nbytes), "upload")
+
+    def put_async(self, ctx, arr):
+        """Upload `arr` (a PinnedArray's array) on ctx's stream, no wait."""
+        ctx.check(ctx.L.vtg_upload_async(ctx.h, self.ptr, arr.ctypes.data, arr.nbytes), "upload_async")
 
     def free(self):
         if self.ptr:
