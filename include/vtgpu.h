@@ -137,7 +137,9 @@ enum { VTG_SOLVE_F64 = 0, VTG_SOLVE_MIXED = 1 };
 /* n points tracked from pyramid I to pyramid J from level `top` down (the OP_LK batch loop, src/kern/track.c:275-292):
  * xy[2i], xy[2i+1] the point in level-0 pixels, guess the same shape or NULL for zero. fb_max > 0 adds the
  * forward-backward check (lk_flowback_*). out: n records of struct lk_out (32 B). All device pointers. Window
- * 5..48 x 5..31; top < levels of both pyramids. */
+ * 5..48 x 5..31; top < levels of both pyramids. The launch orders its points in the context's LK scratch (see
+ * vtg_lk_set_order), grown on a call that needs more: a launch to be captured in a graph is made once eagerly
+ * first, at its largest size, and a captured launch keeps the scratch it was captured with. */
 int vtg_lk_track(vtg_ctx *ctx, const vtg_pyr *I, const vtg_pyr *J, int top, const struct lk_cfg *cfg, int n,
                  const float *xy, const float *guess, float fb_max, struct lk_out *out);
 /* The same with the solve mode chosen. */
@@ -171,6 +173,10 @@ int vtg_lk_set_waves(vtg_ctx *ctx, int waves);
  * windows: three points' vector lanes in three rows, the fourth row computing their second block's 5 columns), or 0
  * (default: 3 where the window allows it, else 4). Results are the same bit for bit; only time differs. */
 int vtg_lk_set_ppw(vtg_ctx *ctx, int ppw);
+/* Whether an LK launch tracks its points in spatial order (1, the default: each pair's points by 32 x 32-pixel cell of
+ * their position, so a wavefront's points are neighbours) or in the order given (0). Records land at each point's own
+ * index either way and are the same bit for bit; only time differs. */
+int vtg_lk_set_order(vtg_ctx *ctx, int on);
 
 /* ------------------------------------------------------------------ autotuner (no counterpart in the reference, whose
  * schedule is fixed at build time: one thread per row range or point, track.c:210-292) */

@@ -97,6 +97,23 @@ int scratch(vtg_ctx *c, size_t bytes, void **p)
     return VTG_OK;
 }
 
+int lk_scratch(vtg_ctx *c, size_t bytes, void **p)
+{
+    if (bytes > c->lk_buf_bytes) {
+        if (c->lk_buf) {
+            VTG_HIP(c, hipStreamSynchronize(c->stream));
+            VTG_HIP(c, hipFree(c->lk_buf));
+            c->lk_buf = nullptr;
+            c->lk_buf_bytes = 0;
+        }
+        const size_t want = (bytes + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+        VTG_HIP(c, hipMalloc(&c->lk_buf, want));
+        c->lk_buf_bytes = want;
+    }
+    *p = c->lk_buf;
+    return VTG_OK;
+}
+
 int pinned(vtg_ctx *c, size_t bytes, void **p)
 {
     if (bytes > c->pinned_bytes) {
@@ -163,6 +180,8 @@ int vtg_ctx_destroy(vtg_ctx *c)
         (void)hipFree(c->filt);
     if (c->fused)
         (void)hipFree(c->fused);
+    if (c->lk_buf)
+        (void)hipFree(c->lk_buf);
     if (c->pinned)
         (void)hipHostFree(c->pinned);
     if (c->join)

@@ -79,7 +79,17 @@ struct LkArgs {
     float fb_max;
     int fb_top;                 // the flowback's top level (-1: the pair's own)
     lk_out *out;
+    // the launch's points in spatial order (or null: in the caller's order): slot j of pair q tracks the point
+    // order[obase[q] + j], a global index into xy, guess and out
+    const int *order;
+    int obase[BMAX];
 };
+
+// The global index of the point in slot `local` of pair q.
+__device__ __forceinline__ int point_at(const LkArgs &a, int q, int local)
+{
+    return a.order ? a.order[a.obase[q] + local] : a.start[q] + local;
+}
 
 // A level element in its own units, and how many Q6 units one of them is.
 template <typename T> struct Ld;
@@ -889,7 +899,7 @@ __global__ __launch_bounds__(512) void lk_kernel(LkArgs a)
     const int q = __builtin_amdgcn_readfirstlane(pair_of(a.pre, a.npairs, s0));
     const int local = s0 - a.pre[q] + (lane >> 4);
     const bool live = local < a.cnt[q];
-    const int pt = a.start[q] + (live ? local : 0);
+    const int pt = live ? point_at(a, q, local) : 0;
     // an empty slot starts far outside every level: it fits none, reads nothing and writes nothing
     float x = -1e30f, y = -1e30f, gx = 0.0f, gy = 0.0f;
     if (live) {
@@ -1345,7 +1355,7 @@ __global__ __launch_bounds__(512) void lk3_kernel(LkArgs a)
     const Lane3 ln = lane3<WX>(lane);
     const int local = s0 - a.pre[q] + ln.p;
     const bool live = local < a.cnt[q];
-    const int pt = a.start[q] + (live ? local : 0);
+    const int pt = live ? point_at(a, q, local) : 0;
     // an empty slot starts far outside every level: it fits none, reads nothing and writes nothing
     float x = -1e30f, y = -1e30f, gx = 0.0f, gy = 0.0f;
     if (live) {
@@ -1763,6 +1773,146 @@ int check_pair(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top)
 
 Cfg to_cfg(const lk_cfg *cfg) { return Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->eps, cfg->min_eig, cfg->max_residual}; }
 
+// ---------------------------------------------------------------- a launch's points in spatial order
+// The points of each pair are tracked in the order of the 32 x 32-pixel cell that holds their level-0 position (cells
+// row by row; points of one cell in any order), so a wavefront's points are neighbours: their windows share cache lines
+// and their iteration counts are alike. A record depends on its own point alone, so the order changes time only. A
+// counting sort: clear the cells' counts, count (each point's rank in its cell from the atomic add), scan each pair's
+// counts, scatter. Every kernel is a plain launch (no memset), so the launch can be captured in a graph.
+constexpr int OCELL = 5; // log2 of the cell size in pixels
+
+struct OrderArgs {
+    int npairs, total;
+    int start[BMAX], pbase[BMAX + 1], cbase[BMAX + 1], cw[BMAX], ch[BMAX];
+    const float *xy;
+    int *cells, *rank, *order;
+};
+
+__device__ __forceinline__ int order_pair(const OrderArgs &a, int i)
+{
+    int lo = 0, hi = a.npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (i >= a.pbase[mid])
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// The cell of point g of pair q (positions outside the frame, or not numbers, go to its border cells).
+__device__ __forceinline__ int order_cell(const OrderArgs &a, int q, int g)
+{
+    const float x = fminf(fmaxf(a.xy[2 * g], 0.0f), (float)((a.cw[q] << OCELL) - 1));
+    const float y = fminf(fmaxf(a.xy[2 * g + 1], 0.0f), (float)((a.ch[q] << OCELL) - 1));
+    return a.cbase[q] + ((int)y >> OCELL) * a.cw[q] + ((int)x >> OCELL);
+}
+
+__global__ __launch_bounds__(256) void order_clear(int *cells, int n)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        cells[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void order_count(OrderArgs a)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.total)
+        return;
+    const int q = order_pair(a, i), g = a.start[q] + (i - a.pbase[q]);
+    a.rank[i] = atomicAdd(&a.cells[order_cell(a, q, g)], 1);
+}
+
+// The exclusive scan of each pair's cell counts in place: one workgroup per pair, 8 cells a thread per round.
+__global__ __launch_bounds__(256) void order_scan(OrderArgs a)
+{
+    __shared__ int part[256];
+    const int q = blockIdx.x, t = threadIdx.x;
+    int *c = a.cells + a.cbase[q];
+    const int n = a.cbase[q + 1] - a.cbase[q];
+    int carry = 0;
+    for (int b = 0; b < n; b += 256 * 8) {
+        int v[8], s = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int i = b + t * 8 + k;
+            v[k] = i < n ? c[i] : 0;
+            s += v[k];
+        }
+        part[t] = s;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int u = t >= off ? part[t - off] : 0;
+            __syncthreads();
+            part[t] += u;
+            __syncthreads();
+        }
+        int run = carry + part[t] - s;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int i = b + t * 8 + k;
+            if (i < n)
+                c[i] = run;
+            run += v[k];
+        }
+        carry += part[255];
+        __syncthreads(); // part is rewritten next round
+    }
+}
+
+__global__ __launch_bounds__(256) void order_scatter(OrderArgs a)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.total)
+        return;
+    const int q = order_pair(a, i), g = a.start[q] + (i - a.pbase[q]);
+    a.order[a.pbase[q] + a.cells[order_cell(a, q, g)] + a.rank[i]] = g;
+}
+
+// The order of launch a's points (pairs sel, level-0 sizes from I) into the context's LK scratch; sets a.order and
+// a.obase.
+int order_points(vtg_ctx *c, LkArgs &a, const std::vector<int> &sel, const vtg_pyr *const *I)
+{
+    OrderArgs o;
+    o.npairs = a.npairs;
+    int total = 0, cells = 0;
+    for (int q = 0; q < a.npairs; q++) {
+        const vtg_pyr *p = I[sel[q]];
+        o.start[q] = a.start[q];
+        o.pbase[q] = total;
+        o.cbase[q] = cells;
+        o.cw[q] = (p->w + (1 << OCELL) - 1) >> OCELL;
+        o.ch[q] = (p->h + (1 << OCELL) - 1) >> OCELL;
+        total += a.cnt[q];
+        cells += o.cw[q] * o.ch[q];
+        a.obase[q] = o.pbase[q];
+    }
+    o.pbase[a.npairs] = total;
+    o.cbase[a.npairs] = cells;
+    o.total = total;
+    o.xy = a.xy;
+    void *buf = nullptr;
+    const int rc = lk_scratch(c, ((size_t)cells + 2 * (size_t)total) * sizeof(int), &buf);
+    if (rc)
+        return rc;
+    o.cells = (int *)buf;
+    o.rank = o.cells + cells;
+    o.order = o.rank + total;
+    const int gp = (total + 255) / 256;
+    hipEvent_t ev;
+    timer_begin(c, "lk_order", &ev);
+    hipLaunchKernelGGL(order_clear, dim3(std::min((cells + 255) / 256, 1024)), dim3(256), 0, c->stream, o.cells,
+                       cells);
+    hipLaunchKernelGGL(order_count, dim3(gp), dim3(256), 0, c->stream, o);
+    hipLaunchKernelGGL(order_scan, dim3(o.npairs), dim3(256), 0, c->stream, o);
+    hipLaunchKernelGGL(order_scatter, dim3(gp), dim3(256), 0, c->stream, o);
+    timer_end(c, "lk_order", ev);
+    VTG_HIP(c, hipGetLastError());
+    a.order = o.order;
+    return VTG_OK;
+}
+
 }  // namespace
 
 // One launch for pairs sel[0 .. m) (m <= BMAX; indices into the caller's arrays) with `waves` wavefronts per workgroup.
@@ -1797,6 +1947,12 @@ static int launch_group(vtg_ctx *c, const std::vector<int> &sel, const vtg_pyr *
     a.fb_max = fb_max;
     a.fb_top = fb_top;
     a.out = out;
+    a.order = nullptr;
+    if (c->lk_order) {
+        const int rc = order_points(c, a, sel, I);
+        if (rc)
+            return rc;
+    }
     if (waves < 1 || waves > 8)
         return fail(c, VTG_EINVAL, "vtg_lk_track: %d waves per workgroup (1..8)", waves);
     // fewer wavefronts per workgroup where the shape's LDS would pass the device's per-workgroup limit (the schedule
@@ -1964,6 +2120,7 @@ int vtg_pyr_lk_fused(vtg_ctx *c, int npairs, vtg_pyr *const *I, vtg_pyr *const *
     a.lk.fb_max = fb_max;
     a.lk.fb_top = -1;
     a.lk.out = out;
+    a.lk.order = nullptr;
     a.lk.waves = 4;
     a.lk.per_xcd = 0;
     const int nctr = FCTR + FB * 2 * kMaxLevels;
@@ -1996,6 +2153,14 @@ int vtg_lk_set_waves(vtg_ctx *c, int waves)
     if (!c || waves < 1 || waves > 8)
         return fail(c, VTG_EINVAL, "vtg_lk_set_waves: %d (1..8)", waves);
     c->lk_waves = waves;
+    return VTG_OK;
+}
+
+int vtg_lk_set_order(vtg_ctx *c, int on)
+{
+    if (!c || (on != 0 && on != 1))
+        return fail(c, VTG_EINVAL, "vtg_lk_set_order: %d (0 or 1)", on);
+    c->lk_order = on;
     return VTG_OK;
 }
 

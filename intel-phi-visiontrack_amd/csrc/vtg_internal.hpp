@@ -60,6 +60,9 @@ struct vtg_ctx {
     void *filt = nullptr;                    // the contrast stretch's tables and per-frame tables (built on first use)
     void *fused = nullptr;                   // the fused launch's work counter, error word and per-level done counts
     bool fused_init = false;
+    void *lk_buf = nullptr;                  // the LK launches' point order (grown on demand, stream-ordered reuse)
+    size_t lk_buf_bytes = 0;
+    int lk_order = 1;                        // track a launch's points in spatial order (0: in the caller's order)
     hipEvent_t join = nullptr;               // for vtg_ctx_wait
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;
@@ -119,6 +122,8 @@ void timer_end(vtg_ctx *c, const char *name, hipEvent_t a);
 
 // Context scratch of at least `bytes` (device), stream-ordered reuse.
 int scratch(vtg_ctx *c, size_t bytes, void **p);
+// The LK launches' own scratch (apart from the searches', which another stream may still read).
+int lk_scratch(vtg_ctx *c, size_t bytes, void **p);
 int pinned(vtg_ctx *c, size_t bytes, void **p);
 
 PyrView view(const vtg_pyr *p);

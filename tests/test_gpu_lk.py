@@ -310,10 +310,18 @@ def test_schedules_bit_identical_and_batch(ctx, oracle):
                                     dxy, fb_max=0.5)
             results[ppw, waves] = out.get(np.zeros(offs[-1], vtgpu.LK_OUT_DTYPE))
             out.free()
+    # the points in the caller's order instead of spatial order
+    ctx.check(Lb.vtg_lk_set_order(ctx.h, 0))
+    out = ctx.lk_track_many([p[0] for p in pairs], [p[1] for p in pairs], [p[2] for p in pairs], offs, cfg, dxy,
+                            fb_max=0.5)
+    results["unordered"] = out.get(np.zeros(offs[-1], vtgpu.LK_OUT_DTYPE))
+    out.free()
+    ctx.check(Lb.vtg_lk_set_order(ctx.h, 1))
     for key in results:
         assert results[key].tobytes() == results[3, 4].tobytes(), key
     assert Lb.vtg_lk_set_waves(ctx.h, 9) == -22
     assert Lb.vtg_lk_set_ppw(ctx.h, 5) == -22
+    assert Lb.vtg_lk_set_order(ctx.h, 2) == -22
     ctx.check(Lb.vtg_lk_set_waves(ctx.h, 1))
     ctx.check(Lb.vtg_lk_set_ppw(ctx.h, 0))
     for k, (PI, PJ, top) in enumerate(pairs):
