@@ -91,6 +91,8 @@ class Batched:
     def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph, one_stream=False):
         self.v, self.B, self.graph, self.one_stream = vtgpu, batch, graph, one_stream
         self.A, self.S = vtgpu.Context(0), vtgpu.Context(0)
+        # the points are a raster grid, already in spatial order: no reordering launch
+        self.A.check(self.A.L.vtg_lk_set_order(self.A.h, 0))
         self.copy = [vtgpu.Context(0), vtgpu.Context(0)]
         self.frames, self.corners = d_frames, corners
         self.groups = len(d_frames) // batch
