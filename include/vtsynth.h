@@ -50,6 +50,28 @@ void vts_grid(int nx, int ny, int w, int h, float *xs, float *ys);
 /* n points uniform in [margin, w - 1 - margin] x [margin, h - 1 - margin], from seed. */
 void vts_points(uint64_t seed, int n, int w, int h, int margin, float *xs, float *ys);
 
+/* Layered scenes with analytic motion (the scene set of the reference's tracker tolerances, tests/scenes.md:11-39, as
+ * image-plane motions): a still background and movers, each a fractal value-noise texture (wavelengths 72 px for the
+ * background at 60 % contrast, 12-16 px for the movers) moving by a closed-form function of time (seconds), drawn back
+ * to front, so a pixel shows the topmost layer holding it; sizes are for a 640-wide frame and scale with the width.
+ *   TRANSLATE   an 160 x 128 object moving 48, 17.6 px/s over a still background
+ *   CAMERA      the camera moves (128 sin 0.8t, 48 sin 1.3t) px: every layer goes with it
+ *   SHAPES      a disc of radius 64 on a Lissajous path, and a second 80 x 64 mover
+ *   BRIGHTNESS  the object at 32 px/s under a frame gain 1 + 0.25 sin 1.3t and offset 16 sin 0.7t grey
+ *   NOISE       the object at 32 px/s, 24 sin 1.1t px, under N(0, 4^2) grey of noise
+ *   OCCLUSION   the object at 12.8 px/s, a bar as wide as half of it sweeping over its upper half at 88 px/s, nearer */
+enum { VTS_SC_TRANSLATE, VTS_SC_CAMERA, VTS_SC_SHAPES, VTS_SC_BRIGHTNESS, VTS_SC_NOISE, VTS_SC_OCCLUSION, VTS_SC_COUNT };
+/* The scene's frame at time t (w x h bytes, rows `pitch` apart): layers from `seed`, noise (NOISE only) from noise_seed.
+ * 0, or -1 for a bad argument. */
+int vts_scene_frame(int scene, double t, uint8_t *f, int w, int h, int pitch, uint64_t seed, uint64_t noise_seed,
+                    int threads);
+/* Where the point (x, y) of the frame at time t0 is at time t1: moved with the topmost layer holding it at t0.
+ * Returns that layer (0 background, 1 object, 2 second mover, 3 bar), or -1 for a bad argument. */
+int vts_scene_map(int scene, int w, int h, uint64_t seed, double t0, double x, double y, double t1, double *mx,
+                  double *my);
+/* The object's centre and half sizes at time t (a disc: the radius in both). */
+int vts_scene_object(int scene, int w, int h, double t, double *cx, double *cy, double *hx, double *hy);
+
 /* The generator's xorshift64* stream, exposed so callers draw sizes and counts from the same source. */
 uint64_t vts_rand(uint64_t *s);
 double vts_uniform(uint64_t *s); /* [0, 1) */

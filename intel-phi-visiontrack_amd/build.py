@@ -18,7 +18,8 @@ ARCH = "gfx950"
 # -packed-fp32-ops: no v_pk_*_f32 from the compiler (their results have been unreliable on these boxes).
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-I" + INC, "-I" + CSRC,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
-HIP_SOURCES = ["ctx.cpp", "pyramid.hip", "lk.hip", "ncc.hip", "filters.hip", "still.hip", "stereo.hip", "tune.cpp", "serve.cpp", "door.cpp"]
+HIP_SOURCES = ["ctx.cpp", "pyramid.hip", "lk.hip", "ncc.hip", "filters.hip", "still.hip", "stereo.hip", "tune.cpp", "serve.cpp", "door.cpp",
+               "track.cpp"]
 
 LIB_GPU = os.path.join(HERE, "libvtgpu.so")
 LIB_SYNTH = os.path.join(HERE, "libvtsynth.so")

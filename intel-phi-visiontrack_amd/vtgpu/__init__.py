@@ -269,6 +269,10 @@ def synth():
     L.vts_truth.argtypes = [C.c_void_p, C.c_int, C.c_int, D, D, C.POINTER(D), C.POINTER(D)]
     L.vts_grid.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
     L.vts_points.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.vts_scene_frame.argtypes = [C.c_int, D, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64, C.c_int]
+    L.vts_scene_map.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, D, D, D, D, C.POINTER(D), C.POINTER(D)]
+    L.vts_scene_object.argtypes = [C.c_int, C.c_int, C.c_int, D, C.POINTER(D), C.POINTER(D), C.POINTER(D),
+                                   C.POINTER(D)]
     L.vts_rand.argtypes = [C.POINTER(C.c_uint64)]
     L.vts_rand.restype = C.c_uint64
     L.vts_uniform.argtypes = [C.POINTER(C.c_uint64)]
@@ -310,6 +314,31 @@ def frame(w, h, seed, dx, dy, theta_deg=0.0, noise=2.0, noise_seed=0, threads=8)
     m = Motion(dx, dy, theta_deg)
     synth().vts_frame(f.ctypes.data, w, h, w, seed, C.byref(BLOB_SCENE), C.byref(m), noise, noise_seed, threads)
     return f
+
+
+SCENES = ["translate", "camera", "shapes", "brightness", "noise", "occlusion"]
+
+
+def scene_frame(scene, t, w, h, seed=1, noise_seed=0, threads=8):
+    """Frame of a layered scene (vts_scene_frame) at time t seconds."""
+    f = np.zeros((h, w), np.uint8)
+    rc = synth().vts_scene_frame(SCENES.index(scene), t, f.ctypes.data, w, h, w, seed, noise_seed, threads)
+    if rc:
+        raise VtgError(f"vts_scene_frame: {rc}")
+    return f
+
+
+def scene_map(scene, w, h, t0, x, y, t1, seed=1):
+    """(layer, mx, my): where the point (x, y) of the frame at t0 is at t1."""
+    mx, my = C.c_double(), C.c_double()
+    k = synth().vts_scene_map(SCENES.index(scene), w, h, seed, t0, float(x), float(y), t1, C.byref(mx), C.byref(my))
+    return k, mx.value, my.value
+
+
+def scene_object(scene, w, h, t):
+    v = [C.c_double() for _ in range(4)]
+    synth().vts_scene_object(SCENES.index(scene), w, h, t, *[C.byref(x) for x in v])
+    return tuple(x.value for x in v)
 
 
 def truth(motion, w, h, xs, ys):

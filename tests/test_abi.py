@@ -72,3 +72,26 @@ def test_grid_points():
     assert len(xs) == 16200 and xs[0] == np.float32(1920 / 360) and ys[-1] == np.float32(1080 - 6)
     px, py = vtgpu.points(3, 1000, 640, 480, 20)
     assert px.min() >= 20 and px.max() <= 619 and py.min() >= 20 and py.max() <= 459
+
+
+def test_vtg_track_exports_every_declared_symbol():
+    lib = C.CDLL(vtgpu.LIB_GPU)
+    names = _declared("vtg_track.h", "vtg_tk_")
+    assert len(names) >= 12
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_headers_compile_as_c(tmp_path):
+    """A C program includes the public headers and takes every tracker entry point's address (gcc, no link)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("no gcc")
+    names = _declared("vtg_track.h", "vtg_tk_")
+    src = tmp_path / "use.c"
+    src.write_text('#include "vtg_track.h"\n#include "vtsynth.h"\nvoid *table[] = {' +
+                   ", ".join(f"(void *){n}" for n in names) + "};\n")
+    r = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-c", "-o", str(tmp_path / "use.o"),
+                        "-I" + os.path.join(REPO, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
