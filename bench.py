@@ -674,9 +674,13 @@ def cpu_baseline(f0s, f1s, corners, xs, ys, budget_s, gpu_out):
     dref = np.maximum(np.abs(g["x"] - ref["x"]), np.abs(g["y"] - ref["y"]))[ok_ref]
     # storage precision against the FP64 oracle (pyramids fp16 / bf16 / exact, fp64 or mixed solve)
     import vtgpu
-    prec = precision_eval(vtgpu, o, vtgpu.Context(0), threads=cores)
-    log("precision: " + ", ".join(f"{k} p99 {v['textured']['p99_px']:.2e} px" for k, v in prec.items()
-                                  if isinstance(v, dict)))
+    pctx = vtgpu.Context(0)
+    prec = precision_eval(vtgpu, o, pctx, threads=cores)
+    # the Scharr gradient variant (no reference counterpart) against the oracle's FP64 Scharr tier, per storage
+    prec["scharr_variant"] = precision_eval(vtgpu, o, pctx, threads=cores, grad=1)
+    for tag, pr in (("sobel", prec), ("scharr", prec["scharr_variant"])):
+        log(f"precision ({tag}): " + ", ".join(f"{k} p99 {v['textured']['p99_px']:.2e} px" for k, v in pr.items()
+                                               if isinstance(v, dict) and "textured" in v))
     return ({"value": round(1.0 / med, 4), "unit": "frame-pairs/s", "cores": cores, "kind": "port",
              "sample": sample, "host": host_cpu(), "one_thread": {"value": round(1.0 / one, 5), "unit": "frame-pairs/s",
                                                                   "cores": 1},

@@ -177,6 +177,13 @@ int vtg_lk_set_ppw(vtg_ctx *ctx, int ppw);
  * their position, so a wavefront's points are neighbours) or in the order given (0). Records land at each point's own
  * index either way and are the same bit for bit; only time differs. */
 int vtg_lk_set_order(vtg_ctx *ctx, int on);
+/* The template's gradient operator for the context's later LK launches. VTG_GRAD_SOBEL (the default) is the
+ * reference's: [1 2 1] smoothing across, [-1 0 1] along, / 8 per grey level (lk_ref.c:30-31, lk_vx.c:60-75), and only
+ * it reproduces the reference's records. VTG_GRAD_SCHARR is a labelled variant with no counterpart in the reference:
+ * [3 10 3] across, [-1 0 1] along, / 32, computed as fmaf(centre difference, 10, 3 (sum of the outer differences))
+ * / 2048 in Q6 units. Everything else (window, sums, driver, statuses) is unchanged. */
+enum { VTG_GRAD_SOBEL = 0, VTG_GRAD_SCHARR = 1 };
+int vtg_lk_set_gradient(vtg_ctx *ctx, int grad);
 
 /* ------------------------------------------------------------------ autotuner (no counterpart in the reference, whose
  * schedule is fixed at build time: one thread per row range or point, track.c:210-292) */

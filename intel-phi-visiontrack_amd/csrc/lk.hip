@@ -8,7 +8,8 @@
 // the blocks in order (lk_vx.c:82-94, 119-126, 160-168), and the 16 lane sums meet in the reference's butterfly
 // (partners i^1, i^2, i^4, i^8; vx_hsum, src/common/vx.h:213-221) through four DPP adds inside the row. Every fp32
 // value of the reference's patch arithmetic is produced by the same operations on the same inputs: the bilinear
-// window S (x-interpolation per image row, y-interpolation between rows), the Sobel gradients / 512, It = T - J, the
+// window S (x-interpolation per image row, y-interpolation between rows), the Sobel gradients / 512 (a context may
+// choose the Scharr variant instead, vtg_lk_set_gradient, which has no reference counterpart), It = T - J, the
 // residual's |It| / 64 and (It / 64)^2. So the structure tensor, the mismatch vector and the residual sums equal the
 // reference's bit for bit. Lanes past the window's width (the last block of a window narrower than a whole block) get
 // zero gradients as in the reference (lk_vx.c:82-85) and read a column inside the window, so every value they touch is
@@ -58,7 +59,20 @@ namespace {
 struct Cfg {
     int w, h, max_iter;
     float eps, min_eig, max_res;
+    int grad; // VTG_GRAD_SOBEL or VTG_GRAD_SCHARR
 };
+
+// The gradient operator's weights: outer and centre rows of the smoothing across, and the factor after the sum
+// (per stored unit: the storage scale times 1/512 for Sobel's [1 2 1] / 8, 1/2048 for Scharr's [3 10 3] / 32, both per
+// Q6 unit). Sobel's outer weight 1 makes its product exact, so its values are the reference's bit for bit.
+struct GradW {
+    float we, wc, k;
+};
+__device__ __forceinline__ GradW grad_w(int grad, float kg)
+{
+    const bool sch = grad == VTG_GRAD_SCHARR;
+    return GradW{sch ? 3.0f : 1.0f, sch ? 10.0f : 2.0f, sch ? kg * 0.25f : kg};
+}
 
 constexpr int BMAX = 64; // pairs per launch (inline in the kernel arguments)
 constexpr int NXCD = 8;  // XCDs of the device; workgroup i runs on XCD i mod 8
@@ -343,8 +357,8 @@ __device__ __forceinline__ void wave_sync()
 
 // The template at (px, py) (build_vx, lk_vx.c:36-97): the bilinear window S over rows -1 .. h and, per lane and block,
 // columns c - 1, c, c + 1 of its window column c (four taps of each image row, x-interpolated, then y-interpolated
-// with the row above), the Sobel gradients / 512 and T = S's middle; returns G = the butterfly totals of Tx^2,
-// Tx Ty, Ty^2 accumulated over rows, then blocks.
+// with the row above), the Sobel gradients / 512 (or the Scharr variant's / 2048) and T = S's middle; returns G = the
+// butterfly totals of Tx^2, Tx Ty, Ty^2 accumulated over rows, then blocks.
 template <int NV, int HMAX, int WX, typename T>
 __device__ __forceinline__ void build(const Level &lv, double px, double py, const Cfg &c, const Lanes<NV> &ln,
                                       Tmpl<NV, HMAX, WX> &tp, float *G)
@@ -353,6 +367,7 @@ __device__ __forceinline__ void build(const Level &lv, double px, double py, con
     constexpr bool EXACT = SH::EXACT;
     constexpr float ks = Ld<T>::scale, kg = ks * (1.0f / 512.0f);
     constexpr int K = NV == 1 ? 8 : 4; // image rows whose loads are in flight ahead of their use
+    const GradW gw = grad_w(c.grad, kg);
     const double flx = floor(px), fly = floor(py);
     const float fx = (float)(px - flx), fy = (float)(py - fly);
     unsigned pitch;
@@ -404,9 +419,9 @@ __device__ __forceinline__ void build(const Level &lv, double px, double py, con
                 } else {
                     // window row rr = r - 2 from S rows rr (S0), rr + 1 (S1), rr + 2 (S)
                     const int rr = r - 2;
-                    float tx = fmaf(d1[k], 2.0f, d0[k] + dS) * kg;
+                    float tx = fmaf(d1[k], gw.wc, (d0[k] + dS) * gw.we) * gw.k;
                     const float vl = S[0] - S0[k][0], vc = S[1] - S0[k][1], vr = S[2] - S0[k][2];
-                    float ty = fmaf(vc, 2.0f, vl + vr) * kg;
+                    float ty = fmaf(vc, gw.wc, (vl + vr) * gw.we) * gw.k;
                     const bool on = SH::full(k) || ln.on[k];
                     tx = on ? tx : 0.0f;
                     ty = on ? ty : 0.0f;
@@ -1004,11 +1019,12 @@ template <int HMAX> struct Tmpl3 {
 
 // build_vx (lk_vx.c:36-97) in this layout: the same per-column arithmetic as build, one column per lane.
 template <int HMAX, int WX, typename T>
-__device__ __forceinline__ void build3(const Level &lv, double px, double py, const Lane3 &ln, Tmpl3<HMAX> &tp,
-                                       float *G)
+__device__ __forceinline__ void build3(const Level &lv, double px, double py, int grad, const Lane3 &ln,
+                                       Tmpl3<HMAX> &tp, float *G)
 {
     constexpr float ks = Ld<T>::scale, kg = ks * (1.0f / 512.0f);
     constexpr int K = 4; // image rows whose loads are in flight ahead of their use
+    const GradW gw = grad_w(grad, kg);
     const double flx = floor(px), fly = floor(py);
     const float fx = (float)(px - flx), fy = (float)(py - fly);
     unsigned pitch;
@@ -1048,9 +1064,9 @@ __device__ __forceinline__ void build3(const Level &lv, double px, double py, co
             } else {
                 // window row rr = r - 2 from S rows rr (S0), rr + 1 (S1), rr + 2 (S)
                 const int rr = r - 2;
-                const float tx = fmaf(d1, 2.0f, d0 + dS) * kg;
+                const float tx = fmaf(d1, gw.wc, (d0 + dS) * gw.we) * gw.k;
                 const float vl = S[0] - S0[0], vc = S[1] - S0[1], vr = S[2] - S0[2];
-                const float ty = fmaf(vc, 2.0f, vl + vr) * kg;
+                const float ty = fmaf(vc, gw.wc, (vl + vr) * gw.we) * gw.k;
                 tp.t[rr] = S1[1] * ks;
                 tp.tx[rr] = tx;
                 tp.ty[rr] = ty;
@@ -1247,9 +1263,9 @@ __device__ __forceinline__ void track3(const PyrView &I, const PyrView &J, int t
         }
         float G[3];
         if (std::is_same<T0, T1>::value || L > 0)
-            build3<HMAX, WX, T1>(li, px, py, ln, tp, G);
+            build3<HMAX, WX, T1>(li, px, py, c.grad, ln, tp, G);
         else
-            build3<HMAX, WX, T0>(li, px, py, ln, tp, G);
+            build3<HMAX, WX, T0>(li, px, py, c.grad, ln, tp, G);
         const double gxx = G[0], gxy = G[1], gyy = G[2];
         const Solve s = solve2(gxx, gxy, gyy, area, mixed);
         if (L == 0)
@@ -1771,7 +1787,10 @@ int check_pair(vtg_ctx *c, const vtg_pyr *I, const vtg_pyr *J, int top)
     return VTG_OK;
 }
 
-Cfg to_cfg(const lk_cfg *cfg) { return Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->eps, cfg->min_eig, cfg->max_residual}; }
+Cfg to_cfg(const vtg_ctx *c, const lk_cfg *cfg)
+{
+    return Cfg{cfg->w, cfg->h, cfg->max_iter, cfg->eps, cfg->min_eig, cfg->max_residual, c->lk_grad};
+}
 
 // ---------------------------------------------------------------- a launch's points in spatial order
 // The points of each pair are tracked in the order of the 32 x 32-pixel cell that holds their level-0 position (cells
@@ -1940,7 +1959,7 @@ static int launch_group(vtg_ctx *c, const std::vector<int> &sel, const vtg_pyr *
     a.slots = acc;
     if (a.slots <= 0)
         return VTG_OK;
-    a.cfg = to_cfg(cfg);
+    a.cfg = to_cfg(c, cfg);
     a.solve = solve_mode;
     a.xy = xy;
     a.guess = guess;
@@ -2113,7 +2132,7 @@ int vtg_pyr_lk_fused(vtg_ctx *c, int npairs, vtg_pyr *const *I, vtg_pyr *const *
     a.lk.slots = acc;
     if (acc > 0 && (!xy || !out))
         return fail(c, VTG_EINVAL, "vtg_pyr_lk_fused: null points or output");
-    a.lk.cfg = to_cfg(cfg);
+    a.lk.cfg = to_cfg(c, cfg);
     a.lk.solve = solve_mode;
     a.lk.xy = xy;
     a.lk.guess = guess;
@@ -2161,6 +2180,14 @@ int vtg_lk_set_order(vtg_ctx *c, int on)
     if (!c || (on != 0 && on != 1))
         return fail(c, VTG_EINVAL, "vtg_lk_set_order: %d (0 or 1)", on);
     c->lk_order = on;
+    return VTG_OK;
+}
+
+int vtg_lk_set_gradient(vtg_ctx *c, int grad)
+{
+    if (!c || (grad != VTG_GRAD_SOBEL && grad != VTG_GRAD_SCHARR))
+        return fail(c, VTG_EINVAL, "vtg_lk_set_gradient: %d (VTG_GRAD_SOBEL or VTG_GRAD_SCHARR)", grad);
+    c->lk_grad = grad;
     return VTG_OK;
 }
 

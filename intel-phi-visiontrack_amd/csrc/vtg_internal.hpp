@@ -63,6 +63,7 @@ struct vtg_ctx {
     void *lk_buf = nullptr;                  // the LK launches' point order (grown on demand, stream-ordered reuse)
     size_t lk_buf_bytes = 0;
     int lk_order = 1;                        // track a launch's points in spatial order (0: in the caller's order)
+    int lk_grad = 0;                         // VTG_GRAD_SOBEL or VTG_GRAD_SCHARR
     hipEvent_t join = nullptr;               // for vtg_ctx_wait
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;

@@ -18,6 +18,7 @@ PAD_X, PAD_Y, Q6 = 32, 4, 64
 STORE_Q6, STORE_F32, STORE_F16, STORE_BF16, STORE_Q6W, STORE_Q6F, STORE_F32_U8 = 0, 1, 2, 3, 4, 5, 6
 BUILD_CONTRAST, BUILD_PRE_GAUSS = 1, 2
 SOLVE_F64, SOLVE_MIXED = 0, 1
+GRAD_SOBEL, GRAD_SCHARR = 0, 1
 LK_OK, LK_OUT, LK_LOWEIG, LK_RESIDUAL, LK_FLOWBACK = 0, 1, 2, 4, 8
 
 LK_OUT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("min_eig", "<f4"), ("residual", "<f4"),
@@ -177,7 +178,7 @@ def lib():
         "vtg_lk_track": [V, V, V, I, C.POINTER(LkCfg), I, V, V, F, V],
         "vtg_lk_track_ex": [V, V, V, I, C.POINTER(LkCfg), I, V, V, F, I, V],
         "vtg_lk_track_many": [V, I, V, V, V, V, C.POINTER(LkCfg), V, V, F, I, V], "vtg_lk_set_waves": [V, I],
-        "vtg_lk_set_ppw": [V, I], "vtg_lk_set_order": [V, I],
+        "vtg_lk_set_ppw": [V, I], "vtg_lk_set_order": [V, I], "vtg_lk_set_gradient": [V, I],
         "vtg_pyr_lk_fused": [V, I, V, V, V, V, V, V, V, V, C.POINTER(LkCfg), V, V, F, I, V],
         "vtg_ctx_device_error": [V, C.POINTER(I)],
         "vtg_ncc_template": [V, V, I, I, I, I, I, C.POINTER(V)], "vtg_ncc_template_destroy": [V, V],
@@ -237,7 +238,7 @@ EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", 
            "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_download_async", "vtg_upload_async", "vtg_host_alloc", "vtg_host_free", "vtg_timing_enable", "vtg_timing_read",
            "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_build_ex",
            "vtg_pyr_build_many_ex", "vtg_img_gauss5", "vtg_img_norm", "vtg_img_minmax", "vtg_pyr_level",
-           "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_waves", "vtg_lk_set_ppw", "vtg_lk_set_order", "vtg_pyr_lk_fused", "vtg_ctx_device_error", "vtg_ncc_template", "vtg_ncc_template_destroy",
+           "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_waves", "vtg_lk_set_ppw", "vtg_lk_set_order", "vtg_lk_set_gradient", "vtg_pyr_lk_fused", "vtg_ctx_device_error", "vtg_ncc_template", "vtg_ncc_template_destroy",
            "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_template_update_many", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_fixed", "vtg_ncc_search_many", "vtg_ncc_search_many_split",
            "vtg_tune_pair", "vtg_tune_get", "vtg_tune_reset", "vtg_srv_create", "vtg_srv_destroy", "vtg_srv_error",
            "vtg_srv_open", "vtg_srv_close", "vtg_srv_submit", "vtg_srv_poll", "vtg_srv_wait", "vtg_srv_get_stats",

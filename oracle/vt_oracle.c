@@ -11,6 +11,10 @@
  *   LK f64 tier                  agreement with the lane tier within 1e-3 px on converged points, and exact
  *                                translation truth on the m2 inputs (|d| <= 14.4 px: EPE within the reference's
  *                                measured 0.0209 px p95).
+ *   LK Scharr tiers              the closed form of G's smaller eigenvalue on the plane x y^2 + x^2 y (whose exact
+ *                                gradients differ from both operators' by their smoothing's second moment: 1/2 for
+ *                                Sobel, 3/8 for Scharr), the lane64 and f64 Scharr tiers within 1e-3 px of each
+ *                                other, and exact translation truth on the m2 inputs.
  *   NCC                          golden vectors from oracle/_ref's own tk_correlate (32 x 21, tests/golden/ncc_ref.json),
  *                                the invariants of src/host/vt_ncc_check.c, closed forms (affine copies score +-1),
  *                                and a textbook double ZNCC by brute force on small windows.
@@ -348,11 +352,13 @@ static long floor_ld(long double v)
 }
 
 /* The template window at (px, py): bilinear S over rows -1..H and 16-lane blocks from one block left of the window
- * to one right of it; Sobel / 512 on S; G summed per lane in row-then-block order (lk_vx.c:36-97). */
-static void lane_build(const struct vt_plane *pl, long double px, long double py, const struct lk_cfg *c,
+ * to one right of it; Sobel / 512 on S (or the Scharr variant, / 2048); G summed per lane in row-then-block order
+ * (lk_vx.c:36-97). Sobel's outer weight is 1, so its product is exact and the arithmetic is the reference's. */
+static void lane_build(const struct vt_plane *pl, long double px, long double py, const struct lk_cfg *c, int scharr,
                        struct lane_tile *t, float *g)
 {
-    static const float two = 2.0f, gscale = 1.0f / 512.0f;
+    const float we = scharr ? 3.0f : 1.0f, wc = scharr ? 10.0f : 2.0f;
+    const float gscale = scharr ? 1.0f / 2048.0f : 1.0f / 512.0f;
     const long ix = floor_ld(px), iy = floor_ld(py);
     const float fx = (float)(px - (long double)ix), fy = (float)(py - (long double)iy);
     const int W = c->w, H = c->h, nv = (W + 15) / 16, ncol = 16 * (nv + 2);
@@ -379,9 +385,9 @@ static void lane_build(const struct vt_plane *pl, long double px, long double py
             for (int j = 0; j < NL; j++) {
                 const int q = 16 * (k + 1) + j;
                 const float du = u[q + 1] - u[q - 1], dm = m[q + 1] - m[q - 1], dd = d[q + 1] - d[q - 1];
-                float tx = fmaf(dm, two, du + dd) * gscale;
+                float tx = fmaf(dm, wc, (du + dd) * we) * gscale;
                 const float vl = d[q - 1] - u[q - 1], vc = d[q] - u[q], vr = d[q + 1] - u[q + 1];
-                float ty = fmaf(vc, two, vl + vr) * gscale;
+                float ty = fmaf(vc, wc, (vl + vr) * we) * gscale;
                 if (k == nv - 1 && tail && j >= tail)
                     tx = ty = 0.0f;
                 t->t[r][16 * k + j] = m[q];
@@ -462,20 +468,31 @@ static void lane_residual(const struct vt_plane *pl, long double qx, long double
 #define REAL long double
 #define SQRT sqrtl
 #define LANE_NAME lane_track
+#define LANE_SCHARR 0
 #include "lane_track.inc"
 #undef REAL
 #undef SQRT
 #undef LANE_NAME
+#undef LANE_SCHARR
 
 /* The same walk with an fp64 driver: the patch arithmetic (and so every fp32 sum) is the lane tier's; only the
  * driver's steps round to 53 instead of 64 bits. */
 #define REAL double
 #define SQRT sqrt
 #define LANE_NAME lane64_track
+#define LANE_SCHARR 0
+#include "lane_track.inc"
+#undef LANE_NAME
+#undef LANE_SCHARR
+
+/* The fp64-driver walk with the Scharr gradient variant (no counterpart in the reference). */
+#define LANE_NAME lane64_scharr_track
+#define LANE_SCHARR 1
 #include "lane_track.inc"
 #undef REAL
 #undef SQRT
 #undef LANE_NAME
+#undef LANE_SCHARR
 
 /* ------------------------------------------------------------------ LK, f64 tier */
 
@@ -491,9 +508,11 @@ static inline double bilin(const struct vt_plane *pl, long x, long y, double fx,
     return top + fy * (bot - top);
 }
 
-static void f64_build(const struct vt_plane *pl, double px, double py, const struct lk_cfg *c, struct f64_tile *t,
-                      double *g)
+static void f64_build(const struct vt_plane *pl, double px, double py, const struct lk_cfg *c, int scharr,
+                      struct f64_tile *t, double *g)
 {
+    /* Sobel [1 2 1] / 8 or Scharr [3 10 3] / 32 across, [-1 0 1] along, per Q6 unit (/ 64) */
+    const double we = scharr ? 3.0 : 1.0, wc = scharr ? 10.0 : 2.0, sc = scharr ? 2048.0 : 512.0;
     const double fl_x = floor(px), fl_y = floor(py);
     const long ix = (long)fl_x, iy = (long)fl_y;
     const double fx = px - fl_x, fy = py - fl_y;
@@ -507,8 +526,8 @@ static void f64_build(const struct vt_plane *pl, double px, double py, const str
         for (int k = 0; k < W; k++) {
             const double *u = s[r], *m = s[r + 1], *d = s[r + 2];
             const int i = k + 1;
-            const double dx = ((u[i + 1] - u[i - 1]) + 2.0 * (m[i + 1] - m[i - 1]) + (d[i + 1] - d[i - 1])) / 512.0;
-            const double dy = ((d[i - 1] - u[i - 1]) + 2.0 * (d[i] - u[i]) + (d[i + 1] - u[i + 1])) / 512.0;
+            const double dx = (we * (u[i + 1] - u[i - 1]) + wc * (m[i + 1] - m[i - 1]) + we * (d[i + 1] - d[i - 1])) / sc;
+            const double dy = (we * (d[i - 1] - u[i - 1]) + wc * (d[i] - u[i]) + we * (d[i + 1] - u[i + 1])) / sc;
             t->t[r][k] = m[i];
             t->tx[r][k] = dx;
             t->ty[r][k] = dy;
@@ -541,8 +560,8 @@ static void f64_iterate(const struct vt_plane *pl, double qx, double qy, const s
     }
 }
 
-static void f64_track(const struct vt_plane *I, const struct vt_plane *J, int top, const struct lk_cfg *c, float x,
-                      float y, float gx0, float gy0, struct lk_out *o)
+static void f64_track(const struct vt_plane *I, const struct vt_plane *J, int top, const struct lk_cfg *c, int scharr,
+                      float x, float y, float gx0, float gy0, struct lk_out *o)
 {
     struct f64_tile tile;
     double gx = (double)gx0 * ldexp(1.0, -top), gy = (double)gy0 * ldexp(1.0, -top);
@@ -561,7 +580,7 @@ static void f64_track(const struct vt_plane *I, const struct vt_plane *J, int to
             continue;
         }
         double g[3];
-        f64_build(&I[L], px, py, c, &tile, g);
+        f64_build(&I[L], px, py, c, scharr, &tile, g);
         const double det = g[0] * g[2] - g[1] * g[1], tr = g[0] + g[2], df = g[0] - g[2];
         const double eig = (tr - sqrt(df * df + 4 * g[1] * g[1])) * 0.5 / area;
         if (L == 0)
@@ -632,10 +651,12 @@ static void f64_track(const struct vt_plane *I, const struct vt_plane *J, int to
 void vo_lk_track(int tier, const struct vt_plane *I, const struct vt_plane *J, int top, const struct lk_cfg *c,
                  float x, float y, float gx, float gy, struct lk_out *o)
 {
-    if (tier == VO_F64)
-        f64_track(I, J, top, c, x, y, gx, gy, o);
+    if (tier == VO_F64 || tier == VO_F64_SCHARR)
+        f64_track(I, J, top, c, tier == VO_F64_SCHARR, x, y, gx, gy, o);
     else if (tier == VO_LANE64)
         lane64_track(I, J, top, c, x, y, gx, gy, o);
+    else if (tier == VO_LANE64_SCHARR)
+        lane64_scharr_track(I, J, top, c, x, y, gx, gy, o);
     else
         lane_track(I, J, top, c, x, y, gx, gy, o);
 }

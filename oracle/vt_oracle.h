@@ -57,7 +57,10 @@ void vo_pyr_build(struct vt_plane *lv, const uint8_t *img, int stride, int level
 void vo_pyr_build_mt(struct vt_plane *lv, const uint8_t *img, int stride, int levels, int threads);
 
 /* --- LK (src/kern/lk_driver.inc, lk_vx.c, lk_ref.c) --- */
-enum { VO_LANE = 0, VO_F64 = 1, VO_LANE64 = 2 };
+/* VO_LANE64_SCHARR and VO_F64_SCHARR: the tiers VO_LANE64 and VO_F64 with the Scharr gradient variant in place of
+ * the reference's Sobel (no counterpart in the reference; the GPU's VTG_GRAD_SCHARR): [3 10 3] across, [-1 0 1] along,
+ * / 32 per grey level, fp32 as fmaf(centre difference, 10, 3 (outer differences' sum)) / 2048 in Q6 units. */
+enum { VO_LANE = 0, VO_F64 = 1, VO_LANE64 = 2, VO_LANE64_SCHARR = 3, VO_F64_SCHARR = 4 };
 void vo_lk_track(int tier, const struct vt_plane *I, const struct vt_plane *J, int top, const struct lk_cfg *c,
                  float x, float y, float gx, float gy, struct lk_out *o);
 void vo_lk_flowback(int tier, const struct vt_plane *I, const struct vt_plane *J, int top, const struct lk_cfg *c,

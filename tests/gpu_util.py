@@ -18,18 +18,19 @@ def flow_diff(a, b):
 NUDGES = [(1e-3, 0.0), (-1e-3, 0.0), (0.0, 1e-3), (0.0, -1e-3)]
 
 
-def conditioning(oracle, I, J, top, cfg, xs, ys, gx=None, gy=None, fb=0.0, tol=0.005, threads=16):
-    """(well, f64): the FP64 oracle's result, and which points are well-conditioned: moving the start point by
-    1e-3 px along either axis, either way, keeps the status and moves the displacement by at most `tol` px
-    (amplification <= 5). On the other points the answer depends on rounding far below the 0.01 px parity band
-    (a window barely above min_eig, an iteration that runs to max_iter): no arithmetic pins them."""
+def conditioning(oracle, I, J, top, cfg, xs, ys, gx=None, gy=None, fb=0.0, tol=0.005, threads=16, tier=1):
+    """(well, f64): the FP64 oracle's result (tier 1, or 4 for the Scharr gradient variant), and which points are
+    well-conditioned: moving the start point by 1e-3 px along either axis, either way, keeps the status and moves the
+    displacement by at most `tol` px (amplification <= 5). On the other points the answer depends on rounding far
+    below the 0.01 px parity band (a window barely above min_eig, an iteration that runs to max_iter): no arithmetic
+    pins them."""
     xs = np.asarray(xs, np.float32)
     ys = np.asarray(ys, np.float32)
-    base = oracle.lk_batch(1, I, J, top, cfg, xs, ys, gx, gy, fb, threads)
+    base = oracle.lk_batch(tier, I, J, top, cfg, xs, ys, gx, gy, fb, threads)
     well = np.ones(len(xs), bool)
     for dx, dy in NUDGES:
         px, py = xs + np.float32(dx), ys + np.float32(dy)
-        p = oracle.lk_batch(1, I, J, top, cfg, px, py, gx, gy, fb, threads)
+        p = oracle.lk_batch(tier, I, J, top, cfg, px, py, gx, gy, fb, threads)
         move = np.maximum(np.abs((p["x"].astype(np.float64) - px) - (base["x"].astype(np.float64) - xs)),
                           np.abs((p["y"].astype(np.float64) - py) - (base["y"].astype(np.float64) - ys)))
         well &= (p["status"] == base["status"]) & ((base["status"] != 0) | (move <= tol))
@@ -70,11 +71,13 @@ def cov_condition(out):
     return cond
 
 
-def precision_eval(vtgpu, oracle, ctx, w=1920, h=1080, levels=4, nx=180, ny=90, win=21, seed=77, threads=16):
+def precision_eval(vtgpu, oracle, ctx, w=1920, h=1080, levels=4, nx=180, ny=90, win=21, seed=77, threads=16,
+                   grad=0):
     """Pyramid storage x solve mode against the FP64 oracle on one frame pair with about 10% of the grid in near-flat
     patches (texture amplitude 3 grey). Per combination: statuses equal to the oracle's, and the flow error over the
     converged, well-conditioned points, textured and flat apart, plus the share of converged points whose G has a
-    condition number above 1e4 (where the mixed solve goes to fp64)."""
+    condition number above 1e4 (where the mixed solve goes to fp64). grad: the gradient operator (0 Sobel, 1 the
+    Scharr variant, against the oracle's FP64 Scharr tier); the context is left on Sobel."""
     rects = flat_rects(w, h, seed)
     f0, f1, _ = vtgpu.frame_pair(w, h, seed, flat_amp=3.0, flat=np.array(rects).ravel())
     xs, ys = vtgpu.grid(nx, ny, w, h)
@@ -82,9 +85,10 @@ def precision_eval(vtgpu, oracle, ctx, w=1920, h=1080, levels=4, nx=180, ny=90, 
     k0, I = oracle.pyramid(f0, w, h, levels, threads=threads)
     k1, J = oracle.pyramid(f1, w, h, levels, threads=threads)
     cfg = vtgpu.LkCfg(win, win, 10, 0.01, 1e-4, 40.0, 0)
-    well, f64 = conditioning(oracle, I, J, levels - 1, cfg, xs, ys, threads=threads)
+    well, f64 = conditioning(oracle, I, J, levels - 1, cfg, xs, ys, threads=threads, tier=4 if grad else 1)
     res = {"points": int(len(xs)), "flat_points": int(flat.sum()), "well_conditioned": int(well.sum()),
-           "oracle_converged": int((f64["status"] == 0).sum())}
+           "oracle_converged": int((f64["status"] == 0).sum()), "gradient": "scharr" if grad else "sobel"}
+    ctx.check(vtgpu.lib().vtg_lk_set_gradient(ctx.h, grad))
     names = {vtgpu.STORE_Q6: "q6", vtgpu.STORE_F32: "f32", vtgpu.STORE_F16: "f16", vtgpu.STORE_BF16: "bf16"}
     for st, sname in names.items():
         p0, p1 = ctx.pyramid(w, h, levels, st), ctx.pyramid(w, h, levels, st)
@@ -106,6 +110,7 @@ def precision_eval(vtgpu, oracle, ctx, w=1920, h=1080, levels=4, nx=180, ny=90, 
             res[f"{sname}/{vname}"] = r
         p0.destroy()
         p1.destroy()
+    ctx.check(vtgpu.lib().vtg_lk_set_gradient(ctx.h, 0))
     return res
 
 

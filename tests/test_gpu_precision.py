@@ -1,7 +1,8 @@
 """Pyramid storage precision against the FP64 oracle: 1080p, 16,200 grid points, about 10% of them in near-flat patches
 (3 grey of texture). Exact storage (Q6 u8/u16, fp32) must reproduce the oracle to the parity band; fp16 and bf16
 storage quantize levels 1 and up and are held to the bounds they meet with margin, on the well-conditioned textured
-points. The mixed solve (fp32, fp64 above condition number 1e4) stays within the same bounds as the fp64 solve."""
+points. The mixed solve (fp32, fp64 above condition number 1e4) stays within the same bounds as the fp64 solve. The Scharr
+gradient variant (no reference counterpart) is held to the same bounds against the oracle's FP64 Scharr tier."""
 import pytest
 
 import vtgpu
@@ -10,10 +11,10 @@ from gpu_util import precision_eval, say
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def prec(ctx, oracle):
-    say("precision eval 1080p")
-    return precision_eval(vtgpu, oracle, ctx)
+@pytest.fixture(scope="module", params=["sobel", "scharr"])
+def prec(ctx, oracle, request):
+    say("precision eval 1080p", request.param)
+    return precision_eval(vtgpu, oracle, ctx, grad=1 if request.param == "scharr" else 0)
 
 
 def test_workload_shape(prec):

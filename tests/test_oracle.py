@@ -125,6 +125,65 @@ def test_m2_f64_accuracy(oracle):
         assert np.max(np.abs(lane["x"] - out["x"])[both]) < 1e-3
 
 
+def _eig_closed_form(px, py, cx, cy, half, m2):
+    """G's smaller eigenvalue per window pixel for the plane f = X Y^2 + 3 X^2 Y (X = x - cx, Y = y - cy) at integer
+    (px, py): a 3 x 3 operator with [-1 0 1] / 2 along and smoothing weights (sum 1, symmetric, second moment m2)
+    across gives exactly d/dx = Y^2 + m2 + 6XY and d/dy = 2XY + 3 (X^2 + m2)."""
+    X = np.arange(px - half, px + half + 1, dtype=np.float64)[None, :] - cx
+    Y = np.arange(py - half, py + half + 1, dtype=np.float64)[:, None] - cy
+    dx = Y * Y + m2 + 6 * X * Y
+    dy = 2 * X * Y + 3 * (X * X + m2)
+    a, b, c = (dx * dx).sum(), (dx * dy).sum(), (dy * dy).sum()
+    return (a + c - np.sqrt((a - c) ** 2 + 4 * b * b)) / 2 / dx.size
+
+
+def test_gradient_operators_closed_form(oracle):
+    """Both gradient operators of every LK tier against the closed form on a polynomial plane: Sobel's smoothing
+    [1 2 1] / 4 has second moment 1/2, Scharr's [3 10 3] / 16 has 3/8; a wrong weight, scale or axis moves the
+    eigenvalue far outside the tolerance (checked: the two operators' closed forms differ by far more)."""
+    n, cx, cy = 64, 32.0, 32.0
+    hp = HostPlane(n, n)
+    x = np.arange(n, dtype=np.float64)[None, :] - cx
+    y = np.arange(n, dtype=np.float64)[:, None] - cy
+    hp.view()[:] = (64.0 * (x * y * y + 3 * x * x * y)).astype(np.float32)  # Q6, exact integers below 2^24
+    arr = (type(hp.c) * 1)(hp.c)
+    cfg = lk_cfg(21, 21, 10, 0.01, 1e-4, 1e9)
+    pts = [(30, 34), (35, 29), (32, 32), (27, 38)]
+    xs = np.array([p[0] for p in pts], np.float32)
+    ys = np.array([p[1] for p in pts], np.float32)
+    for tier, m2, rtol in ((0, 0.5, 2e-5), (1, 0.5, 1e-6), (2, 0.5, 2e-5), (3, 0.375, 2e-5), (4, 0.375, 1e-6)):
+        out = oracle.lk_batch(tier, arr, arr, 0, cfg, xs, ys, threads=1)
+        for i, (px, py) in enumerate(pts):
+            want = _eig_closed_form(px, py, cx, cy, 10, m2)
+            other = _eig_closed_form(px, py, cx, cy, 10, 0.875 - m2)
+            assert abs(other - want) > 20 * rtol * want, (px, py)
+            assert abs(float(out["min_eig"][i]) - want) <= rtol * want, (tier, px, py, out["min_eig"][i], want)
+
+
+def test_m2_scharr_accuracy(oracle):
+    """The Scharr variant's tiers (no reference counterpart) on the m2 inputs, whose displacement is exact: the same
+    EPE band as the Sobel tiers within 5 % (median <= 0.012, p95 <= 0.022 px; measured 0.0114 and 0.02094 at the
+    worst displacement, Sobel's 0.0114 and 0.0209), the fp32 lane arithmetic within 1e-3 px of fp64, and different
+    answers from Sobel."""
+    g, w, h, cfg, xs, ys = _m2_setup(oracle)
+    off = g["offset"]
+    keep0, I = oracle.pyramid(oracle.m2_render(w, h, off, off), w, h, g["levels"])
+    moved = 0
+    for case in g["cases"][1:9]:
+        keep1, J = oracle.pyramid(oracle.m2_render(w, h, off - case["dx"], off - case["dy"]), w, h, g["levels"])
+        f64 = oracle.lk_batch(4, I, J, g["levels"] - 1, cfg, xs, ys)
+        lane = oracle.lk_batch(3, I, J, g["levels"] - 1, cfg, xs, ys)
+        sobel = oracle.lk_batch(2, I, J, g["levels"] - 1, cfg, xs, ys)
+        ok = f64["status"] == 0
+        epe = np.hypot(f64["x"] - xs - case["dx"] / 4, f64["y"] - ys - case["dy"] / 4)[ok]
+        assert ok.mean() >= 0.9
+        assert np.median(epe) <= 0.012 and np.percentile(epe, 95) <= 0.022, (case, np.median(epe))
+        both = ok & (lane["status"] == 0)
+        assert np.max(np.abs(lane["x"] - f64["x"])[both]) < 1e-3
+        moved += int((lane["x"] != sobel["x"]).sum())
+    assert moved > 0
+
+
 def test_lk_reference_blob_workload(oracle):
     """The lane tier equals oracle/_ref's lk_track_vx on the 640 x 480 blob workload (fixture written from it)."""
     import vtgpu
