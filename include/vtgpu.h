@@ -256,9 +256,9 @@ int vtg_ncc_search_async(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int 
                          struct vtg_cell *cells, struct vtg_found *found_dev);
 /* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid it). vtg_ncc_search_async with every byte of working memory (the per-search
- * record, the cell map when cells is NULL, the matrix-core cross terms) in a fixed device array of 9 MiB, one per
- * device, looked up when a context is created and held until the process ends: enough for one exhaustive 64 x 64
- * search of a 1920 x 1080 level. Allocates nothing, ever; a search that needs more is refused with VTG_ERANGE. The
+ * record, the cell map when cells is NULL) in a fixed device array of 256 KiB, one per device, looked up when a
+ * context is created and held until the process ends: enough for one exhaustive 64 x 64 search of a 1920 x 1080
+ * level (94 KiB), not for one of a 3840 x 2160 level. Allocates nothing, ever; a search that needs more is refused with VTG_ERANGE. The
  * caller passes the pyramid, the template and the outputs. Every call on a device shares that device's array, which
  * keeps the last call's contents: calls that overlap on one device (another context's stream, another host thread)
  * race, so order them. Results equal vtg_ncc_search_async's bit for bit. */
@@ -271,12 +271,11 @@ int vtg_ncc_search_fixed(vtg_ctx *ctx, const vtg_pyr *p, const vtg_tmpl *t, int 
  * Stream-ordered, no host wait; a search whose window clips to nothing gets score -2 and cells 0. */
 int vtg_ncc_search_many(vtg_ctx *ctx, int n, const vtg_pyr *const *p, const vtg_tmpl *const *t, const int *region,
                         int step, struct vtg_cell *const *cells, struct vtg_found *found_dev);
-/* vtg_ncc_search_many with its stages on two contexts of one device: the cross terms (the map's matrix products) on
- * map_ctx, then, ordered after them, the scoring, top 8, refinement and pick on finish_ctx. A caller overlaps the
- * finish with other work on map_ctx (e.g. LK), joining finish_ctx with vtg_ctx_wait before reading found_dev. The
- * working memory is map_ctx's scratch, and finish_ctx reads it until its work is done: join finish_ctx into map_ctx
- * (vtg_ctx_wait(map_ctx, finish_ctx)) before the next search on map_ctx, split or not, or the two race. finish_ctx
- * must not be destroyed while the search is in flight. */
+/* vtg_ncc_search_many run on finish_ctx, ordered after the work submitted to map_ctx so far (both contexts on one
+ * device): the cell maps, top 8, refinement and pick all run on finish_ctx, beside whatever map_ctx does next (e.g.
+ * LK of the same pyramids). A caller joins finish_ctx (vtg_ctx_wait) before reading found_dev or rebuilding the
+ * searched pyramids on map_ctx. The working memory is finish_ctx's scratch, reused in finish_ctx's stream order.
+ * finish_ctx must not be destroyed while the search is in flight. */
 int vtg_ncc_search_many_split(vtg_ctx *map_ctx, vtg_ctx *finish_ctx, int n, const vtg_pyr *const *p,
                               const vtg_tmpl *const *t, const int *region, int step, struct vtg_cell *const *cells,
                               struct vtg_found *found_dev);

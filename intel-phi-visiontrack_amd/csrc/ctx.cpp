@@ -83,8 +83,6 @@ int scratch(vtg_ctx *c, size_t bytes, void **p)
     if (bytes > c->scratch_bytes) {
         if (c->scratch) {
             VTG_HIP(c, hipStreamSynchronize(c->stream));
-            if (c->peer)
-                VTG_HIP(c, hipStreamSynchronize(c->peer->stream));
             VTG_HIP(c, hipFree(c->scratch));
             c->scratch = nullptr;
             c->scratch_bytes = 0;

@@ -7,12 +7,14 @@
 // so with u = bytes and t = fl(64 u_t - mean) the same sums are exact integers up to one known term:
 //     st  = 4096 (sum u u_t - u0 sum u_t) - 64 mean (sum u - n u0)       [+ 64 sum (u - u0) (t - 64 u_t + mean)]
 //     sum v = 64 (sum u - n u0),   sum v^2 = 4096 (sum u^2 - 2 u0 sum u + n u0^2)
-// The bracketed term is the fp32 rounding of the stored template (|t - 64 u_t + mean| <= 2^-11 for |t| < 2^14):
-// at most 64 * 255 * 2^-11 per pixel, random in sign; it is left out, which moves a score by ~1e-8 (bounded by
-// 1.3e-5 only if every rounding had the same sign and the window were at the energy floor).
-// The search of a 64-wide template computes sum u u_t on the int8 matrix cores (ncc_xcorr_mfma: the cross term of
-// every corner as a banded matrix product, exact i32), then ncc_cells_score forms the box sums sum u, sum u^2 of every
-// corner's window by sliding sums over a tile in LDS (down the columns, then along the rows) and scores in fp64.
+// The bracketed term is the fp32 rounding of the stored template, e = t - 64 u_t + mean. It is 0 for every pixel
+// when tw th is a power of two (then 64 u_t - mean fits in 24 bits). Otherwise the fast maps leave it out only where
+// the template kernel's bound on what that moves a score, 4 * 64 * 255 sum |e| / norm, is at most 1e-6; a template
+// above it (near-black ones reach 1.8e-5) is flagged and its search's map runs in the reference's arithmetic.
+// The search of a 64-wide template computes sum u u_t on the int8 matrix cores (ncc_xcorr_cells: the cross term of
+// every corner as a banded matrix product, exact i32), then, in the same workgroup, the box sums sum u, sum u^2 of
+// every corner's window by sliding sums over the frame tile in LDS (down the columns, then along the rows by DPP
+// prefix scans) and the score in fp64; the cross terms never leave the chip.
 // Narrower templates (16, 32) take the dot4 kernel below, which computes sum u u_t with v_dot4_u32_u8 (4 byte
 // products per lane per instruction, exact u32) beside the same box sums.
 //
@@ -95,7 +97,18 @@ struct TJob {
     float *t, *raw, *nm;
     long long *su;
     uint32_t *words; // level 0 (u8) only
+    int *slow;       // the template's rounding term may move a score by more than 1e-6 (u8 levels)
 };
+
+// The template's rounding term. The matrix-core and dot4 maps score a corner from the identity in the header, which
+// takes the stored template t_k as 64 u_k - mean exactly. It is fl(64 u_k - mean), off by e_k (|e_k| <= half an ulp
+// of t_k); the score st / sqrt(norm var) then misses 64 sum (u - u0) e_k / sqrt(norm var), and since a corner is
+// scored only where var > norm / 16 and |u - u0| <= 255, that is below 4 * 64 * 255 sum |e_k| / norm. When tw th is a
+// power of two every e_k is 0 (the mean of multiples of 64 over a power of two has at most 6 + log2(tw th) fractional
+// bits, so 64 u_k - mean fits in 24); otherwise the template kernel computes the bound and flags the template when it
+// exceeds 1e-6, and the fast maps leave that search to ncc_cells_generic, the reference's arithmetic.
+constexpr double ROUND_BOUND = 4.0 * 64.0 * 255.0;
+constexpr double ROUND_LIMIT = 1e-6;
 
 constexpr int TMAX = 8; // templates per launch
 
@@ -106,12 +119,14 @@ struct TBatch {
 
 template <typename T>
 __device__ __forceinline__ void template_one(Level lv, int x0, int y0, int tw, int th, float *t, float *raw,
-                                             float *out_norm_mean, long long *out_sum_u, uint32_t *words, int wpr)
+                                             float *out_norm_mean, long long *out_sum_u, uint32_t *words, int wpr,
+                                             int *out_slow)
 {
     __shared__ float px[64 * 64];
     __shared__ int ipart[256];
     __shared__ float part[16];
     __shared__ float mean_s;
+    __shared__ double epart[4];
     const int tid = threadIdx.x, n = tw * th;
     const T *base = (const T *)lv.p;
     {
@@ -191,11 +206,16 @@ __device__ __forceinline__ void template_one(Level lv, int x0, int y0, int tw, i
     }
     __syncthreads();
     const float mean = mean_s;
+    double es = 0.0; // sum |e_k| of this thread's pixels
     for (int i = tid; i < n; i += 256) {
         const float z = px[i] - mean;
+        es += fabs((double)z - ((double)px[i] - (double)mean));
         px[i] = z;
         t[i] = z;
     }
+    es = wave_sum_d(es);
+    if ((tid & 63) == 0)
+        epart[tid >> 6] = es;
     __syncthreads();
     if (tid < 16) {
         // lane tid: columns tid, tid + 16, ... of each row in row order, a sequential fp32 chain; 4 rows of loads
@@ -218,8 +238,11 @@ __device__ __forceinline__ void template_one(Level lv, int x0, int y0, int tw, i
     }
     __syncthreads();
     if (tid == 0) {
-        out_norm_mean[0] = tree16(part);
+        const float norm = tree16(part);
+        out_norm_mean[0] = norm;
         out_norm_mean[1] = mean;
+        const double e_sum = epart[0] + epart[1] + epart[2] + epart[3];
+        *out_slow = words && ROUND_BOUND * e_sum > ROUND_LIMIT * (double)norm;
     }
 }
 
@@ -228,7 +251,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void ncc_template_kernel(TBatch b)
 {
     const TJob &j = b.j[blockIdx.x];
-    template_one<T>(j.lv, j.x0, j.y0, b.tw, b.th, j.t, j.raw, j.nm, j.su, j.words, b.wpr);
+    template_one<T>(j.lv, j.x0, j.y0, b.tw, b.th, j.t, j.raw, j.nm, j.su, j.words, b.wpr, j.slow);
 }
 
 // ---------------------------------------------------------------- scores at given corners, the reference's arithmetic
@@ -324,7 +347,7 @@ struct SJob {
     vtg_cell *cells;
     vtg_found *found;
     Aux *aux;
-    int *prod;               // MFMA path: the cross term of every corner of the padded map, 16 cols x 16 rows wide
+    const int *slow;         // the template is flagged slow (its rounding term): the map is ncc_cells_generic's
 };
 
 constexpr int SMAX = 8; // searches per launch
@@ -333,6 +356,7 @@ constexpr int SMAX = 8; // searches per launch
 struct SBatch {
     SJob j[SMAX];
     int n, tw, th, step;
+    int slow_only; // ncc_cells_generic: only the searches whose template is flagged slow
 };
 
 // ---------------------------------------------------------------- cell map, generic: one wave per cell
@@ -343,6 +367,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void ncc_cells_generic(SBatch b)
 {
     const SJob &a = b.j[blockIdx.y];
+    if (b.slow_only && !*a.slow)
+        return; // the whole workgroup: this search's map is the fast kernel's
     const int ncell = a.cols * a.rows;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int cell = blockIdx.x * 4 + wave;
@@ -393,8 +419,8 @@ __global__ __launch_bounds__(256) void ncc_cells_fast(SBatch b)
     __shared__ float cv[4][16];
     __shared__ int cx[4][16], cyy[4][16];
     const SJob &a = b.j[blockIdx.z];
-    if ((int)blockIdx.x * 16 >= a.cols || (int)blockIdx.y >= a.rows)
-        return; // the whole workgroup: no barrier is pending
+    if ((int)blockIdx.x * 16 >= a.cols || (int)blockIdx.y >= a.rows || *a.slow)
+        return; // the whole workgroup (a slow template's map is ncc_cells_generic's): no barrier is pending
     const uint8_t *img = (const uint8_t *)a.lv.p;
     const int ih = a.lv.h, pitch = a.lv.pitch, tw = b.tw, th = b.th, stp = b.step;
     constexpr int LW = NX / 4 + WPR + 1;
@@ -569,7 +595,7 @@ __global__ __launch_bounds__(256) void ncc_cells_fast(SBatch b)
 //     P[x][y] += sum_k a[R][x + k] * b[d - (y - y_t)][k]          (k = 0 .. 63; b rows outside 0 .. th-1 are zero)
 // i.e. one v_mfma_i32_16x16x64_i8 with A = 16 x 64 bytes of image row R sliding in x and B = 64 x 16 template bytes,
 // column y holding template row d - y. Both operands are biased to signed bytes (a = u - 128, b = u_t - 128), which
-// the scoring kernel undoes exactly: sum u u_t = sum a b + 128 (sum u + sum u_t) - 16384 n. 64 of the th + 15 bands
+// the scoring undoes exactly: sum u u_t = sum a b + 128 (sum u + sum u_t) - 16384 n. 64 of the th + 15 bands
 // of a tile are full, so th / (th + 15) of the matrix work is useful (81 % at th = 64).
 // A lane's A operand (row m = lane % 16, bytes 16 (lane / 16) .. + 15 of the window) is 16 unaligned bytes of an LDS
 // row: 5 aligned words funnel-shifted by the lane's byte offset. Its B operand (column n = lane % 16, the same 16 k)
@@ -588,11 +614,6 @@ constexpr int XC_LOADS = ((YBK + 63) * (XS / 4) + 255) / 256; // tile words per 
 // Templates shorter than 33 rows run as 33 rows whose extra rows are zero, so the band of every tile is at least 48
 // rows and the phases of the product loop come in one order.
 __host__ __device__ constexpr int xcorr_th(int th) { return th < 33 ? 33 : th; }
-
-__host__ __device__ constexpr size_t xcorr_lds_bytes(int th)
-{
-    return (size_t)(YBK + xcorr_th(th) - 1) * XS + (size_t)(xcorr_th(th) + 30) * TS;
-}
 
 // Rows r0 .. r1 - 1 of a phase in which y-tiles TLO .. THI are active: each row's A operands (XT x-tiles) and, per
 // active tile, its B operand and XT products.
@@ -621,18 +642,13 @@ __device__ __forceinline__ void xcorr_phase(int r0, int r1, const uint32_t *arow
     }
 }
 
-// Grid: (column blocks of XB corners, row blocks of YBK corners, searches) over each search's padded map
-// (16 cols x 16 rows corners); blocks past a search's own map leave at once.
-__global__ __launch_bounds__(256) void ncc_xcorr_mfma(SBatch b)
+// The cross terms of the workgroup's 128 x 128 corners (block origin bx0, by0 in map corners) into acc: the image tile
+// and the template into LDS, then the product loop. Returns the tile's byte offset of corner column 0 (X0 - A). The
+// tile stays in LDS: rows Y0 .. Y0 + YBK + xcorr_th(th) - 2 from column A, XS bytes a row, biased to signed bytes.
+__device__ __forceinline__ int xcorr_block(const SJob &a, int th, int bx0, int by0, uint32_t *lds, v4i (&acc)[YT][XT])
 {
-    extern __shared__ uint32_t lds[];
-    const SJob &a = b.j[blockIdx.z];
-    const int pcols = 16 * a.cols, prows = 16 * a.rows;
-    const int bx0 = blockIdx.x * XB, by0 = blockIdx.y * YBK; // block origin in map corners
-    if (bx0 >= pcols || by0 >= prows)
-        return; // the whole workgroup: no barrier is pending
     const uint8_t *img = (const uint8_t *)a.lv.p;
-    const int ih = a.lv.h, pitch = a.lv.pitch, th = b.th, the = xcorr_th(th);
+    const int ih = a.lv.h, pitch = a.lv.pitch, the = xcorr_th(th);
     const int X0 = a.x0 + bx0, Y0 = a.y0 + by0;
     const int A = X0 & ~3, off = X0 - A;
     constexpr int XW = XS / 4, TW = TS / 4;
@@ -674,7 +690,6 @@ __global__ __launch_bounds__(256) void ncc_xcorr_mfma(SBatch b)
     const int sh = o0 & 3;
     const uint32_t *arow = tile + (64 * wy) * XW + (o0 >> 2);
     const char *brow = (const char *)tpl + (15 - n16) * TS + 16 * g; // + d TS: template row d - n16 (+15)
-    v4i acc[YT][XT];
 #pragma unroll
     for (int t = 0; t < YT; t++)
 #pragma unroll
@@ -690,29 +705,7 @@ __global__ __launch_bounds__(256) void ncc_xcorr_mfma(SBatch b)
     xcorr_phase<1, 3>(band, band + 16, arow, brow, sh, acc);
     xcorr_phase<2, 3>(band + 16, band + 32, arow, brow, sh, acc);
     xcorr_phase<3, 3>(band + 32, band + 48, arow, brow, sh, acc);
-#pragma unroll
-    for (int t = 0; t < YT; t++)
-#pragma unroll
-        for (int mx = 0; mx < XT; mx++) {
-            const int tx = bx0 + 64 * wx + 16 * mx, ty = by0 + 64 * wy + 16 * t;
-            if (tx < pcols && ty < prows)
-                *(v4i *)(a.prod + (size_t)(ty + n16) * pcols + tx + 4 * g) = acc[t][mx];
-        }
-}
-
-// The cell map from the MFMA cross terms: the frame tile in LDS; for each corner row, the window sums of every byte
-// column (sum u, sum u^2 over th rows, slid down by one row at a time) and their prefix sums within each 64-column
-// chunk (a DPP scan per wave), so a corner's box sums are two differences of prefixes plus, for a box that ends in the
-// next chunk, its own chunk's total; one barrier a corner row; the score in fp64
-// and the first best corner of each 16 x 16 cell (OP_SEARCH, track.c:301-329). Thread c owns corner column X0 + c
-// and byte column c (threads 0 .. 62 also column 256 + c); the 16 lanes of a cell column reduce by shuffles.
-// Grid: (column blocks of 16 cells, cell rows, searches).
-constexpr int SC_LW = NX / 4 + 64 / 4 + 2;   // tile row in words: the 256 corners, the template's reach, alignment
-constexpr int SC_LOADS = 26;                 // tile words per thread: (16 + 64 - 1) rows x SC_LW <= 256 x 26
-
-__host__ __device__ constexpr size_t score_lds_words(int th)
-{
-    return (size_t)(NY + th - 1) * SC_LW + 2 * 2 * 320;
+    return off;
 }
 
 template <int CTRL, int ROW_MASK>
@@ -736,128 +729,135 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
     return v;
 }
 
-// POW2: tw th is a power of two, so sj^2 / n is the exact product sj^2 (1 / n) (the same double, no division).
+// The cell map of a 64-wide template in one pass: the cross terms of the workgroup's 128 x 128 corners on the matrix
+// cores (xcorr_block), then every corner's box sums and score in the same workgroup, so the cross terms never leave
+// the chip. After one barrier (the template's space becomes the stashes) each wave works alone on the 64 x 64 corners
+// whose cross terms it holds: lane l owns corner column l of them and, for its box sums, byte columns l and l + 64 of
+// the wave's 127; it slides their column sums (sum u, sum u^2 over th rows) down the frame tile in LDS, and four DPP
+// prefix scans per corner row give every lane its box (columns l .. l + 63: the first half's total minus its prefix
+// before l, plus the second half's prefix before l) with no workgroup barrier. The wave moves its accumulators to its
+// own stash 8 corner rows at a time, so lane l reads corner column l's cross term. With su, sq the box sums, u0 the
+// window's first byte and n = tw th, the score is the header's identity in fp64 on exact integers: sum u u_t = cross
+// + 128 (su + sum u_t) - 16384 n, st = 4096 (sum u u_t - u0 sum u_t) - 64 mean (su - n u0), var = 4096 (sq - 2 u0 su
+// + n u0^2) - (64 (su - n u0))^2 / n, score st / sqrt(norm var), 0 where var <= norm / 16 (POW2: n a power of two,
+// so the division is the exact product by 1 / n). Each 16 corner rows close a row of the wave's 4 cells, the first
+// best corner of each kept (OP_SEARCH, track.c:301-329). Grid: (column blocks of XB corners, row blocks of YBK
+// corners, searches) over each search's padded map (16 cols x 16 rows corners); blocks past a search's own map leave
+// at once.
+constexpr int XC_SR = 8;             // corner rows a wave stashes at a time
+constexpr int XC_SS = 64 + 4;        // stash row stride in words: 64 corners and a pad
+constexpr int XC_SW = XC_SR * XC_SS; // stash words per wave
+
+__host__ __device__ constexpr size_t xcells_lds_bytes(int th)
+{
+    // the tile, then the larger of the template and the four waves' stashes
+    return (size_t)(YBK + xcorr_th(th) - 1) * XS +
+           ((size_t)(xcorr_th(th) + 30) * TS > (size_t)4 * XC_SW * 4 ? (size_t)(xcorr_th(th) + 30) * TS
+                                                                      : (size_t)4 * XC_SW * 4);
+}
+
+// LDS written by some lanes of a wave, then read by others of the same wave: the wave's LDS operations complete in
+// order, and the fences keep the compiler from moving them across.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool POW2>
-__global__ __launch_bounds__(256) void ncc_cells_score(SBatch b)
+__global__ __launch_bounds__(256) void ncc_xcorr_cells(SBatch b)
 {
     extern __shared__ uint32_t lds[];
     const SJob &a = b.j[blockIdx.z];
-    if ((int)blockIdx.x * 16 >= a.cols || (int)blockIdx.y >= a.rows)
-        return; // the whole workgroup: no barrier is pending
-    const uint8_t *img = (const uint8_t *)a.lv.p;
-    const int ih = a.lv.h, pitch = a.lv.pitch, tw = b.tw, th = b.th, stp = b.step;
-    const int LR = NY + th - 1, VB = NX + tw - 1;
-    uint32_t *E = lds + LR * SC_LW; // [row parity][quantity][320]: each chunk's exclusive prefix sums of the column sums
-    const int X0 = a.x0 + blockIdx.x * NX, Y0 = a.y0 + blockIdx.y * NY;
-    const int A = X0 & ~3, off = X0 - A;
-    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int pcols = 16 * a.cols;
-    const int cx = blockIdx.x * NX + tid; // this thread's corner column in the map
-    const bool in_map = cx < pcols;
-    int pv[NY];
-    {
-        // every load (the tile, this column's 16 cross terms) issued before the first store
-        const int nw = LR * SC_LW;
-        uint32_t v[SC_LOADS];
-#pragma unroll
-        for (int u = 0; u < SC_LOADS; u++) {
-            const int i = tid + 256 * u;
-            const int r = i / SC_LW, c = i - r * SC_LW;
-            const int gy = Y0 + r < ih ? Y0 + r : ih - 1;
-            const int gx = A + 4 * c;
-            v[u] = (i < nw && gx + 3 < pitch) ? *(const uint32_t *)(img + (size_t)gy * pitch + gx) : 0u;
-        }
-        const int *prow = a.prod + (size_t)(blockIdx.y * NY) * pcols + (in_map ? cx : 0);
-#pragma unroll
-        for (int d = 0; d < NY; d++)
-            pv[d] = in_map ? prow[(size_t)d * pcols] : 0;
-#pragma unroll
-        for (int u = 0; u < SC_LOADS; u++) {
-            const int i = tid + 256 * u;
-            if (i < nw)
-                lds[i] = v[u];
-        }
-    }
-    __syncthreads();
-    const uint8_t *tb = (const uint8_t *)lds;
-    constexpr int RB = 4 * SC_LW; // tile row in bytes
-    const uint8_t *col0 = tb + off + tid, *col1 = col0 + 256;
-    const bool two = tid < VB - 256; // this thread also owns byte column 256 + tid
-    const int x = X0 + tid;
-    const bool x_ok = in_map && x <= a.x1 && ((x - a.x0) % stp) == 0;
-    const double n = (double)(tw * th), norm = (double)a.nm[0], mean = (double)a.nm[1];
+    const int pcols = 16 * a.cols, prows = 16 * a.rows;
+    const int bx0 = blockIdx.x * XB, by0 = blockIdx.y * YBK; // block origin in map corners
+    if (bx0 >= pcols || by0 >= prows || *a.slow)
+        return; // the whole workgroup (a slow template's map is ncc_cells_generic's): no barrier is pending
+    const int th = b.th, stp = b.step;
+    v4i acc[YT][XT];
+    const int off = xcorr_block(a, th, bx0, by0, lds, acc);
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int wx = wave & 1, wy = wave >> 1, n16 = lane & 15, g = lane >> 4;
+    const int TR = YBK + xcorr_th(th) - 1;
+    int *st = (int *)(lds + TR * (XS / 4)) + wave * XC_SW; // this wave's stash
+    // byte columns 64 wx + lane (first half) and 64 wx + 64 + lane (second half) of the block's corner space
+    const uint8_t *ca = (const uint8_t *)lds + off + 64 * wx + lane, *cb = ca + 64;
+    const int yw = 64 * wy; // the wave's first corner row in the block
+    const double n = (double)(64 * th), norm = (double)a.nm[0], mean = (double)a.nm[1];
     const double inv_n = 1.0 / n; // exact when POW2
     const double sum_t = (double)a.sum_t[0];
+    const int cx = bx0 + 64 * wx + lane, x = a.x0 + cx; // this lane's corner column in the map, and in the level
+    const bool x_ok = cx < pcols && x <= a.x1 && ((x - a.x0) % stp) == 0;
+    __syncthreads(); // every wave is done with the template: its space takes the stashes
+    uint32_t s1a = 0, s2a = 0, s1b = 0, s2b = 0; // column sums of the lane's two byte columns
+    for (int r = 0; r < th; r++) {
+        const uint32_t ua = ca[(yw + r) * XS] ^ 0x80u, ub = cb[(yw + r) * XS] ^ 0x80u;
+        s1a += ua;
+        s2a += ua * ua;
+        s1b += ub;
+        s2b += ub * ub;
+    }
     float best_v = -3.0f;
     int best_x = 0, best_y = 0;
-    uint32_t s1a = 0, s2a = 0, s1b = 0, s2b = 0;
-    for (int d = 0; d < NY; d++) {
-        // the column sums of corner row d: rows d .. d + th - 1 of byte columns tid (and 256 + tid)
-        if (d == 0) {
-            for (int r = 0; r < th; r++) {
-                const uint32_t ua = col0[r * RB], ub = two ? col1[r * RB] : 0u;
-                s1a += ua;
-                s2a += ua * ua;
-                s1b += ub;
-                s2b += ub * ub;
-            }
-        } else {
-            const uint32_t ia = col0[(d + th - 1) * RB], oa = col0[(d - 1) * RB];
-            s1a += ia - oa;
-            s2a += ia * ia - oa * oa;
-            if (two) {
-                const uint32_t ib = col1[(d + th - 1) * RB], ob = col1[(d - 1) * RB];
+#pragma unroll
+    for (int h = 0; h < 8; h++) {
+        // the wave's corner rows 8 h .. 8 h + 7: y-tile h / 2, held by its lanes with n16 / 8 == h % 2
+        wave_lds_sync(); // the last 8 rows' stash reads are done
+        if ((n16 >> 3) == (h & 1))
+#pragma unroll
+            for (int mx = 0; mx < XT; mx++)
+                *(v4i *)(st + (n16 & 7) * XC_SS + 16 * mx + 4 * g) = acc[h >> 1][mx];
+        wave_lds_sync();
+        for (int rr = 0; rr < XC_SR; rr++) {
+            const int yb = yw + 8 * h + rr; // corner row in the block
+            if (h > 0 || rr > 0) {
+                const int ri = (yb + th - 1) * XS, ro = (yb - 1) * XS;
+                const uint32_t ia = ca[ri] ^ 0x80u, oa = ca[ro] ^ 0x80u, ib = cb[ri] ^ 0x80u, ob = cb[ro] ^ 0x80u;
+                s1a += ia - oa;
+                s2a += ia * ia - oa * oa;
                 s1b += ib - ob;
                 s2b += ib * ib - ob * ob;
             }
-        }
-        // each 64-column chunk's exclusive prefix sums (columns 0-255: wave w's chunk w; columns 256-319: wave 0's
-        // second chunk) into LDS, double-buffered by row; a box of tw <= 64 columns starting in chunk w ends in chunk w
-        // or w + 1, and in the latter case the chunks' offsets differ by chunk w's total, which the wave holds itself:
-        // one barrier a row
-        const uint32_t i1a = wave_incl_scan(s1a), i2a = wave_incl_scan(s2a);
-        uint32_t *Eb = E + (d & 1) * 640;
-        Eb[tid] = i1a - s1a;
-        Eb[320 + tid] = i2a - s2a;
-        if (wave == 0) {
+            const uint32_t i1a = wave_incl_scan(s1a), i2a = wave_incl_scan(s2a);
             const uint32_t i1b = wave_incl_scan(s1b), i2b = wave_incl_scan(s2b);
-            Eb[256 + lane] = i1b - s1b;
-            Eb[576 + lane] = i2b - s2b;
+            const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)i1a, 63);
+            const uint32_t t2 = (uint32_t)__builtin_amdgcn_readlane((int)i2a, 63);
+            // box of corner column lane: first-half columns lane .. 63, second-half columns 0 .. lane - 1
+            const uint32_t box1 = (t1 - i1a + s1a) + (i1b - s1b);
+            const uint32_t box2 = (t2 - i2a + s2a) + (i2b - s2b);
+            const int u0 = ca[yb * XS] ^ 0x80;
+            const int y = a.y0 + by0 + yb;
+            const double su = (double)box1, sq = (double)box2;
+            const double uut = (double)st[rr * XC_SS + lane] + 128.0 * (su + sum_t) - 16384.0 * n; // exact
+            const double pc = uut - (double)u0 * sum_t;
+            const double sv = su - n * (double)u0;
+            const double stv = 4096.0 * pc - 64.0 * mean * sv;
+            const double sj = 64.0 * sv;
+            const double sjj = 4096.0 * (sq - 2.0 * (double)u0 * su + n * (double)u0 * (double)u0);
+            const double var = sjj - (POW2 ? sj * sj * inv_n : sj * sj / n);
+            const double sc = (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : stv / sqrt(norm * var);
+            const float v = (float)sc;
+            const bool ok = x_ok && y <= a.y1 && ((y - a.y0) % stp) == 0;
+            if (ok && better(v, y, x, best_v, best_y, best_x))
+                best_v = v, best_x = x, best_y = y;
         }
-        const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)i1a, 63);
-        const uint32_t t2 = (uint32_t)__builtin_amdgcn_readlane((int)i2a, 63);
-        __syncthreads();
-        // corner (x, y): box sums over columns tid .. tid + tw - 1 of the column sums
-        const bool cross = lane + tw >= 64;
-        const uint32_t box1 = Eb[tid + tw] - Eb[tid] + (cross ? t1 : 0u);
-        const uint32_t box2 = Eb[320 + tid + tw] - Eb[320 + tid] + (cross ? t2 : 0u);
-        const int y = Y0 + d;
-        const int u0 = col0[d * RB];
-        const double su = (double)box1, sq = (double)box2;
-        const double uut = (double)pv[d] + 128.0 * (su + sum_t) - 16384.0 * n; // sum u u_t, exact
-        const double pc = uut - (double)u0 * sum_t;
-        const double sv = su - n * (double)u0;
-        const double st = 4096.0 * pc - 64.0 * mean * sv;
-        const double sj = 64.0 * sv;
-        const double sjj = 4096.0 * (sq - 2.0 * (double)u0 * su + n * (double)u0 * (double)u0);
-        const double var = sjj - (POW2 ? sj * sj * inv_n : sj * sj / n);
-        const double sc = (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : st / sqrt(norm * var);
-        const float v = (float)sc;
-        const bool ok = x_ok && y <= a.y1 && ((y - a.y0) % stp) == 0;
-        if (ok && better(v, y, x, best_v, best_y, best_x))
-            best_v = v, best_x = x, best_y = y;
-    }
-    // the cell of this thread: lanes 16 c .. 16 c + 15 of a wave hold cell column c of the wave
+        if (h & 1) {
+            // a row of the wave's 4 cells: lanes 16 c .. 16 c + 15 hold cell column c
 #pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
-        const float v2 = __shfl_xor(best_v, o);
-        const int x2 = __shfl_xor(best_x, o), y2 = __shfl_xor(best_y, o);
-        if (better(v2, y2, x2, best_v, best_y, best_x))
-            best_v = v2, best_x = x2, best_y = y2;
+            for (int o = 1; o < 16; o <<= 1) {
+                const float v2 = __shfl_xor(best_v, o);
+                const int x2 = __shfl_xor(best_x, o), y2 = __shfl_xor(best_y, o);
+                if (better(v2, y2, x2, best_v, best_y, best_x))
+                    best_v = v2, best_x = x2, best_y = y2;
+            }
+            const int ccol = (bx0 >> 4) + 4 * wx + (lane >> 4), crow = (by0 >> 4) + 4 * wy + (h >> 1);
+            if ((lane & 15) == 0 && ccol < a.cols && crow < a.rows)
+                a.cells[(size_t)crow * a.cols + ccol] = vtg_cell{best_v, best_x, best_y};
+            best_v = -3.0f;
+            best_x = best_y = 0;
+        }
     }
-    const int col = blockIdx.x * 16 + (tid >> 4);
-    if ((tid & 15) == 0 && col < a.cols)
-        a.cells[(size_t)blockIdx.y * a.cols + col] = vtg_cell{best_v, best_x, best_y};
 }
 
 // ---------------------------------------------------------------- finish: top 8, half-pixel refinement, peaks
@@ -1251,6 +1251,7 @@ int launch_ncc_templates(vtg_ctx *c, int n, vtg_tmpl *const *t, const vtg_pyr *c
             j.raw = t[k]->raw;
             j.nm = (float *)t[k]->block;
             j.su = (long long *)((char *)t[k]->block + 16);
+            j.slow = (int *)((char *)t[k]->block + 24);
             j.words = ncc_elem(p[k], level[k]) == 1 ? t[k]->words : nullptr;
         }
         const int rc = by_level_type(c, p0, l0, [&](auto *tag) -> int {
@@ -1294,8 +1295,8 @@ static size_t aux_bytes() { return (sizeof(Aux) + 255) & ~(size_t)255; }
 // One launch per stage for up to SMAX searches that share template size, level element type and step.
 enum { MAP_GENERIC = 0, MAP_DOT4 = 1, MAP_MFMA = 2 };
 
-// The map's products run on c's stream; the scoring pass (MFMA form) and the finish (top 8, refinement, pick) on f's,
-// after the products (f may be c).
+// Every stage (the cell maps, top 8, refinement, pick) runs on f's stream, after the work submitted to c so far (f may
+// be c).
 static int launch_batch(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, int mode, int wpr, SBatch &b)
 {
     int maxcols = 0, maxrows = 0, maxcell = 0;
@@ -1304,59 +1305,64 @@ static int launch_batch(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, in
         maxrows = std::max(maxrows, b.j[k].rows);
         maxcell = std::max(maxcell, b.j[k].cols * b.j[k].rows);
     }
+    if (f != c) {
+        int rc = vtg_ctx_wait(f, c);
+        if (rc)
+            return fail(c, rc, "vtg_ncc_search: %s", vtg_ctx_error(f));
+    }
     hipEvent_t ev;
     if (maxcell > 0) {
         if (mode == MAP_MFMA) {
             const dim3 gx((16 * maxcols + XB - 1) / XB, (16 * maxrows + YBK - 1) / YBK, b.n);
-            timer_begin(c, "ncc_xcorr", &ev);
-            hipLaunchKernelGGL(ncc_xcorr_mfma, gx, dim3(256), xcorr_lds_bytes(b.th), c->stream, b);
-            VTG_HIP(c, hipGetLastError());
-            timer_end(c, "ncc_xcorr", ev);
-            if (f != c) {
-                int rc = vtg_ctx_wait(f, c);
-                if (rc)
-                    return fail(c, rc, "vtg_ncc_search: %s", vtg_ctx_error(f));
-            }
-            const dim3 gs((maxcols + 15) / 16, maxrows, b.n);
-            timer_begin(f, "ncc_cells", &ev);
             const int nt = b.tw * b.th;
+            timer_begin(f, "ncc_xcorr_cells", &ev);
             if ((nt & (nt - 1)) == 0)
-                hipLaunchKernelGGL(ncc_cells_score<true>, gs, dim3(256), score_lds_words(b.th) * sizeof(uint32_t),
-                                   f->stream, b);
+                hipLaunchKernelGGL(ncc_xcorr_cells<true>, gx, dim3(256), xcells_lds_bytes(b.th), f->stream, b);
             else
-                hipLaunchKernelGGL(ncc_cells_score<false>, gs, dim3(256), score_lds_words(b.th) * sizeof(uint32_t),
-                                   f->stream, b);
+                hipLaunchKernelGGL(ncc_xcorr_cells<false>, gx, dim3(256), xcells_lds_bytes(b.th), f->stream, b);
             VTG_HIP(c, hipGetLastError());
-            timer_end(f, "ncc_cells", ev);
+            timer_end(f, "ncc_xcorr_cells", ev);
         } else if (mode == MAP_DOT4) {
             const dim3 grid((maxcols + 15) / 16, maxrows, b.n);
             const size_t lds = fast_lds_words(b.th, b.tw, wpr) * sizeof(uint32_t);
-            timer_begin(c, "ncc_cells", &ev);
+            timer_begin(f, "ncc_cells", &ev);
             if (wpr == 17)
-                hipLaunchKernelGGL(ncc_cells_fast<17>, grid, dim3(256), lds, c->stream, b);
+                hipLaunchKernelGGL(ncc_cells_fast<17>, grid, dim3(256), lds, f->stream, b);
             else if (wpr == 9)
-                hipLaunchKernelGGL(ncc_cells_fast<9>, grid, dim3(256), lds, c->stream, b);
+                hipLaunchKernelGGL(ncc_cells_fast<9>, grid, dim3(256), lds, f->stream, b);
             else
-                hipLaunchKernelGGL(ncc_cells_fast<5>, grid, dim3(256), lds, c->stream, b);
+                hipLaunchKernelGGL(ncc_cells_fast<5>, grid, dim3(256), lds, f->stream, b);
             VTG_HIP(c, hipGetLastError());
-            timer_end(c, "ncc_cells", ev);
+            timer_end(f, "ncc_cells", ev);
         } else {
             int rc = by_level_type(c, p0, level, [&](auto *tag) -> int {
                 using T = std::remove_pointer_t<decltype(tag)>;
-                timer_begin(c, "ncc_cells_generic", &ev);
-                hipLaunchKernelGGL(ncc_cells_generic<T>, dim3((maxcell + 3) / 4, b.n), dim3(256), 0, c->stream, b);
+                timer_begin(f, "ncc_cells_generic", &ev);
+                hipLaunchKernelGGL(ncc_cells_generic<T>, dim3((maxcell + 3) / 4, b.n), dim3(256), 0, f->stream, b);
                 VTG_HIP(c, hipGetLastError());
-                timer_end(c, "ncc_cells_generic", ev);
+                timer_end(f, "ncc_cells_generic", ev);
                 return VTG_OK;
             });
             if (rc)
                 return rc;
         }
-    }
-    if (f != c && mode != MAP_MFMA) {
-        int rc = vtg_ctx_wait(f, c);
-        if (rc)
-            return fail(c, rc, "vtg_ncc_search: %s", vtg_ctx_error(f));
+        const int nt = b.tw * b.th;
+        if (mode != MAP_GENERIC && (nt & (nt - 1)) != 0) {
+            // a template whose rounding term the fast map would leave out (flagged by the template kernel): its
+            // search's map in the reference's arithmetic instead; every other search's workgroups leave at once
+            SBatch s = b;
+            s.slow_only = 1;
+            int rc = by_level_type(c, p0, level, [&](auto *tag) -> int {
+                using T = std::remove_pointer_t<decltype(tag)>;
+                timer_begin(f, "ncc_cells_generic", &ev);
+                hipLaunchKernelGGL(ncc_cells_generic<T>, dim3((maxcell + 3) / 4, b.n), dim3(256), 0, f->stream, s);
+                VTG_HIP(c, hipGetLastError());
+                timer_end(f, "ncc_cells_generic", ev);
+                return VTG_OK;
+            });
+            if (rc)
+                return rc;
+        }
     }
     timer_begin(f, "ncc_top8", &ev);
     hipLaunchKernelGGL(ncc_top8, dim3(b.n), dim3(1024), 0, f->stream, b);
@@ -1437,8 +1443,6 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
         ncell[k] = (x1 < x0 || y1 < y0) ? 0 : ((x1 - x0) / 16 + 1) * ((y1 - y0) / 16 + 1);
         if (!(cells && cells[k]))
             bytes += ((size_t)ncell[k] * sizeof(vtg_cell) + 255) & ~(size_t)255;
-        if (mode == MAP_MFMA)
-            bytes += (size_t)ncell[k] * 256 * sizeof(int); // the cross terms of the padded map
     }
     void *sc = nullptr;
     int rc;
@@ -1447,13 +1451,9 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
             return fail(c, VTG_ERANGE, "vtg_ncc_search_fixed: the search needs %zu bytes of working memory, the fixed "
                                        "array holds %zu", bytes, fixed_bytes);
         sc = fixed;
-    } else {
-        // A split search's finish context reads this scratch after the call returns (the caller joins it before the
-        // next search here, as vtgpu.h asks). The peer stays set, so growing the scratch later still waits for it.
-        if (f != c)
-            c->peer = f;
-        if ((rc = scratch(c, bytes, &sc)))
-            return rc;
+    } else if ((rc = scratch(f, bytes, &sc))) {
+        // every stage runs on f's stream, so f's scratch is reused in stream order
+        return rc;
     }
     char *aux0 = (char *)sc, *fnd0 = aux0 + (size_t)n * auxb;
     char *cel = fnd0 + (found_dev ? 0 : (size_t)n * 256);
@@ -1473,6 +1473,7 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
             j.t = t[k]->t;
             j.nm = (const float *)t[k]->block;
             j.sum_t = (const long long *)((const char *)t[k]->block + 16);
+            j.slow = (const int *)((const char *)t[k]->block + 24);
             j.words = t[k]->words;
             j.x0 = w[0], j.x1 = w[1], j.y0 = w[2], j.y1 = w[3];
             j.cols = ncell[k] ? (w[1] - w[0]) / 16 + 1 : 0;
@@ -1485,11 +1486,6 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
             }
             j.found = found_dev ? found_dev + k : (vtg_found *)(fnd0 + (size_t)k * 256);
             j.aux = (Aux *)(aux0 + (size_t)k * auxb);
-            j.prod = nullptr;
-            if (mode == MAP_MFMA) {
-                j.prod = (int *)cel;
-                cel += (size_t)ncell[k] * 256 * sizeof(int);
-            }
         }
         if ((rc = launch_batch(c, f, p[0], level0, mode, t0->words_per_row, b)))
             return rc;
@@ -1501,8 +1497,9 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
 
 // The working memory of vtg_ncc_search_fixed: a fixed array, one copy per device, held until the process ends. Host
 // code reaches it only through the address hipGetSymbolAddress gives for the current device (the name itself is a
-// host-side stand-in, not a device address). Enough for one exhaustive 64 x 64 search of a 1920 x 1080 level.
-static constexpr size_t kFixedBytes = (size_t)9 << 20;
+// host-side stand-in, not a device address). Enough for one exhaustive 64 x 64 search of a 1920 x 1080 level (its
+// record and 7,424-cell map take 94 KiB), not for one of a 3840 x 2160 level (367 KiB).
+static constexpr size_t kFixedBytes = (size_t)256 << 10;
 static __device__ __align__(256) unsigned char scratch[kFixedBytes];
 
 namespace vtg {

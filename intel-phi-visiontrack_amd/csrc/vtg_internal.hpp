@@ -56,7 +56,6 @@ struct vtg_ctx {
     // scratch owned by the context (grown on demand, freed on destroy)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
-    vtg_ctx *peer = nullptr;                 // another context whose stream also reads the scratch (split search)
     void *filt = nullptr;                    // the contrast stretch's tables and per-frame tables (built on first use)
     void *fused = nullptr;                   // the fused launch's work counter, error word and per-level done counts
     bool fused_init = false;

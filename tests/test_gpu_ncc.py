@@ -307,9 +307,9 @@ def test_search_split_matches_single_context(ctx, tw, th):
 
 
 def test_split_searches_in_a_row(ctx):
-    """Split searches back to back on one map context, each joined as vtgpu.h asks before the next search there (the
-    second over smaller windows), then a one-context search needing more scratch than before (the scratch grows while
-    the earlier finish context may still hold it): every record equals the one-context call's."""
+    """Split searches back to back with no join between them (the second over smaller windows: both use the finish
+    context's scratch, in its stream order), joined once, then one-context searches on the map context, the first
+    needing more scratch than before: every record equals the one-context call's."""
     import ctypes as C
     other = vtgpu.Context(0)
     sizes = [(1920, 1080), (640, 480), (1280, 720)]
@@ -327,7 +327,6 @@ def test_split_searches_in_a_row(ctx):
     bufs = [ctx.alloc(fs * n) for _ in range(5)]
     say("ncc split searches in a row")
     ctx.ncc_search_many_split(other, [p[1] for p in pyrs], tms, big, 1, bufs[0])
-    ctx.wait(other)
     ctx.ncc_search_many_split(other, [p[1] for p in pyrs], tms, small, 1, bufs[1])
     ctx.wait(other)
     many = [p[1] for p in pyrs] * 3
@@ -403,3 +402,73 @@ def test_search_q6f_equals_q6(ctx, tw, th, level):
         p0.destroy()
         p1.destroy()
     assert out[0] == out[1] == out[2]
+
+
+def _rounding_bound(img, x0, y0, tw, th, app, norm):
+    """4 * 64 * 255 * sum |e_k| / norm for the u8 template at (x0, y0): e_k = t_k - (64 u_k - mean), the stored
+    template's fp32 rounding (the template equals the oracle's bit for bit, test_template_exact)."""
+    u = img[y0:y0 + th, x0:x0 + tw].astype(np.float64)
+    mean = np.float32(64.0 * u.sum() / (tw * th))
+    app = np.asarray(app, np.float32).reshape(th, tw)
+    assert np.array_equal(app, np.float32(64.0) * u.astype(np.float32) - mean)
+    e = app.astype(np.float64) - (64.0 * u - np.float64(mean))
+    return 4 * 64 * 255 * np.abs(e).sum() / float(norm)
+
+
+@pytest.mark.parametrize("kind,tw,th", [("dark01", 64, 48), ("dark01", 32, 24), ("dark01", 64, 64),
+                                        ("uniform", 64, 40)])
+def test_search_template_rounding(ctx, oracle, kind, tw, th):
+    """The fast maps (matrix cores for 64-wide templates, dot4 for 16/32-wide ones) take the stored template as
+    64 u - mean exactly. Where tw th is not a power of two and the rounding term's bound exceeds 1e-6 (a near-black
+    template of 0s and 1s: 1.8e-5), the template kernel flags it and the search's map is the reference's arithmetic:
+    the u8 level's map equals the fp32 level's (ncc_cells_generic) bit for bit. Below the bound (a power-of-two size,
+    where every e_k is 0, or a uniform texture) the fast map runs, within 1e-5 of the oracle. Every search picks what
+    the oracle picks."""
+    w, h = 400, 300
+    rng = np.random.default_rng(11 + tw + th)
+    if kind == "dark01":
+        f0 = (rng.random((h, w)) < 0.15).astype(np.uint8)
+    else:
+        f0 = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    f1 = np.roll(f0, (4, 7), axis=(0, 1))
+    noise = rng.random((h, w)) < 0.02
+    f1[noise] = (1 - f1[noise]) if kind == "dark01" else rng.integers(0, 256, int(noise.sum()))
+    f1 = np.ascontiguousarray(f1)
+    p0, p1 = ctx.pyramid(w, h, 1), ctx.pyramid(w, h, 1)
+    q1 = ctx.pyramid(w, h, 1, vtgpu.STORE_F32)
+    p0.build(f0)
+    p1.build(f1)
+    q1.build(f1)
+    want = kind == "dark01" and (tw * th) & (tw * th - 1) != 0  # a template to flag: one whose mean rounds
+    y0 = 120
+    for x0 in range(150, 190):
+        t = ctx.ncc_template(p0, 0, x0, y0, tw, th)
+        app, norm = t.read()
+        bound = _rounding_bound(f0, x0, y0, tw, th, app, norm)
+        if bound > 0 or not want:
+            break
+        t.destroy()
+    flagged = bound > 1e-6
+    assert flagged == want, bound
+    ncell = ((w - tw - 2 - 1) // 16 + 1) * ((h - th - 2 - 1) // 16 + 1)
+    cq, cf = ctx.alloc(12 * ncell), ctx.alloc(12 * ncell)
+    say("ncc template rounding", kind, tw, th, "bound", bound)
+    fq = ctx.ncc_search(p1, t, 1, w, 1, h, 1, cq)
+    ff = ctx.ncc_search(q1, t, 1, w, 1, h, 1, cf)
+    gq = cq.get(np.zeros(ncell, vtgpu.CELL_DTYPE))
+    gf = cf.get(np.zeros(ncell, vtgpu.CELL_DTYPE))
+    k1, a1 = oracle.pyramid(f1, w, h, 1)
+    of, oc = oracle.ncc_search(k1[0], app, tw, th, norm, 1, w, 1, h, 1)
+    _check_cells(gq, oc, lambda x, y: oracle.ncc_at(k1[0], app, tw, th, norm, int(x), int(y)))
+    if flagged:
+        assert gq.tobytes() == gf.tobytes()
+    print(f"[ncc] rounding {kind} {tw}x{th}: bound {bound:.3g}, flagged {flagged}, found ({fq.hx},{fq.hy}) "
+          f"{fq.ncc:.6f} vs ({of.hx},{of.hy}) {float(of.ncc):.6f}")
+    for f in (fq, ff):
+        assert (f.hx, f.hy) == (of.hx, of.hy)
+        assert abs(f.ncc - float(of.ncc)) <= 1e-5 and abs(f.second - float(of.second)) <= 1e-5
+    for x in (cq, cf):
+        x.free()
+    t.destroy()
+    for p in (p0, p1, q1):
+        p.destroy()

@@ -184,6 +184,37 @@ def test_m2_scharr_accuracy(oracle):
     assert moved > 0
 
 
+def test_template_rounding_term(oracle):
+    """The identity the GPU's fast NCC maps score with takes the stored template t_k = fl(64 u_k - mean) as exactly
+    64 u_k - mean. On a u8 level with tw th a power of two it is exact: every e_k = t_k - (64 u_k - mean) is 0 (the
+    mean has at most 6 + log2(tw th) fractional bits), on uniform, near-black and 4-grey templates. Otherwise it need not be,
+    and the bound 4 * 64 * 255 * sum |e| / norm on what leaving it out moves a score (the GPU flags templates above 1e-6
+    for the exact kernel) is below 1e-7 on a uniform texture and 1.8e-5 on a near-black template of 0s and 1s."""
+    w, h = 200, 150
+    rng = np.random.default_rng(5)
+
+    def bound(img, tw, th):
+        a = _l0(oracle, img, w, h)
+        app, norm = oracle.ncc_template(a, 10, 10, tw, th)
+        app = np.asarray(app, np.float32).reshape(th, tw)
+        u = img[10:10 + th, 10:10 + tw].astype(np.float64)
+        mean = np.float32(64.0 * u.sum() / (tw * th))
+        assert np.array_equal(app, np.float32(64.0) * u.astype(np.float32) - mean)  # the oracle's own rounding
+        e = app.astype(np.float64) - (64.0 * u - np.float64(mean))
+        return np.abs(e).sum(), 4 * 64 * 255 * np.abs(e).sum() / float(norm)
+
+    images = {"uniform": rng.integers(0, 256, (h, w)), "dark01": (rng.random((h, w)) < 0.15),
+              "grey0-3": rng.integers(0, 4, (h, w))}
+    images = {k: np.ascontiguousarray(v.astype(np.uint8)) for k, v in images.items()}
+    for kind, img in images.items():
+        for tw, th in ((64, 64), (32, 32), (16, 16), (64, 32), (32, 16), (16, 64)):
+            assert bound(img, tw, th)[0] == 0.0, (kind, tw, th)
+    assert bound(images["uniform"], 64, 40)[1] < 1e-7
+    for tw, th in ((64, 48), (32, 24), (64, 21)):
+        es, b = bound(images["dark01"], tw, th)
+        assert es > 0 and 1e-5 < b < 3e-5, (tw, th, b)
+
+
 def test_lk_reference_blob_workload(oracle):
     """The lane tier equals oracle/_ref's lk_track_vx on the 640 x 480 blob workload (fixture written from it)."""
     import vtgpu
