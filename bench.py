@@ -84,9 +84,9 @@ def pyr_bytes(w, h, levels, elem):
 
 
 class Batched:
-    """B pairs per step on two contexts (A: pyramids, cross terms and LK; S: the searches' scoring and finish), the
-    results of each step downloaded on one of two copy contexts while the next step runs (double-buffered pinned
-    host records)."""
+    """B pairs per step on two contexts (A: pyramids, template cuts and LK; S: the searches, beside LK), the results
+    of each step downloaded on one of two copy contexts while the next step runs (double-buffered pinned host
+    records)."""
 
     def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph, one_stream=False):
         self.v, self.B, self.graph, self.one_stream = vtgpu, batch, graph, one_stream
@@ -115,10 +115,10 @@ class Batched:
         self.graphs = {}
 
     def enqueue(self, g, serial=False):
-        """All device work of group g's batch. Context A: the 2B pyramids, the B template cuts, the B searches' cross
-        terms (matrix cores), then LK of all pairs; context S, after the cross terms: the searches' scoring and finish,
-        beside LK. A waits for S at the end. serial: everything on A, one kernel after another (for timing each
-        kernel alone)."""
+        """All device work of group g's batch. Context A: the 2B pyramids, the B template cuts, then LK of all pairs;
+        context S, after the template cuts: the B searches (cell maps on the matrix cores, top 8, refinement), beside
+        LK. A waits for S at the end. serial: everything on A, one kernel after another (for timing each kernel
+        alone)."""
         A, S, B = self.A, self.S, self.B
         serial = serial or self.one_stream
         pyrs = self.pyrs[g if self.graph else 0]
@@ -708,7 +708,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=72, help="device-resident frame pairs rotated through")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
     ap.add_argument("--one-stream", action="store_true",
-                    help="the searches' scoring on the LK stream too (no fork and join inside the step)")
+                    help="the searches on the LK stream too (no fork and join inside the step)")
     ap.add_argument("--plain-steps", type=int, default=8)
     ap.add_argument("--batch-steps", type=int, default=20, help="graph replays of the uneven 64-pair batch (0: skip)")
     ap.add_argument("--serve-frames", type=int, default=300, help="frames per stream of the serving runs (0: skip)")

@@ -1295,20 +1295,14 @@ static size_t aux_bytes() { return (sizeof(Aux) + 255) & ~(size_t)255; }
 // One launch per stage for up to SMAX searches that share template size, level element type and step.
 enum { MAP_GENERIC = 0, MAP_DOT4 = 1, MAP_MFMA = 2 };
 
-// Every stage (the cell maps, top 8, refinement, pick) runs on f's stream, after the work submitted to c so far (f may
-// be c).
-static int launch_batch(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, int mode, int wpr, SBatch &b)
+// The cell maps of a group of searches on f's stream (errors reported on c).
+static int launch_maps(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, int mode, int wpr, const SBatch &b)
 {
     int maxcols = 0, maxrows = 0, maxcell = 0;
     for (int k = 0; k < b.n; k++) {
         maxcols = std::max(maxcols, b.j[k].cols);
         maxrows = std::max(maxrows, b.j[k].rows);
         maxcell = std::max(maxcell, b.j[k].cols * b.j[k].rows);
-    }
-    if (f != c) {
-        int rc = vtg_ctx_wait(f, c);
-        if (rc)
-            return fail(c, rc, "vtg_ncc_search: %s", vtg_ctx_error(f));
     }
     hipEvent_t ev;
     if (maxcell > 0) {
@@ -1364,6 +1358,13 @@ static int launch_batch(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, in
                 return rc;
         }
     }
+    return VTG_OK;
+}
+
+// Top 8, refinement and pick of a group of searches on f's stream, after its maps (errors reported on c).
+static int launch_finish(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, const SBatch &b)
+{
+    hipEvent_t ev;
     timer_begin(f, "ncc_top8", &ev);
     hipLaunchKernelGGL(ncc_top8, dim3(b.n), dim3(1024), 0, f->stream, b);
     VTG_HIP(c, hipGetLastError());
@@ -1395,9 +1396,10 @@ static int launch_batch(vtg_ctx *c, vtg_ctx *f, const vtg_pyr *p0, int level, in
 }
 
 // n searches into device records: each window clipped to where the template fits (as search_slot clips), maps in
-// cells[k] or the context's scratch, picks in found_dev[k] or, when found_dev is null, in the scratch (returned in
-// *found_scratch). Stream-ordered; no allocation once the scratch has grown to the batch's size.
-// fixed: working memory of fixed_bytes to use instead of the context's scratch (nothing is allocated then).
+// cells[k] or f's scratch, picks in found_dev[k] or, when found_dev is null, in the scratch (returned in
+// *found_scratch). Every stage runs on f's stream after the work submitted to c so far (f may be c).
+// Stream-ordered; no allocation once the scratch has grown to the batch's size. fixed: working memory of fixed_bytes
+// to use instead of the scratch (nothing is allocated then).
 static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, const vtg_tmpl *const *t,
                        const int *region, int step, vtg_cell *const *cells, vtg_found *found_dev,
                        vtg_found **found_scratch, void *fixed = nullptr, size_t fixed_bytes = 0)
@@ -1455,6 +1457,8 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
         // every stage runs on f's stream, so f's scratch is reused in stream order
         return rc;
     }
+    if (f != c && (rc = vtg_ctx_wait(f, c)))
+        return fail(c, rc, "vtg_ncc_search: %s", vtg_ctx_error(f));
     char *aux0 = (char *)sc, *fnd0 = aux0 + (size_t)n * auxb;
     char *cel = fnd0 + (found_dev ? 0 : (size_t)n * 256);
     if (found_scratch)
@@ -1487,7 +1491,8 @@ static int search_many(vtg_ctx *c, vtg_ctx *f, int n, const vtg_pyr *const *p, c
             j.found = found_dev ? found_dev + k : (vtg_found *)(fnd0 + (size_t)k * 256);
             j.aux = (Aux *)(aux0 + (size_t)k * auxb);
         }
-        if ((rc = launch_batch(c, f, p[0], level0, mode, t0->words_per_row, b)))
+        if ((rc = launch_maps(c, f, p[0], level0, mode, t0->words_per_row, b)) ||
+            (rc = launch_finish(c, f, p[0], level0, b)))
             return rc;
     }
     return VTG_OK;
