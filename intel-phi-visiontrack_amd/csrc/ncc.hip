@@ -737,10 +737,12 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 // prefix scans per corner row give every lane its box (columns l .. l + 63: the first half's total minus its prefix
 // before l, plus the second half's prefix before l) with no workgroup barrier. The wave moves its accumulators to its
 // own stash 8 corner rows at a time, so lane l reads corner column l's cross term. With su, sq the box sums, u0 the
-// window's first byte and n = tw th, the score is the header's identity in fp64 on exact integers: sum u u_t = cross
-// + 128 (su + sum u_t) - 16384 n, st = 4096 (sum u u_t - u0 sum u_t) - 64 mean (su - n u0), var = 4096 (sq - 2 u0 su
-// + n u0^2) - (64 (su - n u0))^2 / n, score st / sqrt(norm var), 0 where var <= norm / 16 (POW2: n a power of two,
-// so the division is the exact product by 1 / n). Each 16 corner rows close a row of the wave's 4 cells, the first
+// window's first byte and n = tw th, the score is the header's identity on exact integers: sum u u_t = cross + 128 (su
+// + sum u_t) - 16384 n, st = 4096 (sum u u_t - u0 sum u_t) - 64 mean (su - n u0), var = 4096 (sq - 2 u0 su + n u0^2)
+// - (64 (su - n u0))^2 / n; st / 64 and var n / 4096 are exact in fp64, and the score st / sqrt(norm var), 0 where
+// var <= norm / 16, takes one rounded quotient by n (exact when POW2: n a power of two) and a reciprocal square root
+// refined by two Newton steps (relative error near 1e-16, far inside the float the map keeps). Each 16 corner rows
+// close a row of the wave's 4 cells, the first
 // best corner of each kept (OP_SEARCH, track.c:301-329). Grid: (column blocks of XB corners, row blocks of YBK
 // corners, searches) over each search's padded map (16 cols x 16 rows corners); blocks past a search's own map leave
 // at once.
@@ -784,11 +786,12 @@ __global__ __launch_bounds__(256) void ncc_xcorr_cells(SBatch b)
     // byte columns 64 wx + lane (first half) and 64 wx + 64 + lane (second half) of the block's corner space
     const uint8_t *ca = (const uint8_t *)lds + off + 64 * wx + lane, *cb = ca + 64;
     const int yw = 64 * wy; // the wave's first corner row in the block
-    const double n = (double)(64 * th), norm = (double)a.nm[0], mean = (double)a.nm[1];
-    const double inv_n = 1.0 / n; // exact when POW2
-    const double sum_t = (double)a.sum_t[0];
+    const int n = 64 * th, sum_t = (int)a.sum_t[0];
+    const double norm = (double)a.nm[0], mean = (double)a.nm[1];
+    const double nn = norm * (double)n;      // exact: 24 and 13 significant bits
+    const double inv_n = 1.0 / (double)n;    // exact when POW2
     const int cx = bx0 + 64 * wx + lane, x = a.x0 + cx; // this lane's corner column in the map, and in the level
-    const bool x_ok = cx < pcols && x <= a.x1 && ((x - a.x0) % stp) == 0;
+    const bool x_ok = cx < pcols && x <= a.x1 && ((x - a.x0) & (stp - 1)) == 0; // step 1 or 2
     __syncthreads(); // every wave is done with the template: its space takes the stashes
     uint32_t s1a = 0, s2a = 0, s1b = 0, s2b = 0; // column sums of the lane's two byte columns
     for (int r = 0; r < th; r++) {
@@ -828,19 +831,26 @@ __global__ __launch_bounds__(256) void ncc_xcorr_cells(SBatch b)
             const uint32_t box2 = (t2 - i2a + s2a) + (i2b - s2b);
             const int u0 = ca[yb * XS] ^ 0x80;
             const int y = a.y0 + by0 + yb;
-            const double su = (double)box1, sq = (double)box2;
-            const double uut = (double)st[rr * XC_SS + lane] + 128.0 * (su + sum_t) - 16384.0 * n; // exact
-            const double pc = uut - (double)u0 * sum_t;
-            const double sv = su - n * (double)u0;
-            const double stv = 4096.0 * pc - 64.0 * mean * sv;
-            const double sj = 64.0 * sv;
-            const double sjj = 4096.0 * (sq - 2.0 * (double)u0 * su + n * (double)u0 * (double)u0);
-            const double var = sjj - (POW2 ? sj * sj * inv_n : sj * sj / n);
-            const double sc = (var <= norm * 0.0625 || norm <= 0.0) ? 0.0 : stv / sqrt(norm * var);
-            const float v = (float)sc;
-            const bool ok = x_ok && y <= a.y1 && ((y - a.y0) % stp) == 0;
-            if (ok && better(v, y, x, best_v, best_y, best_x))
-                best_v = v, best_x = x, best_y = y;
+            // exact integers (every true value fits in 31 bits; the 32-bit arithmetic wraps consistently):
+            // pc = sum (u - u0) u_t, s1 = sum (u - u0), s2 = sum (u - u0)^2
+            const int pc = st[rr * XC_SS + lane] + 128 * ((int)box1 + sum_t) - 16384 * n - u0 * sum_t;
+            const int s1 = (int)box1 - n * u0;
+            const int s2 = (int)(box2 - 2u * (uint32_t)u0 * box1 + (uint32_t)(n * u0 * u0));
+            // st = 64 b and var = 4096 an / n with b = 64 pc - mean s1 and an = n s2 - s1^2, both exact in fp64; the
+            // score st / sqrt(norm var) = b / sqrt(norm an / n), scored where var > norm / 16, i.e. 65536 an > norm n
+            const double ds1 = (double)s1;
+            const double b = fma(-mean, ds1, 64.0 * (double)pc);
+            const double an = fma((double)n, (double)s2, -(ds1 * ds1));
+            const double w = POW2 ? norm * an * inv_n : norm * an / (double)n;
+            double r = __builtin_amdgcn_rsq(w); // refined twice by Newton's step r (3 - w r^2) / 2
+            r = fma(r, fma(-0.5 * w * r, r, 0.5), r);
+            r = fma(r, fma(-0.5 * w * r, r, 0.5), r);
+            const float v = (65536.0 * an <= nn || norm <= 0.0) ? 0.0f : (float)(b * r);
+            const bool ok = x_ok && y <= a.y1 && ((y - a.y0) & (stp - 1)) == 0;
+            const bool take = ok && better(v, y, x, best_v, best_y, best_x);
+            best_v = take ? v : best_v;
+            best_x = take ? x : best_x;
+            best_y = take ? y : best_y;
         }
         if (h & 1) {
             // a row of the wave's 4 cells: lanes 16 c .. 16 c + 15 hold cell column c
