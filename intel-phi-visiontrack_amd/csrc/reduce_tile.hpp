@@ -68,6 +68,20 @@ template <> struct CopyConv<float, uint8_t> {
 template <bool EXACT> struct AccT { using type = float; };
 template <> struct AccT<true> { using type = int; };
 
+// Four frame bytes (one aligned word) into four level-0 elements at p, 16-byte aligned for fp32, 4 for bytes.
+template <typename TC> __device__ __forceinline__ void store4(TC *p, uint32_t w)
+{
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+        p[b] = CopyConv<TC, uint8_t>::get((uint8_t)(w >> (8 * b)));
+}
+template <> __device__ __forceinline__ void store4<float>(float *p, uint32_t w)
+{
+    *(float4 *)p = make_float4((float)(w & 255u) * 64.0f, (float)((w >> 8) & 255u) * 64.0f,
+                               (float)((w >> 16) & 255u) * 64.0f, (float)(w >> 24) * 64.0f);
+}
+template <> __device__ __forceinline__ void store4<uint8_t>(uint8_t *p, uint32_t w) { *(uint32_t *)p = w; }
+
 // Reduce for a batch of jobs, one 64 x 16 output tile per workgroup. The 131 x 35 input tile (2-pixel halo,
 // coordinates clamped: the replicated border) is read with every load of a thread issued before the first is used,
 // then summed in LDS: 5 rows at each even row, then 5 columns at each even column. EXACT: integer sums and one
@@ -87,27 +101,71 @@ __device__ __forceinline__ void reduce_tile(const RJob &J, int tile, typename Ac
     const int ox0 = txi * TW, oy0 = tyi * TH;
     const int ix0 = 2 * ox0 - 2, iy0 = 2 * oy0 - 2;
     const TS *src = (const TS *)J.src;
-    constexpr int NIN = IH * IW, NLD = (NIN + 255) / 256;
-    TS raw[NLD];
+    bool fast = false;
+    if constexpr (sizeof(TS) == 1) {
+        // A u8 source tile whose columns all lie inside the frame, rows and pointers word-aligned: the input read as
+        // 34 aligned words a row (columns ix0 - 2 .. ix0 + 133), the level-0 copies written a word (4 pixels) at a
+        // time. Rows are still clamped. The other tiles take the element path below; both store the same values.
+        fast = txi > 0 && ix0 + IW + 3 <= J.sw && (J.sp & 3) == 0 && ((size_t)J.src & 3) == 0 &&
+               (!COPY || (((size_t)J.copy & 15) == 0 && (J.cp * sizeof(TC)) % 16 == 0 &&
+                          (!J.copy8 || (((size_t)J.copy8 & 3) == 0 && (J.cp8 & 3) == 0))));
+        if (fast) {
+            constexpr int NW = IW / 4 + 2, NWI = IH * NW, NWL = (NWI + 255) / 256; // 34 words a row
+            const uint8_t *s8 = (const uint8_t *)J.src + ix0 - 2;                  // word-aligned: ix0 = 128 txi - 2
+            uint32_t v[NWL];
 #pragma unroll
-    for (int k = 0; k < NLD; k++) {
-        const int i = threadIdx.x + 256 * k, ic = i < NIN ? i : NIN - 1;
-        const int r = ic / IW, cc = ic - r * IW;
-        raw[k] = src[(size_t)clampi(iy0 + r, 0, J.sh - 1) * J.sp + clampi(ix0 + cc, 0, J.sw - 1)];
+            for (int k = 0; k < NWL; k++) {
+                const int i = threadIdx.x + 256 * k, ic = i < NWI ? i : NWI - 1;
+                const int r = ic / NW, w = ic - r * NW;
+                v[k] = *(const uint32_t *)(s8 + (size_t)clampi(iy0 + r, 0, J.sh - 1) * J.sp + 4 * w);
+            }
+#pragma unroll
+            for (int k = 0; k < NWL; k++) {
+                const int i = threadIdx.x + 256 * k;
+                if (i < NWI) {
+                    const int r = i / NW, w = i - r * NW;
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        const int cc = 4 * w - 2 + b; // byte b of word w is column ix0 + cc
+                        if (cc >= 0 && cc < IW)
+                            tin[r][cc] = (Acc)Q6<TS>::get((TS)(v[k] >> (8 * b)));
+                    }
+                    if (COPY) {
+                        // words 1 .. 2 TW / 4 of rows 2 .. 2 TH + 1 are the tile's own level-0 pixels
+                        const int yy = iy0 + r, xx = ix0 - 2 + 4 * w;
+                        if (r >= 2 && r < 2 + 2 * TH && w >= 1 && w <= 2 * TW / 4 && yy < J.sh) {
+                            store4<TC>((TC *)J.copy + (size_t)yy * J.cp + xx, v[k]);
+                            if (J.copy8)
+                                *(uint32_t *)((uint8_t *)J.copy8 + (size_t)yy * J.cp8 + xx) = v[k];
+                        }
+                    }
+                }
+            }
+        }
     }
+    if (!fast) {
+        constexpr int NIN = IH * IW, NLD = (NIN + 255) / 256;
+        TS raw[NLD];
 #pragma unroll
-    for (int k = 0; k < NLD; k++) {
-        const int i = threadIdx.x + 256 * k;
-        if (i < NIN) {
-            const int r = i / IW, cc = i - r * IW;
-            tin[r][cc] = (Acc)Q6<TS>::get(raw[k]);
-            if (COPY) {
-                const int yy = iy0 + r, xx = ix0 + cc;
-                if (r >= 2 && r < 2 + 2 * TH && cc >= 2 && cc < 2 + 2 * TW && yy < J.sh && xx < J.sw) {
-                    ((TC *)J.copy)[(size_t)yy * J.cp + xx] = CopyConv<TC, TS>::get(raw[k]);
-                    if constexpr (sizeof(TS) == 1)
-                        if (J.copy8)
-                            ((uint8_t *)J.copy8)[(size_t)yy * J.cp8 + xx] = (uint8_t)raw[k];
+        for (int k = 0; k < NLD; k++) {
+            const int i = threadIdx.x + 256 * k, ic = i < NIN ? i : NIN - 1;
+            const int r = ic / IW, cc = ic - r * IW;
+            raw[k] = src[(size_t)clampi(iy0 + r, 0, J.sh - 1) * J.sp + clampi(ix0 + cc, 0, J.sw - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < NLD; k++) {
+            const int i = threadIdx.x + 256 * k;
+            if (i < NIN) {
+                const int r = i / IW, cc = i - r * IW;
+                tin[r][cc] = (Acc)Q6<TS>::get(raw[k]);
+                if (COPY) {
+                    const int yy = iy0 + r, xx = ix0 + cc;
+                    if (r >= 2 && r < 2 + 2 * TH && cc >= 2 && cc < 2 + 2 * TW && yy < J.sh && xx < J.sw) {
+                        ((TC *)J.copy)[(size_t)yy * J.cp + xx] = CopyConv<TC, TS>::get(raw[k]);
+                        if constexpr (sizeof(TS) == 1)
+                            if (J.copy8)
+                                ((uint8_t *)J.copy8)[(size_t)yy * J.cp8 + xx] = (uint8_t)raw[k];
+                    }
                 }
             }
         }
