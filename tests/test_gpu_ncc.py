@@ -268,8 +268,8 @@ def test_template_update_many_matches_single(ctx):
 
 @pytest.mark.parametrize("tw,th", [(64, 64), (32, 21)])
 def test_search_split_matches_single_context(ctx, tw, th):
-    """The search with its cross terms on one context and its scoring and finish on another (overlapping other work
-    on the first) gives the records of the one-context call, field for field."""
+    """The search run on another context (ordered after this one's work so far, beside whatever this one does next)
+    gives the records of the one-context call, field for field."""
     import ctypes as C
     other = vtgpu.Context(0)
     sizes = [(1920, 1080), (640, 480), (1280, 720), (800, 600), (1920, 1080), (640, 480), (960, 540), (1024, 768),
