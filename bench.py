@@ -84,9 +84,9 @@ def pyr_bytes(w, h, levels, elem):
 
 
 class Batched:
-    """B pairs per step on two contexts (A: pyramids, template cuts and LK; S: the searches, beside LK), the results
-    of each step downloaded on one of two copy contexts while the next step runs (double-buffered pinned host
-    records)."""
+    """B pairs per step on two contexts (A: pyramids and LK; S: the template cuts and searches, reading the device
+    frames in place, beside them), the results of each step downloaded on one of two copy contexts while the next step
+    runs (double-buffered pinned host records)."""
 
     def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph, one_stream=False):
         self.v, self.B, self.graph, self.one_stream = vtgpu, batch, graph, one_stream
@@ -97,6 +97,9 @@ class Batched:
         self.frames, self.corners = d_frames, corners
         self.groups = len(d_frames) // batch
         A = self.A
+        # one-level pyramids over the device frames themselves: the template cuts and searches read the frames in
+        # place, so they need not wait for the pyramids
+        self.wraps = [(A.pyramid_wrap(f0, W, H), A.pyramid_wrap(f1, W, H)) for f0, f1 in d_frames]
         self.pyrs = [[(A.pyramid(W, H, LEVELS, vtgpu.STORE_F32_U8), A.pyramid(W, H, LEVELS, vtgpu.STORE_F32_U8))
                       for _ in range(batch)]
                      for _ in range(self.groups if graph else 1)]
@@ -115,27 +118,27 @@ class Batched:
         self.graphs = {}
 
     def enqueue(self, g, serial=False):
-        """All device work of group g's batch. Context A: the 2B pyramids, the B template cuts, then LK of all pairs;
-        context S, after the template cuts: the B searches (cell maps on the matrix cores, top 8, refinement), beside
-        LK. A waits for S at the end. serial: everything on A, one kernel after another (for timing each kernel
-        alone)."""
+        """All device work of group g's batch. Context S, forked at the step's start: the B template cuts from the
+        frames 0 and the B exhaustive searches of the frames 1 (cell maps on the matrix cores, top 8, refinement), all
+        reading the device frames in place; context A meanwhile: the 2B pyramids, then LK of all pairs, then the join
+        with S. serial: everything on A, one kernel after another (for timing each kernel alone)."""
         A, S, B = self.A, self.S, self.B
         serial = serial or self.one_stream
         pyrs = self.pyrs[g if self.graph else 0]
         fr = self.frames[g * B:(g + 1) * B]
-        A.build_many([p for pair in pyrs for p in pair], [f for pair in fr for f in pair])
+        wr = self.wraps[g * B:(g + 1) * B]
+        N = A if serial else S
+        if not serial:
+            S.wait(A)
         corners = [self.corners[(g * B + k) % UNIQ] for k in range(B)]
         if self.tmpl[0] is None:
-            self.tmpl = [A.ncc_template(pyrs[k][0], 0, *corners[k], TPL, TPL) for k in range(B)]
+            self.tmpl = [N.ncc_template(wr[k][0], 0, *corners[k], TPL, TPL) for k in range(B)]
         else:  # the B templates re-cut in one launch
-            A.ncc_templates_update(self.tmpl, [p[0] for p in pyrs], [0] * B, corners)
+            N.ncc_templates_update(self.tmpl, [w[0] for w in wr], [0] * B, corners)
         # the B exhaustive searches: one launch per stage for all of them
         regions = [(1, W, 1, H)] * B
-        if serial:
-            A.ncc_search_many([p[1] for p in pyrs], self.tmpl, regions, 1, self.founds[g], cells=self.cells)
-        else:
-            A.ncc_search_many_split(S, [p[1] for p in pyrs], self.tmpl, regions, 1, self.founds[g],
-                                    cells=self.cells)
+        N.ncc_search_many([w[1] for w in wr], self.tmpl, regions, 1, self.founds[g], cells=self.cells)
+        A.build_many([p for pair in pyrs for p in pair], [f for pair in fr for f in pair])
         A.lk_track_many([p[0] for p in pyrs], [p[1] for p in pyrs], [LEVELS - 1] * B, self.offs, self.cfg, self.xy,
                         out=self.outs[g])
         A.wait(S)

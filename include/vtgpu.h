@@ -109,6 +109,12 @@ typedef struct vtg_pyr vtg_pyr;
  * (w + 1) / 2 per step (src/kern/img.h:30-31). */
 int vtg_pyr_create(vtg_ctx *ctx, int w, int h, int levels, int storage, vtg_pyr **out);
 int vtg_pyr_destroy(vtg_ctx *ctx, vtg_pyr *p);
+/* No counterpart in the reference (whose level 0 is always tk_build's own plane): a one-level VTG_STORE_Q6 pyramid
+ * whose level 0 is the caller's u8 device frame itself, read in place and never copied, so a search of a frame
+ * (template cut, vtg_ncc_search*) can run before or beside the build of its pyramid. frame: device memory, 4-byte
+ * aligned, pitch a multiple of 4 bytes; it must outlive the pyramid and stay unchanged while work reads it. Building
+ * a wrapped pyramid is refused (VTG_EINVAL); vtg_pyr_destroy frees only its descriptor. */
+int vtg_pyr_wrap(vtg_ctx *ctx, int w, int h, const uint8_t *frame, int pitch, vtg_pyr **out);
 /* Build every level from a u8 frame (w bytes per row, `pitch` apart): level 0 = 64 x byte
  * (img_u8_to_q6_*, src/kern/img.h:16), level l = reduce(level l-1) (img_reduce_*, src/kern/img.h:30-33):
  * (sum of (1 4 6 4 1)^2 x src over the 5 x 5 at (2x, 2y), replicated border, + 128) >> 8.

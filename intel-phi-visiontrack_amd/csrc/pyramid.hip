@@ -388,6 +388,43 @@ int vtg_pyr_create(vtg_ctx *c, int w, int h, int levels, int storage, vtg_pyr **
     return pyr_new(c, w, h, levels, storage, nullptr, out);
 }
 
+int vtg_pyr_wrap(vtg_ctx *c, int w, int h, const uint8_t *frame, int pitch, vtg_pyr **out)
+{
+    if (!c || !out)
+        return VTG_EINVAL;
+    *out = nullptr;
+    if (!frame || pitch < w || (pitch & 3) || ((uintptr_t)frame & 3))
+        return fail(c, VTG_EINVAL, "vtg_pyr_wrap: a device frame with a word-aligned start and pitch >= width, a "
+                                   "multiple of 4");
+    if (w < 8 || h < 8 || w > 16384 || h > 16384)
+        return fail(c, VTG_ERANGE, "vtg_pyr_wrap: frame %dx%d outside 8..16384", w, h);
+    vtg_pyr *p = new vtg_pyr();
+    p->w = w;
+    p->h = h;
+    p->levels = 1;
+    p->storage = VTG_STORE_Q6;
+    p->wrapped = true;
+    p->lv[0] = Level{frame, w, h, pitch};
+    p->offset[0] = 0;
+    p->elem[0] = 1;
+    p->bytes = round_up(sizeof(PyrView), 256); // the block holds the descriptor only
+    hipError_t e = hipMalloc(&p->block, p->bytes);
+    if (e != hipSuccess) {
+        delete p;
+        return hip_fail(c, e, "vtg_pyr_wrap: hipMalloc");
+    }
+    p->dview = (PyrView *)p->block;
+    const PyrView v = view(p);
+    e = hipMemcpy(p->dview, &v, sizeof v, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(p->block);
+        delete p;
+        return hip_fail(c, e, "vtg_pyr_wrap: descriptor");
+    }
+    *out = p;
+    return VTG_OK;
+}
+
 int vtg_pyr_destroy(vtg_ctx *c, vtg_pyr *p)
 {
     if (!c || !p)
@@ -401,6 +438,8 @@ int vtg_pyr_destroy(vtg_ctx *c, vtg_pyr *p)
 
 static int check_flags(vtg_ctx *c, const vtg_pyr *p, int flags)
 {
+    if (p->wrapped)
+        return fail(c, VTG_EINVAL, "vtg_pyr_build: a wrapped frame is read, never built");
     if (flags & ~(VTG_BUILD_CONTRAST | VTG_BUILD_PRE_GAUSS))
         return fail(c, VTG_EINVAL, "vtg_pyr_build: unknown flags 0x%x", flags);
     if (flags && p->storage != VTG_STORE_Q6W && p->storage != VTG_STORE_F32)

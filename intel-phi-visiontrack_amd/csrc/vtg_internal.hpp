@@ -82,6 +82,7 @@ struct vtg_pyr {
     int elem[vtg::kMaxLevels];
     vtg::PyrView *dview = nullptr;           // the levels' descriptor in device memory (read by batched kernels)
     bool owned = true;                       // block from hipMalloc (else placed in a caller's memory)
+    bool wrapped = false;                    // level 0 is a caller's device frame (vtg_pyr_wrap): never built
     vtg::Level byte0{nullptr, 0, 0, 0};      // VTG_STORE_F32_U8: the u8 frame beside level 0
     size_t byte0_off = 0;
 };

@@ -168,6 +168,7 @@ def lib():
         "vtg_timing_enable": [V, I], "vtg_timing_read": [V, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "vtg_timing_reset": [V],
         "vtg_pyr_create": [V, I, I, I, I, C.POINTER(V)], "vtg_pyr_destroy": [V, V],
+        "vtg_pyr_wrap": [V, I, I, V, I, C.POINTER(V)],
         "vtg_pyr_build": [V, V, V, I, I], "vtg_pyr_build_many": [V, I, V, V, V, I],
         "vtg_pyr_build_ex": [V, V, V, I, I, I], "vtg_pyr_build_many_ex": [V, I, V, V, V, I, I],
         "vtg_img_gauss5": [V, C.POINTER(Plane), C.POINTER(Plane), I, I],
@@ -236,7 +237,7 @@ def lib():
 # every symbol include/vtgpu.h declares
 EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", "vtg_ctx_error", "vtg_ctx_stream", "vtg_alloc",
            "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_download_async", "vtg_upload_async", "vtg_host_alloc", "vtg_host_free", "vtg_timing_enable", "vtg_timing_read",
-           "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_build_ex",
+           "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_wrap", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_build_ex",
            "vtg_pyr_build_many_ex", "vtg_img_gauss5", "vtg_img_norm", "vtg_img_minmax", "vtg_pyr_level",
            "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_waves", "vtg_lk_set_ppw", "vtg_lk_set_order", "vtg_lk_set_gradient", "vtg_pyr_lk_fused", "vtg_ctx_device_error", "vtg_ncc_template", "vtg_ncc_template_destroy",
            "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_template_update_many", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_fixed", "vtg_ncc_search_many", "vtg_ncc_search_many_split",
@@ -431,6 +432,10 @@ class Context:
     # pyramids
     def pyramid(self, w, h, levels, storage=STORE_Q6):
         return Pyramid(self, w, h, levels, storage)
+
+    def pyramid_wrap(self, frame, w, h, pitch=None):
+        """A one-level Q6 pyramid whose level 0 is the device frame (DevBuf) itself, read in place (vtg_pyr_wrap)."""
+        return Pyramid(self, w, h, 1, STORE_Q6, wrap=(frame, pitch or w))
 
     def stereo_scan_many(self, left, right, level, xy, dmin, dmax):
         """The row scan of n features (xy: DevBuf of 2 n floats) at `level`; returns (n,) STEREO_SCAN_DTYPE."""
@@ -681,10 +686,14 @@ class _View(DevBuf):
 
 
 class Pyramid:
-    def __init__(self, ctx, w, h, levels, storage):
+    def __init__(self, ctx, w, h, levels, storage, wrap=None):
         self.ctx, self.w, self.h, self.levels, self.storage = ctx, w, h, levels, storage
         p = C.c_void_p()
-        ctx.check(ctx.L.vtg_pyr_create(ctx.h, w, h, levels, storage, C.byref(p)), "pyr_create")
+        if wrap is None:
+            ctx.check(ctx.L.vtg_pyr_create(ctx.h, w, h, levels, storage, C.byref(p)), "pyr_create")
+        else:
+            self.frame = wrap[0]  # kept alive: the pyramid reads it in place
+            ctx.check(ctx.L.vtg_pyr_wrap(ctx.h, w, h, wrap[0].ptr, wrap[1], C.byref(p)), "pyr_wrap")
         self.h = p
 
     def build(self, frame, on_device=False, pitch=None, flags=0):

@@ -472,3 +472,39 @@ def test_search_template_rounding(ctx, oracle, kind, tw, th):
     t.destroy()
     for p in (p0, p1, q1):
         p.destroy()
+
+
+def test_wrapped_frame_search_equals_pyramid(ctx):
+    """vtg_pyr_wrap: a one-level pyramid over the device frame itself. The template cut from it and the exhaustive
+    search of it equal those of the built pyramid, record and cell map bit for bit; building it, or wrapping a frame
+    whose pitch is not a multiple of 4, is refused."""
+    import ctypes as C
+    w, h = 640, 480
+    f0, f1, _ = vtgpu.frame_pair(w, h, 515)
+    d0, d1 = ctx.to_device(np.ascontiguousarray(f0)), ctx.to_device(np.ascontiguousarray(f1))
+    p0, p1 = ctx.pyramid(w, h, 3), ctx.pyramid(w, h, 3)
+    p0.build(d0, on_device=True)
+    p1.build(d1, on_device=True)
+    q0, q1 = ctx.pyramid_wrap(d0, w, h), ctx.pyramid_wrap(d1, w, h)
+    say("ncc search of a wrapped frame")
+    ta, tb = ctx.ncc_template(p0, 0, 200, 150, 64, 64), ctx.ncc_template(q0, 0, 200, 150, 64, 64)
+    aa, na = ta.read()
+    ab, nb = tb.read()
+    assert np.array_equal(aa, ab) and na == nb
+    ncell = ((w - 66 - 1) // 16 + 1) * ((h - 66 - 1) // 16 + 1)
+    ca, cb = ctx.alloc(12 * ncell), ctx.alloc(12 * ncell)
+    fa = ctx.ncc_search(p1, ta, 1, w, 1, h, 1, ca)
+    fb = ctx.ncc_search(q1, tb, 1, w, 1, h, 1, cb)
+    assert bytes(fa) == bytes(fb) and fa.ncc > 0.9
+    assert ca.get(np.zeros(3 * ncell, np.int32)).tobytes() == cb.get(np.zeros(3 * ncell, np.int32)).tobytes()
+    with pytest.raises(vtgpu.VtgError, match=r"\(-22\)"):
+        q1.build(d1, on_device=True)
+    L, out = vtgpu.lib(), C.c_void_p()
+    assert L.vtg_pyr_wrap(ctx.h, w, h, d0.ptr, w + 2, C.byref(out)) == -22
+    assert L.vtg_pyr_wrap(ctx.h, w, h, d0.ptr, w - 4, C.byref(out)) == -22
+    for t in (ta, tb):
+        t.destroy()
+    for p in (p0, p1, q0, q1):
+        p.destroy()
+    for x in (ca, cb, d0, d1):
+        x.free()
