@@ -116,6 +116,7 @@ class Batched:
         self.h_out = [vtgpu.PinnedArray(A, (NPTS * batch,), vtgpu.LK_OUT_DTYPE) for _ in range(2)]
         self.h_found = [vtgpu.PinnedArray(A, (self.fs * batch,), np.uint8) for _ in range(2)]
         self.graphs = {}
+        self.marks = {}
 
     def enqueue(self, g, serial=False):
         """All device work of group g's batch. Context S, forked at the step's start: the B template cuts from the
@@ -156,21 +157,25 @@ class Batched:
                 self.graphs[g] = self.A.graph_end()
 
     def launch(self, i):
-        """Step i: its group's work, then its results (every LK record, every pick) downloaded on copy context i % 2
-        into pinned buffer i % 2, no host wait."""
+        """Step i: its group's work and a mark after it on A, no host wait."""
         g = i % self.groups
         if self.graph:
             self.A.graph_launch(self.graphs[g])
         else:
             self.enqueue(g)
-        cp = self.copy[i % 2]
-        cp.wait(self.A)
-        self.outs[g].get_async(cp, self.h_out[i % 2].arr)
-        self.founds[g].get_async(cp, self.h_found[i % 2].arr)
+        self.marks[i] = self.A.mark()
 
     def finish(self, i):
-        """Wait for step i's results on the host; returns its LK records and picks."""
-        self.copy[i % 2].sync()
+        """Wait for step i's work (its mark), then download its results (every LK record, every pick) on copy
+        context i % 2 into pinned buffer i % 2 and return them. The host waits for the mark instead of the copy
+        context waiting for A on the device: a wait queued on the device holds up everything behind it on its
+        hardware queue, which the graphs' own nodes may share (measured 10 % slower)."""
+        g = i % self.groups
+        self.A.mark_wait(self.marks.pop(i))
+        cp = self.copy[i % 2]
+        self.outs[g].get_async(cp, self.h_out[i % 2].arr)
+        self.founds[g].get_async(cp, self.h_found[i % 2].arr)
+        cp.sync()
         found = (self.v.Found * self.B).from_buffer_copy(self.h_found[i % 2].arr.tobytes())
         return self.h_out[i % 2].arr, found
 

@@ -47,6 +47,12 @@ const char *vtg_ctx_error(const vtg_ctx *ctx);
  * two contexts are two streams, and this is how a caller overlaps independent stages, e.g. the NCC search on one
  * context beside LK on another once both pyramids exist. No host wait. */
 int vtg_ctx_wait(vtg_ctx *ctx, vtg_ctx *other);
+/* A point in the context's stream, for a host that waits for some of its work while later work stays queued (no
+ * counterpart in the reference, whose work is synchronous). vtg_ctx_mark records one after the work submitted so far
+ * and returns its number (1, 2, ... in order); vtg_ctx_mark_wait returns once the work before that mark has completed.
+ * The last 64 marks are exact; waiting on an older one may wait for a later point. Not under graph capture. */
+int vtg_ctx_mark(vtg_ctx *ctx, uint64_t *mark);
+int vtg_ctx_mark_wait(vtg_ctx *ctx, uint64_t mark);
 /* The context's hipStream_t, as an opaque pointer (for callers that order their own work after it). */
 void *vtg_ctx_stream(vtg_ctx *ctx);
 

@@ -172,6 +172,9 @@ int vtg_ctx_destroy(vtg_ctx *c)
     }
     for (auto e : c->free_events)
         (void)hipEventDestroy(e);
+    for (auto e : c->marks)
+        if (e)
+            (void)hipEventDestroy(e);
     if (c->scratch)
         (void)hipFree(c->scratch);
     if (c->filt)
@@ -209,6 +212,31 @@ int vtg_ctx_wait(vtg_ctx *c, vtg_ctx *other)
         VTG_HIP(c, hipEventCreateWithFlags(&other->join, hipEventDisableTiming));
     VTG_HIP(c, hipEventRecord(other->join, other->stream));
     VTG_HIP(c, hipStreamWaitEvent(c->stream, other->join, 0));
+    return VTG_OK;
+}
+
+int vtg_ctx_mark(vtg_ctx *c, uint64_t *mark)
+{
+    if (!c || !mark)
+        return VTG_EINVAL;
+    const uint64_t m = c->mark_next;
+    hipEvent_t &e = c->marks[m % vtg_ctx::kMarks];
+    if (!e)
+        VTG_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    VTG_HIP(c, hipEventRecord(e, c->stream));
+    c->mark_next = m + 1;
+    *mark = m;
+    return VTG_OK;
+}
+
+int vtg_ctx_mark_wait(vtg_ctx *c, uint64_t mark)
+{
+    if (!c)
+        return VTG_EINVAL;
+    if (mark == 0 || mark >= c->mark_next)
+        return fail(c, VTG_EINVAL, "vtg_ctx_mark_wait: mark %llu was not recorded", (unsigned long long)mark);
+    // the slot holds this mark or a later one recorded over it; either way it completes no earlier than this mark
+    VTG_HIP(c, hipEventSynchronize(c->marks[mark % vtg_ctx::kMarks]));
     return VTG_OK;
 }
 

@@ -64,6 +64,9 @@ struct vtg_ctx {
     int lk_order = 1;                        // track a launch's points in spatial order (0: in the caller's order)
     int lk_grad = 0;                         // VTG_GRAD_SOBEL or VTG_GRAD_SCHARR
     hipEvent_t join = nullptr;               // for vtg_ctx_wait
+    static constexpr int kMarks = 64;
+    hipEvent_t marks[kMarks] = {};           // vtg_ctx_mark: mark m recorded in slot m % kMarks
+    uint64_t mark_next = 1;
     void *pinned = nullptr;                  // small pinned host buffer for results
     size_t pinned_bytes = 0;
     int lk_ppw = 0;                          // LK points per wavefront where no pick applies (0: the shape's best)

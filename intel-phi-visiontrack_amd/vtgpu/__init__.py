@@ -169,6 +169,7 @@ def lib():
         "vtg_timing_reset": [V],
         "vtg_pyr_create": [V, I, I, I, I, C.POINTER(V)], "vtg_pyr_destroy": [V, V],
         "vtg_pyr_wrap": [V, I, I, V, I, C.POINTER(V)],
+        "vtg_ctx_mark": [V, C.POINTER(C.c_uint64)], "vtg_ctx_mark_wait": [V, C.c_uint64],
         "vtg_pyr_build": [V, V, V, I, I], "vtg_pyr_build_many": [V, I, V, V, V, I],
         "vtg_pyr_build_ex": [V, V, V, I, I, I], "vtg_pyr_build_many_ex": [V, I, V, V, V, I, I],
         "vtg_img_gauss5": [V, C.POINTER(Plane), C.POINTER(Plane), I, I],
@@ -237,7 +238,7 @@ def lib():
 # every symbol include/vtgpu.h declares
 EXPORTS = ["vtg_ctx_create", "vtg_ctx_destroy", "vtg_ctx_sync", "vtg_ctx_wait", "vtg_ctx_error", "vtg_ctx_stream", "vtg_alloc",
            "vtg_free", "vtg_upload", "vtg_download", "vtg_copy_d2d", "vtg_download_async", "vtg_upload_async", "vtg_host_alloc", "vtg_host_free", "vtg_timing_enable", "vtg_timing_read",
-           "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_wrap", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_build_ex",
+           "vtg_timing_reset", "vtg_pyr_create", "vtg_pyr_destroy", "vtg_pyr_wrap", "vtg_ctx_mark", "vtg_ctx_mark_wait", "vtg_pyr_build", "vtg_pyr_build_many", "vtg_pyr_build_ex",
            "vtg_pyr_build_many_ex", "vtg_img_gauss5", "vtg_img_norm", "vtg_img_minmax", "vtg_pyr_level",
            "vtg_pyr_read_level", "vtg_lk_track", "vtg_lk_track_ex", "vtg_lk_track_many", "vtg_lk_set_waves", "vtg_lk_set_ppw", "vtg_lk_set_order", "vtg_lk_set_gradient", "vtg_pyr_lk_fused", "vtg_ctx_device_error", "vtg_ncc_template", "vtg_ncc_template_destroy",
            "vtg_ncc_template_read", "vtg_ncc_template_update", "vtg_ncc_template_update_many", "vtg_ncc_search", "vtg_ncc_search_async", "vtg_ncc_search_fixed", "vtg_ncc_search_many", "vtg_ncc_search_many_split",
@@ -403,6 +404,15 @@ class Context:
     def wait(self, other):
         """Later work of this context starts after everything submitted to `other` so far."""
         self.check(self.L.vtg_ctx_wait(self.h, other.h), "wait")
+
+    def mark(self):
+        """A point after the work submitted so far; mark_wait(m) returns once that work has completed."""
+        m = C.c_uint64()
+        self.check(self.L.vtg_ctx_mark(self.h, C.byref(m)), "mark")
+        return m.value
+
+    def mark_wait(self, m):
+        self.check(self.L.vtg_ctx_mark_wait(self.h, m), "mark_wait")
 
     # memory
     def alloc(self, nbytes):

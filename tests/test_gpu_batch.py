@@ -215,3 +215,45 @@ def test_download_async_into_pinned(ctx):
     pin.free()
     d.free()
     other.close()
+
+
+def test_marks(ctx):
+    """vtg_ctx_mark / vtg_ctx_mark_wait: marks number 1, 2, ... in order; after waiting for a mark the work before it
+    is complete (a pyramid built and LK's records written equal a fully synchronised run) while later work may still
+    be queued; marks never recorded are refused; a mark older than the ring of 64 still waits correctly."""
+    f0, f1, _ = vtgpu.frame_pair(1280, 720, 41)
+    p0, p1 = ctx.pyramid(1280, 720, 4), ctx.pyramid(1280, 720, 4)
+    xs, ys = vtgpu.grid(60, 30, 1280, 720)
+    cfg = vtgpu.LkCfg(21, 21, 10, 0.01, 1e-4, 40.0, 0)
+    say("marks")
+    p0.build(f0)
+    p1.build(f1)
+    ref = ctx.lk_track_host(p0, p1, 3, cfg, xs, ys)
+    dxy = ctx.to_device(np.stack([xs, ys], 1).ravel().astype(np.float32))
+    out = ctx.alloc(32 * len(xs))
+    m0 = ctx.mark()
+    ctx.lk_track_many([p0], [p1], [3], [0, len(xs)], cfg, dxy, out=out)
+    m1 = ctx.mark()
+    assert m1 == m0 + 1
+    later = ctx.alloc(32 * len(xs))
+    ctx.lk_track_many([p0], [p1], [3], [0, len(xs)], cfg, dxy, out=later)  # later work, maybe still queued
+    ctx.mark_wait(m1)
+    pin = vtgpu.PinnedArray(ctx, (len(xs),), vtgpu.LK_OUT_DTYPE)
+    other = vtgpu.Context(0)
+    out.get_async(other, pin.arr)
+    other.sync()
+    assert pin.arr.tobytes() == ref.tobytes()
+    for bad in (0, ctx.mark() + 1):
+        with pytest.raises(vtgpu.VtgError, match=r"\(-22\)"):
+            ctx.mark_wait(bad)
+    first = ctx.mark()
+    for _ in range(70):
+        ctx.mark()
+    ctx.mark_wait(first)
+    ctx.sync()
+    pin.free()
+    for b in (out, later, dxy):
+        b.free()
+    p0.destroy()
+    p1.destroy()
+    other.close()
