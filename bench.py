@@ -9,13 +9,14 @@ the host.
 Two forms run, on the same inputs:
   plain    the reference's fp32 Q6 planes, one pair at a time, one launch per pyramid level per frame, the NCC map
            in the reference's own fp32 arithmetic (the straightforward port);
-  batched  the headline: B pairs per step, exact u8/u16 Q6 pyramids built with one launch per level for all 2B
-           pyramids, one LK launch for all B pairs, the B searches on a second stream overlapping LK, the step
-           captured once as a HIP graph and replayed.
+  batched  the headline: B pairs per step, exact fp32 Q6 pyramids built with one launch per level for all 2B
+           pyramids, one LK launch for all B pairs, the B searches on a second stream overlapping the pyramids and
+           LK, each step's launches issued as they are (the same step replayed as captured HIP graphs is measured
+           beside it: `graph_replay`).
 A step of the batched form is one batch of B pairs; `value` is pairs per second. Inputs rotate over more than the
 256 MB Infinity Cache. Prints one JSON line; details go to stderr.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-graph] [--cpu-seconds S] [--batch-steps N] [--serve-frames F]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--graph] [--cpu-seconds S] [--batch-steps N] [--serve-frames F]
 """
 import argparse
 import ctypes as C
@@ -88,7 +89,7 @@ class Batched:
     frames in place, beside them), the results of each step downloaded on one of two copy contexts while the next step
     runs (double-buffered pinned host records)."""
 
-    def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph, one_stream=False):
+    def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph, one_stream=False, pyr_sets=0):
         self.v, self.B, self.graph, self.one_stream = vtgpu, batch, graph, one_stream
         self.A, self.S = vtgpu.Context(0), vtgpu.Context(0)
         # the points are a raster grid, already in spatial order: no reordering launch
@@ -102,7 +103,7 @@ class Batched:
         self.wraps = [(A.pyramid_wrap(f0, W, H), A.pyramid_wrap(f1, W, H)) for f0, f1 in d_frames]
         self.pyrs = [[(A.pyramid(W, H, LEVELS, vtgpu.STORE_F32_U8), A.pyramid(W, H, LEVELS, vtgpu.STORE_F32_U8))
                       for _ in range(batch)]
-                     for _ in range(self.groups if graph else 1)]
+                     for _ in range(pyr_sets or (self.groups if graph else 1))]
         self.xy = A.to_device(np.tile(np.stack([xs, ys], 1).ravel(), batch))
         # each group's own records, so a step's download never races the next step's writes
         self.outs = [A.alloc(32 * NPTS * batch) for _ in range(self.groups)]
@@ -125,7 +126,7 @@ class Batched:
         with S. serial: everything on A, one kernel after another (for timing each kernel alone)."""
         A, S, B = self.A, self.S, self.B
         serial = serial or self.one_stream
-        pyrs = self.pyrs[g if self.graph else 0]
+        pyrs = self.pyrs[g % len(self.pyrs)]
         fr = self.frames[g * B:(g + 1) * B]
         wr = self.wraps[g * B:(g + 1) * B]
         N = A if serial else S
@@ -178,6 +179,22 @@ class Batched:
         cp.sync()
         found = (self.v.Found * self.B).from_buffer_copy(self.h_found[i % 2].arr.tobytes())
         return self.h_out[i % 2].arr, found
+
+    def close(self):
+        """Everything the steps own, then the contexts (the device frames are the caller's)."""
+        self.A.sync()
+        self.S.sync()
+        for g in self.graphs.values():
+            self.A.graph_destroy(g)
+        for p in [p for ps in self.pyrs for pair in ps for p in pair] + [p for pair in self.wraps for p in pair]:
+            p.destroy()
+        for t in self.tmpl:
+            if t is not None:
+                t.destroy()
+        for b in [self.xy] + self.outs + self.cells + self.founds + self.h_out + self.h_found:
+            b.free()
+        for c in [self.A, self.S] + self.copy:
+            c.close()
 
 
 def plain_rate(vtgpu, d_frames, corners, xs, ys, steps):
@@ -563,9 +580,44 @@ def config1(o, cores, reps=7):
     return res
 
 
+def timed_steps(bt, steps, warmup):
+    """warmup untimed steps, then steps timed: step i+1 is launched before step i's results are waited for, so
+    downloads overlap the next step. Returns the seconds and each step's latency in ms (the launch of a step to its
+    results on the host)."""
+    for i in range(warmup):
+        bt.launch(i)
+        bt.finish(i)
+    lat, ts = [], []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ts.append(time.perf_counter())
+        bt.launch(i)
+        if i >= 1:
+            bt.finish(i - 1)
+            lat.append(time.perf_counter() - ts[i - 1])
+    bt.finish(steps - 1)
+    lat.append(time.perf_counter() - ts[steps - 1])
+    return time.perf_counter() - t0, np.array(lat) * 1e3
+
+
+def other_form(vtgpu, d_frames, corners, a, xs, ys):
+    """The headline step in its other launch form (captured graphs replayed, or with --graph the launches issued as
+    they are), same inputs, a.graph_steps steps after 3 untimed ones."""
+    bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=not a.graph, one_stream=a.one_stream,
+                 pyr_sets=a.pyr_sets)
+    bt.prepare()
+    total, lat_ms = timed_steps(bt, a.graph_steps, 3)
+    r = {"graph": not a.graph, "steps": a.graph_steps, "pairs_per_s": round(a.graph_steps * a.batch / total, 2),
+         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
+         "late_steps": int((lat_ms > np.median(lat_ms) + 0.5).sum())}
+    bt.close()
+    log(f"other launch form: {r}")
+    return r
+
+
 def pcie_rate(vtgpu, bt, f0s, f1s, steps):
     """The batched step with its frames arriving in pinned host memory: each step's 2B frames are uploaded over PCIe on
-    an upload context while the previous step computes, and the step's graph waits for its upload. Pairs per second
+    an upload context while the previous step computes, and the step waits for its upload. Pairs per second
     including the uploads."""
     B = bt.B
     up = [vtgpu.Context(0), vtgpu.Context(0)]
@@ -714,9 +766,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=8, help="frame pairs per step of the batched form")
     ap.add_argument("--pairs", type=int, default=72, help="device-resident frame pairs rotated through")
-    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
+    ap.add_argument("--graph", action="store_true", help="the headline steps replay captured HIP graphs instead of "
+                    "issuing their launches")
+    ap.add_argument("--graph-steps", type=int, default=30,
+                    help="steps of the other form (graph replay, or issued launches with --graph) measured beside the "
+                         "headline (0: skip)")
     ap.add_argument("--one-stream", action="store_true",
                     help="the searches on the LK stream too (no fork and join inside the step)")
+    ap.add_argument("--pyr-sets", type=int, default=0, help="pyramid sets the steps rotate through (0: one per "
+                    "frame group with graphs, one without)")
     ap.add_argument("--plain-steps", type=int, default=8)
     ap.add_argument("--batch-steps", type=int, default=20, help="graph replays of the uneven 64-pair batch (0: skip)")
     ap.add_argument("--serve-frames", type=int, default=300, help="frames per stream of the serving runs (0: skip)")
@@ -754,27 +812,11 @@ def main():
     d_frames = [(up.to_device(f0s[i % UNIQ]), up.to_device(f1s[i % UNIQ])) for i in range(pairs)]
     up.sync()
 
-    bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=not a.no_graph, one_stream=a.one_stream)
+    bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=a.graph, one_stream=a.one_stream,
+                 pyr_sets=a.pyr_sets)
     bt.prepare()
-    for i in range(a.warmup):
-        bt.launch(i)
-        bt.finish(i)
-    # timed: step i+1 is launched before step i's results are waited for, so downloads overlap the next step;
-    # latency = launch of a step to its results on the host
-    lat = []
-    t0 = time.perf_counter()
-    ts = []
-    for i in range(a.steps):
-        ts.append(time.perf_counter())
-        bt.launch(i)
-        if i >= 1:
-            bt.finish(i - 1)
-            lat.append(time.perf_counter() - ts[i - 1])
-    bt.finish(a.steps - 1)
-    lat.append(time.perf_counter() - ts[a.steps - 1])
-    total = time.perf_counter() - t0
+    total, lat_ms = timed_steps(bt, a.steps, a.warmup)
     pairs_per_s = a.steps * a.batch / total
-    lat_ms = np.array(lat) * 1e3
     # per-pair algorithmic counts from each pair's own LK records (group 0 holds pairs 0..B-1)
     bt.launch(0)
     h_out, h_found = bt.finish(0)
@@ -830,6 +872,7 @@ def main():
                     "ratio": round(plain_detail["stage_ms_per_pair"][s] / v, 2) if v > 0 else None}
                 for s, v in bstage.items()}
     pcie = pcie_rate(vtgpu, bt, f0s, f1s, a.steps) if a.pcie_steps else {}
+    other = other_form(vtgpu, d_frames, corners, a, xs, ys) if a.graph_steps > 0 else {}
     batch = batching(vtgpu, a.batch_steps) if a.batch_steps > 0 else {}
     serve = serving(vtgpu, a.serve_frames) if a.serve_frames > 0 else {}
     still = still_bench(vtgpu, a.still_frames) if a.still_frames > 0 else {}
@@ -850,12 +893,13 @@ def main():
                    "storage": "f32_u8 (fp32 Q6 levels: exact integers; the u8 frame kept for the NCC search; the "
                               "level-1 reduce also writes the fp32 level 0 and the kept frame: 5 B per level-0 pixel "
                               "beyond the pyramid's algorithmic bytes, which LK's fp32 reads repay)",
-                   "lk_points_per_wavefront": 3, "graph": not a.no_graph,
+                   "lk_points_per_wavefront": 3, "graph": a.graph,
                    "parallelism": "1 GPU, 1 stream" if a.one_stream else "1 GPU, 2 streams",
                    "input_rotation": f"{pairs} device frame pairs ({pairs * 2 * W * H / 2 ** 20:.0f} MiB)"},
         "plain_pairs_per_s": round(plain, 2), "vs_plain": round(pairs_per_s / plain, 2),
         "vs_plain_by_stage": by_stage, "plain_ncc_map_kernel": plain_detail["ncc_map_kernel"],
         "pcie_inclusive": pcie,
+        "other_launch_form": other,
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4), "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "latency_unit": f"one step = {a.batch} pairs, submit to results on the host",
         "lk_pts_per_s": round(a.batch * NPTS / (lk_us * 1e-6), 1),
