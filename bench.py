@@ -89,7 +89,8 @@ class Batched:
     frames in place, beside them), the results of each step downloaded on one of two copy contexts while the next step
     runs (double-buffered pinned host records)."""
 
-    def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph, one_stream=False, pyr_sets=0):
+    def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph, one_stream=False, pyr_sets=0,
+                 storage=None):
         self.v, self.B, self.graph, self.one_stream = vtgpu, batch, graph, one_stream
         self.A, self.S = vtgpu.Context(0), vtgpu.Context(0)
         # the points are a raster grid, already in spatial order: no reordering launch
@@ -101,7 +102,8 @@ class Batched:
         # one-level pyramids over the device frames themselves: the template cuts and searches read the frames in
         # place, so they need not wait for the pyramids
         self.wraps = [(A.pyramid_wrap(f0, W, H), A.pyramid_wrap(f1, W, H)) for f0, f1 in d_frames]
-        self.pyrs = [[(A.pyramid(W, H, LEVELS, vtgpu.STORE_F32_U8), A.pyramid(W, H, LEVELS, vtgpu.STORE_F32_U8))
+        st = vtgpu.STORE_F32 if storage is None else storage
+        self.pyrs = [[(A.pyramid(W, H, LEVELS, st), A.pyramid(W, H, LEVELS, st))
                       for _ in range(batch)]
                      for _ in range(pyr_sets or (self.groups if graph else 1))]
         self.xy = A.to_device(np.tile(np.stack([xs, ys], 1).ravel(), batch))
@@ -580,6 +582,10 @@ def config1(o, cores, reps=7):
     return res
 
 
+def store_of(vtgpu, name):
+    return {"f32": vtgpu.STORE_F32, "f32_u8": vtgpu.STORE_F32_U8}[name]
+
+
 def timed_steps(bt, steps, warmup):
     """warmup untimed steps, then steps timed: step i+1 is launched before step i's results are waited for, so
     downloads overlap the next step. Returns the seconds and each step's latency in ms (the launch of a step to its
@@ -604,7 +610,7 @@ def other_form(vtgpu, d_frames, corners, a, xs, ys):
     """The headline step in its other launch form (captured graphs replayed, or with --graph the launches issued as
     they are), same inputs, a.graph_steps steps after 3 untimed ones."""
     bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=not a.graph, one_stream=a.one_stream,
-                 pyr_sets=a.pyr_sets)
+                 pyr_sets=a.pyr_sets, storage=store_of(vtgpu, a.storage))
     bt.prepare()
     total, lat_ms = timed_steps(bt, a.graph_steps, 3)
     r = {"graph": not a.graph, "steps": a.graph_steps, "pairs_per_s": round(a.graph_steps * a.batch / total, 2),
@@ -773,6 +779,8 @@ def main():
                          "headline (0: skip)")
     ap.add_argument("--one-stream", action="store_true",
                     help="the searches on the LK stream too (no fork and join inside the step)")
+    ap.add_argument("--storage", choices=["f32", "f32_u8"], default="f32",
+                    help="level storage of the batched step's pyramids (f32_u8 also keeps the u8 frame beside level 0)")
     ap.add_argument("--pyr-sets", type=int, default=0, help="pyramid sets the steps rotate through (0: one per "
                     "frame group with graphs, one without)")
     ap.add_argument("--plain-steps", type=int, default=8)
@@ -813,7 +821,7 @@ def main():
     up.sync()
 
     bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=a.graph, one_stream=a.one_stream,
-                 pyr_sets=a.pyr_sets)
+                 pyr_sets=a.pyr_sets, storage=store_of(vtgpu, a.storage))
     bt.prepare()
     total, lat_ms = timed_steps(bt, a.steps, a.warmup)
     pairs_per_s = a.steps * a.batch / total
@@ -890,9 +898,11 @@ def main():
                                "per pair", "w": W, "h": H, "levels": LEVELS, "points_per_pair": NPTS,
                    "window": f"{WIN}x{WIN}", "ncc_template": f"{TPL}x{TPL}", "ncc_corners": npos,
                    "pairs_per_step": a.batch,
-                   "storage": "f32_u8 (fp32 Q6 levels: exact integers; the u8 frame kept for the NCC search; the "
-                              "level-1 reduce also writes the fp32 level 0 and the kept frame: 5 B per level-0 pixel "
-                              "beyond the pyramid's algorithmic bytes, which LK's fp32 reads repay)",
+                   "storage": {"f32": "f32 (fp32 Q6 levels: exact integers; the searches read the u8 frames in "
+                                       "place; the level-1 reduce also writes the fp32 level 0: 4 B per level-0 pixel "
+                                       "beyond the pyramid's algorithmic bytes, which LK's fp32 reads repay)",
+                               "f32_u8": "f32_u8 (as f32, and the u8 frame kept beside level 0: 5 B per level-0 "
+                                         "pixel beyond the pyramid's algorithmic bytes)"}[a.storage],
                    "lk_points_per_wavefront": 3, "graph": a.graph,
                    "parallelism": "1 GPU, 1 stream" if a.one_stream else "1 GPU, 2 streams",
                    "input_rotation": f"{pairs} device frame pairs ({pairs * 2 * W * H / 2 ** 20:.0f} MiB)"},
