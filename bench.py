@@ -12,7 +12,7 @@ Two forms run, on the same inputs:
   batched  the headline: B pairs per step, exact fp32 Q6 pyramids built with one launch per level for all 2B
            pyramids, one LK launch for all B pairs, the B searches on a second stream overlapping the pyramids and
            LK, each step's launches issued as they are (the same step replayed as captured HIP graphs is measured
-           beside it: `graph_replay`).
+           beside it: `other_launch_form`).
 A step of the batched form is one batch of B pairs; `value` is pairs per second. Inputs rotate over more than the
 256 MB Infinity Cache. Prints one JSON line; details go to stderr.
 
@@ -87,11 +87,15 @@ def pyr_bytes(w, h, levels, elem):
 class Batched:
     """B pairs per step on two contexts (A: pyramids and LK; S: the template cuts and searches, reading the device
     frames in place, beside them), the results of each step downloaded on one of two copy contexts while the next step
-    runs (double-buffered pinned host records)."""
+    runs (double-buffered pinned host records). Issued launches need no order between A and S: the two share no
+    buffer, each runs its steps in its own order, and the host waits for a mark on each before a step's download.
+    Captured graphs fork S from A at each step's start and join it at the end."""
 
     def __init__(self, vtgpu, d_frames, corners, batch, xs, ys, graph, one_stream=False, pyr_sets=0,
-                 storage=None):
+                 storage=None, join=False):
         self.v, self.B, self.graph, self.one_stream = vtgpu, batch, graph, one_stream
+        # a captured graph's second branch must fork from A and join it again; issued launches need neither
+        self.join = graph or join
         self.A, self.S = vtgpu.Context(0), vtgpu.Context(0)
         # the points are a raster grid, already in spatial order: no reordering launch
         self.A.check(self.A.L.vtg_lk_set_order(self.A.h, 0))
@@ -122,17 +126,17 @@ class Batched:
         self.marks = {}
 
     def enqueue(self, g, serial=False):
-        """All device work of group g's batch. Context S, forked at the step's start: the B template cuts from the
-        frames 0 and the B exhaustive searches of the frames 1 (cell maps on the matrix cores, top 8, refinement), all
-        reading the device frames in place; context A meanwhile: the 2B pyramids, then LK of all pairs, then the join
-        with S. serial: everything on A, one kernel after another (for timing each kernel alone)."""
+        """All device work of group g's batch. Context S: the B template cuts from the frames 0 and the B exhaustive
+        searches of the frames 1 (cell maps on the matrix cores, top 8, refinement), all reading the device frames in
+        place; context A beside it: the 2B pyramids, then LK of all pairs (with join: S forked from A at the start and
+        joined to it at the end). serial: everything on A, one kernel after another (for timing each kernel alone)."""
         A, S, B = self.A, self.S, self.B
         serial = serial or self.one_stream
         pyrs = self.pyrs[g % len(self.pyrs)]
         fr = self.frames[g * B:(g + 1) * B]
         wr = self.wraps[g * B:(g + 1) * B]
         N = A if serial else S
-        if not serial:
+        if not serial and self.join:
             S.wait(A)
         corners = [self.corners[(g * B + k) % UNIQ] for k in range(B)]
         if self.tmpl[0] is None:
@@ -145,7 +149,8 @@ class Batched:
         A.build_many([p for pair in pyrs for p in pair], [f for pair in fr for f in pair])
         A.lk_track_many([p[0] for p in pyrs], [p[1] for p in pyrs], [LEVELS - 1] * B, self.offs, self.cfg, self.xy,
                         out=self.outs[g])
-        A.wait(S)
+        if not serial and self.join:
+            A.wait(S)
 
     def prepare(self):
         """Grow scratch and templates eagerly once per group, then capture each group's step."""
@@ -159,14 +164,19 @@ class Batched:
                 self.enqueue(g)
                 self.graphs[g] = self.A.graph_end()
 
-    def launch(self, i):
-        """Step i: its group's work and a mark after it on A, no host wait."""
+    def launch(self, i, after=None):
+        """Step i: its group's work and a mark after it on each context it ran on, no host wait. after: a context
+        whose work so far the step's work waits for (the frames' upload)."""
         g = i % self.groups
+        if after is not None:
+            self.A.wait(after)
+            if not self.join:
+                self.S.wait(after)
         if self.graph:
             self.A.graph_launch(self.graphs[g])
         else:
             self.enqueue(g)
-        self.marks[i] = self.A.mark()
+        self.marks[i] = (self.A.mark(), None if self.join or self.one_stream else self.S.mark())
 
     def finish(self, i):
         """Wait for step i's work (its mark), then download its results (every LK record, every pick) on copy
@@ -174,7 +184,10 @@ class Batched:
         context waiting for A on the device: a wait queued on the device holds up everything behind it on its
         hardware queue, which the graphs' own nodes may share (measured 10 % slower)."""
         g = i % self.groups
-        self.A.mark_wait(self.marks.pop(i))
+        ma, ms = self.marks.pop(i)
+        self.A.mark_wait(ma)
+        if ms is not None:
+            self.S.mark_wait(ms)
         cp = self.copy[i % 2]
         self.outs[g].get_async(cp, self.h_out[i % 2].arr)
         self.founds[g].get_async(cp, self.h_found[i % 2].arr)
@@ -610,7 +623,7 @@ def other_form(vtgpu, d_frames, corners, a, xs, ys):
     """The headline step in its other launch form (captured graphs replayed, or with --graph the launches issued as
     they are), same inputs, a.graph_steps steps after 3 untimed ones."""
     bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=not a.graph, one_stream=a.one_stream,
-                 pyr_sets=a.pyr_sets, storage=store_of(vtgpu, a.storage))
+                 pyr_sets=a.pyr_sets, storage=store_of(vtgpu, a.storage), join=a.join)
     bt.prepare()
     total, lat_ms = timed_steps(bt, a.graph_steps, 3)
     r = {"graph": not a.graph, "steps": a.graph_steps, "pairs_per_s": round(a.graph_steps * a.batch / total, 2),
@@ -646,14 +659,12 @@ def pcie_rate(vtgpu, bt, f0s, f1s, steps):
     upload(0)
     up[0].sync()
     for i in range(3):  # warm
-        bt.A.wait(up[i % 2])
-        bt.launch(i)
+        bt.launch(i, after=up[i % 2])
         upload(i + 1)
         bt.finish(i)
     t0 = time.perf_counter()
     for i in range(steps):
-        bt.A.wait(up[i % 2])
-        bt.launch(i)
+        bt.launch(i, after=up[i % 2])
         upload(i + 1)
         if i >= 1:
             bt.finish(i - 1)
@@ -777,6 +788,8 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=30,
                     help="steps of the other form (graph replay, or issued launches with --graph) measured beside the "
                          "headline (0: skip)")
+    ap.add_argument("--join", action="store_true", help="issued launches: fork the searches' context from the LK "
+                    "context at each step's start and join it at the end, as a captured graph must")
     ap.add_argument("--one-stream", action="store_true",
                     help="the searches on the LK stream too (no fork and join inside the step)")
     ap.add_argument("--storage", choices=["f32", "f32_u8"], default="f32",
@@ -821,7 +834,7 @@ def main():
     up.sync()
 
     bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=a.graph, one_stream=a.one_stream,
-                 pyr_sets=a.pyr_sets, storage=store_of(vtgpu, a.storage))
+                 pyr_sets=a.pyr_sets, storage=store_of(vtgpu, a.storage), join=a.join)
     bt.prepare()
     total, lat_ms = timed_steps(bt, a.steps, a.warmup)
     pairs_per_s = a.steps * a.batch / total
@@ -903,7 +916,7 @@ def main():
                                        "beyond the pyramid's algorithmic bytes, which LK's fp32 reads repay)",
                                "f32_u8": "f32_u8 (as f32, and the u8 frame kept beside level 0: 5 B per level-0 "
                                          "pixel beyond the pyramid's algorithmic bytes)"}[a.storage],
-                   "lk_points_per_wavefront": 3, "graph": a.graph,
+                   "lk_points_per_wavefront": 3, "graph": a.graph, "fork_join": a.graph or a.join,
                    "parallelism": "1 GPU, 1 stream" if a.one_stream else "1 GPU, 2 streams",
                    "input_rotation": f"{pairs} device frame pairs ({pairs * 2 * W * H / 2 ** 20:.0f} MiB)"},
         "plain_pairs_per_s": round(plain, 2), "vs_plain": round(pairs_per_s / plain, 2),
