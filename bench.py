@@ -152,6 +152,15 @@ class Batched:
         if not serial and self.join:
             A.wait(S)
 
+    def tune(self):
+        """The autotuner on pair 0 of the workload: its picks (reduce tile height, LK schedule) for the 1080p class
+        apply to every later build and LK launch of that class on context A."""
+        p0, p1 = self.pyrs[0][0]
+        f0, f1 = self.frames[0]
+        t = self.A.tune_pair(p0, p1, f0, f1, LEVELS - 1, self.cfg, self.xy)
+        log(f"autotuner picks for the step's class: {t}")
+        return t
+
     def prepare(self):
         """Grow scratch and templates eagerly once per group, then capture each group's step."""
         for g in range(self.groups):
@@ -790,6 +799,8 @@ def main():
                          "headline (0: skip)")
     ap.add_argument("--join", action="store_true", help="issued launches: fork the searches' context from the LK "
                     "context at each step's start and join it at the end, as a captured graph must")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner on pair 0 before the timed steps and use "
+                    "its picks")
     ap.add_argument("--one-stream", action="store_true",
                     help="the searches on the LK stream too (no fork and join inside the step)")
     ap.add_argument("--storage", choices=["f32", "f32_u8"], default="f32",
@@ -835,6 +846,7 @@ def main():
 
     bt = Batched(vtgpu, d_frames, corners, a.batch, xs, ys, graph=a.graph, one_stream=a.one_stream,
                  pyr_sets=a.pyr_sets, storage=store_of(vtgpu, a.storage), join=a.join)
+    tuned = bt.tune() if a.tune else None
     bt.prepare()
     total, lat_ms = timed_steps(bt, a.steps, a.warmup)
     pairs_per_s = a.steps * a.batch / total
@@ -917,6 +929,7 @@ def main():
                                "f32_u8": "f32_u8 (as f32, and the u8 frame kept beside level 0: 5 B per level-0 "
                                          "pixel beyond the pyramid's algorithmic bytes)"}[a.storage],
                    "lk_points_per_wavefront": 3, "graph": a.graph, "fork_join": a.graph or a.join,
+                   "autotuned": tuned,
                    "parallelism": "1 GPU, 1 stream" if a.one_stream else "1 GPU, 2 streams",
                    "input_rotation": f"{pairs} device frame pairs ({pairs * 2 * W * H / 2 ** 20:.0f} MiB)"},
         "plain_pairs_per_s": round(plain, 2), "vs_plain": round(pairs_per_s / plain, 2),
