@@ -257,3 +257,77 @@ def test_marks(ctx):
     p0.destroy()
     p1.destroy()
     other.close()
+
+
+def test_two_contexts_unordered(ctx):
+    """The batched step as bench.py issues it: context B cuts the templates and runs the searches on one-level
+    pyramids over the device frames, context A builds the pyramids and runs LK, with nothing ordering A and B (they
+    share no buffer), steps issued back to back over two groups of frames, and the host waiting for a mark on each
+    context before reading a step's results. Every step's LK records and picks equal the same calls made one after
+    another on a single context."""
+    W_, H_, L, B, T = 640, 480, 3, 2, 32
+    groups = []
+    for g in range(2):
+        fr = []
+        for k in range(B):
+            f0, f1, _ = vtgpu.frame_pair(W_, H_, 300 + 10 * g + k)
+            fr.append((ctx.to_device(f0), ctx.to_device(f1)))
+        groups.append(fr)
+    xs, ys = vtgpu.grid(40, 30, W_, H_)
+    n = len(xs)
+    xy = ctx.to_device(np.tile(np.stack([xs, ys], 1).ravel(), B).astype(np.float32))
+    cfg = vtgpu.LkCfg(21, 21, 10, 0.01, 1e-4, 40.0, 0)
+    corners = [(100 + 37 * k, 80 + 23 * k) for k in range(B)]
+    fs = C.sizeof(vtgpu.Found)
+
+    def run(A, S, steps):
+        wraps = [[(A.pyramid_wrap(f0, W_, H_), A.pyramid_wrap(f1, W_, H_)) for f0, f1 in fr] for fr in groups]
+        pyrs = [(A.pyramid(W_, H_, L, vtgpu.STORE_F32), A.pyramid(W_, H_, L, vtgpu.STORE_F32)) for _ in range(B)]
+        tm = [S.ncc_template(wraps[0][k][0], 0, *corners[k], T, T) for k in range(B)]
+        S.sync()
+        outs = [A.alloc(32 * n * B) for _ in range(steps)]
+        founds = [A.alloc(fs * B) for _ in range(steps)]
+        marks = []
+        for i in range(steps):
+            g = i % 2
+            S.ncc_templates_update(tm, [w[0] for w in wraps[g]], [0] * B, corners)
+            S.ncc_search_many([w[1] for w in wraps[g]], tm, [(1, W_ - T, 1, H_ - T)] * B, 1, founds[i])
+            A.build_many([p for pair in pyrs for p in pair], [f for pair in groups[g] for f in pair])
+            A.lk_track_many([p[0] for p in pyrs], [p[1] for p in pyrs], [L - 1] * B, [0, n, 2 * n], cfg, xy,
+                            out=outs[i])
+            marks.append((A.mark(), S.mark()))
+        res = []
+        for i in range(steps):
+            A.mark_wait(marks[i][0])
+            S.mark_wait(marks[i][1])
+            rec = np.zeros(n * B, vtgpu.LK_OUT_DTYPE)
+            outs[i].get(rec)
+            fb = np.zeros(fs * B, np.uint8)
+            founds[i].get(fb)
+            res.append((rec, fb))
+        A.sync()
+        S.sync()
+        for b in outs + founds:
+            b.free()
+        for t in tm:
+            t.destroy()
+        for p in [p for pair in pyrs for p in pair] + [p for ws in wraps for pair in ws for p in pair]:
+            p.destroy()
+        return res
+
+    say("two contexts, unordered, 4 steps over 2 frame groups")
+    B2 = vtgpu.Context(0)
+    got = run(ctx, B2, 4)
+    ref = run(ctx, ctx, 4)
+    for i, ((r1, f1), (r2, f2)) in enumerate(zip(got, ref)):
+        assert r1.tobytes() == r2.tobytes(), f"step {i}: LK records differ"
+        assert f1.tobytes() == f2.tobytes(), f"step {i}: picks differ"
+        assert (r1["status"] == 0).mean() > 0.5
+    # the two groups' steps differ (the check is not vacuous)
+    assert got[0][0].tobytes() != got[1][0].tobytes()
+    B2.close()
+    xy.free()
+    for fr in groups:
+        for f0, f1 in fr:
+            f0.free()
+            f1.free()
