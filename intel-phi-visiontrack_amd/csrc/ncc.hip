@@ -733,9 +733,11 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 // cores (xcorr_block), then every corner's box sums and score in the same workgroup, so the cross terms never leave
 // the chip. After one barrier (the template's space becomes the stashes) each wave works alone on the 64 x 64 corners
 // whose cross terms it holds: lane l owns corner column l of them and, for its box sums, byte columns l and l + 64 of
-// the wave's 127; it slides their column sums (sum u, sum u^2 over th rows) down the frame tile in LDS, and four DPP
-// prefix scans per corner row give every lane its box (columns l .. l + 63: the first half's total minus its prefix
-// before l, plus the second half's prefix before l) with no workgroup barrier. The wave moves its accumulators to its
+// the wave's 127. Its box (columns l .. l + 63) is the first half's total plus the sum over the columns k < l of
+// (column l + 64's sum - column k's): the lane slides that difference of its two column sums (sum u, sum u^2 over th
+// rows) down the frame tile in LDS, one DPP prefix scan per sum and corner row gives the sum over k < l, and the
+// first half's total slides down the row sums of the wave's first 64 byte columns, which the workgroup computes once
+// per tile row (v_dot4_u32_u8); no workgroup barrier. The wave moves its accumulators to its
 // own stash 8 corner rows at a time, so lane l reads corner column l's cross term. With su, sq the box sums, u0 the
 // window's first byte and n = tw th, the score is the header's identity on exact integers: sum u u_t = cross + 128 (su
 // + sum u_t) - 16384 n, st = 4096 (sum u u_t - u0 sum u_t) - 64 mean (su - n u0), var = 4096 (sq - 2 u0 su + n u0^2)
@@ -750,12 +752,18 @@ constexpr int XC_SR = 8;             // corner rows a wave stashes at a time
 constexpr int XC_SS = 64 + 4;        // stash row stride in words: 64 corners and a pad
 constexpr int XC_SW = XC_SR * XC_SS; // stash words per wave
 
-__host__ __device__ constexpr size_t xcells_lds_bytes(int th)
+__host__ __device__ constexpr size_t xcells_rs_offset(int th)
 {
     // the tile, then the larger of the template and the four waves' stashes
     return (size_t)(YBK + xcorr_th(th) - 1) * XS +
            ((size_t)(xcorr_th(th) + 30) * TS > (size_t)4 * XC_SW * 4 ? (size_t)(xcorr_th(th) + 30) * TS
                                                                       : (size_t)4 * XC_SW * 4);
+}
+
+__host__ __device__ constexpr size_t xcells_lds_bytes(int th)
+{
+    // then the row sums: per tile row and wave column, sum u and sum u^2 of the wave's first 64 byte columns
+    return xcells_rs_offset(th) + (size_t)(YBK + xcorr_th(th) - 1) * 2 * 2 * 4;
 }
 
 // LDS written by some lanes of a wave, then read by others of the same wave: the wave's LDS operations complete in
@@ -793,13 +801,34 @@ __global__ __launch_bounds__(256) void ncc_xcorr_cells(SBatch b)
     const int cx = bx0 + 64 * wx + lane, x = a.x0 + cx; // this lane's corner column in the map, and in the level
     const bool x_ok = cx < pcols && x <= a.x1 && ((x - a.x0) & (stp - 1)) == 0; // step 1 or 2
     __syncthreads(); // every wave is done with the template: its space takes the stashes
-    uint32_t s1a = 0, s2a = 0, s1b = 0, s2b = 0; // column sums of the lane's two byte columns
+    // rs[2 (2 r + c) + q]: tile row r's sum u (q = 0) and sum u^2 (q = 1) over byte columns off + 64 c .. + 63
+    uint32_t *rs = (uint32_t *)((char *)lds + xcells_rs_offset(th));
+    for (int i = tid; i < 2 * TR; i += 256) {
+        const int r = i >> 1, o = off + 64 * (i & 1), sh = 8 * (o & 3);
+        const uint32_t *w = (const uint32_t *)((const uint8_t *)lds + r * XS) + (o >> 2);
+        uint32_t q1 = 0, q2 = 0;
+#pragma unroll
+        for (int k = 0; k <= 16; k++) {
+            // 17 words: the first without its bytes before o, the last with only its bytes before o + 64 (none when
+            // o is word-aligned)
+            const uint32_t m = k == 0 ? 0xffffffffu << sh : k == 16 ? (sh ? (1u << sh) - 1u : 0u) : 0xffffffffu;
+            const uint32_t u = (w[k] ^ 0x80808080u) & m;
+            q1 = __builtin_amdgcn_udot4(u, 0x01010101u, q1, false);
+            q2 = __builtin_amdgcn_udot4(u, u, q2, false);
+        }
+        rs[2 * i] = q1;
+        rs[2 * i + 1] = q2;
+    }
+    __syncthreads();
+    const uint32_t *rw = rs + 2 * wx; // this wave's row sums: rw[4 r], rw[4 r + 1]
+    // d1, d2: the lane's second column's sums minus its first column's; t1, t2: the first half's totals
+    uint32_t d1 = 0, d2 = 0, t1 = 0, t2 = 0;
     for (int r = 0; r < th; r++) {
         const uint32_t ua = ca[(yw + r) * XS] ^ 0x80u, ub = cb[(yw + r) * XS] ^ 0x80u;
-        s1a += ua;
-        s2a += ua * ua;
-        s1b += ub;
-        s2b += ub * ub;
+        d1 += ub - ua;
+        d2 += ub * ub - ua * ua;
+        t1 += rw[4 * (yw + r)];
+        t2 += rw[4 * (yw + r) + 1];
     }
     float best_v = -3.0f;
     int best_x = 0, best_y = 0;
@@ -817,18 +846,15 @@ __global__ __launch_bounds__(256) void ncc_xcorr_cells(SBatch b)
             if (h > 0 || rr > 0) {
                 const int ri = (yb + th - 1) * XS, ro = (yb - 1) * XS;
                 const uint32_t ia = ca[ri] ^ 0x80u, oa = ca[ro] ^ 0x80u, ib = cb[ri] ^ 0x80u, ob = cb[ro] ^ 0x80u;
-                s1a += ia - oa;
-                s2a += ia * ia - oa * oa;
-                s1b += ib - ob;
-                s2b += ib * ib - ob * ob;
+                d1 += (ib - ob) - (ia - oa);
+                d2 += (ib * ib - ob * ob) - (ia * ia - oa * oa);
+                t1 += rw[4 * (yb + th - 1)] - rw[4 * (yb - 1)];
+                t2 += rw[4 * (yb + th - 1) + 1] - rw[4 * (yb - 1) + 1];
             }
-            const uint32_t i1a = wave_incl_scan(s1a), i2a = wave_incl_scan(s2a);
-            const uint32_t i1b = wave_incl_scan(s1b), i2b = wave_incl_scan(s2b);
-            const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)i1a, 63);
-            const uint32_t t2 = (uint32_t)__builtin_amdgcn_readlane((int)i2a, 63);
-            // box of corner column lane: first-half columns lane .. 63, second-half columns 0 .. lane - 1
-            const uint32_t box1 = (t1 - i1a + s1a) + (i1b - s1b);
-            const uint32_t box2 = (t2 - i2a + s2a) + (i2b - s2b);
+            // box of corner column lane: first-half columns lane .. 63, second-half columns 0 .. lane - 1, i.e. the
+            // first half's total plus the exclusive prefix of the differences (the arithmetic wraps consistently)
+            const uint32_t box1 = t1 + (wave_incl_scan(d1) - d1);
+            const uint32_t box2 = t2 + (wave_incl_scan(d2) - d2);
             const int u0 = ca[yb * XS] ^ 0x80;
             const int y = a.y0 + by0 + yb;
             // exact integers (every true value fits in 31 bits; the 32-bit arithmetic wraps consistently):
