@@ -800,6 +800,7 @@ __global__ __launch_bounds__(256) void ncc_xcorr_cells(SBatch b)
     const double inv_n = 1.0 / (double)n;    // exact when POW2
     const int cx = bx0 + 64 * wx + lane, x = a.x0 + cx; // this lane's corner column in the map, and in the level
     const bool x_ok = cx < pcols && x <= a.x1 && ((x - a.x0) & (stp - 1)) == 0; // step 1 or 2
+    const int ay0 = a.y0, ay1 = a.y1;
     __syncthreads(); // every wave is done with the template: its space takes the stashes
     // rs[2 (2 r + c) + q]: tile row r's sum u (q = 0) and sum u^2 (q = 1) over byte columns off + 64 c .. + 63
     uint32_t *rs = (uint32_t *)((char *)lds + xcells_rs_offset(th));
@@ -872,8 +873,10 @@ __global__ __launch_bounds__(256) void ncc_xcorr_cells(SBatch b)
             r = fma(r, fma(-0.5 * w * r, r, 0.5), r);
             r = fma(r, fma(-0.5 * w * r, r, 0.5), r);
             const float v = (65536.0 * an <= nn || norm <= 0.0) ? 0.0f : (float)(b * r);
-            const bool ok = x_ok && y <= a.y1 && ((y - a.y0) & (stp - 1)) == 0;
-            const bool take = ok && better(v, y, x, best_v, best_y, best_x);
+            // the lane's corners come in increasing y, so an earlier one wins a tie: better() reduces to a higher
+            // score here (bitwise, so no branch; the window's bounds read once, above)
+            const bool ok = x_ok & (y <= ay1) & (((y - ay0) & (stp - 1)) == 0);
+            const bool take = ok & (v > best_v);
             best_v = take ? v : best_v;
             best_x = take ? x : best_x;
             best_y = take ? y : best_y;
