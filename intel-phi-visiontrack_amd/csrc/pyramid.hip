@@ -25,9 +25,9 @@ template <typename TS, typename TD, typename TC, bool COPY, bool EXACT, int TH>
 __global__ __launch_bounds__(256) void reduce_batch(RJobs jobs)
 {
     int jn = 0;
-    for (int k = 1; k < jobs.n; k++)
-        if ((int)blockIdx.x >= jobs.j[k].t0)
-            jn = k;
+#pragma unroll
+    for (int k = 1; k < JMAX; k++)
+        jn = (k < jobs.n && (int)blockIdx.x >= jobs.t0s[k]) ? k : jn;
     const RJob &J = jobs.j[jn];
     __shared__ typename AccT<EXACT>::type lds[tile_lds_elems<TH>()];
     reduce_tile<TS, TD, TC, COPY, EXACT, TH>(J, (int)blockIdx.x - J.t0, lds);
@@ -79,6 +79,7 @@ int launch_jobs_th(vtg_ctx *c, const std::vector<RJob> &all)
             j.t0 = tiles;
             tiles += j.tiles_x * ((j.dh + TH - 1) / TH);
             js.j[k] = j;
+            js.t0s[k] = j.t0;
         }
         hipEvent_t ev;
         timer_begin(c, "pyr_reduce", &ev);

@@ -55,6 +55,7 @@ struct RJob {
 constexpr int JMAX = 32;
 struct RJobs {
     RJob j[JMAX];
+    int t0s[JMAX]; // j[k].t0 again, side by side: a workgroup finds its job from a few wide scalar loads
     int n;
 };
 
